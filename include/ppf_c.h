@@ -1,0 +1,194 @@
+/*
+ * ppf_c.h -- C ABI of the MI355X-native PPF detection library
+ * (libppf_amd.so).  It is the drop-in boundary for SurfaceMan/surface_match's
+ * ppf::Detector (ppf.h:10-62): every entry point takes plain pointers and
+ * sizes, reports errors through a status code plus ppf_last_error(), and is
+ * synchronous at the API edge, like the reference's member functions.
+ *
+ * Point clouds cross the boundary as the reference's PointCloud does
+ * (type.h:22-33): points and normals as n x 3 float arrays (the memory
+ * layout of std::vector<Eigen::Vector3f>), a bounding box as 6 floats
+ * {min.x,min.y,min.z,max.x,max.y,max.z} (diameter 0 or NULL = compute it),
+ * and a view point as 3 floats (NULL or NaN = none).  Poses are returned as
+ * 4x4 column-major floats, the layout of Eigen::Matrix4f.
+ *
+ * Status codes map onto the reference's exceptions: PPF_ERR_RANGE is
+ * std::range_error, PPF_ERR_RUNTIME is std::runtime_error (same messages).
+ */
+#ifndef PPF_C_H
+#define PPF_C_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#if defined(__GNUC__)
+#define PPF_API __attribute__((visibility("default")))
+#else
+#define PPF_API
+#endif
+
+enum {
+    PPF_OK          = 0,
+    PPF_ERR_RANGE   = 1, /* std::range_error in the reference   */
+    PPF_ERR_RUNTIME = 2, /* std::runtime_error in the reference */
+    PPF_ERR_DEVICE  = 3, /* HIP runtime / device failure        */
+    PPF_ERR_ARG     = 4  /* NULL handle or buffer               */
+};
+
+typedef struct ppf_detector_s *ppf_detector;
+typedef struct ppf_scene_s    *ppf_scene;
+
+/* TrainParam, type.h:39-74 (smoothNormal is the reference's bool). */
+typedef struct {
+    float featDistanceStepRel;
+    int   featAngleResolution;
+    float poseRefRelSamplingDistance;
+    int   knnNormal;
+    int   smoothNormal;
+} ppf_train_param;
+
+/* MatchParam, type.h:80-176. */
+typedef struct {
+    int   numMatches;
+    int   knnNormal;
+    int   smoothNormal;
+    int   invertNormal;
+    float maxOverlapDistRel;
+    float maxOverlapDistAbs;
+    int   sparsePoseRefinement;
+    int   densePoseRefinement;
+    int   poseRefNumSteps;
+    float poseRefDistThresholdRel;
+    float poseRefDistThresholdAbs;
+    float poseRefScoringDistRel;
+    float poseRefScoringDistAbs;
+} ppf_match_param;
+
+/* MatchResult, type.h:178-181: the sampled scene and the key points.  The
+ * caller supplies buffers of the given capacities (points); the library
+ * writes min(count, cap) points and always reports the full count. */
+typedef struct {
+    float *sampledPoints, *sampledNormals;
+    size_t sampledCap, nSampled;
+    float *keyPoints, *keyNormals;
+    size_t keyCap, nKey;
+} ppf_match_result;
+
+/* Work counters and stage timings of one match (no reference counterpart;
+ * the stage names are the reference's Timer labels, ppf.cpp:196-436). */
+typedef struct {
+    int    nScene, nValid, nSampled, nKeypoints, nVotedKeypoints, nPoses, nClusters,
+           nHypotheses, nSparseIcp, nDenseIcp, nResults, nDense;
+    unsigned long long nPairs, nKeyHits, nVotes;
+    float  msSample, msNormals, msKeypoints, msVote, msCluster, msIcpPrepare, msIcp, msTotal;
+    float  msVoteKernel; /* device time of the vote kernel alone */
+} ppf_match_stats;
+
+/* -- parameters (TrainParam/MatchParam constructors, type.cpp:71-127) -- */
+PPF_API int ppf_train_param_default(ppf_train_param *p);
+PPF_API int ppf_match_param_default(ppf_match_param *p);
+PPF_API int ppf_train_param_check(const ppf_train_param *p);
+PPF_API int ppf_match_param_check(const ppf_match_param *p);
+
+/* -- Detector (ppf.h:10-62) -- */
+PPF_API int  ppf_detector_create(ppf_detector *out);          /* Detector::Detector  ppf.cpp:26-31 */
+PPF_API void ppf_detector_destroy(ppf_detector d);            /* Detector::~Detector ppf.cpp:33-34 */
+
+/* Detector::trainModel, ppf.h:25-26 / ppf.cpp:36-147 */
+PPF_API int ppf_train_model(ppf_detector d, const float *points, const float *normals, size_t n,
+                            const float *box6, const float *viewPoint, float samplingDistanceRel,
+                            const ppf_train_param *param);
+
+/* Detector::matchScene, ppf.h:39-42 / ppf.cpp:149-510.  Writes up to cap
+ * poses (16 floats each, column-major 4x4) and scores; *nOut = number of
+ * matches (<= numMatches).  result and stats may be NULL. */
+PPF_API int ppf_match_scene(ppf_detector d, const float *points, const float *normals, size_t n,
+                            const float *box6, const float *viewPoint, float samplingDistanceRel,
+                            float keyPointFraction, float minScore, const ppf_match_param *param,
+                            float *poses16, float *scores, size_t cap, size_t *nOut,
+                            ppf_match_result *result, ppf_match_stats *stats);
+
+/* Detector::save / load, ppf.h:49,57 / ppf.cpp:512-547 (model file layout
+ * of serialize.cpp:5-186, unchanged). */
+PPF_API int ppf_save(ppf_detector d, const char *path);
+PPF_API int ppf_load(ppf_detector d, const char *path);
+
+/* message of the last failed call on this thread (what() of the
+ * reference's exception) */
+PPF_API const char *ppf_last_error(void);
+
+/* -- device-resident scenes (no reference counterpart: the reference has no
+ * device; these keep a scene in HBM so repeated matches skip the upload) -- */
+PPF_API int ppf_scene_upload(ppf_detector d, const float *points, const float *normals, size_t n,
+                             const float *box6, const float *viewPoint, ppf_scene *out);
+PPF_API void ppf_scene_free(ppf_scene s);
+PPF_API int ppf_match_scene_resident(ppf_detector d, ppf_scene s, float samplingDistanceRel,
+                                     float keyPointFraction, float minScore,
+                                     const ppf_match_param *param, float *poses16, float *scores,
+                                     size_t cap, size_t *nOut, ppf_match_stats *stats);
+
+/* -- trained-model introspection (the IMPL fields, privateType.h:56-66) -- */
+typedef struct {
+    int   nSampled, nReSampled, nKeys, nEntries, angleNum;
+    float diameter, samplingDistanceRel;
+    float boxMin[3], boxMax[3];
+} ppf_model_info;
+PPF_API int ppf_model_get_info(ppf_detector d, ppf_model_info *info);
+/* sampled / resampled model (n x 3 each) and the hash table in CSR form:
+ * keys[nKeys] ascending, offs[nKeys+1], refs[nEntries], alphas[nEntries];
+ * any pointer may be NULL. */
+PPF_API int ppf_model_get_sampled(ppf_detector d, float *pts, float *nrm, float *rePts,
+                                  float *reNrm);
+PPF_API int ppf_model_get_table(ppf_detector d, uint32_t *keys, int *offs, uint32_t *refs,
+                                float *alphas);
+
+/* -- stage entry points (each replaces one reference function; used by the
+ * parity tests and by callers composing their own pipeline) -- */
+
+/* computePPF + computeAlpha (util.cpp:779-872) of (p1,n1) against n
+ * partners (n x 3 arrays); tail != 0 applies the scalar-tail formula
+ * (util.cpp:712-730) to all partners. */
+PPF_API int ppf_pair_features(const float p1n1[6], const float *pts, const float *nrm, size_t n,
+                              float angleStep, float distStep, int tail, uint32_t *keys,
+                              float *alphas);
+
+/* the vote loop of matchScene (ppf.cpp:247-409) over given key points of a
+ * given sampled scene (n x 3 points/normals, positions as the vote stage
+ * sees them).  Writes candidates (refId, angleId, vote) as 3 ints per slot,
+ * 3 slots per key point (-1 = none) and, if accDump != NULL, the
+ * accumulator bins of the first dumpCount key points (refNum x angleNum
+ * ints each). */
+PPF_API int ppf_vote(ppf_detector d, const float *pts, const float *nrm, size_t n,
+                     const int *keypoints, size_t nk, float minScore, int *cand3,
+                     int32_t *accDump, size_t dumpCount, float *poses12, int *nPoses);
+
+/* voxel-grid downsampling (replaces samplePointCloud, util.cpp:107-139) */
+PPF_API int ppf_voxel_sample(const float *pts, size_t n, const int *subset, size_t nSubset,
+                             float step, const float boxMin[3], int *out, size_t *nOut);
+/* exact radius search: per query the ids with dist2 < r2, ascending
+ * (RadiusResultSet, nanoflann.hpp:364-418); counts[nq], ids packed in query
+ * order up to cap */
+PPF_API int ppf_radius_search(const float *pts, size_t n, const float *q, size_t nq, float r2,
+                              float cell, int *counts, int *ids, size_t cap);
+/* k nearest by (dist2, id) (KNNResultSet, nanoflann.hpp:169-242) */
+PPF_API int ppf_knn_search(const float *pts, size_t n, const float *q, size_t nq, int k,
+                           float cell, int *ids, float *d2);
+/* estimateNormal (util.cpp:272-334), snapshot semantics; pts/nrm in-out */
+PPF_API int ppf_estimate_normals(float *pts, float *nrm, size_t n, const int *valid,
+                                 size_t nValid, const int *query, size_t nq, int k, int smooth,
+                                 int invert, const float viewPoint[3]);
+
+/* L0 scalar math on the device, for bitwise host/device checks:
+ * op 0 atanf(a), 1 atan2f(a,b), 2 a/b, 3 sqrtf(a), 4 murmur of 4 uint words
+ * (a,b bit patterns as words 0,1 and 2,3 = 0), 5 quant ceil(a/b), 6 quant trunc,
+ * 7 double sqrt((double)a*b) rounded to float bits of the double's high word */
+PPF_API int ppf_device_math(int op, const float *a, const float *b, size_t n, uint32_t *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,226 @@
+"""MI355X-native PPF detection path, drop-in for surface_match's ppf::Detector.
+
+Python mirror of the reference's public API (ppf.h:10-62, type.h:8-181):
+``Detector.trainModel / matchScene / save / load`` with ``PointCloud``,
+``TrainParam``, ``MatchParam`` and ``MatchResult``.  Every call goes
+through the C ABI of libppf_amd.so (include/ppf_c.h) into gfx950 HIP
+kernels; nothing here computes on the CPU beyond argument marshalling.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _native as N
+
+__all__ = ["PointCloud", "TrainParam", "MatchParam", "MatchResult", "Detector", "BoundingBox"]
+
+
+@dataclass
+class BoundingBox:
+    """type.h:8-20"""
+    min: np.ndarray = field(default_factory=lambda: np.zeros(3, np.float32))
+    max: np.ndarray = field(default_factory=lambda: np.zeros(3, np.float32))
+
+    def size(self):
+        return np.asarray(self.max, np.float32) - np.asarray(self.min, np.float32)
+
+    def center(self):
+        return (np.asarray(self.min, np.float32) + np.asarray(self.max, np.float32)) / np.float32(2)
+
+    def diameter(self) -> float:
+        s = self.size()
+        return float(np.sqrt(np.float32(s[0] * s[0] + s[1] * s[1]) + s[2] * s[2]))
+
+
+@dataclass
+class PointCloud:
+    """type.h:22-33: points / normals are n x 3 float arrays; normals are
+    present when their count equals the point count (type.cpp:36-38)."""
+    point: np.ndarray = field(default_factory=lambda: np.zeros((0, 3), np.float32))
+    normal: np.ndarray = field(default_factory=lambda: np.zeros((0, 3), np.float32))
+    box: BoundingBox = field(default_factory=BoundingBox)
+    viewPoint: np.ndarray = field(default_factory=lambda: np.full(3, np.nan, np.float32))
+
+    def hasNormal(self) -> bool:
+        return len(self.point) > 0 and len(self.point) == len(self.normal)
+
+    def size(self) -> int:
+        return len(self.point)
+
+    def empty(self) -> bool:
+        return len(self.point) == 0
+
+    def _c_args(self):
+        p = N.f32(np.reshape(self.point, (-1, 3)))
+        n = N.f32(np.reshape(self.normal, (-1, 3))) if self.hasNormal() else None
+        box = N.f32(np.concatenate([np.asarray(self.box.min, np.float32).ravel(),
+                                    np.asarray(self.box.max, np.float32).ravel()]))
+        vp = N.f32(np.asarray(self.viewPoint, np.float32).ravel())
+        return p, n, box, vp
+
+
+def TrainParam(featDistanceStepRel=0.04, featAngleResolution=30, poseRefRelSamplingDistance=0.01,
+               knnNormal=10, smoothNormal=True) -> N.TrainParam:
+    """type.h:71-73 / type.cpp:71-89 (raises ValueError = std::range_error)."""
+    p = N.TrainParam(featDistanceStepRel, featAngleResolution, poseRefRelSamplingDistance,
+                     knnNormal, int(bool(smoothNormal)))
+    N.check(N.lib().ppf_train_param_check(C.byref(p)))
+    return p
+
+
+def MatchParam(numMatches=1, knnNormal=10, smoothNormal=True, invertNormal=False,
+               maxOverlapDistRel=0.5, maxOverlapDistAbs=0.0, sparsePoseRefinement=True,
+               densePoseRefinement=True, poseRefNumSteps=5, poseRefDistThresholdRel=0.1,
+               poseRefDistThresholdAbs=0.0, poseRefScoringDistRel=0.01,
+               poseRefScoringDistAbs=0.0) -> N.MatchParam:
+    """type.h:170-175 / type.cpp:91-127."""
+    p = N.MatchParam(numMatches, knnNormal, int(bool(smoothNormal)), int(bool(invertNormal)),
+                     maxOverlapDistRel, maxOverlapDistAbs, int(bool(sparsePoseRefinement)),
+                     int(bool(densePoseRefinement)), poseRefNumSteps, poseRefDistThresholdRel,
+                     poseRefDistThresholdAbs, poseRefScoringDistRel, poseRefScoringDistAbs)
+    N.check(N.lib().ppf_match_param_check(C.byref(p)))
+    return p
+
+
+@dataclass
+class MatchResult:
+    """type.h:178-181"""
+    sampledScene: PointCloud = field(default_factory=PointCloud)
+    keyPoint: PointCloud = field(default_factory=PointCloud)
+
+
+class Detector:
+    """ppf::Detector (ppf.h:10-62) backed by libppf_amd.so."""
+
+    def __init__(self):
+        h = C.c_void_p()
+        N.check(N.lib().ppf_detector_create(C.byref(h)))
+        self._h = h
+        self.last_stats: dict | None = None
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            N.lib().ppf_detector_destroy(h)
+            self._h = None
+
+    def trainModel(self, model: PointCloud, samplingDistanceRel: float = 0.04,
+                   param: N.TrainParam | None = None):
+        p, n, box, vp = model._c_args()
+        param = param or TrainParam()
+        N.check(N.lib().ppf_train_model(self._h, N.ptr(p), N.ptr(n), len(p), N.ptr(box),
+                                        N.ptr(vp), samplingDistanceRel, C.byref(param)))
+
+    def matchScene(self, scene: PointCloud, samplingDistanceRel: float = 0.04,
+                   keyPointFraction: float = 0.2, minScore: float = 0.2,
+                   param: N.MatchParam | None = None, matchResult: MatchResult | None = None):
+        """Returns (poses [k,4,4] float32, scores [k]) like the reference's
+        output vectors (ppf.cpp:501-507)."""
+        p, n, box, vp = scene._c_args()
+        param = param or MatchParam()
+        cap = max(int(param.numMatches), 1)
+        poses = np.zeros((cap, 16), np.float32)
+        scores = np.zeros(cap, np.float32)
+        nout = C.c_size_t(0)
+        st = N.MatchStats()
+        mr = None
+        bufs = []
+        if matchResult is not None:
+            cap_s = len(p)
+            sp, sn = np.zeros((cap_s, 3), np.float32), np.zeros((cap_s, 3), np.float32)
+            kp, kn = np.zeros((cap_s, 3), np.float32), np.zeros((cap_s, 3), np.float32)
+            bufs = [sp, sn, kp, kn]
+            mr = N.MatchResultC(N.ptr(sp), N.ptr(sn), cap_s, 0, N.ptr(kp), N.ptr(kn), cap_s, 0)
+        N.check(N.lib().ppf_match_scene(self._h, N.ptr(p), N.ptr(n), len(p), N.ptr(box), N.ptr(vp),
+                                        samplingDistanceRel, keyPointFraction, minScore,
+                                        C.byref(param), N.ptr(poses), N.ptr(scores), cap,
+                                        C.byref(nout), C.byref(mr) if mr else None, C.byref(st)))
+        self.last_stats = st.as_dict()
+        if matchResult is not None:
+            sp, sn, kp, kn = bufs
+            matchResult.sampledScene = PointCloud(sp[:mr.nSampled].copy(), sn[:mr.nSampled].copy())
+            matchResult.keyPoint = PointCloud(kp[:mr.nKey].copy(), kn[:mr.nKey].copy())
+        k = nout.value
+        # column-major 4x4 -> row-major numpy
+        return poses[:k].reshape(k, 4, 4).transpose(0, 2, 1).copy(), scores[:k].copy()
+
+    def save(self, filename: str):
+        N.check(N.lib().ppf_save(self._h, str(filename).encode()))
+
+    def load(self, filename: str):
+        N.check(N.lib().ppf_load(self._h, str(filename).encode()))
+
+    # -- introspection of the trained model (privateType.h:56-66)
+    def model_info(self) -> N.ModelInfo:
+        info = N.ModelInfo()
+        N.check(N.lib().ppf_model_get_info(self._h, C.byref(info)))
+        return info
+
+    def sampled_model(self):
+        info = self.model_info()
+        sp = np.zeros((info.nSampled, 3), np.float32)
+        sn = np.zeros((info.nSampled, 3), np.float32)
+        rp = np.zeros((info.nReSampled, 3), np.float32)
+        rn = np.zeros((info.nReSampled, 3), np.float32)
+        N.check(N.lib().ppf_model_get_sampled(self._h, N.ptr(sp), N.ptr(sn), N.ptr(rp), N.ptr(rn)))
+        return sp, sn, rp, rn
+
+    def table(self):
+        info = self.model_info()
+        keys = np.zeros(max(info.nKeys, 1), np.uint32)
+        offs = np.zeros(info.nKeys + 1, np.int32)
+        refs = np.zeros(max(info.nEntries, 1), np.uint32)
+        al = np.zeros(max(info.nEntries, 1), np.float32)
+        N.check(N.lib().ppf_model_get_table(self._h, N.ptr(keys), N.ptr(offs), N.ptr(refs),
+                                            N.ptr(al)))
+        return keys[:info.nKeys], offs, refs[:info.nEntries], al[:info.nEntries]
+
+    def vote(self, pts, nrm, keypoints, minScore=0.2, dump=0):
+        """The vote stage alone over a given sampled scene (ppf.cpp:247-409):
+        returns (candidates [nk,3,3] (ref, bin, vote; -1 = none), poses [np,3,4],
+        accumulator bins of the first `dump` key points)."""
+        pts, nrm, kp = N.f32(pts), N.f32(nrm), N.i32(keypoints)
+        info = self.model_info()
+        cand = np.zeros((len(kp), 3, 3), np.int32)
+        acc = np.zeros((max(dump, 1), info.nSampled, info.angleNum), np.int32)
+        poses = np.zeros((len(kp) * 3, 12), np.float32)
+        npo = C.c_int(0)
+        N.check(N.lib().ppf_vote(self._h, N.ptr(pts), N.ptr(nrm), len(pts), N.ptr(kp), len(kp),
+                                 minScore, N.ptr(cand), N.ptr(acc) if dump else None, dump,
+                                 N.ptr(poses), C.byref(npo)))
+        return cand, poses[:npo.value].reshape(-1, 3, 4), acc[:dump]
+
+
+class ResidentScene:
+    """A scene kept in HBM for repeated matches (ppf_scene_upload)."""
+
+    def __init__(self, det: Detector, scene: PointCloud):
+        p, n, box, vp = scene._c_args()
+        h = C.c_void_p()
+        N.check(N.lib().ppf_scene_upload(det._h, N.ptr(p), N.ptr(n), len(p), N.ptr(box),
+                                         N.ptr(vp), C.byref(h)))
+        self._h = h
+        self._det = det
+
+    def match(self, samplingDistanceRel=0.04, keyPointFraction=0.2, minScore=0.2, param=None):
+        param = param or MatchParam()
+        cap = max(int(param.numMatches), 1)
+        poses = np.zeros((cap, 16), np.float32)
+        scores = np.zeros(cap, np.float32)
+        nout = C.c_size_t(0)
+        st = N.MatchStats()
+        N.check(N.lib().ppf_match_scene_resident(self._det._h, self._h, samplingDistanceRel,
+                                                 keyPointFraction, minScore, C.byref(param),
+                                                 N.ptr(poses), N.ptr(scores), cap, C.byref(nout),
+                                                 C.byref(st)))
+        k = nout.value
+        return poses[:k].reshape(k, 4, 4).transpose(0, 2, 1).copy(), scores[:k].copy(), st.as_dict()
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            N.lib().ppf_scene_free(h)
+            self._h = None

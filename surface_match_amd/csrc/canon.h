@@ -1,0 +1,349 @@
+// canon.h -- the PPF path's canonical FP32 arithmetic, shared by host and
+// gfx950 device code.
+//
+// Every function here is a fixed sequence of IEEE-754 single-precision
+// operations (+ - * / sqrt floor ceil, correctly rounded on both the host
+// and gfx950) with no fused multiply-add: the library is compiled with
+// -ffp-contract=off and this header also pins contraction off.  Hash keys,
+// alphas and angle bins computed here are therefore identical bit for bit
+// on every device and on the host.
+//
+// Reference: SurfaceMan/surface_match util.cpp:459-484 (transformRT,
+// computeAlpha), :643-777 (murmurhash3, hashPPF), :697-827 (angleA2B,
+// computePPF), ppf.cpp:316-346 (angle bins), :393-405 (poses).
+#pragma once
+
+#include <stdint.h>
+#include <math.h>
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define PPF_HD __host__ __device__ inline
+#else
+#define PPF_HD inline
+#endif
+
+#pragma clang fp contract(off)
+
+namespace ppfc {
+
+constexpr float kPi    = 3.14159265358979323846f;
+constexpr float kTwoPi = 2.0f * 3.14159265358979323846f;  // ppf.cpp:24 const float M_2PI
+constexpr float kPiO2  = 1.5707963267948966192f;
+constexpr float kPiO4  = 0.7853981633974483096f;
+
+PPF_HD bool is_nan(float v) { return v != v; }
+PPF_HD bool sign_bit(float v) {
+    union { float f; uint32_t u; } c{v};
+    return (c.u >> 31) != 0u;
+}
+
+// arctangent by the method of the Cephes library's atanf (S. L. Moshier):
+// two range reductions to |x| <= tan(pi/8) and an odd degree-9 polynomial.
+PPF_HD float atanf_c(float xx) {
+    float x   = xx;
+    bool  neg = false;
+    if (x < 0.0f) {
+        neg = true;
+        x   = -x;
+    }
+    float y;
+    if (x > 2.414213562373095f) {
+        y = kPiO2;
+        x = -(1.0f / x);
+    } else if (x > 0.4142135623730950f) {
+        y = kPiO4;
+        x = (x - 1.0f) / (x + 1.0f);
+    } else {
+        y = 0.0f;
+    }
+    float z = x * x;
+    float p = 8.05374449538e-2f;
+    p       = p * z - 1.38776856032e-1f;
+    p       = p * z + 1.99777106478e-1f;
+    p       = p * z - 3.33329491539e-1f;
+    p       = p * z;
+    p       = p * x;
+    p       = p + x;
+    y       = y + p;
+    return neg ? -y : y;
+}
+
+// atan2 with the C99 Annex F treatment of signed zeros.
+PPF_HD float atan2f_c(float y, float x) {
+    if (y == 0.0f) {
+        if (x > 0.0f || (x == 0.0f && !sign_bit(x)))
+            return y;
+        return sign_bit(y) ? -kPi : kPi;
+    }
+    if (x == 0.0f)
+        return y > 0.0f ? kPiO2 : -kPiO2;
+    float z = atanf_c(y / x);
+    if (x < 0.0f) {
+        float w = y > 0.0f ? kPi : -kPi;
+        return w + z;
+    }
+    return z;
+}
+
+// MurmurHash3_x86_32 of four little-endian 32-bit words with seed 42
+// (util.cpp:643-688 with len = 16, :732-763).
+PPF_HD uint32_t rotl32(uint32_t x, int r) { return (x << r) | (x >> (32 - r)); }
+PPF_HD uint32_t murmur_key4(uint32_t q0, uint32_t q1, uint32_t q2, uint32_t q3) {
+    const uint32_t c1 = 0xcc9e2d51u, c2 = 0x1b873593u;
+    uint32_t       h  = 42u;
+    uint32_t       ks[4] = {q0, q1, q2, q3};
+#if defined(__HIP_DEVICE_COMPILE__)
+#pragma unroll
+#endif
+    for (int i = 0; i < 4; i++) {
+        uint32_t k = ks[i] * c1;
+        k          = rotl32(k, 15) * c2;
+        h ^= k;
+        h = rotl32(h, 13) * 5u + 0xe6546b64u;
+    }
+    h ^= 16u;
+    h ^= h >> 16;
+    h *= 0x85ebca6bu;
+    h ^= h >> 13;
+    h *= 0xc2b2ae35u;
+    h ^= h >> 16;
+    return h;
+}
+
+// quantisation: SIMD body ceil + cast to uint32 (NaN lane -> 0x80000000 as
+// cvttps2dq gives), util.cpp:771-774; scalar tail int() truncation, :691-692.
+PPF_HD uint32_t quant(float v, float step, bool tail) {
+    float r = v / step;
+    if (is_nan(r))
+        return 0x80000000u;
+    if (tail)
+        return (uint32_t)(int32_t)r;
+    return (uint32_t)(int32_t)ceilf(r);
+}
+
+PPF_HD float angle_a2b(float ax, float ay, float az, float bx, float by, float bz) {
+    float i     = ay * bz - az * by;
+    float j     = az * bx - ax * bz;
+    float k     = ax * by - ay * bx;
+    float cross = sqrtf(i * i + j * j + k * k);
+    float dot   = ax * bx + ay * by + az * bz;
+    return atan2f_c(cross, dot);
+}
+
+// computePPF for one partner (body util.cpp:802-818, tail :712-730).
+PPF_HD uint32_t ppf_key(float p1x, float p1y, float p1z, float n1x, float n1y, float n1z,
+                        float p2x, float p2y, float p2z, float n2x, float n2y, float n2z,
+                        float angleStep, float distStep, bool tail) {
+    float dx = p2x - p1x;
+    float dy = p2y - p1y;
+    float dz = p2z - p1z;
+    float dn = sqrtf(dx * dx + dy * dy + dz * dz);
+    float f1, f2, f3;
+    if (tail && !(dn > 0.0f)) {
+        f1 = f2 = f3 = 0.0f;
+    } else {
+        float ux = dx / dn, uy = dy / dn, uz = dz / dn;
+        f1 = angle_a2b(ux, uy, uz, n1x, n1y, n1z);
+        f2 = angle_a2b(ux, uy, uz, n2x, n2y, n2z);
+        f3 = angle_a2b(n1x, n1y, n1z, n2x, n2y, n2z);
+    }
+    return murmur_key4(quant(f1, angleStep, tail), quant(f2, angleStep, tail),
+                       quant(f3, angleStep, tail), quant(dn, distStep, tail));
+}
+
+// Local frame (transformRT, util.cpp:459-475): minimal rotation taking n to
+// +x, t = -R p; 3x4 row-major.  Closed-form Rodrigues, identical to the
+// oracle's restatement.
+PPF_HD void frame(float px, float py, float pz, float c, float ny, float nz, float rt[12]) {
+    float R[9];
+    if (ny == 0.0f && nz == 0.0f) {
+        float sx = c >= 0.0f ? 1.0f : -1.0f;
+        R[0] = sx;   R[1] = 0.0f; R[2] = 0.0f;
+        R[3] = 0.0f; R[4] = 1.0f; R[5] = 0.0f;
+        R[6] = 0.0f; R[7] = 0.0f; R[8] = sx;
+    } else {
+        float s2  = ny * ny + nz * nz;
+        float k   = (1.0f - c) / s2;
+        float kny = k * ny, knz = k * nz;
+        float off = -(kny * nz);
+        R[0] = c;   R[1] = ny;           R[2] = nz;
+        R[3] = -ny; R[4] = c + knz * nz; R[5] = off;
+        R[6] = -nz; R[7] = off;          R[8] = c + kny * ny;
+    }
+    for (int r = 0; r < 3; r++) {
+        rt[r * 4 + 0] = R[r * 3 + 0];
+        rt[r * 4 + 1] = R[r * 3 + 1];
+        rt[r * 4 + 2] = R[r * 3 + 2];
+        rt[r * 4 + 3] = -(R[r * 3 + 0] * px + R[r * 3 + 1] * py + R[r * 3 + 2] * pz);
+    }
+}
+
+// computeAlpha (util.cpp:477-484, :853-866): sign flip only at +-fl(pi)
+// with z != 0 (the value of sin(alpha)*z > 0 for every finite alpha).
+PPF_HD float alpha_of(const float rt[12], float x, float y, float z) {
+    float yy = rt[4] * x + rt[5] * y + rt[6] * z + rt[7];
+    float zz = rt[8] * x + rt[9] * y + rt[10] * z + rt[11];
+    float a  = atan2f_c(-zz, yy);
+    if ((a == kPi || a == -kPi) && zz != 0.0f)
+        a = -a;
+    return a;
+}
+
+// angle bin of a bucket entry (ppf.cpp:316-340); tail = the entry is among
+// the bucket's last size%8.
+PPF_HD int angle_bin(float alphaModel, float alphaScene, float sMaxId, int angleNum, bool tail) {
+    float a;
+    if (!tail) {
+        float vs = alphaScene - kPi;
+        a        = alphaModel - vs;
+        if (a > kTwoPi)
+            a = a - kTwoPi;
+        if (a < 0.0f)
+            a = a + kTwoPi;
+    } else {
+        float s = (float)((double)alphaScene - 3.14159265358979323846);
+        a       = alphaModel - s;
+        if (a < 0.0f)
+            a = a + kTwoPi;
+        if (a > kTwoPi)
+            a = a - kTwoPi;
+    }
+    int bin = (int)floorf(sMaxId * a);
+    bin     = bin < 0 ? 0 : bin;
+    bin     = bin > angleNum - 1 ? angleNum - 1 : bin;
+    return bin;
+}
+
+// nanoflann L2 metric for DIM = 3 (nanoflann.hpp:545-581): ((0+dx^2)+dy^2)+dz^2.
+PPF_HD float dist2(float qx, float qy, float qz, float px, float py, float pz) {
+    float dx = qx - px, dy = qy - py, dz = qz - pz;
+    float r  = 0.0f;
+    r        = r + dx * dx;
+    r        = r + dy * dy;
+    r        = r + dz * dz;
+    return r;
+}
+
+// Shoemake's quaternion of a rotation matrix (Eigen's branch structure),
+// q = (w, x, y, z), R row-major 3x3.
+PPF_HD void quat_from_rot(const float R[9], float q[4]) {
+    float t = R[0] + R[4] + R[8];
+    if (t > 0.0f) {
+        t    = sqrtf(t + 1.0f);
+        q[0] = 0.5f * t;
+        t    = 0.5f / t;
+        q[1] = (R[7] - R[5]) * t;
+        q[2] = (R[2] - R[6]) * t;
+        q[3] = (R[3] - R[1]) * t;
+    } else {
+        int i = 0;
+        if (R[4] > R[0])
+            i = 1;
+        if (R[8] > R[i * 3 + i])
+            i = 2;
+        int j = (i + 1) % 3, k = (j + 1) % 3;
+        t = sqrtf(R[i * 3 + i] - R[j * 3 + j] - R[k * 3 + k] + 1.0f);
+        float v[3];
+        v[i] = 0.5f * t;
+        t    = 0.5f / t;
+        q[0] = (R[k * 3 + j] - R[j * 3 + k]) * t;
+        v[j] = (R[j * 3 + i] + R[i * 3 + j]) * t;
+        v[k] = (R[k * 3 + i] + R[i * 3 + k]) * t;
+        q[1] = v[0];
+        q[2] = v[1];
+        q[3] = v[2];
+    }
+}
+
+PPF_HD void rot_from_quat(const float q[4], float R[9]) {
+    float w = q[0], x = q[1], y = q[2], z = q[3];
+    float tx = 2.0f * x, ty = 2.0f * y, tz = 2.0f * z;
+    float twx = tx * w, twy = ty * w, twz = tz * w;
+    float txx = tx * x, txy = ty * x, txz = tz * x;
+    float tyy = ty * y, tyz = tz * y, tzz = tz * z;
+    R[0] = 1.0f - (tyy + tzz);
+    R[1] = txy - twz;
+    R[2] = txz + twy;
+    R[3] = txy + twz;
+    R[4] = 1.0f - (txx + tzz);
+    R[5] = tyz - twx;
+    R[6] = txz - twy;
+    R[7] = tyz + twx;
+    R[8] = 1.0f - (txx + tyy);
+}
+
+PPF_HD float angle_of_quat(const float q[4]) {
+    float n = sqrtf(q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+    return 2.0f * atan2f_c(n, fabsf(q[0]));
+}
+
+// T = inv(Ts) * (Rx(alpha) * Tm), rigid inverse; cs = (cos, sin) of the
+// bin's pose angle.  3x4 row-major.
+PPF_HD void compose_pose(const float Ts[12], const float Tm[12], float c, float s, float out[12]) {
+    float M[12];
+    for (int col = 0; col < 4; col++) {
+        M[0 + col] = Tm[0 + col];
+        M[4 + col] = c * Tm[4 + col] - s * Tm[8 + col];
+        M[8 + col] = s * Tm[4 + col] + c * Tm[8 + col];
+    }
+    float d0 = M[3] - Ts[3], d1 = M[7] - Ts[7], d2 = M[11] - Ts[11];
+    for (int r = 0; r < 3; r++) {
+        for (int col = 0; col < 3; col++)
+            out[r * 4 + col] = Ts[0 * 4 + r] * M[0 * 4 + col] + Ts[1 * 4 + r] * M[1 * 4 + col] +
+                               Ts[2 * 4 + r] * M[2 * 4 + col];
+        out[r * 4 + 3] = Ts[0 * 4 + r] * d0 + Ts[1 * 4 + r] * d1 + Ts[2 * 4 + r] * d2;
+    }
+}
+
+// Cyclic Jacobi eigen-decomposition of a symmetric n x n FP64 matrix (Golub
+// & Van Loan, Matrix Computations, sec. 8.5).  a is destroyed; v receives the
+// eigenvectors as columns, w the eigenvalues.
+template <int N>
+PPF_HD void jacobi_eig(double *a, double *v, double *w) {
+    for (int i = 0; i < N; i++)
+        for (int j = 0; j < N; j++)
+            v[i * N + j] = (i == j) ? 1.0 : 0.0;
+    for (int sweep = 0; sweep < 60; sweep++) {
+        double off = 0.0, diag = 0.0;
+        for (int i = 0; i < N; i++) {
+            diag = diag + a[i * N + i] * a[i * N + i];
+            for (int j = i + 1; j < N; j++)
+                off = off + a[i * N + j] * a[i * N + j];
+        }
+        if (!(off > 1e-30 * diag) || off == 0.0)
+            break;
+        for (int p = 0; p < N - 1; p++)
+            for (int q = p + 1; q < N; q++) {
+                double apq = a[p * N + q];
+                if (apq == 0.0)
+                    continue;
+                double app = a[p * N + p], aqq = a[q * N + q];
+                double theta = (aqq - app) / (2.0 * apq);
+                double t     = 1.0 / (fabs(theta) + sqrt(theta * theta + 1.0));
+                if (theta < 0.0)
+                    t = -t;
+                double c = 1.0 / sqrt(t * t + 1.0);
+                double s = t * c;
+                for (int k = 0; k < N; k++) {
+                    double akp = a[k * N + p], akq = a[k * N + q];
+                    a[k * N + p] = c * akp - s * akq;
+                    a[k * N + q] = s * akp + c * akq;
+                }
+                for (int k = 0; k < N; k++) {
+                    double apk = a[p * N + k], aqk = a[q * N + k];
+                    a[p * N + k] = c * apk - s * aqk;
+                    a[q * N + k] = s * apk + c * aqk;
+                }
+                for (int k = 0; k < N; k++) {
+                    double vkp = v[k * N + p], vkq = v[k * N + q];
+                    v[k * N + p] = c * vkp - s * vkq;
+                    v[k * N + q] = s * vkp + c * vkq;
+                }
+            }
+    }
+    for (int i = 0; i < N; i++)
+        w[i] = a[i * N + i];
+}
+
+}  // namespace ppfc

@@ -1,0 +1,367 @@
+// cluster.hip -- pose clustering on gfx950 (util.cpp:486-601).
+//
+// clusterPose walks the poses by votes (descending) and puts each one in
+// the first existing cluster whose centre (its first pose) is within 0.1 D
+// in translation and angleStep in rotation-angle magnitude, else opens a new
+// cluster.  The outcome is fixed by one rule -- pose i joins the earliest
+// centre among its matching predecessors, and is a centre iff none of them
+// is -- which is resolved here in parallel rounds: a pose decides as soon as
+// every matching predecessor before its first matching centre has decided.
+// Each round is a pass of the workgroup over the undecided poses; the
+// result equals the sequential loop's.  clusterPose2's overlap NMS has the
+// same structure and is resolved the same way.
+#include "ppf_dev.h"
+
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+
+namespace ppfd {
+
+enum : int { UNDECIDED = -2, CENTER = -1 };
+
+struct SortedPoses {
+    float *tx, *ty, *tz, *angle, *votes;
+    int   *ord;  // sorted position -> input index
+};
+
+__global__ void vote_keys_kernel(const PoseRec *p, int n, uint32_t *keys, int *vals) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    keys[i] = 0xFFFFFFFFu - __float_as_uint(p[i].votes);  // votes > 0: descending
+    vals[i] = i;
+}
+
+__global__ void gather_sorted_kernel(const PoseRec *p, const int *ord, int n, SortedPoses s) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const PoseRec &r = p[ord[i]];
+    s.tx[i]    = r.pose[3];
+    s.ty[i]    = r.pose[7];
+    s.tz[i]    = r.pose[11];
+    s.angle[i] = r.angle;
+    s.votes[i] = r.votes;
+}
+
+// comparePose (util.cpp:493-497)
+__device__ __forceinline__ bool pose_match(const SortedPoses &s, int i, int j, float dThr,
+                                           float aThr) {
+    float dx = s.tx[i] - s.tx[j], dy = s.ty[i] - s.ty[j], dz = s.tz[i] - s.tz[j];
+    float d  = sqrtf(dx * dx + dy * dy + dz * dz);
+    float ph = fabsf(s.angle[i] - s.angle[j]);
+    return d < dThr && ph < aThr;
+}
+
+// Greedy clustering rounds.  status[i]: UNDECIDED, CENTER, or the sorted
+// position of the centre i joined.  Candidate predecessors come from the
+// translation grid (cell >= dThr), so only 27 cells are examined.
+__global__ void __launch_bounds__(1024) cluster_rounds_kernel(SortedPoses s, int n, GridView g,
+                                                              float dThr, float aThr, int *status,
+                                                              int *roundsOut) {
+    __shared__ int sChanged, sUndecided;
+    int rounds = 0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x)
+        status[i] = UNDECIDED;
+    __syncthreads();
+    for (;;) {
+        if (threadIdx.x == 0) {
+            sChanged   = 0;
+            sUndecided = 0;
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < n; i += blockDim.x) {
+            if (status[i] != UNDECIDED)
+                continue;
+            float qx = s.tx[i], qy = s.ty[i], qz = s.tz[i];
+            int   minCenter = 0x7fffffff, minUnd = 0x7fffffff;
+            int   xlo = cell_coord(qx - dThr, g.ox, g.cell, g.dx), xhi = cell_coord(qx + dThr, g.ox, g.cell, g.dx);
+            int   ylo = cell_coord(qy - dThr, g.oy, g.cell, g.dy), yhi = cell_coord(qy + dThr, g.oy, g.cell, g.dy);
+            int   zlo = cell_coord(qz - dThr, g.oz, g.cell, g.dz), zhi = cell_coord(qz + dThr, g.oz, g.cell, g.dz);
+            for (int cz = zlo; cz <= zhi; cz++)
+                for (int cy = ylo; cy <= yhi; cy++)
+                    for (int cx = xlo; cx <= xhi; cx++) {
+                        int st, ct;
+                        if (!cell_lookup(g, cx, cy, cz, st, ct))
+                            continue;
+                        for (int t = st; t < st + ct; t++) {
+                            int j = g.id[t];
+                            if (j >= i || !pose_match(s, i, j, dThr, aThr))
+                                continue;
+                            int sj = status[j];
+                            if (sj == CENTER)
+                                minCenter = min(minCenter, j);
+                            else if (sj == UNDECIDED)
+                                minUnd = min(minUnd, j);
+                        }
+                    }
+            if (minUnd < minCenter) {
+                atomicAdd(&sUndecided, 1);
+                continue;
+            }
+            status[i] = (minCenter == 0x7fffffff) ? CENTER : minCenter;
+            sChanged  = 1;
+        }
+        __syncthreads();
+        rounds++;
+        int und = sUndecided, ch = sChanged;
+        __syncthreads();
+        if (und == 0 || !ch)
+            break;
+        // make this round's decisions visible to the next round
+        __threadfence_block();
+    }
+    if (threadIdx.x == 0)
+        *roundsOut = rounds;
+}
+
+// cluster index of each centre = its rank among centres (creation order)
+__global__ void center_rank_kernel(const int *status, int n, int *isCenter) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        isCenter[i] = status[i] == CENTER ? 1 : 0;
+}
+__global__ void member_cluster_kernel(const int *status, const int *rank, int n, uint32_t *cl,
+                                      int *vals) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    int c = status[i] == CENTER ? i : status[i];
+    cl[i]   = (uint32_t)rank[c];
+    vals[i] = i;
+}
+
+// avgClusters + avgQuaternionMarkley (util.cpp:553-601): one thread per
+// cluster sums its members in sorted order (FP32 as the reference), then
+// an FP64 Jacobi eigen-solve of sum(q q^T)/M.
+__global__ void cluster_avg_kernel(const PoseRec *p, const int *ord, const int *members,
+                                   const int *mOff, int nc, Hypo *out, float *angleOut) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc)
+        return;
+    float tx = 0, ty = 0, tz = 0, votes = 0;
+    float A[16];
+    for (int r = 0; r < 16; r++)
+        A[r] = 0.0f;
+    int b = mOff[c], e = mOff[c + 1];
+    for (int t = b; t < e; t++) {
+        const PoseRec &r = p[ord[members[t]]];
+        tx    = tx + r.pose[3];
+        ty    = ty + r.pose[7];
+        tz    = tz + r.pose[11];
+        votes = votes + r.votes;
+        for (int x = 0; x < 4; x++)
+            for (int y = 0; y < 4; y++)
+                A[x * 4 + y] = A[x * 4 + y] + r.q[x] * r.q[y];
+    }
+    int    m = e - b;
+    double Ad[16], V[16], W[4];
+    for (int r = 0; r < 16; r++)
+        Ad[r] = (double)(A[r] / (float)m);
+    ppfc::jacobi_eig<4>(Ad, V, W);
+    int mx = 0;
+    for (int r = 1; r < 4; r++)
+        if (W[r] > W[mx])
+            mx = r;
+    double qv[4] = {V[0 * 4 + mx], V[1 * 4 + mx], V[2 * 4 + mx], V[3 * 4 + mx]};
+    double nn    = sqrt(qv[0] * qv[0] + qv[1] * qv[1] + qv[2] * qv[2] + qv[3] * qv[3]);
+    double sg    = qv[0] < 0.0 ? -1.0 : 1.0;
+    float  q[4];
+    for (int r = 0; r < 4; r++)
+        q[r] = (float)(sg * qv[r] / nn);
+    float R[9];
+    ppfc::rot_from_quat(q, R);
+    Hypo &o = out[c];
+    for (int r = 0; r < 3; r++) {
+        o.pose[r * 4 + 0] = R[r * 3 + 0];
+        o.pose[r * 4 + 1] = R[r * 3 + 1];
+        o.pose[r * 4 + 2] = R[r * 3 + 2];
+    }
+    float fm    = (float)m;
+    o.pose[3]   = tx / fm;
+    o.pose[7]   = ty / fm;
+    o.pose[11]  = tz / fm;
+    o.votes     = votes;
+    float q2[4];
+    ppfc::quat_from_rot(R, q2);
+    angleOut[c] = ppfc::angle_of_quat(q2);
+}
+
+// clusterPose2 (util.cpp:522-551) on the vote-sorted cluster list: pose j is
+// absorbed by the earliest kept pose within the overlap distance.  C is
+// small (hundreds), so predecessors are scanned directly.
+__global__ void __launch_bounds__(1024) nms_kernel(const Hypo *h, const int *ord, int C,
+                                                   const float *center, float thr, int *status,
+                                                   float *tr, Hypo *out, int *nOut) {
+    __shared__ int sChanged, sUndecided;
+    float sq = thr * thr;
+    for (int i = threadIdx.x; i < C; i += blockDim.x) {
+        const float *P = h[ord[i]].pose;
+        for (int r = 0; r < 3; r++)
+            tr[3 * i + r] = P[r * 4 + 0] * center[0] + P[r * 4 + 1] * center[1] +
+                            P[r * 4 + 2] * center[2] + P[r * 4 + 3];
+        status[i] = UNDECIDED;
+    }
+    __syncthreads();
+    for (;;) {
+        if (threadIdx.x == 0) {
+            sChanged   = 0;
+            sUndecided = 0;
+        }
+        __syncthreads();
+        for (int j = threadIdx.x; j < C; j += blockDim.x) {
+            if (status[j] != UNDECIDED)
+                continue;
+            int minKept = 0x7fffffff, minUnd = 0x7fffffff;
+            for (int i = 0; i < j; i++) {
+                float dx = tr[3 * i] - tr[3 * j], dy = tr[3 * i + 1] - tr[3 * j + 1],
+                      dz = tr[3 * i + 2] - tr[3 * j + 2];
+                if (!(dx * dx + dy * dy + dz * dz < sq))
+                    continue;
+                int si = status[i];
+                if (si == CENTER) {
+                    minKept = i;
+                    break;
+                }
+                if (si == UNDECIDED) {
+                    minUnd = i;
+                    break;
+                }
+            }
+            if (minUnd < minKept) {
+                atomicAdd(&sUndecided, 1);
+                continue;
+            }
+            status[j] = (minKept == 0x7fffffff) ? CENTER : minKept;
+            sChanged  = 1;
+        }
+        __syncthreads();
+        int und = sUndecided, ch = sChanged;
+        __syncthreads();
+        if (und == 0 || !ch)
+            break;
+    }
+    if (threadIdx.x == 0) {
+        int cnt = 0;
+        for (int i = 0; i < C; i++) {
+            if (status[i] != CENTER)
+                continue;
+            Hypo o = h[ord[i]];
+            for (int j = i + 1; j < C; j++)
+                if (status[j] == i)
+                    o.votes = o.votes + h[ord[j]].votes;
+            out[cnt++] = o;
+        }
+        *nOut = cnt;
+    }
+}
+
+__global__ void hypo_keys_kernel(const Hypo *h, int n, uint32_t *keys, int *vals) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    keys[i] = 0xFFFFFFFFu - __float_as_uint(h[i].votes);
+    vals[i] = i;
+}
+
+static void sort_pairs(Workspace &ws, uint32_t *k, uint32_t *k2, int *v, int *v2, int n,
+                       hipStream_t s) {
+    size_t tb = 0;
+    PPF_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, k, k2, v, v2, n, 0, 32, s));
+    PPF_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(ws.cubTmp.reserve(tb), tb, k, k2, v, v2, n, 0,
+                                                     32, s));
+}
+
+void cluster_poses(Workspace &ws, const PoseRec *poses, int np, float dThr, float aThr,
+                   const float center[3], float nmsThr, std::vector<Hypo> &out, int *nClusters,
+                   hipStream_t s) {
+    out.clear();
+    if (nClusters)
+        *nClusters = 0;
+    if (np <= 0)
+        return;
+    // every scratch array is sized for np up front (nc <= np), so no buffer
+    // is reallocated while kernels that use it may still be queued
+    uint32_t *k1 = ws.clSortKey.as<uint32_t>(np), *k2 = ws.clSortKey2.as<uint32_t>(np);
+    int      *v1 = ws.clSortVal.as<int>(np), *v2 = ws.clSortVal2.as<int>(np);
+    uint32_t *k3 = ws.clPred.as<uint32_t>(np);
+    float    *buf = ws.clCenters.as<float>((size_t)np * 6);
+    int      *status = ws.clStatus.as<int>((size_t)np * 4 + 8);
+    int      *members = ws.clMembers.as<int>((size_t)np * 2 + 2);
+    int      *cntsBuf = ws.clPredOff.as<int>((size_t)np * 3 + 4);
+    Hypo     *centers = ws.clNms.as<Hypo>((size_t)np * 2 + 2);
+    vote_keys_kernel<<<div_up(np, 256), 256, 0, s>>>(poses, np, k1, v1);
+    sort_pairs(ws, k1, k2, v1, v2, np, s);
+    int *ord = v2;  // sorted position -> pose index (stable: ties keep (keypoint, rank) order)
+    SortedPoses sp{buf, buf + np, buf + 2 * (size_t)np, buf + 3 * (size_t)np, buf + 4 * (size_t)np,
+                   ord};
+    gather_sorted_kernel<<<div_up(np, 256), 256, 0, s>>>(poses, ord, np, sp);
+    // translation grid: ids are sorted positions
+    grid_build(ws.gNormal, sp.tx, sp.ty, sp.tz, nullptr, nullptr, nullptr, nullptr, np, dThr, s);
+    int *rank   = status + np;
+    int *isC    = status + 2 * (size_t)np;
+    int *misc   = status + 3 * (size_t)np;
+    cluster_rounds_kernel<<<1, 1024, 0, s>>>(sp, np, ws.gNormal.view, dThr, aThr, status, misc);
+    center_rank_kernel<<<div_up(np, 256), 256, 0, s>>>(status, np, isC);
+    size_t tb = 0;
+    PPF_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, isC, rank, np, s));
+    PPF_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(ws.cubTmp.reserve(tb), tb, isC, rank, np, s));
+    int lastRank = 0, lastIs = 0;
+    PPF_HIP_CHECK(hipMemcpyAsync(&lastRank, rank + np - 1, sizeof(int), hipMemcpyDeviceToHost, s));
+    PPF_HIP_CHECK(hipMemcpyAsync(&lastIs, isC + np - 1, sizeof(int), hipMemcpyDeviceToHost, s));
+    PPF_HIP_CHECK(hipStreamSynchronize(s));
+    int nc = lastRank + lastIs;
+    if (nClusters)
+        *nClusters = nc;
+    // members grouped by cluster, ascending sorted position inside
+    member_cluster_kernel<<<div_up(np, 256), 256, 0, s>>>(status, rank, np, k1, v1);
+    int endBit = 1;
+    while ((1 << endBit) < nc + 1)
+        endBit++;
+    tb = 0;
+    PPF_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, k1, k3, v1, members, np, 0,
+                                                     endBit, s));
+    int *mOff    = members + np;
+    PPF_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(ws.cubTmp.reserve(tb), tb, k1, k3, v1, members,
+                                                     np, 0, endBit, s));
+    // cluster offsets: counts by run-length of k3 (clusters are 0..nc-1, all non-empty)
+    int *cnts = cntsBuf;
+    uint32_t *uk = reinterpret_cast<uint32_t *>(k1);
+    int      *nr = misc + 1;
+    tb = 0;
+    PPF_HIP_CHECK(hipcub::DeviceRunLengthEncode::Encode(nullptr, tb, k3, uk, cnts, nr, np, s));
+    PPF_HIP_CHECK(hipcub::DeviceRunLengthEncode::Encode(ws.cubTmp.reserve(tb), tb, k3, uk, cnts,
+                                                        nr, np, s));
+    PPF_HIP_CHECK(hipMemsetAsync(cnts + nc, 0, sizeof(int), s));
+    tb = 0;
+    PPF_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, cnts, mOff, nc + 1, s));
+    PPF_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(ws.cubTmp.reserve(tb), tb, cnts, mOff, nc + 1, s));
+    // v2 still holds ord (the second sort wrote to `members`)
+    float *angles  = buf + (size_t)np * 5;
+    cluster_avg_kernel<<<div_up(nc, 64), 64, 0, s>>>(poses, ord, members, mOff, nc, centers,
+                                                     angles);
+    // NMS on clusters sorted by votes (stable on creation order)
+    uint32_t *hk = k1;
+    int      *hv = cntsBuf + nc + 2;
+    uint32_t *hk2 = k3;
+    int      *hv2 = status;  // reuse
+    hypo_keys_kernel<<<div_up(nc, 256), 256, 0, s>>>(centers, nc, hk, hv);
+    sort_pairs(ws, hk, hk2, hv, hv2, nc, s);
+    float *cdev = reinterpret_cast<float *>(misc + 4);
+    PPF_HIP_CHECK(hipMemcpyAsync(cdev, center, sizeof(float) * 3, hipMemcpyHostToDevice, s));
+    int  *nmsStatus = rank;  // reuse (nc <= np)
+    float *tr       = buf;   // reuse
+    Hypo *kept      = centers + nc;
+    int  *nOut      = misc + 2;
+    nms_kernel<<<1, 1024, 0, s>>>(centers, hv2, nc, cdev, nmsThr, nmsStatus, tr, kept, nOut);
+    int nk = 0;
+    PPF_HIP_CHECK(hipMemcpyAsync(&nk, nOut, sizeof(int), hipMemcpyDeviceToHost, s));
+    PPF_HIP_CHECK(hipStreamSynchronize(s));
+    out.resize(nk);
+    PPF_HIP_CHECK(hipMemcpyAsync(out.data(), kept, sizeof(Hypo) * nk, hipMemcpyDeviceToHost, s));
+    PPF_HIP_CHECK(hipStreamSynchronize(s));
+    PPF_HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace ppfd

@@ -1,0 +1,1273 @@
+// detector.hip -- host orchestration of the PPF path and the C ABI
+// (include/ppf_c.h).
+//
+// Detector::trainModel (ppf.cpp:36-147) and Detector::matchScene
+// (ppf.cpp:149-510) are restated stage by stage; each stage runs on the GPU
+// (grid.hip, table.hip, vote.hip, cluster.hip, icp.hip) and the host keeps
+// only the bookkeeping: parameter derivation, small read-backs, and the
+// ordered accept/reject walk over refined hypotheses.  Model files keep the
+// reference's layout (ppf.cpp:512-547, serialize.cpp:5-186).
+#include "ppf_dev.h"
+#include "../../include/ppf_c.h"
+
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <fstream>
+#include <memory>
+#include <numeric>
+#include <string>
+#include <vector>
+
+using namespace ppfd;
+
+namespace {
+
+thread_local std::string g_lastError;
+
+const int   kVersion = 100;      // ppf.cpp:21
+const int   kMagic   = 0x7F27F;  // ppf.cpp:22
+
+struct HostCloud {
+    std::vector<float> p, n;  // n x 3 each (n may be empty)
+    float boxMin[3] = {0, 0, 0}, boxMax[3] = {0, 0, 0};
+    float vp[3] = {NAN, NAN, NAN};
+    int   size() const { return (int)(p.size() / 3); }
+};
+
+struct HostModel {
+    float           samplingDistanceRel = 0.04f;
+    ppf_train_param param{};
+    HostCloud       sampled, reSampled;
+    std::vector<uint32_t> keys, refs;
+    std::vector<int>      offs;
+    std::vector<float>    alphas;
+};
+
+float box_diameter(const float *lo, const float *hi) {
+    float dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
+    return std::sqrt(dx * dx + dy * dy + dz * dz);
+}
+
+void normalize3(float *v) {
+    float l = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    v[0]    = v[0] / l;
+    v[1]    = v[1] / l;
+    v[2]    = v[2] / l;
+}
+
+bool finite3(const float *v) {
+    return std::isfinite(v[0]) && std::isfinite(v[1]) && std::isfinite(v[2]);
+}
+
+// computeBoundingBox (util.cpp:178-207) over the valid indices
+void bbox_host(const float *pts, const std::vector<int> &valid, float lo[3], float hi[3]) {
+    const float *p0 = pts + 3 * (valid.empty() ? 0 : valid[0]);
+    for (int a = 0; a < 3; a++)
+        lo[a] = hi[a] = p0[a];
+    for (int i : valid) {
+        const float *p = pts + 3 * (size_t)i;
+        for (int a = 0; a < 3; a++) {
+            if (p[a] > hi[a])
+                hi[a] = p[a];
+            else if (p[a] < lo[a])
+                lo[a] = p[a];
+        }
+    }
+}
+
+// removeNan (util.cpp:86-105)
+std::vector<int> remove_nan(const float *pts, const float *nrm, int n) {
+    std::vector<int> v;
+    v.reserve(n);
+    for (int i = 0; i < n; i++) {
+        if (!finite3(pts + 3 * (size_t)i))
+            continue;
+        if (nrm && !finite3(nrm + 3 * (size_t)i))
+            continue;
+        v.push_back(i);
+    }
+    return v;
+}
+
+__global__ void split_kernel(const float *aos, int n, float *x, float *y, float *z) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    x[i] = aos[3 * (size_t)i];
+    y[i] = aos[3 * (size_t)i + 1];
+    z[i] = aos[3 * (size_t)i + 2];
+}
+__global__ void fill_nan_kernel(float *a, float *b, float *c, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        a[i] = b[i] = c[i] = NAN;
+}
+// normalizeNormal (util.cpp:163-176)
+__global__ void normalize_kernel(float *x, float *y, float *z, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    float a = x[i], b = y[i], c = z[i];
+    float l = sqrtf(a * a + b * b + c * c);
+    x[i]    = a / l;
+    y[i]    = b / l;
+    z[i]    = c / l;
+}
+__global__ void gather_aos_kernel(const int *idx, int m, const float *x, const float *y,
+                                  const float *z, float *out) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= m)
+        return;
+    int i          = idx[t];
+    out[3 * t]     = x[i];
+    out[3 * t + 1] = y[i];
+    out[3 * t + 2] = z[i];
+}
+
+// Cloud held on the device: originals (as uploaded) and working copies that
+// the pipeline mutates (normal estimation moves points to centroids).
+struct DevCloud {
+    int    n = 0;
+    bool   hasNormal = false;
+    DevBuf aos, ox, oy, oz, onx, ony, onz, px, py, pz, nx, ny, nz, valid;
+    int    nValid = 0;
+    float  boxMin[3], boxMax[3], vp[3];
+
+    void upload(const float *pts, const float *nrm, int count, const float *box6,
+                const float *viewPoint, hipStream_t s) {
+        n         = count;
+        hasNormal = nrm != nullptr;
+        std::vector<int> v = remove_nan(pts, hasNormal ? nrm : nullptr, n);
+        nValid   = (int)v.size();
+        if (box6 && box_diameter(box6, box6 + 3) != 0.0f) {
+            std::memcpy(boxMin, box6, sizeof boxMin);
+            std::memcpy(boxMax, box6 + 3, sizeof boxMax);
+        } else {
+            bbox_host(pts, v, boxMin, boxMax);
+        }
+        for (int a = 0; a < 3; a++)
+            vp[a] = viewPoint ? viewPoint[a] : NAN;
+        int m = std::max(n, 1);
+        float *A = aos.as<float>((size_t)m * 3);
+        PPF_HIP_CHECK(hipMemcpyAsync(A, pts, sizeof(float) * 3 * n, hipMemcpyHostToDevice, s));
+        split_kernel<<<div_up(m, 256), 256, 0, s>>>(A, n, ox.as<float>(m), oy.as<float>(m),
+                                                     oz.as<float>(m));
+        onx.as<float>(m);
+        ony.as<float>(m);
+        onz.as<float>(m);
+        if (hasNormal) {
+            PPF_HIP_CHECK(hipMemcpyAsync(A, nrm, sizeof(float) * 3 * n, hipMemcpyHostToDevice, s));
+            split_kernel<<<div_up(m, 256), 256, 0, s>>>(A, n, onx.get<float>(), ony.get<float>(),
+                                                         onz.get<float>());
+        }
+        PPF_HIP_CHECK(hipMemcpyAsync(valid.as<int>(std::max(nValid, 1)), v.data(),
+                                     sizeof(int) * nValid, hipMemcpyHostToDevice, s));
+        px.as<float>(m);
+        py.as<float>(m);
+        pz.as<float>(m);
+        nx.as<float>(m);
+        ny.as<float>(m);
+        nz.as<float>(m);
+        PPF_HIP_CHECK(hipStreamSynchronize(s));
+    }
+
+    // working copies := originals
+    void reset(hipStream_t s) {
+        size_t b = sizeof(float) * n;
+        PPF_HIP_CHECK(hipMemcpyAsync(px.get<float>(), ox.get<float>(), b, hipMemcpyDeviceToDevice, s));
+        PPF_HIP_CHECK(hipMemcpyAsync(py.get<float>(), oy.get<float>(), b, hipMemcpyDeviceToDevice, s));
+        PPF_HIP_CHECK(hipMemcpyAsync(pz.get<float>(), oz.get<float>(), b, hipMemcpyDeviceToDevice, s));
+        if (hasNormal) {
+            PPF_HIP_CHECK(hipMemcpyAsync(nx.get<float>(), onx.get<float>(), b, hipMemcpyDeviceToDevice, s));
+            PPF_HIP_CHECK(hipMemcpyAsync(ny.get<float>(), ony.get<float>(), b, hipMemcpyDeviceToDevice, s));
+            PPF_HIP_CHECK(hipMemcpyAsync(nz.get<float>(), onz.get<float>(), b, hipMemcpyDeviceToDevice, s));
+            if (n > 0)
+                normalize_kernel<<<div_up(n, 256), 256, 0, s>>>(nx.get<float>(), ny.get<float>(),
+                                                                 nz.get<float>(), n);
+        } else if (n > 0) {
+            fill_nan_kernel<<<div_up(n, 256), 256, 0, s>>>(nx.get<float>(), ny.get<float>(),
+                                                            nz.get<float>(), n);
+        }
+    }
+};
+
+}  // namespace
+
+struct ppf_scene_s {
+    DevCloud cloud;
+};
+
+struct ppf_detector_s {
+    hipStream_t  stream = nullptr;
+    bool         trained = false;
+    HostModel    hm;
+    DevModel     dm;
+    Workspace    ws;
+    TrainScratch ts;
+    VoteConfig   vcfg;
+    IcpConfig    icfg;
+    int          refTail = 1;
+    DevCloud     tmpScene;
+    DevBuf       idxA, idxB, idxC, idxD;  // sampled / keypoints / dense / scratch
+    hipEvent_t   ev[10] = {};
+
+    ppf_detector_s() {
+        PPF_HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        for (auto &e : ev)
+            PPF_HIP_CHECK(hipEventCreate(&e));
+    }
+    ~ppf_detector_s() {
+        for (auto &e : ev)
+            if (e)
+                (void)hipEventDestroy(e);
+        if (stream)
+            (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+float elapsed(hipEvent_t a, hipEvent_t b) {
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, a, b) != hipSuccess)
+        return 0.0f;
+    return ms;
+}
+
+// TrainParam constructor checks (type.cpp:71-89)
+void check_train_param(const ppf_train_param &p) {
+    if (p.featDistanceStepRel > 1 || p.featDistanceStepRel <= 0)
+        throw std::range_error("Invalid Input: featDistanceStepRel range mismatch in TrainParam()");
+    if (p.featAngleResolution <= 0)
+        throw std::range_error("Invalid Input: featAngleResolution range mismatch in TrainParam()");
+    if (p.poseRefRelSamplingDistance > 1 || p.poseRefRelSamplingDistance <= 0)
+        throw std::range_error(
+            "Invalid Input: poseRefRelSamplingDistance range mismatch in TrainParam()");
+}
+
+// MatchParam constructor checks (type.cpp:91-127), including the reference's
+// guard on !densePoseRefinement
+void check_match_param(const ppf_match_param &p) {
+    if (p.numMatches < 1)
+        throw std::range_error("Invalid Input: numMatches range mismatch in MatchParam()");
+    if ((p.maxOverlapDistRel < 0) && (p.maxOverlapDistAbs < 0))
+        throw std::range_error("Invalid Input: maxOverlapDist range mismatch in MatchParam()");
+    if (!p.densePoseRefinement) {
+        if (p.poseRefNumSteps < 1)
+            throw std::range_error("Invalid Input: poseRefNumSteps range mismatch in MatchParam()");
+        if ((p.poseRefDistThresholdRel < 0) && (p.poseRefDistThresholdAbs < 0))
+            throw std::range_error(
+                "Invalid Input: poseRefDistThreshold range mismatch in MatchParam()");
+        if ((p.poseRefScoringDistRel < 0) && (p.poseRefScoringDistAbs < 0))
+            throw std::range_error(
+                "Invalid Input: poseRefScoringDist range mismatch in MatchParam()");
+    }
+}
+
+void download_aos(const int *idxDev, int m, const DevCloud &c, bool normals, std::vector<float> &out,
+                  DevBuf &scratch, hipStream_t s) {
+    out.resize((size_t)m * 3);
+    if (m == 0)
+        return;
+    float *d = scratch.as<float>((size_t)m * 3);
+    if (normals)
+        gather_aos_kernel<<<div_up(m, 256), 256, 0, s>>>(idxDev, m, c.nx.get<float>(),
+                                                          c.ny.get<float>(), c.nz.get<float>(), d);
+    else
+        gather_aos_kernel<<<div_up(m, 256), 256, 0, s>>>(idxDev, m, c.px.get<float>(),
+                                                          c.py.get<float>(), c.pz.get<float>(), d);
+    PPF_HIP_CHECK(hipMemcpyAsync(out.data(), d, sizeof(float) * 3 * m, hipMemcpyDeviceToHost, s));
+    PPF_HIP_CHECK(hipStreamSynchronize(s));
+}
+
+void train_impl(ppf_detector_s &D, const float *pts, const float *nrm, int n, const float *box6,
+                const float *vp, float rel, const ppf_train_param &param) {
+    // ppf.cpp:39-44
+    if (rel > 1 || rel < 0)
+        throw std::range_error("Invalid Input: samplingDistanceRel range mismatch in trainModel");
+    if (n <= 0 || !pts)
+        throw std::runtime_error("Invalid Input: empty model in trainModel");
+    check_train_param(param);
+    hipStream_t s = D.stream;
+    DevCloud   &c = D.tmpScene;
+    c.upload(pts, nrm, n, box6, vp, s);
+    c.reset(s);
+    float Dm           = box_diameter(c.boxMin, c.boxMax);             // ppf.cpp:54
+    float sampleStep   = Dm * rel;                                     // :56
+    float reSampleStep = Dm * param.poseRefRelSamplingDistance;        // :57
+    float distanceStep = Dm * param.featDistanceStepRel;               // :58
+    float angleStep    = ppfc::kTwoPi / (float)param.featAngleResolution;  // :59
+    int  *cnt          = D.ws.smallCnt.as<int>(8);
+    int  *s1 = D.idxA.as<int>(std::max(c.nValid, 1)), *s2 = D.idxB.as<int>(std::max(c.nValid, 1));
+    int   n1 = voxel_sample(D.ws, c.px.get<float>(), c.py.get<float>(), c.pz.get<float>(),
+                            c.valid.get<int>(), c.nValid, sampleStep, c.boxMin, s1, cnt, s);
+    int   n2 = voxel_sample(D.ws, c.px.get<float>(), c.py.get<float>(), c.pz.get<float>(),
+                            c.valid.get<int>(), c.nValid, reSampleStep, c.boxMin, s2, cnt, s);
+    if (!c.hasNormal) {
+        float cellHint = sampleStep * 0.5f;
+        estimate_normals(D.ws, n, c.px.get<float>(), c.py.get<float>(), c.pz.get<float>(),
+                         c.nx.get<float>(), c.ny.get<float>(), c.nz.get<float>(),
+                         c.valid.get<int>(), c.nValid, s1, n1, param.knnNormal, param.smoothNormal,
+                         0, c.vp, cellHint, s);
+        estimate_normals(D.ws, n, c.px.get<float>(), c.py.get<float>(), c.pz.get<float>(),
+                         c.nx.get<float>(), c.ny.get<float>(), c.nz.get<float>(),
+                         c.valid.get<int>(), c.nValid, s2, n2, param.knnNormal, param.smoothNormal,
+                         0, c.vp, cellHint, s);
+    }
+    HostModel hm;
+    hm.samplingDistanceRel = rel;
+    hm.param               = param;
+    DevBuf scratch;
+    download_aos(s1, n1, c, false, hm.sampled.p, scratch, s);
+    download_aos(s1, n1, c, true, hm.sampled.n, scratch, s);
+    download_aos(s2, n2, c, false, hm.reSampled.p, scratch, s);
+    download_aos(s2, n2, c, true, hm.reSampled.n, scratch, s);
+    // extraIndices normalises the gathered normals (util.cpp:153)
+    for (int i = 0; i < n1; i++)
+        normalize3(&hm.sampled.n[3 * (size_t)i]);
+    for (int i = 0; i < n2; i++)
+        normalize3(&hm.reSampled.n[3 * (size_t)i]);
+    for (HostCloud *hc : {&hm.sampled, &hm.reSampled}) {
+        std::memcpy(hc->boxMin, c.boxMin, sizeof c.boxMin);
+        std::memcpy(hc->boxMax, c.boxMax, sizeof c.boxMax);
+    }
+    // table on the device from the sampled model (uploaded as SoA)
+    DevModel &dm = D.dm;
+    model_upload(dm, n1, hm.sampled.p.data(), hm.sampled.n.data(), n2, hm.reSampled.p.data(),
+                 hm.reSampled.n.data(), {}, std::vector<int>(1, 0), {}, {},
+                 param.featAngleResolution, Dm, param.featDistanceStepRel, s);
+    train_table(D.ts, n1, dm.sx.get<float>(), dm.sy.get<float>(), dm.sz.get<float>(),
+                dm.snx.get<float>(), dm.sny.get<float>(), dm.snz.get<float>(), angleStep,
+                distanceStep, D.refTail, hm.keys, hm.offs, hm.refs, hm.alphas, s);
+    model_upload(dm, n1, hm.sampled.p.data(), hm.sampled.n.data(), n2, hm.reSampled.p.data(),
+                 hm.reSampled.n.data(), hm.keys, hm.offs, hm.refs, hm.alphas,
+                 param.featAngleResolution, Dm, param.featDistanceStepRel, s);
+    D.hm      = std::move(hm);
+    D.trained = true;
+}
+
+void upload_model(ppf_detector_s &D) {
+    HostModel &hm = D.hm;
+    float      Dm = box_diameter(hm.sampled.boxMin, hm.sampled.boxMax);  // ppf.cpp:183
+    model_upload(D.dm, hm.sampled.size(), hm.sampled.p.data(), hm.sampled.n.data(),
+                 hm.reSampled.size(), hm.reSampled.p.data(), hm.reSampled.n.data(), hm.keys,
+                 hm.offs, hm.refs, hm.alphas, hm.param.featAngleResolution, Dm,
+                 hm.param.featDistanceStepRel, D.stream);
+}
+
+struct MatchOut {
+    std::vector<std::pair<float, std::vector<float>>> results;  // score, pose 3x4
+};
+
+void write_result_cloud(ppf_detector_s &D, DevCloud &c, const int *idxDev, int m, float *pOut,
+                        float *nOut, size_t cap, size_t &countOut) {
+    countOut = (size_t)m;
+    size_t w = std::min((size_t)m, cap);
+    if (w == 0)
+        return;
+    std::vector<float> tmp;
+    DevBuf             scratch;
+    if (pOut) {
+        download_aos(idxDev, (int)w, c, false, tmp, scratch, D.stream);
+        std::memcpy(pOut, tmp.data(), sizeof(float) * 3 * w);
+    }
+    if (nOut) {
+        download_aos(idxDev, (int)w, c, true, tmp, scratch, D.stream);
+        for (size_t i = 0; i < w; i++)
+            normalize3(&tmp[3 * i]);
+        std::memcpy(nOut, tmp.data(), sizeof(float) * 3 * w);
+    }
+}
+
+void match_impl(ppf_detector_s &D, DevCloud &c, float rel, float kpFrac, float minScore,
+                const ppf_match_param &p, MatchOut &out, ppf_match_result *mr,
+                ppf_match_stats *st) {
+    // ppf.cpp:155-168
+    if (!D.trained)
+        throw std::runtime_error("No trained model in matchScene");
+    if (rel > 1 || rel < 0)
+        throw std::range_error("Invalid Input: samplingDistanceRel range mismatch in matchScene");
+    if (kpFrac > 1 || kpFrac < 0)
+        throw std::range_error("Invalid Input: keyPointFraction range mismatch in matchScene");
+    if (minScore > 1 || minScore < 0)
+        throw std::range_error("Invalid Input: minScore range mismatch in matchScene");
+    if (c.n <= 0)
+        throw std::runtime_error("Invalid Input: empty scene in matchScene");
+    check_match_param(p);
+    hipStream_t s   = D.stream;
+    HostModel  &hm  = D.hm;
+    DevModel   &dm  = D.dm;
+    ppf_match_stats stl{};
+    if (!st)
+        st = &stl;
+    std::memset(st, 0, sizeof *st);
+    PPF_HIP_CHECK(hipEventRecord(D.ev[0], s));
+    c.reset(s);
+    // ppf.cpp:183-193
+    const float Dm           = box_diameter(hm.sampled.boxMin, hm.sampled.boxMax);
+    const float angleStep    = ppfc::kTwoPi / (float)hm.param.featAngleResolution;
+    const int   refNum       = hm.sampled.size();
+    const float reSampleStep = Dm * hm.param.poseRefRelSamplingDistance;
+    const float sampleStep   = Dm * rel;  // :200
+    // :223-240
+    const float voteThreshold = (float)refNum * minScore;
+    float maxOverlapDist = 0;
+    if (p.maxOverlapDistRel > 0) maxOverlapDist = Dm * p.maxOverlapDistRel;
+    if (p.maxOverlapDistAbs > 0) maxOverlapDist = p.maxOverlapDistAbs;
+    float poseRefDist = 0;
+    if (p.poseRefDistThresholdRel > 0) poseRefDist = Dm * p.poseRefDistThresholdRel;
+    if (p.poseRefDistThresholdAbs > 0) poseRefDist = p.poseRefDistThresholdAbs;
+    float scoringDist = 0;
+    if (p.poseRefScoringDistRel > 0) scoringDist = Dm * p.poseRefScoringDistRel;
+    if (p.poseRefScoringDistAbs > 0) scoringDist = p.poseRefScoringDistAbs;
+
+    float *PX = c.px.get<float>(), *PY = c.py.get<float>(), *PZ = c.pz.get<float>();
+    float *NX = c.nx.get<float>(), *NY = c.ny.get<float>(), *NZ = c.nz.get<float>();
+    int   *cnt = D.ws.smallCnt.as<int>(8);
+    int   *samp = D.idxA.as<int>(std::max(c.nValid, 1));
+    // scene sample1 (ppf.cpp:199-203)
+    int ns = voxel_sample(D.ws, PX, PY, PZ, c.valid.get<int>(), c.nValid, sampleStep, c.boxMin,
+                          samp, cnt, s);
+    PPF_HIP_CHECK(hipEventRecord(D.ev[1], s));
+    // normals (ppf.cpp:204-211)
+    if (!c.hasNormal)
+        estimate_normals(D.ws, c.n, PX, PY, PZ, NX, NY, NZ, c.valid.get<int>(), c.nValid, samp, ns,
+                         p.knnNormal, p.smoothNormal, p.invertNormal, c.vp, sampleStep * 0.5f, s);
+    PPF_HIP_CHECK(hipEventRecord(D.ev[2], s));
+    // key points (ppf.cpp:212-216)
+    const float keySampleStep = std::sqrt(1.0f / kpFrac) * sampleStep;
+    int *kp = D.idxB.as<int>(std::max(ns, 1));
+    int  nk = voxel_sample(D.ws, PX, PY, PZ, samp, ns, keySampleStep, c.boxMin, kp, cnt, s);
+    PPF_HIP_CHECK(hipEventRecord(D.ev[3], s));
+    if (mr) {  // ppf.cpp:242-245
+        write_result_cloud(D, c, kp, nk, mr->keyPoints, mr->keyNormals, mr->keyCap, mr->nKey);
+        write_result_cloud(D, c, samp, ns, mr->sampledPoints, mr->sampledNormals, mr->sampledCap,
+                           mr->nSampled);
+    }
+    // vote (ppf.cpp:247-411)
+    grid_build(D.ws.gVote, PX, PY, PZ, NX, NY, NZ, samp, ns, Dm * 0.5f, s);
+    VoteStats vs{};
+    PPF_HIP_CHECK(hipEventRecord(D.ev[4], s));
+    int np = vote_and_poses(D.ws, dm, D.ws.gVote.view, PX, PY, PZ, NX, NY, NZ, kp, nk,
+                            voteThreshold, D.vcfg, &vs, nullptr, 0, s);
+    PPF_HIP_CHECK(hipEventRecord(D.ev[5], s));
+    // cluster (ppf.cpp:413-420)
+    std::vector<Hypo> hyp;
+    float center[3];
+    for (int a = 0; a < 3; a++)
+        center[a] = (hm.sampled.boxMin[a] + hm.sampled.boxMax[a]) / 2.0f;
+    int nClusters = 0;
+    cluster_poses(D.ws, D.ws.poses.get<PoseRec>(), np, 0.1f * Dm, angleStep, center,
+                  maxOverlapDist, hyp, &nClusters, s);
+    PPF_HIP_CHECK(hipEventRecord(D.ev[6], s));
+    // dense prep (ppf.cpp:427-434)
+    int *dense = D.idxC.as<int>(std::max(c.nValid, 1));
+    int  nd    = 0;
+    if (p.densePoseRefinement) {
+        nd = voxel_sample(D.ws, PX, PY, PZ, c.valid.get<int>(), c.nValid, reSampleStep, c.boxMin,
+                          dense, cnt, s);
+        if (!c.hasNormal)
+            estimate_normals(D.ws, c.n, PX, PY, PZ, NX, NY, NZ, c.valid.get<int>(), c.nValid,
+                             dense, nd, p.knnNormal, 0, p.invertNormal, c.vp, sampleStep * 0.5f, s);
+    }
+    PPF_HIP_CHECK(hipEventRecord(D.ev[7], s));
+    // icp (ppf.cpp:436-495)
+    std::vector<int> cand;
+    for (int i = 0; i < (int)hyp.size(); i++)
+        if (!(hyp[i].votes < voteThreshold))
+            cand.push_back(i);
+    std::vector<float> poseNow(hyp.size() * 12), scoreNow(hyp.size());
+    std::vector<char>  alive(hyp.size(), 0);
+    for (int i : cand) {
+        std::memcpy(&poseNow[12 * (size_t)i], hyp[i].pose, sizeof(float) * 12);
+        scoreNow[i] = hyp[i].votes;
+        alive[i]    = 1;
+    }
+    int nSparse = 0, nDenseIcp = 0;
+    bool needGrids = p.sparsePoseRefinement && !cand.empty();
+    if (needGrids) {
+        grid_build(D.ws.gSparse, PX, PY, PZ, NX, NY, NZ, samp, ns, sampleStep, s);
+        grid_build(D.ws.gFull, PX, PY, PZ, nullptr, nullptr, nullptr, c.valid.get<int>(),
+                   c.nValid, std::max(scoringDist * 2.0f, sampleStep * 0.25f), s);
+        IcpCriteria sc{5, poseRefDist, sampleStep * 0.5f, sampleStep, 0.05f};  // ppf.cpp:423
+        std::vector<float>        init(cand.size() * 12);
+        std::vector<IcpResultDev> res(cand.size());
+        for (size_t k = 0; k < cand.size(); k++)
+            std::memcpy(&init[12 * k], hyp[cand[k]].pose, sizeof(float) * 12);
+        icp_batch(D.ws, refNum, dm.sx.get<float>(), dm.sy.get<float>(), dm.sz.get<float>(),
+                  dm.snx.get<float>(), dm.sny.get<float>(), dm.snz.get<float>(),
+                  D.ws.gSparse.view, D.ws.gFull.view, init.data(), (int)cand.size(), sc,
+                  scoringDist, D.icfg, res.data(), s);
+        nSparse = (int)cand.size();
+        std::vector<int> survivors;
+        for (size_t k = 0; k < cand.size(); k++) {
+            int i = cand[k];
+            if (!res[k].converged) {
+                alive[i] = 0;
+                continue;
+            }
+            std::memcpy(&poseNow[12 * (size_t)i], res[k].pose, sizeof(float) * 12);
+            float score = (float)res[k].inliers / (float)refNum;
+            if (score > 1.0f)
+                score = 1.0f;
+            scoreNow[i] = score;
+            if (score < minScore) {
+                alive[i] = 0;
+                continue;
+            }
+            survivors.push_back(i);
+        }
+        if (p.densePoseRefinement && !survivors.empty()) {
+            grid_build(D.ws.gDense, PX, PY, PZ, NX, NY, NZ, dense, nd, reSampleStep * 2.0f, s);
+            IcpCriteria dc{p.poseRefNumSteps, poseRefDist, reSampleStep * 0.5f, sampleStep, 0.05f};
+            std::vector<float>        init2(survivors.size() * 12);
+            std::vector<IcpResultDev> res2(survivors.size());
+            for (size_t k = 0; k < survivors.size(); k++)
+                std::memcpy(&init2[12 * k], &poseNow[12 * (size_t)survivors[k]], sizeof(float) * 12);
+            IcpConfig dcfg = D.icfg;
+            dcfg.block     = std::max(dcfg.block, 256);
+            icp_batch(D.ws, hm.reSampled.size(), dm.rx.get<float>(), dm.ry.get<float>(),
+                      dm.rz.get<float>(), dm.rnx.get<float>(), dm.rny.get<float>(),
+                      dm.rnz.get<float>(), D.ws.gDense.view, D.ws.gFull.view, init2.data(),
+                      (int)survivors.size(), dc, scoringDist, dcfg, res2.data(), s);
+            nDenseIcp = (int)survivors.size();
+            for (size_t k = 0; k < survivors.size(); k++) {
+                int i = survivors[k];
+                if (!res2[k].converged) {
+                    alive[i] = 0;
+                    continue;
+                }
+                std::memcpy(&poseNow[12 * (size_t)i], res2[k].pose, sizeof(float) * 12);
+                float score = (float)res2[k].inliers / (float)hm.reSampled.size();
+                if (score > 1.0f)
+                    score = 1.0f;
+                scoreNow[i] = score;
+            }
+        }
+    }
+    // ordered accept walk (ppf.cpp:439-495)
+    for (int i : cand) {
+        if (!alive[i])
+            continue;
+        float score = scoreNow[i];
+        if ((p.sparsePoseRefinement || p.densePoseRefinement) && score < minScore)
+            continue;
+        out.results.emplace_back(score, std::vector<float>(&poseNow[12 * (size_t)i],
+                                                           &poseNow[12 * (size_t)i] + 12));
+        if ((int)out.results.size() >= p.numMatches)
+            break;
+    }
+    // ppf.cpp:498-499 (stable on ties)
+    std::stable_sort(out.results.begin(), out.results.end(),
+                     [](const auto &a, const auto &b) { return a.first > b.first; });
+    PPF_HIP_CHECK(hipEventRecord(D.ev[8], s));
+    PPF_HIP_CHECK(hipEventSynchronize(D.ev[8]));
+    st->nScene          = c.n;
+    st->nValid          = c.nValid;
+    st->nSampled        = ns;
+    st->nKeypoints      = nk;
+    st->nPoses          = np;
+    st->nClusters       = nClusters;
+    st->nHypotheses     = (int)hyp.size();
+    st->nSparseIcp      = nSparse;
+    st->nDenseIcp       = nDenseIcp;
+    st->nResults        = (int)out.results.size();
+    st->nDense          = nd;
+    st->nPairs          = vs.pairs;
+    st->nKeyHits        = vs.hits;
+    st->nVotes          = vs.votes;
+    st->msSample        = elapsed(D.ev[0], D.ev[1]);
+    st->msNormals       = elapsed(D.ev[1], D.ev[2]);
+    st->msKeypoints     = elapsed(D.ev[2], D.ev[3]);
+    st->msVote          = elapsed(D.ev[3], D.ev[5]);
+    st->msVoteKernel    = elapsed(D.ev[4], D.ev[5]);
+    st->msCluster       = elapsed(D.ev[5], D.ev[6]);
+    st->msIcpPrepare    = elapsed(D.ev[6], D.ev[7]);
+    st->msIcp           = elapsed(D.ev[7], D.ev[8]);
+    st->msTotal         = elapsed(D.ev[0], D.ev[8]);
+}
+
+void write_poses(const MatchOut &o, float *poses16, float *scores, size_t cap, size_t *nOut) {
+    size_t n = o.results.size();
+    if (nOut)
+        *nOut = n;
+    for (size_t i = 0; i < n && i < cap; i++) {
+        const float *P = o.results[i].second.data();
+        if (poses16) {
+            float *M = poses16 + 16 * i;  // column-major 4x4 (Eigen::Matrix4f)
+            for (int r = 0; r < 3; r++)
+                for (int c = 0; c < 4; c++)
+                    M[c * 4 + r] = P[r * 4 + c];
+            M[3] = M[7] = M[11] = 0.0f;
+            M[15]               = 1.0f;
+        }
+        if (scores)
+            scores[i] = o.results[i].first;
+    }
+}
+
+// ---------------------------------------------------------------- model file
+
+template <class T> void wr(std::ofstream &f, const T &v) {
+    f.write(reinterpret_cast<const char *>(&v), sizeof(T));
+}
+template <class T> void rd(std::ifstream &f, T &v) {
+    f.read(reinterpret_cast<char *>(&v), sizeof(T));
+}
+
+void write_cloud(std::ofstream &f, const HostCloud &c) {  // serialize.cpp:59-86
+    uint32_t np = (uint32_t)c.size();
+    wr(f, np);
+    f.write(reinterpret_cast<const char *>(c.p.data()), sizeof(float) * c.p.size());
+    uint32_t nn = (uint32_t)(c.n.size() / 3);
+    wr(f, nn);
+    f.write(reinterpret_cast<const char *>(c.n.data()), sizeof(float) * c.n.size());
+    f.write(reinterpret_cast<const char *>(c.boxMin), sizeof(float) * 3);
+    f.write(reinterpret_cast<const char *>(c.boxMax), sizeof(float) * 3);
+    f.write(reinterpret_cast<const char *>(c.vp), sizeof(float) * 3);
+}
+
+void read_cloud(std::ifstream &f, HostCloud &c) {
+    uint32_t np = 0, nn = 0;
+    rd(f, np);
+    c.p.resize((size_t)np * 3);
+    f.read(reinterpret_cast<char *>(c.p.data()), sizeof(float) * c.p.size());
+    rd(f, nn);
+    c.n.resize((size_t)nn * 3);
+    f.read(reinterpret_cast<char *>(c.n.data()), sizeof(float) * c.n.size());
+    f.read(reinterpret_cast<char *>(c.boxMin), sizeof(float) * 3);
+    f.read(reinterpret_cast<char *>(c.boxMax), sizeof(float) * 3);
+    f.read(reinterpret_cast<char *>(c.vp), sizeof(float) * 3);
+}
+
+void save_impl(const ppf_detector_s &D, const char *path) {
+    // ppf.cpp:512-527 (the file is created before the model check, as there)
+    std::ofstream f(path, std::ios::out | std::ios::binary);
+    if (!f.is_open())
+        throw std::runtime_error(std::string("failed to open file:") + path);
+    if (!D.trained)
+        throw std::runtime_error("No trained model in save");
+    const HostModel &m = D.hm;
+    wr(f, kMagic);
+    wr(f, kVersion);
+    wr(f, m.samplingDistanceRel);
+    wr(f, m.param.featDistanceStepRel);  // serialize.cpp:180-186
+    wr(f, m.param.featAngleResolution);
+    wr(f, m.param.poseRefRelSamplingDistance);
+    wr(f, m.param.knnNormal);
+    uint8_t smooth = m.param.smoothNormal ? 1 : 0;
+    wr(f, smooth);
+    write_cloud(f, m.sampled);
+    write_cloud(f, m.reSampled);
+    uint32_t nk = (uint32_t)m.keys.size();  // serialize.cpp:164-170
+    wr(f, nk);
+    for (uint32_t k = 0; k < nk; k++) {
+        wr(f, m.keys[k]);
+        uint32_t len = (uint32_t)(m.offs[k + 1] - m.offs[k]);
+        wr(f, len);
+        f.write(reinterpret_cast<const char *>(&m.refs[m.offs[k]]), sizeof(uint32_t) * len);
+        wr(f, len);
+        f.write(reinterpret_cast<const char *>(&m.alphas[m.offs[k]]), sizeof(float) * len);
+    }
+    f.close();
+}
+
+void load_impl(ppf_detector_s &D, const char *path) {
+    std::ifstream f(path, std::ios::binary);  // ppf.cpp:529-547
+    if (!f.is_open())
+        throw std::runtime_error(std::string("failed to open file:") + path);
+    D.trained = false;
+    int magic = 0, version = 0;
+    rd(f, magic);
+    if (magic != kMagic)
+        throw std::runtime_error(std::string("unsupported file format:") + path);
+    rd(f, version);  // not checked, as in the reference
+    HostModel m;
+    rd(f, m.samplingDistanceRel);
+    rd(f, m.param.featDistanceStepRel);
+    rd(f, m.param.featAngleResolution);
+    rd(f, m.param.poseRefRelSamplingDistance);
+    rd(f, m.param.knnNormal);
+    uint8_t smooth = 0;
+    rd(f, smooth);
+    m.param.smoothNormal = smooth ? 1 : 0;
+    read_cloud(f, m.sampled);
+    read_cloud(f, m.reSampled);
+    uint32_t nk = 0;
+    rd(f, nk);
+    // buckets keep the file's intra-bucket order (it decides which entries
+    // take the bin tail rule); buckets are re-ordered by key for the CSR.
+    std::vector<std::pair<uint32_t, std::pair<std::vector<uint32_t>, std::vector<float>>>> b(nk);
+    for (uint32_t k = 0; k < nk && f.good(); k++) {
+        uint32_t len = 0, lenA = 0;
+        rd(f, b[k].first);
+        rd(f, len);
+        b[k].second.first.resize(len);
+        f.read(reinterpret_cast<char *>(b[k].second.first.data()), sizeof(uint32_t) * len);
+        rd(f, lenA);
+        b[k].second.second.resize(lenA);
+        f.read(reinterpret_cast<char *>(b[k].second.second.data()), sizeof(float) * lenA);
+        if (lenA != len)
+            throw std::runtime_error(std::string("corrupt hash table in:") + path);
+    }
+    if (!f.good())
+        throw std::runtime_error(std::string("truncated model file:") + path);
+    std::stable_sort(b.begin(), b.end(), [](const auto &x, const auto &y) { return x.first < y.first; });
+    m.offs.assign(1, 0);
+    for (auto &e : b) {
+        m.keys.push_back(e.first);
+        m.refs.insert(m.refs.end(), e.second.first.begin(), e.second.first.end());
+        m.alphas.insert(m.alphas.end(), e.second.second.begin(), e.second.second.end());
+        m.offs.push_back((int)m.refs.size());
+    }
+    D.hm = std::move(m);
+    upload_model(D);
+    D.trained = true;
+}
+
+template <class F> int guarded(F &&f) {
+    try {
+        f();
+        return PPF_OK;
+    } catch (const std::range_error &e) {
+        g_lastError = e.what();
+        return PPF_ERR_RANGE;
+    } catch (const HipError &e) {
+        g_lastError = e.what();
+        return PPF_ERR_DEVICE;
+    } catch (const std::exception &e) {
+        g_lastError = e.what();
+        return PPF_ERR_RUNTIME;
+    } catch (...) {
+        g_lastError = "unknown error";
+        return PPF_ERR_RUNTIME;
+    }
+}
+
+#define PPF_REQUIRE(cond, msg)                 \
+    do {                                       \
+        if (!(cond)) {                         \
+            g_lastError = msg;                 \
+            return PPF_ERR_ARG;                \
+        }                                      \
+    } while (0)
+
+// ---------------------------------------------------------------- small kernels for stage APIs
+
+__global__ void radius_kernel(GridView g, const float *q, int nq, float r2, int *counts) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nq)
+        return;
+    float qx = q[3 * t], qy = q[3 * t + 1], qz = q[3 * t + 2];
+    float r  = sqrtf(r2) * 1.0001f + 1e-3f * g.cell;
+    int   xlo = cell_coord(qx - r, g.ox, g.cell, g.dx), xhi = cell_coord(qx + r, g.ox, g.cell, g.dx);
+    int   ylo = cell_coord(qy - r, g.oy, g.cell, g.dy), yhi = cell_coord(qy + r, g.oy, g.cell, g.dy);
+    int   zlo = cell_coord(qz - r, g.oz, g.cell, g.dz), zhi = cell_coord(qz + r, g.oz, g.cell, g.dz);
+    int   c = 0;
+    for (int cz = zlo; cz <= zhi; cz++)
+        for (int cy = ylo; cy <= yhi; cy++)
+            for (int cx = xlo; cx <= xhi; cx++) {
+                int s0, c0;
+                if (!cell_lookup(g, cx, cy, cz, s0, c0))
+                    continue;
+                for (int k = s0; k < s0 + c0; k++)
+                    if (ppfc::dist2(qx, qy, qz, g.x[k], g.y[k], g.z[k]) < r2)
+                        c++;
+            }
+    counts[t] = c;
+}
+
+__global__ void radius_fill_kernel(GridView g, const float *q, int nq, float r2, const int *offs,
+                                   int *ids, int cap) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nq)
+        return;
+    float qx = q[3 * t], qy = q[3 * t + 1], qz = q[3 * t + 2];
+    float r  = sqrtf(r2) * 1.0001f + 1e-3f * g.cell;
+    int   xlo = cell_coord(qx - r, g.ox, g.cell, g.dx), xhi = cell_coord(qx + r, g.ox, g.cell, g.dx);
+    int   ylo = cell_coord(qy - r, g.oy, g.cell, g.dy), yhi = cell_coord(qy + r, g.oy, g.cell, g.dy);
+    int   zlo = cell_coord(qz - r, g.oz, g.cell, g.dz), zhi = cell_coord(qz + r, g.oz, g.cell, g.dz);
+    int   o = offs[t];
+    for (int cz = zlo; cz <= zhi; cz++)
+        for (int cy = ylo; cy <= yhi; cy++)
+            for (int cx = xlo; cx <= xhi; cx++) {
+                int s0, c0;
+                if (!cell_lookup(g, cx, cy, cz, s0, c0))
+                    continue;
+                for (int k = s0; k < s0 + c0; k++)
+                    if (ppfc::dist2(qx, qy, qz, g.x[k], g.y[k], g.z[k]) < r2) {
+                        if (o < cap)
+                            ids[o] = g.id[k];
+                        o++;
+                    }
+            }
+}
+
+__global__ void knn_kernel(GridView g, const float *q, int nq, int k, int *ids, float *d2) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nq)
+        return;
+    KnnList<32> L;
+    grid_knn<32>(g, q[3 * t], q[3 * t + 1], q[3 * t + 2], k, L);
+    for (int i = 0; i < 32; i++) {
+        if (i >= k)
+            break;
+        ids[(size_t)t * k + i] = i < L.cnt ? L.id[i] : -1;
+        d2[(size_t)t * k + i]  = i < L.cnt ? L.d[i] : INFINITY;
+    }
+}
+
+__global__ void math_kernel(int op, const float *a, const float *b, int n, uint32_t *out) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    float x = a[i], y = b ? b[i] : 0.0f;
+    float r = 0.0f;
+    switch (op) {
+        case 0: r = ppfc::atanf_c(x); break;
+        case 1: r = ppfc::atan2f_c(x, y); break;
+        case 2: r = x / y; break;
+        case 3: r = sqrtf(x); break;
+        case 4:
+            out[i] = ppfc::murmur_key4(__float_as_uint(x), __float_as_uint(y), 0u, 0u);
+            return;
+        case 5: out[i] = ppfc::quant(x, y, false); return;
+        case 6: out[i] = ppfc::quant(x, y, true); return;
+        case 7: {
+            double d = sqrt((double)x * (double)y);
+            unsigned long long u = __double_as_longlong(d);
+            out[i] = (uint32_t)(u >> 32) ^ (uint32_t)u;
+            return;
+        }
+        default: break;
+    }
+    out[i] = __float_as_uint(r);
+}
+
+hipStream_t util_stream() {
+    static hipStream_t s = nullptr;
+    if (!s)
+        PPF_HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    return s;
+}
+
+struct TmpCloud {
+    DevBuf x, y, z, aos;
+    void up(const float *p, int n, hipStream_t s) {
+        int m = std::max(n, 1);
+        PPF_HIP_CHECK(hipMemcpyAsync(aos.as<float>((size_t)m * 3), p, sizeof(float) * 3 * n,
+                                     hipMemcpyHostToDevice, s));
+        split_kernel<<<div_up(m, 256), 256, 0, s>>>(aos.get<float>(), n, x.as<float>(m),
+                                                     y.as<float>(m), z.as<float>(m));
+    }
+};
+
+}  // namespace
+
+// ======================================================================== C ABI
+
+extern "C" {
+
+PPF_API int ppf_train_param_default(ppf_train_param *p) {
+    PPF_REQUIRE(p, "null parameter");
+    *p = ppf_train_param{0.04f, 30, 0.01f, 10, 1};  // type.h:71-73
+    return PPF_OK;
+}
+
+PPF_API int ppf_match_param_default(ppf_match_param *p) {
+    PPF_REQUIRE(p, "null parameter");
+    *p = ppf_match_param{1, 10, 1, 0, 0.5f, 0.0f, 1, 1, 5, 0.1f, 0.0f, 0.01f, 0.0f};  // type.h:170-175
+    return PPF_OK;
+}
+
+PPF_API int ppf_train_param_check(const ppf_train_param *p) {
+    PPF_REQUIRE(p, "null parameter");
+    return guarded([&] { check_train_param(*p); });
+}
+
+PPF_API int ppf_match_param_check(const ppf_match_param *p) {
+    PPF_REQUIRE(p, "null parameter");
+    return guarded([&] { check_match_param(*p); });
+}
+
+PPF_API int ppf_detector_create(ppf_detector *out) {
+    PPF_REQUIRE(out, "null output handle");
+    *out = nullptr;
+    return guarded([&] {
+        int dev = 0;
+        PPF_HIP_CHECK(hipGetDeviceCount(&dev));
+        if (dev < 1)
+            throw HipError("no HIP device available");
+        *out = new ppf_detector_s();
+    });
+}
+
+PPF_API void ppf_detector_destroy(ppf_detector d) { delete d; }
+
+PPF_API int ppf_train_model(ppf_detector d, const float *points, const float *normals, size_t n,
+                            const float *box6, const float *viewPoint, float rel,
+                            const ppf_train_param *param) {
+    PPF_REQUIRE(d, "null detector");
+    return guarded([&] {
+        ppf_train_param p;
+        if (param)
+            p = *param;
+        else
+            ppf_train_param_default(&p);
+        train_impl(*d, points, normals, (int)n, box6, viewPoint, rel, p);
+    });
+}
+
+PPF_API int ppf_match_scene(ppf_detector d, const float *points, const float *normals, size_t n,
+                            const float *box6, const float *viewPoint, float rel, float kpFrac,
+                            float minScore, const ppf_match_param *param, float *poses16,
+                            float *scores, size_t cap, size_t *nOut, ppf_match_result *result,
+                            ppf_match_stats *stats) {
+    PPF_REQUIRE(d, "null detector");
+    return guarded([&] {
+        if (!d->trained)
+            throw std::runtime_error("No trained model in matchScene");
+        ppf_match_param p;
+        if (param)
+            p = *param;
+        else
+            ppf_match_param_default(&p);
+        if (n == 0 || !points)
+            throw std::runtime_error("Invalid Input: empty scene in matchScene");
+        d->tmpScene.upload(points, normals, (int)n, box6, viewPoint, d->stream);
+        MatchOut o;
+        match_impl(*d, d->tmpScene, rel, kpFrac, minScore, p, o, result, stats);
+        write_poses(o, poses16, scores, cap, nOut);
+    });
+}
+
+PPF_API int ppf_save(ppf_detector d, const char *path) {
+    PPF_REQUIRE(d && path, "null argument");
+    return guarded([&] { save_impl(*d, path); });
+}
+
+PPF_API int ppf_load(ppf_detector d, const char *path) {
+    PPF_REQUIRE(d && path, "null argument");
+    return guarded([&] { load_impl(*d, path); });
+}
+
+PPF_API const char *ppf_last_error(void) { return g_lastError.c_str(); }
+
+PPF_API int ppf_scene_upload(ppf_detector d, const float *points, const float *normals, size_t n,
+                             const float *box6, const float *viewPoint, ppf_scene *out) {
+    PPF_REQUIRE(d && out, "null argument");
+    *out = nullptr;
+    return guarded([&] {
+        auto sc = std::make_unique<ppf_scene_s>();
+        sc->cloud.upload(points, normals, (int)n, box6, viewPoint, d->stream);
+        *out = sc.release();
+    });
+}
+
+PPF_API void ppf_scene_free(ppf_scene s) { delete s; }
+
+PPF_API int ppf_match_scene_resident(ppf_detector d, ppf_scene sc, float rel, float kpFrac,
+                                     float minScore, const ppf_match_param *param, float *poses16,
+                                     float *scores, size_t cap, size_t *nOut,
+                                     ppf_match_stats *stats) {
+    PPF_REQUIRE(d && sc, "null argument");
+    return guarded([&] {
+        ppf_match_param p;
+        if (param)
+            p = *param;
+        else
+            ppf_match_param_default(&p);
+        MatchOut o;
+        match_impl(*d, sc->cloud, rel, kpFrac, minScore, p, o, nullptr, stats);
+        write_poses(o, poses16, scores, cap, nOut);
+    });
+}
+
+PPF_API int ppf_model_get_info(ppf_detector d, ppf_model_info *info) {
+    PPF_REQUIRE(d && info, "null argument");
+    return guarded([&] {
+        if (!d->trained)
+            throw std::runtime_error("No trained model");
+        const HostModel &m     = d->hm;
+        info->nSampled         = m.sampled.size();
+        info->nReSampled       = m.reSampled.size();
+        info->nKeys            = (int)m.keys.size();
+        info->nEntries         = (int)m.refs.size();
+        info->angleNum         = m.param.featAngleResolution;
+        info->diameter         = box_diameter(m.sampled.boxMin, m.sampled.boxMax);
+        info->samplingDistanceRel = m.samplingDistanceRel;
+        std::memcpy(info->boxMin, m.sampled.boxMin, sizeof info->boxMin);
+        std::memcpy(info->boxMax, m.sampled.boxMax, sizeof info->boxMax);
+    });
+}
+
+PPF_API int ppf_model_get_sampled(ppf_detector d, float *pts, float *nrm, float *rePts,
+                                  float *reNrm) {
+    PPF_REQUIRE(d, "null detector");
+    return guarded([&] {
+        if (!d->trained)
+            throw std::runtime_error("No trained model");
+        const HostModel &m = d->hm;
+        if (pts) std::memcpy(pts, m.sampled.p.data(), sizeof(float) * m.sampled.p.size());
+        if (nrm) std::memcpy(nrm, m.sampled.n.data(), sizeof(float) * m.sampled.n.size());
+        if (rePts) std::memcpy(rePts, m.reSampled.p.data(), sizeof(float) * m.reSampled.p.size());
+        if (reNrm) std::memcpy(reNrm, m.reSampled.n.data(), sizeof(float) * m.reSampled.n.size());
+    });
+}
+
+PPF_API int ppf_model_get_table(ppf_detector d, uint32_t *keys, int *offs, uint32_t *refs,
+                                float *alphas) {
+    PPF_REQUIRE(d, "null detector");
+    return guarded([&] {
+        if (!d->trained)
+            throw std::runtime_error("No trained model");
+        const HostModel &m = d->hm;
+        if (keys) std::memcpy(keys, m.keys.data(), sizeof(uint32_t) * m.keys.size());
+        if (offs) std::memcpy(offs, m.offs.data(), sizeof(int) * m.offs.size());
+        if (refs) std::memcpy(refs, m.refs.data(), sizeof(uint32_t) * m.refs.size());
+        if (alphas) std::memcpy(alphas, m.alphas.data(), sizeof(float) * m.alphas.size());
+    });
+}
+
+PPF_API int ppf_pair_features(const float p1n1[6], const float *pts, const float *nrm, size_t n,
+                              float angleStep, float distStep, int tail, uint32_t *keys,
+                              float *alphas) {
+    PPF_REQUIRE(p1n1 && (pts || n == 0) && (nrm || n == 0) && keys && alphas, "null argument");
+    return guarded([&] {
+        hipStream_t s = util_stream();
+        TmpCloud    P, N;
+        P.up(pts, (int)n, s);
+        N.up(nrm, (int)n, s);
+        DevBuf kd, ad;
+        uint32_t *k = kd.as<uint32_t>(std::max<size_t>(n, 1));
+        float    *a = ad.as<float>(std::max<size_t>(n, 1));
+        pair_features(p1n1, (int)n, P.x.get<float>(), P.y.get<float>(), P.z.get<float>(),
+                      N.x.get<float>(), N.y.get<float>(), N.z.get<float>(), angleStep, distStep,
+                      tail, k, a, s);
+        PPF_HIP_CHECK(hipMemcpyAsync(keys, k, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, s));
+        PPF_HIP_CHECK(hipMemcpyAsync(alphas, a, sizeof(float) * n, hipMemcpyDeviceToHost, s));
+        PPF_HIP_CHECK(hipStreamSynchronize(s));
+    });
+}
+
+PPF_API int ppf_vote(ppf_detector d, const float *pts, const float *nrm, size_t n,
+                     const int *keypoints, size_t nk, float minScore, int *cand3, int32_t *accDump,
+                     size_t dumpCount, float *poses12, int *nPoses) {
+    PPF_REQUIRE(d && pts && nrm && keypoints && cand3, "null argument");
+    return guarded([&] {
+        if (!d->trained)
+            throw std::runtime_error("No trained model in matchScene");
+        hipStream_t s = d->stream;
+        DevCloud   &c = d->tmpScene;
+        c.upload(pts, nrm, (int)n, nullptr, nullptr, s);
+        // the vote stage reads positions and normals as given
+        size_t b = sizeof(float) * n;
+        PPF_HIP_CHECK(hipMemcpyAsync(c.px.get<float>(), c.ox.get<float>(), b, hipMemcpyDeviceToDevice, s));
+        PPF_HIP_CHECK(hipMemcpyAsync(c.py.get<float>(), c.oy.get<float>(), b, hipMemcpyDeviceToDevice, s));
+        PPF_HIP_CHECK(hipMemcpyAsync(c.pz.get<float>(), c.oz.get<float>(), b, hipMemcpyDeviceToDevice, s));
+        PPF_HIP_CHECK(hipMemcpyAsync(c.nx.get<float>(), c.onx.get<float>(), b, hipMemcpyDeviceToDevice, s));
+        PPF_HIP_CHECK(hipMemcpyAsync(c.ny.get<float>(), c.ony.get<float>(), b, hipMemcpyDeviceToDevice, s));
+        PPF_HIP_CHECK(hipMemcpyAsync(c.nz.get<float>(), c.onz.get<float>(), b, hipMemcpyDeviceToDevice, s));
+        const HostModel &hm = d->hm;
+        float Dm = box_diameter(hm.sampled.boxMin, hm.sampled.boxMax);
+        // every point is a sampled point
+        std::vector<int> all(n);
+        std::iota(all.begin(), all.end(), 0);
+        int *samp = d->idxA.as<int>(std::max<size_t>(n, 1));
+        int *kp   = d->idxB.as<int>(std::max<size_t>(nk, 1));
+        PPF_HIP_CHECK(hipMemcpyAsync(samp, all.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
+        PPF_HIP_CHECK(hipMemcpyAsync(kp, keypoints, sizeof(int) * nk, hipMemcpyHostToDevice, s));
+        grid_build(d->ws.gVote, c.px.get<float>(), c.py.get<float>(), c.pz.get<float>(),
+                   c.nx.get<float>(), c.ny.get<float>(), c.nz.get<float>(), samp, (int)n, Dm * 0.5f,
+                   s);
+        DevBuf dump;
+        int32_t *dd = nullptr;
+        if (accDump && dumpCount) {
+            dd = dump.as<int32_t>(dumpCount * (size_t)d->dm.nSampled * d->dm.angleNum);
+            PPF_HIP_CHECK(hipMemsetAsync(dd, 0, sizeof(int32_t) * dumpCount * d->dm.nSampled * d->dm.angleNum, s));
+        }
+        float voteThreshold = (float)hm.sampled.size() * minScore;
+        int   np = vote_and_poses(d->ws, d->dm, d->ws.gVote.view, c.px.get<float>(),
+                                  c.py.get<float>(), c.pz.get<float>(), c.nx.get<float>(),
+                                  c.ny.get<float>(), c.nz.get<float>(), kp, (int)nk, voteThreshold,
+                                  d->vcfg, nullptr, dd, (int)dumpCount, s);
+        std::vector<PoseRec> pr(np);
+        PPF_HIP_CHECK(hipMemcpyAsync(pr.data(), d->ws.poses.get<PoseRec>(), sizeof(PoseRec) * np,
+                                     hipMemcpyDeviceToHost, s));
+        if (dd)
+            PPF_HIP_CHECK(hipMemcpyAsync(accDump, dd,
+                                         sizeof(int32_t) * dumpCount * d->dm.nSampled * d->dm.angleNum,
+                                         hipMemcpyDeviceToHost, s));
+        PPF_HIP_CHECK(hipStreamSynchronize(s));
+        for (size_t k = 0; k < nk * 9; k++)
+            cand3[k] = -1;
+        for (int i = 0; i < np; i++) {
+            int *c3 = cand3 + ((size_t)pr[i].keypoint * 3 + pr[i].rank) * 3;
+            c3[0]   = pr[i].refId;
+            c3[1]   = pr[i].angleId;
+            c3[2]   = (int)pr[i].votes;
+            if (poses12)
+                std::memcpy(poses12 + 12 * (size_t)i, pr[i].pose, sizeof(float) * 12);
+        }
+        if (nPoses)
+            *nPoses = np;
+    });
+}
+
+PPF_API int ppf_voxel_sample(const float *pts, size_t n, const int *subset, size_t nSubset,
+                             float step, const float boxMin[3], int *out, size_t *nOut) {
+    PPF_REQUIRE(pts && boxMin && out && nOut, "null argument");
+    return guarded([&] {
+        hipStream_t s = util_stream();
+        TmpCloud    P;
+        P.up(pts, (int)n, s);
+        static Workspace *wsp = new Workspace();  // lives until exit, never destroyed
+        Workspace        &ws  = *wsp;
+        DevBuf sub, o, cnt;
+        int    m  = subset ? (int)nSubset : (int)n;
+        int   *sd = nullptr;
+        if (subset) {
+            sd = sub.as<int>(std::max(m, 1));
+            PPF_HIP_CHECK(hipMemcpyAsync(sd, subset, sizeof(int) * m, hipMemcpyHostToDevice, s));
+        }
+        int *od = o.as<int>(std::max(m, 1));
+        int  c  = voxel_sample(ws, P.x.get<float>(), P.y.get<float>(), P.z.get<float>(), sd, m,
+                               step, boxMin, od, cnt.as<int>(4), s);
+        PPF_HIP_CHECK(hipMemcpyAsync(out, od, sizeof(int) * c, hipMemcpyDeviceToHost, s));
+        PPF_HIP_CHECK(hipStreamSynchronize(s));
+        *nOut = (size_t)c;
+    });
+}
+
+PPF_API int ppf_radius_search(const float *pts, size_t n, const float *q, size_t nq, float r2,
+                              float cell, int *counts, int *ids, size_t cap) {
+    PPF_REQUIRE(pts && q && counts, "null argument");
+    return guarded([&] {
+        hipStream_t s = util_stream();
+        TmpCloud    P, Q;
+        P.up(pts, (int)n, s);
+        Grid g;
+        grid_build(g, P.x.get<float>(), P.y.get<float>(), P.z.get<float>(), nullptr, nullptr,
+                   nullptr, nullptr, (int)n, cell, s);
+        DevBuf qd, cd, od, idd, tmp;
+        float *qq = qd.as<float>(std::max<size_t>(nq, 1) * 3);
+        PPF_HIP_CHECK(hipMemcpyAsync(qq, q, sizeof(float) * 3 * nq, hipMemcpyHostToDevice, s));
+        int *c = cd.as<int>(nq + 1), *o = od.as<int>(nq + 1);
+        if (nq)
+            radius_kernel<<<div_up(nq, 128), 128, 0, s>>>(g.view, qq, (int)nq, r2, c);
+        PPF_HIP_CHECK(hipMemcpyAsync(counts, c, sizeof(int) * nq, hipMemcpyDeviceToHost, s));
+        PPF_HIP_CHECK(hipStreamSynchronize(s));
+        if (ids && cap) {
+            std::vector<int> off(nq + 1, 0);
+            for (size_t t = 0; t < nq; t++)
+                off[t + 1] = off[t] + counts[t];
+            PPF_HIP_CHECK(hipMemcpyAsync(o, off.data(), sizeof(int) * (nq + 1), hipMemcpyHostToDevice, s));
+            int *idv = idd.as<int>(cap);
+            if (nq)
+                radius_fill_kernel<<<div_up(nq, 128), 128, 0, s>>>(g.view, qq, (int)nq, r2, o, idv,
+                                                                    (int)cap);
+            size_t w = std::min<size_t>(cap, (size_t)off[nq]);
+            PPF_HIP_CHECK(hipMemcpyAsync(ids, idv, sizeof(int) * w, hipMemcpyDeviceToHost, s));
+            PPF_HIP_CHECK(hipStreamSynchronize(s));
+            // ascending ids per query, as the oracle reports them
+            for (size_t t = 0; t < nq; t++) {
+                size_t b = std::min<size_t>(off[t], w), e = std::min<size_t>(off[t + 1], w);
+                std::sort(ids + b, ids + e);
+            }
+        }
+    });
+}
+
+PPF_API int ppf_knn_search(const float *pts, size_t n, const float *q, size_t nq, int k,
+                           float cell, int *ids, float *d2) {
+    PPF_REQUIRE(pts && q && ids && d2 && k >= 1 && k <= 32, "bad argument");
+    return guarded([&] {
+        hipStream_t s = util_stream();
+        TmpCloud    P;
+        P.up(pts, (int)n, s);
+        Grid g;
+        grid_build(g, P.x.get<float>(), P.y.get<float>(), P.z.get<float>(), nullptr, nullptr,
+                   nullptr, nullptr, (int)n, cell, s);
+        DevBuf qd, idd, dd;
+        float *qq = qd.as<float>(std::max<size_t>(nq, 1) * 3);
+        PPF_HIP_CHECK(hipMemcpyAsync(qq, q, sizeof(float) * 3 * nq, hipMemcpyHostToDevice, s));
+        int   *iv = idd.as<int>(std::max<size_t>(nq, 1) * k);
+        float *dv = dd.as<float>(std::max<size_t>(nq, 1) * k);
+        if (nq)
+            knn_kernel<<<div_up(nq, 128), 128, 0, s>>>(g.view, qq, (int)nq, k, iv, dv);
+        PPF_HIP_CHECK(hipMemcpyAsync(ids, iv, sizeof(int) * nq * k, hipMemcpyDeviceToHost, s));
+        PPF_HIP_CHECK(hipMemcpyAsync(d2, dv, sizeof(float) * nq * k, hipMemcpyDeviceToHost, s));
+        PPF_HIP_CHECK(hipStreamSynchronize(s));
+    });
+}
+
+PPF_API int ppf_estimate_normals(float *pts, float *nrm, size_t n, const int *valid,
+                                 size_t nValid, const int *query, size_t nq, int k, int smooth,
+                                 int invert, const float viewPoint[3]) {
+    PPF_REQUIRE(pts && nrm && valid && query, "null argument");
+    return guarded([&] {
+        hipStream_t s = util_stream();
+        TmpCloud    P, N;
+        P.up(pts, (int)n, s);
+        N.up(nrm, (int)n, s);
+        DevBuf vd, qd;
+        int   *v  = vd.as<int>(std::max<size_t>(nValid, 1));
+        int   *qq = qd.as<int>(std::max<size_t>(nq, 1));
+        PPF_HIP_CHECK(hipMemcpyAsync(v, valid, sizeof(int) * nValid, hipMemcpyHostToDevice, s));
+        PPF_HIP_CHECK(hipMemcpyAsync(qq, query, sizeof(int) * nq, hipMemcpyHostToDevice, s));
+        static Workspace *wsp = new Workspace();  // lives until exit, never destroyed
+        Workspace        &ws  = *wsp;
+        // cell ~ typical kNN radius: twice the cube root of volume per point
+        std::vector<int> vv(valid, valid + nValid);
+        float lo[3], hi[3];
+        bbox_host(pts, vv, lo, hi);
+        double vol = 1.0;
+        for (int a = 0; a < 3; a++)
+            vol *= (double)(hi[a] - lo[a]) + 1e-3;
+        float cell = (float)(2.0 * std::cbrt(vol / std::max<size_t>(nValid, 1)));
+        estimate_normals(ws, (int)n, P.x.get<float>(), P.y.get<float>(), P.z.get<float>(),
+                         N.x.get<float>(), N.y.get<float>(), N.z.get<float>(), v, (int)nValid, qq,
+                         (int)nq, k, smooth, invert, viewPoint, cell, s);
+        DevBuf outb;
+        float *o = outb.as<float>(std::max<size_t>(n, 1) * 3);
+        int    m = std::max<int>((int)n, 1);
+        std::vector<int> all(n);
+        std::iota(all.begin(), all.end(), 0);
+        DevBuf ad;
+        int *alld = ad.as<int>(m);
+        PPF_HIP_CHECK(hipMemcpyAsync(alld, all.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
+        if (n) {
+            gather_aos_kernel<<<div_up(n, 256), 256, 0, s>>>(alld, (int)n, P.x.get<float>(),
+                                                              P.y.get<float>(), P.z.get<float>(), o);
+            PPF_HIP_CHECK(hipMemcpyAsync(pts, o, sizeof(float) * 3 * n, hipMemcpyDeviceToHost, s));
+            PPF_HIP_CHECK(hipStreamSynchronize(s));
+            gather_aos_kernel<<<div_up(n, 256), 256, 0, s>>>(alld, (int)n, N.x.get<float>(),
+                                                              N.y.get<float>(), N.z.get<float>(), o);
+            PPF_HIP_CHECK(hipMemcpyAsync(nrm, o, sizeof(float) * 3 * n, hipMemcpyDeviceToHost, s));
+        }
+        PPF_HIP_CHECK(hipStreamSynchronize(s));
+    });
+}
+
+PPF_API int ppf_device_math(int op, const float *a, const float *b, size_t n, uint32_t *out) {
+    PPF_REQUIRE(a && out, "null argument");
+    return guarded([&] {
+        hipStream_t s = util_stream();
+        DevBuf      ad, bd, od;
+        float      *A = ad.as<float>(std::max<size_t>(n, 1));
+        float      *B = b ? bd.as<float>(std::max<size_t>(n, 1)) : nullptr;
+        uint32_t   *O = od.as<uint32_t>(std::max<size_t>(n, 1));
+        PPF_HIP_CHECK(hipMemcpyAsync(A, a, sizeof(float) * n, hipMemcpyHostToDevice, s));
+        if (b)
+            PPF_HIP_CHECK(hipMemcpyAsync(B, b, sizeof(float) * n, hipMemcpyHostToDevice, s));
+        if (n)
+            math_kernel<<<div_up(n, 256), 256, 0, s>>>(op, A, B, (int)n, O);
+        PPF_HIP_CHECK(hipMemcpyAsync(out, O, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, s));
+        PPF_HIP_CHECK(hipStreamSynchronize(s));
+    });
+}
+
+}  // extern "C"

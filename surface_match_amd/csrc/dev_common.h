@@ -1,0 +1,341 @@
+// dev_common.h -- device plumbing shared by the PPF kernels: error checks,
+// growable device buffers, block-level scans and the hashed uniform grid.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <stdexcept>
+#include <string>
+#include <vector>
+#include <cstdint>
+#include <cfloat>
+
+#include "canon.h"
+
+namespace ppfd {
+
+struct HipError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+#define PPF_HIP_CHECK(expr)                                                                    \
+    do {                                                                                       \
+        hipError_t e_ = (expr);                                                                \
+        if (e_ != hipSuccess)                                                                  \
+            throw ::ppfd::HipError(std::string("HIP error ") + hipGetErrorString(e_) + " at " + \
+                                   __FILE__ + ":" + std::to_string(__LINE__) + " (" #expr ")"); \
+    } while (0)
+
+// A device allocation that only ever grows; reused across calls so the
+// steady-state match path performs no hipMalloc.
+struct DevBuf {
+    void  *ptr   = nullptr;
+    size_t bytes = 0;
+    DevBuf()     = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() {
+        if (ptr)
+            (void)hipFree(ptr);
+    }
+    void *reserve(size_t b) {
+        if (b <= bytes)
+            return ptr;
+        if (ptr)
+            PPF_HIP_CHECK(hipFree(ptr));
+        ptr   = nullptr;
+        bytes = 0;
+        size_t nb = b < 256 ? 256 : b + b / 8;
+        PPF_HIP_CHECK(hipMalloc(&ptr, nb));
+        bytes = nb;
+        return ptr;
+    }
+    template <class T> T *as(size_t n) { return reinterpret_cast<T *>(reserve(n * sizeof(T))); }
+    template <class T> T *get() const { return reinterpret_cast<T *>(ptr); }
+};
+
+inline int div_up(long long a, long long b) { return (int)((a + b - 1) / b); }
+
+// --------------------------------------------------------------------------
+// hashed uniform grid
+// --------------------------------------------------------------------------
+// Points of a cloud (or a subset) are sorted by cell (stable, so ids stay
+// ascending inside a cell); occupied cells live in an open-addressing table
+// {key+1, start, count, 0} (key+1 == 0 marks an empty slot).  Cells are
+// floor((p - org) / cell) per axis, clamped into [0, dim).
+struct GridView {
+    float           ox, oy, oz, cell;
+    int             dx, dy, dz;
+    uint32_t        hmask;
+    const uint4    *htab;
+    const float    *x, *y, *z;
+    const float    *nx, *ny, *nz;  // optional (nullptr)
+    const int      *id;            // original point id of each sorted slot
+    int             n;
+};
+
+__device__ __forceinline__ int cell_coord(float v, float o, float cell, int dim) {
+    int c = (int)floorf((v - o) / cell);
+    c     = c < 0 ? 0 : c;
+    return c >= dim ? dim - 1 : c;
+}
+__device__ __forceinline__ int cell_coord_unclamped(float v, float o, float cell) {
+    float f = floorf((v - o) / cell);
+    f       = fminf(fmaxf(f, -1.0e6f), 1.0e6f);
+    return (int)f;
+}
+__device__ __forceinline__ uint32_t cell_hash(uint32_t key) {
+    key *= 0x9E3779B1u;
+    return key ^ (key >> 16);
+}
+__device__ __forceinline__ bool cell_lookup(const GridView &g, int cx, int cy, int cz, int &start,
+                                            int &count) {
+    uint32_t key = (uint32_t)((cz * g.dy + cy) * g.dx + cx) + 1u;
+    uint32_t h   = cell_hash(key) & g.hmask;
+    for (;;) {
+        uint4 s = g.htab[h];
+        if (s.x == 0u)
+            return false;
+        if (s.x == key) {
+            start = (int)s.y;
+            count = (int)s.z;
+            return true;
+        }
+        h = (h + 1u) & g.hmask;
+    }
+}
+
+// squared distance from q to the axis-aligned box of cell (cx,cy,cz)
+__device__ __forceinline__ float cell_min_d2(const GridView &g, int cx, int cy, int cz, float qx,
+                                             float qy, float qz) {
+    // box grown by 1e-3 cell so float binning at cell faces never prunes a point
+    float m  = 1e-3f * g.cell;
+    float lx = g.ox + (float)cx * g.cell - m, ly = g.oy + (float)cy * g.cell - m,
+          lz = g.oz + (float)cz * g.cell - m, e = g.cell + 2.0f * m;
+    float ex = fmaxf(fmaxf(lx - qx, qx - (lx + e)), 0.0f);
+    float ey = fmaxf(fmaxf(ly - qy, qy - (ly + e)), 0.0f);
+    float ez = fmaxf(fmaxf(lz - qz, qz - (lz + e)), 0.0f);
+    return ex * ex + ey * ey + ez * ez;
+}
+
+// k nearest neighbours by (dist2, id) with ring expansion.  Results are
+// sorted ascending; returns the count found (< k only if the grid holds
+// fewer points).  KMAX is a compile-time bound on k.
+template <int KMAX>
+struct KnnList {
+    float d[KMAX];
+    int   id[KMAX];
+    int   cnt;
+    __device__ __forceinline__ void init() { cnt = 0; }
+    __device__ __forceinline__ void push(float dd, int ii, int k) {
+        if (cnt == k && !(dd < d[k - 1] || (dd == d[k - 1] && ii < id[k - 1])))
+            return;
+        int pos = 0;
+#pragma unroll
+        for (int i = 0; i < KMAX; i++)
+            pos += (i < cnt && (d[i] < dd || (d[i] == dd && id[i] < ii))) ? 1 : 0;
+#pragma unroll
+        for (int i = KMAX - 1; i > 0; i--) {
+            float od = d[i - 1];
+            int   oi = id[i - 1];
+            if (i > pos) {
+                d[i]  = od;
+                id[i] = oi;
+            } else if (i == pos) {
+                d[i]  = dd;
+                id[i] = ii;
+            }
+        }
+        if (pos == 0) {
+            d[0]  = dd;
+            id[0] = ii;
+        }
+        cnt = cnt < k ? cnt + 1 : k;
+    }
+};
+
+// Visit the cells of Chebyshev ring r around (c0x,c0y,c0z), calling f(start,count)
+// for occupied cells whose box is within bound2 of q.
+template <class F>
+__device__ __forceinline__ void visit_ring(const GridView &g, int c0x, int c0y, int c0z, int r,
+                                           float qx, float qy, float qz, float bound2, F &&f) {
+    int zlo = max(c0z - r, 0), zhi = min(c0z + r, g.dz - 1);
+    int ylo = max(c0y - r, 0), yhi = min(c0y + r, g.dy - 1);
+    int xlo = max(c0x - r, 0), xhi = min(c0x + r, g.dx - 1);
+    for (int cz = zlo; cz <= zhi; cz++) {
+        bool zs = (cz == c0z - r) || (cz == c0z + r);
+        for (int cy = ylo; cy <= yhi; cy++) {
+            bool ys = (cy == c0y - r) || (cy == c0y + r);
+            if (zs || ys) {
+                for (int cx = xlo; cx <= xhi; cx++) {
+                    if (cell_min_d2(g, cx, cy, cz, qx, qy, qz) > bound2)
+                        continue;
+                    int s, c;
+                    if (cell_lookup(g, cx, cy, cz, s, c))
+                        f(s, c);
+                }
+            } else {
+                // only the two x-end cells of this row lie on the shell
+                int xs[2] = {c0x - r, c0x + r};
+                for (int e = 0; e < (r == 0 ? 1 : 2); e++) {
+                    int cx = xs[e];
+                    if (cx < 0 || cx >= g.dx)
+                        continue;
+                    if (cell_min_d2(g, cx, cy, cz, qx, qy, qz) > bound2)
+                        continue;
+                    int s, c;
+                    if (cell_lookup(g, cx, cy, cz, s, c))
+                        f(s, c);
+                }
+            }
+        }
+    }
+}
+
+// distance from q to the boundary of the (2r+1)^3 block of cells around c0;
+// every point not yet visited is at least this far away.  Returns a huge
+// value once the block covers the whole grid.
+__device__ __forceinline__ float ring_safe(const GridView &g, int c0x, int c0y, int c0z, int r,
+                                           float qx, float qy, float qz) {
+    if (c0x - r <= 0 && c0x + r >= g.dx - 1 && c0y - r <= 0 && c0y + r >= g.dy - 1 &&
+        c0z - r <= 0 && c0z + r >= g.dz - 1)
+        return FLT_MAX;
+    float s = FLT_MAX;
+    if (c0x - r > 0)        s = fminf(s, qx - (g.ox + (float)(c0x - r) * g.cell));
+    if (c0x + r < g.dx - 1) s = fminf(s, (g.ox + (float)(c0x + r + 1) * g.cell) - qx);
+    if (c0y - r > 0)        s = fminf(s, qy - (g.oy + (float)(c0y - r) * g.cell));
+    if (c0y + r < g.dy - 1) s = fminf(s, (g.oy + (float)(c0y + r + 1) * g.cell) - qy);
+    if (c0z - r > 0)        s = fminf(s, qz - (g.oz + (float)(c0z - r) * g.cell));
+    if (c0z + r < g.dz - 1) s = fminf(s, (g.oz + (float)(c0z + r + 1) * g.cell) - qz);
+    s = s - 1e-3f * g.cell;
+    return s > 0.0f ? s : 0.0f;
+}
+
+template <int KMAX>
+__device__ void grid_knn(const GridView &g, float qx, float qy, float qz, int k,
+                         KnnList<KMAX> &L) {
+    L.init();
+    if (g.n == 0)
+        return;
+    int c0x = cell_coord_unclamped(qx, g.ox, g.cell);
+    int c0y = cell_coord_unclamped(qy, g.oy, g.cell);
+    int c0z = cell_coord_unclamped(qz, g.oz, g.cell);
+    int maxR = max(max(g.dx, g.dy), g.dz) + max(max(abs(c0x), abs(c0y)), abs(c0z)) + 1;
+    for (int r = 0; r <= maxR; r++) {
+        float bound2 = (L.cnt == k) ? L.d[k - 1] : FLT_MAX;
+        visit_ring(g, c0x, c0y, c0z, r, qx, qy, qz, bound2, [&](int s, int c) {
+            for (int t = s; t < s + c; t++) {
+                float dd = ppfc::dist2(qx, qy, qz, g.x[t], g.y[t], g.z[t]);
+                L.push(dd, g.id[t], k);
+            }
+        });
+        float safe = ring_safe(g, c0x, c0y, c0z, r, qx, qy, qz);
+        if (safe == FLT_MAX)
+            break;
+        if (L.cnt == k && L.d[k - 1] < safe * safe)
+            break;
+    }
+}
+
+// nearest neighbour by (dist2, id); returns the sorted slot (-1 if empty).
+__device__ inline int grid_nn1(const GridView &g, float qx, float qy, float qz, float &bestD,
+                               int &bestId) {
+    bestD      = FLT_MAX;
+    bestId     = 0x7fffffff;
+    int bestS  = -1;
+    if (g.n == 0)
+        return -1;
+    int c0x  = cell_coord_unclamped(qx, g.ox, g.cell);
+    int c0y  = cell_coord_unclamped(qy, g.oy, g.cell);
+    int c0z  = cell_coord_unclamped(qz, g.oz, g.cell);
+    int maxR = max(max(g.dx, g.dy), g.dz) + max(max(abs(c0x), abs(c0y)), abs(c0z)) + 1;
+    for (int r = 0; r <= maxR; r++) {
+        visit_ring(g, c0x, c0y, c0z, r, qx, qy, qz, bestD, [&](int s, int c) {
+            for (int t = s; t < s + c; t++) {
+                float dd = ppfc::dist2(qx, qy, qz, g.x[t], g.y[t], g.z[t]);
+                int   ii = g.id[t];
+                if (dd < bestD || (dd == bestD && ii < bestId)) {
+                    bestD  = dd;
+                    bestId = ii;
+                    bestS  = t;
+                }
+            }
+        });
+        float safe = ring_safe(g, c0x, c0y, c0z, r, qx, qy, qz);
+        if (safe == FLT_MAX)
+            break;
+        if (bestS >= 0 && bestD < safe * safe)
+            break;
+    }
+    return bestS;
+}
+
+// true if any indexed point has dist2 < r2
+__device__ inline bool grid_any_within(const GridView &g, float qx, float qy, float qz, float r2) {
+    if (g.n == 0)
+        return false;
+    float r   = sqrtf(r2) * 1.0001f + 1e-3f * g.cell;
+    int   xlo = cell_coord(qx - r, g.ox, g.cell, g.dx), xhi = cell_coord(qx + r, g.ox, g.cell, g.dx);
+    int   ylo = cell_coord(qy - r, g.oy, g.cell, g.dy), yhi = cell_coord(qy + r, g.oy, g.cell, g.dy);
+    int   zlo = cell_coord(qz - r, g.oz, g.cell, g.dz), zhi = cell_coord(qz + r, g.oz, g.cell, g.dz);
+    for (int cz = zlo; cz <= zhi; cz++)
+        for (int cy = ylo; cy <= yhi; cy++)
+            for (int cx = xlo; cx <= xhi; cx++) {
+                int s, c;
+                if (!cell_lookup(g, cx, cy, cz, s, c))
+                    continue;
+                for (int t = s; t < s + c; t++)
+                    if (ppfc::dist2(qx, qy, qz, g.x[t], g.y[t], g.z[t]) < r2)
+                        return true;
+            }
+    return false;
+}
+
+// --------------------------------------------------------------------------
+// host side of the grid
+// --------------------------------------------------------------------------
+struct Grid {
+    DevBuf   keys, keys2, vals, vals2, htab, x, y, z, nx, ny, nz, tmp, bbox;
+    GridView view{};
+    int      capN = 0;
+};
+
+// Build a grid over points (X,Y,Z SoA of a cloud) restricted to `subset`
+// (device int list of length m; nullptr = all m points).  cellHint is the
+// requested cell edge; it grows if the grid would exceed 1023 cells per
+// axis.  withNormals gathers NX/NY/NZ too.  Synchronises `s` once (bbox).
+void grid_build(Grid &g, const float *X, const float *Y, const float *Z, const float *NX,
+                const float *NY, const float *NZ, const int *subset, int m, float cellHint,
+                hipStream_t s);
+
+// block-wide exclusive scan of one int per thread (blockDim a multiple of 64, <= 1024)
+template <int BLOCK>
+__device__ inline int block_excl_scan(int v, int *lds /* >= BLOCK/64+1 */, int &total) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    int       x    = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        int y = __shfl_up(x, o, 64);
+        if (lane >= o)
+            x += y;
+    }
+    if (lane == 63)
+        lds[wid] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int acc = 0;
+        for (int w = 0; w < BLOCK / 64; w++) {
+            int t  = lds[w];
+            lds[w] = acc;
+            acc += t;
+        }
+        lds[BLOCK / 64] = acc;
+    }
+    __syncthreads();
+    int res = lds[wid] + x - v;
+    total   = lds[BLOCK / 64];
+    __syncthreads();
+    return res;
+}
+
+}  // namespace ppfd

@@ -1,0 +1,634 @@
+// grid.hip -- hashed uniform grid build, voxel-grid downsampling and
+// kNN-PCA normal estimation for gfx950.
+//
+// Voxel sampling replaces the reference's kd-tree greedy Poisson sampling
+// (util.cpp:107-139); normal estimation follows util.cpp:234-334 with
+// snapshot semantics (all reads see the positions/normals at stage start,
+// all write-backs happen afterwards).
+#include "ppf_dev.h"
+
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cmath>
+
+namespace ppfd {
+
+// ---------------------------------------------------------------- bbox
+
+__device__ __forceinline__ int f2ord(float f) {
+    int i = __float_as_int(f);
+    return i >= 0 ? i : i ^ 0x7fffffff;
+}
+__device__ __forceinline__ float ord2f(int i) { return __int_as_float(i >= 0 ? i : i ^ 0x7fffffff); }
+
+__global__ void bbox_init_kernel(int *bb) {
+    if (threadIdx.x < 3)
+        bb[threadIdx.x] = 0x7fffffff;
+    else if (threadIdx.x < 6)
+        bb[threadIdx.x] = (int)0x80000000;
+}
+
+__global__ void __launch_bounds__(256) bbox_kernel(const float *X, const float *Y, const float *Z,
+                                                   const int *subset, int m, int *bb) {
+    float lo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, hi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < m; t += gridDim.x * blockDim.x) {
+        int   i = subset ? subset[t] : t;
+        float v[3] = {X[i], Y[i], Z[i]};
+        for (int a = 0; a < 3; a++) {
+            lo[a] = fminf(lo[a], v[a]);
+            hi[a] = fmaxf(hi[a], v[a]);
+        }
+    }
+    for (int a = 0; a < 3; a++) {
+        for (int o = 32; o > 0; o >>= 1) {
+            lo[a] = fminf(lo[a], __shfl_xor(lo[a], o, 64));
+            hi[a] = fmaxf(hi[a], __shfl_xor(hi[a], o, 64));
+        }
+    }
+    if ((threadIdx.x & 63) == 0)
+        for (int a = 0; a < 3; a++) {
+            atomicMin(bb + a, f2ord(lo[a]));
+            atomicMax(bb + 3 + a, f2ord(hi[a]));
+        }
+}
+
+__global__ void bbox_fin_kernel(int *bb) {
+    if (threadIdx.x < 6) {
+        float f = ord2f(bb[threadIdx.x]);
+        bb[threadIdx.x] = __float_as_int(f);
+    }
+}
+
+void device_bbox(const float *X, const float *Y, const float *Z, const int *subset, int m,
+                 float *bb6dev, hipStream_t s) {
+    int *bb = reinterpret_cast<int *>(bb6dev);
+    bbox_init_kernel<<<1, 64, 0, s>>>(bb);
+    if (m > 0) {
+        int blocks = std::min(div_up(m, 256), 1024);
+        bbox_kernel<<<blocks, 256, 0, s>>>(X, Y, Z, subset, m, bb);
+    }
+    bbox_fin_kernel<<<1, 64, 0, s>>>(bb);
+}
+
+// ---------------------------------------------------------------- grid
+
+__global__ void grid_keys_kernel(const float *X, const float *Y, const float *Z, const int *subset,
+                                 int m, float ox, float oy, float oz, float cell, int dx, int dy,
+                                 int dz, uint32_t *keys, int *vals) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= m)
+        return;
+    int i  = subset ? subset[t] : t;
+    int cx = cell_coord(X[i], ox, cell, dx);
+    int cy = cell_coord(Y[i], oy, cell, dy);
+    int cz = cell_coord(Z[i], oz, cell, dz);
+    keys[t] = (uint32_t)((cz * dy + cy) * dx + cx);
+    vals[t] = i;
+}
+
+__global__ void grid_insert_kernel(const uint32_t *keys, int m, uint4 *htab, uint32_t hmask) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= m)
+        return;
+    if (s > 0 && keys[s] == keys[s - 1])
+        return;
+    uint32_t key = keys[s] + 1u;
+    uint32_t h   = cell_hash(key) & hmask;
+    for (;;) {
+        uint32_t prev = atomicCAS(&htab[h].x, 0u, key);
+        if (prev == 0u) {
+            htab[h].y = (uint32_t)s;
+            return;
+        }
+        h = (h + 1u) & hmask;
+    }
+}
+
+__global__ void grid_count_kernel(const uint32_t *keys, int m, uint4 *htab, uint32_t hmask) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= m)
+        return;
+    if (s < m - 1 && keys[s] == keys[s + 1])
+        return;
+    uint32_t key = keys[s] + 1u;
+    uint32_t h   = cell_hash(key) & hmask;
+    for (;;) {
+        if (htab[h].x == key) {
+            htab[h].z = (uint32_t)(s + 1) - htab[h].y;
+            return;
+        }
+        h = (h + 1u) & hmask;
+    }
+}
+
+__global__ void gather3_kernel(const int *idx, int m, const float *A, const float *B,
+                               const float *Cc, float *a, float *b, float *c) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= m)
+        return;
+    int i = idx[s];
+    a[s]  = A[i];
+    b[s]  = B[i];
+    c[s]  = Cc[i];
+}
+
+void grid_build(Grid &g, const float *X, const float *Y, const float *Z, const float *NX,
+                const float *NY, const float *NZ, const int *subset, int m, float cellHint,
+                hipStream_t s) {
+    float *bb = g.bbox.as<float>(8);
+    device_bbox(X, Y, Z, subset, m, bb, s);
+    float hb[6];
+    PPF_HIP_CHECK(hipMemcpyAsync(hb, bb, sizeof hb, hipMemcpyDeviceToHost, s));
+    PPF_HIP_CHECK(hipStreamSynchronize(s));
+    if (m == 0)
+        for (int a = 0; a < 6; a++)
+            hb[a] = 0.0f;
+    float ext  = std::max({hb[3] - hb[0], hb[4] - hb[1], hb[5] - hb[2], 0.0f});
+    float cell = cellHint > 0.0f ? cellHint : 1.0f;
+    if (ext / cell > 1000.0f)
+        cell = ext / 1000.0f;
+    // cap the total number of cells at 2^30 so cell keys fit 30 bits
+    int dims[3];
+    for (int a = 0; a < 3; a++)
+        dims[a] = (int)std::floor((hb[3 + a] - hb[a]) / cell) + 1;
+    uint32_t hsize = 64;
+    while (hsize < 2u * (uint32_t)std::max(m, 1))
+        hsize <<= 1;
+
+    uint32_t *keys  = g.keys.as<uint32_t>(std::max(m, 1));
+    uint32_t *keys2 = g.keys2.as<uint32_t>(std::max(m, 1));
+    int      *vals  = g.vals.as<int>(std::max(m, 1));
+    int      *vals2 = g.vals2.as<int>(std::max(m, 1));
+    uint4    *htab  = g.htab.as<uint4>(hsize);
+    float    *x     = g.x.as<float>(std::max(m, 1));
+    float    *y     = g.y.as<float>(std::max(m, 1));
+    float    *z     = g.z.as<float>(std::max(m, 1));
+    PPF_HIP_CHECK(hipMemsetAsync(htab, 0, sizeof(uint4) * hsize, s));
+    if (m > 0) {
+        grid_keys_kernel<<<div_up(m, 256), 256, 0, s>>>(X, Y, Z, subset, m, hb[0], hb[1], hb[2],
+                                                         cell, dims[0], dims[1], dims[2], keys,
+                                                         vals);
+        int endBit = 1;
+        while ((1ull << endBit) < (unsigned long long)dims[0] * dims[1] * dims[2])
+            endBit++;
+        size_t tb = 0;
+        PPF_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, keys, keys2, vals, vals2, m,
+                                                         0, endBit, s));
+        void *tmp = g.tmp.reserve(tb);
+        PPF_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(tmp, tb, keys, keys2, vals, vals2, m, 0,
+                                                         endBit, s));
+        grid_insert_kernel<<<div_up(m, 256), 256, 0, s>>>(keys2, m, htab, hsize - 1);
+        grid_count_kernel<<<div_up(m, 256), 256, 0, s>>>(keys2, m, htab, hsize - 1);
+        gather3_kernel<<<div_up(m, 256), 256, 0, s>>>(vals2, m, X, Y, Z, x, y, z);
+    }
+    GridView &v = g.view;
+    v.ox = hb[0];
+    v.oy = hb[1];
+    v.oz = hb[2];
+    v.cell  = cell;
+    v.dx    = dims[0];
+    v.dy    = dims[1];
+    v.dz    = dims[2];
+    v.hmask = hsize - 1;
+    v.htab  = htab;
+    v.x = x;
+    v.y = y;
+    v.z = z;
+    v.id = vals2;
+    v.n  = m;
+    v.nx = v.ny = v.nz = nullptr;
+    if (NX && m > 0) {
+        float *nx = g.nx.as<float>(m), *ny = g.ny.as<float>(m), *nz = g.nz.as<float>(m);
+        gather3_kernel<<<div_up(m, 256), 256, 0, s>>>(vals2, m, NX, NY, NZ, nx, ny, nz);
+        v.nx = nx;
+        v.ny = ny;
+        v.nz = nz;
+    }
+    PPF_HIP_CHECK(hipGetLastError());
+}
+
+// ---------------------------------------------------------------- voxel sampling
+
+__device__ __forceinline__ unsigned long long voxel_key(float px, float py, float pz,
+                                                        const float *bmin, float step,
+                                                        float &d) {
+    float              p[3] = {px, py, pz}, c[3];
+    unsigned long long key  = 0;
+    for (int a = 0; a < 3; a++) {
+        int vi = (int)floorf((p[a] - bmin[a]) / step);
+        vi     = vi < -(1 << 20) ? -(1 << 20) : vi;
+        vi     = vi > (1 << 20) - 1 ? (1 << 20) - 1 : vi;
+        c[a]   = bmin[a] + ((float)vi + 0.5f) * step;
+        key    = (key << 21) | (unsigned long long)(vi + (1 << 20));
+    }
+    d = ppfc::dist2(px, py, pz, c[0], c[1], c[2]);
+    return key;
+}
+
+__device__ __forceinline__ uint32_t vox_hash(unsigned long long k) {
+    k ^= k >> 33;
+    k *= 0xff51afd7ed558ccdull;
+    k ^= k >> 33;
+    return (uint32_t)k;
+}
+
+struct VoxArgs {
+    const float *X, *Y, *Z;
+    const int   *subset;
+    int          m;
+    float        step;
+    float        bmin[3];
+    unsigned long long *tkey;   // key + 1, 0 = empty
+    unsigned long long *tbest;  // (dist bits << 32) | id
+    uint32_t     hmask;
+};
+
+__global__ void voxel_insert_kernel(VoxArgs a) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.m)
+        return;
+    int                i = a.subset ? a.subset[t] : t;
+    float              d;
+    unsigned long long key = voxel_key(a.X[i], a.Y[i], a.Z[i], a.bmin, a.step, d) + 1ull;
+    unsigned long long pk  = ((unsigned long long)__float_as_uint(d) << 32) | (uint32_t)i;
+    uint32_t           h   = vox_hash(key) & a.hmask;
+    for (;;) {
+        unsigned long long prev = atomicCAS(&a.tkey[h], 0ull, key);
+        if (prev == 0ull || prev == key) {
+            atomicMin(&a.tbest[h], pk);
+            return;
+        }
+        h = (h + 1u) & a.hmask;
+    }
+}
+
+__global__ void voxel_flag_kernel(VoxArgs a, char *flags) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.m)
+        return;
+    int                i = a.subset ? a.subset[t] : t;
+    float              d;
+    unsigned long long key = voxel_key(a.X[i], a.Y[i], a.Z[i], a.bmin, a.step, d) + 1ull;
+    unsigned long long pk  = ((unsigned long long)__float_as_uint(d) << 32) | (uint32_t)i;
+    uint32_t           h   = vox_hash(key) & a.hmask;
+    for (;;) {
+        if (a.tkey[h] == key) {
+            flags[t] = (a.tbest[h] == pk) ? 1 : 0;
+            return;
+        }
+        h = (h + 1u) & a.hmask;
+    }
+}
+
+__global__ void iota_kernel(int *o, int m) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < m)
+        o[t] = t;
+}
+
+int voxel_sample(Workspace &ws, const float *X, const float *Y, const float *Z,
+                 const int *subset, int m, float step, const float bmin[3], int *out,
+                 int *countDev, hipStream_t s) {
+    uint32_t hsize = 64;
+    while (hsize < 2u * (uint32_t)std::max(m, 1))
+        hsize <<= 1;
+    VoxArgs a;
+    a.X = X;
+    a.Y = Y;
+    a.Z = Z;
+    a.subset = subset;
+    a.m      = m;
+    a.step   = step;
+    for (int k = 0; k < 3; k++)
+        a.bmin[k] = bmin[k];
+    a.tkey  = ws.voxKey.as<unsigned long long>(hsize);
+    a.tbest = ws.voxBest.as<unsigned long long>(hsize);
+    a.hmask = hsize - 1;
+    char *flags = ws.voxFlag.as<char>(std::max(m, 1));
+    const int *src = subset;
+    if (!src) {
+        int *io = ws.voxIota.as<int>(std::max(m, 1));
+        if (m > 0)
+            iota_kernel<<<div_up(m, 256), 256, 0, s>>>(io, m);
+        src = io;
+    }
+    PPF_HIP_CHECK(hipMemsetAsync(a.tkey, 0, sizeof(unsigned long long) * hsize, s));
+    PPF_HIP_CHECK(hipMemsetAsync(a.tbest, 0xff, sizeof(unsigned long long) * hsize, s));
+    if (m > 0) {
+        voxel_insert_kernel<<<div_up(m, 256), 256, 0, s>>>(a);
+        voxel_flag_kernel<<<div_up(m, 256), 256, 0, s>>>(a, flags);
+    }
+    size_t tb = 0;
+    PPF_HIP_CHECK(hipcub::DeviceSelect::Flagged(nullptr, tb, src, flags, out, countDev, m, s));
+    void *tmp = ws.cubTmp.reserve(tb);
+    PPF_HIP_CHECK(hipcub::DeviceSelect::Flagged(tmp, tb, src, flags, out, countDev, m, s));
+    int cnt = 0;
+    PPF_HIP_CHECK(hipMemcpyAsync(&cnt, countDev, sizeof(int), hipMemcpyDeviceToHost, s));
+    PPF_HIP_CHECK(hipStreamSynchronize(s));
+    return cnt;
+}
+
+// ---------------------------------------------------------------- normals
+
+constexpr int KNN_MAX = 32;
+
+// PCA of one point's k nearest neighbours (computeNormal, util.cpp:234-270)
+// from the snapshot grid: FP32 centroid + shifted covariance in kNN order,
+// FP64 Jacobi eigen-solve, eigenvector of the smallest |eigenvalue|.
+__device__ int pca_point(const GridView &g, float qx, float qy, float qz, int k, float cen[3],
+                         float nrm[3], int *nbOut) {
+    KnnList<KNN_MAX> L;
+    grid_knn<KNN_MAX>(g, qx, qy, qz, k, L);
+    int cnt = L.cnt;
+    if (cnt < 3)
+        return 0;
+    // neighbours' coordinates are read back from the sorted grid arrays via
+    // their ids; the grid stores positions in sorted order, so look them up
+    // through the original arrays passed in as g.nx.. (positions snapshot)
+    float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+    for (int t = 0; t < KNN_MAX; t++) {
+        if (t >= cnt)
+            break;
+        int id = L.id[t];
+        sx     = sx + g.nx[id];
+        sy     = sy + g.ny[id];
+        sz     = sz + g.nz[id];
+    }
+    float fk = (float)cnt;
+    cen[0] = sx / fk;
+    cen[1] = sy / fk;
+    cen[2] = sz / fk;
+    float c00 = 0, c01 = 0, c02 = 0, c11 = 0, c12 = 0, c22 = 0;
+    for (int t = 0; t < KNN_MAX; t++) {
+        if (t >= cnt)
+            break;
+        int   id = L.id[t];
+        float x  = g.nx[id] - cen[0];
+        float y  = g.ny[id] - cen[1];
+        float z  = g.nz[id] - cen[2];
+        c00 = c00 + x * x;
+        c01 = c01 + x * y;
+        c02 = c02 + x * z;
+        c11 = c11 + y * y;
+        c12 = c12 + y * z;
+        c22 = c22 + z * z;
+    }
+    double A[9] = {c00, c01, c02, c01, c11, c12, c02, c12, c22}, V[9], W[3];
+    ppfc::jacobi_eig<3>(A, V, W);
+    int mi = 0;
+    for (int t = 1; t < 3; t++)
+        if (fabs(W[t]) < fabs(W[mi]))
+            mi = t;
+    double vx = V[0 * 3 + mi], vy = V[1 * 3 + mi], vz = V[2 * 3 + mi];
+    double nn = sqrt(vx * vx + vy * vy + vz * vz);
+    nrm[0]    = (float)(vx / nn);
+    nrm[1]    = (float)(vy / nn);
+    nrm[2]    = (float)(vz / nn);
+    if (nbOut)
+        for (int t = 0; t < KNN_MAX; t++)
+            if (t < cnt)
+                nbOut[t] = L.id[t];
+    return cnt;
+}
+
+__device__ __forceinline__ bool finite3(float a, float b, float c) {
+    return isfinite(a) && isfinite(b) && isfinite(c);
+}
+
+struct NormArgs {
+    GridView     g;        // grid over valid points; g.nx/ny/nz = snapshot positions (full arrays)
+    const float *N0x, *N0y, *N0z;  // snapshot normals (full arrays)
+    const int   *query;
+    int          nq, k, smooth;
+    float       *rawX, *rawY, *rawZ, *cenX, *cenY, *cenZ;  // per point
+    int         *have;     // per point: 1 = fresh PCA
+    int         *nbq;      // nq * k
+    int         *cntq;     // nq
+    int         *touch;    // per point: neighbour needing a PCA
+};
+
+__global__ void __launch_bounds__(128) normals_query_kernel(NormArgs a) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.nq)
+        return;
+    int id = a.query[t];
+    if (finite3(a.N0x[id], a.N0y[id], a.N0z[id])) {
+        a.cntq[t] = 0;
+        return;
+    }
+    float cen[3], nrm[3];
+    int   nb[KNN_MAX];
+    int   cnt = pca_point(a.g, a.g.nx[id], a.g.ny[id], a.g.nz[id], a.k, cen, nrm, nb);
+    a.cntq[t] = cnt;
+    if (!cnt)
+        return;
+    a.rawX[id] = nrm[0];
+    a.rawY[id] = nrm[1];
+    a.rawZ[id] = nrm[2];
+    a.cenX[id] = cen[0];
+    a.cenY[id] = cen[1];
+    a.cenZ[id] = cen[2];
+    a.have[id] = 1;
+    for (int x = 0; x < KNN_MAX; x++) {
+        if (x >= cnt)
+            break;
+        a.nbq[(size_t)t * a.k + x] = nb[x];
+        if (a.smooth) {
+            int j = nb[x];
+            if (!finite3(a.N0x[j], a.N0y[j], a.N0z[j]))
+                a.touch[j] = 1;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(128) normals_touch_kernel(NormArgs a, const int *list, int nl) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nl)
+        return;
+    int j = list[t];
+    if (a.have[j])
+        return;
+    float cen[3], nrm[3];
+    if (pca_point(a.g, a.g.nx[j], a.g.ny[j], a.g.nz[j], a.k, cen, nrm, nullptr)) {
+        a.rawX[j] = nrm[0];
+        a.rawY[j] = nrm[1];
+        a.rawZ[j] = nrm[2];
+        a.cenX[j] = cen[0];
+        a.cenY[j] = cen[1];
+        a.cenZ[j] = cen[2];
+        a.have[j] = 2;
+    }
+}
+
+// final normal of each query (smoothing, util.cpp:292-306)
+__global__ void normals_final_kernel(NormArgs a, float *outN /* nq*3 */) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.nq)
+        return;
+    int id  = a.query[t];
+    int cnt = a.cntq[t];
+    if (!cnt) {
+        outN[3 * t + 0] = a.N0x[id];
+        outN[3 * t + 1] = a.N0y[id];
+        outN[3 * t + 2] = a.N0z[id];
+        return;
+    }
+    float n0x = a.rawX[id], n0y = a.rawY[id], n0z = a.rawZ[id];
+    if (!a.smooth) {
+        outN[3 * t + 0] = n0x;
+        outN[3 * t + 1] = n0y;
+        outN[3 * t + 2] = n0z;
+        return;
+    }
+    float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f;
+    for (int x = 0; x < cnt; x++) {
+        int   j = a.nbq[(size_t)t * a.k + x];
+        float jx, jy, jz;
+        if (a.have[j]) {
+            jx = a.rawX[j];
+            jy = a.rawY[j];
+            jz = a.rawZ[j];
+        } else {
+            jx = a.N0x[j];
+            jy = a.N0y[j];
+            jz = a.N0z[j];
+        }
+        float d = jx * n0x + jy * n0y + jz * n0z;
+        if (d > 0.0f) {
+            s0 = s0 + jx;
+            s1 = s1 + jy;
+            s2 = s2 + jz;
+        } else {
+            s0 = s0 - jx;
+            s1 = s1 - jy;
+            s2 = s2 - jz;
+        }
+    }
+    float l = sqrtf(s0 * s0 + s1 * s1 + s2 * s2);
+    outN[3 * t + 0] = s0 / l;
+    outN[3 * t + 1] = s1 / l;
+    outN[3 * t + 2] = s2 / l;
+}
+
+// write-back of every fresh PCA (point := centroid, normal := raw)
+__global__ void normals_writeback_kernel(NormArgs a, int n, float *PX, float *PY, float *PZ,
+                                         float *NX, float *NY, float *NZ) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || !a.have[i])
+        return;
+    PX[i] = a.cenX[i];
+    PY[i] = a.cenY[i];
+    PZ[i] = a.cenZ[i];
+    NX[i] = a.rawX[i];
+    NY[i] = a.rawY[i];
+    NZ[i] = a.rawZ[i];
+}
+
+// queries take their final normal, then orientation (util.cpp:309-333)
+__global__ void normals_orient_kernel(NormArgs a, const float *outN, float *PX, float *PY,
+                                      float *PZ, float *NX, float *NY, float *NZ, int invert,
+                                      int vpFinite, float vx, float vy, float vz) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.nq)
+        return;
+    int   id = a.query[t];
+    float nx = outN[3 * t + 0], ny = outN[3 * t + 1], nz = outN[3 * t + 2];
+    float d;
+    if (vpFinite) {
+        float dx = vx - PX[id], dy = vy - PY[id], dz = vz - PZ[id];
+        d        = nx * dx + ny * dy + nz * dz;
+    } else {
+        d = nx * 0.0f + ny * 0.0f + nz * 1.0f;
+    }
+    bool flip = invert ? (d > 0.0f) : (d < 0.0f);
+    if (flip) {
+        nx = -nx;
+        ny = -ny;
+        nz = -nz;
+    }
+    NX[id] = nx;
+    NY[id] = ny;
+    NZ[id] = nz;
+}
+
+__global__ void flag_to_list_kernel(const int *touch, int n, char *flags) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        flags[i] = touch[i] ? 1 : 0;
+}
+
+void estimate_normals(Workspace &ws, int n, float *PX, float *PY, float *PZ, float *NX, float *NY,
+                      float *NZ, const int *valid, int nValid, const int *query, int nq, int k,
+                      int smooth, int invert, const float vp[3], float cellHint, hipStream_t s) {
+    if (k < 3)
+        k = 3;
+    if (k > KNN_MAX)
+        throw std::range_error("knnNormal above 32 is not supported");
+    // snapshot copies
+    float *P0x = ws.p0x.as<float>(n), *P0y = ws.p0y.as<float>(n), *P0z = ws.p0z.as<float>(n);
+    float *N0x = ws.n0x.as<float>(n), *N0y = ws.n0y.as<float>(n), *N0z = ws.n0z.as<float>(n);
+    PPF_HIP_CHECK(hipMemcpyAsync(P0x, PX, sizeof(float) * n, hipMemcpyDeviceToDevice, s));
+    PPF_HIP_CHECK(hipMemcpyAsync(P0y, PY, sizeof(float) * n, hipMemcpyDeviceToDevice, s));
+    PPF_HIP_CHECK(hipMemcpyAsync(P0z, PZ, sizeof(float) * n, hipMemcpyDeviceToDevice, s));
+    PPF_HIP_CHECK(hipMemcpyAsync(N0x, NX, sizeof(float) * n, hipMemcpyDeviceToDevice, s));
+    PPF_HIP_CHECK(hipMemcpyAsync(N0y, NY, sizeof(float) * n, hipMemcpyDeviceToDevice, s));
+    PPF_HIP_CHECK(hipMemcpyAsync(N0z, NZ, sizeof(float) * n, hipMemcpyDeviceToDevice, s));
+    grid_build(ws.gNormal, P0x, P0y, P0z, nullptr, nullptr, nullptr, valid, nValid, cellHint, s);
+
+    NormArgs a;
+    a.g    = ws.gNormal.view;
+    a.g.nx = P0x;  // kNN coordinates come from the snapshot, indexed by id
+    a.g.ny = P0y;
+    a.g.nz = P0z;
+    a.N0x  = N0x;
+    a.N0y  = N0y;
+    a.N0z  = N0z;
+    a.query  = query;
+    a.nq     = nq;
+    a.k      = k;
+    a.smooth = smooth;
+    a.rawX = ws.rawX.as<float>(n);
+    a.rawY = ws.rawY.as<float>(n);
+    a.rawZ = ws.rawZ.as<float>(n);
+    a.cenX = ws.cenX.as<float>(n);
+    a.cenY = ws.cenY.as<float>(n);
+    a.cenZ = ws.cenZ.as<float>(n);
+    a.have  = ws.have.as<int>(n);
+    a.touch = ws.touch.as<int>(n);
+    a.nbq   = ws.nbq.as<int>((size_t)std::max(nq, 1) * k);
+    a.cntq  = ws.cntq.as<int>(std::max(nq, 1));
+    float *outN = ws.outN.as<float>((size_t)std::max(nq, 1) * 3);
+    PPF_HIP_CHECK(hipMemsetAsync(a.have, 0, sizeof(int) * n, s));
+    PPF_HIP_CHECK(hipMemsetAsync(a.touch, 0, sizeof(int) * n, s));
+    if (nq > 0)
+        normals_query_kernel<<<div_up(nq, 128), 128, 0, s>>>(a);
+    if (smooth) {
+        char *flags = ws.voxFlag.as<char>(n);
+        int  *io    = ws.voxIota.as<int>(n);
+        int  *list  = ws.touchList.as<int>(n);
+        int  *cntD  = ws.counter.as<int>(4);
+        flag_to_list_kernel<<<div_up(n, 256), 256, 0, s>>>(a.touch, n, flags);
+        iota_kernel<<<div_up(n, 256), 256, 0, s>>>(io, n);
+        size_t tb = 0;
+        PPF_HIP_CHECK(hipcub::DeviceSelect::Flagged(nullptr, tb, io, flags, list, cntD, n, s));
+        void *tmp = ws.cubTmp.reserve(tb);
+        PPF_HIP_CHECK(hipcub::DeviceSelect::Flagged(tmp, tb, io, flags, list, cntD, n, s));
+        int nl = 0;
+        PPF_HIP_CHECK(hipMemcpyAsync(&nl, cntD, sizeof(int), hipMemcpyDeviceToHost, s));
+        PPF_HIP_CHECK(hipStreamSynchronize(s));
+        if (nl > 0)
+            normals_touch_kernel<<<div_up(nl, 128), 128, 0, s>>>(a, list, nl);
+    }
+    if (nq > 0)
+        normals_final_kernel<<<div_up(nq, 256), 256, 0, s>>>(a, outN);
+    normals_writeback_kernel<<<div_up(n, 256), 256, 0, s>>>(a, n, PX, PY, PZ, NX, NY, NZ);
+    int vpFinite = vp && std::isfinite(vp[0]) && std::isfinite(vp[1]) && std::isfinite(vp[2]);
+    if (nq > 0)
+        normals_orient_kernel<<<div_up(nq, 256), 256, 0, s>>>(
+            a, outN, PX, PY, PZ, NX, NY, NZ, invert, vpFinite, vpFinite ? vp[0] : 0.0f,
+            vpFinite ? vp[1] : 0.0f, vpFinite ? vp[2] : 0.0f);
+    PPF_HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace ppfd

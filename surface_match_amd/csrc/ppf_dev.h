@@ -1,0 +1,145 @@
+// ppf_dev.h -- internal interface between the PPF host orchestration and
+// the gfx950 kernels.
+#pragma once
+
+#include "dev_common.h"
+
+namespace ppfd {
+
+// Reusable device scratch of one detector (grown on demand, never shrunk).
+struct Workspace {
+    DevBuf voxKey, voxBest, voxFlag, voxIota, cubTmp, counter;
+    DevBuf p0x, p0y, p0z, n0x, n0y, n0z, rawX, rawY, rawZ, cenX, cenY, cenZ, have, touch,
+        touchList, nbq, cntq, outN;
+    Grid   gNormal, gVote, gSparse, gDense, gFull;
+    // vote
+    DevBuf kpX, kpY, kpZ, kpNX, kpNY, kpNZ, cand, candTile, poses, voteStats, accDump, voteCnt,
+        smallCnt;
+    // clustering
+    DevBuf clSortKey, clSortKey2, clSortVal, clSortVal2, clStatus, clPred, clPredOff,
+        clCenters, clNms, clMembers;
+    // icp
+    DevBuf icpPose, icpCur, icpState, icpHash, icpPartial, icpOut, icpInl;
+};
+
+// ---------------------------------------------------------------- model
+
+// Device form of the trained model: the hash table as CSR buckets reached
+// through an open-addressing key table, plus per-reference local frames.
+struct DevModel {
+    int      nSampled = 0, nReSampled = 0, angleNum = 30, nKeys = 0, nEntries = 0;
+    float    diameter = 0, angleStep = 0, distStep = 0;
+    uint32_t tmask = 0;
+    DevBuf   sx, sy, sz, snx, sny, snz;      // sampled model (SoA)
+    DevBuf   rx, ry, rz, rnx, rny, rnz;      // resampled model (SoA)
+    DevBuf   frames;                         // nSampled x 12 (3x4 row-major)
+    DevBuf   entries;                        // uint2 {ref, alpha bits}, bucket-major
+    DevBuf   slots;                          // uint4 {key, start, len, 0}
+    DevBuf   cosSin;                         // 2 * angleNum floats
+};
+
+// Records of the model pair stage before grouping (train on device).
+struct TrainScratch {
+    DevBuf keys, keys2, idx, idx2, refs, alphas, flags, cnt, tmp, ukeys, ucount, uoffs;
+};
+
+// Build the model table on device from the sampled model (SoA device
+// arrays); returns host copies of keys/offsets/refs/alphas (bucket-major,
+// buckets by ascending key, insertion order (i asc, j asc) inside).
+void train_table(TrainScratch &ts, int n, const float *px, const float *py, const float *pz,
+                 const float *nx, const float *ny, const float *nz, float angleStep,
+                 float distStep, int refTail, std::vector<uint32_t> &keys, std::vector<int> &offs,
+                 std::vector<uint32_t> &refs, std::vector<float> &alphas, hipStream_t s);
+
+// Pair features for testing: key/alpha of (p1,n1) against partners.
+void pair_features(const float *p1n1 /*6 floats host*/, int n, const float *px, const float *py,
+                   const float *pz, const float *nx, const float *ny, const float *nz,
+                   float angleStep, float distStep, int tail, uint32_t *keys, float *alphas,
+                   hipStream_t s);
+
+// Upload buckets + frames of a model already on the host.
+void model_upload(DevModel &m, int nSampled, const float *sp, const float *sn, int nRe,
+                  const float *rp, const float *rn, const std::vector<uint32_t> &keys,
+                  const std::vector<int> &offs, const std::vector<uint32_t> &refs,
+                  const std::vector<float> &alphas, int angleNum, float diameter,
+                  float distStepRel, hipStream_t s);
+
+// ---------------------------------------------------------------- grid/sample/normals
+void device_bbox(const float *X, const float *Y, const float *Z, const int *subset, int m,
+                 float *bb6dev, hipStream_t s);
+int  voxel_sample(Workspace &ws, const float *X, const float *Y, const float *Z,
+                  const int *subset, int m, float step, const float bmin[3], int *out,
+                  int *countDev, hipStream_t s);
+void estimate_normals(Workspace &ws, int n, float *PX, float *PY, float *PZ, float *NX,
+                      float *NY, float *NZ, const int *valid, int nValid, const int *query,
+                      int nq, int k, int smooth, int invert, const float vp[3], float cellHint,
+                      hipStream_t s);
+
+// ---------------------------------------------------------------- vote
+
+// One pose hypothesis from the vote stage (A11/A12).
+struct PoseRec {
+    float pose[12];
+    float votes;
+    float angle;
+    float q[4];
+    int   keypoint;
+    int   rank;
+    int   refId;
+    int   angleId;
+};
+
+struct VoteStats {
+    unsigned long long pairs, hits, votes, candidates;
+};
+
+struct VoteConfig {
+    int block       = 1024;  // threads per workgroup
+    int xcdRemap    = 1;     // keep spatially adjacent keypoints on one XCD
+};
+
+// Vote every keypoint (indices into the sampled-scene grid's clouds),
+// extract peaks and build poses.  Returns the number of poses; the poses
+// are in (keypoint, rank) order in ws.poses.
+int vote_and_poses(Workspace &ws, const DevModel &m, const GridView &g, const float *PX,
+                   const float *PY, const float *PZ, const float *NX, const float *NY,
+                   const float *NZ, const int *keypoints, int nk, float voteThreshold,
+                   const VoteConfig &cfg, VoteStats *stats, int32_t *accDump, int dumpCount,
+                   hipStream_t s);
+int vote_max_tile_refs(int angleNum, int block);
+
+// ---------------------------------------------------------------- clustering
+// Greedy pose clustering + Markley mean + overlap NMS (util.cpp:486-601).
+// Returns the NMS'd hypotheses (votes desc) in host memory.
+struct Hypo {
+    float pose[12];
+    float votes;
+};
+void cluster_poses(Workspace &ws, const PoseRec *posesDev, int np, float distThr, float angleThr,
+                   const float center[3], float nmsThr, std::vector<Hypo> &out, int *nClusters,
+                   hipStream_t s);
+
+// ---------------------------------------------------------------- ICP
+struct IcpCriteria {
+    int   iterations;
+    float rejectDist, mseMin, mseMax, tolerance;
+};
+struct IcpResultDev {
+    float pose[12];
+    int   type;
+    float mse, convergeRate;
+    int   iterations, converged;
+    int   inliers;
+};
+struct IcpConfig {
+    int block = 256;
+    int mode  = 0;   // 0 symmetric (reference), 1 point-to-plane
+};
+// Refine H hypotheses (poses12 host) of src against dst in one persistent
+// launch; scores by inliers against `full` within scoringDist.
+void icp_batch(Workspace &ws, int nSrc, const float *sx, const float *sy, const float *sz,
+               const float *snx, const float *sny, const float *snz, const GridView &dst,
+               const GridView &full, const float *init12, int H, const IcpCriteria &c,
+               float scoringDist, const IcpConfig &cfg, IcpResultDev *out, hipStream_t s);
+
+}  // namespace ppfd

@@ -1,0 +1,474 @@
+// vote.hip -- scene voting, peak extraction and pose construction on gfx950.
+//
+// Reference: ppf.cpp:247-409.  Per keypoint the reference radius-searches
+// the sampled scene (radius = model diameter), computes the PPF key and
+// alpha of every neighbour, looks the key up in the model table and adds
+// one vote per bucket entry to a refNum x (angleNum+1) int accumulator, then
+// extracts up to three peaks.
+//
+// Here one workgroup owns one keypoint (and one tile of model references if
+// the accumulator does not fit in LDS).  The accumulator lives in LDS
+// without the reference's per-ref count slot (it always equals the sum of
+// the ref's bins, so the peak stage recomputes it).  Neighbour candidates
+// come from the hashed grid cells overlapping the vote sphere; hits are
+// compacted per chunk and their bucket entries are voted as one flattened
+// range striped over the workgroup's lanes, so consecutive lanes read
+// consecutive entries of a bucket and bucket-size skew does not idle lanes.
+#include "ppf_dev.h"
+
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+
+namespace ppfd {
+
+constexpr int VOTE_MAX_CELLS = 512;
+constexpr int VOTE_STATIC_LDS = 24 * 1024;   // bound on the kernel's static LDS
+constexpr int LDS_LIMIT       = 160 * 1024;  // gfx950 LDS per workgroup
+
+int vote_max_tile_refs(int angleNum, int block) {
+    (void)block;
+    return (LDS_LIMIT - VOTE_STATIC_LDS) / (angleNum * 4);
+}
+
+struct VoteArgs {
+    GridView     g;  // sampled scene, sorted, with normals
+    const float *PX, *PY, *PZ, *NX, *NY, *NZ;
+    const int   *kp;
+    int          nk;
+    const uint4 *slots;
+    uint32_t     tmask;
+    const uint2 *entries;
+    int          refNum, angleNum, tileRefs, nTiles;
+    float        angleStep, distStep, sMaxId, D2, searchR, voteThreshold;
+    unsigned long long *cand;   // [nk * nTiles * 3]
+    unsigned long long *stats;  // pairs, hits, votes (optional)
+    int32_t     *accDump;       // [dumpCount * refNum * angleNum] (optional)
+    int          dumpCount;
+    int          xcdRemap;
+};
+
+__device__ __forceinline__ bool table_find(const uint4 *slots, uint32_t tmask, uint32_t key,
+                                           uint32_t &start, uint32_t &len) {
+    uint32_t h = key & tmask;
+    for (;;) {
+        uint4 s = slots[h];
+        if (s.z == 0u)
+            return false;
+        if (s.x == key) {
+            start = s.y;
+            len   = s.z;
+            return true;
+        }
+        h = (h + 1u) & tmask;
+    }
+}
+
+// candidate code: (vote << 32) | ((0xFFFFF - ref) << 11) | bin -- larger is
+// better under the reference's (vote desc, ref asc) order.
+__device__ __forceinline__ void top3_insert(unsigned long long c, unsigned long long &a,
+                                            unsigned long long &b, unsigned long long &d) {
+    if (c > a) {
+        d = b;
+        b = a;
+        a = c;
+    } else if (c > b) {
+        d = b;
+        b = c;
+    } else if (c > d) {
+        d = c;
+    }
+}
+
+template <int BLOCK>
+__global__ void __launch_bounds__(BLOCK) vote_kernel(VoteArgs a) {
+    extern __shared__ uint32_t acc[];
+    __shared__ int   sCellStart[VOTE_MAX_CELLS];
+    __shared__ int   sCellPre[VOTE_MAX_CELLS + 1];
+    __shared__ int   sPPre[BLOCK + 1];
+    __shared__ uint32_t sPStart[BLOCK];
+    __shared__ int   sPLen[BLOCK];
+    __shared__ float sPAlpha[BLOCK];
+    __shared__ int   sScan[BLOCK / 64 + 1];
+    __shared__ unsigned long long sTop[(BLOCK / 64) * 3];
+    __shared__ int   sRed[BLOCK / 64];
+
+    const int tid = threadIdx.x;
+    // block -> (keypoint, tile); optionally keep contiguous keypoint ranges on
+    // one XCD (consecutive workgroups are dealt round-robin to the 8 XCDs).
+    // Workgroups are dealt round-robin to the 8 XCDs (each with its own L2).
+    // With xcdRemap, XCD x works through a contiguous run of key points, so
+    // neighbouring key points -- which read the same scene cells and similar
+    // buckets -- share an L2.  The grid is padded to 8 * per * nTiles blocks.
+    int b    = blockIdx.x;
+    int tile = b % a.nTiles;
+    int kpBlock;
+    if (a.xcdRemap) {
+        int kb  = b / a.nTiles;
+        int per = (a.nk + 7) / 8;
+        kpBlock = (kb % 8) * per + kb / 8;
+    } else {
+        kpBlock = b / a.nTiles;
+    }
+    if (kpBlock >= a.nk)
+        return;
+    const int k      = kpBlock;
+    const int r0     = tile * a.tileRefs;
+    const int r1     = min(r0 + a.tileRefs, a.refNum);
+    const int nrTile = r1 - r0;
+    unsigned long long *candOut = a.cand + ((size_t)k * a.nTiles + tile) * 3;
+
+    const int   kpId = a.kp[k];
+    const float qx = a.PX[kpId], qy = a.PY[kpId], qz = a.PZ[kpId];
+    const float nx1 = a.NX[kpId], ny1 = a.NY[kpId], nz1 = a.NZ[kpId];
+    if (ppfc::is_nan(nx1) || ppfc::is_nan(ny1) || ppfc::is_nan(nz1)) {
+        if (tid < 3)
+            candOut[tid] = 0ull;
+        return;
+    }
+    float rt[12];
+    ppfc::frame(qx, qy, qz, nx1, ny1, nz1, rt);
+
+    for (int i = tid; i < nrTile * a.angleNum; i += BLOCK)
+        acc[i] = 0u;
+
+    // ---- cells overlapping the vote sphere
+    const GridView &g = a.g;
+    int xlo = cell_coord(qx - a.searchR, g.ox, g.cell, g.dx), xhi = cell_coord(qx + a.searchR, g.ox, g.cell, g.dx);
+    int ylo = cell_coord(qy - a.searchR, g.oy, g.cell, g.dy), yhi = cell_coord(qy + a.searchR, g.oy, g.cell, g.dy);
+    int zlo = cell_coord(qz - a.searchR, g.oz, g.cell, g.dz), zhi = cell_coord(qz + a.searchR, g.oz, g.cell, g.dz);
+    int ncx = xhi - xlo + 1, ncy = yhi - ylo + 1, ncz = zhi - zlo + 1;
+    int nCells = ncx * ncy * ncz;  // <= VOTE_MAX_CELLS by construction of the grid
+    int myCnt = 0, myStart = 0;
+    if (tid < nCells) {
+        int cx = xlo + tid % ncx, cy = ylo + (tid / ncx) % ncy, cz = zlo + tid / (ncx * ncy);
+        if (cell_min_d2(g, cx, cy, cz, qx, qy, qz) <= a.D2) {
+            int s0, c0;
+            if (cell_lookup(g, cx, cy, cz, s0, c0)) {
+                myStart = s0;
+                myCnt   = c0;
+            }
+        }
+    }
+    int total;
+    int pre = block_excl_scan<BLOCK>(myCnt, sScan, total);
+    if (tid < nCells) {
+        sCellStart[tid] = myStart;
+        sCellPre[tid]   = pre;
+    }
+    if (tid == 0)
+        sCellPre[nCells] = total;
+    __syncthreads();
+
+    auto slot_of = [&](int c) -> int {
+        int lo = 0, hi = nCells - 1;  // largest cell with sCellPre[cell] <= c
+        while (lo < hi) {
+            int mid = (lo + hi + 1) >> 1;
+            if (sCellPre[mid] <= c)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        return sCellStart[lo] + (c - sCellPre[lo]);
+    };
+
+    // ---- neighbour count (radiusSearch incl. self, ppf.cpp:273-276)
+    int cnt = 0;
+    for (int c = tid; c < total; c += BLOCK) {
+        int s = slot_of(c);
+        if (ppfc::dist2(qx, qy, qz, g.x[s], g.y[s], g.z[s]) < a.D2)
+            cnt++;
+    }
+    for (int o = 32; o > 0; o >>= 1)
+        cnt += __shfl_xor(cnt, o, 64);
+    if ((tid & 63) == 0)
+        sRed[tid >> 6] = cnt;
+    __syncthreads();
+    int searched = 0;
+    for (int w = 0; w < BLOCK / 64; w++)
+        searched += sRed[w];
+    if ((float)searched < a.voteThreshold) {
+        if (tid < 3)
+            candOut[tid] = 0ull;
+        return;
+    }
+
+    // ---- pairs + votes, one chunk of BLOCK candidates at a time
+    unsigned long long nPairs = 0, nHits = 0, nVotes = 0;
+    for (int base = 0; base < total; base += BLOCK) {
+        int      c   = base + tid;
+        int      hit = 0;
+        uint32_t bStart = 0, bLen = 0;
+        float    alphaS = 0.0f;
+        if (c < total) {
+            int   s  = slot_of(c);
+            float px = g.x[s], py = g.y[s], pz = g.z[s];
+            if (ppfc::dist2(qx, qy, qz, px, py, pz) < a.D2 && g.id[s] != kpId) {
+                nPairs++;
+                alphaS = ppfc::alpha_of(rt, px, py, pz);
+                uint32_t key = ppfc::ppf_key(qx, qy, qz, nx1, ny1, nz1, px, py, pz, g.nx[s],
+                                             g.ny[s], g.nz[s], a.angleStep, a.distStep, false);
+                if (table_find(a.slots, a.tmask, key, bStart, bLen) && !ppfc::is_nan(alphaS))
+                    hit = 1;
+            }
+        }
+        int nHit, totalE;
+        int pos  = block_excl_scan<BLOCK>(hit, sScan, nHit);
+        int lpre = block_excl_scan<BLOCK>(hit ? (int)bLen : 0, sScan, totalE);
+        if (hit) {
+            sPStart[pos] = bStart;
+            sPPre[pos]   = lpre;
+            sPAlpha[pos] = alphaS;
+            sPLen[pos]   = (int)bLen;
+        }
+        if (tid == 0)
+            sPPre[nHit] = totalE;
+        __syncthreads();
+        nHits += hit;
+        for (int e = tid; e < totalE; e += BLOCK) {
+            int lo = 0, hi = nHit - 1;
+            while (lo < hi) {
+                int mid = (lo + hi + 1) >> 1;
+                if (sPPre[mid] <= e)
+                    lo = mid;
+                else
+                    hi = mid - 1;
+            }
+            int  off  = e - sPPre[lo];
+            int  L    = sPLen[lo];
+            bool tail = off >= L - (L & 7);
+            uint2 ent = a.entries[sPStart[lo] + off];
+            int   ref = (int)ent.x;
+            if (ref >= r0 && ref < r1) {
+                int bin = ppfc::angle_bin(__uint_as_float(ent.y), sPAlpha[lo], a.sMaxId,
+                                          a.angleNum, tail);
+                atomicAdd(&acc[(ref - r0) * a.angleNum + bin], 1u);
+                nVotes++;
+            }
+        }
+        __syncthreads();
+    }
+
+    if (k < a.dumpCount && a.accDump) {
+        for (int i = tid; i < nrTile * a.angleNum; i += BLOCK)
+            a.accDump[((size_t)k * a.refNum + r0) * a.angleNum + i] = (int32_t)acc[i];
+    }
+    if (a.stats && tile == 0) {
+        for (int o = 32; o > 0; o >>= 1) {
+            nPairs += __shfl_xor(nPairs, o, 64);
+            nHits += __shfl_xor(nHits, o, 64);
+        }
+        if ((tid & 63) == 0) {
+            atomicAdd(a.stats + 0, nPairs);
+            atomicAdd(a.stats + 1, nHits);
+        }
+    }
+    if (a.stats) {
+        for (int o = 32; o > 0; o >>= 1)
+            nVotes += __shfl_xor(nVotes, o, 64);
+        if ((tid & 63) == 0)
+            atomicAdd(a.stats + 2, nVotes);
+    }
+
+    // ---- peaks (ppf.cpp:363-391)
+    const float thre = a.voteThreshold / 2.0f;
+    const int   A    = a.angleNum, maxI = A - 1;
+    unsigned long long t0 = 0, t1 = 0, t2 = 0;
+    for (int r = tid; r < nrTile; r += BLOCK) {
+        const uint32_t *bins = acc + r * A;
+        int sum = 0, j = 0, best = -1;
+        for (int x = 0; x < A; x++) {
+            int v = (int)bins[x];
+            sum += v;
+            if (v > best) {
+                best = v;
+                j    = x;
+            }
+        }
+        if ((float)sum < thre || (float)best < thre)
+            continue;
+        int v = best + (int)((j == 0) ? bins[maxI] : bins[j - 1]) +
+                (int)((j == maxI) ? bins[0] : bins[j + 1]);
+        unsigned long long code = ((unsigned long long)(uint32_t)v << 32) |
+                                  ((unsigned long long)(0xFFFFFu - (uint32_t)(r0 + r)) << 11) |
+                                  (uint32_t)j;
+        top3_insert(code, t0, t1, t2);
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        unsigned long long u0 = __shfl_xor(t0, o, 64), u1 = __shfl_xor(t1, o, 64),
+                           u2 = __shfl_xor(t2, o, 64);
+        top3_insert(u0, t0, t1, t2);
+        top3_insert(u1, t0, t1, t2);
+        top3_insert(u2, t0, t1, t2);
+    }
+    // after the xor-butterfly every lane holds the wave's top-3 (duplicates
+    // cannot arise: each lane inserts distinct codes from disjoint refs, and
+    // the butterfly merges disjoint halves at each level)
+    if ((tid & 63) == 0) {
+        sTop[(tid >> 6) * 3 + 0] = t0;
+        sTop[(tid >> 6) * 3 + 1] = t1;
+        sTop[(tid >> 6) * 3 + 2] = t2;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        unsigned long long b0 = 0, b1 = 0, b2 = 0;
+        for (int w = 0; w < (BLOCK / 64) * 3; w++)
+            top3_insert(sTop[w], b0, b1, b2);
+        candOut[0] = b0;
+        candOut[1] = b1;
+        candOut[2] = b2;
+    }
+}
+
+// Merge tiles, apply the 0.95 filter (ppf.cpp:393-397) and build the poses
+// T = inv(Ts) * Rx(alpha) * Tm (ppf.cpp:399-405).  One thread per keypoint.
+__global__ void poses_kernel(const unsigned long long *cand, int nk, int nTiles, const int *kp,
+                             const float *PX, const float *PY, const float *PZ, const float *NX,
+                             const float *NY, const float *NZ, const float *frames,
+                             const float *cosSin, int angleNum, PoseRec *slots, int *counts) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nk)
+        return;
+    unsigned long long b0 = 0, b1 = 0, b2 = 0;
+    for (int t = 0; t < nTiles * 3; t++)
+        top3_insert(cand[(size_t)k * nTiles * 3 + t], b0, b1, b2);
+    unsigned long long bs[3] = {b0, b1, b2};
+    int n = 0;
+    if (b0) {
+        float bestVote = (float)(uint32_t)(b0 >> 32);
+        float thr      = (float)((double)bestVote * 0.95);
+        int   id       = kp[k];
+        float Ts[12];
+        ppfc::frame(PX[id], PY[id], PZ[id], NX[id], NY[id], NZ[id], Ts);
+        for (int r = 0; r < 3; r++) {
+            unsigned long long c = bs[r];
+            if (!c)
+                break;
+            float vote = (float)(uint32_t)(c >> 32);
+            if (vote < thr)
+                continue;
+            int ref = (int)(0xFFFFFu - (uint32_t)((c >> 11) & 0x1FFFFFu));
+            int bin = (int)(c & 0x7FFu);
+            PoseRec &P = slots[(size_t)k * 3 + n];
+            float Tm[12];
+            for (int x = 0; x < 12; x++)
+                Tm[x] = frames[(size_t)ref * 12 + x];
+            ppfc::compose_pose(Ts, Tm, cosSin[2 * bin], cosSin[2 * bin + 1], P.pose);
+            float R[9] = {P.pose[0], P.pose[1], P.pose[2], P.pose[4], P.pose[5],
+                          P.pose[6], P.pose[8], P.pose[9], P.pose[10]};
+            ppfc::quat_from_rot(R, P.q);
+            P.angle    = ppfc::angle_of_quat(P.q);
+            P.votes    = vote;
+            P.keypoint = k;
+            P.rank     = n;
+            P.refId    = ref;
+            P.angleId  = bin;
+            n++;
+        }
+    }
+    counts[k] = n;
+}
+
+__global__ void compact_poses_kernel(const PoseRec *slots, const int *counts, const int *offs,
+                                     int nk, PoseRec *out) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nk)
+        return;
+    for (int r = 0; r < counts[k]; r++)
+        out[offs[k] + r] = slots[(size_t)k * 3 + r];
+}
+
+template <int BLOCK>
+static void launch_vote(const VoteArgs &a, int blocks, size_t lds, hipStream_t s) {
+    PPF_HIP_CHECK(hipFuncSetAttribute((const void *)vote_kernel<BLOCK>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    vote_kernel<BLOCK><<<blocks, BLOCK, lds, s>>>(a);
+}
+
+int vote_and_poses(Workspace &ws, const DevModel &m, const GridView &g, const float *PX,
+                   const float *PY, const float *PZ, const float *NX, const float *NY,
+                   const float *NZ, const int *keypoints, int nk, float voteThreshold,
+                   const VoteConfig &cfg, VoteStats *stats, int32_t *accDump, int dumpCount,
+                   hipStream_t s) {
+    if (nk <= 0 || m.nSampled <= 0)
+        return 0;
+    int tileRefs = std::min(m.nSampled, vote_max_tile_refs(m.angleNum, cfg.block));
+    int nTiles   = div_up(m.nSampled, tileRefs);
+    tileRefs     = div_up(m.nSampled, nTiles);
+    float cell   = g.cell;
+    float D      = m.diameter;
+    if ((int)((2.0f * D) / cell + 3) > 8)
+        throw std::runtime_error("vote grid cell is smaller than half the model diameter");
+    VoteArgs a;
+    a.g  = g;
+    a.PX = PX;
+    a.PY = PY;
+    a.PZ = PZ;
+    a.NX = NX;
+    a.NY = NY;
+    a.NZ = NZ;
+    a.kp = keypoints;
+    a.nk = nk;
+    a.slots    = m.slots.get<uint4>();
+    a.tmask    = m.tmask;
+    a.entries  = m.entries.get<uint2>();
+    a.refNum   = m.nSampled;
+    a.angleNum = m.angleNum;
+    a.tileRefs = tileRefs;
+    a.nTiles   = nTiles;
+    a.angleStep = m.angleStep;
+    a.distStep  = m.distStep;
+    a.sMaxId    = (float)(m.angleNum - 1) / ppfc::kTwoPi;
+    a.D2        = D * D;            // ppf.cpp:191
+    a.searchR   = D * 1.0001f + 1e-3f * cell;
+    a.voteThreshold = voteThreshold;
+    a.cand  = ws.cand.as<unsigned long long>((size_t)nk * nTiles * 3);
+    a.stats = nullptr;
+    if (stats) {
+        a.stats = ws.voteStats.as<unsigned long long>(4);
+        PPF_HIP_CHECK(hipMemsetAsync(a.stats, 0, 32, s));
+    }
+    a.accDump   = accDump;
+    a.dumpCount = accDump ? dumpCount : 0;
+    a.xcdRemap  = cfg.xcdRemap;
+    size_t lds  = (size_t)tileRefs * m.angleNum * 4;
+    int blocks  = (cfg.xcdRemap ? 8 * ((nk + 7) / 8) : nk) * nTiles;
+    switch (cfg.block) {
+        case 256: launch_vote<256>(a, blocks, lds, s); break;
+        case 512: launch_vote<512>(a, blocks, lds, s); break;
+        default: launch_vote<1024>(a, blocks, lds, s); break;
+    }
+    PPF_HIP_CHECK(hipGetLastError());
+
+    PoseRec *pslots = reinterpret_cast<PoseRec *>(ws.candTile.as<PoseRec>((size_t)nk * 3));
+    int     *counts = ws.voteCnt.as<int>((size_t)nk * 2 + 2);
+    int     *offs   = counts + nk + 1;
+    poses_kernel<<<div_up(nk, 128), 128, 0, s>>>(a.cand, nk, nTiles, keypoints, PX, PY, PZ, NX,
+                                                  NY, NZ, m.frames.get<float>(),
+                                                  m.cosSin.get<float>(), m.angleNum, pslots,
+                                                  counts);
+    size_t tb = 0;
+    PPF_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, counts, offs, nk + 1, s));
+    // counts[nk] must be 0 so offs[nk] is the total
+    PPF_HIP_CHECK(hipMemsetAsync(counts + nk, 0, sizeof(int), s));
+    PPF_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(ws.cubTmp.reserve(tb), tb, counts, offs,
+                                                   nk + 1, s));
+    PoseRec *out = ws.poses.as<PoseRec>((size_t)nk * 3);
+    compact_poses_kernel<<<div_up(nk, 128), 128, 0, s>>>(pslots, counts, offs, nk, out);
+    int np = 0;
+    PPF_HIP_CHECK(hipMemcpyAsync(&np, offs + nk, sizeof(int), hipMemcpyDeviceToHost, s));
+    if (stats) {
+        unsigned long long st[4];
+        PPF_HIP_CHECK(hipMemcpyAsync(st, a.stats, 32, hipMemcpyDeviceToHost, s));
+        PPF_HIP_CHECK(hipStreamSynchronize(s));
+        stats->pairs = st[0];
+        stats->hits  = st[1];
+        stats->votes = st[2];
+    }
+    PPF_HIP_CHECK(hipStreamSynchronize(s));
+    if (stats)
+        stats->candidates = (unsigned long long)np;
+    return np;
+}
+
+}  // namespace ppfd

@@ -1,0 +1,276 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU oracle.
+
+Integer/byte/index results (hash keys, tables, neighbour sets, samples, vote
+accumulators, peaks) must be bit-exact; floating-point results are checked
+against the tolerances stated in each test.
+"""
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+
+from conftest import rng
+
+pytestmark = pytest.mark.gpu
+
+TWO_PI_F = np.float32(2.0) * np.float32(math.pi)
+
+
+@pytest.fixture(scope="module")
+def N():
+    from surface_match_amd import _native
+    _native.lib()
+    return _native
+
+
+def dev_math(N, op, a, b=None):
+    a = np.ascontiguousarray(a, np.float32)
+    b = None if b is None else np.ascontiguousarray(b, np.float32)
+    out = np.zeros(len(a), np.uint32)
+    N.check(N.lib().ppf_device_math(op, N.ptr(a), N.ptr(b), len(a), N.ptr(out)))
+    return out
+
+
+def test_l0_scalar_math_bitwise(N, oracle_mod):
+    O = oracle_mod
+    r = rng(1)
+    a = np.concatenate([r.normal(size=4000) * 3, r.uniform(-1e4, 1e4, 1000),
+                        [0.0, -0.0, 1.0, -1.0, 2.414213562373095, 0.41421356, 1e-30, -1e-30,
+                         np.inf, -np.inf]]).astype(np.float32)
+    b = np.concatenate([r.normal(size=4000) * 3, r.uniform(-1e4, 1e4, 1000),
+                        [-1.0, -1.0, 0.0, -0.0, 1.0, 1.0, -1.0, -1.0, 1.0, 1.0]]).astype(np.float32)
+    got = dev_math(N, 0, a)
+    exp = np.array([O.lib().orc_atanf(float(x)) for x in a], np.float32).view(np.uint32)
+    assert np.array_equal(got, exp)
+    got = dev_math(N, 1, a, b)
+    exp = np.array([O.lib().orc_atan2f(float(x), float(y)) for x, y in zip(a, b)],
+                   np.float32).view(np.uint32)
+    assert np.array_equal(got, exp)
+    # IEEE division and square root are correctly rounded on gfx950
+    assert np.array_equal(dev_math(N, 2, a, b), (a / b).view(np.uint32))
+    aa = np.abs(a)
+    assert np.array_equal(dev_math(N, 3, aa), np.sqrt(aa).view(np.uint32))
+    # quantisation
+    f = r.uniform(0, math.pi, 3000).astype(np.float32)
+    st = np.full(3000, TWO_PI_F / np.float32(30), np.float32)
+    assert np.array_equal(dev_math(N, 5, f, st), np.ceil(f / st).astype(np.int32).view(np.uint32))
+    assert np.array_equal(dev_math(N, 6, f, st), (f / st).astype(np.int32).view(np.uint32))
+    # murmur of (a bits, b bits, 0, 0)
+    got = dev_math(N, 4, a, b)
+    exp = np.array([O.lib().orc_hash_key4(np.array([x, y, 0, 0], np.uint32))
+                    for x, y in zip(a.view(np.uint32), b.view(np.uint32))], np.uint32)
+    assert np.array_equal(got, exp)
+
+
+def _random_pairs(seed, n):
+    r = rng(seed)
+    p = r.uniform(-100, 100, (n, 3)).astype(np.float32)
+    nn = r.normal(size=(n, 3))
+    nn = (nn / np.linalg.norm(nn, axis=1, keepdims=True)).astype(np.float32)
+    return p, nn
+
+
+def test_l1_pair_features_bitwise(N, oracle_mod):
+    O = oracle_mod
+    p, n = _random_pairs(2, 3000)
+    # adversarial partners: coincident point, parallel / antiparallel normals,
+    # normals along +-x, coplanar pairs
+    q1 = np.array([1.0, 2.0, 3.0], np.float32)
+    m1 = np.array([0.0, 0.0, 1.0], np.float32)
+    extra_p = np.array([q1, q1 + [5, 0, 0], q1 + [0, 5, 0], q1 + [0, 0, 7], q1 + [3, 4, 0]],
+                       np.float32)
+    extra_n = np.array([m1, m1, -m1, [1, 0, 0], [-1, 0, 0]], np.float32)
+    pts = np.concatenate([p, extra_p]).astype(np.float32)
+    nrm = np.concatenate([n, extra_n]).astype(np.float32)
+    ang = TWO_PI_F / np.float32(30)
+    dist = np.float32(5.89)
+    for p1n1 in (np.concatenate([q1, m1]), np.array([0, 0, 0, 1, 0, 0], np.float32),
+                 np.array([0, 0, 0, -1, 0, 0], np.float32), np.concatenate([p[0], n[0]])):
+        p1n1 = p1n1.astype(np.float32)
+        for tail in (0, 1):
+            keys = np.zeros(len(pts), np.uint32)
+            al = np.zeros(len(pts), np.float32)
+            N.check(N.lib().ppf_pair_features(N.ptr(p1n1), N.ptr(pts), N.ptr(nrm), len(pts), ang,
+                                              dist, tail, N.ptr(keys), N.ptr(al)))
+            rt = O.frame(p1n1[:3], p1n1[3:])
+            ek = np.array([O.ppf_key(p1n1[:3], p1n1[3:], pts[j], nrm[j], ang, dist, tail)[0]
+                           for j in range(len(pts))], np.uint32)
+            ea = np.array([O.alpha(rt, pts[j]) for j in range(len(pts))], np.float32)
+            assert np.array_equal(keys, ek)
+            assert np.array_equal(al.view(np.uint32), ea.view(np.uint32))
+
+
+@pytest.fixture(scope="module")
+def trained(N, workload1):
+    from surface_match_amd import Detector, PointCloud
+    mp, mn = workload1["model"]
+    det = Detector()
+    det.trainModel(PointCloud(mp, mn), 0.04)
+    return det
+
+
+def test_l3_table_bitwise(trained, oracle_mod, workload1):
+    O = oracle_mod
+    sp, sn, rp, rn = trained.sampled_model()
+    info = trained.model_info()
+    keys, offs, refs, al = trained.table()
+    ot = O.OracleTable(sp, sn, TWO_PI_F / np.float32(30), np.float32(info.diameter) * np.float32(0.04))
+    assert np.array_equal(keys, ot.keys)
+    assert np.array_equal(offs, ot.offs)
+    assert np.array_equal(refs, ot.refs)
+    assert np.array_equal(al.view(np.uint32), ot.alphas.view(np.uint32))
+    # the sampled model is the oracle's training sample
+    mp, mn = workload1["model"]
+    om = O.train(mp, mn, 0.04)
+    osp, osn = om.sampled()
+    assert np.array_equal(osp, sp) and np.array_equal(osn, sn)
+    orp, orn = om.resampled()
+    assert np.array_equal(orp, rp) and np.array_equal(orn, rn)
+
+
+def _vote_scene(workload1, oracle_mod, D):
+    """A sampled scene with normals, as the vote stage sees it."""
+    O = oracle_mod
+    sc = workload1["scene"].points
+    lo = sc.min(0)
+    samp = O.voxel_sample(sc, np.float32(D) * np.float32(0.04), lo)
+    n = np.full_like(sc, np.nan)
+    valid = np.arange(len(sc), dtype=np.int32)
+    pts, nrm = O.estimate_normals(sc, n, valid, samp, 10, 1, 0, (620.0, 100.0, 500.0))
+    return pts[samp].copy(), nrm[samp].copy()
+
+
+def test_l4_vote_accumulators_bitwise(trained, oracle_mod, workload1):
+    O = oracle_mod
+    info = trained.model_info()
+    D = np.float32(info.diameter)
+    pts, nrm = _vote_scene(workload1, O, D)
+    sp, sn, _, _ = trained.sampled_model()
+    ang = TWO_PI_F / np.float32(30)
+    dist = D * np.float32(0.04)
+    ot = O.OracleTable(sp, sn, ang, dist)
+    r = rng(5)
+    kps = np.sort(r.choice(len(pts), 48, replace=False)).astype(np.int32)
+    cand, poses, acc = trained.vote(pts, nrm, kps, minScore=0.2, dump=len(kps))
+    refNum = info.nSampled
+    thr = np.float32(refNum) * np.float32(0.2)
+    D2 = D * D
+    for t, k in enumerate(kps):
+        d = pts - pts[k]
+        d2 = ((np.float32(0) + d[:, 0] * d[:, 0]) + d[:, 1] * d[:, 1]) + d[:, 2] * d[:, 2]
+        nb = np.nonzero(d2 < D2)[0]
+        if np.float32(len(nb)) < thr:
+            assert (cand[t] == -1).all()
+            continue
+        nb = nb[nb != k]
+        oacc = O.vote_keypoint(ot, refNum, 30, pts[k], nrm[k], pts[nb], nrm[nb], ang, dist)
+        assert np.array_equal(acc[t], oacc[:, 1:]), f"keypoint {k}"
+        assert np.array_equal(acc[t].sum(1), oacc[:, 0])
+        pk = O.peaks(oacc, refNum, 30, thr)
+        got = [tuple(c) for c in cand[t] if c[0] >= 0]
+        assert got == [(a, b, int(v)) for a, b, v in pk], f"keypoint {k}"
+
+
+def test_l2_voxel_and_search(N, oracle_mod, workload1):
+    O = oracle_mod
+    sc = workload1["scene"].points
+    lo = sc.min(0).astype(np.float32)
+    for step in (5.89, 1.47, 18.6):
+        out = np.zeros(len(sc), np.int32)
+        nout = C.c_size_t(0)
+        N.check(N.lib().ppf_voxel_sample(N.ptr(sc), len(sc), None, 0, step, N.ptr(lo), N.ptr(out),
+                                         C.byref(nout)))
+        assert np.array_equal(out[:nout.value], O.voxel_sample(sc, step, lo))
+    r = rng(7)
+    q = sc[r.choice(len(sc), 300, replace=False)] + r.normal(scale=0.5, size=(300, 3)).astype(np.float32)
+    q = q.astype(np.float32)
+    r2 = np.float32(6.0) ** 2
+    counts = np.zeros(len(q), np.int32)
+    ids = np.zeros(len(q) * 400, np.int32)
+    N.check(N.lib().ppf_radius_search(N.ptr(sc), len(sc), N.ptr(q), len(q), r2, 3.0,
+                                      N.ptr(counts), N.ptr(ids), len(ids)))
+    g = O.OracleGrid(sc, 4.0)
+    off = 0
+    for t in range(len(q)):
+        exp = g.radius(q[t], r2)
+        assert counts[t] == len(exp)
+        assert np.array_equal(ids[off:off + counts[t]], exp)
+        off += counts[t]
+    k = 10
+    kid = np.zeros(len(q) * k, np.int32)
+    kd = np.zeros(len(q) * k, np.float32)
+    N.check(N.lib().ppf_knn_search(N.ptr(sc), len(sc), N.ptr(q), len(q), k, 2.0, N.ptr(kid),
+                                   N.ptr(kd)))
+    for t in range(len(q)):
+        eid, ed = g.knn(q[t], k)
+        assert np.array_equal(kid[t * k:(t + 1) * k], eid)
+        assert np.array_equal(kd[t * k:(t + 1) * k].view(np.uint32), ed.view(np.uint32))
+
+
+def test_l5_normals(N, oracle_mod, workload1):
+    O = oracle_mod
+    sc = workload1["scene"].points
+    lo = sc.min(0)
+    samp = O.voxel_sample(sc, 5.89, lo)
+    nrm0 = np.full_like(sc, np.nan)
+    valid = np.arange(len(sc), dtype=np.int32)
+    vp = np.array([620, 100, 500], np.float32)
+    for smooth in (1, 0):
+        ep, en = O.estimate_normals(sc, nrm0, valid, samp, 10, smooth, 0, vp)
+        gp, gn = sc.copy(), nrm0.copy()
+        N.check(N.lib().ppf_estimate_normals(N.ptr(gp), N.ptr(gn), len(sc), N.ptr(valid),
+                                             len(valid), N.ptr(samp), len(samp), 10, smooth, 0,
+                                             N.ptr(vp)))
+        # positions: bit-exact; normals: bit-exact expected, <= 1e-5 rad required
+        assert np.array_equal(np.isnan(gp), np.isnan(ep))
+        m = ~np.isnan(ep[:, 0])
+        assert np.array_equal(gp[m], ep[m])
+        mn = ~np.isnan(en[:, 0])
+        assert np.array_equal(mn, ~np.isnan(gn[:, 0]))
+        cosang = np.clip(np.sum(gn[mn].astype(np.float64) * en[mn], 1), -1, 1)
+        assert np.degrees(np.arccos(cosang)).max() < 1e-3
+        print("normals bit-exact fraction", np.mean(np.all(gn[mn] == en[mn], 1)))
+
+
+def test_match_workload1_poses(trained, oracle_mod, workload1):
+    from surface_match_amd import PointCloud, MatchParam, synth
+    O = oracle_mod
+    sc = workload1["scene"]
+    info = trained.model_info()
+    D = info.diameter
+    cloud = PointCloud(sc.points)
+    cloud.viewPoint = np.array(synth.VIEWPOINT, np.float32)
+    poses, scores = trained.matchScene(cloud, 0.04, 0.1, 0.2, MatchParam(numMatches=3))
+    assert len(poses) == 3
+    for P in poses:
+        k, ang, dt = synth.pose_error(P, sc.gt_poses, D)
+        assert ang < 0.5 and dt < 0.005 * D, (k, ang, dt)
+    mp, mn = workload1["model"]
+    om = O.train(mp, mn, 0.04)
+    opos, osc, _ = O.match(om, sc.points, None, 0.04, 0.1, 0.2, vp=synth.VIEWPOINT,
+                           param=O.default_match_param(numMatches=3))
+    # same instances found, poses within 0.1 deg / 0.1% D of the FP64 oracle refinement
+    for P, s in zip(poses, scores):
+        errs = [synth.pose_error(P, [np.vstack([Q, [0, 0, 0, 1]])], D) for Q in opos]
+        best = min(errs, key=lambda e: e[1] + e[2])
+        assert best[1] < 0.1 and best[2] < 0.001 * D, best
+    print("gpu scores", scores, "oracle scores", osc, trained.last_stats)
+
+
+def test_save_load_roundtrip(trained, tmp_path):
+    from surface_match_amd import Detector
+    f = tmp_path / "m.model"
+    trained.save(str(f))
+    d2 = Detector()
+    d2.load(str(f))
+    a, b = trained.table(), d2.table()
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
+    s1, s2 = trained.sampled_model(), d2.sampled_model()
+    for x, y in zip(s1, s2):
+        assert np.array_equal(x, y)
+    # byte-identical re-save
+    f2 = tmp_path / "m2.model"
+    d2.save(str(f2))
+    assert f.read_bytes() == f2.read_bytes()
