@@ -927,23 +927,33 @@ int orc_estimate_normals(int n, float *pts, float *nrm, const int *valid, int nV
     char  *have = (char *)calloc(n, 1);
     int   *nbq  = (int *)malloc(sizeof(int) * 64 * (size_t)(nQuery ? nQuery : 1));
     int   *cntq = (int *)calloc(nQuery ? nQuery : 1, sizeof(int));
+#pragma omp parallel for schedule(dynamic, 64)
     for (int t = 0; t < nQuery; t++) {
         int id = query[t];
         if (is_finite3(nrm0 + 3 * id))
             continue; /* computeNormal returns early, util.cpp:238-239 */
         cntq[t] = pca_normal(&g, pos0, id, k, cen + 3 * id, rawN + 3 * id, nbq + 64 * t);
-        if (cntq[t])
-            have[id] = 1;
     }
+    for (int t = 0; t < nQuery; t++)
+        if (cntq[t])
+            have[query[t]] = 1;
     if (smooth) {
+        /* neighbours still lacking a normal get their own PCA (util.cpp:292-295) */
+        char *need = (char *)calloc(n, 1);
         for (int t = 0; t < nQuery; t++)
             for (int x = 0; x < cntq[t]; x++) {
                 int j = nbq[64 * t + x];
-                if (have[j] || is_finite3(nrm0 + 3 * j))
-                    continue;
-                if (pca_normal(&g, pos0, j, k, cen + 3 * j, rawN + 3 * j, NULL))
-                    have[j] = 1;
+                if (!have[j] && !is_finite3(nrm0 + 3 * j))
+                    need[j] = 1;
             }
+#pragma omp parallel for schedule(dynamic, 256)
+        for (int j = 0; j < n; j++)
+            if (need[j] && pca_normal(&g, pos0, j, k, cen + 3 * j, rawN + 3 * j, NULL))
+                need[j] = 2;
+        for (int j = 0; j < n; j++)
+            if (need[j] == 2)
+                have[j] = 1;
+        free(need);
     }
     /* final normals of the queries */
     float *outN = (float *)malloc(sizeof(float) * 3 * (size_t)(nQuery ? nQuery : 1));
@@ -1314,10 +1324,14 @@ static iter_t icp_iteration(int nSrc, const float *sp, const float *sn, const or
      * pairs ordered by first occurrence of the dst id in src order. */
     int npair = 0;
     int *order = (int *)malloc(sizeof(int) * (nSrc ? nSrc : 1));
+#pragma omp parallel for schedule(static, 64)
     for (int i = 0; i < nSrc; i++) {
         float d;
         nnId[i] = nn1(dst, sp + 3 * i, &d);
         nnD[i]  = d;
+    }
+    for (int i = 0; i < nSrc; i++) {
+        float d = nnD[i];
         if (nnId[i] < 0 || d > rej2)
             continue;
         int q = nnId[i];
@@ -1465,14 +1479,19 @@ static iter_t icp_iteration(int nSrc, const float *sp, const float *sn, const or
     }
     memcpy(r.pose, U, sizeof U);
     /* mean NN distance of the corresponded src points after the step */
-    double sum = 0;
+    double *dd = (double *)malloc(sizeof(double) * npair);
+#pragma omp parallel for schedule(static, 64)
     for (int t = 0; t < npair; t++) {
         int   i = best[order[t]];
         float pp[3], d;
         xform_pt(U, sp + 3 * i, pp);
         nn1(dst, pp, &d);
-        sum += sqrt((double)d);
+        dd[t] = sqrt((double)d);
     }
+    double sum = 0;
+    for (int t = 0; t < npair; t++)
+        sum += dd[t];
+    free(dd);
     r.mse = (float)(sum / npair);
     for (int t = 0; t < npair; t++)
         best[order[t]] = -1;
@@ -1572,6 +1591,7 @@ int orc_inliers(int nSrc, const float *srcP, const float pose[12], const orc_gri
         T[i] = pose[i];
     float r2  = dist * dist;
     int   cnt = 0;
+#pragma omp parallel for reduction(+ : cnt) schedule(static, 64)
     for (int i = 0; i < nSrc; i++) {
         float p[3];
         /* transformPointCloud in float: R p + t */
@@ -1731,6 +1751,10 @@ int orc_match(const orc_model *m, int n, const float *pts, const float *nrm, con
     memset(st, 0, sizeof *st);
     if (rel > 1 || rel < 0 || kpFrac > 1 || kpFrac < 0 || minScore > 1 || minScore < 0 || n <= 0)
         return -1;
+#ifdef _OPENMP
+    int prevThreads = omp_get_max_threads();
+    omp_set_num_threads(nThreads > 0 ? nThreads : 1);
+#endif
     double t0 = now_ms();
     float *P  = (float *)malloc(sizeof(float) * 3 * (size_t)n);
     float *N  = (float *)malloc(sizeof(float) * 3 * (size_t)n);
@@ -1955,5 +1979,8 @@ int orc_match(const orc_model *m, int n, const float *pts, const float *nrm, con
     free(nms);
     free(dense);
     free(res);
+#ifdef _OPENMP
+    omp_set_num_threads(prevThreads);
+#endif
     return nr;
 }

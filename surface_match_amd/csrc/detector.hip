@@ -569,6 +569,7 @@ void match_impl(ppf_detector_s &D, DevCloud &c, float rel, float kpFrac, float m
     st->nValid          = c.nValid;
     st->nSampled        = ns;
     st->nKeypoints      = nk;
+    st->nVotedKeypoints = vs.voted;
     st->nPoses          = np;
     st->nClusters       = nClusters;
     st->nHypotheses     = (int)hyp.size();
@@ -583,7 +584,7 @@ void match_impl(ppf_detector_s &D, DevCloud &c, float rel, float kpFrac, float m
     st->msNormals       = elapsed(D.ev[1], D.ev[2]);
     st->msKeypoints     = elapsed(D.ev[2], D.ev[3]);
     st->msVote          = elapsed(D.ev[3], D.ev[5]);
-    st->msVoteKernel    = elapsed(D.ev[4], D.ev[5]);
+    st->msVoteKernel    = vs.kernelMs;
     st->msCluster       = elapsed(D.ev[5], D.ev[6]);
     st->msIcpPrepare    = elapsed(D.ev[6], D.ev[7]);
     st->msIcp           = elapsed(D.ev[7], D.ev[8]);

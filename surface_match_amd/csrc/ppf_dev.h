@@ -20,6 +20,13 @@ struct Workspace {
         clCenters, clNms, clMembers;
     // icp
     DevBuf icpPose, icpCur, icpState, icpHash, icpPartial, icpOut, icpInl;
+    // events bracketing the vote kernel (created on first use)
+    hipEvent_t voteEv[2] = {nullptr, nullptr};
+    ~Workspace() {
+        for (auto &e : voteEv)
+            if (e)
+                (void)hipEventDestroy(e);
+    }
 };
 
 // ---------------------------------------------------------------- model
@@ -91,6 +98,8 @@ struct PoseRec {
 
 struct VoteStats {
     unsigned long long pairs, hits, votes, candidates;
+    int   voted;      // key points that passed the neighbour-count threshold
+    float kernelMs;   // device time of the vote kernel launch alone
 };
 
 struct VoteConfig {

@@ -262,6 +262,8 @@ __global__ void __launch_bounds__(BLOCK) vote_kernel(VoteArgs a) {
             atomicAdd(a.stats + 0, nPairs);
             atomicAdd(a.stats + 1, nHits);
         }
+        if (tid == 0)
+            atomicAdd(a.stats + 3, 1ull);
     }
     if (a.stats) {
         for (int o = 32; o > 0; o >>= 1)
@@ -433,12 +435,17 @@ int vote_and_poses(Workspace &ws, const DevModel &m, const GridView &g, const fl
     a.xcdRemap  = cfg.xcdRemap;
     size_t lds  = (size_t)tileRefs * m.angleNum * 4;
     int blocks  = (cfg.xcdRemap ? 8 * ((nk + 7) / 8) : nk) * nTiles;
+    for (auto &e : ws.voteEv)
+        if (!e)
+            PPF_HIP_CHECK(hipEventCreate(&e));
+    PPF_HIP_CHECK(hipEventRecord(ws.voteEv[0], s));
     switch (cfg.block) {
         case 256: launch_vote<256>(a, blocks, lds, s); break;
         case 512: launch_vote<512>(a, blocks, lds, s); break;
         default: launch_vote<1024>(a, blocks, lds, s); break;
     }
     PPF_HIP_CHECK(hipGetLastError());
+    PPF_HIP_CHECK(hipEventRecord(ws.voteEv[1], s));
 
     PoseRec *pslots = reinterpret_cast<PoseRec *>(ws.candTile.as<PoseRec>((size_t)nk * 3));
     int     *counts = ws.voteCnt.as<int>((size_t)nk * 2 + 2);
@@ -464,10 +471,15 @@ int vote_and_poses(Workspace &ws, const DevModel &m, const GridView &g, const fl
         stats->pairs = st[0];
         stats->hits  = st[1];
         stats->votes = st[2];
+        stats->voted = (int)st[3];
     }
     PPF_HIP_CHECK(hipStreamSynchronize(s));
-    if (stats)
+    if (stats) {
         stats->candidates = (unsigned long long)np;
+        float ms          = 0.0f;
+        PPF_HIP_CHECK(hipEventElapsedTime(&ms, ws.voteEv[0], ws.voteEv[1]));
+        stats->kernelMs = ms;
+    }
     return np;
 }
 
