@@ -182,6 +182,46 @@ PPF_API int ppf_estimate_normals(float *pts, float *nrm, size_t n, const int *va
                                  size_t nValid, const int *query, size_t nq, int k, int smooth,
                                  int invert, const float viewPoint[3]);
 
+/* -- ICP on explicit hypotheses (ICP::regist with a prebuilt search
+ * structure and several initial poses, icp.h:86-90 / icp.cpp:214-278) -- */
+typedef struct ppf_icp_target_s *ppf_icp_target;
+/* ConvergenceCriteria, icp.h:7-25 */
+typedef struct {
+    int   iterations;
+    float rejectDist, mseMin, mseMax, tolerance;
+} ppf_icp_criteria;
+/* ConvergenceResult, icp.h:34-44, plus the inlier count of util.cpp:628-641 */
+typedef struct {
+    float pose16[16]; /* column-major 4x4 */
+    int   type;       /* 0 ITER, 1 MSE, 2 CONVERGE_RATE, 3 NO_CORRESPONDS */
+    float mse, convergeRate;
+    int   iterations, converged, inliers;
+} ppf_icp_result;
+/* the target cloud with normals, gridded once (cell: grid edge, 0 = auto) */
+PPF_API int  ppf_icp_target_create(const float *pts, const float *nrm, size_t n, float cell,
+                                   ppf_icp_target *out);
+PPF_API void ppf_icp_target_destroy(ppf_icp_target t);
+/* refine nPoses initial poses of src (with normals).  mode 0 = the
+ * reference's symmetric objective, 1 = point-to-plane.  batched != 0: one
+ * persistent launch for all poses (replayed from a captured hipGraph);
+ * batched == 0: the conventional refiner, one pose at a time.  block: threads
+ * per hypothesis workgroup in the batched launch (0 = tuned default).
+ * deviceMs (optional) receives the refinement's device time. */
+PPF_API int ppf_icp_refine(ppf_icp_target t, const float *src, const float *srcNrm, size_t nSrc,
+                           const float *initPoses16, size_t nPoses, const ppf_icp_criteria *c,
+                           int mode, int batched, int block, float scoringDist,
+                           ppf_icp_result *out, float *deviceMs);
+/* point-to-plane cost adjoint (no reference counterpart; its Jacobian rows
+ * are icp.cpp:318-320's): for each pose T = U(xi) T0, U(xi) = [R(w),
+ * c - R(w)c + v], the correspondences of the refiner, and out[36] per pose =
+ * {cost, count, c[3], grad[6] (= J^T r at xi = 0), J^T J[21] (upper
+ * triangle, row-major), 0 x4}; center 0 = origin, 1 = winners' centroid.
+ * corr (optional, nPoses x nSrc) receives each source point's target id
+ * (-1 = none). */
+PPF_API int ppf_icp_adjoint(ppf_icp_target t, const float *src, size_t nSrc, const float *poses16,
+                            const double *xi6, size_t nPoses, float rejectDist, int center,
+                            double *out, int *corr);
+
 /* L0 scalar math on the device, for bitwise host/device checks:
  * op 0 atanf(a), 1 atan2f(a,b), 2 a/b, 3 sqrtf(a), 4 murmur of 4 uint words
  * (a,b bit patterns as words 0,1 and 2,3 = 0), 5 quant ceil(a/b), 6 quant trunc,

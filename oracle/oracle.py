@@ -173,6 +173,7 @@ def lib():
                                      C.POINTER(Criteria), C.c_int, C.POINTER(IcpResult)]
         L.orc_inliers.restype = C.c_int
         L.orc_inliers.argtypes = [C.c_int, fp, fp, C.POINTER(Grid), C.c_float]
+        L.orc_p2pl_system.argtypes = [C.c_int, fp, dp, ip, fp, fp, dp, dp, C.c_int, dp]
         L.orc_train.argtypes = [C.c_int, fp, C.c_void_p, C.c_void_p, C.c_void_p, C.c_float,
                                 C.POINTER(TrainParam), C.c_int, C.POINTER(Model)]
         L.orc_free_model.argtypes = [C.POINTER(Model)]
@@ -294,6 +295,15 @@ def estimate_normals(pts, nrm, valid, query, k=10, smooth=1, invert=0, vp=None):
                                len(valid), np.ascontiguousarray(query, np.int32), len(query), k,
                                smooth, invert, _ptr(vpa))
     return pts, nrm
+
+
+def p2pl_system(src, T12, corr, dst, dstn, center, xi, tukey=0):
+    out = np.zeros(28)
+    lib().orc_p2pl_system(len(src), f32(src), np.ascontiguousarray(T12, np.float64),
+                          np.ascontiguousarray(corr, np.int32), f32(dst), f32(dstn),
+                          np.ascontiguousarray(center, np.float64),
+                          np.ascontiguousarray(xi, np.float64), tukey, out)
+    return out
 
 
 def train(pts, nrm=None, rel=0.04, param=None, box6=None, vp=None, ref_tail=1):

@@ -1605,6 +1605,70 @@ int orc_inliers(int nSrc, const float *srcP, const float pose[12], const orc_gri
     return cnt;
 }
 
+/* Point-to-plane system in FP64 for fixed correspondences (the system of
+ * icp.cpp:289-324): x_i = U(xi) T p_i with U(xi) = [R(w), c - R(w) c + v],
+ * r_i = n_q.(x_i - q_i), J_i = [(x_i - c) x n_q ; n_q] (icp.cpp:318-320),
+ * weights 1 or TukeyLossWeight(|r|, 2) (icp.cpp:284-287).
+ * out[28] = {cost = 1/2 sum w r^2, grad[6] = sum w r J, JtJ[21] upper}. */
+void orc_p2pl_system(int nSrc, const float *srcP, const double T[12], const int *corr,
+                     const float *dstP, const float *dstN, const double c[3], const double xi[6],
+                     int tukey, double out[28]) {
+    double R[9], th = sqrt(xi[0] * xi[0] + xi[1] * xi[1] + xi[2] * xi[2]);
+    if (th > 0) {
+        double k[3] = {xi[0] / th, xi[1] / th, xi[2] / th}, cs = cos(th), sn = sin(th);
+        double K[9] = {0, -k[2], k[1], k[2], 0, -k[0], -k[1], k[0], 0};
+        for (int r = 0; r < 3; r++)
+            for (int q = 0; q < 3; q++) {
+                double kk = 0;
+                for (int m = 0; m < 3; m++)
+                    kk += K[r * 3 + m] * K[m * 3 + q];
+                R[r * 3 + q] = ((r == q) ? 1.0 : 0.0) + sn * K[r * 3 + q] + (1 - cs) * kk;
+            }
+    } else {
+        for (int i = 0; i < 9; i++)
+            R[i] = (i % 4 == 0) ? 1.0 : 0.0;
+    }
+    for (int i = 0; i < 28; i++)
+        out[i] = 0;
+    for (int i = 0; i < nSrc; i++) {
+        int q = corr[i];
+        if (q < 0)
+            continue;
+        double y[3], x[3], d[3], n[3], J[6];
+        for (int r = 0; r < 3; r++)
+            y[r] = T[r * 4] * srcP[3 * i] + T[r * 4 + 1] * srcP[3 * i + 1] +
+                   T[r * 4 + 2] * srcP[3 * i + 2] + T[r * 4 + 3];
+        for (int r = 0; r < 3; r++)
+            x[r] = R[r * 3] * (y[0] - c[0]) + R[r * 3 + 1] * (y[1] - c[1]) +
+                   R[r * 3 + 2] * (y[2] - c[2]) + c[r] + xi[3 + r];
+        for (int r = 0; r < 3; r++) {
+            n[r] = dstN[3 * q + r];
+            d[r] = x[r] - c[r];
+        }
+        double res = (x[0] - dstP[3 * q]) * n[0] + (x[1] - dstP[3 * q + 1]) * n[1] +
+                     (x[2] - dstP[3 * q + 2]) * n[2];
+        double w = 1.0;
+        if (tukey) {
+            double e = fabs(res) / 2.0;
+            e        = e < 1.0 ? e : 1.0;
+            w        = (1.0 - e * e) * (1.0 - e * e);
+        }
+        J[0] = d[1] * n[2] - d[2] * n[1];
+        J[1] = d[2] * n[0] - d[0] * n[2];
+        J[2] = d[0] * n[1] - d[1] * n[0];
+        J[3] = n[0];
+        J[4] = n[1];
+        J[5] = n[2];
+        out[0] += 0.5 * w * res * res;
+        for (int k = 0; k < 6; k++)
+            out[1 + k] += w * res * J[k];
+        int m = 7;
+        for (int a = 0; a < 6; a++)
+            for (int b = a; b < 6; b++)
+                out[m++] += w * J[a] * J[b];
+    }
+}
+
 /* ====================================================================== */
 /* end to end                                                             */
 /* ====================================================================== */

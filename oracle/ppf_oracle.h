@@ -153,6 +153,9 @@ int orc_icp_regist(int nSrc, const float *srcP, const float *srcN, const orc_gri
                    int mode, orc_icp_result *res);
 int orc_inliers(int nSrc, const float *srcP, const float pose[12], const orc_grid *g,
                 float dist);
+void orc_p2pl_system(int nSrc, const float *srcP, const double T[12], const int *corr,
+                     const float *dstP, const float *dstN, const double c[3], const double xi[6],
+                     int tukey, double out[28]);
 
 /* ---- end-to-end (ppf.cpp:36-147,149-510) ----------------------------- */
 typedef struct {

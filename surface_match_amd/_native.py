@@ -58,6 +58,17 @@ class ModelInfo(C.Structure):
                 ("boxMax", C.c_float * 3)]
 
 
+class IcpCriteria(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("rejectDist", C.c_float), ("mseMin", C.c_float),
+                ("mseMax", C.c_float), ("tolerance", C.c_float)]
+
+
+class IcpResult(C.Structure):
+    _fields_ = [("pose16", C.c_float * 16), ("type", C.c_int), ("mse", C.c_float),
+                ("convergeRate", C.c_float), ("iterations", C.c_int), ("converged", C.c_int),
+                ("inliers", C.c_int)]
+
+
 _lib = None
 
 # every function declared in include/ppf_c.h
@@ -68,6 +79,7 @@ EXPORTS = [
     "ppf_scene_free", "ppf_match_scene_resident", "ppf_model_get_info", "ppf_model_get_sampled",
     "ppf_model_get_table", "ppf_pair_features", "ppf_vote", "ppf_voxel_sample",
     "ppf_radius_search", "ppf_knn_search", "ppf_estimate_normals", "ppf_device_math",
+    "ppf_icp_target_create", "ppf_icp_target_destroy", "ppf_icp_refine", "ppf_icp_adjoint",
 ]
 
 
@@ -106,6 +118,11 @@ def lib() -> C.CDLL:
         "ppf_knn_search": (i, [vp, sz, vp, sz, i, f, vp, vp]),
         "ppf_estimate_normals": (i, [vp, vp, sz, vp, sz, vp, sz, i, i, i, vp]),
         "ppf_device_math": (i, [i, vp, vp, sz, vp]),
+        "ppf_icp_target_create": (i, [vp, vp, sz, f, C.POINTER(vp)]),
+        "ppf_icp_target_destroy": (None, [vp]),
+        "ppf_icp_refine": (i, [vp, vp, vp, sz, vp, sz, C.POINTER(IcpCriteria), i, i, i, f,
+                               vp, C.POINTER(f)]),
+        "ppf_icp_adjoint": (i, [vp, vp, sz, vp, vp, sz, f, i, vp, vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

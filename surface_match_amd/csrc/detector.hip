@@ -200,6 +200,23 @@ struct ppf_scene_s {
     DevCloud cloud;
 };
 
+struct ppf_icp_target_s {
+    hipStream_t stream = nullptr;
+    DevBuf      x, y, z, nx, ny, nz, aos;
+    Grid        grid, full;
+    Workspace   ws;
+    DevBuf      sx, sy, sz, snx, sny, snz, saos;
+    int         n = 0;
+    hipEvent_t  ev[2] = {nullptr, nullptr};
+    ~ppf_icp_target_s() {
+        for (auto &e : ev)
+            if (e)
+                (void)hipEventDestroy(e);
+        if (stream)
+            (void)hipStreamDestroy(stream);
+    }
+};
+
 struct ppf_detector_s {
     hipStream_t  stream = nullptr;
     bool         trained = false;
@@ -1250,6 +1267,138 @@ PPF_API int ppf_estimate_normals(float *pts, float *nrm, size_t n, const int *va
             PPF_HIP_CHECK(hipMemcpyAsync(nrm, o, sizeof(float) * 3 * n, hipMemcpyDeviceToHost, s));
         }
         PPF_HIP_CHECK(hipStreamSynchronize(s));
+    });
+}
+
+static void pose16_to_12(const float *p16, float *p12) {
+    for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 4; c++)
+            p12[r * 4 + c] = p16[c * 4 + r];
+}
+
+PPF_API int ppf_icp_target_create(const float *pts, const float *nrm, size_t n, float cell,
+                                  ppf_icp_target *out) {
+    PPF_REQUIRE(pts && nrm && out && n > 0, "bad argument");
+    *out = nullptr;
+    return guarded([&] {
+        auto t = std::make_unique<ppf_icp_target_s>();
+        PPF_HIP_CHECK(hipStreamCreateWithFlags(&t->stream, hipStreamNonBlocking));
+        for (auto &e : t->ev)
+            PPF_HIP_CHECK(hipEventCreate(&e));
+        hipStream_t s = t->stream;
+        int         m = (int)n;
+        float      *A = t->aos.as<float>((size_t)m * 3);
+        PPF_HIP_CHECK(hipMemcpyAsync(A, pts, sizeof(float) * 3 * n, hipMemcpyHostToDevice, s));
+        split_kernel<<<div_up(m, 256), 256, 0, s>>>(A, m, t->x.as<float>(m), t->y.as<float>(m),
+                                                     t->z.as<float>(m));
+        PPF_HIP_CHECK(hipStreamSynchronize(s));
+        PPF_HIP_CHECK(hipMemcpyAsync(A, nrm, sizeof(float) * 3 * n, hipMemcpyHostToDevice, s));
+        split_kernel<<<div_up(m, 256), 256, 0, s>>>(A, m, t->nx.as<float>(m), t->ny.as<float>(m),
+                                                     t->nz.as<float>(m));
+        if (!(cell > 0.0f)) {
+            std::vector<int> v(n);
+            std::iota(v.begin(), v.end(), 0);
+            float lo[3], hi[3];
+            bbox_host(pts, v, lo, hi);
+            double vol = 1.0;
+            for (int a = 0; a < 3; a++)
+                vol *= (double)(hi[a] - lo[a]) + 1e-3;
+            cell = (float)(2.0 * std::cbrt(vol / (double)n));
+        }
+        grid_build(t->grid, t->x.get<float>(), t->y.get<float>(), t->z.get<float>(),
+                   t->nx.get<float>(), t->ny.get<float>(), t->nz.get<float>(), nullptr, m, cell, s);
+        grid_build(t->full, t->x.get<float>(), t->y.get<float>(), t->z.get<float>(), nullptr,
+                   nullptr, nullptr, nullptr, m, cell, s);
+        t->n = m;
+        *out = t.release();
+    });
+}
+
+PPF_API void ppf_icp_target_destroy(ppf_icp_target t) { delete t; }
+
+static void upload_src(ppf_icp_target_s &t, const float *src, const float *nrm, int n) {
+    hipStream_t s = t.stream;
+    int         m = std::max(n, 1);
+    float      *A = t.saos.as<float>((size_t)m * 3);
+    PPF_HIP_CHECK(hipMemcpyAsync(A, src, sizeof(float) * 3 * n, hipMemcpyHostToDevice, s));
+    split_kernel<<<div_up(m, 256), 256, 0, s>>>(A, n, t.sx.as<float>(m), t.sy.as<float>(m),
+                                                 t.sz.as<float>(m));
+    PPF_HIP_CHECK(hipStreamSynchronize(s));
+    t.snx.as<float>(m);
+    t.sny.as<float>(m);
+    t.snz.as<float>(m);
+    if (nrm) {
+        PPF_HIP_CHECK(hipMemcpyAsync(A, nrm, sizeof(float) * 3 * n, hipMemcpyHostToDevice, s));
+        split_kernel<<<div_up(m, 256), 256, 0, s>>>(A, n, t.snx.get<float>(), t.sny.get<float>(),
+                                                     t.snz.get<float>());
+        PPF_HIP_CHECK(hipStreamSynchronize(s));
+    }
+}
+
+PPF_API int ppf_icp_refine(ppf_icp_target t, const float *src, const float *srcNrm, size_t nSrc,
+                           const float *initPoses16, size_t nPoses, const ppf_icp_criteria *c,
+                           int mode, int batched, int block, float scoringDist,
+                           ppf_icp_result *out, float *deviceMs) {
+    PPF_REQUIRE(t && src && srcNrm && initPoses16 && c && out, "null argument");
+    return guarded([&] {
+        upload_src(*t, src, srcNrm, (int)nSrc);
+        std::vector<float> init(12 * nPoses);
+        for (size_t h = 0; h < nPoses; h++)
+            pose16_to_12(initPoses16 + 16 * h, &init[12 * h]);
+        IcpCriteria ic{c->iterations, c->rejectDist, c->mseMin, c->mseMax, c->tolerance};
+        std::vector<IcpResultDev> res(nPoses);
+        hipStream_t s = t->stream;
+        PPF_HIP_CHECK(hipEventRecord(t->ev[0], s));
+        if (batched) {
+            IcpConfig cfg;
+            cfg.mode  = mode;
+            cfg.block = block > 0 ? block : 256;
+            icp_batch(t->ws, (int)nSrc, t->sx.get<float>(), t->sy.get<float>(), t->sz.get<float>(),
+                      t->snx.get<float>(), t->sny.get<float>(), t->snz.get<float>(),
+                      t->grid.view, t->full.view, init.data(), (int)nPoses, ic, scoringDist, cfg,
+                      nullptr, s);
+        } else {
+            icp_plain(t->ws, (int)nSrc, t->sx.get<float>(), t->sy.get<float>(), t->sz.get<float>(),
+                      t->snx.get<float>(), t->sny.get<float>(), t->snz.get<float>(),
+                      t->grid.view, t->full.view, init.data(), (int)nPoses, ic, scoringDist, mode,
+                      nullptr, s);
+        }
+        PPF_HIP_CHECK(hipEventRecord(t->ev[1], s));
+        PPF_HIP_CHECK(hipMemcpyAsync(res.data(), t->ws.icpOut.get<IcpResultDev>(),
+                                     sizeof(IcpResultDev) * nPoses, hipMemcpyDeviceToHost, s));
+        PPF_HIP_CHECK(hipStreamSynchronize(s));
+        if (deviceMs)
+            *deviceMs = elapsed(t->ev[0], t->ev[1]);
+        for (size_t h = 0; h < nPoses; h++) {
+            ppf_icp_result &o = out[h];
+            const float    *P = res[h].pose;
+            for (int r = 0; r < 3; r++)
+                for (int c2 = 0; c2 < 4; c2++)
+                    o.pose16[c2 * 4 + r] = P[r * 4 + c2];
+            o.pose16[3] = o.pose16[7] = o.pose16[11] = 0.0f;
+            o.pose16[15]   = 1.0f;
+            o.type         = res[h].type;
+            o.mse          = res[h].mse;
+            o.convergeRate = res[h].convergeRate;
+            o.iterations   = res[h].iterations;
+            o.converged    = res[h].converged;
+            o.inliers      = res[h].inliers;
+        }
+    });
+}
+
+PPF_API int ppf_icp_adjoint(ppf_icp_target t, const float *src, size_t nSrc, const float *poses16,
+                            const double *xi6, size_t nPoses, float rejectDist, int center,
+                            double *out, int *corr) {
+    PPF_REQUIRE(t && src && poses16 && xi6 && out, "null argument");
+    return guarded([&] {
+        upload_src(*t, src, nullptr, (int)nSrc);
+        std::vector<float> T0(12 * nPoses);
+        for (size_t h = 0; h < nPoses; h++)
+            pose16_to_12(poses16 + 16 * h, &T0[12 * h]);
+        icp_adjoint(t->ws, (int)nSrc, t->sx.get<float>(), t->sy.get<float>(), t->sz.get<float>(),
+                    t->grid.view, T0.data(), xi6, (int)nPoses, rejectDist, center, out, corr,
+                    t->stream);
     });
 }
 

@@ -211,17 +211,26 @@ __device__ __forceinline__ float ring_safe(const GridView &g, int c0x, int c0y, 
     return s > 0.0f ? s : 0.0f;
 }
 
+// smallest Chebyshev ring around c0 that reaches a cell of the grid
+// (rings before it are empty when the query lies outside the grid)
+__device__ __forceinline__ int first_ring(const GridView &g, int c0x, int c0y, int c0z) {
+    int rx = c0x < 0 ? -c0x : (c0x >= g.dx ? c0x - (g.dx - 1) : 0);
+    int ry = c0y < 0 ? -c0y : (c0y >= g.dy ? c0y - (g.dy - 1) : 0);
+    int rz = c0z < 0 ? -c0z : (c0z >= g.dz ? c0z - (g.dz - 1) : 0);
+    return max(max(rx, ry), rz);
+}
+
 template <int KMAX>
 __device__ void grid_knn(const GridView &g, float qx, float qy, float qz, int k,
                          KnnList<KMAX> &L) {
     L.init();
-    if (g.n == 0)
+    if (g.n == 0 || !(isfinite(qx) && isfinite(qy) && isfinite(qz)))
         return;
     int c0x = cell_coord_unclamped(qx, g.ox, g.cell);
     int c0y = cell_coord_unclamped(qy, g.oy, g.cell);
     int c0z = cell_coord_unclamped(qz, g.oz, g.cell);
     int maxR = max(max(g.dx, g.dy), g.dz) + max(max(abs(c0x), abs(c0y)), abs(c0z)) + 1;
-    for (int r = 0; r <= maxR; r++) {
+    for (int r = first_ring(g, c0x, c0y, c0z); r <= maxR; r++) {
         float bound2 = (L.cnt == k) ? L.d[k - 1] : FLT_MAX;
         visit_ring(g, c0x, c0y, c0z, r, qx, qy, qz, bound2, [&](int s, int c) {
             for (int t = s; t < s + c; t++) {
@@ -243,13 +252,13 @@ __device__ inline int grid_nn1(const GridView &g, float qx, float qy, float qz, 
     bestD      = FLT_MAX;
     bestId     = 0x7fffffff;
     int bestS  = -1;
-    if (g.n == 0)
+    if (g.n == 0 || !(isfinite(qx) && isfinite(qy) && isfinite(qz)))
         return -1;
     int c0x  = cell_coord_unclamped(qx, g.ox, g.cell);
     int c0y  = cell_coord_unclamped(qy, g.oy, g.cell);
     int c0z  = cell_coord_unclamped(qz, g.oz, g.cell);
     int maxR = max(max(g.dx, g.dy), g.dz) + max(max(abs(c0x), abs(c0y)), abs(c0z)) + 1;
-    for (int r = 0; r <= maxR; r++) {
+    for (int r = first_ring(g, c0x, c0y, c0z); r <= maxR; r++) {
         visit_ring(g, c0x, c0y, c0z, r, qx, qy, qz, bestD, [&](int s, int c) {
             for (int t = s; t < s + c; t++) {
                 float dd = ppfc::dist2(qx, qy, qz, g.x[t], g.y[t], g.z[t]);
@@ -272,7 +281,7 @@ __device__ inline int grid_nn1(const GridView &g, float qx, float qy, float qz, 
 
 // true if any indexed point has dist2 < r2
 __device__ inline bool grid_any_within(const GridView &g, float qx, float qy, float qz, float r2) {
-    if (g.n == 0)
+    if (g.n == 0 || !(isfinite(qx) && isfinite(qy) && isfinite(qz)))
         return false;
     float r   = sqrtf(r2) * 1.0001f + 1e-3f * g.cell;
     int   xlo = cell_coord(qx - r, g.ox, g.cell, g.dx), xhi = cell_coord(qx + r, g.ox, g.cell, g.dx);

@@ -20,6 +20,7 @@
 #include "ppf_dev.h"
 
 #include <algorithm>
+#include <cstring>
 
 namespace ppfd {
 
@@ -152,324 +153,385 @@ __device__ void rodrigues(const double w[3], double R[9]) {
         }
 }
 
+// ---------------------------------------------------------------- shared per-point steps
+//
+// The batched kernel (one workgroup per hypothesis, state in LDS) and the
+// plain multi-kernel refiner (one hypothesis at a time, state in HBM) call
+// the same functions below, so they perform the same arithmetic.
+
+struct IcpState {
+    double pose[12], cur[12], U[12];
+    float  curF[12], UF[12], means[6];
+    float  mse, rate;
+    int    iters, type, conv, stop, valid;
+};
+
+__device__ void state_init(IcpState &st, const float *init) {
+    bool ident = true;
+    for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 4; c++) {
+            float v = init[r * 4 + c], e = (r == c) ? 1.0f : 0.0f;
+            if (fabsf(v - e) > 1e-5f)
+                ident = false;
+        }
+    for (int i = 0; i < 12; i++) {
+        st.pose[i] = init[i];
+        st.cur[i]  = ident ? ((i % 5 == 0) ? 1.0 : 0.0) : st.pose[i];
+        st.curF[i] = (float)st.cur[i];
+    }
+    st.mse   = FLT_MAX;
+    st.rate  = 1.0f;
+    st.iters = 0;
+    st.type  = 0;
+    st.conv  = 0;
+    st.stop  = 0;
+    st.valid = 0;
+}
+
+__device__ __forceinline__ void xform(const float *T, float x, float y, float z, float &ox,
+                                      float &oy, float &oz) {
+    ox = T[0] * x + T[1] * y + T[2] * z + T[3];
+    oy = T[4] * x + T[5] * y + T[6] * z + T[7];
+    oz = T[8] * x + T[9] * y + T[10] * z + T[11];
+}
+
+struct HypoScratch {
+    uint32_t           *hkey;
+    unsigned long long *hbest;
+    int                *nnSlot, *hslot;
+    float              *nnD;
+    uint32_t            T;
+};
+
+__device__ __forceinline__ HypoScratch scratch_of(const IcpArgs &a, int h) {
+    HypoScratch x;
+    x.hkey   = a.hkey + (size_t)h * a.T;
+    x.hbest  = a.hbest + (size_t)h * a.T;
+    x.nnSlot = a.nnSlot + (size_t)h * a.nSrc;
+    x.hslot  = a.hslot + (size_t)h * a.nSrc;
+    x.nnD    = a.nnD + (size_t)h * a.nSrc;
+    x.T      = a.T;
+    return x;
+}
+
+// findCorresponds for one source point (icp.cpp:123-167): nearest scene
+// point, rejection beyond rejectDist, per-scene-point dedupe by 64-bit
+// atomicMin on (dist2 bits, source index)
+__device__ void corr_point(const IcpArgs &a, const HypoScratch &x, int i, const float *curF,
+                           float rej2) {
+    float px, py, pz;
+    xform(curF, a.sx[i], a.sy[i], a.sz[i], px, py, pz);
+    float d2;
+    int   did;
+    int   s  = grid_nn1(a.dst, px, py, pz, d2, did);
+    int   hs = -1;
+    if (s >= 0 && !(d2 > rej2)) {
+        uint32_t key = (uint32_t)did + 1u;
+        uint32_t hh  = icp_hash(key) & (x.T - 1);
+        for (;;) {
+            uint32_t prev = atomicCAS(&x.hkey[hh], 0u, key);
+            if (prev == 0u || prev == key)
+                break;
+            hh = (hh + 1u) & (x.T - 1);
+        }
+        atomicMin(&x.hbest[hh], ((unsigned long long)__float_as_uint(d2) << 32) | (uint32_t)i);
+        hs = (int)hh;
+    } else {
+        s = -1;
+    }
+    x.nnSlot[i] = s;
+    x.hslot[i]  = hs;
+    x.nnD[i]    = d2;
+}
+
+__device__ __forceinline__ bool is_winner(const HypoScratch &x, int i) {
+    if (x.nnSlot[i] < 0)
+        return false;
+    unsigned long long pk = ((unsigned long long)__float_as_uint(x.nnD[i]) << 32) | (uint32_t)i;
+    return x.hbest[x.hslot[i]] == pk;
+}
+
+__device__ __forceinline__ void add_means(const IcpArgs &a, const HypoScratch &x, int i,
+                                          const float *curF, CF (&m7)[7]) {
+    if (!is_winner(x, i))
+        return;
+    float px, py, pz;
+    xform(curF, a.sx[i], a.sy[i], a.sz[i], px, py, pz);
+    int s = x.nnSlot[i];
+    cf_add(m7[0], px);
+    cf_add(m7[1], py);
+    cf_add(m7[2], pz);
+    cf_add(m7[3], a.dst.x[s]);
+    cf_add(m7[4], a.dst.y[s]);
+    cf_add(m7[5], a.dst.z[s]);
+    cf_add(m7[6], 1.0f);
+}
+
+// one row of the normal equations: symmetric objective (icp.cpp:56-65) or
+// point-to-plane about the source centroid
+__device__ __forceinline__ void eq_row(const IcpArgs &a, int i, int s, const float *curF,
+                                       const float *means, int mode, float J[6], float &r) {
+    float px, py, pz;
+    xform(curF, a.sx[i], a.sy[i], a.sz[i], px, py, pz);
+    float qx = a.dst.x[s], qy = a.dst.y[s], qz = a.dst.z[s];
+    float nqx = a.dst.nx[s], nqy = a.dst.ny[s], nqz = a.dst.nz[s];
+    float ux = px - means[0], uy = py - means[1], uz = pz - means[2];
+    if (mode == 0) {
+        float mx = a.snx[i], my = a.sny[i], mz = a.snz[i];
+        float npx = curF[0] * mx + curF[1] * my + curF[2] * mz;
+        float npy = curF[4] * mx + curF[5] * my + curF[6] * mz;
+        float npz = curF[8] * mx + curF[9] * my + curF[10] * mz;
+        float Nx = npx + nqx, Ny = npy + nqy, Nz = npz + nqz;
+        float vx = qx - means[3], vy = qy - means[4], vz = qz - means[5];
+        float sx = ux + vx, sy = uy + vy, sz = uz + vz;
+        r    = (ux - vx) * Nx + (uy - vy) * Ny + (uz - vz) * Nz;
+        J[0] = sy * Nz - sz * Ny;
+        J[1] = sz * Nx - sx * Nz;
+        J[2] = sx * Ny - sy * Nx;
+        J[3] = Nx;
+        J[4] = Ny;
+        J[5] = Nz;
+    } else {
+        r    = (px - qx) * nqx + (py - qy) * nqy + (pz - qz) * nqz;
+        J[0] = uy * nqz - uz * nqy;
+        J[1] = uz * nqx - ux * nqz;
+        J[2] = ux * nqy - uy * nqx;
+        J[3] = nqx;
+        J[4] = nqy;
+        J[5] = nqz;
+    }
+}
+
+__device__ __forceinline__ void add_eq(const IcpArgs &a, const HypoScratch &x, int i,
+                                       const float *curF, const float *means, int mode,
+                                       CF (&e27)[27]) {
+    if (!is_winner(x, i))
+        return;
+    float J[6], r;
+    eq_row(a, i, x.nnSlot[i], curF, means, mode, J, r);
+    int k = 0;
+#pragma unroll
+    for (int p = 0; p < 6; p++)
+#pragma unroll
+        for (int q = p; q < 6; q++)
+            cf_add(e27[k++], J[p] * J[q]);
+#pragma unroll
+    for (int p = 0; p < 6; p++)
+        cf_add(e27[21 + p], r * J[p]);
+}
+
+__device__ __forceinline__ void add_mse(const IcpArgs &a, const HypoScratch &x, int i,
+                                        const float *curF, const float *UF, CF (&m1)[1]) {
+    if (!is_winner(x, i))
+        return;
+    float px, py, pz, qx, qy, qz;
+    xform(curF, a.sx[i], a.sy[i], a.sz[i], px, py, pz);
+    xform(UF, px, py, pz, qx, qy, qz);
+    float d2;
+    int   did;
+    grid_nn1(a.dst, qx, qy, qz, d2, did);
+    cf_add(m1[0], sqrtf(d2));
+}
+
+__device__ __forceinline__ void clear_point(const HypoScratch &x, int i) {
+    int hs = x.hslot[i];
+    if (hs >= 0) {
+        x.hkey[hs]  = 0u;
+        x.hbest[hs] = ~0ull;
+    }
+}
+
+// FP64 solve + update (icp.cpp:67-86 for the symmetric objective; Rodrigues
+// about the source centroid for point-to-plane)
+__device__ void solve_update(const double *red, const float *means, int mode, IcpState &st) {
+    double A[36], rhs[6], u[6];
+    int    k = 0;
+    for (int p = 0; p < 6; p++)
+        for (int q = p; q < 6; q++) {
+            A[p * 6 + q] = red[k];
+            A[q * 6 + p] = red[k];
+            k++;
+        }
+    for (int p = 0; p < 6; p++)
+        rhs[p] = -red[21 + p];
+    sym_solve6(A, rhs, u);
+    double U[12];
+    if (mode == 0) {
+        double an    = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+        double theta = atan(an);
+        double ax[3] = {0, 0, 0};
+        if (an > 0)
+            for (int i = 0; i < 3; i++)
+                ax[i] = u[i] / an;
+        double W[9] = {0, -ax[2], ax[1], ax[2], 0, -ax[0], -ax[1], ax[0], 0}, W2[9], Ri[9], R[9];
+        for (int r = 0; r < 3; r++)
+            for (int q = 0; q < 3; q++) {
+                double s2 = 0;
+                for (int m = 0; m < 3; m++)
+                    s2 += W[r * 3 + m] * W[m * 3 + q];
+                W2[r * 3 + q] = s2;
+            }
+        double st_ = sin(theta), ct = cos(theta);
+        for (int i = 0; i < 9; i++)
+            Ri[i] = ((i % 4 == 0) ? 1.0 : 0.0) + st_ * W[i] + (1.0 - ct) * W2[i];
+        for (int r = 0; r < 3; r++)
+            for (int q = 0; q < 3; q++) {
+                double s2 = 0;
+                for (int m = 0; m < 3; m++)
+                    s2 += Ri[r * 3 + m] * Ri[m * 3 + q];
+                R[r * 3 + q] = s2;
+            }
+        double pm[3] = {means[0], means[1], means[2]}, qm[3] = {means[3], means[4], means[5]};
+        double t2[3] = {ct * u[3], ct * u[4], ct * u[5]};
+        for (int r = 0; r < 3; r++) {
+            double v = 0;
+            for (int m = 0; m < 3; m++)
+                v += R[r * 3 + m] * (-pm[m]) + Ri[r * 3 + m] * t2[m];
+            U[r * 4 + 0] = R[r * 3 + 0];
+            U[r * 4 + 1] = R[r * 3 + 1];
+            U[r * 4 + 2] = R[r * 3 + 2];
+            U[r * 4 + 3] = v + qm[r];
+        }
+    } else {
+        double R[9];
+        rodrigues(u, R);
+        double c[3] = {means[0], means[1], means[2]};
+        for (int r = 0; r < 3; r++) {
+            double Rc = R[r * 3 + 0] * c[0] + R[r * 3 + 1] * c[1] + R[r * 3 + 2] * c[2];
+            U[r * 4 + 0] = R[r * 3 + 0];
+            U[r * 4 + 1] = R[r * 3 + 1];
+            U[r * 4 + 2] = R[r * 3 + 2];
+            U[r * 4 + 3] = c[r] - Rc + u[3 + r];
+        }
+    }
+    for (int i = 0; i < 12; i++) {
+        st.U[i]  = U[i];
+        st.UF[i] = (float)U[i];
+    }
+}
+
+// ICP::regist's convergence rules for one iteration (icp.cpp:236-271)
+__device__ void converge(IcpState &st, float mse, const IcpCriteria &c) {
+    int vp = st.valid;
+    if (vp > 6 && mse < c.mseMax)
+        st.conv = 1;
+    bool stop = false;
+    if (mse < c.mseMin) {
+        st.type = 1;
+        stop    = true;
+    }
+    float rate = (st.mse - mse) / st.mse;
+    if (rate < c.tolerance) {
+        st.type = 2;
+        stop    = true;
+    }
+    if (st.iters++ >= c.iterations) {
+        st.type = 0;
+        stop    = true;
+    }
+    if (rate > 0.0f) {
+        st.mse  = mse;
+        st.rate = rate;
+        compose3x4(st.U, st.pose, st.pose);
+    }
+    if (!stop) {
+        compose3x4(st.U, st.cur, st.cur);
+        for (int i = 0; i < 12; i++)
+            st.curF[i] = (float)st.cur[i];
+    }
+    st.stop = stop ? 1 : 0;
+}
+
+__device__ __forceinline__ void write_result(IcpResultDev &o, const IcpState &st, int inl) {
+    for (int i = 0; i < 12; i++)
+        o.pose[i] = (float)st.pose[i];
+    o.type         = st.type;
+    o.mse          = st.mse;
+    o.convergeRate = st.rate;
+    o.iterations   = st.iters;
+    o.converged    = st.conv;
+    o.inliers      = inl;
+}
+
+__device__ __forceinline__ bool inlier_point(const IcpArgs &a, int i, const float *PF, float r2) {
+    float px, py, pz;
+    xform(PF, a.sx[i], a.sy[i], a.sz[i], px, py, pz);
+    return grid_any_within(a.full, px, py, pz, r2);
+}
+
+// ---------------------------------------------------------------- batched
+
 template <int BLOCK>
 __global__ void __launch_bounds__(BLOCK) icp_kernel(IcpArgs a) {
     const int h = blockIdx.x;
     if (h >= a.H)
         return;
-    __shared__ double sPose[12], sCur[12], sU[12];
-    __shared__ float  sCurF[12], sUF[12];
-    __shared__ double sRed[40];
-    __shared__ CF     sLds[(BLOCK / 64) * 34];
-    __shared__ float  sMse, sRate;
-    __shared__ int    sIters, sType, sConv, sStop, sValid;
-    __shared__ int    sCnt[BLOCK / 64];
-
-    const int tid = threadIdx.x;
-    const int n   = a.nSrc;
-    uint32_t *hkey = a.hkey + (size_t)h * a.T;
-    unsigned long long *hbest = a.hbest + (size_t)h * a.T;
-    int   *nnSlot = a.nnSlot + (size_t)h * n;
-    int   *hslot  = a.hslot + (size_t)h * n;
-    float *nnD    = a.nnD + (size_t)h * n;
-
-    if (tid == 0) {
-        bool ident = true;
-        for (int r = 0; r < 3; r++)
-            for (int c = 0; c < 4; c++) {
-                float v = a.init[(size_t)h * 12 + r * 4 + c], e = (r == c) ? 1.0f : 0.0f;
-                if (fabsf(v - e) > 1e-5f)
-                    ident = false;
-            }
-        for (int i = 0; i < 12; i++) {
-            sPose[i] = a.init[(size_t)h * 12 + i];
-            sCur[i]  = ident ? ((i % 5 == 0) ? 1.0 : 0.0) : sPose[i];
-        }
-        sMse   = FLT_MAX;
-        sRate  = 1.0f;
-        sIters = 0;
-        sType  = 0;
-        sConv  = 0;
-        sStop  = 0;
-    }
+    __shared__ IcpState st;
+    __shared__ double   sRed[40];
+    __shared__ CF       sLds[(BLOCK / 64) * 27];
+    __shared__ int      sCnt[BLOCK / 64];
+    __shared__ float    sPF[12];
+    const int         tid = threadIdx.x;
+    const int         n   = a.nSrc;
+    const HypoScratch x   = scratch_of(a, h);
+    if (tid == 0)
+        state_init(st, a.init + (size_t)h * 12);
     for (uint32_t t = tid; t < a.T; t += BLOCK) {
-        hkey[t]  = 0u;
-        hbest[t] = ~0ull;
+        x.hkey[t]  = 0u;
+        x.hbest[t] = ~0ull;
     }
     __syncthreads();
     const float rej2 = a.c.rejectDist * a.c.rejectDist;
-
-    while (!sStop && sIters < a.c.iterations) {
-        if (tid < 12)
-            sCurF[tid] = (float)sCur[tid];
+    while (!st.stop && st.iters < a.c.iterations) {
+        for (int i = tid; i < n; i += BLOCK)
+            corr_point(a, x, i, st.curF, rej2);
         __syncthreads();
-        // ---- 1. correspondences
-        for (int i = tid; i < n; i += BLOCK) {
-            float x = a.sx[i], y = a.sy[i], z = a.sz[i];
-            float px = sCurF[0] * x + sCurF[1] * y + sCurF[2] * z + sCurF[3];
-            float py = sCurF[4] * x + sCurF[5] * y + sCurF[6] * z + sCurF[7];
-            float pz = sCurF[8] * x + sCurF[9] * y + sCurF[10] * z + sCurF[11];
-            float d2;
-            int   did;
-            int   s = grid_nn1(a.dst, px, py, pz, d2, did);
-            int   hs = -1;
-            if (s >= 0 && !(d2 > rej2)) {
-                uint32_t key = (uint32_t)did + 1u;
-                uint32_t hh  = icp_hash(key) & (a.T - 1);
-                for (;;) {
-                    uint32_t prev = atomicCAS(&hkey[hh], 0u, key);
-                    if (prev == 0u || prev == key)
-                        break;
-                    hh = (hh + 1u) & (a.T - 1);
-                }
-                atomicMin(&hbest[hh], ((unsigned long long)__float_as_uint(d2) << 32) | (uint32_t)i);
-                hs = (int)hh;
-            } else {
-                s = -1;
-            }
-            nnSlot[i] = s;
-            hslot[i]  = hs;
-            nnD[i]    = d2;
-        }
-        __syncthreads();
-        // ---- 2. winners, means
         CF m7[7];
         for (int k = 0; k < 7; k++)
             m7[k] = CF{0.0f, 0.0f};
-        for (int i = tid; i < n; i += BLOCK) {
-            int s = nnSlot[i];
-            if (s < 0)
-                continue;
-            unsigned long long pk = ((unsigned long long)__float_as_uint(nnD[i]) << 32) | (uint32_t)i;
-            if (hbest[hslot[i]] != pk)
-                continue;
-            float x = a.sx[i], y = a.sy[i], z = a.sz[i];
-            cf_add(m7[0], sCurF[0] * x + sCurF[1] * y + sCurF[2] * z + sCurF[3]);
-            cf_add(m7[1], sCurF[4] * x + sCurF[5] * y + sCurF[6] * z + sCurF[7]);
-            cf_add(m7[2], sCurF[8] * x + sCurF[9] * y + sCurF[10] * z + sCurF[11]);
-            cf_add(m7[3], a.dst.x[s]);
-            cf_add(m7[4], a.dst.y[s]);
-            cf_add(m7[5], a.dst.z[s]);
-            cf_add(m7[6], 1.0f);
-        }
+        for (int i = tid; i < n; i += BLOCK)
+            add_means(a, x, i, st.curF, m7);
         block_reduce_cf<BLOCK, 7>(m7, sLds, sRed);
-        const int   valid = (int)sRed[6];
-        const float pmx = (float)(sRed[0] / valid), pmy = (float)(sRed[1] / valid),
-                    pmz = (float)(sRed[2] / valid);
-        const float qmx = (float)(sRed[3] / valid), qmy = (float)(sRed[4] / valid),
-                    qmz = (float)(sRed[5] / valid);
+        const int valid = (int)sRed[6];
         if (valid < 6) {
             if (tid == 0) {
-                sType = 3;  // NO_CORRESPONDS
-                sStop = 1;
+                st.type = 3;  // NO_CORRESPONDS
+                st.stop = 1;
             }
             __syncthreads();
             break;
         }
-        // ---- 3. normal equations (27 compensated sums: 21 of A, 6 of b)
+        if (tid < 6)
+            st.means[tid] = (float)(sRed[tid] / valid);
+        __syncthreads();
         CF e27[27];
         for (int k = 0; k < 27; k++)
             e27[k] = CF{0.0f, 0.0f};
-        for (int i = tid; i < n; i += BLOCK) {
-            int s = nnSlot[i];
-            if (s < 0)
-                continue;
-            unsigned long long pk = ((unsigned long long)__float_as_uint(nnD[i]) << 32) | (uint32_t)i;
-            if (hbest[hslot[i]] != pk)
-                continue;
-            float x = a.sx[i], y = a.sy[i], z = a.sz[i];
-            float px = sCurF[0] * x + sCurF[1] * y + sCurF[2] * z + sCurF[3];
-            float py = sCurF[4] * x + sCurF[5] * y + sCurF[6] * z + sCurF[7];
-            float pz = sCurF[8] * x + sCurF[9] * y + sCurF[10] * z + sCurF[11];
-            float qx = a.dst.x[s], qy = a.dst.y[s], qz = a.dst.z[s];
-            float nqx = a.dst.nx[s], nqy = a.dst.ny[s], nqz = a.dst.nz[s];
-            float J[6], r;
-            if (a.mode == 0) {
-                float mx = a.snx[i], my = a.sny[i], mz = a.snz[i];
-                float npx = sCurF[0] * mx + sCurF[1] * my + sCurF[2] * mz;
-                float npy = sCurF[4] * mx + sCurF[5] * my + sCurF[6] * mz;
-                float npz = sCurF[8] * mx + sCurF[9] * my + sCurF[10] * mz;
-                float Nx = npx + nqx, Ny = npy + nqy, Nz = npz + nqz;
-                float ux = px - pmx, uy = py - pmy, uz = pz - pmz;
-                float vx = qx - qmx, vy = qy - qmy, vz = qz - qmz;
-                float sx = ux + vx, sy = uy + vy, sz = uz + vz;
-                r    = (ux - vx) * Nx + (uy - vy) * Ny + (uz - vz) * Nz;
-                J[0] = sy * Nz - sz * Ny;
-                J[1] = sz * Nx - sx * Nz;
-                J[2] = sx * Ny - sy * Nx;
-                J[3] = Nx;
-                J[4] = Ny;
-                J[5] = Nz;
-            } else {
-                // point-to-plane about the source centroid: r = n_q.(p - q),
-                // J = [(p - c) x n_q ; n_q]
-                float ux = px - pmx, uy = py - pmy, uz = pz - pmz;
-                r    = (px - qx) * nqx + (py - qy) * nqy + (pz - qz) * nqz;
-                J[0] = uy * nqz - uz * nqy;
-                J[1] = uz * nqx - ux * nqz;
-                J[2] = ux * nqy - uy * nqx;
-                J[3] = nqx;
-                J[4] = nqy;
-                J[5] = nqz;
-            }
-            int k = 0;
-#pragma unroll
-            for (int p = 0; p < 6; p++)
-#pragma unroll
-                for (int q = p; q < 6; q++)
-                    cf_add(e27[k++], J[p] * J[q]);
-#pragma unroll
-            for (int p = 0; p < 6; p++)
-                cf_add(e27[21 + p], r * J[p]);
-        }
+        for (int i = tid; i < n; i += BLOCK)
+            add_eq(a, x, i, st.curF, st.means, a.mode, e27);
         block_reduce_cf<BLOCK, 27>(e27, sLds, sRed);
-        // ---- 4. FP64 solve + update (one lane)
         if (tid == 0) {
-            double A[36], rhs[6], u[6];
-            int    k = 0;
-            for (int p = 0; p < 6; p++)
-                for (int q = p; q < 6; q++) {
-                    A[p * 6 + q] = sRed[k];
-                    A[q * 6 + p] = sRed[k];
-                    k++;
-                }
-            for (int p = 0; p < 6; p++)
-                rhs[p] = -sRed[21 + p];
-            sym_solve6(A, rhs, u);
-            double U[12];
-            if (a.mode == 0) {
-                // symmetric_rigid_matching's update (icp.cpp:70-86)
-                double an    = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
-                double theta = atan(an);
-                double ax[3] = {0, 0, 0};
-                if (an > 0)
-                    for (int i = 0; i < 3; i++)
-                        ax[i] = u[i] / an;
-                double W[9] = {0, -ax[2], ax[1], ax[2], 0, -ax[0], -ax[1], ax[0], 0}, W2[9], Ri[9],
-                       R[9];
-                for (int r = 0; r < 3; r++)
-                    for (int q = 0; q < 3; q++) {
-                        double s2 = 0;
-                        for (int m = 0; m < 3; m++)
-                            s2 += W[r * 3 + m] * W[m * 3 + q];
-                        W2[r * 3 + q] = s2;
-                    }
-                double st = sin(theta), ct = cos(theta);
-                for (int i = 0; i < 9; i++)
-                    Ri[i] = ((i % 4 == 0) ? 1.0 : 0.0) + st * W[i] + (1.0 - ct) * W2[i];
-                for (int r = 0; r < 3; r++)
-                    for (int q = 0; q < 3; q++) {
-                        double s2 = 0;
-                        for (int m = 0; m < 3; m++)
-                            s2 += Ri[r * 3 + m] * Ri[m * 3 + q];
-                        R[r * 3 + q] = s2;
-                    }
-                double pm[3] = {sRed[0] * 0 + pmx, pmy, pmz}, qm[3] = {qmx, qmy, qmz};
-                double t2[3] = {ct * u[3], ct * u[4], ct * u[5]};
-                for (int r = 0; r < 3; r++) {
-                    double v = 0;
-                    for (int m = 0; m < 3; m++)
-                        v += R[r * 3 + m] * (-pm[m]) + Ri[r * 3 + m] * t2[m];
-                    U[r * 4 + 0] = R[r * 3 + 0];
-                    U[r * 4 + 1] = R[r * 3 + 1];
-                    U[r * 4 + 2] = R[r * 3 + 2];
-                    U[r * 4 + 3] = v + qm[r];
-                }
-            } else {
-                double R[9];
-                rodrigues(u, R);
-                double c[3] = {pmx, pmy, pmz};
-                for (int r = 0; r < 3; r++) {
-                    double Rc = R[r * 3 + 0] * c[0] + R[r * 3 + 1] * c[1] + R[r * 3 + 2] * c[2];
-                    U[r * 4 + 0] = R[r * 3 + 0];
-                    U[r * 4 + 1] = R[r * 3 + 1];
-                    U[r * 4 + 2] = R[r * 3 + 2];
-                    U[r * 4 + 3] = c[r] - Rc + u[3 + r];
-                }
-            }
-            for (int i = 0; i < 12; i++) {
-                sU[i]  = U[i];
-                sUF[i] = (float)U[i];
-            }
-            sValid = valid;
+            solve_update(sRed, st.means, a.mode, st);
+            st.valid = valid;
         }
         __syncthreads();
-        // ---- 5. mean NN distance after the update (icp.cpp:182-194)
         CF m1[1] = {CF{0.0f, 0.0f}};
-        for (int i = tid; i < n; i += BLOCK) {
-            int s = nnSlot[i];
-            if (s < 0)
-                continue;
-            unsigned long long pk = ((unsigned long long)__float_as_uint(nnD[i]) << 32) | (uint32_t)i;
-            if (hbest[hslot[i]] != pk)
-                continue;
-            float x = a.sx[i], y = a.sy[i], z = a.sz[i];
-            float px = sCurF[0] * x + sCurF[1] * y + sCurF[2] * z + sCurF[3];
-            float py = sCurF[4] * x + sCurF[5] * y + sCurF[6] * z + sCurF[7];
-            float pz = sCurF[8] * x + sCurF[9] * y + sCurF[10] * z + sCurF[11];
-            float qx = sUF[0] * px + sUF[1] * py + sUF[2] * pz + sUF[3];
-            float qy = sUF[4] * px + sUF[5] * py + sUF[6] * pz + sUF[7];
-            float qz = sUF[8] * px + sUF[9] * py + sUF[10] * pz + sUF[11];
-            float d2;
-            int   did;
-            grid_nn1(a.dst, qx, qy, qz, d2, did);
-            cf_add(m1[0], sqrtf(d2));
-        }
+        for (int i = tid; i < n; i += BLOCK)
+            add_mse(a, x, i, st.curF, st.UF, m1);
         block_reduce_cf<BLOCK, 1>(m1, sLds, sRed);
-        // clear this iteration's hash slots
-        for (int i = tid; i < n; i += BLOCK) {
-            int hs = hslot[i];
-            if (hs >= 0) {
-                hkey[hs]  = 0u;
-                hbest[hs] = ~0ull;
-            }
-        }
-        // ---- 6. convergence (icp.cpp:233-272)
-        if (tid == 0) {
-            float mse = (float)(sRed[0] / (double)sValid);
-            int   vp  = sValid;
-            if (vp > 6 && mse < a.c.mseMax)
-                sConv = 1;
-            bool stop = false;
-            if (mse < a.c.mseMin) {
-                sType = 1;
-                stop  = true;
-            }
-            float rate = (sMse - mse) / sMse;
-            if (rate < a.c.tolerance) {
-                sType = 2;
-                stop  = true;
-            }
-            if (sIters++ >= a.c.iterations) {
-                sType = 0;
-                stop  = true;
-            }
-            if (rate > 0.0f) {
-                sMse  = mse;
-                sRate = rate;
-                compose3x4(sU, sPose, sPose);
-            }
-            if (!stop)
-                compose3x4(sU, sCur, sCur);
-            sStop = stop ? 1 : 0;
-        }
+        for (int i = tid; i < n; i += BLOCK)
+            clear_point(x, i);
+        if (tid == 0)
+            converge(st, (float)(sRed[0] / (double)st.valid), a.c);
         __syncthreads();
     }
-    // ---- scoring: inliner (util.cpp:628-641) of the refined pose
-    __shared__ float sPF[12];
     if (tid < 12)
-        sPF[tid] = (float)sPose[tid];
+        sPF[tid] = (float)st.pose[tid];
     __syncthreads();
     float r2  = a.scoringDist * a.scoringDist;
     int   cnt = 0;
-    for (int i = tid; i < n; i += BLOCK) {
-        float x = a.sx[i], y = a.sy[i], z = a.sz[i];
-        float px = sPF[0] * x + sPF[1] * y + sPF[2] * z + sPF[3];
-        float py = sPF[4] * x + sPF[5] * y + sPF[6] * z + sPF[7];
-        float pz = sPF[8] * x + sPF[9] * y + sPF[10] * z + sPF[11];
-        if (grid_any_within(a.full, px, py, pz, r2))
-            cnt++;
-    }
+    for (int i = tid; i < n; i += BLOCK)
+        cnt += inlier_point(a, i, sPF, r2) ? 1 : 0;
     for (int o = 32; o > 0; o >>= 1)
         cnt += __shfl_xor(cnt, o, 64);
     if ((tid & 63) == 0)
@@ -479,24 +541,130 @@ __global__ void __launch_bounds__(BLOCK) icp_kernel(IcpArgs a) {
         int tot = 0;
         for (int w = 0; w < BLOCK / 64; w++)
             tot += sCnt[w];
-        IcpResultDev &o = a.out[h];
-        for (int i = 0; i < 12; i++)
-            o.pose[i] = sPF[i];
-        o.type         = sType;
-        o.mse          = sMse;
-        o.convergeRate = sRate;
-        o.iterations   = sIters;
-        o.converged    = sConv;
-        o.inliers      = tot;
+        write_result(a.out[h], st, tot);
     }
 }
 
-void icp_batch(Workspace &ws, int nSrc, const float *sx, const float *sy, const float *sz,
-               const float *snx, const float *sny, const float *snz, const GridView &dst,
-               const GridView &full, const float *init12, int H, const IcpCriteria &c,
-               float scoringDist, const IcpConfig &cfg, IcpResultDev *out, hipStream_t s) {
-    if (H <= 0)
+// ---------------------------------------------------------------- plain
+//
+// The conventional GPU refiner: one hypothesis at a time, each step a grid
+// of 256-thread blocks over the source points with per-block partial sums,
+// and a host check of the stop flag after every iteration.
+
+constexpr int PLAIN_BLOCK = 256;
+
+__global__ void __launch_bounds__(PLAIN_BLOCK) plain_init_kernel(IcpArgs a, IcpState *st) {
+    if (threadIdx.x == 0 && blockIdx.x == 0)
+        state_init(*st, a.init);
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < a.T; t += gridDim.x * blockDim.x) {
+        a.hkey[t]  = 0u;
+        a.hbest[t] = ~0ull;
+    }
+}
+
+__global__ void __launch_bounds__(PLAIN_BLOCK) plain_corr_kernel(IcpArgs a, const IcpState *st) {
+    int i = blockIdx.x * PLAIN_BLOCK + threadIdx.x;
+    if (i < a.nSrc)
+        corr_point(a, scratch_of(a, 0), i, st->curF, a.c.rejectDist * a.c.rejectDist);
+}
+
+template <int PASS>  // 0 means, 1 normal equations, 2 mse
+__global__ void __launch_bounds__(PLAIN_BLOCK) plain_sum_kernel(IcpArgs a, const IcpState *st,
+                                                                double *partial) {
+    __shared__ CF     sLds[(PLAIN_BLOCK / 64) * 27];
+    __shared__ double sRed[40];
+    int               i = blockIdx.x * PLAIN_BLOCK + threadIdx.x;
+    HypoScratch       x = scratch_of(a, 0);
+    if (PASS == 0) {
+        CF v[7];
+        for (int k = 0; k < 7; k++)
+            v[k] = CF{0.0f, 0.0f};
+        if (i < a.nSrc)
+            add_means(a, x, i, st->curF, v);
+        block_reduce_cf<PLAIN_BLOCK, 7>(v, sLds, sRed);
+        if (threadIdx.x < 7)
+            partial[blockIdx.x * 27 + threadIdx.x] = sRed[threadIdx.x];
+    } else if (PASS == 1) {
+        CF v[27];
+        for (int k = 0; k < 27; k++)
+            v[k] = CF{0.0f, 0.0f};
+        if (i < a.nSrc)
+            add_eq(a, x, i, st->curF, st->means, a.mode, v);
+        block_reduce_cf<PLAIN_BLOCK, 27>(v, sLds, sRed);
+        if (threadIdx.x < 27)
+            partial[blockIdx.x * 27 + threadIdx.x] = sRed[threadIdx.x];
+    } else {
+        CF v[1] = {CF{0.0f, 0.0f}};
+        if (i < a.nSrc)
+            add_mse(a, x, i, st->curF, st->UF, v);
+        block_reduce_cf<PLAIN_BLOCK, 1>(v, sLds, sRed);
+        if (threadIdx.x == 0)
+            partial[blockIdx.x * 27] = sRed[0];
+    }
+}
+
+// reduce per-block partials in block order, then the serial step of PASS
+template <int PASS>
+__global__ void plain_serial_kernel(IcpArgs a, IcpState *st, const double *partial, int nb) {
+    if (threadIdx.x != 0)
         return;
+    double red[27];
+    int    M = PASS == 0 ? 7 : (PASS == 1 ? 27 : 1);
+    for (int k = 0; k < M; k++) {
+        double s = 0.0;
+        for (int b = 0; b < nb; b++)
+            s += partial[b * 27 + k];
+        red[k] = s;
+    }
+    if (PASS == 0) {
+        int valid = (int)red[6];
+        st->valid = valid;
+        if (valid < 6) {
+            st->type = 3;
+            st->stop = 1;
+            return;
+        }
+        for (int k = 0; k < 6; k++)
+            st->means[k] = (float)(red[k] / valid);
+    } else if (PASS == 1) {
+        solve_update(red, st->means, a.mode, *st);
+    } else {
+        converge(*st, (float)(red[0] / (double)st->valid), a.c);
+    }
+}
+
+__global__ void __launch_bounds__(PLAIN_BLOCK) plain_clear_kernel(IcpArgs a) {
+    int i = blockIdx.x * PLAIN_BLOCK + threadIdx.x;
+    if (i < a.nSrc)
+        clear_point(scratch_of(a, 0), i);
+}
+
+__global__ void __launch_bounds__(PLAIN_BLOCK) plain_inlier_kernel(IcpArgs a, const IcpState *st,
+                                                                   int *count) {
+    int   i = blockIdx.x * PLAIN_BLOCK + threadIdx.x;
+    float PF[12];
+    for (int k = 0; k < 12; k++)
+        PF[k] = (float)st->pose[k];
+    int c = (i < a.nSrc && inlier_point(a, i, PF, a.scoringDist * a.scoringDist)) ? 1 : 0;
+    for (int o = 32; o > 0; o >>= 1)
+        c += __shfl_xor(c, o, 64);
+    if ((threadIdx.x & 63) == 0 && c)
+        atomicAdd(count, c);
+}
+
+__global__ void plain_result_kernel(const IcpState *st, const int *count, IcpResultDev *out) {
+    if (threadIdx.x == 0)
+        write_result(*out, *st, *count);
+}
+
+// ---------------------------------------------------------------- host
+
+static_assert(sizeof(IcpArgs) <= sizeof(GraphCache::args), "graph key too small");
+
+static IcpArgs make_args(Workspace &ws, int nSrc, const float *sx, const float *sy,
+                         const float *sz, const float *snx, const float *sny, const float *snz,
+                         const GridView &dst, const GridView &full, int H, const IcpCriteria &c,
+                         float scoringDist, int mode) {
     if (c.iterations < 1 || c.mseMin < 0 || c.mseMax < c.mseMin || c.tolerance > 1 ||
         c.tolerance < 0 || c.rejectDist < 0)
         throw std::runtime_error("Invalid ConvergenceCriteria at ICP::regist");
@@ -515,34 +683,251 @@ void icp_batch(Workspace &ws, int nSrc, const float *sx, const float *sy, const 
     a.full = full;
     a.c    = c;
     a.scoringDist = scoringDist;
-    a.mode = cfg.mode;
+    a.mode = mode;
     uint32_t T = 64;
     while (T < 2u * (uint32_t)std::max(nSrc, 1))
         T <<= 1;
     a.T      = T;
-    a.hkey   = ws.icpHash.as<uint32_t>((size_t)H * T * 3);
+    a.hkey   = ws.icpHash.as<uint32_t>((size_t)H * T);
     a.hbest  = ws.icpPartial.as<unsigned long long>((size_t)H * T);
     int *ib  = ws.icpState.as<int>((size_t)H * nSrc * 3);
     a.nnSlot = ib;
     a.hslot  = ib + (size_t)H * nSrc;
     a.nnD    = reinterpret_cast<float *>(ib + 2 * (size_t)H * nSrc);
+    a.init   = nullptr;
+    a.out    = nullptr;
+    return a;
+}
+
+static void launch_icp(const IcpArgs &a, int block, hipStream_t s) {
+    switch (block) {
+        case 64: icp_kernel<64><<<a.H, 64, 0, s>>>(a); break;
+        case 128: icp_kernel<128><<<a.H, 128, 0, s>>>(a); break;
+        case 512: icp_kernel<512><<<a.H, 512, 0, s>>>(a); break;
+        case 1024: icp_kernel<1024><<<a.H, 1024, 0, s>>>(a); break;
+        default: icp_kernel<256><<<a.H, 256, 0, s>>>(a); break;
+    }
+}
+
+void icp_batch(Workspace &ws, int nSrc, const float *sx, const float *sy, const float *sz,
+               const float *snx, const float *sny, const float *snz, const GridView &dst,
+               const GridView &full, const float *init12, int H, const IcpCriteria &c,
+               float scoringDist, const IcpConfig &cfg, IcpResultDev *out, hipStream_t s) {
+    if (H <= 0)
+        return;
+    IcpArgs a = make_args(ws, nSrc, sx, sy, sz, snx, sny, snz, dst, full, H, c, scoringDist,
+                          cfg.mode);
     float *init = ws.icpPose.as<float>((size_t)H * 12);
     PPF_HIP_CHECK(hipMemcpyAsync(init, init12, sizeof(float) * 12 * H, hipMemcpyHostToDevice, s));
     a.init = init;
     IcpResultDev *od = ws.icpOut.as<IcpResultDev>(H);
     a.out = od;
-    switch (cfg.block) {
-        case 64: icp_kernel<64><<<H, 64, 0, s>>>(a); break;
-        case 128: icp_kernel<128><<<H, 128, 0, s>>>(a); break;
-        case 512: icp_kernel<512><<<H, 512, 0, s>>>(a); break;
-        case 1024: icp_kernel<1024><<<H, 1024, 0, s>>>(a); break;
-        default: icp_kernel<256><<<H, 256, 0, s>>>(a); break;
+    if (cfg.graph) {
+        // the persistent launch captured once per shape and replayed
+        GraphCache &gc = ws.icpGraph;
+        bool same = gc.exec && gc.H == H && gc.nSrc == nSrc && gc.block == cfg.block &&
+                    std::memcmp(&gc.args, &a, sizeof a) == 0;
+        if (!same) {
+            if (gc.exec)
+                PPF_HIP_CHECK(hipGraphExecDestroy(gc.exec));
+            gc.exec = nullptr;
+            hipGraph_t g;
+            PPF_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            launch_icp(a, cfg.block, s);
+            PPF_HIP_CHECK(hipStreamEndCapture(s, &g));
+            PPF_HIP_CHECK(hipGraphInstantiate(&gc.exec, g, nullptr, nullptr, 0));
+            PPF_HIP_CHECK(hipGraphDestroy(g));
+            gc.H     = H;
+            gc.nSrc  = nSrc;
+            gc.block = cfg.block;
+            std::memcpy(&gc.args, &a, sizeof a);
+        }
+        PPF_HIP_CHECK(hipGraphLaunch(gc.exec, s));
+    } else {
+        launch_icp(a, cfg.block, s);
     }
     PPF_HIP_CHECK(hipGetLastError());
     if (out) {
         PPF_HIP_CHECK(hipMemcpyAsync(out, od, sizeof(IcpResultDev) * H, hipMemcpyDeviceToHost, s));
         PPF_HIP_CHECK(hipStreamSynchronize(s));
     }
+}
+
+void icp_plain(Workspace &ws, int nSrc, const float *sx, const float *sy, const float *sz,
+               const float *snx, const float *sny, const float *snz, const GridView &dst,
+               const GridView &full, const float *init12, int H, const IcpCriteria &c,
+               float scoringDist, int mode, IcpResultDev *out, hipStream_t s) {
+    if (H <= 0)
+        return;
+    IcpArgs a = make_args(ws, nSrc, sx, sy, sz, snx, sny, snz, dst, full, 1, c, scoringDist, mode);
+    int nb = div_up(std::max(nSrc, 1), PLAIN_BLOCK);
+    float *init = ws.icpPose.as<float>((size_t)H * 12);
+    PPF_HIP_CHECK(hipMemcpyAsync(init, init12, sizeof(float) * 12 * H, hipMemcpyHostToDevice, s));
+    IcpState     *st      = reinterpret_cast<IcpState *>(ws.icpCur.as<IcpState>(1));
+    double       *partial = ws.icpInl.as<double>((size_t)nb * 27 + 32);
+    int          *count   = reinterpret_cast<int *>(partial + (size_t)nb * 27);
+    IcpResultDev *od      = ws.icpOut.as<IcpResultDev>(H);
+    for (int h = 0; h < H; h++) {
+        a.init = init + 12 * (size_t)h;
+        plain_init_kernel<<<std::max(1, div_up(a.T, PLAIN_BLOCK)), PLAIN_BLOCK, 0, s>>>(a, st);
+        for (int it = 0; it < c.iterations; it++) {
+            plain_corr_kernel<<<nb, PLAIN_BLOCK, 0, s>>>(a, st);
+            plain_sum_kernel<0><<<nb, PLAIN_BLOCK, 0, s>>>(a, st, partial);
+            plain_serial_kernel<0><<<1, 64, 0, s>>>(a, st, partial, nb);
+            int stop = 0;
+            PPF_HIP_CHECK(hipMemcpyAsync(&stop, &st->stop, sizeof(int), hipMemcpyDeviceToHost, s));
+            PPF_HIP_CHECK(hipStreamSynchronize(s));
+            if (stop)
+                break;
+            plain_sum_kernel<1><<<nb, PLAIN_BLOCK, 0, s>>>(a, st, partial);
+            plain_serial_kernel<1><<<1, 64, 0, s>>>(a, st, partial, nb);
+            plain_sum_kernel<2><<<nb, PLAIN_BLOCK, 0, s>>>(a, st, partial);
+            plain_clear_kernel<<<nb, PLAIN_BLOCK, 0, s>>>(a);
+            plain_serial_kernel<2><<<1, 64, 0, s>>>(a, st, partial, nb);
+            PPF_HIP_CHECK(hipMemcpyAsync(&stop, &st->stop, sizeof(int), hipMemcpyDeviceToHost, s));
+            PPF_HIP_CHECK(hipStreamSynchronize(s));
+            if (stop)
+                break;
+        }
+        PPF_HIP_CHECK(hipMemsetAsync(count, 0, sizeof(int), s));
+        plain_inlier_kernel<<<nb, PLAIN_BLOCK, 0, s>>>(a, st, count);
+        plain_result_kernel<<<1, 64, 0, s>>>(st, count, od + h);
+    }
+    PPF_HIP_CHECK(hipGetLastError());
+    if (out) {
+        PPF_HIP_CHECK(hipMemcpyAsync(out, od, sizeof(IcpResultDev) * H, hipMemcpyDeviceToHost, s));
+        PPF_HIP_CHECK(hipStreamSynchronize(s));
+    }
+}
+
+// ---------------------------------------------------------------- adjoint
+//
+// Point-to-plane cost E = 1/2 sum_i r_i^2, r_i = n_q.(x_i - q_i), over the
+// refiner's correspondences at the pose T = U(xi) T0, where
+// U(xi) = [R(w), c - R(w) c + v] (Rodrigues, xi = (w, v), c = the rotation
+// centre: the origin or the winners' centroid).  Its gradient at xi = 0 is
+// J^T r with J_i = [(x_i - c) x n_q ; n_q] -- the Jacobian rows of the
+// reference's point-to-plane solver (icp.cpp:318-320) -- and J^T J is the
+// Gauss-Newton matrix the refiner solves with (mode 1).
+
+struct AdjArgs {
+    IcpArgs       b;
+    const double *xi;      // H x 6
+    const float  *T0;      // H x 12
+    int           center;  // 0 origin, 1 centroid of the winners
+    double       *out;     // H x 36: cost, count, c[3], grad[6], JtJ[21], pad[4]
+    int          *corr;    // H x nSrc dst id of each winner (-1 = none), optional
+};
+
+template <int BLOCK>
+__global__ void __launch_bounds__(BLOCK) adjoint_kernel(AdjArgs a) {
+    const int h = blockIdx.x;
+    if (h >= a.b.H)
+        return;
+    __shared__ float  sT[12], sC[3];
+    __shared__ double sRed[40];
+    __shared__ CF     sLds[(BLOCK / 64) * 28];
+    const int         tid = threadIdx.x;
+    const int         n   = a.b.nSrc;
+    const HypoScratch x   = scratch_of(a.b, h);
+    if (tid == 0) {
+        // T = U(xi) T0 about the origin; the centred form is applied below
+        double w[3] = {a.xi[h * 6 + 0], a.xi[h * 6 + 1], a.xi[h * 6 + 2]}, R[9];
+        rodrigues(w, R);
+        double U[12], T0[12], T[12];
+        for (int i = 0; i < 12; i++)
+            T0[i] = a.T0[(size_t)h * 12 + i];
+        for (int r = 0; r < 3; r++) {
+            for (int c = 0; c < 3; c++)
+                U[r * 4 + c] = R[r * 3 + c];
+            U[r * 4 + 3] = a.xi[h * 6 + 3 + r];
+        }
+        compose3x4(U, T0, T);
+        for (int i = 0; i < 12; i++)
+            sT[i] = (float)T[i];
+    }
+    for (uint32_t t = tid; t < a.b.T; t += BLOCK) {
+        x.hkey[t]  = 0u;
+        x.hbest[t] = ~0ull;
+    }
+    __syncthreads();
+    const float rej2 = a.b.c.rejectDist * a.b.c.rejectDist;
+    for (int i = tid; i < n; i += BLOCK)
+        corr_point(a.b, x, i, sT, rej2);
+    __syncthreads();
+    CF m7[7];
+    for (int k = 0; k < 7; k++)
+        m7[k] = CF{0.0f, 0.0f};
+    for (int i = tid; i < n; i += BLOCK)
+        add_means(a.b, x, i, sT, m7);
+    block_reduce_cf<BLOCK, 7>(m7, sLds, sRed);
+    const double cnt = sRed[6];
+    if (tid < 3)
+        sC[tid] = (a.center && cnt > 0) ? (float)(sRed[tid] / cnt) : 0.0f;
+    __syncthreads();
+    float means[6] = {sC[0], sC[1], sC[2], 0.0f, 0.0f, 0.0f};
+    CF    e28[28];
+    for (int k = 0; k < 28; k++)
+        e28[k] = CF{0.0f, 0.0f};
+    for (int i = tid; i < n; i += BLOCK) {
+        bool win = is_winner(x, i);
+        if (a.corr)
+            a.corr[(size_t)h * n + i] = win ? a.b.dst.id[x.nnSlot[i]] : -1;
+        if (!win)
+            continue;
+        float J[6], r;
+        eq_row(a.b, i, x.nnSlot[i], sT, means, 1, J, r);
+        cf_add(e28[0], 0.5f * r * r);
+        for (int p = 0; p < 6; p++)
+            cf_add(e28[1 + p], r * J[p]);
+        int k = 7;
+        for (int p = 0; p < 6; p++)
+            for (int q = p; q < 6; q++)
+                cf_add(e28[k++], J[p] * J[q]);
+    }
+    block_reduce_cf<BLOCK, 28>(e28, sLds, sRed);
+    if (tid == 0) {
+        double *o = a.out + (size_t)h * 36;
+        o[0] = sRed[0];
+        o[1] = cnt;
+        o[2] = sC[0];
+        o[3] = sC[1];
+        o[4] = sC[2];
+        for (int k = 0; k < 27; k++)
+            o[5 + k] = sRed[1 + k];
+        for (int k = 32; k < 36; k++)
+            o[k] = 0.0;
+    }
+    for (int i = tid; i < n; i += BLOCK)
+        clear_point(x, i);
+}
+
+void icp_adjoint(Workspace &ws, int nSrc, const float *sx, const float *sy, const float *sz,
+                 const GridView &dst, const float *T0host, const double *xiHost, int H,
+                 float rejectDist, int center, double *outHost, int *corrHost, hipStream_t s) {
+    if (H <= 0)
+        return;
+    IcpCriteria c{1, rejectDist, 0.0f, 0.0f, 0.0f};
+    AdjArgs     a;
+    a.b = make_args(ws, nSrc, sx, sy, sz, sx, sy, sz, dst, dst, H, c, 0.0f, 1);
+    float  *T0 = ws.icpPose.as<float>((size_t)H * 12);
+    double *xi = ws.icpCur.as<double>((size_t)H * 6);
+    double *od = ws.icpInl.as<double>((size_t)H * 36);
+    int    *cd = corrHost ? ws.icpOut.as<int>((size_t)H * nSrc) : nullptr;
+    PPF_HIP_CHECK(hipMemcpyAsync(T0, T0host, sizeof(float) * 12 * H, hipMemcpyHostToDevice, s));
+    PPF_HIP_CHECK(hipMemcpyAsync(xi, xiHost, sizeof(double) * 6 * H, hipMemcpyHostToDevice, s));
+    a.xi     = xi;
+    a.T0     = T0;
+    a.center = center;
+    a.out    = od;
+    a.corr   = cd;
+    adjoint_kernel<256><<<H, 256, 0, s>>>(a);
+    PPF_HIP_CHECK(hipGetLastError());
+    PPF_HIP_CHECK(hipMemcpyAsync(outHost, od, sizeof(double) * 36 * H, hipMemcpyDeviceToHost, s));
+    if (corrHost)
+        PPF_HIP_CHECK(hipMemcpyAsync(corrHost, cd, sizeof(int) * (size_t)H * nSrc,
+                                     hipMemcpyDeviceToHost, s));
+    PPF_HIP_CHECK(hipStreamSynchronize(s));
 }
 
 }  // namespace ppfd

@@ -6,6 +6,17 @@
 
 namespace ppfd {
 
+// A captured launch and the argument bytes it was captured with.
+struct GraphCache {
+    hipGraphExec_t exec = nullptr;
+    int            H = 0, nSrc = 0, block = 0;
+    unsigned char  args[1024];
+    ~GraphCache() {
+        if (exec)
+            (void)hipGraphExecDestroy(exec);
+    }
+};
+
 // Reusable device scratch of one detector (grown on demand, never shrunk).
 struct Workspace {
     DevBuf voxKey, voxBest, voxFlag, voxIota, cubTmp, counter;
@@ -20,6 +31,7 @@ struct Workspace {
         clCenters, clNms, clMembers;
     // icp
     DevBuf icpPose, icpCur, icpState, icpHash, icpPartial, icpOut, icpInl;
+    GraphCache icpGraph;
     // events bracketing the vote kernel (created on first use)
     hipEvent_t voteEv[2] = {nullptr, nullptr};
     ~Workspace() {
@@ -143,6 +155,7 @@ struct IcpResultDev {
 struct IcpConfig {
     int block = 256;
     int mode  = 0;   // 0 symmetric (reference), 1 point-to-plane
+    int graph = 1;   // replay the persistent launch from a captured hipGraph
 };
 // Refine H hypotheses (poses12 host) of src against dst in one persistent
 // launch; scores by inliers against `full` within scoringDist.
@@ -150,5 +163,19 @@ void icp_batch(Workspace &ws, int nSrc, const float *sx, const float *sy, const 
                const float *snx, const float *sny, const float *snz, const GridView &dst,
                const GridView &full, const float *init12, int H, const IcpCriteria &c,
                float scoringDist, const IcpConfig &cfg, IcpResultDev *out, hipStream_t s);
+// The conventional per-hypothesis GPU refiner (grid over source points per
+// step, host stop check per iteration), kept for comparison with icp_batch.
+void icp_plain(Workspace &ws, int nSrc, const float *sx, const float *sy, const float *sz,
+               const float *snx, const float *sny, const float *snz, const GridView &dst,
+               const GridView &full, const float *init12, int H, const IcpCriteria &c,
+               float scoringDist, int mode, IcpResultDev *out, hipStream_t s);
+
+// Point-to-plane cost, se(3) gradient and Gauss-Newton J^T J of H poses
+// (T = U(xi) T0) against dst, with the refiner's correspondence rule.  out:
+// H x 36 doubles {cost, count, centre[3], grad[6], JtJ[21], 0 x4}; corr
+// (optional): H x nSrc dst ids of the winners (-1 = none).
+void icp_adjoint(Workspace &ws, int nSrc, const float *sx, const float *sy, const float *sz,
+                 const GridView &dst, const float *T0host, const double *xiHost, int H,
+                 float rejectDist, int center, double *outHost, int *corrHost, hipStream_t s);
 
 }  // namespace ppfd

@@ -172,29 +172,12 @@ __global__ void __launch_bounds__(BLOCK) vote_kernel(VoteArgs a) {
         return sCellStart[lo] + (c - sCellPre[lo]);
     };
 
-    // ---- neighbour count (radiusSearch incl. self, ppf.cpp:273-276)
-    int cnt = 0;
-    for (int c = tid; c < total; c += BLOCK) {
-        int s = slot_of(c);
-        if (ppfc::dist2(qx, qy, qz, g.x[s], g.y[s], g.z[s]) < a.D2)
-            cnt++;
-    }
-    for (int o = 32; o > 0; o >>= 1)
-        cnt += __shfl_xor(cnt, o, 64);
-    if ((tid & 63) == 0)
-        sRed[tid >> 6] = cnt;
-    __syncthreads();
-    int searched = 0;
-    for (int w = 0; w < BLOCK / 64; w++)
-        searched += sRed[w];
-    if ((float)searched < a.voteThreshold) {
-        if (tid < 3)
-            candOut[tid] = 0ull;
-        return;
-    }
-
-    // ---- pairs + votes, one chunk of BLOCK candidates at a time
+    // ---- pairs + votes, one chunk of BLOCK candidates at a time.  The
+    // neighbour count of the reference's skip rule (radiusSearch incl. self
+    // < voteThreshold, ppf.cpp:273-276) is taken in the same pass; a key
+    // point that fails it has its accumulator discarded below.
     unsigned long long nPairs = 0, nHits = 0, nVotes = 0;
+    int inSphere = 0;
     for (int base = 0; base < total; base += BLOCK) {
         int      c   = base + tid;
         int      hit = 0;
@@ -203,7 +186,9 @@ __global__ void __launch_bounds__(BLOCK) vote_kernel(VoteArgs a) {
         if (c < total) {
             int   s  = slot_of(c);
             float px = g.x[s], py = g.y[s], pz = g.z[s];
-            if (ppfc::dist2(qx, qy, qz, px, py, pz) < a.D2 && g.id[s] != kpId) {
+            bool  in = ppfc::dist2(qx, qy, qz, px, py, pz) < a.D2;
+            inSphere += in ? 1 : 0;
+            if (in && g.id[s] != kpId) {
                 nPairs++;
                 alphaS = ppfc::alpha_of(rt, px, py, pz);
                 uint32_t key = ppfc::ppf_key(qx, qy, qz, nx1, ny1, nz1, px, py, pz, g.nx[s],
@@ -249,9 +234,24 @@ __global__ void __launch_bounds__(BLOCK) vote_kernel(VoteArgs a) {
         __syncthreads();
     }
 
+    for (int o = 32; o > 0; o >>= 1)
+        inSphere += __shfl_xor(inSphere, o, 64);
+    if ((tid & 63) == 0)
+        sRed[tid >> 6] = inSphere;
+    __syncthreads();
+    int searched = 0;
+    for (int w = 0; w < BLOCK / 64; w++)
+        searched += sRed[w];
+    const bool voted = !((float)searched < a.voteThreshold);
+
     if (k < a.dumpCount && a.accDump) {
         for (int i = tid; i < nrTile * a.angleNum; i += BLOCK)
             a.accDump[((size_t)k * a.refNum + r0) * a.angleNum + i] = (int32_t)acc[i];
+    }
+    if (!voted) {
+        if (tid < 3)
+            candOut[tid] = 0ull;
+        return;
     }
     if (a.stats && tile == 0) {
         for (int o = 32; o > 0; o >>= 1) {

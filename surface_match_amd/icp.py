@@ -1,0 +1,104 @@
+"""Batched ICP refinement and the point-to-plane adjoint on the GPU.
+
+``IcpTarget`` mirrors ``ICP::regist(src, dst, kdtree, initPoses)``
+(icp.h:86-90, icp.cpp:214-278) with the target gridded once.  ``refine``
+runs either the batched persistent launch (all hypotheses in one kernel,
+replayed from a captured hipGraph) or the conventional per-hypothesis
+refiner.  ``adjoint`` returns the point-to-plane cost, its se(3) gradient
+and the Gauss-Newton J^T J for many poses at once.
+
+``PointToPlaneCost`` (needs torch) wraps ``adjoint`` as a
+torch.autograd.Function of the pose twist, evaluated on host tensors.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _native as N
+
+
+def _pose16(T):
+    T = np.asarray(T, np.float32)
+    if T.shape[-2:] == (3, 4):
+        T4 = np.zeros(T.shape[:-2] + (4, 4), np.float32)
+        T4[..., :3, :] = T
+        T4[..., 3, 3] = 1.0
+        T = T4
+    # row-major 4x4 -> column-major 16 floats
+    return np.ascontiguousarray(np.swapaxes(T, -1, -2).reshape(-1, 16), np.float32)
+
+
+class IcpTarget:
+    def __init__(self, pts, nrm, cell: float = 0.0):
+        self._pts = N.f32(pts)
+        self._nrm = N.f32(nrm)
+        h = C.c_void_p()
+        N.check(N.lib().ppf_icp_target_create(N.ptr(self._pts), N.ptr(self._nrm), len(self._pts),
+                                              cell, C.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            N.lib().ppf_icp_target_destroy(h)
+            self._h = None
+
+    def refine(self, src, src_nrm, init_poses, iterations=10, reject_dist=10.0, mse_min=0.0,
+               mse_max=1e30, tolerance=0.0, mode=1, batched=True, block=0, scoring_dist=1.0):
+        """Returns (poses [H,4,4], results dict of arrays, device ms)."""
+        src, src_nrm = N.f32(src), N.f32(src_nrm)
+        P = _pose16(init_poses)
+        H = len(P)
+        crit = N.IcpCriteria(iterations, reject_dist, mse_min, mse_max, tolerance)
+        out = (N.IcpResult * H)()
+        ms = C.c_float(0)
+        N.check(N.lib().ppf_icp_refine(self._h, N.ptr(src), N.ptr(src_nrm), len(src), N.ptr(P), H,
+                                       C.byref(crit), mode, int(batched), block, scoring_dist,
+                                       out, C.byref(ms)))
+        poses = np.array([np.array(o.pose16, np.float32).reshape(4, 4).T for o in out])
+        res = {k: np.array([getattr(o, k) for o in out]) for k in
+               ("type", "mse", "convergeRate", "iterations", "converged", "inliers")}
+        return poses, res, ms.value
+
+    def adjoint(self, src, poses, xi=None, reject_dist=10.0, center=1, with_corr=False):
+        """Per pose: dict(cost, count, center, grad[6], JtJ[6x6]) at T = U(xi) T0."""
+        src = N.f32(src)
+        P = _pose16(poses)
+        H = len(P)
+        xi = np.zeros((H, 6)) if xi is None else np.ascontiguousarray(xi, np.float64).reshape(H, 6)
+        out = np.zeros((H, 36))
+        corr = np.zeros((H, len(src)), np.int32) if with_corr else None
+        N.check(N.lib().ppf_icp_adjoint(self._h, N.ptr(src), len(src), N.ptr(P), N.ptr(xi), H,
+                                        reject_dist, center, N.ptr(out), N.ptr(corr)))
+        JtJ = np.zeros((H, 6, 6))
+        iu = np.triu_indices(6)
+        JtJ[:, iu[0], iu[1]] = out[:, 11:32]
+        JtJ[:, iu[1], iu[0]] = out[:, 11:32]
+        r = dict(cost=out[:, 0], count=out[:, 1], center=out[:, 2:5], grad=out[:, 5:11], JtJ=JtJ)
+        if with_corr:
+            r["corr"] = corr
+        return r
+
+
+def point_to_plane_cost():
+    """torch.autograd.Function: cost(xi) for H poses, gradient = J^T r.
+
+    Gradients are exact at xi = 0 (the Gauss-Newton increment about the
+    current pose), which is how the refiner uses them."""
+    import torch
+
+    class PointToPlaneCost(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, xi, target: IcpTarget, src, poses, reject_dist=10.0, center=1):
+            r = target.adjoint(src, poses, xi.detach().cpu().numpy(), reject_dist, center)
+            ctx.save_for_backward(torch.from_numpy(r["grad"]).to(xi.dtype))
+            return torch.from_numpy(r["cost"]).to(xi.dtype)
+
+        @staticmethod
+        def backward(ctx, g):
+            (grad,) = ctx.saved_tensors
+            return g[:, None] * grad, None, None, None, None, None
+
+    return PointToPlaneCost

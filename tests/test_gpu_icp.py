@@ -1,0 +1,147 @@
+"""GPU ICP: batched vs plain refinement vs the FP64 oracle refinement, and
+the point-to-plane adjoint vs finite differences and icp.cpp's Jacobian."""
+import ctypes as C
+import math
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import rng
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _perturb(T, r, deg=2.0, mm=2.0):
+    w = np.radians(r.normal(scale=deg, size=3))
+    th = np.linalg.norm(w)
+    k = w / th
+    K = np.array([[0, -k[2], k[1]], [k[2], 0, -k[0]], [-k[1], k[0], 0]])
+    R = np.eye(3) + math.sin(th) * K + (1 - math.cos(th)) * K @ K
+    P = np.eye(4)
+    P[:3, :3] = R
+    P[:3, 3] = r.normal(scale=mm, size=3)
+    return P @ T
+
+
+@pytest.fixture(scope="module")
+def icp_case():
+    from surface_match_amd import synth
+    r = rng(70)
+    src, srcn = synth.make_model(2500, 71)
+    # target: two rigid copies sampled independently, analytic normals
+    tp, tn, gts = [], [], []
+    for k in range(2):
+        p, n = synth.make_model(30000, 72 + k)
+        R = synth.random_rotation(r)
+        t = r.uniform(-150, 150, 3)
+        tp.append(p @ R.T + t)
+        tn.append(n @ R.T)
+        T = np.eye(4)
+        T[:3, :3] = R
+        T[:3, 3] = t
+        gts.append(T)
+    dst = np.concatenate(tp).astype(np.float32)
+    dstn = np.concatenate(tn).astype(np.float32)
+    dst += r.normal(scale=0.05, size=dst.shape).astype(np.float32)
+    inits = np.array([_perturb(gts[h % 2], r) for h in range(16)])
+    return src, srcn, dst, dstn, gts, inits
+
+
+def test_batched_icp_matches_fp64_oracle(icp_case, oracle_mod):
+    from surface_match_amd import synth
+    from surface_match_amd.icp import IcpTarget
+    O = oracle_mod
+    src, srcn, dst, dstn, gts, inits = icp_case
+    tgt = IcpTarget(dst, dstn, 2.0)
+    D = 147.3
+    g = O.OracleGrid(dst, 2.0)
+    for mode in (0, 1):
+        poses, res, ms = tgt.refine(src, srcn, inits, iterations=10, reject_dist=15.0,
+                                    mse_min=0.0, mse_max=10.0, tolerance=0.0, mode=mode,
+                                    batched=True, scoring_dist=1.0)
+        plain, pres, pms = tgt.refine(src, srcn, inits, iterations=10, reject_dist=15.0,
+                                      mse_min=0.0, mse_max=10.0, tolerance=0.0, mode=mode,
+                                      batched=False, scoring_dist=1.0)
+        for h in range(len(inits)):
+            crit = O.Criteria(10, 15.0, 0.0, 10.0, 0.0)
+            ores = O.IcpResult()
+            O.lib().orc_icp_regist(len(src), src, srcn, C.byref(g._g), dstn,
+                                   np.ascontiguousarray(inits[h][:3].ravel(), np.float32),
+                                   C.byref(crit), mode, C.byref(ores))
+            OP = np.array(ores.pose, np.float64).reshape(3, 4)
+            for P in (poses[h], plain[h]):
+                ang = synth.rotation_angle_deg(OP[:, :3].T @ P[:3, :3])
+                dt = np.linalg.norm(OP[:, 3] - P[:3, 3])
+                assert ang < 0.1 and dt < 0.001 * D, (mode, h, ang, dt)
+            # converged to the ground truth too
+            k, ang, dt = synth.pose_error(poses[h], gts, D)
+            assert ang < 0.2 and dt < 0.002 * D
+        print(f"mode {mode}: batched {ms:.3f} ms, plain {pms:.3f} ms for {len(inits)} poses")
+
+
+def test_adjoint_vs_fd_and_icp_jacobian(icp_case, oracle_mod):
+    from surface_match_amd.icp import IcpTarget
+    O = oracle_mod
+    src, srcn, dst, dstn, gts, inits = icp_case
+    tgt = IcpTarget(dst, dstn, 2.0)
+    for center in (1, 0):
+        r = tgt.adjoint(src, inits[:4], reject_dist=15.0, center=center, with_corr=True)
+        for h in range(4):
+            T = inits[h][:3].ravel()
+            c = r["center"][h]
+            base = O.p2pl_system(src, T, r["corr"][h], dst, dstn, c, np.zeros(6))
+            # cost, gradient and J^T J against the FP64 restatement of icp.cpp:318-324
+            assert abs(r["cost"][h] - base[0]) <= 1e-4 * base[0] + 1e-6
+            assert np.allclose(r["grad"][h], base[1:7], rtol=1e-4, atol=1e-3 * np.abs(base[1:7]).max())
+            iu = np.triu_indices(6)
+            assert np.allclose(r["JtJ"][h][iu], base[7:], rtol=1e-4,
+                               atol=1e-5 * np.abs(base[7:]).max())
+            # gradient against central finite differences of the FP64 cost (h = 1e-4)
+            fd = np.zeros(6)
+            for k in range(6):
+                e = np.zeros(6)
+                e[k] = 1e-4
+                fp = O.p2pl_system(src, T, r["corr"][h], dst, dstn, c, e)[0]
+                fm = O.p2pl_system(src, T, r["corr"][h], dst, dstn, c, -e)[0]
+                fd[k] = (fp - fm) / 2e-4
+            rel = np.linalg.norm(r["grad"][h] - fd) / np.linalg.norm(fd)
+            assert rel < 1e-3, rel
+            print(f"center {center} pose {h}: grad rel err vs FD {rel:.2e}")
+
+
+def test_torch_autograd_binding(tmp_path):
+    """The optional torch binding, in its own process (torch brings its own
+    HIP runtime; it only ever sees host tensors here)."""
+    try:
+        import importlib.util
+        if importlib.util.find_spec("torch") is None:
+            pytest.skip("torch not installed")
+    except Exception:
+        pytest.skip("torch not importable")
+    script = tmp_path / "t.py"
+    script.write_text(f"""
+import sys, numpy as np
+sys.path.insert(0, {ROOT!r})
+import torch
+from surface_match_amd import synth
+from surface_match_amd.icp import IcpTarget, point_to_plane_cost
+src, _ = synth.make_model(1500, 80)
+dst, dn = synth.make_model(20000, 81)
+T0 = np.eye(4); T0[:3, 3] = [0.5, -0.3, 0.2]
+tgt = IcpTarget(dst, dn, 2.0)
+F = point_to_plane_cost()
+xi = torch.zeros(1, 6, dtype=torch.float64, requires_grad=True)
+cost = F.apply(xi, tgt, src, T0[None], 10.0, 1)
+cost.sum().backward()
+r = tgt.adjoint(src, T0[None], reject_dist=10.0, center=1)
+assert np.allclose(xi.grad.numpy(), r["grad"]), (xi.grad, r["grad"])
+assert abs(cost.item() - r["cost"][0]) < 1e-9
+print("torch binding ok", cost.item(), xi.grad.numpy())
+""")
+    out = subprocess.run([sys.executable, str(script)], capture_output=True, text=True, timeout=600)
+    assert out.returncode == 0, out.stdout[-3000:] + out.stderr[-3000:]
+    print(out.stdout)
