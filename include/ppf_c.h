@@ -131,6 +131,20 @@ PPF_API int ppf_match_scene_resident(ppf_detector d, ppf_scene s, float sampling
                                      const ppf_match_param *param, float *poses16, float *scores,
                                      size_t cap, size_t *nOut, ppf_match_stats *stats);
 
+/* -- autotuning (no reference counterpart): times the vote kernel over
+ * workgroup sizes {256, 512, 1024} x XCD-aware key-point mapping {off, on}
+ * and the batched ICP over {64, 128, 256, 512, 1024} threads per hypothesis
+ * on the given resident scene, and keeps the fastest in the detector. -- */
+typedef struct {
+    int   voteBlock, voteXcdRemap, icpBlock;
+    float voteMs, icpMs;
+    int   configsTried;
+} ppf_tune_report;
+PPF_API int ppf_autotune(ppf_detector d, ppf_scene s, float samplingDistanceRel,
+                         float keyPointFraction, float minScore, const ppf_match_param *param,
+                         ppf_tune_report *report);
+PPF_API int ppf_set_tuning(ppf_detector d, int voteBlock, int voteXcdRemap, int icpBlock);
+
 /* -- trained-model introspection (the IMPL fields, privateType.h:56-66) -- */
 typedef struct {
     int   nSampled, nReSampled, nKeys, nEntries, angleNum;

@@ -219,6 +219,14 @@ class ResidentScene:
         k = nout.value
         return poses[:k].reshape(k, 4, 4).transpose(0, 2, 1).copy(), scores[:k].copy(), st.as_dict()
 
+    def autotune(self, samplingDistanceRel=0.04, keyPointFraction=0.2, minScore=0.2, param=None):
+        """Pick the vote/ICP launch shapes that run fastest on this scene."""
+        param = param or MatchParam()
+        rep = N.TuneReport()
+        N.check(N.lib().ppf_autotune(self._det._h, self._h, samplingDistanceRel, keyPointFraction,
+                                     minScore, C.byref(param), C.byref(rep)))
+        return {k: getattr(rep, k) for k, _ in rep._fields_}
+
     def __del__(self):
         h = getattr(self, "_h", None)
         if h:

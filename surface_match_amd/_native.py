@@ -69,6 +69,11 @@ class IcpResult(C.Structure):
                 ("inliers", C.c_int)]
 
 
+class TuneReport(C.Structure):
+    _fields_ = [("voteBlock", C.c_int), ("voteXcdRemap", C.c_int), ("icpBlock", C.c_int),
+                ("voteMs", C.c_float), ("icpMs", C.c_float), ("configsTried", C.c_int)]
+
+
 _lib = None
 
 # every function declared in include/ppf_c.h
@@ -80,6 +85,7 @@ EXPORTS = [
     "ppf_model_get_table", "ppf_pair_features", "ppf_vote", "ppf_voxel_sample",
     "ppf_radius_search", "ppf_knn_search", "ppf_estimate_normals", "ppf_device_math",
     "ppf_icp_target_create", "ppf_icp_target_destroy", "ppf_icp_refine", "ppf_icp_adjoint",
+    "ppf_autotune", "ppf_set_tuning",
 ]
 
 
@@ -123,6 +129,8 @@ def lib() -> C.CDLL:
         "ppf_icp_refine": (i, [vp, vp, vp, sz, vp, sz, C.POINTER(IcpCriteria), i, i, i, f,
                                vp, C.POINTER(f)]),
         "ppf_icp_adjoint": (i, [vp, vp, sz, vp, vp, sz, f, i, vp, vp]),
+        "ppf_autotune": (i, [vp, vp, f, f, f, C.POINTER(MatchParam), C.POINTER(TuneReport)]),
+        "ppf_set_tuning": (i, [vp, i, i, i]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

@@ -545,7 +545,6 @@ void match_impl(ppf_detector_s &D, DevCloud &c, float rel, float kpFrac, float m
             for (size_t k = 0; k < survivors.size(); k++)
                 std::memcpy(&init2[12 * k], &poseNow[12 * (size_t)survivors[k]], sizeof(float) * 12);
             IcpConfig dcfg = D.icfg;
-            dcfg.block     = std::max(dcfg.block, 256);
             icp_batch(D.ws, hm.reSampled.size(), dm.rx.get<float>(), dm.ry.get<float>(),
                       dm.rz.get<float>(), dm.rnx.get<float>(), dm.rny.get<float>(),
                       dm.rnz.get<float>(), D.ws.gDense.view, D.ws.gFull.view, init2.data(),
@@ -1001,6 +1000,73 @@ PPF_API int ppf_match_scene_resident(ppf_detector d, ppf_scene sc, float rel, fl
         MatchOut o;
         match_impl(*d, sc->cloud, rel, kpFrac, minScore, p, o, nullptr, stats);
         write_poses(o, poses16, scores, cap, nOut);
+    });
+}
+
+PPF_API int ppf_set_tuning(ppf_detector d, int voteBlock, int voteXcdRemap, int icpBlock) {
+    PPF_REQUIRE(d, "null detector");
+    PPF_REQUIRE(voteBlock == 256 || voteBlock == 512 || voteBlock == 1024, "vote block must be 256, 512 or 1024");
+    PPF_REQUIRE(icpBlock == 64 || icpBlock == 128 || icpBlock == 256 || icpBlock == 512 || icpBlock == 1024,
+                "icp block must be 64..1024, a power of two");
+    d->vcfg.block    = voteBlock;
+    d->vcfg.xcdRemap = voteXcdRemap ? 1 : 0;
+    d->icfg.block    = icpBlock;
+    return PPF_OK;
+}
+
+PPF_API int ppf_autotune(ppf_detector d, ppf_scene sc, float rel, float kpFrac, float minScore,
+                         const ppf_match_param *param, ppf_tune_report *rep) {
+    PPF_REQUIRE(d && sc && rep, "null argument");
+    return guarded([&] {
+        ppf_match_param p;
+        if (param)
+            p = *param;
+        else
+            ppf_match_param_default(&p);
+        VoteConfig bestV = d->vcfg;
+        IcpConfig  bestI = d->icfg;
+        float      bestVoteMs = 1e30f, bestIcpMs = 1e30f;
+        int        tried = 0;
+        auto run = [&](float &voteMs, float &icpMs) {
+            voteMs = icpMs = 1e30f;
+            for (int rep2 = 0; rep2 < 2; rep2++) {
+                MatchOut        o;
+                ppf_match_stats st{};
+                match_impl(*d, sc->cloud, rel, kpFrac, minScore, p, o, nullptr, &st);
+                voteMs = std::min(voteMs, st.msVoteKernel);
+                icpMs  = std::min(icpMs, st.msIcp);
+            }
+        };
+        for (int block : {256, 512, 1024})
+            for (int remap : {0, 1}) {
+                d->vcfg.block    = block;
+                d->vcfg.xcdRemap = remap;
+                float v, i;
+                run(v, i);
+                tried++;
+                if (v < bestVoteMs) {
+                    bestVoteMs = v;
+                    bestV      = d->vcfg;
+                }
+            }
+        d->vcfg = bestV;
+        for (int block : {64, 128, 256, 512, 1024}) {
+            d->icfg.block = block;
+            float v, i;
+            run(v, i);
+            tried++;
+            if (i < bestIcpMs) {
+                bestIcpMs = i;
+                bestI     = d->icfg;
+            }
+        }
+        d->icfg               = bestI;
+        rep->voteBlock        = bestV.block;
+        rep->voteXcdRemap     = bestV.xcdRemap;
+        rep->icpBlock         = bestI.block;
+        rep->voteMs           = bestVoteMs;
+        rep->icpMs            = bestIcpMs;
+        rep->configsTried     = tried;
     });
 }
 

@@ -246,9 +246,12 @@ __device__ void grid_knn(const GridView &g, float qx, float qy, float qz, int k,
     }
 }
 
-// nearest neighbour by (dist2, id); returns the sorted slot (-1 if empty).
+// nearest neighbour by (dist2, id) among points with dist2 <= maxD2;
+// returns the sorted slot (-1 if none).  With maxD2 = FLT_MAX it is the
+// unbounded 1-NN (findClosestPoint, util.cpp:603-626); a finite bound keeps
+// far queries from walking empty rings.
 __device__ inline int grid_nn1(const GridView &g, float qx, float qy, float qz, float &bestD,
-                               int &bestId) {
+                               int &bestId, float maxD2 = FLT_MAX) {
     bestD      = FLT_MAX;
     bestId     = 0x7fffffff;
     int bestS  = -1;
@@ -259,10 +262,12 @@ __device__ inline int grid_nn1(const GridView &g, float qx, float qy, float qz, 
     int c0z  = cell_coord_unclamped(qz, g.oz, g.cell);
     int maxR = max(max(g.dx, g.dy), g.dz) + max(max(abs(c0x), abs(c0y)), abs(c0z)) + 1;
     for (int r = first_ring(g, c0x, c0y, c0z); r <= maxR; r++) {
-        visit_ring(g, c0x, c0y, c0z, r, qx, qy, qz, bestD, [&](int s, int c) {
+        visit_ring(g, c0x, c0y, c0z, r, qx, qy, qz, fminf(bestD, maxD2), [&](int s, int c) {
             for (int t = s; t < s + c; t++) {
                 float dd = ppfc::dist2(qx, qy, qz, g.x[t], g.y[t], g.z[t]);
                 int   ii = g.id[t];
+                if (dd > maxD2)
+                    continue;
                 if (dd < bestD || (dd == bestD && ii < bestId)) {
                     bestD  = dd;
                     bestId = ii;
@@ -274,6 +279,8 @@ __device__ inline int grid_nn1(const GridView &g, float qx, float qy, float qz, 
         if (safe == FLT_MAX)
             break;
         if (bestS >= 0 && bestD < safe * safe)
+            break;
+        if (maxD2 < safe * safe)  // nothing unvisited can be within the bound
             break;
     }
     return bestS;

@@ -223,7 +223,8 @@ __device__ void corr_point(const IcpArgs &a, const HypoScratch &x, int i, const 
     xform(curF, a.sx[i], a.sy[i], a.sz[i], px, py, pz);
     float d2;
     int   did;
-    int   s  = grid_nn1(a.dst, px, py, pz, d2, did);
+    // the pair is rejected beyond rejectDist, so the search stops there
+    int   s  = grid_nn1(a.dst, px, py, pz, d2, did, rej2);
     int   hs = -1;
     if (s >= 0 && !(d2 > rej2)) {
         uint32_t key = (uint32_t)did + 1u;

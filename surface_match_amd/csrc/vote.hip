@@ -23,6 +23,7 @@
 namespace ppfd {
 
 constexpr int VOTE_MAX_CELLS = 512;
+constexpr int SEG            = 8;  // bucket entries voted by one lane
 constexpr int VOTE_STATIC_LDS = 24 * 1024;   // bound on the kernel's static LDS
 constexpr int LDS_LIMIT       = 160 * 1024;  // gfx950 LDS per workgroup
 
@@ -197,38 +198,53 @@ __global__ void __launch_bounds__(BLOCK) vote_kernel(VoteArgs a) {
                     hit = 1;
             }
         }
-        int nHit, totalE;
+        // Each hit's bucket is cut into segments of SEG entries; a lane
+        // votes one whole segment.  The reference's tail rule covers the
+        // last len % 8 entries, which start at a multiple of 8, so a segment
+        // is either entirely body or entirely tail.
+        int nHit, totalSeg;
+        int nSeg = hit ? (int)((bLen + SEG - 1) / SEG) : 0;
         int pos  = block_excl_scan<BLOCK>(hit, sScan, nHit);
-        int lpre = block_excl_scan<BLOCK>(hit ? (int)bLen : 0, sScan, totalE);
+        int spre = block_excl_scan<BLOCK>(nSeg, sScan, totalSeg);
         if (hit) {
             sPStart[pos] = bStart;
-            sPPre[pos]   = lpre;
+            sPPre[pos]   = spre;
             sPAlpha[pos] = alphaS;
             sPLen[pos]   = (int)bLen;
         }
-        if (tid == 0)
-            sPPre[nHit] = totalE;
         __syncthreads();
         nHits += hit;
-        for (int e = tid; e < totalE; e += BLOCK) {
-            int lo = 0, hi = nHit - 1;
+        for (int t = tid; t < totalSeg; t += BLOCK) {
+            int lo = 0, hi = nHit - 1;  // hit owning segment t
             while (lo < hi) {
                 int mid = (lo + hi + 1) >> 1;
-                if (sPPre[mid] <= e)
+                if (sPPre[mid] <= t)
                     lo = mid;
                 else
                     hi = mid - 1;
             }
-            int  off  = e - sPPre[lo];
-            int  L    = sPLen[lo];
-            bool tail = off >= L - (L & 7);
-            uint2 ent = a.entries[sPStart[lo] + off];
-            int   ref = (int)ent.x;
-            if (ref >= r0 && ref < r1) {
-                int bin = ppfc::angle_bin(__uint_as_float(ent.y), sPAlpha[lo], a.sMaxId,
-                                          a.angleNum, tail);
-                atomicAdd(&acc[(ref - r0) * a.angleNum + bin], 1u);
-                nVotes++;
+            const int   L     = sPLen[lo];
+            const int   e0    = (t - sPPre[lo]) * SEG;
+            const int   cnt   = min(SEG, L - e0);
+            const bool  tail  = e0 >= L - (L & 7);
+            const float aS    = sPAlpha[lo];
+            const uint2 *ent  = a.entries + sPStart[lo] + e0;
+            uint2 v[SEG];
+#pragma unroll
+            for (int k = 0; k < SEG; k++)
+                if (k < cnt)
+                    v[k] = ent[k];
+#pragma unroll
+            for (int k = 0; k < SEG; k++) {
+                if (k >= cnt)
+                    break;
+                int ref = (int)v[k].x;
+                if (ref >= r0 && ref < r1) {
+                    int bin = ppfc::angle_bin(__uint_as_float(v[k].y), aS, a.sMaxId, a.angleNum,
+                                              tail);
+                    atomicAdd(&acc[(ref - r0) * a.angleNum + bin], 1u);
+                    nVotes++;
+                }
             }
         }
         __syncthreads();

@@ -1,0 +1,76 @@
+"""The device arithmetic header (canon.h) compiled for the host agrees bit for
+bit with the oracle -- the same FP32 sequences, written twice.  CPU only."""
+import ctypes as C
+import math
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import rng
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+TWO_PI_F = np.float32(2.0) * np.float32(math.pi)
+
+
+@pytest.fixture(scope="module")
+def H(tmp_path_factory):
+    so = str(tmp_path_factory.mktemp("canon") / "canon_host.so")
+    subprocess.check_call(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",
+                           "-fno-fast-math", os.path.join(ROOT, "tests", "cpp", "canon_host.cpp"),
+                           "-o", so])
+    L = C.CDLL(so)
+    fp = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+    L.ch_atan2f.restype = C.c_float
+    L.ch_atan2f.argtypes = [C.c_float, C.c_float]
+    L.ch_ppf_key.restype = C.c_uint32
+    L.ch_ppf_key.argtypes = [fp, fp, fp, fp, C.c_float, C.c_float, C.c_int]
+    L.ch_frame.argtypes = [fp, fp, fp]
+    L.ch_alpha.restype = C.c_float
+    L.ch_alpha.argtypes = [fp, fp]
+    L.ch_bin.restype = C.c_int
+    L.ch_bin.argtypes = [C.c_float, C.c_float, C.c_int, C.c_int]
+    L.ch_pose.argtypes = [fp, fp, C.c_float, C.c_float, fp]
+    L.ch_quat.argtypes = [fp, fp]
+    return L
+
+
+def test_canon_matches_oracle(H, oracle_mod):
+    O = oracle_mod
+    r = rng(90)
+    ang = TWO_PI_F / np.float32(30)
+    for _ in range(3000):
+        p1, p2 = r.uniform(-100, 100, (2, 3)).astype(np.float32)
+        n1, n2 = r.normal(size=(2, 3))
+        n1 = (n1 / np.linalg.norm(n1)).astype(np.float32)
+        n2 = (n2 / np.linalg.norm(n2)).astype(np.float32)
+        for tail in (0, 1):
+            assert H.ch_ppf_key(p1, n1, p2, n2, ang, np.float32(5.3), tail) == \
+                O.ppf_key(p1, n1, p2, n2, ang, np.float32(5.3), tail)[0]
+        rt = np.zeros(12, np.float32)
+        H.ch_frame(p1, n1, rt)
+        assert np.array_equal(rt, O.frame(p1, n1))
+        a = H.ch_alpha(rt, p2)
+        assert np.float32(a).view(np.uint32) == np.float32(O.alpha(rt, p2)).view(np.uint32)
+        am, as_ = r.uniform(-math.pi, math.pi, 2).astype(np.float32)
+        for tail in (0, 1):
+            assert H.ch_bin(am, as_, 30, tail) == O.lib().orc_angle_bin(float(am), float(as_), 30, tail)
+        y, x = r.normal(size=2).astype(np.float32)
+        assert np.float32(H.ch_atan2f(y, x)) == np.float32(O.lib().orc_atan2f(float(y), float(x)))
+    # poses: compose_pose with the oracle's cos/sin of bin j
+    for j in range(30):
+        p1, p2 = r.uniform(-100, 100, (2, 3)).astype(np.float32)
+        n1, n2 = r.normal(size=(2, 3))
+        Ts = O.frame(p1, (n1 / np.linalg.norm(n1)).astype(np.float32))
+        Tm = O.frame(p2, (n2 / np.linalg.norm(n2)).astype(np.float32))
+        a0 = TWO_PI_F * np.float32(j) / np.float32(29)
+        a = np.float32(np.float64(a0) - math.pi)
+        out = np.zeros(12, np.float32)
+        H.ch_pose(Ts, Tm, np.float32(math.cos(float(a))), np.float32(math.sin(float(a))), out)
+        assert np.array_equal(out, O.pose(Ts, Tm, j, 30))
+        R = out.reshape(3, 4)[:, :3].copy()
+        q1, q2 = np.zeros(4, np.float32), np.zeros(4, np.float32)
+        H.ch_quat(np.ascontiguousarray(R.ravel()), q1)
+        O.lib().orc_quat_from_rot(np.ascontiguousarray(R.ravel()), q2)
+        assert np.array_equal(q1, q2)
