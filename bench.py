@@ -55,6 +55,67 @@ def cpu_baseline(w, threads: int):
     return ms, st
 
 
+def icp_batching(det, H_per_instance=64, n_inst=8, seed=0x5EED0003):
+    """Batched vs plain GPU point-to-plane ICP on the same hypotheses."""
+    import math
+    from surface_match_amd import synth
+    from surface_match_amd.icp import IcpTarget
+    r = np.random.Generator(np.random.PCG64(seed))
+    _, _, src, srcn = det.sampled_model()          # the trained resampled model
+    tp, tn, gts = [], [], []
+    for k in range(n_inst):
+        p, n = synth.make_model(40000, seed + 10 + k)
+        R = synth.random_rotation(r)
+        t = r.uniform(-250, 250, 3)
+        tp.append(p @ R.T + t)
+        tn.append(n @ R.T)
+        T = np.eye(4)
+        T[:3, :3], T[:3, 3] = R, t
+        gts.append(T)
+    dst = (np.concatenate(tp) + r.normal(scale=0.1, size=(40000 * n_inst, 3))).astype(np.float32)
+    dstn = np.concatenate(tn).astype(np.float32)
+    inits, owner = [], []
+    for k in range(n_inst):
+        for _ in range(H_per_instance):
+            w = np.radians(r.normal(scale=2.0, size=3))
+            th = np.linalg.norm(w)
+            a = w / th
+            K = np.array([[0, -a[2], a[1]], [a[2], 0, -a[0]], [-a[1], a[0], 0]])
+            P = np.eye(4)
+            P[:3, :3] = np.eye(3) + math.sin(th) * K + (1 - math.cos(th)) * K @ K
+            P[:3, 3] = r.normal(scale=2.0, size=3)
+            inits.append(P @ gts[k])
+            owner.append(k)
+    inits = np.array(inits)
+    tgt = IcpTarget(dst, dstn, 2.0)
+    D = det.model_info().diameter
+    kw = dict(iterations=10, reject_dist=0.1 * D, mse_min=0.0, mse_max=1e30, tolerance=0.0,
+              mode=1, scoring_dist=0.01 * D)
+    tgt.refine(src, srcn, inits[:8], batched=True, **kw)          # warm-up + graph capture
+    tgt.refine(src, srcn, inits[:8], batched=False, **kw)
+    t0 = time.perf_counter()
+    pb, _, msb = tgt.refine(src, srcn, inits, batched=True, **kw)
+    wb = time.perf_counter() - t0
+    nplain = 64
+    t0 = time.perf_counter()
+    pp, _, msp = tgt.refine(src, srcn, inits[:nplain], batched=False, **kw)
+    wp = time.perf_counter() - t0
+
+    def ok(P, k):
+        ang = synth.rotation_angle_deg(gts[k][:3, :3].T @ P[:3, :3])
+        return ang <= 0.1 and np.linalg.norm(gts[k][:3, 3] - P[:3, 3]) <= 0.001 * D
+    fb = float(np.mean([ok(pb[h], owner[h]) for h in range(len(inits))]))
+    fp = float(np.mean([ok(pp[h], owner[h]) for h in range(nplain)]))
+    diff = max(synth.rotation_angle_deg(pb[h][:3, :3].T @ pp[h][:3, :3]) for h in range(nplain))
+    return {"hypotheses": len(inits), "source_points": int(len(src)), "iterations": 10,
+            "mode": "point-to-plane", "batched_poses_per_s": round(len(inits) / wb, 1),
+            "plain_poses_per_s": round(nplain / wp, 1),
+            "speedup": round((len(inits) / wb) / (nplain / wp), 2),
+            "batched_device_ms": round(msb, 3), "plain_device_ms_per_pose": round(msp / nplain, 4),
+            "batched_within_0.1deg_0.1pctD": fb, "plain_within_0.1deg_0.1pctD": fp,
+            "max_batched_vs_plain_deg": round(float(diff), 6)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -136,6 +197,7 @@ def main():
                      "kernel_ms": round(vote_ms, 4)},
         "results": found,
     }
+    out["icp_batching"] = icp_batching(det)
     if not args.no_cpu_baseline:
         threads = min(os.cpu_count() or 1, 8)
         cms, cst = cpu_baseline(w, threads)

@@ -1426,21 +1426,29 @@ static iter_t icp_iteration(int nSrc, const float *sp, const float *sn, const or
             U[r2 * 4 + 3] = v + qm[r2];
         }
     } else {
-        /* point-to-plane: r = n_q.(p - q), J = [p x n_q ; n_q], x = -(J^T J)^+ J^T r,
-         * update exp(x^) via Rodrigues on omega, translation v */
-        double A[36] = {0}, b[6] = {0};
+        /* point-to-plane about the source centroid c (the de-meaning of the
+         * reference's point_to_plane, icp.cpp:304-308, with unit weights):
+         * r = n_q.(p - q), J = [(p - c) x n_q ; n_q], x = -(J^T J)^+ J^T r,
+         * update p -> R(w)(p - c) + c + v */
+        double A[36] = {0}, b[6] = {0}, cm[3] = {0, 0, 0};
+        for (int t = 0; t < npair; t++)
+            for (int a = 0; a < 3; a++)
+                cm[a] += sp[3 * best[order[t]] + a];
+        for (int a = 0; a < 3; a++)
+            cm[a] /= npair;
         for (int t = 0; t < npair; t++) {
             int    q = order[t], i = best[q];
-            double p[3], qq[3], nn[3], x[6];
+            double p[3], qq[3], nn[3], x[6], d[3];
             for (int a = 0; a < 3; a++) {
                 p[a]  = sp[3 * i + a];
                 qq[a] = dst->pts[3 * q + a];
                 nn[a] = dstN[3 * q + a];
+                d[a]  = p[a] - cm[a];
             }
             double ri = (p[0] - qq[0]) * nn[0] + (p[1] - qq[1]) * nn[1] + (p[2] - qq[2]) * nn[2];
-            x[0] = p[1] * nn[2] - p[2] * nn[1];
-            x[1] = p[2] * nn[0] - p[0] * nn[2];
-            x[2] = p[0] * nn[1] - p[1] * nn[0];
+            x[0] = d[1] * nn[2] - d[2] * nn[1];
+            x[1] = d[2] * nn[0] - d[0] * nn[2];
+            x[2] = d[0] * nn[1] - d[1] * nn[0];
             x[3] = nn[0];
             x[4] = nn[1];
             x[5] = nn[2];
@@ -1471,10 +1479,11 @@ static iter_t icp_iteration(int nSrc, const float *sp, const float *sn, const or
                 R[k] = (k % 4 == 0) ? 1.0 : 0.0;
         }
         for (int r2 = 0; r2 < 3; r2++) {
+            double Rc = R[r2 * 3 + 0] * cm[0] + R[r2 * 3 + 1] * cm[1] + R[r2 * 3 + 2] * cm[2];
             U[r2 * 4 + 0] = R[r2 * 3 + 0];
             U[r2 * 4 + 1] = R[r2 * 3 + 1];
             U[r2 * 4 + 2] = R[r2 * 3 + 2];
-            U[r2 * 4 + 3] = u[3 + r2];
+            U[r2 * 4 + 3] = cm[r2] - Rc + u[3 + r2];
         }
     }
     memcpy(r.pose, U, sizeof U);

@@ -227,17 +227,14 @@ def test_l5_normals(N, oracle_mod, workload1):
         N.check(N.lib().ppf_estimate_normals(N.ptr(gp), N.ptr(gn), len(sc), N.ptr(valid),
                                              len(valid), N.ptr(samp), len(samp), 10, smooth, 0,
                                              N.ptr(vp)))
-        # positions: bit-exact; normals: bit-exact expected, <= 1e-5 rad required
+        # positions and normals: bit-exact (same kNN order, same FP32 sums,
+        # same FP64 Jacobi sequence on both sides)
         assert np.array_equal(np.isnan(gp), np.isnan(ep))
         m = ~np.isnan(ep[:, 0])
         assert np.array_equal(gp[m], ep[m])
         mn = ~np.isnan(en[:, 0])
         assert np.array_equal(mn, ~np.isnan(gn[:, 0]))
-        cosang = np.clip(np.sum(gn[mn].astype(np.float64) * en[mn], 1), -1, 1)
-        err = np.degrees(np.arccos(cosang))
-        print(f"smooth={smooth}: normals bit-exact fraction",
-              np.mean(np.all(gn[mn] == en[mn], 1)), "max err deg", err.max())
-        assert err.max() < 1e-3
+        assert np.array_equal(gn[mn], en[mn]), f"smooth={smooth}"
 
 
 def test_match_workload1_poses(trained, oracle_mod, workload1):
