@@ -91,11 +91,15 @@ def icp_batching(det, H_per_instance=64, n_inst=8, seed=0x5EED0003):
     D = det.model_info().diameter
     kw = dict(iterations=10, reject_dist=0.1 * D, mse_min=0.0, mse_max=1e30, tolerance=0.0,
               mode=1, scoring_dist=0.01 * D)
-    tgt.refine(src, srcn, inits[:8], batched=True, **kw)          # warm-up + graph capture
+    tgt.refine(src, srcn, inits, batched=True, **kw)          # warm-up + graph capture
+    tgt.refine(src, srcn, inits, batched="persistent", **kw)
     tgt.refine(src, srcn, inits[:8], batched=False, **kw)
     t0 = time.perf_counter()
     pb, _, msb = tgt.refine(src, srcn, inits, batched=True, **kw)
     wb = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    pq, _, msq = tgt.refine(src, srcn, inits, batched="persistent", **kw)
+    wq = time.perf_counter() - t0
     nplain = 64
     t0 = time.perf_counter()
     pp, _, msp = tgt.refine(src, srcn, inits[:nplain], batched=False, **kw)
@@ -110,6 +114,10 @@ def icp_batching(det, H_per_instance=64, n_inst=8, seed=0x5EED0003):
     return {"hypotheses": len(inits), "source_points": int(len(src)), "iterations": 10,
             "mode": "point-to-plane", "batched_poses_per_s": round(len(inits) / wb, 1),
             "plain_poses_per_s": round(nplain / wp, 1),
+            "persistent_poses_per_s": round(len(inits) / wq, 1),
+            "persistent_device_ms": round(msq, 3),
+            "max_persistent_vs_spread_deg": round(float(max(
+                synth.rotation_angle_deg(pb[h][:3, :3].T @ pq[h][:3, :3]) for h in range(len(inits)))), 6),
             "speedup": round((len(inits) / wb) / (nplain / wp), 2),
             "batched_device_ms": round(msb, 3), "plain_device_ms_per_pose": round(msp / nplain, 4),
             "batched_within_0.1deg_0.1pctD": fb, "plain_within_0.1deg_0.1pctD": fp,
@@ -188,7 +196,8 @@ def main():
         },
         "stage_ms": {k: round(float(np.mean([s[k] for s in stats])), 3) for k in
                      ("msSample", "msNormals", "msKeypoints", "msVote", "msVoteKernel",
-                      "msCluster", "msIcpPrepare", "msIcp", "msTotal")},
+                      "msCluster", "msIcpPrepare", "msIcp", "msIcpSparse", "msIcpDense",
+                      "msTotal")},
         "vote_throughput": {"keypoints_per_s": round(st["nKeypoints"] / (vote_ms * 1e-3)),
                             "votes_per_s": round(st["nVotes"] / (vote_ms * 1e-3))},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,

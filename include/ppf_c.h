@@ -86,6 +86,7 @@ typedef struct {
     unsigned long long nPairs, nKeyHits, nVotes;
     float  msSample, msNormals, msKeypoints, msVote, msCluster, msIcpPrepare, msIcp, msTotal;
     float  msVoteKernel; /* device time of the vote kernel alone */
+    float  msIcpSparse, msIcpDense; /* device time of the two batched refinements */
 } ppf_match_stats;
 
 /* -- parameters (TrainParam/MatchParam constructors, type.cpp:71-127) -- */
@@ -133,17 +134,20 @@ PPF_API int ppf_match_scene_resident(ppf_detector d, ppf_scene s, float sampling
 
 /* -- autotuning (no reference counterpart): times the vote kernel over
  * workgroup sizes {256, 512, 1024} x XCD-aware key-point mapping {off, on}
- * and the batched ICP over {64, 128, 256, 512, 1024} threads per hypothesis
- * on the given resident scene, and keeps the fastest in the detector. -- */
+ * and the batched ICP over {spread, persistent} x {64, 128, 256, 512, 1024}
+ * threads per workgroup on the given resident scene, and keeps the fastest
+ * in the detector. -- */
 typedef struct {
     int   voteBlock, voteXcdRemap, icpBlock;
     float voteMs, icpMs;
     int   configsTried;
+    int   icpSpread; /* 1: each hypothesis spread over workgroups, 0: persistent */
 } ppf_tune_report;
 PPF_API int ppf_autotune(ppf_detector d, ppf_scene s, float samplingDistanceRel,
                          float keyPointFraction, float minScore, const ppf_match_param *param,
                          ppf_tune_report *report);
-PPF_API int ppf_set_tuning(ppf_detector d, int voteBlock, int voteXcdRemap, int icpBlock);
+PPF_API int ppf_set_tuning(ppf_detector d, int voteBlock, int voteXcdRemap, int icpBlock,
+                           int icpSpread);
 
 /* -- trained-model introspection (the IMPL fields, privateType.h:56-66) -- */
 typedef struct {
@@ -216,10 +220,12 @@ PPF_API int  ppf_icp_target_create(const float *pts, const float *nrm, size_t n,
                                    ppf_icp_target *out);
 PPF_API void ppf_icp_target_destroy(ppf_icp_target t);
 /* refine nPoses initial poses of src (with normals).  mode 0 = the
- * reference's symmetric objective, 1 = point-to-plane.  batched != 0: one
- * persistent launch for all poses (replayed from a captured hipGraph);
- * batched == 0: the conventional refiner, one pose at a time.  block: threads
- * per hypothesis workgroup in the batched launch (0 = tuned default).
+ * reference's symmetric objective, 1 = point-to-plane.  batched == 1: all
+ * poses in one replayed hipGraph, each pose cut into workgroups of `block`
+ * source points over the whole chip; batched == 2: all poses in one
+ * persistent launch, one workgroup of `block` threads per pose; batched == 0:
+ * the conventional refiner, one pose at a time with a host stop check per
+ * iteration.  block: 0 = 256.
  * deviceMs (optional) receives the refinement's device time. */
 PPF_API int ppf_icp_refine(ppf_icp_target t, const float *src, const float *srcNrm, size_t nSrc,
                            const float *initPoses16, size_t nPoses, const ppf_icp_criteria *c,

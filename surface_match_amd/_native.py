@@ -45,7 +45,7 @@ class MatchStats(C.Structure):
                [(n, C.c_ulonglong) for n in ("nPairs", "nKeyHits", "nVotes")] + \
                [(n, C.c_float) for n in ("msSample", "msNormals", "msKeypoints", "msVote",
                                          "msCluster", "msIcpPrepare", "msIcp", "msTotal",
-                                         "msVoteKernel")]
+                                         "msVoteKernel", "msIcpSparse", "msIcpDense")]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -71,7 +71,8 @@ class IcpResult(C.Structure):
 
 class TuneReport(C.Structure):
     _fields_ = [("voteBlock", C.c_int), ("voteXcdRemap", C.c_int), ("icpBlock", C.c_int),
-                ("voteMs", C.c_float), ("icpMs", C.c_float), ("configsTried", C.c_int)]
+                ("voteMs", C.c_float), ("icpMs", C.c_float), ("configsTried", C.c_int),
+                ("icpSpread", C.c_int)]
 
 
 _lib = None
@@ -130,7 +131,7 @@ def lib() -> C.CDLL:
                                vp, C.POINTER(f)]),
         "ppf_icp_adjoint": (i, [vp, vp, sz, vp, vp, sz, f, i, vp, vp]),
         "ppf_autotune": (i, [vp, vp, f, f, f, C.POINTER(MatchParam), C.POINTER(TuneReport)]),
-        "ppf_set_tuning": (i, [vp, i, i, i]),
+        "ppf_set_tuning": (i, [vp, i, i, i, i]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

@@ -229,7 +229,7 @@ struct ppf_detector_s {
     int          refTail = 1;
     DevCloud     tmpScene;
     DevBuf       idxA, idxB, idxC, idxD;  // sampled / keypoints / dense / scratch
-    hipEvent_t   ev[10] = {};
+    hipEvent_t   ev[11] = {};
 
     ppf_detector_s() {
         PPF_HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
@@ -514,10 +514,13 @@ void match_impl(ppf_detector_s &D, DevCloud &c, float rel, float kpFrac, float m
         std::vector<IcpResultDev> res(cand.size());
         for (size_t k = 0; k < cand.size(); k++)
             std::memcpy(&init[12 * k], hyp[cand[k]].pose, sizeof(float) * 12);
+        PPF_HIP_CHECK(hipEventRecord(D.ev[9], s));
         icp_batch(D.ws, refNum, dm.sx.get<float>(), dm.sy.get<float>(), dm.sz.get<float>(),
                   dm.snx.get<float>(), dm.sny.get<float>(), dm.snz.get<float>(),
                   D.ws.gSparse.view, D.ws.gFull.view, init.data(), (int)cand.size(), sc,
                   scoringDist, D.icfg, res.data(), s);
+        PPF_HIP_CHECK(hipEventRecord(D.ev[10], s));
+        st->msIcpSparse = elapsed(D.ev[9], D.ev[10]);
         nSparse = (int)cand.size();
         std::vector<int> survivors;
         for (size_t k = 0; k < cand.size(); k++) {
@@ -545,10 +548,13 @@ void match_impl(ppf_detector_s &D, DevCloud &c, float rel, float kpFrac, float m
             for (size_t k = 0; k < survivors.size(); k++)
                 std::memcpy(&init2[12 * k], &poseNow[12 * (size_t)survivors[k]], sizeof(float) * 12);
             IcpConfig dcfg = D.icfg;
+            PPF_HIP_CHECK(hipEventRecord(D.ev[9], s));
             icp_batch(D.ws, hm.reSampled.size(), dm.rx.get<float>(), dm.ry.get<float>(),
                       dm.rz.get<float>(), dm.rnx.get<float>(), dm.rny.get<float>(),
                       dm.rnz.get<float>(), D.ws.gDense.view, D.ws.gFull.view, init2.data(),
                       (int)survivors.size(), dc, scoringDist, dcfg, res2.data(), s);
+            PPF_HIP_CHECK(hipEventRecord(D.ev[10], s));
+            st->msIcpDense = elapsed(D.ev[9], D.ev[10]);
             nDenseIcp = (int)survivors.size();
             for (size_t k = 0; k < survivors.size(); k++) {
                 int i = survivors[k];
@@ -1003,7 +1009,8 @@ PPF_API int ppf_match_scene_resident(ppf_detector d, ppf_scene sc, float rel, fl
     });
 }
 
-PPF_API int ppf_set_tuning(ppf_detector d, int voteBlock, int voteXcdRemap, int icpBlock) {
+PPF_API int ppf_set_tuning(ppf_detector d, int voteBlock, int voteXcdRemap, int icpBlock,
+                           int icpSpread) {
     PPF_REQUIRE(d, "null detector");
     PPF_REQUIRE(voteBlock == 256 || voteBlock == 512 || voteBlock == 1024, "vote block must be 256, 512 or 1024");
     PPF_REQUIRE(icpBlock == 64 || icpBlock == 128 || icpBlock == 256 || icpBlock == 512 || icpBlock == 1024,
@@ -1011,6 +1018,7 @@ PPF_API int ppf_set_tuning(ppf_detector d, int voteBlock, int voteXcdRemap, int 
     d->vcfg.block    = voteBlock;
     d->vcfg.xcdRemap = voteXcdRemap ? 1 : 0;
     d->icfg.block    = icpBlock;
+    d->icfg.spread   = icpSpread ? 1 : 0;
     return PPF_OK;
 }
 
@@ -1050,20 +1058,23 @@ PPF_API int ppf_autotune(ppf_detector d, ppf_scene sc, float rel, float kpFrac, 
                 }
             }
         d->vcfg = bestV;
-        for (int block : {64, 128, 256, 512, 1024}) {
-            d->icfg.block = block;
-            float v, i;
-            run(v, i);
-            tried++;
-            if (i < bestIcpMs) {
-                bestIcpMs = i;
-                bestI     = d->icfg;
+        for (int spread : {1, 0})
+            for (int block : {64, 128, 256, 512, 1024}) {
+                d->icfg.spread = spread;
+                d->icfg.block  = block;
+                float v, i;
+                run(v, i);
+                tried++;
+                if (i < bestIcpMs) {
+                    bestIcpMs = i;
+                    bestI     = d->icfg;
+                }
             }
-        }
         d->icfg               = bestI;
         rep->voteBlock        = bestV.block;
         rep->voteXcdRemap     = bestV.xcdRemap;
         rep->icpBlock         = bestI.block;
+        rep->icpSpread        = bestI.spread;
         rep->voteMs           = bestVoteMs;
         rep->icpMs            = bestIcpMs;
         rep->configsTried     = tried;
@@ -1417,8 +1428,9 @@ PPF_API int ppf_icp_refine(ppf_icp_target t, const float *src, const float *srcN
         PPF_HIP_CHECK(hipEventRecord(t->ev[0], s));
         if (batched) {
             IcpConfig cfg;
-            cfg.mode  = mode;
-            cfg.block = block > 0 ? block : 256;
+            cfg.mode   = mode;
+            cfg.block  = block > 0 ? block : 256;
+            cfg.spread = batched == 2 ? 0 : 1;
             icp_batch(t->ws, (int)nSrc, t->sx.get<float>(), t->sy.get<float>(), t->sz.get<float>(),
                       t->snx.get<float>(), t->sny.get<float>(), t->snz.get<float>(),
                       t->grid.view, t->full.view, init.data(), (int)nPoses, ic, scoringDist, cfg,

@@ -203,10 +203,14 @@ struct HypoScratch {
     uint32_t            T;
 };
 
-__device__ __forceinline__ HypoScratch scratch_of(const IcpArgs &a, int h) {
+// `table` selects one of the two dedupe tables of a hypothesis (the spread
+// refiner alternates them between iterations so one can be cleared while
+// the other is in use)
+__device__ __forceinline__ HypoScratch scratch_of(const IcpArgs &a, int h, int table = 0) {
     HypoScratch x;
-    x.hkey   = a.hkey + (size_t)h * a.T;
-    x.hbest  = a.hbest + (size_t)h * a.T;
+    size_t      base = ((size_t)table * a.H + h) * a.T;
+    x.hkey   = a.hkey + base;
+    x.hbest  = a.hbest + base;
     x.nnSlot = a.nnSlot + (size_t)h * a.nSrc;
     x.hslot  = a.hslot + (size_t)h * a.nSrc;
     x.nnD    = a.nnD + (size_t)h * a.nSrc;
@@ -321,6 +325,10 @@ __device__ __forceinline__ void add_eq(const IcpArgs &a, const HypoScratch &x, i
         cf_add(e27[21 + p], r * J[p]);
 }
 
+// distance of the updated point to its nearest scene point (icp.cpp:182-194).
+// The point's previous partner lies at dist2(updated point, partner), so the
+// true nearest neighbour is within that distance: the search is bounded by
+// it and returns the same (dist2, id) as an unbounded one.
 __device__ __forceinline__ void add_mse(const IcpArgs &a, const HypoScratch &x, int i,
                                         const float *curF, const float *UF, CF (&m1)[1]) {
     if (!is_winner(x, i))
@@ -328,9 +336,13 @@ __device__ __forceinline__ void add_mse(const IcpArgs &a, const HypoScratch &x, 
     float px, py, pz, qx, qy, qz;
     xform(curF, a.sx[i], a.sy[i], a.sz[i], px, py, pz);
     xform(UF, px, py, pz, qx, qy, qz);
-    float d2;
-    int   did;
-    grid_nn1(a.dst, qx, qy, qz, d2, did);
+    const int s     = x.nnSlot[i];
+    float     bound = ppfc::dist2(qx, qy, qz, a.dst.x[s], a.dst.y[s], a.dst.z[s]);
+    float     d2;
+    int       did;
+    if (!(bound < FLT_MAX))
+        bound = FLT_MAX;
+    grid_nn1(a.dst, qx, qy, qz, d2, did, bound);
     cf_add(m1[0], sqrtf(d2));
 }
 
@@ -546,6 +558,250 @@ __global__ void __launch_bounds__(BLOCK) icp_kernel(IcpArgs a) {
     }
 }
 
+// ---------------------------------------------------------------- spread
+//
+// The spread batched refiner: every hypothesis is cut into chunks of BLOCK
+// source points and one launch covers all (hypothesis, chunk) pairs, so a
+// few hypotheses over many points fill the chip as well as many over few.
+// An iteration is four launches (correspondences, means, normal equations,
+// mse); the serial step of each (means, FP64 solve, convergence) is done by
+// the last workgroup of the hypothesis to finish, which reduces the
+// per-chunk compensated partials in chunk order (deterministic).  States,
+// counters and stop flags stay in HBM; a stopped hypothesis costs one load
+// per workgroup.  The launch sequence is fixed by the iteration cap, so it
+// is captured once into a hipGraph and replayed.
+
+struct SpreadArgs {
+    IcpArgs   b;
+    IcpState *st;      // H
+    uint32_t *arrive;  // H arrival counters, zero between launches
+    CF       *part;    // H x nChunk x 27 per-chunk partial sums
+    int      *inl;     // H inlier counts
+    int       nChunk;
+};
+
+__device__ __forceinline__ bool hypo_active(const IcpState &st, const IcpCriteria &c) {
+    return !st.stop && st.iters < c.iterations;
+}
+
+// Publish this workgroup's M partial sums; returns true (uniformly) in the
+// last workgroup of hypothesis h to arrive, with the chunk-ordered totals in
+// red[0..M).
+template <int BLOCK, int M>
+__device__ bool arrive_reduce(const SpreadArgs &w, int h, int chunk, CF (&v)[M], CF *lds,
+                              double *red, int *sLast) {
+#pragma unroll
+    for (int k = 0; k < M; k++)
+        for (int o = 1; o < 64; o <<= 1) {
+            CF u;
+            u.hi = __shfl_xor(v[k].hi, o, 64);
+            u.lo = __shfl_xor(v[k].lo, o, 64);
+            v[k] = ((threadIdx.x & o) == 0) ? cf_merge(v[k], u) : cf_merge(u, v[k]);
+        }
+    const int wid = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0)
+#pragma unroll
+        for (int k = 0; k < M; k++)
+            lds[wid * M + k] = v[k];
+    __syncthreads();
+    CF *mine = w.part + ((size_t)h * w.nChunk + chunk) * 27;
+    if (threadIdx.x < M) {
+        CF acc = lds[threadIdx.x];
+        for (int q = 1; q < BLOCK / 64; q++)
+            acc = cf_merge(acc, lds[q * M + threadIdx.x]);
+        mine[threadIdx.x] = acc;
+        __threadfence();
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+        *sLast = (atomicAdd(&w.arrive[h], 1u) == (uint32_t)(w.nChunk - 1)) ? 1 : 0;
+    __syncthreads();
+    if (!*sLast)
+        return false;
+    __threadfence();
+    if (threadIdx.x < M) {
+        const CF *all = w.part + (size_t)h * w.nChunk * 27;
+        CF        acc = all[threadIdx.x];
+        for (int q = 1; q < w.nChunk; q++)
+            acc = cf_merge(acc, all[(size_t)q * 27 + threadIdx.x]);
+        red[threadIdx.x] = (double)acc.hi + (double)acc.lo;
+    }
+    if (threadIdx.x == 0)
+        w.arrive[h] = 0u;
+    __syncthreads();
+    return true;
+}
+
+__global__ void spread_init_kernel(SpreadArgs w) {
+    const size_t gs = (size_t)gridDim.x * blockDim.x;
+    const size_t t0 = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (size_t h = t0; h < (size_t)w.b.H; h += gs) {
+        state_init(w.st[h], w.b.init + h * 12);
+        w.arrive[h] = 0u;
+        w.inl[h]    = 0;
+    }
+    const size_t nt = (size_t)2 * w.b.H * w.b.T;
+    for (size_t t = t0; t < nt; t += gs) {
+        w.b.hkey[t]  = 0u;
+        w.b.hbest[t] = ~0ull;
+    }
+}
+
+template <int BLOCK>
+__global__ void __launch_bounds__(BLOCK) spread_corr_kernel(SpreadArgs w) {
+    const int h = blockIdx.x / w.nChunk, chunk = blockIdx.x - h * w.nChunk;
+    __shared__ float sCur[12];
+    __shared__ int   sTab;
+    const IcpState  &st = w.st[h];
+    if (!hypo_active(st, w.b.c))
+        return;
+    if (threadIdx.x < 12)
+        sCur[threadIdx.x] = st.curF[threadIdx.x];
+    if (threadIdx.x == 0)
+        sTab = st.iters & 1;
+    __syncthreads();
+    const int i = chunk * BLOCK + threadIdx.x;
+    if (i < w.b.nSrc)
+        corr_point(w.b, scratch_of(w.b, h, sTab), i, sCur, w.b.c.rejectDist * w.b.c.rejectDist);
+}
+
+template <int BLOCK>
+__global__ void __launch_bounds__(BLOCK) spread_means_kernel(SpreadArgs w) {
+    const int h = blockIdx.x / w.nChunk, chunk = blockIdx.x - h * w.nChunk;
+    __shared__ float  sCur[12];
+    __shared__ int    sTab, sLast;
+    __shared__ CF     sLds[(BLOCK / 64) * 7];
+    __shared__ double sRed[7];
+    IcpState &st = w.st[h];
+    if (!hypo_active(st, w.b.c))
+        return;
+    if (threadIdx.x < 12)
+        sCur[threadIdx.x] = st.curF[threadIdx.x];
+    if (threadIdx.x == 0)
+        sTab = st.iters & 1;
+    __syncthreads();
+    // the other table was last used by the previous iteration: clear this
+    // chunk's share of it for the next one
+    {
+        const HypoScratch o   = scratch_of(w.b, h, sTab ^ 1);
+        const uint32_t    per = (w.b.T + w.nChunk - 1) / w.nChunk;
+        const uint32_t    lo = per * chunk, hi = min(w.b.T, lo + per);
+        for (uint32_t t = lo + threadIdx.x; t < hi; t += BLOCK) {
+            o.hkey[t]  = 0u;
+            o.hbest[t] = ~0ull;
+        }
+    }
+    CF m7[7];
+    for (int k = 0; k < 7; k++)
+        m7[k] = CF{0.0f, 0.0f};
+    const int i = chunk * BLOCK + threadIdx.x;
+    if (i < w.b.nSrc)
+        add_means(w.b, scratch_of(w.b, h, sTab), i, sCur, m7);
+    if (!arrive_reduce<BLOCK, 7>(w, h, chunk, m7, sLds, sRed, &sLast))
+        return;
+    if (threadIdx.x == 0) {
+        const int valid = (int)sRed[6];
+        st.valid        = valid;
+        if (valid < 6) {
+            st.type = 3;  // NO_CORRESPONDS
+            st.stop = 1;
+        } else {
+            for (int k = 0; k < 6; k++)
+                st.means[k] = (float)(sRed[k] / valid);
+        }
+    }
+}
+
+template <int BLOCK>
+__global__ void __launch_bounds__(BLOCK) spread_eq_kernel(SpreadArgs w) {
+    const int h = blockIdx.x / w.nChunk, chunk = blockIdx.x - h * w.nChunk;
+    __shared__ float  sCur[12], sMeans[6];
+    __shared__ int    sTab, sLast;
+    __shared__ CF     sLds[(BLOCK / 64) * 27];
+    __shared__ double sRed[27];
+    IcpState &st = w.st[h];
+    if (!hypo_active(st, w.b.c))
+        return;
+    if (threadIdx.x < 12)
+        sCur[threadIdx.x] = st.curF[threadIdx.x];
+    if (threadIdx.x < 6)
+        sMeans[threadIdx.x] = st.means[threadIdx.x];
+    if (threadIdx.x == 0)
+        sTab = st.iters & 1;
+    __syncthreads();
+    CF e27[27];
+    for (int k = 0; k < 27; k++)
+        e27[k] = CF{0.0f, 0.0f};
+    const int i = chunk * BLOCK + threadIdx.x;
+    if (i < w.b.nSrc)
+        add_eq(w.b, scratch_of(w.b, h, sTab), i, sCur, sMeans, w.b.mode, e27);
+    if (!arrive_reduce<BLOCK, 27>(w, h, chunk, e27, sLds, sRed, &sLast))
+        return;
+    if (threadIdx.x == 0)
+        solve_update(sRed, sMeans, w.b.mode, st);
+}
+
+template <int BLOCK>
+__global__ void __launch_bounds__(BLOCK) spread_mse_kernel(SpreadArgs w) {
+    const int h = blockIdx.x / w.nChunk, chunk = blockIdx.x - h * w.nChunk;
+    __shared__ float  sCur[12], sU[12];
+    __shared__ int    sTab, sLast;
+    __shared__ CF     sLds[BLOCK / 64];
+    __shared__ double sRed[1];
+    IcpState &st = w.st[h];
+    if (!hypo_active(st, w.b.c))
+        return;
+    if (threadIdx.x < 12) {
+        sCur[threadIdx.x] = st.curF[threadIdx.x];
+        sU[threadIdx.x]   = st.UF[threadIdx.x];
+    }
+    if (threadIdx.x == 0)
+        sTab = st.iters & 1;
+    __syncthreads();
+    CF        m1[1] = {CF{0.0f, 0.0f}};
+    const int i     = chunk * BLOCK + threadIdx.x;
+    if (i < w.b.nSrc)
+        add_mse(w.b, scratch_of(w.b, h, sTab), i, sCur, sU, m1);
+    if (!arrive_reduce<BLOCK, 1>(w, h, chunk, m1, sLds, sRed, &sLast))
+        return;
+    if (threadIdx.x == 0)
+        converge(st, (float)(sRed[0] / (double)st.valid), w.b.c);
+}
+
+template <int BLOCK>
+__global__ void __launch_bounds__(BLOCK) spread_inlier_kernel(SpreadArgs w) {
+    const int h = blockIdx.x / w.nChunk, chunk = blockIdx.x - h * w.nChunk;
+    __shared__ float sPF[12];
+    __shared__ int   sCnt[BLOCK / 64];
+    const IcpState  &st = w.st[h];
+    if (threadIdx.x < 12)
+        sPF[threadIdx.x] = (float)st.pose[threadIdx.x];
+    __syncthreads();
+    const int i = chunk * BLOCK + threadIdx.x;
+    int       c = (i < w.b.nSrc &&
+             inlier_point(w.b, i, sPF, w.b.scoringDist * w.b.scoringDist)) ? 1 : 0;
+    for (int o = 32; o > 0; o >>= 1)
+        c += __shfl_xor(c, o, 64);
+    if ((threadIdx.x & 63) == 0)
+        sCnt[threadIdx.x >> 6] = c;
+    __syncthreads();
+    __shared__ int sLast;
+    if (threadIdx.x == 0) {
+        int tot = 0;
+        for (int q = 0; q < BLOCK / 64; q++)
+            tot += sCnt[q];
+        atomicAdd(&w.inl[h], tot);
+        __threadfence();
+        sLast = (atomicAdd(&w.arrive[h], 1u) == (uint32_t)(w.nChunk - 1)) ? 1 : 0;
+        if (sLast) {
+            __threadfence();
+            write_result(w.b.out[h], st, atomicAdd(&w.inl[h], 0));
+            w.inl[h]    = 0;
+            w.arrive[h] = 0u;
+        }
+    }
+}
+
 // ---------------------------------------------------------------- plain
 //
 // The conventional GPU refiner: one hypothesis at a time, each step a grid
@@ -672,6 +928,7 @@ static IcpArgs make_args(Workspace &ws, int nSrc, const float *sx, const float *
     if (!dst.nx)
         throw std::runtime_error("PointCloud empty or no normal at ICP::regist");
     IcpArgs a;
+    std::memset(&a, 0, sizeof a);  // padding too: the bytes key the graph cache
     a.nSrc = nSrc;
     a.H    = H;
     a.sx = sx;
@@ -689,8 +946,9 @@ static IcpArgs make_args(Workspace &ws, int nSrc, const float *sx, const float *
     while (T < 2u * (uint32_t)std::max(nSrc, 1))
         T <<= 1;
     a.T      = T;
-    a.hkey   = ws.icpHash.as<uint32_t>((size_t)H * T);
-    a.hbest  = ws.icpPartial.as<unsigned long long>((size_t)H * T);
+    // two dedupe tables per hypothesis (the spread refiner alternates them)
+    a.hkey   = ws.icpHash.as<uint32_t>((size_t)2 * H * T);
+    a.hbest  = ws.icpPartial.as<unsigned long long>((size_t)2 * H * T);
     int *ib  = ws.icpState.as<int>((size_t)H * nSrc * 3);
     a.nnSlot = ib;
     a.hslot  = ib + (size_t)H * nSrc;
@@ -710,12 +968,43 @@ static void launch_icp(const IcpArgs &a, int block, hipStream_t s) {
     }
 }
 
+template <int BLOCK>
+static void launch_spread_t(const SpreadArgs &w, hipStream_t s) {
+    const long long nb = (long long)w.b.H * w.nChunk;
+    if (nb > 0x7fffffffLL)
+        throw std::runtime_error("ICP batch too large for one launch");
+    const int g = (int)nb;
+    spread_init_kernel<<<std::min(4096, div_up((long long)2 * w.b.H * w.b.T, 256)), 256, 0, s>>>(w);
+    for (int it = 0; it < w.b.c.iterations; it++) {
+        spread_corr_kernel<BLOCK><<<g, BLOCK, 0, s>>>(w);
+        spread_means_kernel<BLOCK><<<g, BLOCK, 0, s>>>(w);
+        spread_eq_kernel<BLOCK><<<g, BLOCK, 0, s>>>(w);
+        spread_mse_kernel<BLOCK><<<g, BLOCK, 0, s>>>(w);
+    }
+    spread_inlier_kernel<BLOCK><<<g, BLOCK, 0, s>>>(w);
+}
+
+static void launch_spread(const SpreadArgs &w, int block, hipStream_t s) {
+    switch (block) {
+        case 64: launch_spread_t<64>(w, s); break;
+        case 128: launch_spread_t<128>(w, s); break;
+        case 512: launch_spread_t<512>(w, s); break;
+        case 1024: launch_spread_t<1024>(w, s); break;
+        default: launch_spread_t<256>(w, s); break;
+    }
+}
+
+static int norm_block(int b) {
+    return (b == 64 || b == 128 || b == 512 || b == 1024) ? b : 256;
+}
+
 void icp_batch(Workspace &ws, int nSrc, const float *sx, const float *sy, const float *sz,
                const float *snx, const float *sny, const float *snz, const GridView &dst,
                const GridView &full, const float *init12, int H, const IcpCriteria &c,
                float scoringDist, const IcpConfig &cfg, IcpResultDev *out, hipStream_t s) {
     if (H <= 0)
         return;
+    const int block = norm_block(cfg.block);
     IcpArgs a = make_args(ws, nSrc, sx, sy, sz, snx, sny, snz, dst, full, H, c, scoringDist,
                           cfg.mode);
     float *init = ws.icpPose.as<float>((size_t)H * 12);
@@ -723,29 +1012,56 @@ void icp_batch(Workspace &ws, int nSrc, const float *sx, const float *sy, const 
     a.init = init;
     IcpResultDev *od = ws.icpOut.as<IcpResultDev>(H);
     a.out = od;
+    SpreadArgs w;
+    std::memset(&w, 0, sizeof w);
+    if (cfg.spread) {
+        w.b      = a;
+        w.nChunk = div_up(std::max(nSrc, 1), block);
+        w.st     = ws.icpSt.as<IcpState>(H);
+        w.arrive = ws.icpArrive.as<uint32_t>((size_t)2 * H);
+        w.inl    = reinterpret_cast<int *>(w.arrive + H);
+        w.part   = ws.icpPart.as<CF>((size_t)H * w.nChunk * 27);
+    }
+    auto launch = [&] {
+        if (cfg.spread)
+            launch_spread(w, block, s);
+        else
+            launch_icp(a, block, s);
+    };
     if (cfg.graph) {
-        // the persistent launch captured once per shape and replayed
-        GraphCache &gc = ws.icpGraph;
-        bool same = gc.exec && gc.H == H && gc.nSrc == nSrc && gc.block == cfg.block &&
-                    std::memcmp(&gc.args, &a, sizeof a) == 0;
-        if (!same) {
-            if (gc.exec)
-                PPF_HIP_CHECK(hipGraphExecDestroy(gc.exec));
-            gc.exec = nullptr;
+        // the launch sequence captured once per shape and argument set, replayed after
+        static_assert(sizeof(SpreadArgs) <= sizeof(GraphCache::args), "graph key too small");
+        unsigned char key[sizeof(GraphCache::args)] = {};
+        if (cfg.spread)
+            std::memcpy(key, &w, sizeof w);
+        else
+            std::memcpy(key, &a, sizeof a);
+        GraphCache *gc = nullptr;
+        for (auto &e : ws.icpGraph)
+            if (e.exec && e.H == H && e.nSrc == nSrc && e.block == block &&
+                e.spread == cfg.spread && std::memcmp(e.args, key, sizeof key) == 0)
+                gc = &e;
+        if (!gc) {
+            gc = &ws.icpGraph[ws.icpGraphNext];
+            ws.icpGraphNext = (ws.icpGraphNext + 1) % (int)(sizeof ws.icpGraph / sizeof ws.icpGraph[0]);
+            if (gc->exec)
+                PPF_HIP_CHECK(hipGraphExecDestroy(gc->exec));
+            gc->exec = nullptr;
             hipGraph_t g;
             PPF_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-            launch_icp(a, cfg.block, s);
+            launch();
             PPF_HIP_CHECK(hipStreamEndCapture(s, &g));
-            PPF_HIP_CHECK(hipGraphInstantiate(&gc.exec, g, nullptr, nullptr, 0));
+            PPF_HIP_CHECK(hipGraphInstantiate(&gc->exec, g, nullptr, nullptr, 0));
             PPF_HIP_CHECK(hipGraphDestroy(g));
-            gc.H     = H;
-            gc.nSrc  = nSrc;
-            gc.block = cfg.block;
-            std::memcpy(&gc.args, &a, sizeof a);
+            gc->H      = H;
+            gc->nSrc   = nSrc;
+            gc->block  = block;
+            gc->spread = cfg.spread;
+            std::memcpy(gc->args, key, sizeof key);
         }
-        PPF_HIP_CHECK(hipGraphLaunch(gc.exec, s));
+        PPF_HIP_CHECK(hipGraphLaunch(gc->exec, s));
     } else {
-        launch_icp(a, cfg.block, s);
+        launch();
     }
     PPF_HIP_CHECK(hipGetLastError());
     if (out) {

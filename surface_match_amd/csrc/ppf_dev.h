@@ -9,7 +9,7 @@ namespace ppfd {
 // A captured launch and the argument bytes it was captured with.
 struct GraphCache {
     hipGraphExec_t exec = nullptr;
-    int            H = 0, nSrc = 0, block = 0;
+    int            H = 0, nSrc = 0, block = 0, spread = 0;
     unsigned char  args[1024];
     ~GraphCache() {
         if (exec)
@@ -30,8 +30,10 @@ struct Workspace {
     DevBuf clSortKey, clSortKey2, clSortVal, clSortVal2, clStatus, clPred, clPredOff,
         clCenters, clNms, clMembers;
     // icp
-    DevBuf icpPose, icpCur, icpState, icpHash, icpPartial, icpOut, icpInl;
-    GraphCache icpGraph;
+    DevBuf icpPose, icpCur, icpState, icpHash, icpPartial, icpOut, icpInl, icpSt, icpArrive,
+        icpPart;
+    GraphCache icpGraph[4];  // captured batched refinements, replaced round-robin
+    int        icpGraphNext = 0;
     // events bracketing the vote kernel (created on first use)
     hipEvent_t voteEv[2] = {nullptr, nullptr};
     ~Workspace() {
@@ -153,9 +155,11 @@ struct IcpResultDev {
     int   inliers;
 };
 struct IcpConfig {
-    int block = 256;
-    int mode  = 0;   // 0 symmetric (reference), 1 point-to-plane
-    int graph = 1;   // replay the persistent launch from a captured hipGraph
+    int block  = 256;
+    int mode   = 0;  // 0 symmetric (reference), 1 point-to-plane
+    int graph  = 1;  // replay the launch sequence from a captured hipGraph
+    int spread = 1;  // 1: hypotheses cut into chunks of `block` points over
+                     // the whole chip; 0: one persistent workgroup each
 };
 // Refine H hypotheses (poses12 host) of src against dst in one persistent
 // launch; scores by inliers against `full` within scoringDist.

@@ -47,7 +47,11 @@ class IcpTarget:
 
     def refine(self, src, src_nrm, init_poses, iterations=10, reject_dist=10.0, mse_min=0.0,
                mse_max=1e30, tolerance=0.0, mode=1, batched=True, block=0, scoring_dist=1.0):
-        """Returns (poses [H,4,4], results dict of arrays, device ms)."""
+        """Returns (poses [H,4,4], results dict of arrays, device ms).
+
+        batched: True / "spread" = every pose spread over workgroups of
+        `block` source points; "persistent" = one workgroup per pose;
+        False = the conventional one-pose-at-a-time refiner."""
         src, src_nrm = N.f32(src), N.f32(src_nrm)
         P = _pose16(init_poses)
         H = len(P)
@@ -55,7 +59,7 @@ class IcpTarget:
         out = (N.IcpResult * H)()
         ms = C.c_float(0)
         N.check(N.lib().ppf_icp_refine(self._h, N.ptr(src), N.ptr(src_nrm), len(src), N.ptr(P), H,
-                                       C.byref(crit), mode, int(batched), block, scoring_dist,
+                                       C.byref(crit), mode, _batched_code(batched), block, scoring_dist,
                                        out, C.byref(ms)))
         poses = np.array([np.array(o.pose16, np.float32).reshape(4, 4).T for o in out])
         res = {k: np.array([getattr(o, k) for o in out]) for k in
@@ -80,6 +84,16 @@ class IcpTarget:
         if with_corr:
             r["corr"] = corr
         return r
+
+
+def _batched_code(b):
+    if b is False or b == 0:
+        return 0
+    if b == "persistent":
+        return 2
+    if b is True or b == "spread" or b == 1:
+        return 1
+    raise ValueError(f"batched must be True/'spread', 'persistent' or False, not {b!r}")
 
 
 def point_to_plane_cost():
