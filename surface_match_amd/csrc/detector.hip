@@ -229,7 +229,7 @@ struct ppf_detector_s {
     int          refTail = 1;
     DevCloud     tmpScene;
     DevBuf       idxA, idxB, idxC, idxD;  // sampled / keypoints / dense / scratch
-    hipEvent_t   ev[11] = {};
+    hipEvent_t   ev[13] = {};
 
     ppf_detector_s() {
         PPF_HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
@@ -520,7 +520,6 @@ void match_impl(ppf_detector_s &D, DevCloud &c, float rel, float kpFrac, float m
                   D.ws.gSparse.view, D.ws.gFull.view, init.data(), (int)cand.size(), sc,
                   scoringDist, D.icfg, res.data(), s);
         PPF_HIP_CHECK(hipEventRecord(D.ev[10], s));
-        st->msIcpSparse = elapsed(D.ev[9], D.ev[10]);
         nSparse = (int)cand.size();
         std::vector<int> survivors;
         for (size_t k = 0; k < cand.size(); k++) {
@@ -548,13 +547,12 @@ void match_impl(ppf_detector_s &D, DevCloud &c, float rel, float kpFrac, float m
             for (size_t k = 0; k < survivors.size(); k++)
                 std::memcpy(&init2[12 * k], &poseNow[12 * (size_t)survivors[k]], sizeof(float) * 12);
             IcpConfig dcfg = D.icfg;
-            PPF_HIP_CHECK(hipEventRecord(D.ev[9], s));
+            PPF_HIP_CHECK(hipEventRecord(D.ev[11], s));
             icp_batch(D.ws, hm.reSampled.size(), dm.rx.get<float>(), dm.ry.get<float>(),
                       dm.rz.get<float>(), dm.rnx.get<float>(), dm.rny.get<float>(),
                       dm.rnz.get<float>(), D.ws.gDense.view, D.ws.gFull.view, init2.data(),
                       (int)survivors.size(), dc, scoringDist, dcfg, res2.data(), s);
-            PPF_HIP_CHECK(hipEventRecord(D.ev[10], s));
-            st->msIcpDense = elapsed(D.ev[9], D.ev[10]);
+            PPF_HIP_CHECK(hipEventRecord(D.ev[12], s));
             nDenseIcp = (int)survivors.size();
             for (size_t k = 0; k < survivors.size(); k++) {
                 int i = survivors[k];
@@ -611,6 +609,8 @@ void match_impl(ppf_detector_s &D, DevCloud &c, float rel, float kpFrac, float m
     st->msIcpPrepare    = elapsed(D.ev[6], D.ev[7]);
     st->msIcp           = elapsed(D.ev[7], D.ev[8]);
     st->msTotal         = elapsed(D.ev[0], D.ev[8]);
+    st->msIcpSparse     = nSparse ? elapsed(D.ev[9], D.ev[10]) : 0.0f;
+    st->msIcpDense      = nDenseIcp ? elapsed(D.ev[11], D.ev[12]) : 0.0f;
 }
 
 void write_poses(const MatchOut &o, float *poses16, float *scores, size_t cap, size_t *nOut) {
