@@ -132,22 +132,29 @@ PPF_API int ppf_match_scene_resident(ppf_detector d, ppf_scene s, float sampling
                                      const ppf_match_param *param, float *poses16, float *scores,
                                      size_t cap, size_t *nOut, ppf_match_stats *stats);
 
-/* -- autotuning (no reference counterpart): times the vote kernel over
- * workgroup sizes {256, 512, 1024} x XCD-aware key-point mapping {off, on}
- * and the batched ICP over {spread, persistent} x {64, 128, 256, 512, 1024}
- * threads per workgroup on the given resident scene, and keeps the fastest
- * in the detector. -- */
+/* -- launch tuning (no reference counterpart).  ppf_autotune times the
+ * vote kernel over workgroup sizes {256, 512, 1024} x XCD-aware key-point
+ * mapping {off, on} x vote-grid cell {D/2, D/3, D/4}, then the batched ICP
+ * over {spread, persistent} x {64, 128, 256, 512, 1024} threads per
+ * workgroup, on the given resident scene, and keeps the fastest in the
+ * detector.  None of these changes any result. -- */
 typedef struct {
-    int   voteBlock, voteXcdRemap, icpBlock;
-    float voteMs, icpMs;
-    int   configsTried;
-    int   icpSpread; /* 1: each hypothesis spread over workgroups, 0: persistent */
+    int voteBlock;    /* threads per vote workgroup: 256, 512 or 1024 */
+    int voteXcdRemap; /* 1: consecutive key points stay on one XCD */
+    int voteCellDiv;  /* vote grid cell = model diameter / voteCellDiv: 2, 3 or 4 */
+    int icpBlock;     /* threads per ICP workgroup: 64 .. 1024, a power of two */
+    int icpSpread;    /* 1: each hypothesis spread over workgroups, 0: persistent */
+} ppf_tuning;
+typedef struct {
+    ppf_tuning best;
+    float      voteMs, icpMs; /* device time of the vote kernel / both refinements */
+    int        configsTried;
 } ppf_tune_report;
 PPF_API int ppf_autotune(ppf_detector d, ppf_scene s, float samplingDistanceRel,
                          float keyPointFraction, float minScore, const ppf_match_param *param,
                          ppf_tune_report *report);
-PPF_API int ppf_set_tuning(ppf_detector d, int voteBlock, int voteXcdRemap, int icpBlock,
-                           int icpSpread);
+PPF_API int ppf_get_tuning(ppf_detector d, ppf_tuning *out);
+PPF_API int ppf_set_tuning(ppf_detector d, const ppf_tuning *t);
 
 /* -- trained-model introspection (the IMPL fields, privateType.h:56-66) -- */
 typedef struct {

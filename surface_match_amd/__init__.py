@@ -147,6 +147,22 @@ class Detector:
         # column-major 4x4 -> row-major numpy
         return poses[:k].reshape(k, 4, 4).transpose(0, 2, 1).copy(), scores[:k].copy()
 
+    def get_tuning(self) -> dict:
+        """Launch shapes in use (vote block / XCD mapping / grid cell, ICP block / spread)."""
+        t = N.Tuning()
+        N.check(N.lib().ppf_get_tuning(self._h, C.byref(t)))
+        return t.as_dict()
+
+    def set_tuning(self, **kw):
+        """Change launch shapes; results do not depend on them."""
+        t = N.Tuning()
+        N.check(N.lib().ppf_get_tuning(self._h, C.byref(t)))
+        for k, v in kw.items():
+            if k not in t.as_dict():
+                raise KeyError(k)
+            setattr(t, k, int(v))
+        N.check(N.lib().ppf_set_tuning(self._h, C.byref(t)))
+
     def save(self, filename: str):
         N.check(N.lib().ppf_save(self._h, str(filename).encode()))
 
@@ -225,7 +241,8 @@ class ResidentScene:
         rep = N.TuneReport()
         N.check(N.lib().ppf_autotune(self._det._h, self._h, samplingDistanceRel, keyPointFraction,
                                      minScore, C.byref(param), C.byref(rep)))
-        return {k: getattr(rep, k) for k, _ in rep._fields_}
+        return {"best": rep.best.as_dict(), "voteMs": rep.voteMs, "icpMs": rep.icpMs,
+                "configsTried": rep.configsTried}
 
     def __del__(self):
         h = getattr(self, "_h", None)

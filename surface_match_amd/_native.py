@@ -69,10 +69,17 @@ class IcpResult(C.Structure):
                 ("inliers", C.c_int)]
 
 
+class Tuning(C.Structure):
+    _fields_ = [(n, C.c_int) for n in ("voteBlock", "voteXcdRemap", "voteCellDiv", "icpBlock",
+                                       "icpSpread")]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class TuneReport(C.Structure):
-    _fields_ = [("voteBlock", C.c_int), ("voteXcdRemap", C.c_int), ("icpBlock", C.c_int),
-                ("voteMs", C.c_float), ("icpMs", C.c_float), ("configsTried", C.c_int),
-                ("icpSpread", C.c_int)]
+    _fields_ = [("best", Tuning), ("voteMs", C.c_float), ("icpMs", C.c_float),
+                ("configsTried", C.c_int)]
 
 
 _lib = None
@@ -86,7 +93,7 @@ EXPORTS = [
     "ppf_model_get_table", "ppf_pair_features", "ppf_vote", "ppf_voxel_sample",
     "ppf_radius_search", "ppf_knn_search", "ppf_estimate_normals", "ppf_device_math",
     "ppf_icp_target_create", "ppf_icp_target_destroy", "ppf_icp_refine", "ppf_icp_adjoint",
-    "ppf_autotune", "ppf_set_tuning",
+    "ppf_autotune", "ppf_set_tuning", "ppf_get_tuning",
 ]
 
 
@@ -131,7 +138,8 @@ def lib() -> C.CDLL:
                                vp, C.POINTER(f)]),
         "ppf_icp_adjoint": (i, [vp, vp, sz, vp, vp, sz, f, i, vp, vp]),
         "ppf_autotune": (i, [vp, vp, f, f, f, C.POINTER(MatchParam), C.POINTER(TuneReport)]),
-        "ppf_set_tuning": (i, [vp, i, i, i, i]),
+        "ppf_set_tuning": (i, [vp, C.POINTER(Tuning)]),
+        "ppf_get_tuning": (i, [vp, C.POINTER(Tuning)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

@@ -465,7 +465,7 @@ void match_impl(ppf_detector_s &D, DevCloud &c, float rel, float kpFrac, float m
                            mr->nSampled);
     }
     // vote (ppf.cpp:247-411)
-    grid_build(D.ws.gVote, PX, PY, PZ, NX, NY, NZ, samp, ns, Dm * 0.5f, s);
+    grid_build(D.ws.gVote, PX, PY, PZ, NX, NY, NZ, samp, ns, Dm / (float)D.vcfg.cellDiv, s);
     VoteStats vs{};
     PPF_HIP_CHECK(hipEventRecord(D.ev[4], s));
     int np = vote_and_poses(D.ws, dm, D.ws.gVote.view, PX, PY, PZ, NX, NY, NZ, kp, nk,
@@ -1009,16 +1009,37 @@ PPF_API int ppf_match_scene_resident(ppf_detector d, ppf_scene sc, float rel, fl
     });
 }
 
-PPF_API int ppf_set_tuning(ppf_detector d, int voteBlock, int voteXcdRemap, int icpBlock,
-                           int icpSpread) {
-    PPF_REQUIRE(d, "null detector");
-    PPF_REQUIRE(voteBlock == 256 || voteBlock == 512 || voteBlock == 1024, "vote block must be 256, 512 or 1024");
-    PPF_REQUIRE(icpBlock == 64 || icpBlock == 128 || icpBlock == 256 || icpBlock == 512 || icpBlock == 1024,
+namespace {
+ppf_tuning tuning_of(const ppf_detector_s &d) {
+    ppf_tuning t;
+    t.voteBlock    = d.vcfg.block;
+    t.voteXcdRemap = d.vcfg.xcdRemap;
+    t.voteCellDiv  = d.vcfg.cellDiv;
+    t.icpBlock     = d.icfg.block;
+    t.icpSpread    = d.icfg.spread;
+    return t;
+}
+}  // namespace
+
+PPF_API int ppf_get_tuning(ppf_detector d, ppf_tuning *out) {
+    PPF_REQUIRE(d && out, "null argument");
+    *out = tuning_of(*d);
+    return PPF_OK;
+}
+
+PPF_API int ppf_set_tuning(ppf_detector d, const ppf_tuning *t) {
+    PPF_REQUIRE(d && t, "null argument");
+    PPF_REQUIRE(t->voteBlock == 256 || t->voteBlock == 512 || t->voteBlock == 1024,
+                "vote block must be 256, 512 or 1024");
+    PPF_REQUIRE(t->voteCellDiv >= 2 && t->voteCellDiv <= 4, "vote cell divisor must be 2, 3 or 4");
+    PPF_REQUIRE(t->icpBlock == 64 || t->icpBlock == 128 || t->icpBlock == 256 ||
+                    t->icpBlock == 512 || t->icpBlock == 1024,
                 "icp block must be 64..1024, a power of two");
-    d->vcfg.block    = voteBlock;
-    d->vcfg.xcdRemap = voteXcdRemap ? 1 : 0;
-    d->icfg.block    = icpBlock;
-    d->icfg.spread   = icpSpread ? 1 : 0;
+    d->vcfg.block    = t->voteBlock;
+    d->vcfg.xcdRemap = t->voteXcdRemap ? 1 : 0;
+    d->vcfg.cellDiv  = t->voteCellDiv;
+    d->icfg.block    = t->icpBlock;
+    d->icfg.spread   = t->icpSpread ? 1 : 0;
     return PPF_OK;
 }
 
@@ -1046,17 +1067,19 @@ PPF_API int ppf_autotune(ppf_detector d, ppf_scene sc, float rel, float kpFrac, 
             }
         };
         for (int block : {256, 512, 1024})
-            for (int remap : {0, 1}) {
-                d->vcfg.block    = block;
-                d->vcfg.xcdRemap = remap;
-                float v, i;
-                run(v, i);
-                tried++;
-                if (v < bestVoteMs) {
-                    bestVoteMs = v;
-                    bestV      = d->vcfg;
+            for (int remap : {0, 1})
+                for (int div : {2, 3, 4}) {
+                    d->vcfg.block    = block;
+                    d->vcfg.xcdRemap = remap;
+                    d->vcfg.cellDiv  = div;
+                    float v, i;
+                    run(v, i);
+                    tried++;
+                    if (v < bestVoteMs) {
+                        bestVoteMs = v;
+                        bestV      = d->vcfg;
+                    }
                 }
-            }
         d->vcfg = bestV;
         for (int spread : {1, 0})
             for (int block : {64, 128, 256, 512, 1024}) {
@@ -1071,10 +1094,7 @@ PPF_API int ppf_autotune(ppf_detector d, ppf_scene sc, float rel, float kpFrac, 
                 }
             }
         d->icfg               = bestI;
-        rep->voteBlock        = bestV.block;
-        rep->voteXcdRemap     = bestV.xcdRemap;
-        rep->icpBlock         = bestI.block;
-        rep->icpSpread        = bestI.spread;
+        rep->best             = tuning_of(*d);
         rep->voteMs           = bestVoteMs;
         rep->icpMs            = bestIcpMs;
         rep->configsTried     = tried;
@@ -1176,8 +1196,8 @@ PPF_API int ppf_vote(ppf_detector d, const float *pts, const float *nrm, size_t 
         PPF_HIP_CHECK(hipMemcpyAsync(samp, all.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
         PPF_HIP_CHECK(hipMemcpyAsync(kp, keypoints, sizeof(int) * nk, hipMemcpyHostToDevice, s));
         grid_build(d->ws.gVote, c.px.get<float>(), c.py.get<float>(), c.pz.get<float>(),
-                   c.nx.get<float>(), c.ny.get<float>(), c.nz.get<float>(), samp, (int)n, Dm * 0.5f,
-                   s);
+                   c.nx.get<float>(), c.ny.get<float>(), c.nz.get<float>(), samp, (int)n,
+                   Dm / (float)d->vcfg.cellDiv, s);
         DevBuf dump;
         int32_t *dd = nullptr;
         if (accDump && dumpCount) {

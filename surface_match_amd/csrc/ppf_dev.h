@@ -117,8 +117,9 @@ struct VoteStats {
 };
 
 struct VoteConfig {
-    int block       = 1024;  // threads per workgroup
-    int xcdRemap    = 1;     // keep spatially adjacent keypoints on one XCD
+    int block    = 1024;  // threads per workgroup
+    int xcdRemap = 0;     // keep spatially adjacent keypoints on one XCD
+    int cellDiv  = 4;     // vote grid cell = model diameter / cellDiv (2..4)
 };
 
 // Vote every keypoint (indices into the sampled-scene grid's clouds),

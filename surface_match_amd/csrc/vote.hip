@@ -22,9 +22,10 @@
 
 namespace ppfd {
 
-constexpr int VOTE_MAX_CELLS = 512;
+constexpr int VOTE_AXIS_CELLS = 11;  // cells per axis over the vote sphere (host check)
+constexpr int VOTE_MAX_CELLS  = VOTE_AXIS_CELLS * VOTE_AXIS_CELLS * VOTE_AXIS_CELLS;
 constexpr int SEG            = 8;  // bucket entries voted by one lane
-constexpr int VOTE_STATIC_LDS = 24 * 1024;   // bound on the kernel's static LDS
+constexpr int VOTE_STATIC_LDS = 44 * 1024;   // bound on the kernel's static LDS (1024 threads)
 constexpr int LDS_LIMIT       = 160 * 1024;  // gfx950 LDS per workgroup
 
 int vote_max_tile_refs(int angleNum, int block) {
@@ -81,22 +82,96 @@ __device__ __forceinline__ void top3_insert(unsigned long long c, unsigned long 
     }
 }
 
+// Vote one queue of hits (bucket start, length, scene alpha) held by this
+// wave: each bucket is cut into segments of SEG entries and the wave's lanes
+// take segments in turn.  The reference's tail rule covers the last len % 8
+// entries, which start at a multiple of 8, so a segment is entirely body or
+// entirely tail.  Lane-uniform control flow except inside the segment loop.
+template <int QCAP>
+__device__ __forceinline__ void wave_vote(const VoteArgs &a, uint32_t *acc, int r0, int r1,
+                                          const uint32_t *qStart, const uint32_t *qLen,
+                                          const float *qAlpha, int *qPre, int qn,
+                                          unsigned long long &nVotes) {
+    const int lane = threadIdx.x & 63;
+    int       s0 = 0, s1 = 0;
+    if (lane < qn)
+        s0 = (int)((qLen[lane] + SEG - 1) / SEG);
+    if (lane + 64 < qn)
+        s1 = (int)((qLen[lane + 64] + SEG - 1) / SEG);
+    int x0 = s0, x1 = s1;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        int y0 = __shfl_up(x0, o, 64), y1 = __shfl_up(x1, o, 64);
+        if (lane >= o) {
+            x0 += y0;
+            x1 += y1;
+        }
+    }
+    const int tot0     = __shfl(x0, 63, 64);
+    const int totalSeg = tot0 + __shfl(x1, 63, 64);
+    if (lane < qn)
+        qPre[lane] = x0 - s0;
+    if (lane + 64 < qn)
+        qPre[lane + 64] = tot0 + x1 - s1;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    for (int t = lane; t < totalSeg; t += 64) {
+        int lo = 0, hi = qn - 1;  // queue entry owning segment t
+        while (lo < hi) {
+            int mid = (lo + hi + 1) >> 1;
+            if (qPre[mid] <= t)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        const int    L    = (int)qLen[lo];
+        const int    e0   = (t - qPre[lo]) * SEG;
+        const int    cnt  = min(SEG, L - e0);
+        const bool   tail = e0 >= L - (L & 7);
+        const float  aS   = qAlpha[lo];
+        const uint2 *ent  = a.entries + qStart[lo] + e0;
+        uint2        v[SEG];
+#pragma unroll
+        for (int k = 0; k < SEG; k++)
+            if (k < cnt)
+                v[k] = ent[k];
+#pragma unroll
+        for (int k = 0; k < SEG; k++) {
+            if (k >= cnt)
+                break;
+            int ref = (int)v[k].x;
+            if (ref >= r0 && ref < r1) {
+                int bin = ppfc::angle_bin(__uint_as_float(v[k].y), aS, a.sMaxId, a.angleNum, tail);
+                atomicAdd(&acc[(ref - r0) * a.angleNum + bin], 1u);
+                nVotes++;
+            }
+        }
+    }
+    // the queue is refilled after this point
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 template <int BLOCK>
 __global__ void __launch_bounds__(BLOCK) vote_kernel(VoteArgs a) {
+    constexpr int NW   = BLOCK / 64;
+    constexpr int QCAP = 128;  // >= 63 queued + 64 new hits
+    constexpr int CPT  = (VOTE_MAX_CELLS + BLOCK - 1) / BLOCK;
     extern __shared__ uint32_t acc[];
-    __shared__ int   sCellStart[VOTE_MAX_CELLS];
-    __shared__ int   sCellPre[VOTE_MAX_CELLS + 1];
-    __shared__ int   sPPre[BLOCK + 1];
-    __shared__ uint32_t sPStart[BLOCK];
-    __shared__ int   sPLen[BLOCK];
-    __shared__ float sPAlpha[BLOCK];
-    __shared__ int   sScan[BLOCK / 64 + 1];
-    __shared__ unsigned long long sTop[(BLOCK / 64) * 3];
-    __shared__ int   sRed[BLOCK / 64];
+    __shared__ int      sCellStart[VOTE_MAX_CELLS];
+    __shared__ int      sCellPre[VOTE_MAX_CELLS + 1];
+    __shared__ uint32_t qStart[NW][QCAP];
+    __shared__ uint32_t qLen[NW][QCAP];
+    __shared__ float    qAlpha[NW][QCAP];
+    __shared__ int      qPre[NW][QCAP];
+    __shared__ int      sScan[NW + 1];
+    __shared__ int      sNext;
+    __shared__ unsigned long long sTop[NW * 3];
+    __shared__ int      sRed[NW];
 
-    const int tid = threadIdx.x;
-    // block -> (keypoint, tile); optionally keep contiguous keypoint ranges on
-    // one XCD (consecutive workgroups are dealt round-robin to the 8 XCDs).
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     // Workgroups are dealt round-robin to the 8 XCDs (each with its own L2).
     // With xcdRemap, XCD x works through a contiguous run of key points, so
     // neighbouring key points -- which read the same scene cells and similar
@@ -133,61 +208,80 @@ __global__ void __launch_bounds__(BLOCK) vote_kernel(VoteArgs a) {
     for (int i = tid; i < nrTile * a.angleNum; i += BLOCK)
         acc[i] = 0u;
 
-    // ---- cells overlapping the vote sphere
+    // ---- cells overlapping the vote sphere (any order: voting commutes)
     const GridView &g = a.g;
     int xlo = cell_coord(qx - a.searchR, g.ox, g.cell, g.dx), xhi = cell_coord(qx + a.searchR, g.ox, g.cell, g.dx);
     int ylo = cell_coord(qy - a.searchR, g.oy, g.cell, g.dy), yhi = cell_coord(qy + a.searchR, g.oy, g.cell, g.dy);
     int zlo = cell_coord(qz - a.searchR, g.oz, g.cell, g.dz), zhi = cell_coord(qz + a.searchR, g.oz, g.cell, g.dz);
     int ncx = xhi - xlo + 1, ncy = yhi - ylo + 1, ncz = zhi - zlo + 1;
-    int nCells = ncx * ncy * ncz;  // <= VOTE_MAX_CELLS by construction of the grid
-    int myCnt = 0, myStart = 0;
-    if (tid < nCells) {
-        int cx = xlo + tid % ncx, cy = ylo + (tid / ncx) % ncy, cz = zlo + tid / (ncx * ncy);
-        if (cell_min_d2(g, cx, cy, cz, qx, qy, qz) <= a.D2) {
-            int s0, c0;
-            if (cell_lookup(g, cx, cy, cz, s0, c0)) {
-                myStart = s0;
-                myCnt   = c0;
+    int nCells = ncx * ncy * ncz;  // <= VOTE_MAX_CELLS: the host bounds cells per axis
+    int cStart[CPT], cCnt[CPT], mySum = 0;
+#pragma unroll
+    for (int j = 0; j < CPT; j++) {
+        int ci   = tid * CPT + j;
+        cStart[j] = 0;
+        cCnt[j]   = 0;
+        if (ci < nCells) {
+            int cx = xlo + ci % ncx, cy = ylo + (ci / ncx) % ncy, cz = zlo + ci / (ncx * ncy);
+            if (cell_min_d2(g, cx, cy, cz, qx, qy, qz) <= a.D2) {
+                int s0, c0;
+                if (cell_lookup(g, cx, cy, cz, s0, c0)) {
+                    cStart[j] = s0;
+                    cCnt[j]   = c0;
+                }
             }
         }
+        mySum += cCnt[j];
     }
     int total;
-    int pre = block_excl_scan<BLOCK>(myCnt, sScan, total);
-    if (tid < nCells) {
-        sCellStart[tid] = myStart;
-        sCellPre[tid]   = pre;
+    int pre = block_excl_scan<BLOCK>(mySum, sScan, total);
+#pragma unroll
+    for (int j = 0; j < CPT; j++) {
+        int ci = tid * CPT + j;
+        if (ci < nCells) {
+            sCellStart[ci] = cStart[j];
+            sCellPre[ci]   = pre;
+        }
+        pre += cCnt[j];
     }
-    if (tid == 0)
+    if (tid == 0) {
         sCellPre[nCells] = total;
+        sNext            = 0;
+    }
     __syncthreads();
 
-    auto slot_of = [&](int c) -> int {
-        int lo = 0, hi = nCells - 1;  // largest cell with sCellPre[cell] <= c
-        while (lo < hi) {
-            int mid = (lo + hi + 1) >> 1;
-            if (sCellPre[mid] <= c)
-                lo = mid;
-            else
-                hi = mid - 1;
-        }
-        return sCellStart[lo] + (c - sCellPre[lo]);
-    };
-
-    // ---- pairs + votes, one chunk of BLOCK candidates at a time.  The
-    // neighbour count of the reference's skip rule (radiusSearch incl. self
-    // < voteThreshold, ppf.cpp:273-276) is taken in the same pass; a key
-    // point that fails it has its accumulator discarded below.
+    // ---- pairs + votes.  Each wave takes chunks of 64 candidates from a
+    // shared counter, queues its hits and votes them once 64 are queued, with
+    // no workgroup barrier until all candidates are done.  The neighbour
+    // count of the reference's skip rule (radiusSearch incl. self <
+    // voteThreshold, ppf.cpp:273-276) is taken in the same pass; a key point
+    // that fails it has its accumulator discarded below.
+    const int nChunks = (total + 63) >> 6;
     unsigned long long nPairs = 0, nHits = 0, nVotes = 0;
     int inSphere = 0;
-    for (int base = 0; base < total; base += BLOCK) {
-        int      c   = base + tid;
-        int      hit = 0;
-        uint32_t bStart = 0, bLen = 0;
-        float    alphaS = 0.0f;
-        if (c < total) {
-            int   s  = slot_of(c);
-            float px = g.x[s], py = g.y[s], pz = g.z[s];
-            bool  in = ppfc::dist2(qx, qy, qz, px, py, pz) < a.D2;
+    int qn       = 0;  // wave-uniform queue length
+    for (;;) {
+        int t = 0;
+        if (lane == 0)
+            t = atomicAdd(&sNext, 1);
+        t               = __shfl(t, 0, 64);
+        const bool more = t < nChunks;
+        int        hit  = 0;
+        uint32_t   bStart = 0, bLen = 0;
+        float      alphaS = 0.0f;
+        const int  c      = t * 64 + lane;
+        if (more && c < total) {
+            int lo = 0, hi = nCells - 1;  // largest cell with sCellPre[cell] <= c
+            while (lo < hi) {
+                int mid = (lo + hi + 1) >> 1;
+                if (sCellPre[mid] <= c)
+                    lo = mid;
+                else
+                    hi = mid - 1;
+            }
+            const int s  = sCellStart[lo] + (c - sCellPre[lo]);
+            float     px = g.x[s], py = g.y[s], pz = g.z[s];
+            bool      in = ppfc::dist2(qx, qy, qz, px, py, pz) < a.D2;
             inSphere += in ? 1 : 0;
             if (in && g.id[s] != kpId) {
                 nPairs++;
@@ -198,57 +292,27 @@ __global__ void __launch_bounds__(BLOCK) vote_kernel(VoteArgs a) {
                     hit = 1;
             }
         }
-        // Each hit's bucket is cut into segments of SEG entries; a lane
-        // votes one whole segment.  The reference's tail rule covers the
-        // last len % 8 entries, which start at a multiple of 8, so a segment
-        // is either entirely body or entirely tail.
-        int nHit, totalSeg;
-        int nSeg = hit ? (int)((bLen + SEG - 1) / SEG) : 0;
-        int pos  = block_excl_scan<BLOCK>(hit, sScan, nHit);
-        int spre = block_excl_scan<BLOCK>(nSeg, sScan, totalSeg);
+        const unsigned long long m = __ballot(hit);
         if (hit) {
-            sPStart[pos] = bStart;
-            sPPre[pos]   = spre;
-            sPAlpha[pos] = alphaS;
-            sPLen[pos]   = (int)bLen;
+            const int pos       = qn + __popcll(m & ((1ull << lane) - 1ull));
+            qStart[wv][pos]     = bStart;
+            qLen[wv][pos]       = bLen;
+            qAlpha[wv][pos]     = alphaS;
         }
-        __syncthreads();
         nHits += hit;
-        for (int t = tid; t < totalSeg; t += BLOCK) {
-            int lo = 0, hi = nHit - 1;  // hit owning segment t
-            while (lo < hi) {
-                int mid = (lo + hi + 1) >> 1;
-                if (sPPre[mid] <= t)
-                    lo = mid;
-                else
-                    hi = mid - 1;
-            }
-            const int   L     = sPLen[lo];
-            const int   e0    = (t - sPPre[lo]) * SEG;
-            const int   cnt   = min(SEG, L - e0);
-            const bool  tail  = e0 >= L - (L & 7);
-            const float aS    = sPAlpha[lo];
-            const uint2 *ent  = a.entries + sPStart[lo] + e0;
-            uint2 v[SEG];
-#pragma unroll
-            for (int k = 0; k < SEG; k++)
-                if (k < cnt)
-                    v[k] = ent[k];
-#pragma unroll
-            for (int k = 0; k < SEG; k++) {
-                if (k >= cnt)
-                    break;
-                int ref = (int)v[k].x;
-                if (ref >= r0 && ref < r1) {
-                    int bin = ppfc::angle_bin(__uint_as_float(v[k].y), aS, a.sMaxId, a.angleNum,
-                                              tail);
-                    atomicAdd(&acc[(ref - r0) * a.angleNum + bin], 1u);
-                    nVotes++;
-                }
-            }
+        qn += __popcll(m);
+        if (qn >= 64 || (!more && qn > 0)) {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            wave_vote<QCAP>(a, acc, r0, r1, qStart[wv], qLen[wv], qAlpha[wv], qPre[wv], qn,
+                            nVotes);
+            qn = 0;
         }
-        __syncthreads();
+        if (!more)
+            break;
     }
+    __syncthreads();
 
     for (int o = 32; o > 0; o >>= 1)
         inSphere += __shfl_xor(inSphere, o, 64);
@@ -415,8 +479,11 @@ int vote_and_poses(Workspace &ws, const DevModel &m, const GridView &g, const fl
     tileRefs     = div_up(m.nSampled, nTiles);
     float cell   = g.cell;
     float D      = m.diameter;
-    if ((int)((2.0f * D) / cell + 3) > 8)
-        throw std::runtime_error("vote grid cell is smaller than half the model diameter");
+    float searchR = D * 1.0001f + 1e-3f * cell;
+    // cells per axis over [q - R, q + R]: floor(2R / cell) + 2, plus one for
+    // rounding of the two end coordinates
+    if ((int)std::floor(2.0 * searchR / cell) + 3 > VOTE_AXIS_CELLS)
+        throw std::runtime_error("vote grid cell is smaller than a quarter of the model diameter");
     VoteArgs a;
     a.g  = g;
     a.PX = PX;
@@ -438,7 +505,7 @@ int vote_and_poses(Workspace &ws, const DevModel &m, const GridView &g, const fl
     a.distStep  = m.distStep;
     a.sMaxId    = (float)(m.angleNum - 1) / ppfc::kTwoPi;
     a.D2        = D * D;            // ppf.cpp:191
-    a.searchR   = D * 1.0001f + 1e-3f * cell;
+    a.searchR   = searchR;
     a.voteThreshold = voteThreshold;
     a.cand  = ws.cand.as<unsigned long long>((size_t)nk * nTiles * 3);
     a.stats = nullptr;

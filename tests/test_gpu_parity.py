@@ -157,24 +157,37 @@ def test_l4_vote_accumulators_bitwise(trained, oracle_mod, workload1):
     ot = O.OracleTable(sp, sn, ang, dist)
     r = rng(5)
     kps = np.sort(r.choice(len(pts), 48, replace=False)).astype(np.int32)
-    cand, poses, acc = trained.vote(pts, nrm, kps, minScore=0.2, dump=len(kps))
     refNum = info.nSampled
     thr = np.float32(refNum) * np.float32(0.2)
     D2 = D * D
-    for t, k in enumerate(kps):
+    expect = []
+    for k in kps:
         d = pts - pts[k]
         d2 = ((np.float32(0) + d[:, 0] * d[:, 0]) + d[:, 1] * d[:, 1]) + d[:, 2] * d[:, 2]
         nb = np.nonzero(d2 < D2)[0]
         if np.float32(len(nb)) < thr:
-            assert (cand[t] == -1).all()
+            expect.append(None)
             continue
         nb = nb[nb != k]
         oacc = O.vote_keypoint(ot, refNum, 30, pts[k], nrm[k], pts[nb], nrm[nb], ang, dist)
-        assert np.array_equal(acc[t], oacc[:, 1:]), f"keypoint {k}"
-        assert np.array_equal(acc[t].sum(1), oacc[:, 0])
-        pk = O.peaks(oacc, refNum, 30, thr)
-        got = [tuple(c) for c in cand[t] if c[0] >= 0]
-        assert got == [(a, b, int(v)) for a, b, v in pk], f"keypoint {k}"
+        expect.append((oacc, O.peaks(oacc, refNum, 30, thr)))
+    # every launch shape (workgroup size, XCD mapping, grid cell) votes the same
+    base = trained.get_tuning()
+    try:
+        for vb, remap, div in ((1024, 0, 4), (256, 1, 2), (512, 0, 3), (1024, 1, 2)):
+            trained.set_tuning(voteBlock=vb, voteXcdRemap=remap, voteCellDiv=div)
+            cand, poses, acc = trained.vote(pts, nrm, kps, minScore=0.2, dump=len(kps))
+            for t, k in enumerate(kps):
+                if expect[t] is None:
+                    assert (cand[t] == -1).all()
+                    continue
+                oacc, pk = expect[t]
+                assert np.array_equal(acc[t], oacc[:, 1:]), f"keypoint {k} tuning {vb, remap, div}"
+                assert np.array_equal(acc[t].sum(1), oacc[:, 0])
+                got = [tuple(c) for c in cand[t] if c[0] >= 0]
+                assert got == [(a, b, int(v)) for a, b, v in pk], f"keypoint {k}"
+    finally:
+        trained.set_tuning(**base)
 
 
 def test_l2_voxel_and_search(N, oracle_mod, workload1):
@@ -278,3 +291,31 @@ def test_save_load_roundtrip(trained, tmp_path):
     f2 = tmp_path / "m2.model"
     d2.save(str(f2))
     assert f.read_bytes() == f2.read_bytes()
+
+
+def test_autotune_keeps_results(trained, workload1):
+    """The autotuner only changes launch shapes: votes are identical and the
+    refined poses agree to rounding."""
+    from surface_match_amd import PointCloud, MatchParam, ResidentScene, synth
+    sc = workload1["scene"]
+    cloud = PointCloud(sc.points)
+    cloud.viewPoint = np.array(synth.VIEWPOINT, np.float32)
+    rs = ResidentScene(trained, cloud)
+    base = trained.get_tuning()
+    try:
+        p0, s0, st0 = rs.match(0.04, 0.1, 0.2, MatchParam(numMatches=3))
+        rep = rs.autotune(0.04, 0.1, 0.2, MatchParam(numMatches=3))
+        assert rep["configsTried"] == 3 * 2 * 3 + 2 * 5
+        assert trained.get_tuning() == rep["best"]
+        p1, s1, st1 = rs.match(0.04, 0.1, 0.2, MatchParam(numMatches=3))
+        for k in ("nPairs", "nKeyHits", "nVotes", "nPoses", "nHypotheses"):
+            assert st0[k] == st1[k], k
+        assert len(p0) == len(p1)
+        D = trained.model_info().diameter
+        for a, b in zip(p0, p1):
+            assert synth.rotation_angle_deg(a[:3, :3].T @ b[:3, :3]) < 0.05
+            assert np.linalg.norm(a[:3, 3] - b[:3, 3]) < 5e-4 * D
+        assert np.allclose(s0, s1, atol=2.0 / trained.model_info().nReSampled)
+        print("autotune", rep)
+    finally:
+        trained.set_tuning(**base)
