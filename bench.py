@@ -130,7 +130,7 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--vote-block", type=int, default=0)
+    ap.add_argument("--no-icp-batching", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the PPF path runs on a single MI355X "
@@ -206,7 +206,8 @@ def main():
                      "kernel_ms": round(vote_ms, 4)},
         "results": found,
     }
-    out["icp_batching"] = icp_batching(det)
+    if not args.no_icp_batching:
+        out["icp_batching"] = icp_batching(det)
     if not args.no_cpu_baseline:
         threads = min(os.cpu_count() or 1, 8)
         cms, cst = cpu_baseline(w, threads)

@@ -47,16 +47,13 @@ PPF_HD float atanf_c(float xx) {
         neg = true;
         x   = -x;
     }
-    float y;
-    if (x > 2.414213562373095f) {
-        y = kPiO2;
-        x = -(1.0f / x);
-    } else if (x > 0.4142135623730950f) {
-        y = kPiO4;
-        x = (x - 1.0f) / (x + 1.0f);
-    } else {
-        y = 0.0f;
-    }
+    // the reductions -1/x (x > tan 3pi/8) and (x-1)/(x+1) (x > tan pi/8) as
+    // one division; -1/x == -(1/x) and x/1 == x exactly under round to nearest
+    const bool big = x > 2.414213562373095f, mid = x > 0.4142135623730950f;
+    float      y   = big ? kPiO2 : (mid ? kPiO4 : 0.0f);
+    float      num = big ? -1.0f : (mid ? x - 1.0f : x);
+    float      den = big ? x : (mid ? x + 1.0f : 1.0f);
+    x              = num / den;
     float z = x * x;
     float p = 8.05374449538e-2f;
     p       = p * z - 1.38776856032e-1f;

@@ -242,19 +242,26 @@ __global__ void __launch_bounds__(1024) nms_kernel(const Hypo *h, const int *ord
         if (und == 0 || !ch)
             break;
     }
-    if (threadIdx.x == 0) {
-        int cnt = 0;
-        for (int i = 0; i < C; i++) {
-            if (status[i] != CENTER)
-                continue;
+    // kept poses in order, each with its members' votes added in ascending
+    // member order (the reference's accumulation order); one thread per pose
+    __shared__ int sScan[1024 / 64 + 1];
+    int            base = 0;
+    for (int c0 = 0; c0 < C; c0 += 1024) {
+        const int  i      = c0 + (int)threadIdx.x;
+        const bool center = i < C && status[i] == CENTER;
+        int        total;
+        int        pos = block_excl_scan<1024>(center ? 1 : 0, sScan, total);
+        if (center) {
             Hypo o = h[ord[i]];
             for (int j = i + 1; j < C; j++)
                 if (status[j] == i)
                     o.votes = o.votes + h[ord[j]].votes;
-            out[cnt++] = o;
+            out[base + pos] = o;
         }
-        *nOut = cnt;
+        base += total;
     }
+    if (threadIdx.x == 0)
+        *nOut = base;
 }
 
 __global__ void hypo_keys_kernel(const Hypo *h, int n, uint32_t *keys, int *vals) {

@@ -120,15 +120,22 @@ __device__ __forceinline__ float cell_min_d2(const GridView &g, int cx, int cy, 
 
 // k nearest neighbours by (dist2, id) with ring expansion.  Results are
 // sorted ascending; returns the count found (< k only if the grid holds
-// fewer points).  KMAX is a compile-time bound on k.
+// fewer points).  KMAX is a compile-time bound on k; every array index is a
+// compile-time constant after unrolling, so the list stays in registers.
 template <int KMAX>
 struct KnnList {
     float d[KMAX];
     int   id[KMAX];
     int   cnt;
-    __device__ __forceinline__ void init() { cnt = 0; }
+    float wd;  // the k-th entry once the list is full (else +inf)
+    int   wi;
+    __device__ __forceinline__ void init() {
+        cnt = 0;
+        wd  = FLT_MAX;
+        wi  = 0x7fffffff;
+    }
     __device__ __forceinline__ void push(float dd, int ii, int k) {
-        if (cnt == k && !(dd < d[k - 1] || (dd == d[k - 1] && ii < id[k - 1])))
+        if (cnt == k && !(dd < wd || (dd == wd && ii < wi)))
             return;
         int pos = 0;
 #pragma unroll
@@ -151,42 +158,48 @@ struct KnnList {
             id[0] = ii;
         }
         cnt = cnt < k ? cnt + 1 : k;
+        if (cnt == k) {
+#pragma unroll
+            for (int i = 0; i < KMAX; i++)
+                if (i == k - 1) {
+                    wd = d[i];
+                    wi = id[i];
+                }
+        }
     }
 };
 
 // Visit the cells of Chebyshev ring r around (c0x,c0y,c0z), calling f(start,count)
-// for occupied cells whose box is within bound2 of q.
+// for occupied cells whose box is within bound2 of q.  One loop with a single
+// call site: rows off the shell's z/y faces step from the low x end straight
+// to the high x end.
 template <class F>
 __device__ __forceinline__ void visit_ring(const GridView &g, int c0x, int c0y, int c0z, int r,
                                            float qx, float qy, float qz, float bound2, F &&f) {
-    int zlo = max(c0z - r, 0), zhi = min(c0z + r, g.dz - 1);
-    int ylo = max(c0y - r, 0), yhi = min(c0y + r, g.dy - 1);
-    int xlo = max(c0x - r, 0), xhi = min(c0x + r, g.dx - 1);
-    for (int cz = zlo; cz <= zhi; cz++) {
-        bool zs = (cz == c0z - r) || (cz == c0z + r);
-        for (int cy = ylo; cy <= yhi; cy++) {
-            bool ys = (cy == c0y - r) || (cy == c0y + r);
-            if (zs || ys) {
-                for (int cx = xlo; cx <= xhi; cx++) {
-                    if (cell_min_d2(g, cx, cy, cz, qx, qy, qz) > bound2)
-                        continue;
-                    int s, c;
-                    if (cell_lookup(g, cx, cy, cz, s, c))
-                        f(s, c);
-                }
-            } else {
-                // only the two x-end cells of this row lie on the shell
-                int xs[2] = {c0x - r, c0x + r};
-                for (int e = 0; e < (r == 0 ? 1 : 2); e++) {
-                    int cx = xs[e];
-                    if (cx < 0 || cx >= g.dx)
-                        continue;
-                    if (cell_min_d2(g, cx, cy, cz, qx, qy, qz) > bound2)
-                        continue;
-                    int s, c;
-                    if (cell_lookup(g, cx, cy, cz, s, c))
-                        f(s, c);
-                }
+    const int zlo = max(c0z - r, 0), zhi = min(c0z + r, g.dz - 1);
+    const int ylo = max(c0y - r, 0), yhi = min(c0y + r, g.dy - 1);
+    const int xlo = max(c0x - r, 0), xhi = min(c0x + r, g.dx - 1);
+    if (zlo > zhi || ylo > yhi || xlo > xhi)
+        return;
+    int cz = zlo, cy = ylo, cx = xlo;
+    for (;;) {
+        const bool face = (cz == c0z - r) || (cz == c0z + r) || (cy == c0y - r) || (cy == c0y + r);
+        // on a row off the z/y faces only the cells x = c0x -+ r are on the shell
+        const bool on = face || cx == c0x - r || cx == c0x + r;
+        if (on && !(cell_min_d2(g, cx, cy, cz, qx, qy, qz) > bound2)) {
+            int s0, c0;
+            if (cell_lookup(g, cx, cy, cz, s0, c0))
+                f(s0, c0);
+        }
+        // next cell (cx >= xlo >= c0x - r always); off the faces, jump from
+        // the low end (or a clamped interior start) straight to the high end
+        cx = (!face && cx < c0x + r) ? c0x + r : cx + 1;
+        if (cx > xhi) {
+            cx = xlo;
+            if (++cy > yhi) {
+                cy = ylo;
+                if (++cz > zhi)
+                    break;
             }
         }
     }
@@ -231,7 +244,7 @@ __device__ void grid_knn(const GridView &g, float qx, float qy, float qz, int k,
     int c0z = cell_coord_unclamped(qz, g.oz, g.cell);
     int maxR = max(max(g.dx, g.dy), g.dz) + max(max(abs(c0x), abs(c0y)), abs(c0z)) + 1;
     for (int r = first_ring(g, c0x, c0y, c0z); r <= maxR; r++) {
-        float bound2 = (L.cnt == k) ? L.d[k - 1] : FLT_MAX;
+        float bound2 = L.wd;
         visit_ring(g, c0x, c0y, c0z, r, qx, qy, qz, bound2, [&](int s, int c) {
             for (int t = s; t < s + c; t++) {
                 float dd = ppfc::dist2(qx, qy, qz, g.x[t], g.y[t], g.z[t]);
@@ -241,7 +254,7 @@ __device__ void grid_knn(const GridView &g, float qx, float qy, float qz, int k,
         float safe = ring_safe(g, c0x, c0y, c0z, r, qx, qy, qz);
         if (safe == FLT_MAX)
             break;
-        if (L.cnt == k && L.d[k - 1] < safe * safe)
+        if (L.cnt == k && L.wd < safe * safe)
             break;
     }
 }

@@ -336,10 +336,11 @@ constexpr int KNN_MAX = 32;
 // PCA of one point's k nearest neighbours (computeNormal, util.cpp:234-270)
 // from the snapshot grid: FP32 centroid + shifted covariance in kNN order,
 // FP64 Jacobi eigen-solve, eigenvector of the smallest |eigenvalue|.
+template <int KMAX>
 __device__ int pca_point(const GridView &g, float qx, float qy, float qz, int k, float cen[3],
                          float nrm[3], int *nbOut) {
-    KnnList<KNN_MAX> L;
-    grid_knn<KNN_MAX>(g, qx, qy, qz, k, L);
+    KnnList<KMAX> L;
+    grid_knn<KMAX>(g, qx, qy, qz, k, L);
     int cnt = L.cnt;
     if (cnt < 3)
         return 0;
@@ -347,32 +348,34 @@ __device__ int pca_point(const GridView &g, float qx, float qy, float qz, int k,
     // their ids; the grid stores positions in sorted order, so look them up
     // through the original arrays passed in as g.nx.. (positions snapshot)
     float sx = 0.0f, sy = 0.0f, sz = 0.0f;
-    for (int t = 0; t < KNN_MAX; t++) {
-        if (t >= cnt)
-            break;
-        int id = L.id[t];
-        sx     = sx + g.nx[id];
-        sy     = sy + g.ny[id];
-        sz     = sz + g.nz[id];
+#pragma unroll
+    for (int t = 0; t < KMAX; t++) {
+        if (t < cnt) {
+            int id = L.id[t];
+            sx     = sx + g.nx[id];
+            sy     = sy + g.ny[id];
+            sz     = sz + g.nz[id];
+        }
     }
     float fk = (float)cnt;
     cen[0] = sx / fk;
     cen[1] = sy / fk;
     cen[2] = sz / fk;
     float c00 = 0, c01 = 0, c02 = 0, c11 = 0, c12 = 0, c22 = 0;
-    for (int t = 0; t < KNN_MAX; t++) {
-        if (t >= cnt)
-            break;
-        int   id = L.id[t];
-        float x  = g.nx[id] - cen[0];
-        float y  = g.ny[id] - cen[1];
-        float z  = g.nz[id] - cen[2];
-        c00 = c00 + x * x;
-        c01 = c01 + x * y;
-        c02 = c02 + x * z;
-        c11 = c11 + y * y;
-        c12 = c12 + y * z;
-        c22 = c22 + z * z;
+#pragma unroll
+    for (int t = 0; t < KMAX; t++) {
+        if (t < cnt) {
+            int   id = L.id[t];
+            float x  = g.nx[id] - cen[0];
+            float y  = g.ny[id] - cen[1];
+            float z  = g.nz[id] - cen[2];
+            c00 = c00 + x * x;
+            c01 = c01 + x * y;
+            c02 = c02 + x * z;
+            c11 = c11 + y * y;
+            c12 = c12 + y * z;
+            c22 = c22 + z * z;
+        }
     }
     double A[9] = {c00, c01, c02, c01, c11, c12, c02, c12, c22}, V[9], W[3];
     ppfc::jacobi_eig<3>(A, V, W);
@@ -385,10 +388,12 @@ __device__ int pca_point(const GridView &g, float qx, float qy, float qz, int k,
     nrm[0]    = (float)(vx / nn);
     nrm[1]    = (float)(vy / nn);
     nrm[2]    = (float)(vz / nn);
-    if (nbOut)
-        for (int t = 0; t < KNN_MAX; t++)
+    if (nbOut) {
+#pragma unroll
+        for (int t = 0; t < KMAX; t++)
             if (t < cnt)
                 nbOut[t] = L.id[t];
+    }
     return cnt;
 }
 
@@ -408,7 +413,10 @@ struct NormArgs {
     int         *touch;    // per point: neighbour needing a PCA
 };
 
-__global__ void __launch_bounds__(128) normals_query_kernel(NormArgs a) {
+// (at least 4 waves per SIMD: the kNN search is latency-bound)
+template <int KMAX>
+__global__ void __launch_bounds__(128)
+normals_query_kernel(NormArgs a) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= a.nq)
         return;
@@ -418,8 +426,8 @@ __global__ void __launch_bounds__(128) normals_query_kernel(NormArgs a) {
         return;
     }
     float cen[3], nrm[3];
-    int   nb[KNN_MAX];
-    int   cnt = pca_point(a.g, a.g.nx[id], a.g.ny[id], a.g.nz[id], a.k, cen, nrm, nb);
+    int   nb[KMAX];
+    int   cnt = pca_point<KMAX>(a.g, a.g.nx[id], a.g.ny[id], a.g.nz[id], a.k, cen, nrm, nb);
     a.cntq[t] = cnt;
     if (!cnt)
         return;
@@ -430,19 +438,22 @@ __global__ void __launch_bounds__(128) normals_query_kernel(NormArgs a) {
     a.cenY[id] = cen[1];
     a.cenZ[id] = cen[2];
     a.have[id] = 1;
-    for (int x = 0; x < KNN_MAX; x++) {
-        if (x >= cnt)
-            break;
-        a.nbq[(size_t)t * a.k + x] = nb[x];
-        if (a.smooth) {
-            int j = nb[x];
-            if (!finite3(a.N0x[j], a.N0y[j], a.N0z[j]))
-                a.touch[j] = 1;
+#pragma unroll
+    for (int x = 0; x < KMAX; x++) {
+        if (x < cnt) {
+            a.nbq[(size_t)t * a.k + x] = nb[x];
+            if (a.smooth) {
+                int j = nb[x];
+                if (!finite3(a.N0x[j], a.N0y[j], a.N0z[j]))
+                    a.touch[j] = 1;
+            }
         }
     }
 }
 
-__global__ void __launch_bounds__(128) normals_touch_kernel(NormArgs a, const int *list, int nl) {
+template <int KMAX>
+__global__ void __launch_bounds__(128)
+normals_touch_kernel(NormArgs a, const int *list, int nl) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nl)
         return;
@@ -450,7 +461,7 @@ __global__ void __launch_bounds__(128) normals_touch_kernel(NormArgs a, const in
     if (a.have[j])
         return;
     float cen[3], nrm[3];
-    if (pca_point(a.g, a.g.nx[j], a.g.ny[j], a.g.nz[j], a.k, cen, nrm, nullptr)) {
+    if (pca_point<KMAX>(a.g, a.g.nx[j], a.g.ny[j], a.g.nz[j], a.k, cen, nrm, nullptr)) {
         a.rawX[j] = nrm[0];
         a.rawY[j] = nrm[1];
         a.rawZ[j] = nrm[2];
@@ -601,8 +612,13 @@ void estimate_normals(Workspace &ws, int n, float *PX, float *PY, float *PZ, flo
     float *outN = ws.outN.as<float>((size_t)std::max(nq, 1) * 3);
     PPF_HIP_CHECK(hipMemsetAsync(a.have, 0, sizeof(int) * n, s));
     PPF_HIP_CHECK(hipMemsetAsync(a.touch, 0, sizeof(int) * n, s));
-    if (nq > 0)
-        normals_query_kernel<<<div_up(nq, 128), 128, 0, s>>>(a);
+    const bool small = k <= 16;  // kNN list bound: 16 or 32 entries
+    if (nq > 0) {
+        if (small)
+            normals_query_kernel<16><<<div_up(nq, 128), 128, 0, s>>>(a);
+        else
+            normals_query_kernel<KNN_MAX><<<div_up(nq, 128), 128, 0, s>>>(a);
+    }
     if (smooth) {
         char *flags = ws.voxFlag.as<char>(n);
         int  *io    = ws.voxIota.as<int>(n);
@@ -617,8 +633,12 @@ void estimate_normals(Workspace &ws, int n, float *PX, float *PY, float *PZ, flo
         int nl = 0;
         PPF_HIP_CHECK(hipMemcpyAsync(&nl, cntD, sizeof(int), hipMemcpyDeviceToHost, s));
         PPF_HIP_CHECK(hipStreamSynchronize(s));
-        if (nl > 0)
-            normals_touch_kernel<<<div_up(nl, 128), 128, 0, s>>>(a, list, nl);
+        if (nl > 0) {
+            if (small)
+                normals_touch_kernel<16><<<div_up(nl, 128), 128, 0, s>>>(a, list, nl);
+            else
+                normals_touch_kernel<KNN_MAX><<<div_up(nl, 128), 128, 0, s>>>(a, list, nl);
+        }
     }
     if (nq > 0)
         normals_final_kernel<<<div_up(nq, 256), 256, 0, s>>>(a, outN);

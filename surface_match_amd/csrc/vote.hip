@@ -25,7 +25,7 @@ namespace ppfd {
 constexpr int VOTE_AXIS_CELLS = 11;  // cells per axis over the vote sphere (host check)
 constexpr int VOTE_MAX_CELLS  = VOTE_AXIS_CELLS * VOTE_AXIS_CELLS * VOTE_AXIS_CELLS;
 constexpr int SEG            = 8;  // bucket entries voted by one lane
-constexpr int VOTE_STATIC_LDS = 44 * 1024;   // bound on the kernel's static LDS (1024 threads)
+constexpr int VOTE_STATIC_LDS = 52 * 1024;   // bound on the kernel's static LDS (1024 threads)
 constexpr int LDS_LIMIT       = 160 * 1024;  // gfx950 LDS per workgroup
 
 int vote_max_tile_refs(int angleNum, int block) {
@@ -82,6 +82,14 @@ __device__ __forceinline__ void top3_insert(unsigned long long c, unsigned long 
     }
 }
 
+// orders this wave's LDS writes before its later LDS reads (a wave runs in
+// lock step; this keeps the compiler from reordering across the point)
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Vote one queue of hits (bucket start, length, scene alpha) held by this
 // wave: each bucket is cut into segments of SEG entries and the wave's lanes
 // take segments in turn.  The reference's tail rule covers the last len % 8
@@ -113,9 +121,7 @@ __device__ __forceinline__ void wave_vote(const VoteArgs &a, uint32_t *acc, int 
         qPre[lane] = x0 - s0;
     if (lane + 64 < qn)
         qPre[lane + 64] = tot0 + x1 - s1;
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    wave_sync();
     for (int t = lane; t < totalSeg; t += 64) {
         int lo = 0, hi = qn - 1;  // queue entry owning segment t
         while (lo < hi) {
@@ -149,9 +155,7 @@ __device__ __forceinline__ void wave_vote(const VoteArgs &a, uint32_t *acc, int 
         }
     }
     // the queue is refilled after this point
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    wave_sync();
 }
 
 template <int BLOCK>
@@ -166,6 +170,7 @@ __global__ void __launch_bounds__(BLOCK) vote_kernel(VoteArgs a) {
     __shared__ uint32_t qLen[NW][QCAP];
     __shared__ float    qAlpha[NW][QCAP];
     __shared__ int      qPre[NW][QCAP];
+    __shared__ int      pQueue[NW][QCAP];  // sorted-slot ids of queued pairs
     __shared__ int      sScan[NW + 1];
     __shared__ int      sNext;
     __shared__ unsigned long long sTop[NW * 3];
@@ -259,17 +264,47 @@ __global__ void __launch_bounds__(BLOCK) vote_kernel(VoteArgs a) {
     const int nChunks = (total + 63) >> 6;
     unsigned long long nPairs = 0, nHits = 0, nVotes = 0;
     int inSphere = 0;
-    int qn       = 0;  // wave-uniform queue length
+    int pn = 0, qn = 0;  // wave-uniform lengths of the pair and hit queues
+    int *pq = pQueue[wv];
+    // features, key lookup and hit queueing of the first cnt (<= 64) queued
+    // pairs, one per lane
+    auto pair_stage = [&](int cnt) {
+        int      hit    = 0;
+        uint32_t bStart = 0, bLen = 0;
+        float    alphaS = 0.0f;
+        if (lane < cnt) {
+            const int s  = pq[lane];
+            float     px = g.x[s], py = g.y[s], pz = g.z[s];
+            alphaS       = ppfc::alpha_of(rt, px, py, pz);
+            uint32_t key = ppfc::ppf_key(qx, qy, qz, nx1, ny1, nz1, px, py, pz, g.nx[s], g.ny[s],
+                                         g.nz[s], a.angleStep, a.distStep, false);
+            if (table_find(a.slots, a.tmask, key, bStart, bLen) && !ppfc::is_nan(alphaS))
+                hit = 1;
+        }
+        const unsigned long long m = __ballot(hit);
+        if (hit) {
+            const int pos   = qn + __popcll(m & ((1ull << lane) - 1ull));
+            qStart[wv][pos] = bStart;
+            qLen[wv][pos]   = bLen;
+            qAlpha[wv][pos] = alphaS;
+        }
+        nHits += hit;
+        qn += __popcll(m);
+        if (qn >= 64) {
+            wave_sync();
+            wave_vote<QCAP>(a, acc, r0, r1, qStart[wv], qLen[wv], qAlpha[wv], qPre[wv], qn,
+                            nVotes);
+            qn = 0;
+        }
+    };
     for (;;) {
         int t = 0;
         if (lane == 0)
             t = atomicAdd(&sNext, 1);
         t               = __shfl(t, 0, 64);
         const bool more = t < nChunks;
-        int        hit  = 0;
-        uint32_t   bStart = 0, bLen = 0;
-        float      alphaS = 0.0f;
-        const int  c      = t * 64 + lane;
+        const int  c    = t * 64 + lane;
+        int        pc = 0, s = 0;
         if (more && c < total) {
             int lo = 0, hi = nCells - 1;  // largest cell with sCellPre[cell] <= c
             while (lo < hi) {
@@ -279,38 +314,39 @@ __global__ void __launch_bounds__(BLOCK) vote_kernel(VoteArgs a) {
                 else
                     hi = mid - 1;
             }
-            const int s  = sCellStart[lo] + (c - sCellPre[lo]);
-            float     px = g.x[s], py = g.y[s], pz = g.z[s];
-            bool      in = ppfc::dist2(qx, qy, qz, px, py, pz) < a.D2;
+            s        = sCellStart[lo] + (c - sCellPre[lo]);
+            bool in  = ppfc::dist2(qx, qy, qz, g.x[s], g.y[s], g.z[s]) < a.D2;
             inSphere += in ? 1 : 0;
             if (in && g.id[s] != kpId) {
+                pc = 1;
                 nPairs++;
-                alphaS = ppfc::alpha_of(rt, px, py, pz);
-                uint32_t key = ppfc::ppf_key(qx, qy, qz, nx1, ny1, nz1, px, py, pz, g.nx[s],
-                                             g.ny[s], g.nz[s], a.angleStep, a.distStep, false);
-                if (table_find(a.slots, a.tmask, key, bStart, bLen) && !ppfc::is_nan(alphaS))
-                    hit = 1;
             }
         }
-        const unsigned long long m = __ballot(hit);
-        if (hit) {
-            const int pos       = qn + __popcll(m & ((1ull << lane) - 1ull));
-            qStart[wv][pos]     = bStart;
-            qLen[wv][pos]       = bLen;
-            qAlpha[wv][pos]     = alphaS;
-        }
-        nHits += hit;
-        qn += __popcll(m);
-        if (qn >= 64 || (!more && qn > 0)) {
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            wave_vote<QCAP>(a, acc, r0, r1, qStart[wv], qLen[wv], qAlpha[wv], qPre[wv], qn,
-                            nVotes);
-            qn = 0;
+        // compact the pairs so the feature stage runs on full waves
+        const unsigned long long m = __ballot(pc);
+        if (pc)
+            pq[pn + __popcll(m & ((1ull << lane) - 1ull))] = s;
+        pn += __popcll(m);
+        if (pn >= 64) {
+            wave_sync();
+            pair_stage(64);
+            const int rest = pn - 64;
+            const int v    = lane < rest ? pq[64 + lane] : 0;
+            wave_sync();
+            if (lane < rest)
+                pq[lane] = v;
+            pn = rest;
         }
         if (!more)
             break;
+    }
+    if (pn > 0) {
+        wave_sync();
+        pair_stage(pn);
+    }
+    if (qn > 0) {
+        wave_sync();
+        wave_vote<QCAP>(a, acc, r0, r1, qStart[wv], qLen[wv], qAlpha[wv], qPre[wv], qn, nVotes);
     }
     __syncthreads();
 
