@@ -72,7 +72,20 @@ struct GridView {
     const float    *nx, *ny, *nz;  // optional (nullptr)
     const int      *id;            // original point id of each sorted slot
     int             n;
+    // occupancy of coarse cells (4 x 4 x 4 cells each), one bit per coarse
+    // cell, x fastest: lets searches skip empty space without table probes
+    const uint32_t *occ;
+    int             odx, ody;
 };
+
+constexpr int OCC_SHIFT = 2;  // coarse cell = 2^2 cells per axis
+
+__device__ __forceinline__ bool coarse_empty(const GridView &g, int cx, int cy, int cz) {
+    const uint32_t b = ((uint32_t)(cz >> OCC_SHIFT) * (uint32_t)g.ody + (uint32_t)(cy >> OCC_SHIFT)) *
+                           (uint32_t)g.odx +
+                       (uint32_t)(cx >> OCC_SHIFT);
+    return ((g.occ[b >> 5] >> (b & 31u)) & 1u) == 0u;
+}
 
 __device__ __forceinline__ int cell_coord(float v, float o, float cell, int dim) {
     int c = (int)floorf((v - o) / cell);
@@ -186,7 +199,8 @@ __device__ __forceinline__ void visit_ring(const GridView &g, int c0x, int c0y, 
         const bool face = (cz == c0z - r) || (cz == c0z + r) || (cy == c0y - r) || (cy == c0y + r);
         // on a row off the z/y faces only the cells x = c0x -+ r are on the shell
         const bool on = face || cx == c0x - r || cx == c0x + r;
-        if (on && !(cell_min_d2(g, cx, cy, cz, qx, qy, qz) > bound2)) {
+        if (on && !coarse_empty(g, cx, cy, cz) &&
+            !(cell_min_d2(g, cx, cy, cz, qx, qy, qz) > bound2)) {
             int s0, c0;
             if (cell_lookup(g, cx, cy, cz, s0, c0))
                 f(s0, c0);
@@ -311,7 +325,7 @@ __device__ inline bool grid_any_within(const GridView &g, float qx, float qy, fl
         for (int cy = ylo; cy <= yhi; cy++)
             for (int cx = xlo; cx <= xhi; cx++) {
                 int s, c;
-                if (!cell_lookup(g, cx, cy, cz, s, c))
+                if (coarse_empty(g, cx, cy, cz) || !cell_lookup(g, cx, cy, cz, s, c))
                     continue;
                 for (int t = s; t < s + c; t++)
                     if (ppfc::dist2(qx, qy, qz, g.x[t], g.y[t], g.z[t]) < r2)
@@ -324,7 +338,7 @@ __device__ inline bool grid_any_within(const GridView &g, float qx, float qy, fl
 // host side of the grid
 // --------------------------------------------------------------------------
 struct Grid {
-    DevBuf   keys, keys2, vals, vals2, htab, x, y, z, nx, ny, nz, tmp, bbox;
+    DevBuf   keys, keys2, vals, vals2, htab, x, y, z, nx, ny, nz, tmp, bbox, occ;
     GridView view{};
     int      capN = 0;
 };

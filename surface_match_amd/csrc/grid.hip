@@ -46,11 +46,25 @@ __global__ void __launch_bounds__(256) bbox_kernel(const float *X, const float *
             hi[a] = fmaxf(hi[a], __shfl_xor(hi[a], o, 64));
         }
     }
+    // one atomic per block and bound (a few hundred blocks contend per word)
+    __shared__ float sLo[4][3], sHi[4][3];
+    const int w = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0)
         for (int a = 0; a < 3; a++) {
-            atomicMin(bb + a, f2ord(lo[a]));
-            atomicMax(bb + 3 + a, f2ord(hi[a]));
+            sLo[w][a] = lo[a];
+            sHi[w][a] = hi[a];
         }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        const int a = threadIdx.x;
+        float     l = sLo[0][a], h = sHi[0][a];
+        for (int q = 1; q < 4; q++) {
+            l = fminf(l, sLo[q][a]);
+            h = fmaxf(h, sHi[q][a]);
+        }
+        atomicMin(bb + a, f2ord(l));
+        atomicMax(bb + 3 + a, f2ord(h));
+    }
 }
 
 __global__ void bbox_fin_kernel(int *bb) {
@@ -65,7 +79,7 @@ void device_bbox(const float *X, const float *Y, const float *Z, const int *subs
     int *bb = reinterpret_cast<int *>(bb6dev);
     bbox_init_kernel<<<1, 64, 0, s>>>(bb);
     if (m > 0) {
-        int blocks = std::min(div_up(m, 256), 1024);
+        int blocks = std::min(div_up(m, 256), 256);
         bbox_kernel<<<blocks, 256, 0, s>>>(X, Y, Z, subset, m, bb);
     }
     bbox_fin_kernel<<<1, 64, 0, s>>>(bb);
@@ -85,6 +99,21 @@ __global__ void grid_keys_kernel(const float *X, const float *Y, const float *Z,
     int cz = cell_coord(Z[i], oz, cell, dz);
     keys[t] = (uint32_t)((cz * dy + cy) * dx + cx);
     vals[t] = i;
+}
+
+// coarse occupancy bit of every occupied cell (first slot of each cell run)
+__global__ void grid_occ_kernel(const uint32_t *keys, int m, int dx, int dy, int odx, int ody,
+                                uint32_t *occ) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= m || (s > 0 && keys[s] == keys[s - 1]))
+        return;
+    uint32_t k  = keys[s];
+    int      cx = (int)(k % (uint32_t)dx), cy = (int)((k / (uint32_t)dx) % (uint32_t)dy),
+        cz      = (int)(k / ((uint32_t)dx * (uint32_t)dy));
+    uint32_t b  = ((uint32_t)(cz >> OCC_SHIFT) * (uint32_t)ody + (uint32_t)(cy >> OCC_SHIFT)) *
+                     (uint32_t)odx +
+                 (uint32_t)(cx >> OCC_SHIFT);
+    atomicOr(&occ[b >> 5], 1u << (b & 31u));
 }
 
 __global__ void grid_insert_kernel(const uint32_t *keys, int m, uint4 *htab, uint32_t hmask) {
@@ -165,6 +194,11 @@ void grid_build(Grid &g, const float *X, const float *Y, const float *Z, const f
     float    *y     = g.y.as<float>(std::max(m, 1));
     float    *z     = g.z.as<float>(std::max(m, 1));
     PPF_HIP_CHECK(hipMemsetAsync(htab, 0, sizeof(uint4) * hsize, s));
+    const int odx = ((dims[0] - 1) >> OCC_SHIFT) + 1, ody = ((dims[1] - 1) >> OCC_SHIFT) + 1,
+              odz = ((dims[2] - 1) >> OCC_SHIFT) + 1;
+    const size_t occWords = ((size_t)odx * ody * odz + 31) / 32;
+    uint32_t    *occ      = g.occ.as<uint32_t>(occWords);
+    PPF_HIP_CHECK(hipMemsetAsync(occ, 0, sizeof(uint32_t) * occWords, s));
     if (m > 0) {
         grid_keys_kernel<<<div_up(m, 256), 256, 0, s>>>(X, Y, Z, subset, m, hb[0], hb[1], hb[2],
                                                          cell, dims[0], dims[1], dims[2], keys,
@@ -179,6 +213,7 @@ void grid_build(Grid &g, const float *X, const float *Y, const float *Z, const f
         PPF_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(tmp, tb, keys, keys2, vals, vals2, m, 0,
                                                          endBit, s));
         grid_insert_kernel<<<div_up(m, 256), 256, 0, s>>>(keys2, m, htab, hsize - 1);
+        grid_occ_kernel<<<div_up(m, 256), 256, 0, s>>>(keys2, m, dims[0], dims[1], odx, ody, occ);
         grid_count_kernel<<<div_up(m, 256), 256, 0, s>>>(keys2, m, htab, hsize - 1);
         gather3_kernel<<<div_up(m, 256), 256, 0, s>>>(vals2, m, X, Y, Z, x, y, z);
     }
@@ -197,6 +232,9 @@ void grid_build(Grid &g, const float *X, const float *Y, const float *Z, const f
     v.z = z;
     v.id = vals2;
     v.n  = m;
+    v.occ = occ;
+    v.odx = odx;
+    v.ody = ody;
     v.nx = v.ny = v.nz = nullptr;
     if (NX && m > 0) {
         float *nx = g.nx.as<float>(m), *ny = g.ny.as<float>(m), *nz = g.nz.as<float>(m);
