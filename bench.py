@@ -131,6 +131,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-icp-batching", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the PPF path runs on a single MI355X "
@@ -150,6 +151,9 @@ def main():
     param = MatchParam(numMatches=10)
     info = det.model_info()
 
+    # launch shapes picked by the in-project autotuner on this scene (results
+    # do not depend on them), then the untimed warm-up
+    tuning = scene.autotune(0.04, 0.1, 0.2, param) if not args.no_autotune else None
     for _ in range(args.warmup):
         scene.match(0.04, 0.1, 0.2, param)
     times, stats = [], []
@@ -205,6 +209,7 @@ def main():
                      "kernel": "vote_kernel", "algorithmic_bytes": alg,
                      "kernel_ms": round(vote_ms, 4)},
         "results": found,
+        "tuning": det.get_tuning(),
     }
     if not args.no_icp_batching:
         out["icp_batching"] = icp_batching(det)

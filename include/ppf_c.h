@@ -136,19 +136,22 @@ PPF_API int ppf_match_scene_resident(ppf_detector d, ppf_scene s, float sampling
  * vote kernel over workgroup sizes {256, 512, 1024} x XCD-aware key-point
  * mapping {off, on} x vote-grid cell {D/2, D/3, D/4}, then the batched ICP
  * over {spread, persistent} x {64, 128, 256, 512, 1024} threads per
- * workgroup, on the given resident scene, and keeps the fastest in the
- * detector.  None of these changes any result. -- */
+ * workgroup, then the normals grid cell over {1/4, 1/2, 1} sample step, on
+ * the given resident scene, and keeps the fastest in the detector.  None of
+ * these changes any result. -- */
 typedef struct {
     int voteBlock;    /* threads per vote workgroup: 256, 512 or 1024 */
     int voteXcdRemap; /* 1: consecutive key points stay on one XCD */
     int voteCellDiv;  /* vote grid cell = model diameter / voteCellDiv: 2, 3 or 4 */
     int icpBlock;     /* threads per ICP workgroup: 64 .. 1024, a power of two */
     int icpSpread;    /* 1: each hypothesis spread over workgroups, 0: persistent */
+    int normCell4;    /* normals grid cell = sample step * normCell4 / 4: 1, 2 or 4 */
 } ppf_tuning;
 typedef struct {
     ppf_tuning best;
     float      voteMs, icpMs; /* device time of the vote kernel / both refinements */
     int        configsTried;
+    float      normalsMs;     /* device time of both normal estimations */
 } ppf_tune_report;
 PPF_API int ppf_autotune(ppf_detector d, ppf_scene s, float samplingDistanceRel,
                          float keyPointFraction, float minScore, const ppf_match_param *param,

@@ -242,7 +242,7 @@ class ResidentScene:
         N.check(N.lib().ppf_autotune(self._det._h, self._h, samplingDistanceRel, keyPointFraction,
                                      minScore, C.byref(param), C.byref(rep)))
         return {"best": rep.best.as_dict(), "voteMs": rep.voteMs, "icpMs": rep.icpMs,
-                "configsTried": rep.configsTried}
+                "normalsMs": rep.normalsMs, "configsTried": rep.configsTried}
 
     def __del__(self):
         h = getattr(self, "_h", None)

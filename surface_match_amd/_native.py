@@ -71,7 +71,7 @@ class IcpResult(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [(n, C.c_int) for n in ("voteBlock", "voteXcdRemap", "voteCellDiv", "icpBlock",
-                                       "icpSpread")]
+                                       "icpSpread", "normCell4")]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -79,7 +79,7 @@ class Tuning(C.Structure):
 
 class TuneReport(C.Structure):
     _fields_ = [("best", Tuning), ("voteMs", C.c_float), ("icpMs", C.c_float),
-                ("configsTried", C.c_int)]
+                ("configsTried", C.c_int), ("normalsMs", C.c_float)]
 
 
 _lib = None

@@ -226,6 +226,7 @@ struct ppf_detector_s {
     TrainScratch ts;
     VoteConfig   vcfg;
     IcpConfig    icfg;
+    int          normCell4 = 2;  // normals grid cell = sample step * normCell4 / 4
     int          refTail = 1;
     DevCloud     tmpScene;
     DevBuf       idxA, idxB, idxC, idxD;  // sampled / keypoints / dense / scratch
@@ -429,6 +430,7 @@ void match_impl(ppf_detector_s &D, DevCloud &c, float rel, float kpFrac, float m
     const int   refNum       = hm.sampled.size();
     const float reSampleStep = Dm * hm.param.poseRefRelSamplingDistance;
     const float sampleStep   = Dm * rel;  // :200
+    const float normCell     = sampleStep * (float)D.normCell4 * 0.25f;
     // :223-240
     const float voteThreshold = (float)refNum * minScore;
     float maxOverlapDist = 0;
@@ -452,7 +454,7 @@ void match_impl(ppf_detector_s &D, DevCloud &c, float rel, float kpFrac, float m
     // normals (ppf.cpp:204-211)
     if (!c.hasNormal)
         estimate_normals(D.ws, c.n, PX, PY, PZ, NX, NY, NZ, c.valid.get<int>(), c.nValid, samp, ns,
-                         p.knnNormal, p.smoothNormal, p.invertNormal, c.vp, sampleStep * 0.5f, s);
+                         p.knnNormal, p.smoothNormal, p.invertNormal, c.vp, normCell, s);
     PPF_HIP_CHECK(hipEventRecord(D.ev[2], s));
     // key points (ppf.cpp:212-216)
     const float keySampleStep = std::sqrt(1.0f / kpFrac) * sampleStep;
@@ -488,7 +490,7 @@ void match_impl(ppf_detector_s &D, DevCloud &c, float rel, float kpFrac, float m
                           dense, cnt, s);
         if (!c.hasNormal)
             estimate_normals(D.ws, c.n, PX, PY, PZ, NX, NY, NZ, c.valid.get<int>(), c.nValid,
-                             dense, nd, p.knnNormal, 0, p.invertNormal, c.vp, sampleStep * 0.5f, s);
+                             dense, nd, p.knnNormal, 0, p.invertNormal, c.vp, normCell, s);
     }
     PPF_HIP_CHECK(hipEventRecord(D.ev[7], s));
     // icp (ppf.cpp:436-495)
@@ -1017,6 +1019,7 @@ ppf_tuning tuning_of(const ppf_detector_s &d) {
     t.voteCellDiv  = d.vcfg.cellDiv;
     t.icpBlock     = d.icfg.block;
     t.icpSpread    = d.icfg.spread;
+    t.normCell4    = d.normCell4;
     return t;
 }
 }  // namespace
@@ -1038,8 +1041,11 @@ PPF_API int ppf_set_tuning(ppf_detector d, const ppf_tuning *t) {
     d->vcfg.block    = t->voteBlock;
     d->vcfg.xcdRemap = t->voteXcdRemap ? 1 : 0;
     d->vcfg.cellDiv  = t->voteCellDiv;
+    PPF_REQUIRE(t->normCell4 == 1 || t->normCell4 == 2 || t->normCell4 == 4,
+                "normals cell must be 1, 2 or 4 quarters of the sample step");
     d->icfg.block    = t->icpBlock;
     d->icfg.spread   = t->icpSpread ? 1 : 0;
+    d->normCell4     = t->normCell4;
     return PPF_OK;
 }
 
@@ -1094,6 +1100,25 @@ PPF_API int ppf_autotune(ppf_detector d, ppf_scene sc, float rel, float kpFrac, 
                 }
             }
         d->icfg               = bestI;
+        int   bestN           = d->normCell4;
+        float bestNormMs      = 1e30f;
+        for (int q : {1, 2, 4}) {
+            d->normCell4 = q;
+            float ms     = 1e30f;
+            for (int rep2 = 0; rep2 < 2; rep2++) {
+                MatchOut        o;
+                ppf_match_stats st{};
+                match_impl(*d, sc->cloud, rel, kpFrac, minScore, p, o, nullptr, &st);
+                ms = std::min(ms, st.msNormals + st.msIcpPrepare);
+            }
+            tried++;
+            if (ms < bestNormMs) {
+                bestNormMs = ms;
+                bestN      = q;
+            }
+        }
+        d->normCell4          = bestN;
+        rep->normalsMs        = bestNormMs;
         rep->best             = tuning_of(*d);
         rep->voteMs           = bestVoteMs;
         rep->icpMs            = bestIcpMs;

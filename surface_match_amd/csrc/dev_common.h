@@ -150,25 +150,23 @@ struct KnnList {
     __device__ __forceinline__ void push(float dd, int ii, int k) {
         if (cnt == k && !(dd < wd || (dd == wd && ii < wi)))
             return;
-        int pos = 0;
+        // insertion by carrying: the carried entry swaps with every larger one
+        // on its way down; slot cnt (when free) takes the last carried entry
+        float cd = dd;
+        int   ci = ii;
 #pragma unroll
-        for (int i = 0; i < KMAX; i++)
-            pos += (i < cnt && (d[i] < dd || (d[i] == dd && id[i] < ii))) ? 1 : 0;
-#pragma unroll
-        for (int i = KMAX - 1; i > 0; i--) {
-            float od = d[i - 1];
-            int   oi = id[i - 1];
-            if (i > pos) {
-                d[i]  = od;
-                id[i] = oi;
-            } else if (i == pos) {
-                d[i]  = dd;
-                id[i] = ii;
+        for (int i = 0; i < KMAX; i++) {
+            if (i < k && i <= cnt) {
+                const bool take = (i == cnt) || cd < d[i] || (cd == d[i] && ci < id[i]);
+                if (take) {
+                    const float td = d[i];
+                    const int   ti = id[i];
+                    d[i]           = cd;
+                    id[i]          = ci;
+                    cd             = td;
+                    ci             = ti;
+                }
             }
-        }
-        if (pos == 0) {
-            d[0]  = dd;
-            id[0] = ii;
         }
         cnt = cnt < k ? cnt + 1 : k;
         if (cnt == k) {

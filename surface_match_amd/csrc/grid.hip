@@ -371,21 +371,15 @@ int voxel_sample(Workspace &ws, const float *X, const float *Y, const float *Z,
 
 constexpr int KNN_MAX = 32;
 
-// PCA of one point's k nearest neighbours (computeNormal, util.cpp:234-270)
-// from the snapshot grid: FP32 centroid + shifted covariance in kNN order,
-// FP64 Jacobi eigen-solve, eigenvector of the smallest |eigenvalue|.
+// PCA of a point's k nearest neighbours (computeNormal, util.cpp:234-270):
+// FP32 centroid + shifted covariance in kNN order over the snapshot
+// positions (g.nx.. = positions indexed by id), FP64 Jacobi eigen-solve,
+// eigenvector of the smallest |eigenvalue|.
 template <int KMAX>
-__device__ int pca_point(const GridView &g, float qx, float qy, float qz, int k, float cen[3],
-                         float nrm[3], int *nbOut) {
-    KnnList<KMAX> L;
-    grid_knn<KMAX>(g, qx, qy, qz, k, L);
-    int cnt = L.cnt;
-    if (cnt < 3)
-        return 0;
-    // neighbours' coordinates are read back from the sorted grid arrays via
-    // their ids; the grid stores positions in sorted order, so look them up
-    // through the original arrays passed in as g.nx.. (positions snapshot)
-    float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+__device__ void pca_of_list(const GridView &g, const KnnList<KMAX> &L, float cen[3],
+                            float nrm[3]) {
+    const int cnt = L.cnt;
+    float     sx = 0.0f, sy = 0.0f, sz = 0.0f;
 #pragma unroll
     for (int t = 0; t < KMAX; t++) {
         if (t < cnt) {
@@ -426,13 +420,6 @@ __device__ int pca_point(const GridView &g, float qx, float qy, float qz, int k,
     nrm[0]    = (float)(vx / nn);
     nrm[1]    = (float)(vy / nn);
     nrm[2]    = (float)(vz / nn);
-    if (nbOut) {
-#pragma unroll
-        for (int t = 0; t < KMAX; t++)
-            if (t < cnt)
-                nbOut[t] = L.id[t];
-    }
-    return cnt;
 }
 
 __device__ __forceinline__ bool finite3(float a, float b, float c) {
@@ -445,69 +432,84 @@ struct NormArgs {
     const int   *query;
     int          nq, k, smooth;
     float       *rawX, *rawY, *rawZ, *cenX, *cenY, *cenZ;  // per point
-    int         *have;     // per point: 1 = fresh PCA
-    int         *nbq;      // nq * k
-    int         *cntq;     // nq
+    int         *have;     // per point: 1 = fresh PCA of a query, 2 = of a neighbour
+    int         *nbq;      // nq * k: kNN ids of each query
+    int         *cntq;     // nq: kNN count of each query (0: keeps its normal)
     int         *touch;    // per point: neighbour needing a PCA
+    int         *mark;     // per point: 1 = query to estimate, 2 = neighbour to estimate
+    int         *qidx;     // per point: index in the query list
 };
 
-// (at least 4 waves per SIMD: the kNN search is latency-bound)
+// store one estimate (want 1: a query, with its kNN list for smoothing;
+// want 2: a neighbour needed by the smoothing)
 template <int KMAX>
-__global__ void __launch_bounds__(128)
-normals_query_kernel(NormArgs a) {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= a.nq)
+__device__ void store_estimate(const NormArgs &a, int id, int want, const KnnList<KMAX> &L) {
+    const int cnt = L.cnt;
+    if (want == 1)
+        a.cntq[a.qidx[id]] = cnt < 3 ? 0 : cnt;
+    if (cnt < 3)
         return;
-    int id = a.query[t];
-    if (finite3(a.N0x[id], a.N0y[id], a.N0z[id])) {
-        a.cntq[t] = 0;
-        return;
-    }
     float cen[3], nrm[3];
-    int   nb[KMAX];
-    int   cnt = pca_point<KMAX>(a.g, a.g.nx[id], a.g.ny[id], a.g.nz[id], a.k, cen, nrm, nb);
-    a.cntq[t] = cnt;
-    if (!cnt)
-        return;
+    pca_of_list<KMAX>(a.g, L, cen, nrm);
     a.rawX[id] = nrm[0];
     a.rawY[id] = nrm[1];
     a.rawZ[id] = nrm[2];
     a.cenX[id] = cen[0];
     a.cenY[id] = cen[1];
     a.cenZ[id] = cen[2];
-    a.have[id] = 1;
+    a.have[id] = want;
+    if (want != 1)
+        return;
+    const size_t t = (size_t)a.qidx[id];
 #pragma unroll
     for (int x = 0; x < KMAX; x++) {
         if (x < cnt) {
-            a.nbq[(size_t)t * a.k + x] = nb[x];
-            if (a.smooth) {
-                int j = nb[x];
-                if (!finite3(a.N0x[j], a.N0y[j], a.N0z[j]))
-                    a.touch[j] = 1;
-            }
+            const int j     = L.id[x];
+            a.nbq[t * a.k + x] = j;
+            if (a.smooth && !finite3(a.N0x[j], a.N0y[j], a.N0z[j]))
+                a.touch[j] = 1;
         }
     }
 }
 
+// queries with a finite normal keep it; the others are marked for estimation
+__global__ void normals_mark_kernel(NormArgs a) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.nq)
+        return;
+    int id     = a.query[t];
+    a.qidx[id] = t;
+    if (finite3(a.N0x[id], a.N0y[id], a.N0z[id]))
+        a.cntq[t] = 0;
+    else
+        a.mark[id] = 1;
+}
+
+// neighbours that the smoothing needs and that have no estimate yet
+__global__ void normals_touch_mark_kernel(NormArgs a, int n, char *flags, int *iota) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const bool need = a.touch[i] && !a.have[i];
+    if (need)
+        a.mark[i] = 2;
+    flags[i] = need ? 1 : 0;
+    iota[i]  = i;
+}
+
+// Estimation of one point by the general grid kNN (ring expansion)
 template <int KMAX>
-__global__ void __launch_bounds__(128)
-normals_touch_kernel(NormArgs a, const int *list, int nl) {
+__global__ void __launch_bounds__(128) normals_point_kernel(NormArgs a, const int *list, int nl,
+                                                            int want) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nl)
         return;
-    int j = list[t];
-    if (a.have[j])
+    int id = list[t];
+    if (a.mark[id] != want)
         return;
-    float cen[3], nrm[3];
-    if (pca_point<KMAX>(a.g, a.g.nx[j], a.g.ny[j], a.g.nz[j], a.k, cen, nrm, nullptr)) {
-        a.rawX[j] = nrm[0];
-        a.rawY[j] = nrm[1];
-        a.rawZ[j] = nrm[2];
-        a.cenX[j] = cen[0];
-        a.cenY[j] = cen[1];
-        a.cenZ[j] = cen[2];
-        a.have[j] = 2;
-    }
+    KnnList<KMAX> L;
+    grid_knn<KMAX>(a.g, a.g.nx[id], a.g.ny[id], a.g.nz[id], a.k, L);
+    store_estimate<KMAX>(a, id, want, L);
 }
 
 // final normal of each query (smoothing, util.cpp:292-306)
@@ -601,11 +603,6 @@ __global__ void normals_orient_kernel(NormArgs a, const float *outN, float *PX, 
     NZ[id] = nz;
 }
 
-__global__ void flag_to_list_kernel(const int *touch, int n, char *flags) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n)
-        flags[i] = touch[i] ? 1 : 0;
-}
 
 void estimate_normals(Workspace &ws, int n, float *PX, float *PY, float *PZ, float *NX, float *NY,
                       float *NZ, const int *valid, int nValid, const int *query, int nq, int k,
@@ -647,36 +644,40 @@ void estimate_normals(Workspace &ws, int n, float *PX, float *PY, float *PZ, flo
     a.touch = ws.touch.as<int>(n);
     a.nbq   = ws.nbq.as<int>((size_t)std::max(nq, 1) * k);
     a.cntq  = ws.cntq.as<int>(std::max(nq, 1));
+    a.mark  = ws.mark.as<int>(n);
+    a.qidx  = ws.qidx.as<int>(n);
     float *outN = ws.outN.as<float>((size_t)std::max(nq, 1) * 3);
     PPF_HIP_CHECK(hipMemsetAsync(a.have, 0, sizeof(int) * n, s));
     PPF_HIP_CHECK(hipMemsetAsync(a.touch, 0, sizeof(int) * n, s));
+    PPF_HIP_CHECK(hipMemsetAsync(a.mark, 0, sizeof(int) * n, s));
+    if (nq > 0)
+        normals_mark_kernel<<<div_up(nq, 256), 256, 0, s>>>(a);
+
     const bool small = k <= 16;  // kNN list bound: 16 or 32 entries
-    if (nq > 0) {
+    auto estimate = [&](const int *list, int nl, int want) {
+        if (nl <= 0)
+            return;
         if (small)
-            normals_query_kernel<16><<<div_up(nq, 128), 128, 0, s>>>(a);
+            normals_point_kernel<16><<<div_up(nl, 128), 128, 0, s>>>(a, list, nl, want);
         else
-            normals_query_kernel<KNN_MAX><<<div_up(nq, 128), 128, 0, s>>>(a);
-    }
+            normals_point_kernel<KNN_MAX><<<div_up(nl, 128), 128, 0, s>>>(a, list, nl, want);
+    };
+    estimate(query, nq, 1);
     if (smooth) {
+        // the neighbours the smoothing needs that have no estimate yet
         char *flags = ws.voxFlag.as<char>(n);
         int  *io    = ws.voxIota.as<int>(n);
         int  *list  = ws.touchList.as<int>(n);
         int  *cntD  = ws.counter.as<int>(4);
-        flag_to_list_kernel<<<div_up(n, 256), 256, 0, s>>>(a.touch, n, flags);
-        iota_kernel<<<div_up(n, 256), 256, 0, s>>>(io, n);
+        normals_touch_mark_kernel<<<div_up(n, 256), 256, 0, s>>>(a, n, flags, io);
         size_t tb = 0;
         PPF_HIP_CHECK(hipcub::DeviceSelect::Flagged(nullptr, tb, io, flags, list, cntD, n, s));
-        void *tmp = ws.cubTmp.reserve(tb);
-        PPF_HIP_CHECK(hipcub::DeviceSelect::Flagged(tmp, tb, io, flags, list, cntD, n, s));
+        PPF_HIP_CHECK(hipcub::DeviceSelect::Flagged(ws.cubTmp.reserve(tb), tb, io, flags, list,
+                                                    cntD, n, s));
         int nl = 0;
         PPF_HIP_CHECK(hipMemcpyAsync(&nl, cntD, sizeof(int), hipMemcpyDeviceToHost, s));
         PPF_HIP_CHECK(hipStreamSynchronize(s));
-        if (nl > 0) {
-            if (small)
-                normals_touch_kernel<16><<<div_up(nl, 128), 128, 0, s>>>(a, list, nl);
-            else
-                normals_touch_kernel<KNN_MAX><<<div_up(nl, 128), 128, 0, s>>>(a, list, nl);
-        }
+        estimate(list, nl, 2);
     }
     if (nq > 0)
         normals_final_kernel<<<div_up(nq, 256), 256, 0, s>>>(a, outN);

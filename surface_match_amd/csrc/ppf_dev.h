@@ -21,7 +21,7 @@ struct GraphCache {
 struct Workspace {
     DevBuf voxKey, voxBest, voxFlag, voxIota, cubTmp, counter;
     DevBuf p0x, p0y, p0z, n0x, n0y, n0z, rawX, rawY, rawZ, cenX, cenY, cenZ, have, touch,
-        touchList, nbq, cntq, outN;
+        touchList, nbq, cntq, outN, mark, qidx;
     Grid   gNormal, gVote, gSparse, gDense, gFull;
     // vote
     DevBuf kpX, kpY, kpZ, kpNX, kpNY, kpNZ, cand, candTile, poses, voteStats, accDump, voteCnt,

@@ -10,10 +10,14 @@
 // the accumulator does not fit in LDS).  The accumulator lives in LDS
 // without the reference's per-ref count slot (it always equals the sum of
 // the ref's bins, so the peak stage recomputes it).  Neighbour candidates
-// come from the hashed grid cells overlapping the vote sphere; hits are
-// compacted per chunk and their bucket entries are voted as one flattened
-// range striped over the workgroup's lanes, so consecutive lanes read
-// consecutive entries of a bucket and bucket-size skew does not idle lanes.
+// come from the hashed grid cells overlapping the vote sphere.  Each wave
+// works on its own: it takes chunks of 64 candidates from a shared counter,
+// compacts the in-sphere pairs into a per-wave LDS queue so the feature and
+// key stage runs on full waves, queues the key hits, and votes 64 hits at a
+// time by cutting their buckets into segments of 8 entries striped over its
+// lanes (bucket-size skew does not idle lanes).  The last queued hits of all
+// waves are voted by the whole workgroup.  Integer votes commute, so the
+// accumulator does not depend on the order of any of this.
 #include "ppf_dev.h"
 
 #include <hipcub/hipcub.hpp>
@@ -25,7 +29,8 @@ namespace ppfd {
 constexpr int VOTE_AXIS_CELLS = 11;  // cells per axis over the vote sphere (host check)
 constexpr int VOTE_MAX_CELLS  = VOTE_AXIS_CELLS * VOTE_AXIS_CELLS * VOTE_AXIS_CELLS;
 constexpr int SEG            = 8;  // bucket entries voted by one lane
-constexpr int VOTE_STATIC_LDS = 52 * 1024;   // bound on the kernel's static LDS (1024 threads)
+constexpr int VOTE_STATIC_LDS = 55 * 1024;   // bound on the kernel's static LDS (1024 threads)
+constexpr int VOTE_MAX_CHUNKS = 512;         // chunks of 64 candidates with a first-cell entry
 constexpr int LDS_LIMIT       = 160 * 1024;  // gfx950 LDS per workgroup
 
 int vote_max_tile_refs(int angleNum, int block) {
@@ -67,19 +72,16 @@ __device__ __forceinline__ bool table_find(const uint4 *slots, uint32_t tmask, u
 }
 
 // candidate code: (vote << 32) | ((0xFFFFF - ref) << 11) | bin -- larger is
-// better under the reference's (vote desc, ref asc) order.
-__device__ __forceinline__ void top3_insert(unsigned long long c, unsigned long long &a,
-                                            unsigned long long &b, unsigned long long &d) {
-    if (c > a) {
-        d = b;
-        b = a;
-        a = c;
-    } else if (c > b) {
-        d = b;
-        b = c;
-    } else if (c > d) {
-        d = c;
-    }
+// better under the reference's (vote desc, ref asc) order.  The best three
+// codes, descending; insertion is a branch-free min/max network, so the
+// three stay in registers.
+struct Top3 {
+    unsigned long long a, b, c;
+};
+__device__ __forceinline__ Top3 top3_add(Top3 t, unsigned long long x) {
+    const unsigned long long hi1 = max(t.a, x), lo1 = min(t.a, x);
+    const unsigned long long hi2 = max(t.b, lo1), lo2 = min(t.b, lo1);
+    return Top3{hi1, hi2, max(t.c, lo2)};
 }
 
 // orders this wave's LDS writes before its later LDS reads (a wave runs in
@@ -171,6 +173,8 @@ __global__ void __launch_bounds__(BLOCK) vote_kernel(VoteArgs a) {
     __shared__ float    qAlpha[NW][QCAP];
     __shared__ int      qPre[NW][QCAP];
     __shared__ int      pQueue[NW][QCAP];  // sorted-slot ids of queued pairs
+    __shared__ int      sChunkCell[VOTE_MAX_CHUNKS];  // cell holding candidate 64 t
+    __shared__ int      sQn[NW];
     __shared__ int      sScan[NW + 1];
     __shared__ int      sNext;
     __shared__ unsigned long long sTop[NW * 3];
@@ -246,6 +250,9 @@ __global__ void __launch_bounds__(BLOCK) vote_kernel(VoteArgs a) {
         if (ci < nCells) {
             sCellStart[ci] = cStart[j];
             sCellPre[ci]   = pre;
+            // chunk starts 64 t inside [pre, pre + cnt) begin in this cell
+            for (int t = (pre + 63) >> 6; t < min((pre + cCnt[j] + 63) >> 6, VOTE_MAX_CHUNKS); t++)
+                sChunkCell[t] = ci;
         }
         pre += cCnt[j];
     }
@@ -306,18 +313,28 @@ __global__ void __launch_bounds__(BLOCK) vote_kernel(VoteArgs a) {
         const int  c    = t * 64 + lane;
         int        pc = 0, s = 0;
         if (more && c < total) {
-            int lo = 0, hi = nCells - 1;  // largest cell with sCellPre[cell] <= c
-            while (lo < hi) {
-                int mid = (lo + hi + 1) >> 1;
-                if (sCellPre[mid] <= c)
-                    lo = mid;
-                else
-                    hi = mid - 1;
+            int lo;  // the cell holding candidate c: largest cell with sCellPre[cell] <= c
+            if (t < VOTE_MAX_CHUNKS) {
+                lo = sChunkCell[t];  // holds candidate 64 t; c is at most 63 further
+                while (sCellPre[lo + 1] <= c)
+                    lo++;
+            } else {
+                lo = 0;
+                int hi = nCells - 1;
+                while (lo < hi) {
+                    int mid = (lo + hi + 1) >> 1;
+                    if (sCellPre[mid] <= c)
+                        lo = mid;
+                    else
+                        hi = mid - 1;
+                }
             }
-            s        = sCellStart[lo] + (c - sCellPre[lo]);
-            bool in  = ppfc::dist2(qx, qy, qz, g.x[s], g.y[s], g.z[s]) < a.D2;
+            s               = sCellStart[lo] + (c - sCellPre[lo]);
+            const float px  = g.x[s], py = g.y[s], pz = g.z[s];
+            const int   pid = g.id[s];
+            bool        in  = ppfc::dist2(qx, qy, qz, px, py, pz) < a.D2;
             inSphere += in ? 1 : 0;
-            if (in && g.id[s] != kpId) {
+            if (in && pid != kpId) {
                 pc = 1;
                 nPairs++;
             }
@@ -344,9 +361,53 @@ __global__ void __launch_bounds__(BLOCK) vote_kernel(VoteArgs a) {
         wave_sync();
         pair_stage(pn);
     }
-    if (qn > 0) {
-        wave_sync();
-        wave_vote<QCAP>(a, acc, r0, r1, qStart[wv], qLen[wv], qAlpha[wv], qPre[wv], qn, nVotes);
+    // every wave holds < 64 queued hits: vote them all with the whole
+    // workgroup, so no wave's last queue delays the others
+    if (lane == 0)
+        sQn[wv] = qn;
+    __syncthreads();
+    {
+        const int w = tid >> 6, i = tid & 63;  // entry i of wave w's queue (BLOCK = 64 NW)
+        const int L = i < sQn[w] ? (int)qLen[w][i] : 0;
+        int       totSeg;
+        const int pre = block_excl_scan<BLOCK>((L + SEG - 1) / SEG, sScan, totSeg);
+        int      *flat = &pQueue[0][0];  // free now: segment prefix of entry 64 w + i
+        flat[tid]      = pre;
+        __syncthreads();
+        for (int t = tid; t < totSeg; t += BLOCK) {
+            int lo = 0, hi = BLOCK - 1;  // last entry whose prefix <= t (entries hold >= 1 segment)
+            while (lo < hi) {
+                int mid = (lo + hi + 1) >> 1;
+                if (flat[mid] <= t)
+                    lo = mid;
+                else
+                    hi = mid - 1;
+            }
+            const int    ew = lo >> 6, ei = lo & 63;
+            const int    Le = (int)qLen[ew][ei];
+            const int    e0 = (t - flat[lo]) * SEG;
+            const int    cnt = min(SEG, Le - e0);
+            const bool   tail = e0 >= Le - (Le & 7);
+            const float  aS   = qAlpha[ew][ei];
+            const uint2 *ent  = a.entries + qStart[ew][ei] + e0;
+            uint2        v[SEG];
+#pragma unroll
+            for (int kk = 0; kk < SEG; kk++)
+                if (kk < cnt)
+                    v[kk] = ent[kk];
+#pragma unroll
+            for (int kk = 0; kk < SEG; kk++) {
+                if (kk >= cnt)
+                    break;
+                int ref = (int)v[kk].x;
+                if (ref >= r0 && ref < r1) {
+                    int bin = ppfc::angle_bin(__uint_as_float(v[kk].y), aS, a.sMaxId, a.angleNum,
+                                              tail);
+                    atomicAdd(&acc[(ref - r0) * a.angleNum + bin], 1u);
+                    nVotes++;
+                }
+            }
+        }
     }
     __syncthreads();
 
@@ -391,7 +452,7 @@ __global__ void __launch_bounds__(BLOCK) vote_kernel(VoteArgs a) {
     // ---- peaks (ppf.cpp:363-391)
     const float thre = a.voteThreshold / 2.0f;
     const int   A    = a.angleNum, maxI = A - 1;
-    unsigned long long t0 = 0, t1 = 0, t2 = 0;
+    Top3 top{0ull, 0ull, 0ull};
     for (int r = tid; r < nrTile; r += BLOCK) {
         const uint32_t *bins = acc + r * A;
         int sum = 0, j = 0, best = -1;
@@ -410,31 +471,29 @@ __global__ void __launch_bounds__(BLOCK) vote_kernel(VoteArgs a) {
         unsigned long long code = ((unsigned long long)(uint32_t)v << 32) |
                                   ((unsigned long long)(0xFFFFFu - (uint32_t)(r0 + r)) << 11) |
                                   (uint32_t)j;
-        top3_insert(code, t0, t1, t2);
+        top = top3_add(top, code);
     }
     for (int o = 32; o > 0; o >>= 1) {
-        unsigned long long u0 = __shfl_xor(t0, o, 64), u1 = __shfl_xor(t1, o, 64),
-                           u2 = __shfl_xor(t2, o, 64);
-        top3_insert(u0, t0, t1, t2);
-        top3_insert(u1, t0, t1, t2);
-        top3_insert(u2, t0, t1, t2);
+        const unsigned long long u0 = __shfl_xor(top.a, o, 64), u1 = __shfl_xor(top.b, o, 64),
+                                 u2 = __shfl_xor(top.c, o, 64);
+        top = top3_add(top3_add(top3_add(top, u0), u1), u2);
     }
     // after the xor-butterfly every lane holds the wave's top-3 (duplicates
     // cannot arise: each lane inserts distinct codes from disjoint refs, and
     // the butterfly merges disjoint halves at each level)
     if ((tid & 63) == 0) {
-        sTop[(tid >> 6) * 3 + 0] = t0;
-        sTop[(tid >> 6) * 3 + 1] = t1;
-        sTop[(tid >> 6) * 3 + 2] = t2;
+        sTop[(tid >> 6) * 3 + 0] = top.a;
+        sTop[(tid >> 6) * 3 + 1] = top.b;
+        sTop[(tid >> 6) * 3 + 2] = top.c;
     }
     __syncthreads();
     if (tid == 0) {
-        unsigned long long b0 = 0, b1 = 0, b2 = 0;
+        Top3 best{0ull, 0ull, 0ull};
         for (int w = 0; w < (BLOCK / 64) * 3; w++)
-            top3_insert(sTop[w], b0, b1, b2);
-        candOut[0] = b0;
-        candOut[1] = b1;
-        candOut[2] = b2;
+            best = top3_add(best, sTop[w]);
+        candOut[0] = best.a;
+        candOut[1] = best.b;
+        candOut[2] = best.c;
     }
 }
 
@@ -447,10 +506,11 @@ __global__ void poses_kernel(const unsigned long long *cand, int nk, int nTiles,
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nk)
         return;
-    unsigned long long b0 = 0, b1 = 0, b2 = 0;
+    Top3 best{0ull, 0ull, 0ull};
     for (int t = 0; t < nTiles * 3; t++)
-        top3_insert(cand[(size_t)k * nTiles * 3 + t], b0, b1, b2);
-    unsigned long long bs[3] = {b0, b1, b2};
+        best = top3_add(best, cand[(size_t)k * nTiles * 3 + t]);
+    const unsigned long long b0 = best.a;
+    unsigned long long bs[3] = {best.a, best.b, best.c};
     int n = 0;
     if (b0) {
         float bestVote = (float)(uint32_t)(b0 >> 32);
