@@ -311,6 +311,98 @@ __device__ inline int grid_nn1(const GridView &g, float qx, float qy, float qz, 
     return bestS;
 }
 
+// cell (dx, dy, dz) number j of the Chebyshev shell of radius r >= 1, in
+// the order: the z = -r, +r faces, the y = -r, +r faces (|dz| < r), the
+// x = -r, +r faces (|dy|, |dz| < r); (2r+1)^3 - (2r-1)^3 cells in all
+__device__ __forceinline__ void shell_cell(int r, int j, int &dx, int &dy, int &dz) {
+    const int side = 2 * r + 1, inner = 2 * r - 1;
+    const int n1 = side * side, n2 = side * inner, n3 = inner * inner;
+    if (j < 2 * n1) {
+        dz = j < n1 ? -r : r;
+        j  = j < n1 ? j : j - n1;
+        dx = j % side - r;
+        dy = j / side - r;
+    } else if ((j -= 2 * n1) < 2 * n2) {
+        dy = j < n2 ? -r : r;
+        j  = j < n2 ? j : j - n2;
+        dx = j % side - r;
+        dz = j / side - (r - 1);
+    } else {
+        j -= 2 * n2;
+        dx = j < n3 ? -r : r;
+        j  = j < n3 ? j : j - n3;
+        dy = j % inner - (r - 1);
+        dz = j / inner - (r - 1);
+    }
+}
+
+// grid_nn1 by a group of G consecutive lanes (G a power of two <= 64) that
+// all pass the same query: each lane visits every G-th cell of a ring, the
+// group merges its candidates by (dist2, id) after every ring, and every
+// lane returns the same result as grid_nn1 would.  Control flow is uniform
+// within a group.
+template <int G>
+__device__ int grid_nn1_group(const GridView &g, float qx, float qy, float qz, float &bestD,
+                              int &bestId, float maxD2) {
+    const int gl = (int)(threadIdx.x & (G - 1));
+    bestD        = FLT_MAX;
+    bestId       = 0x7fffffff;
+    int bestS    = -1;
+    if (g.n == 0 || !(isfinite(qx) && isfinite(qy) && isfinite(qz)))
+        return -1;
+    const int c0x  = cell_coord_unclamped(qx, g.ox, g.cell);
+    const int c0y  = cell_coord_unclamped(qy, g.oy, g.cell);
+    const int c0z  = cell_coord_unclamped(qz, g.oz, g.cell);
+    const int maxR = max(max(g.dx, g.dy), g.dz) + max(max(abs(c0x), abs(c0y)), abs(c0z)) + 1;
+    for (int r = first_ring(g, c0x, c0y, c0z); r <= maxR; r++) {
+        const int   nShell = r == 0 ? 1 : (2 * r + 1) * (2 * r + 1) * (2 * r + 1) -
+                                             (2 * r - 1) * (2 * r - 1) * (2 * r - 1);
+        const float bound2 = fminf(bestD, maxD2);
+        for (int j = gl; j < nShell; j += G) {
+            int dx = 0, dy = 0, dz = 0;
+            if (r > 0)
+                shell_cell(r, j, dx, dy, dz);
+            const int cx = c0x + dx, cy = c0y + dy, cz = c0z + dz;
+            if (cx < 0 || cx >= g.dx || cy < 0 || cy >= g.dy || cz < 0 || cz >= g.dz)
+                continue;
+            if (coarse_empty(g, cx, cy, cz) || cell_min_d2(g, cx, cy, cz, qx, qy, qz) > bound2)
+                continue;
+            int s0, c;
+            if (!cell_lookup(g, cx, cy, cz, s0, c))
+                continue;
+            for (int t = s0; t < s0 + c; t++) {
+                float dd = ppfc::dist2(qx, qy, qz, g.x[t], g.y[t], g.z[t]);
+                int   ii = g.id[t];
+                if (dd > maxD2)
+                    continue;
+                if (dd < bestD || (dd == bestD && ii < bestId)) {
+                    bestD  = dd;
+                    bestId = ii;
+                    bestS  = t;
+                }
+            }
+        }
+#pragma unroll
+        for (int o = G / 2; o > 0; o >>= 1) {
+            const float od = __shfl_xor(bestD, o, 64);
+            const int   oi = __shfl_xor(bestId, o, 64), os = __shfl_xor(bestS, o, 64);
+            if (od < bestD || (od == bestD && oi < bestId)) {
+                bestD  = od;
+                bestId = oi;
+                bestS  = os;
+            }
+        }
+        const float safe = ring_safe(g, c0x, c0y, c0z, r, qx, qy, qz);
+        if (safe == FLT_MAX)
+            break;
+        if (bestS >= 0 && bestD < safe * safe)
+            break;
+        if (maxD2 < safe * safe)
+            break;
+    }
+    return bestS;
+}
+
 // true if any indexed point has dist2 < r2
 __device__ inline bool grid_any_within(const GridView &g, float qx, float qy, float qz, float r2) {
     if (g.n == 0 || !(isfinite(qx) && isfinite(qy) && isfinite(qz)))

@@ -221,15 +221,9 @@ __device__ __forceinline__ HypoScratch scratch_of(const IcpArgs &a, int h, int t
 // findCorresponds for one source point (icp.cpp:123-167): nearest scene
 // point, rejection beyond rejectDist, per-scene-point dedupe by 64-bit
 // atomicMin on (dist2 bits, source index)
-__device__ void corr_point(const IcpArgs &a, const HypoScratch &x, int i, const float *curF,
-                           float rej2) {
-    float px, py, pz;
-    xform(curF, a.sx[i], a.sy[i], a.sz[i], px, py, pz);
-    float d2;
-    int   did;
-    // the pair is rejected beyond rejectDist, so the search stops there
-    int   s  = grid_nn1(a.dst, px, py, pz, d2, did, rej2);
-    int   hs = -1;
+__device__ __forceinline__ void record_corr(const HypoScratch &x, int i, int s, float d2, int did,
+                                            float rej2) {
+    int hs = -1;
     if (s >= 0 && !(d2 > rej2)) {
         uint32_t key = (uint32_t)did + 1u;
         uint32_t hh  = icp_hash(key) & (x.T - 1);
@@ -247,6 +241,30 @@ __device__ void corr_point(const IcpArgs &a, const HypoScratch &x, int i, const 
     x.nnSlot[i] = s;
     x.hslot[i]  = hs;
     x.nnD[i]    = d2;
+}
+
+__device__ void corr_point(const IcpArgs &a, const HypoScratch &x, int i, const float *curF,
+                           float rej2) {
+    float px, py, pz;
+    xform(curF, a.sx[i], a.sy[i], a.sz[i], px, py, pz);
+    float d2;
+    int   did;
+    // the pair is rejected beyond rejectDist, so the search stops there
+    int s = grid_nn1(a.dst, px, py, pz, d2, did, rej2);
+    record_corr(x, i, s, d2, did, rej2);
+}
+
+// the same by a group of G lanes sharing the search; lane 0 records
+template <int G>
+__device__ void corr_point_group(const IcpArgs &a, const HypoScratch &x, int i, const float *curF,
+                                 float rej2) {
+    float px, py, pz;
+    xform(curF, a.sx[i], a.sy[i], a.sz[i], px, py, pz);
+    float d2;
+    int   did;
+    int   s = grid_nn1_group<G>(a.dst, px, py, pz, d2, did, rej2);
+    if ((threadIdx.x & (G - 1)) == 0)
+        record_corr(x, i, s, d2, did, rej2);
 }
 
 __device__ __forceinline__ bool is_winner(const HypoScratch &x, int i) {
@@ -344,6 +362,26 @@ __device__ __forceinline__ void add_mse(const IcpArgs &a, const HypoScratch &x, 
         bound = FLT_MAX;
     grid_nn1(a.dst, qx, qy, qz, d2, did, bound);
     cf_add(m1[0], sqrtf(d2));
+}
+
+// the same by a group of G lanes sharing the search; lane 0 adds
+template <int G>
+__device__ __forceinline__ void add_mse_group(const IcpArgs &a, const HypoScratch &x, int i,
+                                              const float *curF, const float *UF, CF (&m1)[1]) {
+    if (!is_winner(x, i))
+        return;
+    float px, py, pz, qx, qy, qz;
+    xform(curF, a.sx[i], a.sy[i], a.sz[i], px, py, pz);
+    xform(UF, px, py, pz, qx, qy, qz);
+    const int s     = x.nnSlot[i];
+    float     bound = ppfc::dist2(qx, qy, qz, a.dst.x[s], a.dst.y[s], a.dst.z[s]);
+    float     d2;
+    int       did;
+    if (!(bound < FLT_MAX))
+        bound = FLT_MAX;
+    grid_nn1_group<G>(a.dst, qx, qy, qz, d2, did, bound);
+    if ((threadIdx.x & (G - 1)) == 0)
+        cf_add(m1[0], sqrtf(d2));
 }
 
 __device__ __forceinline__ void clear_point(const HypoScratch &x, int i) {
@@ -577,8 +615,12 @@ struct SpreadArgs {
     uint32_t *arrive;  // H arrival counters, zero between launches
     CF       *part;    // H x nChunk x 27 per-chunk partial sums
     int      *inl;     // H inlier counts
-    int       nChunk;
+    int       nChunk;  // workgroups per hypothesis, one point per lane
+    int       nChunkG; // workgroups per hypothesis, one point per ICP_G lanes
 };
+
+// lanes sharing one nearest-neighbour search in the spread refiner
+constexpr int ICP_G = 8;
 
 __device__ __forceinline__ bool hypo_active(const IcpState &st, const IcpCriteria &c) {
     return !st.stop && st.iters < c.iterations;
@@ -588,8 +630,8 @@ __device__ __forceinline__ bool hypo_active(const IcpState &st, const IcpCriteri
 // last workgroup of hypothesis h to arrive, with the chunk-ordered totals in
 // red[0..M).
 template <int BLOCK, int M>
-__device__ bool arrive_reduce(const SpreadArgs &w, int h, int chunk, CF (&v)[M], CF *lds,
-                              double *red, int *sLast) {
+__device__ bool arrive_reduce(const SpreadArgs &w, int h, int chunk, int nParts, CF (&v)[M],
+                              CF *lds, double *red, int *sLast) {
 #pragma unroll
     for (int k = 0; k < M; k++)
         for (int o = 1; o < 64; o <<= 1) {
@@ -604,7 +646,7 @@ __device__ bool arrive_reduce(const SpreadArgs &w, int h, int chunk, CF (&v)[M],
         for (int k = 0; k < M; k++)
             lds[wid * M + k] = v[k];
     __syncthreads();
-    CF *mine = w.part + ((size_t)h * w.nChunk + chunk) * 27;
+    CF *mine = w.part + ((size_t)h * nParts + chunk) * 27;
     if (threadIdx.x < M) {
         CF acc = lds[threadIdx.x];
         for (int q = 1; q < BLOCK / 64; q++)
@@ -614,15 +656,15 @@ __device__ bool arrive_reduce(const SpreadArgs &w, int h, int chunk, CF (&v)[M],
     }
     __syncthreads();
     if (threadIdx.x == 0)
-        *sLast = (atomicAdd(&w.arrive[h], 1u) == (uint32_t)(w.nChunk - 1)) ? 1 : 0;
+        *sLast = (atomicAdd(&w.arrive[h], 1u) == (uint32_t)(nParts - 1)) ? 1 : 0;
     __syncthreads();
     if (!*sLast)
         return false;
     __threadfence();
     if (threadIdx.x < M) {
-        const CF *all = w.part + (size_t)h * w.nChunk * 27;
+        const CF *all = w.part + (size_t)h * nParts * 27;
         CF        acc = all[threadIdx.x];
-        for (int q = 1; q < w.nChunk; q++)
+        for (int q = 1; q < nParts; q++)
             acc = cf_merge(acc, all[(size_t)q * 27 + threadIdx.x]);
         red[threadIdx.x] = (double)acc.hi + (double)acc.lo;
     }
@@ -649,7 +691,7 @@ __global__ void spread_init_kernel(SpreadArgs w) {
 
 template <int BLOCK>
 __global__ void __launch_bounds__(BLOCK) spread_corr_kernel(SpreadArgs w) {
-    const int h = blockIdx.x / w.nChunk, chunk = blockIdx.x - h * w.nChunk;
+    const int h = blockIdx.x / w.nChunkG, chunk = blockIdx.x - h * w.nChunkG;
     __shared__ float sCur[12];
     __shared__ int   sTab;
     const IcpState  &st = w.st[h];
@@ -660,9 +702,10 @@ __global__ void __launch_bounds__(BLOCK) spread_corr_kernel(SpreadArgs w) {
     if (threadIdx.x == 0)
         sTab = st.iters & 1;
     __syncthreads();
-    const int i = chunk * BLOCK + threadIdx.x;
+    const int i = chunk * (BLOCK / ICP_G) + (int)threadIdx.x / ICP_G;
     if (i < w.b.nSrc)
-        corr_point(w.b, scratch_of(w.b, h, sTab), i, sCur, w.b.c.rejectDist * w.b.c.rejectDist);
+        corr_point_group<ICP_G>(w.b, scratch_of(w.b, h, sTab), i, sCur,
+                                w.b.c.rejectDist * w.b.c.rejectDist);
 }
 
 template <int BLOCK>
@@ -697,7 +740,7 @@ __global__ void __launch_bounds__(BLOCK) spread_means_kernel(SpreadArgs w) {
     const int i = chunk * BLOCK + threadIdx.x;
     if (i < w.b.nSrc)
         add_means(w.b, scratch_of(w.b, h, sTab), i, sCur, m7);
-    if (!arrive_reduce<BLOCK, 7>(w, h, chunk, m7, sLds, sRed, &sLast))
+    if (!arrive_reduce<BLOCK, 7>(w, h, chunk, w.nChunk, m7, sLds, sRed, &sLast))
         return;
     if (threadIdx.x == 0) {
         const int valid = (int)sRed[6];
@@ -735,7 +778,7 @@ __global__ void __launch_bounds__(BLOCK) spread_eq_kernel(SpreadArgs w) {
     const int i = chunk * BLOCK + threadIdx.x;
     if (i < w.b.nSrc)
         add_eq(w.b, scratch_of(w.b, h, sTab), i, sCur, sMeans, w.b.mode, e27);
-    if (!arrive_reduce<BLOCK, 27>(w, h, chunk, e27, sLds, sRed, &sLast))
+    if (!arrive_reduce<BLOCK, 27>(w, h, chunk, w.nChunk, e27, sLds, sRed, &sLast))
         return;
     if (threadIdx.x == 0)
         solve_update(sRed, sMeans, w.b.mode, st);
@@ -743,7 +786,7 @@ __global__ void __launch_bounds__(BLOCK) spread_eq_kernel(SpreadArgs w) {
 
 template <int BLOCK>
 __global__ void __launch_bounds__(BLOCK) spread_mse_kernel(SpreadArgs w) {
-    const int h = blockIdx.x / w.nChunk, chunk = blockIdx.x - h * w.nChunk;
+    const int h = blockIdx.x / w.nChunkG, chunk = blockIdx.x - h * w.nChunkG;
     __shared__ float  sCur[12], sU[12];
     __shared__ int    sTab, sLast;
     __shared__ CF     sLds[BLOCK / 64];
@@ -759,10 +802,10 @@ __global__ void __launch_bounds__(BLOCK) spread_mse_kernel(SpreadArgs w) {
         sTab = st.iters & 1;
     __syncthreads();
     CF        m1[1] = {CF{0.0f, 0.0f}};
-    const int i     = chunk * BLOCK + threadIdx.x;
+    const int i     = chunk * (BLOCK / ICP_G) + (int)threadIdx.x / ICP_G;
     if (i < w.b.nSrc)
-        add_mse(w.b, scratch_of(w.b, h, sTab), i, sCur, sU, m1);
-    if (!arrive_reduce<BLOCK, 1>(w, h, chunk, m1, sLds, sRed, &sLast))
+        add_mse_group<ICP_G>(w.b, scratch_of(w.b, h, sTab), i, sCur, sU, m1);
+    if (!arrive_reduce<BLOCK, 1>(w, h, chunk, w.nChunkG, m1, sLds, sRed, &sLast))
         return;
     if (threadIdx.x == 0)
         converge(st, (float)(sRed[0] / (double)st.valid), w.b.c);
@@ -970,16 +1013,17 @@ static void launch_icp(const IcpArgs &a, int block, hipStream_t s) {
 
 template <int BLOCK>
 static void launch_spread_t(const SpreadArgs &w, hipStream_t s) {
-    const long long nb = (long long)w.b.H * w.nChunk;
+    const long long nb = (long long)w.b.H * w.nChunkG;
     if (nb > 0x7fffffffLL)
         throw std::runtime_error("ICP batch too large for one launch");
-    const int g = (int)nb;
+    const int g  = (int)((long long)w.b.H * w.nChunk);
+    const int gG = (int)nb;
     spread_init_kernel<<<std::min(4096, div_up((long long)2 * w.b.H * w.b.T, 256)), 256, 0, s>>>(w);
     for (int it = 0; it < w.b.c.iterations; it++) {
-        spread_corr_kernel<BLOCK><<<g, BLOCK, 0, s>>>(w);
+        spread_corr_kernel<BLOCK><<<gG, BLOCK, 0, s>>>(w);
         spread_means_kernel<BLOCK><<<g, BLOCK, 0, s>>>(w);
         spread_eq_kernel<BLOCK><<<g, BLOCK, 0, s>>>(w);
-        spread_mse_kernel<BLOCK><<<g, BLOCK, 0, s>>>(w);
+        spread_mse_kernel<BLOCK><<<gG, BLOCK, 0, s>>>(w);
     }
     spread_inlier_kernel<BLOCK><<<g, BLOCK, 0, s>>>(w);
 }
@@ -1016,11 +1060,12 @@ void icp_batch(Workspace &ws, int nSrc, const float *sx, const float *sy, const 
     std::memset(&w, 0, sizeof w);
     if (cfg.spread) {
         w.b      = a;
-        w.nChunk = div_up(std::max(nSrc, 1), block);
+        w.nChunk  = div_up(std::max(nSrc, 1), block);
+        w.nChunkG = div_up(std::max(nSrc, 1), block / ICP_G);
         w.st     = ws.icpSt.as<IcpState>(H);
         w.arrive = ws.icpArrive.as<uint32_t>((size_t)2 * H);
         w.inl    = reinterpret_cast<int *>(w.arrive + H);
-        w.part   = ws.icpPart.as<CF>((size_t)H * w.nChunk * 27);
+        w.part   = ws.icpPart.as<CF>((size_t)H * w.nChunkG * 27);
     }
     auto launch = [&] {
         if (cfg.spread)
