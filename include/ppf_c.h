@@ -87,6 +87,9 @@ typedef struct {
     float  msSample, msNormals, msKeypoints, msVote, msCluster, msIcpPrepare, msIcp, msTotal;
     float  msVoteKernel; /* device time of the vote kernel alone */
     float  msIcpSparse, msIcpDense; /* device time of the two batched refinements */
+    float  msPairKernel;  /* of msVoteKernel: the pair/key-hit kernel (split vote path) */
+    float  msHitsKernel;  /* of msVoteKernel: hit voting + peaks (split vote path) */
+    int    nVoteFallback; /* key points voted by the fused kernel for lack of hit capacity */
 } ppf_match_stats;
 
 /* -- parameters (TrainParam/MatchParam constructors, type.cpp:71-127) -- */
@@ -133,8 +136,8 @@ PPF_API int ppf_match_scene_resident(ppf_detector d, ppf_scene s, float sampling
                                      size_t cap, size_t *nOut, ppf_match_stats *stats);
 
 /* -- launch tuning (no reference counterpart).  ppf_autotune times the
- * vote kernel over workgroup sizes {256, 512, 1024} x XCD-aware key-point
- * mapping {off, on} x vote-grid cell {D/2, D/3, D/4}, then the batched ICP
+ * vote stage over workgroup sizes {256, 512, 1024} x {split, fused} kernels
+ * x vote-grid cell {D/2, D/3, D/4}, then the batched ICP
  * over {spread, persistent} x {64, 128, 256, 512, 1024} threads per
  * workgroup, then the normals grid cell over {1/4, 1/2, 1} sample step, on
  * the given resident scene, and keeps the fastest in the detector.  None of
@@ -146,6 +149,9 @@ typedef struct {
     int icpBlock;     /* threads per ICP workgroup: 64 .. 1024, a power of two */
     int icpSpread;    /* 1: each hypothesis spread over workgroups, 0: persistent */
     int normCell4;    /* normals grid cell = sample step * normCell4 / 4: 1, 2 or 4 */
+    int voteSplit;    /* 1: pair kernel + hit voting kernel, 0: one fused kernel */
+    int voteHitCap;   /* key hits kept per key point on the split path (>= 1); key points
+                         with more are voted by the fused kernel */
 } ppf_tuning;
 typedef struct {
     ppf_tuning best;

@@ -45,7 +45,9 @@ class MatchStats(C.Structure):
                [(n, C.c_ulonglong) for n in ("nPairs", "nKeyHits", "nVotes")] + \
                [(n, C.c_float) for n in ("msSample", "msNormals", "msKeypoints", "msVote",
                                          "msCluster", "msIcpPrepare", "msIcp", "msTotal",
-                                         "msVoteKernel", "msIcpSparse", "msIcpDense")]
+                                         "msVoteKernel", "msIcpSparse", "msIcpDense",
+                                         "msPairKernel", "msHitsKernel")] + \
+               [("nVoteFallback", C.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -71,7 +73,8 @@ class IcpResult(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [(n, C.c_int) for n in ("voteBlock", "voteXcdRemap", "voteCellDiv", "icpBlock",
-                                       "icpSpread", "normCell4")]
+                                       "icpSpread", "normCell4", "voteSplit",
+                                       "voteHitCap")]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -607,6 +607,9 @@ void match_impl(ppf_detector_s &D, DevCloud &c, float rel, float kpFrac, float m
     st->msKeypoints     = elapsed(D.ev[2], D.ev[3]);
     st->msVote          = elapsed(D.ev[3], D.ev[5]);
     st->msVoteKernel    = vs.kernelMs;
+    st->msPairKernel    = vs.pairMs;
+    st->msHitsKernel    = vs.hitsMs;
+    st->nVoteFallback   = vs.overflow;
     st->msCluster       = elapsed(D.ev[5], D.ev[6]);
     st->msIcpPrepare    = elapsed(D.ev[6], D.ev[7]);
     st->msIcp           = elapsed(D.ev[7], D.ev[8]);
@@ -1020,6 +1023,8 @@ ppf_tuning tuning_of(const ppf_detector_s &d) {
     t.icpBlock     = d.icfg.block;
     t.icpSpread    = d.icfg.spread;
     t.normCell4    = d.normCell4;
+    t.voteSplit    = d.vcfg.split;
+    t.voteHitCap   = d.vcfg.hitCap;
     return t;
 }
 }  // namespace
@@ -1041,11 +1046,15 @@ PPF_API int ppf_set_tuning(ppf_detector d, const ppf_tuning *t) {
     d->vcfg.block    = t->voteBlock;
     d->vcfg.xcdRemap = t->voteXcdRemap ? 1 : 0;
     d->vcfg.cellDiv  = t->voteCellDiv;
+    PPF_REQUIRE(t->voteHitCap >= 1 && t->voteHitCap <= (1 << 20),
+                "vote hit capacity must be 1 .. 2^20");
     PPF_REQUIRE(t->normCell4 == 1 || t->normCell4 == 2 || t->normCell4 == 4,
                 "normals cell must be 1, 2 or 4 quarters of the sample step");
     d->icfg.block    = t->icpBlock;
     d->icfg.spread   = t->icpSpread ? 1 : 0;
     d->normCell4     = t->normCell4;
+    d->vcfg.split    = t->voteSplit ? 1 : 0;
+    d->vcfg.hitCap   = t->voteHitCap;
     return PPF_OK;
 }
 
@@ -1073,10 +1082,10 @@ PPF_API int ppf_autotune(ppf_detector d, ppf_scene sc, float rel, float kpFrac, 
             }
         };
         for (int block : {256, 512, 1024})
-            for (int remap : {0, 1})
+            for (int split : {0, 1})
                 for (int div : {2, 3, 4}) {
                     d->vcfg.block    = block;
-                    d->vcfg.xcdRemap = remap;
+                    d->vcfg.split    = split;
                     d->vcfg.cellDiv  = div;
                     float v, i;
                     run(v, i);

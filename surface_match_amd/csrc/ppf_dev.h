@@ -25,7 +25,7 @@ struct Workspace {
     Grid   gNormal, gVote, gSparse, gDense, gFull;
     // vote
     DevBuf kpX, kpY, kpZ, kpNX, kpNY, kpNZ, cand, candTile, poses, voteStats, accDump, voteCnt,
-        smallCnt;
+        smallCnt, hitStart, hitLen, hitAlpha, hitCount;
     // clustering
     DevBuf clSortKey, clSortKey2, clSortVal, clSortVal2, clStatus, clPred, clPredOff,
         clCenters, clNms, clMembers;
@@ -35,7 +35,7 @@ struct Workspace {
     GraphCache icpGraph[4];  // captured batched refinements, replaced round-robin
     int        icpGraphNext = 0;
     // events bracketing the vote kernel (created on first use)
-    hipEvent_t voteEv[2] = {nullptr, nullptr};
+    hipEvent_t voteEv[3] = {nullptr, nullptr, nullptr};
     ~Workspace() {
         for (auto &e : voteEv)
             if (e)
@@ -113,13 +113,18 @@ struct PoseRec {
 struct VoteStats {
     unsigned long long pairs, hits, votes, candidates;
     int   voted;      // key points that passed the neighbour-count threshold
-    float kernelMs;   // device time of the vote kernel launch alone
+    float kernelMs;   // device time of the vote stage kernels
+    float pairMs;     // of which the pair kernel (split path)
+    float hitsMs;     // of which hit voting and any fused fallback (split path)
+    int   overflow;   // key points voted by the fused kernel for lack of hit capacity
 };
 
 struct VoteConfig {
-    int block    = 1024;  // threads per workgroup
+    int block    = 1024;  // threads per voting workgroup
     int xcdRemap = 0;     // keep spatially adjacent keypoints on one XCD
     int cellDiv  = 4;     // vote grid cell = model diameter / cellDiv (2..4)
+    int split    = 1;     // 1: pair kernel + hit voting kernel; 0: fused kernel
+    int hitCap   = 2048;  // hits kept per key point on the split path
 };
 
 // Vote every keypoint (indices into the sampled-scene grid's clouds),

@@ -53,6 +53,14 @@ struct VoteArgs {
     int32_t     *accDump;       // [dumpCount * refNum * angleNum] (optional)
     int          dumpCount;
     int          xcdRemap;
+    const int   *kpSel;         // key points to vote (nullptr: all nk)
+    // split path: hits of key point k at [k * hitCap, k * hitCap + nHit[k])
+    int          hitCap;
+    uint32_t    *hitStart, *hitLen;
+    float       *hitAlpha;
+    int         *nHit, *nbr;    // hit count (-1: overflow), neighbour count incl. self
+    int         *overflow, *nOverflow;
+    int          countPairs;    // fused kernel adds its pairs and hits to stats
 };
 
 __device__ __forceinline__ bool table_find(const uint4 *slots, uint32_t tmask, uint32_t key,
@@ -160,79 +168,50 @@ __device__ __forceinline__ void wave_vote(const VoteArgs &a, uint32_t *acc, int 
     wave_sync();
 }
 
-template <int BLOCK>
-__global__ void __launch_bounds__(BLOCK) vote_kernel(VoteArgs a) {
-    constexpr int NW   = BLOCK / 64;
-    constexpr int QCAP = 128;  // >= 63 queued + 64 new hits
-    constexpr int CPT  = (VOTE_MAX_CELLS + BLOCK - 1) / BLOCK;
-    extern __shared__ uint32_t acc[];
-    __shared__ int      sCellStart[VOTE_MAX_CELLS];
-    __shared__ int      sCellPre[VOTE_MAX_CELLS + 1];
-    __shared__ uint32_t qStart[NW][QCAP];
-    __shared__ uint32_t qLen[NW][QCAP];
-    __shared__ float    qAlpha[NW][QCAP];
-    __shared__ int      qPre[NW][QCAP];
-    __shared__ int      pQueue[NW][QCAP];  // sorted-slot ids of queued pairs
-    __shared__ int      sChunkCell[VOTE_MAX_CHUNKS];  // cell holding candidate 64 t
-    __shared__ int      sQn[NW];
-    __shared__ int      sScan[NW + 1];
-    __shared__ int      sNext;
-    __shared__ unsigned long long sTop[NW * 3];
-    __shared__ int      sRed[NW];
+// ---- pieces shared by the kernels below
 
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    // Workgroups are dealt round-robin to the 8 XCDs (each with its own L2).
-    // With xcdRemap, XCD x works through a contiguous run of key points, so
-    // neighbouring key points -- which read the same scene cells and similar
-    // buckets -- share an L2.  The grid is padded to 8 * per * nTiles blocks.
-    int b    = blockIdx.x;
-    int tile = b % a.nTiles;
-    int kpBlock;
+// block -> (key point, tile).  Workgroups are dealt round-robin to the 8
+// XCDs (each with its own L2).  With xcdRemap, XCD x works through a
+// contiguous run of key points, so neighbouring key points -- which read the
+// same scene cells and similar buckets -- share an L2.  The grid is then
+// padded to 8 * per * nTiles blocks.  Returns -1 for a padding block.
+__device__ __forceinline__ int block_keypoint(const VoteArgs &a, int nTiles, int &tile) {
+    const int b = blockIdx.x;
+    tile        = b % nTiles;
+    int kb      = b / nTiles;
     if (a.xcdRemap) {
-        int kb  = b / a.nTiles;
         int per = (a.nk + 7) / 8;
-        kpBlock = (kb % 8) * per + kb / 8;
-    } else {
-        kpBlock = b / a.nTiles;
+        kb      = (kb % 8) * per + kb / 8;
     }
-    if (kpBlock >= a.nk)
-        return;
-    const int k      = kpBlock;
-    const int r0     = tile * a.tileRefs;
-    const int r1     = min(r0 + a.tileRefs, a.refNum);
-    const int nrTile = r1 - r0;
-    unsigned long long *candOut = a.cand + ((size_t)k * a.nTiles + tile) * 3;
+    if (kb >= a.nk)
+        return -1;
+    return a.kpSel ? a.kpSel[kb] : kb;
+}
 
-    const int   kpId = a.kp[k];
-    const float qx = a.PX[kpId], qy = a.PY[kpId], qz = a.PZ[kpId];
-    const float nx1 = a.NX[kpId], ny1 = a.NY[kpId], nz1 = a.NZ[kpId];
-    if (ppfc::is_nan(nx1) || ppfc::is_nan(ny1) || ppfc::is_nan(nz1)) {
-        if (tid < 3)
-            candOut[tid] = 0ull;
-        return;
-    }
-    float rt[12];
-    ppfc::frame(qx, qy, qz, nx1, ny1, nz1, rt);
-
-    for (int i = tid; i < nrTile * a.angleNum; i += BLOCK)
-        acc[i] = 0u;
-
-    // ---- cells overlapping the vote sphere (any order: voting commutes)
-    const GridView &g = a.g;
+// Cells of the sampled-scene grid overlapping the vote sphere around q (any
+// order: voting commutes), their candidate prefix and, for every chunk of 64
+// candidates below VOTE_MAX_CHUNKS, the cell holding its first candidate.
+// Returns the candidate count; the caller synchronises before use.
+template <int BLOCK>
+__device__ int vote_cells(const VoteArgs &a, float qx, float qy, float qz, int *sCellStart,
+                          int *sCellPre, int *sChunkCell, int *sScan, int &nCells) {
+    constexpr int   CPT = (VOTE_MAX_CELLS + BLOCK - 1) / BLOCK;
+    const GridView &g   = a.g;
+    const int       tid = threadIdx.x;
     int xlo = cell_coord(qx - a.searchR, g.ox, g.cell, g.dx), xhi = cell_coord(qx + a.searchR, g.ox, g.cell, g.dx);
     int ylo = cell_coord(qy - a.searchR, g.oy, g.cell, g.dy), yhi = cell_coord(qy + a.searchR, g.oy, g.cell, g.dy);
     int zlo = cell_coord(qz - a.searchR, g.oz, g.cell, g.dz), zhi = cell_coord(qz + a.searchR, g.oz, g.cell, g.dz);
     int ncx = xhi - xlo + 1, ncy = yhi - ylo + 1, ncz = zhi - zlo + 1;
-    int nCells = ncx * ncy * ncz;  // <= VOTE_MAX_CELLS: the host bounds cells per axis
+    nCells  = ncx * ncy * ncz;  // <= VOTE_MAX_CELLS: the host bounds cells per axis
     int cStart[CPT], cCnt[CPT], mySum = 0;
 #pragma unroll
     for (int j = 0; j < CPT; j++) {
-        int ci   = tid * CPT + j;
+        int ci    = tid * CPT + j;
         cStart[j] = 0;
         cCnt[j]   = 0;
         if (ci < nCells) {
             int cx = xlo + ci % ncx, cy = ylo + (ci / ncx) % ncy, cz = zlo + ci / (ncx * ncy);
-            if (cell_min_d2(g, cx, cy, cz, qx, qy, qz) <= a.D2) {
+            if (!coarse_empty(g, cx, cy, cz) && cell_min_d2(g, cx, cy, cz, qx, qy, qz) <= a.D2) {
                 int s0, c0;
                 if (cell_lookup(g, cx, cy, cz, s0, c0)) {
                     cStart[j] = s0;
@@ -256,10 +235,163 @@ __global__ void __launch_bounds__(BLOCK) vote_kernel(VoteArgs a) {
         }
         pre += cCnt[j];
     }
-    if (tid == 0) {
+    if (tid == 0)
         sCellPre[nCells] = total;
-        sNext            = 0;
+    return total;
+}
+
+// sorted-grid slot of candidate c (chunk t = c / 64)
+__device__ __forceinline__ int candidate_slot(int t, int c, int nCells, const int *sCellStart,
+                                              const int *sCellPre, const int *sChunkCell) {
+    int lo;  // the cell holding candidate c: largest cell with sCellPre[cell] <= c
+    if (t < VOTE_MAX_CHUNKS) {
+        lo = sChunkCell[t];  // holds candidate 64 t; c is at most 63 further
+        while (sCellPre[lo + 1] <= c)
+            lo++;
+    } else {
+        lo     = 0;
+        int hi = nCells - 1;
+        while (lo < hi) {
+            int mid = (lo + hi + 1) >> 1;
+            if (sCellPre[mid] <= c)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
     }
+    return sCellStart[lo] + (c - sCellPre[lo]);
+}
+
+// vote one segment (<= SEG entries starting at entry e0 of a bucket of length
+// L) into the accumulator tile [r0, r1)
+__device__ __forceinline__ void vote_segment(const VoteArgs &a, uint32_t *acc, int r0, int r1,
+                                             uint32_t start, int L, int e0, float aS,
+                                             unsigned long long &nVotes) {
+    const int    cnt  = min(SEG, L - e0);
+    const bool   tail = e0 >= L - (L & 7);
+    const uint2 *ent  = a.entries + start + e0;
+    uint2        v[SEG];
+#pragma unroll
+    for (int kk = 0; kk < SEG; kk++)
+        if (kk < cnt)
+            v[kk] = ent[kk];
+#pragma unroll
+    for (int kk = 0; kk < SEG; kk++) {
+        if (kk >= cnt)
+            break;
+        int ref = (int)v[kk].x;
+        if (ref >= r0 && ref < r1) {
+            int bin = ppfc::angle_bin(__uint_as_float(v[kk].y), aS, a.sMaxId, a.angleNum, tail);
+            atomicAdd(&acc[(ref - r0) * a.angleNum + bin], 1u);
+            nVotes++;
+        }
+    }
+}
+
+// peaks of a voted accumulator tile (ppf.cpp:363-391): the best three
+// (vote, ref, bin) codes of the tile
+template <int BLOCK>
+__device__ void peaks_top3(const VoteArgs &a, const uint32_t *acc, int nrTile, int r0,
+                           unsigned long long *sTop, unsigned long long *candOut) {
+    const int   tid  = threadIdx.x;
+    const float thre = a.voteThreshold / 2.0f;
+    const int   A    = a.angleNum, maxI = A - 1;
+    Top3        top{0ull, 0ull, 0ull};
+    for (int r = tid; r < nrTile; r += BLOCK) {
+        const uint32_t *bins = acc + r * A;
+        int sum = 0, j = 0, best = -1;
+        for (int x = 0; x < A; x++) {
+            int v = (int)bins[x];
+            sum += v;
+            if (v > best) {
+                best = v;
+                j    = x;
+            }
+        }
+        if ((float)sum < thre || (float)best < thre)
+            continue;
+        int v = best + (int)((j == 0) ? bins[maxI] : bins[j - 1]) +
+                (int)((j == maxI) ? bins[0] : bins[j + 1]);
+        unsigned long long code = ((unsigned long long)(uint32_t)v << 32) |
+                                  ((unsigned long long)(0xFFFFFu - (uint32_t)(r0 + r)) << 11) |
+                                  (uint32_t)j;
+        top = top3_add(top, code);
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        const unsigned long long u0 = __shfl_xor(top.a, o, 64), u1 = __shfl_xor(top.b, o, 64),
+                                 u2 = __shfl_xor(top.c, o, 64);
+        top = top3_add(top3_add(top3_add(top, u0), u1), u2);
+    }
+    // after the xor-butterfly every lane holds the wave's top-3 (duplicates
+    // cannot arise: each lane inserts distinct codes from disjoint refs, and
+    // the butterfly merges disjoint halves at each level)
+    if ((tid & 63) == 0) {
+        sTop[(tid >> 6) * 3 + 0] = top.a;
+        sTop[(tid >> 6) * 3 + 1] = top.b;
+        sTop[(tid >> 6) * 3 + 2] = top.c;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        Top3 best{0ull, 0ull, 0ull};
+        for (int w = 0; w < (BLOCK / 64) * 3; w++)
+            best = top3_add(best, sTop[w]);
+        candOut[0] = best.a;
+        candOut[1] = best.b;
+        candOut[2] = best.c;
+    }
+}
+
+// ---- fused kernel: pairs and votes in one workgroup per key point (the
+// path for key points whose hit list overflows the split path's capacity,
+// and a tuning choice on its own)
+template <int BLOCK>
+__global__ void __launch_bounds__(BLOCK) vote_kernel(VoteArgs a) {
+    constexpr int NW   = BLOCK / 64;
+    constexpr int QCAP = 128;  // >= 63 queued + 64 new hits
+    extern __shared__ uint32_t acc[];
+    __shared__ int      sCellStart[VOTE_MAX_CELLS];
+    __shared__ int      sCellPre[VOTE_MAX_CELLS + 1];
+    __shared__ uint32_t qStart[NW][QCAP];
+    __shared__ uint32_t qLen[NW][QCAP];
+    __shared__ float    qAlpha[NW][QCAP];
+    __shared__ int      qPre[NW][QCAP];
+    __shared__ int      pQueue[NW][QCAP];  // sorted-slot ids of queued pairs
+    __shared__ int      sChunkCell[VOTE_MAX_CHUNKS];  // cell holding candidate 64 t
+    __shared__ int      sQn[NW];
+    __shared__ int      sScan[NW + 1];
+    __shared__ int      sNext;
+    __shared__ unsigned long long sTop[NW * 3];
+    __shared__ int      sRed[NW];
+
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    int       tile;
+    const int k = block_keypoint(a, a.nTiles, tile);
+    if (k < 0)
+        return;
+    const int r0     = tile * a.tileRefs;
+    const int r1     = min(r0 + a.tileRefs, a.refNum);
+    const int nrTile = r1 - r0;
+    unsigned long long *candOut = a.cand + ((size_t)k * a.nTiles + tile) * 3;
+
+    const int   kpId = a.kp[k];
+    const float qx = a.PX[kpId], qy = a.PY[kpId], qz = a.PZ[kpId];
+    const float nx1 = a.NX[kpId], ny1 = a.NY[kpId], nz1 = a.NZ[kpId];
+    if (ppfc::is_nan(nx1) || ppfc::is_nan(ny1) || ppfc::is_nan(nz1)) {
+        if (tid < 3)
+            candOut[tid] = 0ull;
+        return;
+    }
+    float rt[12];
+    ppfc::frame(qx, qy, qz, nx1, ny1, nz1, rt);
+
+    for (int i = tid; i < nrTile * a.angleNum; i += BLOCK)
+        acc[i] = 0u;
+    const GridView &g = a.g;
+    int             nCells;
+    const int total = vote_cells<BLOCK>(a, qx, qy, qz, sCellStart, sCellPre, sChunkCell, sScan,
+                                        nCells);
+    if (tid == 0)
+        sNext = 0;
     __syncthreads();
 
     // ---- pairs + votes.  Each wave takes chunks of 64 candidates from a
@@ -313,23 +445,7 @@ __global__ void __launch_bounds__(BLOCK) vote_kernel(VoteArgs a) {
         const int  c    = t * 64 + lane;
         int        pc = 0, s = 0;
         if (more && c < total) {
-            int lo;  // the cell holding candidate c: largest cell with sCellPre[cell] <= c
-            if (t < VOTE_MAX_CHUNKS) {
-                lo = sChunkCell[t];  // holds candidate 64 t; c is at most 63 further
-                while (sCellPre[lo + 1] <= c)
-                    lo++;
-            } else {
-                lo = 0;
-                int hi = nCells - 1;
-                while (lo < hi) {
-                    int mid = (lo + hi + 1) >> 1;
-                    if (sCellPre[mid] <= c)
-                        lo = mid;
-                    else
-                        hi = mid - 1;
-                }
-            }
-            s               = sCellStart[lo] + (c - sCellPre[lo]);
+            s               = candidate_slot(t, c, nCells, sCellStart, sCellPre, sChunkCell);
             const float px  = g.x[s], py = g.y[s], pz = g.z[s];
             const int   pid = g.id[s];
             bool        in  = ppfc::dist2(qx, qy, qz, px, py, pz) < a.D2;
@@ -383,30 +499,9 @@ __global__ void __launch_bounds__(BLOCK) vote_kernel(VoteArgs a) {
                 else
                     hi = mid - 1;
             }
-            const int    ew = lo >> 6, ei = lo & 63;
-            const int    Le = (int)qLen[ew][ei];
-            const int    e0 = (t - flat[lo]) * SEG;
-            const int    cnt = min(SEG, Le - e0);
-            const bool   tail = e0 >= Le - (Le & 7);
-            const float  aS   = qAlpha[ew][ei];
-            const uint2 *ent  = a.entries + qStart[ew][ei] + e0;
-            uint2        v[SEG];
-#pragma unroll
-            for (int kk = 0; kk < SEG; kk++)
-                if (kk < cnt)
-                    v[kk] = ent[kk];
-#pragma unroll
-            for (int kk = 0; kk < SEG; kk++) {
-                if (kk >= cnt)
-                    break;
-                int ref = (int)v[kk].x;
-                if (ref >= r0 && ref < r1) {
-                    int bin = ppfc::angle_bin(__uint_as_float(v[kk].y), aS, a.sMaxId, a.angleNum,
-                                              tail);
-                    atomicAdd(&acc[(ref - r0) * a.angleNum + bin], 1u);
-                    nVotes++;
-                }
-            }
+            const int ew = lo >> 6, ei = lo & 63;
+            vote_segment(a, acc, r0, r1, qStart[ew][ei], (int)qLen[ew][ei], (t - flat[lo]) * SEG,
+                         qAlpha[ew][ei], nVotes);
         }
     }
     __syncthreads();
@@ -417,7 +512,7 @@ __global__ void __launch_bounds__(BLOCK) vote_kernel(VoteArgs a) {
         sRed[tid >> 6] = inSphere;
     __syncthreads();
     int searched = 0;
-    for (int w = 0; w < BLOCK / 64; w++)
+    for (int w = 0; w < NW; w++)
         searched += sRed[w];
     const bool voted = !((float)searched < a.voteThreshold);
 
@@ -430,7 +525,7 @@ __global__ void __launch_bounds__(BLOCK) vote_kernel(VoteArgs a) {
             candOut[tid] = 0ull;
         return;
     }
-    if (a.stats && tile == 0) {
+    if (a.stats && tile == 0 && a.countPairs) {
         for (int o = 32; o > 0; o >>= 1) {
             nPairs += __shfl_xor(nPairs, o, 64);
             nHits += __shfl_xor(nHits, o, 64);
@@ -439,62 +534,246 @@ __global__ void __launch_bounds__(BLOCK) vote_kernel(VoteArgs a) {
             atomicAdd(a.stats + 0, nPairs);
             atomicAdd(a.stats + 1, nHits);
         }
-        if (tid == 0)
-            atomicAdd(a.stats + 3, 1ull);
     }
+    if (a.stats && tile == 0 && tid == 0)
+        atomicAdd(a.stats + 3, 1ull);
     if (a.stats) {
         for (int o = 32; o > 0; o >>= 1)
             nVotes += __shfl_xor(nVotes, o, 64);
         if ((tid & 63) == 0)
             atomicAdd(a.stats + 2, nVotes);
     }
+    peaks_top3<BLOCK>(a, acc, nrTile, r0, sTop, candOut);
+}
 
-    // ---- peaks (ppf.cpp:363-391)
-    const float thre = a.voteThreshold / 2.0f;
-    const int   A    = a.angleNum, maxI = A - 1;
-    Top3 top{0ull, 0ull, 0ull};
-    for (int r = tid; r < nrTile; r += BLOCK) {
-        const uint32_t *bins = acc + r * A;
-        int sum = 0, j = 0, best = -1;
-        for (int x = 0; x < A; x++) {
-            int v = (int)bins[x];
-            sum += v;
-            if (v > best) {
-                best = v;
-                j    = x;
+// ---- split path, stage 1: the pairs of every key point, its neighbour
+// count and its key hits (bucket start, length, scene alpha), written to
+// HBM.  No accumulator, so several small workgroups share a CU and hide
+// each other's latency.  A key point with more than hitCap hits gets
+// nHit = -1 and is voted by the fused kernel instead.
+constexpr int PAIR_BLOCK = 256;
+
+__global__ void __launch_bounds__(PAIR_BLOCK) pair_kernel(VoteArgs a) {
+    constexpr int NW   = PAIR_BLOCK / 64;
+    constexpr int QCAP = 128;
+    __shared__ int sCellStart[VOTE_MAX_CELLS];
+    __shared__ int sCellPre[VOTE_MAX_CELLS + 1];
+    __shared__ int sChunkCell[VOTE_MAX_CHUNKS];
+    __shared__ int pQueue[NW][QCAP];
+    __shared__ int sScan[NW + 1];
+    __shared__ int sNext, sHits, sIn;
+    __shared__ unsigned long long sPairs[NW];
+
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    int       tile;
+    const int k = block_keypoint(a, 1, tile);
+    if (k < 0)
+        return;
+    const int   kpId = a.kp[k];
+    const float qx = a.PX[kpId], qy = a.PY[kpId], qz = a.PZ[kpId];
+    const float nx1 = a.NX[kpId], ny1 = a.NY[kpId], nz1 = a.NZ[kpId];
+    if (ppfc::is_nan(nx1) || ppfc::is_nan(ny1) || ppfc::is_nan(nz1)) {
+        if (tid == 0) {
+            a.nHit[k] = 0;
+            a.nbr[k]  = 0;
+        }
+        return;
+    }
+    float rt[12];
+    ppfc::frame(qx, qy, qz, nx1, ny1, nz1, rt);
+    const GridView &g = a.g;
+    int             nCells;
+    const int total = vote_cells<PAIR_BLOCK>(a, qx, qy, qz, sCellStart, sCellPre, sChunkCell,
+                                             sScan, nCells);
+    if (tid == 0) {
+        sNext = 0;
+        sHits = 0;
+        sIn   = 0;
+    }
+    __syncthreads();
+
+    const int          nChunks = (total + 63) >> 6;
+    const size_t       base    = (size_t)k * a.hitCap;
+    unsigned long long nPairs  = 0;
+    int                inSphere = 0, pn = 0;
+    int               *pq = pQueue[wv];
+    auto pair_stage = [&](int cnt) {
+        int      hit    = 0;
+        uint32_t bStart = 0, bLen = 0;
+        float    alphaS = 0.0f;
+        if (lane < cnt) {
+            const int s  = pq[lane];
+            float     px = g.x[s], py = g.y[s], pz = g.z[s];
+            alphaS       = ppfc::alpha_of(rt, px, py, pz);
+            uint32_t key = ppfc::ppf_key(qx, qy, qz, nx1, ny1, nz1, px, py, pz, g.nx[s], g.ny[s],
+                                         g.nz[s], a.angleStep, a.distStep, false);
+            if (table_find(a.slots, a.tmask, key, bStart, bLen) && !ppfc::is_nan(alphaS))
+                hit = 1;
+        }
+        const unsigned long long m = __ballot(hit);
+        const int                nh = __popcll(m);
+        if (nh == 0)
+            return;
+        int at = 0;
+        if (lane == 0)
+            at = atomicAdd(&sHits, nh);
+        at = __shfl(at, 0, 64);
+        if (hit) {
+            const int pos = at + __popcll(m & ((1ull << lane) - 1ull));
+            if (pos < a.hitCap) {
+                a.hitStart[base + pos] = bStart;
+                a.hitLen[base + pos]   = bLen;
+                a.hitAlpha[base + pos] = alphaS;
             }
         }
-        if ((float)sum < thre || (float)best < thre)
-            continue;
-        int v = best + (int)((j == 0) ? bins[maxI] : bins[j - 1]) +
-                (int)((j == maxI) ? bins[0] : bins[j + 1]);
-        unsigned long long code = ((unsigned long long)(uint32_t)v << 32) |
-                                  ((unsigned long long)(0xFFFFFu - (uint32_t)(r0 + r)) << 11) |
-                                  (uint32_t)j;
-        top = top3_add(top, code);
+    };
+    for (;;) {
+        int t = 0;
+        if (lane == 0)
+            t = atomicAdd(&sNext, 1);
+        t               = __shfl(t, 0, 64);
+        const bool more = t < nChunks;
+        const int  c    = t * 64 + lane;
+        int        pc = 0, s = 0;
+        if (more && c < total) {
+            s               = candidate_slot(t, c, nCells, sCellStart, sCellPre, sChunkCell);
+            const float px  = g.x[s], py = g.y[s], pz = g.z[s];
+            const int   pid = g.id[s];
+            bool        in  = ppfc::dist2(qx, qy, qz, px, py, pz) < a.D2;
+            inSphere += in ? 1 : 0;
+            if (in && pid != kpId) {
+                pc = 1;
+                nPairs++;
+            }
+        }
+        const unsigned long long m = __ballot(pc);
+        if (pc)
+            pq[pn + __popcll(m & ((1ull << lane) - 1ull))] = s;
+        pn += __popcll(m);
+        if (pn >= 64) {
+            wave_sync();
+            pair_stage(64);
+            const int rest = pn - 64;
+            const int v    = lane < rest ? pq[64 + lane] : 0;
+            wave_sync();
+            if (lane < rest)
+                pq[lane] = v;
+            pn = rest;
+        }
+        if (!more)
+            break;
+    }
+    if (pn > 0) {
+        wave_sync();
+        pair_stage(pn);
     }
     for (int o = 32; o > 0; o >>= 1) {
-        const unsigned long long u0 = __shfl_xor(top.a, o, 64), u1 = __shfl_xor(top.b, o, 64),
-                                 u2 = __shfl_xor(top.c, o, 64);
-        top = top3_add(top3_add(top3_add(top, u0), u1), u2);
+        inSphere += __shfl_xor(inSphere, o, 64);
+        nPairs += __shfl_xor(nPairs, o, 64);
     }
-    // after the xor-butterfly every lane holds the wave's top-3 (duplicates
-    // cannot arise: each lane inserts distinct codes from disjoint refs, and
-    // the butterfly merges disjoint halves at each level)
-    if ((tid & 63) == 0) {
-        sTop[(tid >> 6) * 3 + 0] = top.a;
-        sTop[(tid >> 6) * 3 + 1] = top.b;
-        sTop[(tid >> 6) * 3 + 2] = top.c;
+    if (lane == 0) {
+        atomicAdd(&sIn, inSphere);
+        sPairs[wv] = nPairs;
     }
     __syncthreads();
     if (tid == 0) {
-        Top3 best{0ull, 0ull, 0ull};
-        for (int w = 0; w < (BLOCK / 64) * 3; w++)
-            best = top3_add(best, sTop[w]);
-        candOut[0] = best.a;
-        candOut[1] = best.b;
-        candOut[2] = best.c;
+        const int nh = sHits;
+        a.nbr[k]     = sIn;
+        a.nHit[k]    = nh > a.hitCap ? -1 : nh;
+        if (nh > a.hitCap)
+            a.overflow[atomicAdd(a.nOverflow, 1)] = k;
+        // pairs and hits of voted key points, as the fused kernel counts them
+        if (a.stats && !((float)sIn < a.voteThreshold)) {
+            unsigned long long np = 0;
+            for (int w = 0; w < NW; w++)
+                np += sPairs[w];
+            atomicAdd(a.stats + 0, np);
+            atomicAdd(a.stats + 1, (unsigned long long)nh);
+        }
     }
+}
+
+// ---- split path, stage 2: vote the hits of one key point (and tile) into
+// the LDS accumulator, all lanes striped over the buckets' segments, then
+// the peaks.
+template <int BLOCK>
+__global__ void __launch_bounds__(BLOCK) vote_hits_kernel(VoteArgs a) {
+    extern __shared__ uint32_t acc[];
+    __shared__ int      sPre[BLOCK];
+    __shared__ uint32_t sStart[BLOCK];
+    __shared__ int      sLen[BLOCK];
+    __shared__ float    sAlpha[BLOCK];
+    __shared__ int      sScan[BLOCK / 64 + 1];
+    __shared__ unsigned long long sTop[(BLOCK / 64) * 3];
+
+    const int tid = threadIdx.x;
+    int       tile;
+    const int k = block_keypoint(a, a.nTiles, tile);
+    if (k < 0)
+        return;
+    const int r0     = tile * a.tileRefs;
+    const int r1     = min(r0 + a.tileRefs, a.refNum);
+    const int nrTile = r1 - r0;
+    unsigned long long *candOut = a.cand + ((size_t)k * a.nTiles + tile) * 3;
+    const int           n       = a.nHit[k];
+    // skip rule (ppf.cpp:273-276); an overflowed key point is voted by the
+    // fused kernel, which rewrites its candidates
+    const bool voted = !((float)a.nbr[k] < a.voteThreshold);
+    if (n < 0 || (!voted && !(k < a.dumpCount && a.accDump))) {
+        if (tid < 3)
+            candOut[tid] = 0ull;
+        return;
+    }
+    for (int i = tid; i < nrTile * a.angleNum; i += BLOCK)
+        acc[i] = 0u;
+    unsigned long long nVotes = 0;
+    const size_t       base   = (size_t)k * a.hitCap;
+    for (int h0 = 0; h0 < n; h0 += BLOCK) {
+        const int i = h0 + tid;
+        int       L = 0;
+        if (i < n) {
+            L            = (int)a.hitLen[base + i];
+            sStart[tid]  = a.hitStart[base + i];
+            sAlpha[tid]  = a.hitAlpha[base + i];
+            sLen[tid]    = L;
+        }
+        int       totSeg;
+        const int pre = block_excl_scan<BLOCK>((L + SEG - 1) / SEG, sScan, totSeg);
+        sPre[tid]     = pre;
+        __syncthreads();
+        const int nb = min(BLOCK, n - h0);
+        for (int t = tid; t < totSeg; t += BLOCK) {
+            int lo = 0, hi = nb - 1;  // hit owning segment t
+            while (lo < hi) {
+                int mid = (lo + hi + 1) >> 1;
+                if (sPre[mid] <= t)
+                    lo = mid;
+                else
+                    hi = mid - 1;
+            }
+            vote_segment(a, acc, r0, r1, sStart[lo], sLen[lo], (t - sPre[lo]) * SEG, sAlpha[lo],
+                         nVotes);
+        }
+        __syncthreads();
+    }
+    if (k < a.dumpCount && a.accDump) {
+        for (int i = tid; i < nrTile * a.angleNum; i += BLOCK)
+            a.accDump[((size_t)k * a.refNum + r0) * a.angleNum + i] = (int32_t)acc[i];
+    }
+    if (!voted) {
+        if (tid < 3)
+            candOut[tid] = 0ull;
+        return;
+    }
+    if (a.stats) {
+        for (int o = 32; o > 0; o >>= 1)
+            nVotes += __shfl_xor(nVotes, o, 64);
+        if ((tid & 63) == 0)
+            atomicAdd(a.stats + 2, nVotes);
+        if (tid == 0 && tile == 0)
+            atomicAdd(a.stats + 3, 1ull);
+    }
+    peaks_top3<BLOCK>(a, acc, nrTile, r0, sTop, candOut);
 }
 
 // Merge tiles, apply the 0.95 filter (ppf.cpp:393-397) and build the poses
@@ -563,6 +842,13 @@ static void launch_vote(const VoteArgs &a, int blocks, size_t lds, hipStream_t s
     vote_kernel<BLOCK><<<blocks, BLOCK, lds, s>>>(a);
 }
 
+template <int BLOCK>
+static void launch_vote_hits(const VoteArgs &a, int blocks, size_t lds, hipStream_t s) {
+    PPF_HIP_CHECK(hipFuncSetAttribute((const void *)vote_hits_kernel<BLOCK>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    vote_hits_kernel<BLOCK><<<blocks, BLOCK, lds, s>>>(a);
+}
+
 int vote_and_poses(Workspace &ws, const DevModel &m, const GridView &g, const float *PX,
                    const float *PY, const float *PZ, const float *NX, const float *NY,
                    const float *NZ, const int *keypoints, int nk, float voteThreshold,
@@ -609,19 +895,61 @@ int vote_and_poses(Workspace &ws, const DevModel &m, const GridView &g, const fl
         a.stats = ws.voteStats.as<unsigned long long>(4);
         PPF_HIP_CHECK(hipMemsetAsync(a.stats, 0, 32, s));
     }
-    a.accDump   = accDump;
-    a.dumpCount = accDump ? dumpCount : 0;
-    a.xcdRemap  = cfg.xcdRemap;
-    size_t lds  = (size_t)tileRefs * m.angleNum * 4;
-    int blocks  = (cfg.xcdRemap ? 8 * ((nk + 7) / 8) : nk) * nTiles;
+    a.accDump    = accDump;
+    a.dumpCount  = accDump ? dumpCount : 0;
+    a.xcdRemap   = cfg.xcdRemap;
+    a.kpSel      = nullptr;
+    a.countPairs = 1;
+    a.hitCap     = 0;
+    a.hitStart = a.hitLen = nullptr;
+    a.hitAlpha = nullptr;
+    a.nHit = a.nbr = a.overflow = a.nOverflow = nullptr;
+    size_t lds      = (size_t)tileRefs * m.angleNum * 4;
+    auto   padded   = [&](int n) { return cfg.xcdRemap ? 8 * ((n + 7) / 8) : n; };
+    int    blocks   = padded(nk) * nTiles;
     for (auto &e : ws.voteEv)
         if (!e)
             PPF_HIP_CHECK(hipEventCreate(&e));
+    auto fused = [&](const VoteArgs &x, int nb) {
+        switch (cfg.block) {
+            case 256: launch_vote<256>(x, nb, lds, s); break;
+            case 512: launch_vote<512>(x, nb, lds, s); break;
+            default: launch_vote<1024>(x, nb, lds, s); break;
+        }
+    };
+    int nOverflow = 0;
     PPF_HIP_CHECK(hipEventRecord(ws.voteEv[0], s));
-    switch (cfg.block) {
-        case 256: launch_vote<256>(a, blocks, lds, s); break;
-        case 512: launch_vote<512>(a, blocks, lds, s); break;
-        default: launch_vote<1024>(a, blocks, lds, s); break;
+    if (cfg.split) {
+        a.hitCap    = cfg.hitCap;
+        a.hitStart  = ws.hitStart.as<uint32_t>((size_t)nk * a.hitCap);
+        a.hitLen    = ws.hitLen.as<uint32_t>((size_t)nk * a.hitCap);
+        a.hitAlpha  = ws.hitAlpha.as<float>((size_t)nk * a.hitCap);
+        a.nHit      = ws.hitCount.as<int>((size_t)3 * nk + 1);
+        a.nbr       = a.nHit + nk;
+        a.overflow  = a.nHit + 2 * (size_t)nk;
+        a.nOverflow = a.nHit + 3 * (size_t)nk;
+        PPF_HIP_CHECK(hipMemsetAsync(a.nOverflow, 0, sizeof(int), s));
+        pair_kernel<<<padded(nk), PAIR_BLOCK, 0, s>>>(a);
+        PPF_HIP_CHECK(hipGetLastError());
+        PPF_HIP_CHECK(hipEventRecord(ws.voteEv[2], s));
+        switch (cfg.block) {
+            case 256: launch_vote_hits<256>(a, blocks, lds, s); break;
+            case 512: launch_vote_hits<512>(a, blocks, lds, s); break;
+            default: launch_vote_hits<1024>(a, blocks, lds, s); break;
+        }
+        PPF_HIP_CHECK(hipGetLastError());
+        PPF_HIP_CHECK(hipMemcpyAsync(&nOverflow, a.nOverflow, sizeof(int), hipMemcpyDeviceToHost, s));
+        PPF_HIP_CHECK(hipStreamSynchronize(s));
+        if (nOverflow > 0) {
+            // key points with more hits than the capacity: the fused kernel
+            VoteArgs f   = a;
+            f.kpSel      = a.overflow;
+            f.nk         = nOverflow;
+            f.countPairs = 0;
+            fused(f, padded(nOverflow) * nTiles);
+        }
+    } else {
+        fused(a, blocks);
     }
     PPF_HIP_CHECK(hipGetLastError());
     PPF_HIP_CHECK(hipEventRecord(ws.voteEv[1], s));
@@ -658,6 +986,13 @@ int vote_and_poses(Workspace &ws, const DevModel &m, const GridView &g, const fl
         float ms          = 0.0f;
         PPF_HIP_CHECK(hipEventElapsedTime(&ms, ws.voteEv[0], ws.voteEv[1]));
         stats->kernelMs = ms;
+        stats->pairMs   = 0.0f;
+        stats->hitsMs   = 0.0f;
+        if (cfg.split) {
+            PPF_HIP_CHECK(hipEventElapsedTime(&stats->pairMs, ws.voteEv[0], ws.voteEv[2]));
+            stats->hitsMs = ms - stats->pairMs;
+        }
+        stats->overflow = nOverflow;
     }
     return np;
 }

@@ -174,15 +174,20 @@ def test_l4_vote_accumulators_bitwise(trained, oracle_mod, workload1):
     # every launch shape (workgroup size, XCD mapping, grid cell) votes the same
     base = trained.get_tuning()
     try:
-        for vb, remap, div in ((1024, 0, 4), (256, 1, 2), (512, 0, 3), (1024, 1, 2)):
-            trained.set_tuning(voteBlock=vb, voteXcdRemap=remap, voteCellDiv=div)
+        # split and fused kernels; a hit capacity of 40 sends part of the key
+        # points through the fused fallback
+        for vb, remap, div, split, cap in ((1024, 0, 4, 1, 2048), (256, 1, 2, 0, 2048),
+                                           (512, 0, 3, 1, 40), (1024, 1, 2, 0, 2048),
+                                           (256, 0, 4, 1, 2048)):
+            trained.set_tuning(voteBlock=vb, voteXcdRemap=remap, voteCellDiv=div,
+                               voteSplit=split, voteHitCap=cap)
             cand, poses, acc = trained.vote(pts, nrm, kps, minScore=0.2, dump=len(kps))
             for t, k in enumerate(kps):
                 if expect[t] is None:
                     assert (cand[t] == -1).all()
                     continue
                 oacc, pk = expect[t]
-                assert np.array_equal(acc[t], oacc[:, 1:]), f"keypoint {k} tuning {vb, remap, div}"
+                assert np.array_equal(acc[t], oacc[:, 1:]), f"keypoint {k} tuning {vb, remap, div, split, cap}"
                 assert np.array_equal(acc[t].sum(1), oacc[:, 0])
                 got = [tuple(c) for c in cand[t] if c[0] >= 0]
                 assert got == [(a, b, int(v)) for a, b, v in pk], f"keypoint {k}"
