@@ -264,6 +264,79 @@ __global__ void __launch_bounds__(1024) nms_kernel(const Hypo *h, const int *ord
         *nOut = base;
 }
 
+// The same rule walked in order by one wave: pose j takes the first kept
+// pose within the overlap distance (a ballot over the kept list held in
+// LDS), else it is kept.  Used while the kept list fits in LDS.
+constexpr int NMS_SEQ_MAX = 4096;
+
+__global__ void __launch_bounds__(64) nms_seq_kernel(const Hypo *h, const int *ord, int C,
+                                                     const float *center, float thr, int *status,
+                                                     Hypo *out, int *nOut) {
+    // kept: x[C], y[C], z[C], pose index[C]; then every pose's centre x/y/z[C]
+    extern __shared__ float sKept[];
+    float      *kx = sKept, *ky = sKept + C, *kz = sKept + 2 * C;
+    int        *ki = reinterpret_cast<int *>(sKept + 3 * C);
+    float      *px = sKept + 4 * C, *py = sKept + 5 * C, *pz = sKept + 6 * C;
+    const int   lane = threadIdx.x;
+    const float sq   = thr * thr;
+    const float c0 = center[0], c1 = center[1], c2 = center[2];
+    for (int j = lane; j < C; j += 64) {
+        const float *P = h[ord[j]].pose;
+        px[j]          = P[0] * c0 + P[1] * c1 + P[2] * c2 + P[3];
+        py[j]          = P[4] * c0 + P[5] * c1 + P[6] * c2 + P[7];
+        pz[j]          = P[8] * c0 + P[9] * c1 + P[10] * c2 + P[11];
+    }
+    __syncthreads();
+    int nk = 0;  // wave-uniform
+    for (int j = 0; j < C; j++) {
+        const float tx = px[j], ty = py[j], tz = pz[j];
+        int         first = -1;
+        for (int b = 0; b < nk; b += 64) {
+            const int t   = b + lane;
+            bool      hit = false;
+            if (t < nk) {
+                const float dx = kx[t] - tx, dy = ky[t] - ty, dz = kz[t] - tz;
+                hit            = dx * dx + dy * dy + dz * dz < sq;
+            }
+            const unsigned long long m = __ballot(hit);
+            if (m) {
+                first = b + __ffsll((long long)m) - 1;
+                break;
+            }
+        }
+        if (first < 0) {
+            if (lane == 0) {
+                kx[nk]    = tx;
+                ky[nk]    = ty;
+                kz[nk]    = tz;
+                ki[nk]    = j;
+                status[j] = CENTER;
+            }
+            nk++;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        } else if (lane == 0) {
+            status[j] = ki[first];
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    // kept poses in order, each with its members' votes added in ascending
+    // member order (the reference's accumulation order)
+    for (int t = lane; t < nk; t += 64) {
+        const int i = ki[t];
+        Hypo      o = h[ord[i]];
+        for (int j = i + 1; j < C; j++)
+            if (status[j] == i)
+                o.votes = o.votes + h[ord[j]].votes;
+        out[t] = o;
+    }
+    if (lane == 0)
+        *nOut = nk;
+}
+
 __global__ void hypo_keys_kernel(const Hypo *h, int n, uint32_t *keys, int *vals) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n)
@@ -361,7 +434,14 @@ void cluster_poses(Workspace &ws, const PoseRec *poses, int np, float dThr, floa
     float *tr       = buf;   // reuse
     Hypo *kept      = centers + nc;
     int  *nOut      = misc + 2;
-    nms_kernel<<<1, 1024, 0, s>>>(centers, hv2, nc, cdev, nmsThr, nmsStatus, tr, kept, nOut);
+    if (nc <= NMS_SEQ_MAX) {
+        const size_t lds = (size_t)28 * std::max(nc, 1);
+        PPF_HIP_CHECK(hipFuncSetAttribute((const void *)nms_seq_kernel,
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        nms_seq_kernel<<<1, 64, lds, s>>>(centers, hv2, nc, cdev, nmsThr, nmsStatus, kept, nOut);
+    } else {
+        nms_kernel<<<1, 1024, 0, s>>>(centers, hv2, nc, cdev, nmsThr, nmsStatus, tr, kept, nOut);
+    }
     int nk = 0;
     PPF_HIP_CHECK(hipMemcpyAsync(&nk, nOut, sizeof(int), hipMemcpyDeviceToHost, s));
     PPF_HIP_CHECK(hipStreamSynchronize(s));

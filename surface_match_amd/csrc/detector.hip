@@ -226,7 +226,7 @@ struct ppf_detector_s {
     TrainScratch ts;
     VoteConfig   vcfg;
     IcpConfig    icfg;
-    int          normCell4 = 2;  // normals grid cell = sample step * normCell4 / 4
+    int          normCell4 = 4;  // normals grid cell = sample step * normCell4 / 4
     int          refTail = 1;
     DevCloud     tmpScene;
     DevBuf       idxA, idxB, idxC, idxD;  // sampled / keypoints / dense / scratch

@@ -120,9 +120,9 @@ struct VoteStats {
 };
 
 struct VoteConfig {
-    int block    = 1024;  // threads per voting workgroup
+    int block    = 512;   // threads per voting workgroup
     int xcdRemap = 0;     // keep spatially adjacent keypoints on one XCD
-    int cellDiv  = 4;     // vote grid cell = model diameter / cellDiv (2..4)
+    int cellDiv  = 3;     // vote grid cell = model diameter / cellDiv (2..4)
     int split    = 1;     // 1: pair kernel + hit voting kernel; 0: fused kernel
     int hitCap   = 2048;  // hits kept per key point on the split path
 };
@@ -161,7 +161,7 @@ struct IcpResultDev {
     int   inliers;
 };
 struct IcpConfig {
-    int block  = 256;
+    int block  = 512;
     int mode   = 0;  // 0 symmetric (reference), 1 point-to-plane
     int graph  = 1;  // replay the launch sequence from a captured hipGraph
     int spread = 1;  // 1: hypotheses cut into chunks of `block` points over

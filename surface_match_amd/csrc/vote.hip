@@ -288,6 +288,44 @@ __device__ __forceinline__ void vote_segment(const VoteArgs &a, uint32_t *acc, i
     }
 }
 
+// two segments with their entry loads issued together (the second only
+// when has2)
+__device__ __forceinline__ void vote_segment2(const VoteArgs &a, uint32_t *acc, int r0, int r1,
+                                              uint32_t startA, int LA, int eA, float aA,
+                                              bool has2, uint32_t startB, int LB, int eB, float aB,
+                                              unsigned long long &nVotes) {
+    const int    cA = min(SEG, LA - eA), cB = has2 ? min(SEG, LB - eB) : 0;
+    const bool   tA = eA >= LA - (LA & 7), tB = eB >= LB - (LB & 7);
+    const uint2 *pA = a.entries + startA + eA, *pB = a.entries + startB + eB;
+    uint2        vA[SEG], vB[SEG];
+#pragma unroll
+    for (int kk = 0; kk < SEG; kk++) {
+        if (kk < cA)
+            vA[kk] = pA[kk];
+        if (kk < cB)
+            vB[kk] = pB[kk];
+    }
+#pragma unroll
+    for (int kk = 0; kk < SEG; kk++) {
+        if (kk < cA) {
+            int ref = (int)vA[kk].x;
+            if (ref >= r0 && ref < r1) {
+                int bin = ppfc::angle_bin(__uint_as_float(vA[kk].y), aA, a.sMaxId, a.angleNum, tA);
+                atomicAdd(&acc[(ref - r0) * a.angleNum + bin], 1u);
+                nVotes++;
+            }
+        }
+        if (kk < cB) {
+            int ref = (int)vB[kk].x;
+            if (ref >= r0 && ref < r1) {
+                int bin = ppfc::angle_bin(__uint_as_float(vB[kk].y), aB, a.sMaxId, a.angleNum, tB);
+                atomicAdd(&acc[(ref - r0) * a.angleNum + bin], 1u);
+                nVotes++;
+            }
+        }
+    }
+}
+
 // peaks of a voted accumulator tile (ppf.cpp:363-391): the best three
 // (vote, ref, bin) codes of the tile
 template <int BLOCK>
@@ -741,18 +779,32 @@ __global__ void __launch_bounds__(BLOCK) vote_hits_kernel(VoteArgs a) {
         const int pre = block_excl_scan<BLOCK>((L + SEG - 1) / SEG, sScan, totSeg);
         sPre[tid]     = pre;
         __syncthreads();
-        const int nb = min(BLOCK, n - h0);
-        for (int t = tid; t < totSeg; t += BLOCK) {
-            int lo = 0, hi = nb - 1;  // hit owning segment t
+        // each thread votes a contiguous run of segments: one search for the
+        // run's first hit, then a walk; two segments in flight per step
+        const int nb  = min(BLOCK, n - h0);
+        const int spt = (totSeg + BLOCK - 1) / BLOCK;
+        const int t0  = tid * spt, t1 = min(t0 + spt, totSeg);
+        if (t0 < t1) {
+            int lo = 0, hi = nb - 1;  // hit owning segment t0
             while (lo < hi) {
                 int mid = (lo + hi + 1) >> 1;
-                if (sPre[mid] <= t)
+                if (sPre[mid] <= t0)
                     lo = mid;
                 else
                     hi = mid - 1;
             }
-            vote_segment(a, acc, r0, r1, sStart[lo], sLen[lo], (t - sPre[lo]) * SEG, sAlpha[lo],
-                         nVotes);
+            int h = lo;
+            for (int t = t0; t < t1; t += 2) {
+                while (h + 1 < nb && sPre[h + 1] <= t)
+                    h++;
+                int h2 = h;
+                if (t + 1 < t1)
+                    while (h2 + 1 < nb && sPre[h2 + 1] <= t + 1)
+                        h2++;
+                vote_segment2(a, acc, r0, r1, sStart[h], sLen[h], (t - sPre[h]) * SEG, sAlpha[h],
+                              t + 1 < t1, sStart[h2], sLen[h2], (t + 1 - sPre[h2]) * SEG,
+                              sAlpha[h2], nVotes);
+            }
         }
         __syncthreads();
     }
