@@ -136,8 +136,9 @@ PPF_API int ppf_match_scene_resident(ppf_detector d, ppf_scene s, float sampling
                                      size_t cap, size_t *nOut, ppf_match_stats *stats);
 
 /* -- launch tuning (no reference counterpart).  ppf_autotune times the
- * vote stage over workgroup sizes {256, 512, 1024} x {split, fused} kernels
- * x vote-grid cell {D/2, D/3, D/4}, then the batched ICP
+ * vote stage over {split, fused} kernels, then workgroup sizes {256, 512,
+ * 1024}, then the vote-grid cell {D/2, D/3, D/4}, then reference tiles
+ * {1, 2, 3, 4} (one knob at a time), then the batched ICP
  * over {spread, persistent} x {64, 128, 256, 512, 1024} threads per
  * workgroup, then the normals grid cell over {1/4, 1/2, 1} sample step, on
  * the given resident scene, and keeps the fastest in the detector.  None of
@@ -152,6 +153,8 @@ typedef struct {
     int voteSplit;    /* 1: pair kernel + hit voting kernel, 0: one fused kernel */
     int voteHitCap;   /* key hits kept per key point on the split path (>= 1); key points
                          with more are voted by the fused kernel */
+    int voteTiles;    /* at least this many reference tiles per key point (1 .. 16): smaller
+                         LDS accumulators, more workgroups per CU */
 } ppf_tuning;
 typedef struct {
     ppf_tuning best;

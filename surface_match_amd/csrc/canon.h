@@ -187,29 +187,42 @@ PPF_HD float alpha_of(const float rt[12], float x, float y, float z) {
     return a;
 }
 
-// angle bin of a bucket entry (ppf.cpp:316-340); tail = the entry is among
-// the bucket's last size%8.
-PPF_HD int angle_bin(float alphaModel, float alphaScene, float sMaxId, int angleNum, bool tail) {
-    float a;
-    if (!tail) {
-        float vs = alphaScene - kPi;
-        a        = alphaModel - vs;
-        if (a > kTwoPi)
-            a = a - kTwoPi;
-        if (a < 0.0f)
-            a = a + kTwoPi;
-    } else {
-        float s = (float)((double)alphaScene - 3.14159265358979323846);
-        a       = alphaModel - s;
-        if (a < 0.0f)
-            a = a + kTwoPi;
-        if (a > kTwoPi)
-            a = a - kTwoPi;
-    }
+// the scene alpha's part of an angle bin, once per scene pair: the body
+// subtracts alphaScene - fl(pi) in float, the tail alphaScene - M_PI in double
+// rounded to float (ppf.cpp:316-340)
+PPF_HD float bin_offset_body(float alphaScene) { return alphaScene - kPi; }
+PPF_HD float bin_offset_tail(float alphaScene) {
+    return (float)((double)alphaScene - 3.14159265358979323846);
+}
+PPF_HD int bin_of(float a, float sMaxId, int angleNum) {
     int bin = (int)floorf(sMaxId * a);
     bin     = bin < 0 ? 0 : bin;
     bin     = bin > angleNum - 1 ? angleNum - 1 : bin;
     return bin;
+}
+// the same bins as angle_bin from a precomputed offset
+PPF_HD int bin_body(float alphaModel, float offBody, float sMaxId, int angleNum) {
+    float a = alphaModel - offBody;
+    if (a > kTwoPi)
+        a = a - kTwoPi;
+    if (a < 0.0f)
+        a = a + kTwoPi;
+    return bin_of(a, sMaxId, angleNum);
+}
+PPF_HD int bin_tail(float alphaModel, float offTail, float sMaxId, int angleNum) {
+    float a = alphaModel - offTail;
+    if (a < 0.0f)
+        a = a + kTwoPi;
+    if (a > kTwoPi)
+        a = a - kTwoPi;
+    return bin_of(a, sMaxId, angleNum);
+}
+
+// angle bin of a bucket entry (ppf.cpp:316-340); tail = the entry is among
+// the bucket's last size%8.
+PPF_HD int angle_bin(float alphaModel, float alphaScene, float sMaxId, int angleNum, bool tail) {
+    return tail ? bin_tail(alphaModel, bin_offset_tail(alphaScene), sMaxId, angleNum)
+                : bin_body(alphaModel, bin_offset_body(alphaScene), sMaxId, angleNum);
 }
 
 // nanoflann L2 metric for DIM = 3 (nanoflann.hpp:545-581): ((0+dx^2)+dy^2)+dz^2.

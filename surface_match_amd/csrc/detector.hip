@@ -1025,6 +1025,7 @@ ppf_tuning tuning_of(const ppf_detector_s &d) {
     t.normCell4    = d.normCell4;
     t.voteSplit    = d.vcfg.split;
     t.voteHitCap   = d.vcfg.hitCap;
+    t.voteTiles    = d.vcfg.tiles;
     return t;
 }
 }  // namespace
@@ -1048,6 +1049,7 @@ PPF_API int ppf_set_tuning(ppf_detector d, const ppf_tuning *t) {
     d->vcfg.cellDiv  = t->voteCellDiv;
     PPF_REQUIRE(t->voteHitCap >= 1 && t->voteHitCap <= (1 << 20),
                 "vote hit capacity must be 1 .. 2^20");
+    PPF_REQUIRE(t->voteTiles >= 1 && t->voteTiles <= 16, "vote tiles must be 1 .. 16");
     PPF_REQUIRE(t->normCell4 == 1 || t->normCell4 == 2 || t->normCell4 == 4,
                 "normals cell must be 1, 2 or 4 quarters of the sample step");
     d->icfg.block    = t->icpBlock;
@@ -1055,6 +1057,7 @@ PPF_API int ppf_set_tuning(ppf_detector d, const ppf_tuning *t) {
     d->normCell4     = t->normCell4;
     d->vcfg.split    = t->voteSplit ? 1 : 0;
     d->vcfg.hitCap   = t->voteHitCap;
+    d->vcfg.tiles    = t->voteTiles;
     return PPF_OK;
 }
 
@@ -1081,20 +1084,24 @@ PPF_API int ppf_autotune(ppf_detector d, ppf_scene sc, float rel, float kpFrac, 
                 icpMs  = std::min(icpMs, st.msIcp);
             }
         };
-        for (int block : {256, 512, 1024})
-            for (int split : {0, 1})
-                for (int div : {2, 3, 4}) {
-                    d->vcfg.block    = block;
-                    d->vcfg.split    = split;
-                    d->vcfg.cellDiv  = div;
-                    float v, i;
-                    run(v, i);
-                    tried++;
-                    if (v < bestVoteMs) {
-                        bestVoteMs = v;
-                        bestV      = d->vcfg;
-                    }
+        // vote stage: one knob at a time, the others at their best so far
+        auto tryVote = [&](int VoteConfig::*knob, std::initializer_list<int> values) {
+            for (int v0 : values) {
+                d->vcfg       = bestV;
+                d->vcfg.*knob = v0;
+                float v, i;
+                run(v, i);
+                tried++;
+                if (v < bestVoteMs) {
+                    bestVoteMs = v;
+                    bestV      = d->vcfg;
                 }
+            }
+        };
+        tryVote(&VoteConfig::split, {1, 0});
+        tryVote(&VoteConfig::block, {256, 512, 1024});
+        tryVote(&VoteConfig::cellDiv, {2, 3, 4});
+        tryVote(&VoteConfig::tiles, {1, 2, 3, 4});
         d->vcfg = bestV;
         for (int spread : {1, 0})
             for (int block : {64, 128, 256, 512, 1024}) {

@@ -25,7 +25,11 @@ struct Workspace {
     Grid   gNormal, gVote, gSparse, gDense, gFull;
     // vote
     DevBuf kpX, kpY, kpZ, kpNX, kpNY, kpNZ, cand, candTile, poses, voteStats, accDump, voteCnt,
-        smallCnt, hitStart, hitLen, hitAlpha, hitCount;
+        smallCnt, hitStart, hitLen, hitAlpha, hitBucket, hitCount, splits;
+    // the bucket split points in `splits` were computed for this model
+    // generation, tile size and tile count
+    unsigned splitGen = ~0u;
+    int      splitRefs = 0, splitTiles = 0;
     // clustering
     DevBuf clSortKey, clSortKey2, clSortVal, clSortVal2, clStatus, clPred, clPredOff,
         clCenters, clNms, clMembers;
@@ -55,8 +59,12 @@ struct DevModel {
     DevBuf   rx, ry, rz, rnx, rny, rnz;      // resampled model (SoA)
     DevBuf   frames;                         // nSampled x 12 (3x4 row-major)
     DevBuf   entries;                        // uint2 {ref, alpha bits}, bucket-major
-    DevBuf   slots;                          // uint4 {key, start, len, 0}
+    DevBuf   ventries;                       // uint2 {ref x angleNum, alpha bits}, the same order
+    DevBuf   slots;                          // uint4 {key, start, len, bucket index}
+    DevBuf   buckets;                        // uint2 {start, len} per bucket index
     DevBuf   cosSin;                         // 2 * angleNum floats
+    int      refsSorted = 0;                 // every bucket's refs ascending
+    unsigned generation = 0;                 // bumped by every upload
 };
 
 // Records of the model pair stage before grouping (train on device).
@@ -125,6 +133,7 @@ struct VoteConfig {
     int cellDiv  = 3;     // vote grid cell = model diameter / cellDiv (2..4)
     int split    = 1;     // 1: pair kernel + hit voting kernel; 0: fused kernel
     int hitCap   = 2048;  // hits kept per key point on the split path
+    int tiles    = 1;     // at least this many reference tiles per key point
 };
 
 // Vote every keypoint (indices into the sampled-scene grid's clouds),

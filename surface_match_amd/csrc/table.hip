@@ -250,6 +250,11 @@ void model_upload(DevModel &m, int nSampled, const float *sp, const float *sn, i
     }
     PPF_HIP_CHECK(hipMemcpyAsync(m.entries.as<uint2>(std::max(E, 1)), ent.data(), sizeof(uint2) * E,
                                  hipMemcpyHostToDevice, s));
+    std::vector<uint2> vent(E);
+    for (int e = 0; e < E; e++)
+        vent[e] = make_uint2(refs[e] * (uint32_t)angleNum, ent[e].y);
+    PPF_HIP_CHECK(hipMemcpyAsync(m.ventries.as<uint2>(std::max(E, 1)), vent.data(),
+                                 sizeof(uint2) * E, hipMemcpyHostToDevice, s));
     // open-addressing key table, load factor <= 1/2; an empty slot has len 0
     uint32_t T = 64;
     while (T < 2u * (uint32_t)std::max(K, 1))
@@ -259,9 +264,24 @@ void model_upload(DevModel &m, int nSampled, const float *sp, const float *sn, i
         uint32_t h = keys[k] & (T - 1);
         while (slots[h].z != 0)
             h = (h + 1) & (T - 1);
-        slots[h] = make_uint4(keys[k], (uint32_t)offs[k], (uint32_t)(offs[k + 1] - offs[k]), 0);
+        slots[h] = make_uint4(keys[k], (uint32_t)offs[k], (uint32_t)(offs[k + 1] - offs[k]),
+                              (uint32_t)k);
     }
     m.tmask = T - 1;
+    // buckets by index, and whether every bucket lists its references in
+    // ascending order (true for tables built here: insertion order is i
+    // ascending; a loaded table may differ)
+    std::vector<uint2> bk(std::max(K, 1));
+    m.refsSorted = 1;
+    for (int k = 0; k < K; k++) {
+        bk[k] = make_uint2((uint32_t)offs[k], (uint32_t)(offs[k + 1] - offs[k]));
+        for (int e = offs[k] + 1; e < offs[k + 1]; e++)
+            if (refs[e] < refs[e - 1])
+                m.refsSorted = 0;
+    }
+    PPF_HIP_CHECK(hipMemcpyAsync(m.buckets.as<uint2>(bk.size()), bk.data(), sizeof(uint2) * bk.size(),
+                                 hipMemcpyHostToDevice, s));
+    m.generation++;
     PPF_HIP_CHECK(hipMemcpyAsync(m.slots.as<uint4>(T), slots.data(), sizeof(uint4) * T,
                                  hipMemcpyHostToDevice, s));
     // pose angle of each bin: (float)((double)(fl(2pi) j / maxIdx) - pi), ppf.cpp:402

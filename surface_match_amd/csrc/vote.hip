@@ -46,6 +46,7 @@ struct VoteArgs {
     const uint4 *slots;
     uint32_t     tmask;
     const uint2 *entries;
+    const uint2 *ventries;      // the same entries with ref x angleNum (voting)
     int          refNum, angleNum, tileRefs, nTiles;
     float        angleStep, distStep, sMaxId, D2, searchR, voteThreshold;
     unsigned long long *cand;   // [nk * nTiles * 3]
@@ -56,23 +57,27 @@ struct VoteArgs {
     const int   *kpSel;         // key points to vote (nullptr: all nk)
     // split path: hits of key point k at [k * hitCap, k * hitCap + nHit[k])
     int          hitCap;
-    uint32_t    *hitStart, *hitLen;
+    uint32_t    *hitStart, *hitLen, *hitBucket;
     float       *hitAlpha;
+    // with buckets sorted by reference: entry index where tile t of bucket b
+    // starts, splits[b * (nTiles + 1) + t] (nullptr: tiles scan whole buckets)
+    const uint32_t *splits;
     int         *nHit, *nbr;    // hit count (-1: overflow), neighbour count incl. self
     int         *overflow, *nOverflow;
     int          countPairs;    // fused kernel adds its pairs and hits to stats
 };
 
 __device__ __forceinline__ bool table_find(const uint4 *slots, uint32_t tmask, uint32_t key,
-                                           uint32_t &start, uint32_t &len) {
+                                           uint32_t &start, uint32_t &len, uint32_t &bucket) {
     uint32_t h = key & tmask;
     for (;;) {
         uint4 s = slots[h];
         if (s.z == 0u)
             return false;
         if (s.x == key) {
-            start = s.y;
-            len   = s.z;
+            start  = s.y;
+            len    = s.z;
+            bucket = s.w;
             return true;
         }
         h = (h + 1u) & tmask;
@@ -90,6 +95,68 @@ __device__ __forceinline__ Top3 top3_add(Top3 t, unsigned long long x) {
     const unsigned long long hi1 = max(t.a, x), lo1 = min(t.a, x);
     const unsigned long long hi2 = max(t.b, lo1), lo2 = min(t.b, lo1);
     return Top3{hi1, hi2, max(t.c, lo2)};
+}
+
+// Votes of one run of cnt (<= SEG) bucket entries from entry index e0 into
+// the accumulator tile whose first reference row starts at r0A (= r0 x
+// angleNum); entries from tailStart on take the bucket-tail bin rule.  The
+// vote entries hold ref x angleNum, so the accumulator index is one add.
+// Returns the votes cast.
+struct SegRun {
+    uint32_t e0, tailStart;
+    int      cnt;
+    float    offBody, offTail;
+};
+
+template <bool RANGE>
+__device__ __forceinline__ int vote_run_loaded(const VoteArgs &a, uint32_t *acc, int r0A, int r1A,
+                                               const SegRun &g, const uint2 (&v)[SEG]) {
+    const int tailFrom = (int)min(max((long long)g.tailStart - (long long)g.e0, 0ll),
+                                  (long long)g.cnt);
+    int nv = 0;
+#pragma unroll
+    for (int kk = 0; kk < SEG; kk++) {
+        if (kk < g.cnt) {
+            const int refA = (int)v[kk].x;
+            if (!RANGE || (refA >= r0A && refA < r1A)) {
+                const float am  = __uint_as_float(v[kk].y);
+                const int   bin = kk < tailFrom ? ppfc::bin_body(am, g.offBody, a.sMaxId, a.angleNum)
+                                                : ppfc::bin_tail(am, g.offTail, a.sMaxId, a.angleNum);
+                atomicAdd(&acc[refA - r0A + bin], 1u);
+                nv++;
+            }
+        }
+    }
+    return nv;
+}
+
+template <bool RANGE>
+__device__ __forceinline__ int vote_run(const VoteArgs &a, uint32_t *acc, int r0A, int r1A,
+                                        const SegRun &g) {
+    const uint2 *ent = a.ventries + g.e0;
+    uint2        v[SEG];
+#pragma unroll
+    for (int kk = 0; kk < SEG; kk++)
+        if (kk < g.cnt)
+            v[kk] = ent[kk];
+    return vote_run_loaded<RANGE>(a, acc, r0A, r1A, g, v);
+}
+
+// two runs with their entry loads issued together
+template <bool RANGE>
+__device__ __forceinline__ int vote_run2(const VoteArgs &a, uint32_t *acc, int r0A, int r1A,
+                                         const SegRun &g1, const SegRun &g2) {
+    const uint2 *p1 = a.ventries + g1.e0, *p2 = a.ventries + g2.e0;
+    uint2        v1[SEG], v2[SEG];
+#pragma unroll
+    for (int kk = 0; kk < SEG; kk++) {
+        if (kk < g1.cnt)
+            v1[kk] = p1[kk];
+        if (kk < g2.cnt)
+            v2[kk] = p2[kk];
+    }
+    return vote_run_loaded<RANGE>(a, acc, r0A, r1A, g1, v1) +
+           vote_run_loaded<RANGE>(a, acc, r0A, r1A, g2, v2);
 }
 
 // orders this wave's LDS writes before its later LDS reads (a wave runs in
@@ -132,6 +199,8 @@ __device__ __forceinline__ void wave_vote(const VoteArgs &a, uint32_t *acc, int 
     if (lane + 64 < qn)
         qPre[lane + 64] = tot0 + x1 - s1;
     wave_sync();
+    const int r0A = r0 * a.angleNum, r1A = r1 * a.angleNum;
+    int       nv  = 0;
     for (int t = lane; t < totalSeg; t += 64) {
         int lo = 0, hi = qn - 1;  // queue entry owning segment t
         while (lo < hi) {
@@ -141,29 +210,16 @@ __device__ __forceinline__ void wave_vote(const VoteArgs &a, uint32_t *acc, int 
             else
                 hi = mid - 1;
         }
-        const int    L    = (int)qLen[lo];
-        const int    e0   = (t - qPre[lo]) * SEG;
-        const int    cnt  = min(SEG, L - e0);
-        const bool   tail = e0 >= L - (L & 7);
-        const float  aS   = qAlpha[lo];
-        const uint2 *ent  = a.entries + qStart[lo] + e0;
-        uint2        v[SEG];
-#pragma unroll
-        for (int k = 0; k < SEG; k++)
-            if (k < cnt)
-                v[k] = ent[k];
-#pragma unroll
-        for (int k = 0; k < SEG; k++) {
-            if (k >= cnt)
-                break;
-            int ref = (int)v[k].x;
-            if (ref >= r0 && ref < r1) {
-                int bin = ppfc::angle_bin(__uint_as_float(v[k].y), aS, a.sMaxId, a.angleNum, tail);
-                atomicAdd(&acc[(ref - r0) * a.angleNum + bin], 1u);
-                nVotes++;
-            }
-        }
+        const uint32_t st = qStart[lo], L = qLen[lo];
+        SegRun         g;
+        g.e0        = st + (uint32_t)(t - qPre[lo]) * SEG;
+        g.cnt       = (int)min((uint32_t)SEG, st + L - g.e0);
+        g.tailStart = st + L - (L & 7u);
+        g.offBody   = ppfc::bin_offset_body(qAlpha[lo]);
+        g.offTail   = ppfc::bin_offset_tail(qAlpha[lo]);
+        nv += a.nTiles > 1 ? vote_run<true>(a, acc, r0A, r1A, g) : vote_run<false>(a, acc, r0A, r1A, g);
     }
+    nVotes += (unsigned long long)nv;
     // the queue is refilled after this point
     wave_sync();
 }
@@ -260,70 +316,6 @@ __device__ __forceinline__ int candidate_slot(int t, int c, int nCells, const in
         }
     }
     return sCellStart[lo] + (c - sCellPre[lo]);
-}
-
-// vote one segment (<= SEG entries starting at entry e0 of a bucket of length
-// L) into the accumulator tile [r0, r1)
-__device__ __forceinline__ void vote_segment(const VoteArgs &a, uint32_t *acc, int r0, int r1,
-                                             uint32_t start, int L, int e0, float aS,
-                                             unsigned long long &nVotes) {
-    const int    cnt  = min(SEG, L - e0);
-    const bool   tail = e0 >= L - (L & 7);
-    const uint2 *ent  = a.entries + start + e0;
-    uint2        v[SEG];
-#pragma unroll
-    for (int kk = 0; kk < SEG; kk++)
-        if (kk < cnt)
-            v[kk] = ent[kk];
-#pragma unroll
-    for (int kk = 0; kk < SEG; kk++) {
-        if (kk >= cnt)
-            break;
-        int ref = (int)v[kk].x;
-        if (ref >= r0 && ref < r1) {
-            int bin = ppfc::angle_bin(__uint_as_float(v[kk].y), aS, a.sMaxId, a.angleNum, tail);
-            atomicAdd(&acc[(ref - r0) * a.angleNum + bin], 1u);
-            nVotes++;
-        }
-    }
-}
-
-// two segments with their entry loads issued together (the second only
-// when has2)
-__device__ __forceinline__ void vote_segment2(const VoteArgs &a, uint32_t *acc, int r0, int r1,
-                                              uint32_t startA, int LA, int eA, float aA,
-                                              bool has2, uint32_t startB, int LB, int eB, float aB,
-                                              unsigned long long &nVotes) {
-    const int    cA = min(SEG, LA - eA), cB = has2 ? min(SEG, LB - eB) : 0;
-    const bool   tA = eA >= LA - (LA & 7), tB = eB >= LB - (LB & 7);
-    const uint2 *pA = a.entries + startA + eA, *pB = a.entries + startB + eB;
-    uint2        vA[SEG], vB[SEG];
-#pragma unroll
-    for (int kk = 0; kk < SEG; kk++) {
-        if (kk < cA)
-            vA[kk] = pA[kk];
-        if (kk < cB)
-            vB[kk] = pB[kk];
-    }
-#pragma unroll
-    for (int kk = 0; kk < SEG; kk++) {
-        if (kk < cA) {
-            int ref = (int)vA[kk].x;
-            if (ref >= r0 && ref < r1) {
-                int bin = ppfc::angle_bin(__uint_as_float(vA[kk].y), aA, a.sMaxId, a.angleNum, tA);
-                atomicAdd(&acc[(ref - r0) * a.angleNum + bin], 1u);
-                nVotes++;
-            }
-        }
-        if (kk < cB) {
-            int ref = (int)vB[kk].x;
-            if (ref >= r0 && ref < r1) {
-                int bin = ppfc::angle_bin(__uint_as_float(vB[kk].y), aB, a.sMaxId, a.angleNum, tB);
-                atomicAdd(&acc[(ref - r0) * a.angleNum + bin], 1u);
-                nVotes++;
-            }
-        }
-    }
 }
 
 // peaks of a voted accumulator tile (ppf.cpp:363-391): the best three
@@ -455,7 +447,8 @@ __global__ void __launch_bounds__(BLOCK) vote_kernel(VoteArgs a) {
             alphaS       = ppfc::alpha_of(rt, px, py, pz);
             uint32_t key = ppfc::ppf_key(qx, qy, qz, nx1, ny1, nz1, px, py, pz, g.nx[s], g.ny[s],
                                          g.nz[s], a.angleStep, a.distStep, false);
-            if (table_find(a.slots, a.tmask, key, bStart, bLen) && !ppfc::is_nan(alphaS))
+            uint32_t bucket;
+            if (table_find(a.slots, a.tmask, key, bStart, bLen, bucket) && !ppfc::is_nan(alphaS))
                 hit = 1;
         }
         const unsigned long long m = __ballot(hit);
@@ -537,9 +530,17 @@ __global__ void __launch_bounds__(BLOCK) vote_kernel(VoteArgs a) {
                 else
                     hi = mid - 1;
             }
-            const int ew = lo >> 6, ei = lo & 63;
-            vote_segment(a, acc, r0, r1, qStart[ew][ei], (int)qLen[ew][ei], (t - flat[lo]) * SEG,
-                         qAlpha[ew][ei], nVotes);
+            const int      ew = lo >> 6, ei = lo & 63;
+            const uint32_t st = qStart[ew][ei], L = qLen[ew][ei];
+            SegRun         gr;
+            gr.e0        = st + (uint32_t)(t - flat[lo]) * SEG;
+            gr.cnt       = (int)min((uint32_t)SEG, st + L - gr.e0);
+            gr.tailStart = st + L - (L & 7u);
+            gr.offBody   = ppfc::bin_offset_body(qAlpha[ew][ei]);
+            gr.offTail   = ppfc::bin_offset_tail(qAlpha[ew][ei]);
+            const int r0A = r0 * a.angleNum, r1A = r1 * a.angleNum;
+            nVotes += (unsigned long long)(a.nTiles > 1 ? vote_run<true>(a, acc, r0A, r1A, gr)
+                                                        : vote_run<false>(a, acc, r0A, r1A, gr));
         }
     }
     __syncthreads();
@@ -637,7 +638,7 @@ __global__ void __launch_bounds__(PAIR_BLOCK) pair_kernel(VoteArgs a) {
     int               *pq = pQueue[wv];
     auto pair_stage = [&](int cnt) {
         int      hit    = 0;
-        uint32_t bStart = 0, bLen = 0;
+        uint32_t bStart = 0, bLen = 0, bucket = 0;
         float    alphaS = 0.0f;
         if (lane < cnt) {
             const int s  = pq[lane];
@@ -645,7 +646,7 @@ __global__ void __launch_bounds__(PAIR_BLOCK) pair_kernel(VoteArgs a) {
             alphaS       = ppfc::alpha_of(rt, px, py, pz);
             uint32_t key = ppfc::ppf_key(qx, qy, qz, nx1, ny1, nz1, px, py, pz, g.nx[s], g.ny[s],
                                          g.nz[s], a.angleStep, a.distStep, false);
-            if (table_find(a.slots, a.tmask, key, bStart, bLen) && !ppfc::is_nan(alphaS))
+            if (table_find(a.slots, a.tmask, key, bStart, bLen, bucket) && !ppfc::is_nan(alphaS))
                 hit = 1;
         }
         const unsigned long long m = __ballot(hit);
@@ -659,9 +660,10 @@ __global__ void __launch_bounds__(PAIR_BLOCK) pair_kernel(VoteArgs a) {
         if (hit) {
             const int pos = at + __popcll(m & ((1ull << lane) - 1ull));
             if (pos < a.hitCap) {
-                a.hitStart[base + pos] = bStart;
-                a.hitLen[base + pos]   = bLen;
-                a.hitAlpha[base + pos] = alphaS;
+                a.hitStart[base + pos]  = bStart;
+                a.hitLen[base + pos]    = bLen;
+                a.hitAlpha[base + pos]  = alphaS;
+                a.hitBucket[base + pos] = bucket;
             }
         }
     };
@@ -738,9 +740,9 @@ template <int BLOCK>
 __global__ void __launch_bounds__(BLOCK) vote_hits_kernel(VoteArgs a) {
     extern __shared__ uint32_t acc[];
     __shared__ int      sPre[BLOCK];
-    __shared__ uint32_t sStart[BLOCK];
-    __shared__ int      sLen[BLOCK];
-    __shared__ float    sAlpha[BLOCK];
+    __shared__ uint32_t sLo[BLOCK], sTail[BLOCK];
+    __shared__ int      sCnt[BLOCK];
+    __shared__ float    sOffB[BLOCK], sOffT[BLOCK];
     __shared__ int      sScan[BLOCK / 64 + 1];
     __shared__ unsigned long long sTop[(BLOCK / 64) * 3];
 
@@ -767,25 +769,42 @@ __global__ void __launch_bounds__(BLOCK) vote_hits_kernel(VoteArgs a) {
     unsigned long long nVotes = 0;
     const size_t       base   = (size_t)k * a.hitCap;
     for (int h0 = 0; h0 < n; h0 += BLOCK) {
-        const int i = h0 + tid;
-        int       L = 0;
+        // this tile's entries of each hit's bucket: the whole bucket, or with
+        // ref-sorted buckets only the slice holding the tile's references
+        const int i   = h0 + tid;
+        int       cnt = 0;
         if (i < n) {
-            L            = (int)a.hitLen[base + i];
-            sStart[tid]  = a.hitStart[base + i];
-            sAlpha[tid]  = a.hitAlpha[base + i];
-            sLen[tid]    = L;
+            const uint32_t st = a.hitStart[base + i], L = a.hitLen[base + i];
+            uint32_t       lo = st, hi = st + L;
+            if (a.splits) {
+                const uint32_t *sp = a.splits + (size_t)a.hitBucket[base + i] * (a.nTiles + 1);
+                lo                 = sp[tile];
+                hi                 = sp[tile + 1];
+            }
+            cnt            = (int)(hi - lo);
+            sLo[tid]       = lo;
+            sTail[tid]     = st + L - (L & 7u);
+            const float aS = a.hitAlpha[base + i];
+            sOffB[tid]     = ppfc::bin_offset_body(aS);
+            sOffT[tid]     = ppfc::bin_offset_tail(aS);
         }
         int       totSeg;
-        const int pre = block_excl_scan<BLOCK>((L + SEG - 1) / SEG, sScan, totSeg);
+        const int pre = block_excl_scan<BLOCK>((cnt + SEG - 1) / SEG, sScan, totSeg);
         sPre[tid]     = pre;
+        sCnt[tid]     = cnt;
         __syncthreads();
-        // each thread votes a contiguous run of segments: one search for the
-        // run's first hit, then a walk; two segments in flight per step
+        // each thread votes a contiguous run of segments (SEG entries of one
+        // bucket each; different lanes, different buckets, so the LDS atomics
+        // of a wave scatter over the accumulator): one search for the run's
+        // first hit, then a walk (hits with no segment are skipped); two
+        // segments in flight per step
         const int nb  = min(BLOCK, n - h0);
         const int spt = (totSeg + BLOCK - 1) / BLOCK;
         const int t0  = tid * spt, t1 = min(t0 + spt, totSeg);
+        const int r0A = r0 * a.angleNum, r1A = r1 * a.angleNum;
+        int       nv  = 0;
         if (t0 < t1) {
-            int lo = 0, hi = nb - 1;  // hit owning segment t0
+            int lo = 0, hi = nb - 1;  // last hit whose prefix <= t0
             while (lo < hi) {
                 int mid = (lo + hi + 1) >> 1;
                 if (sPre[mid] <= t0)
@@ -797,15 +816,29 @@ __global__ void __launch_bounds__(BLOCK) vote_hits_kernel(VoteArgs a) {
             for (int t = t0; t < t1; t += 2) {
                 while (h + 1 < nb && sPre[h + 1] <= t)
                     h++;
-                int h2 = h;
-                if (t + 1 < t1)
+                SegRun g1, g2;
+                g1.e0        = sLo[h] + (uint32_t)(t - sPre[h]) * SEG;
+                g1.cnt       = min(SEG, (int)(sLo[h] + sCnt[h] - g1.e0));
+                g1.tailStart = sTail[h];
+                g1.offBody   = sOffB[h];
+                g1.offTail   = sOffT[h];
+                g2           = g1;
+                g2.cnt       = 0;
+                if (t + 1 < t1) {
+                    int h2 = h;
                     while (h2 + 1 < nb && sPre[h2 + 1] <= t + 1)
                         h2++;
-                vote_segment2(a, acc, r0, r1, sStart[h], sLen[h], (t - sPre[h]) * SEG, sAlpha[h],
-                              t + 1 < t1, sStart[h2], sLen[h2], (t + 1 - sPre[h2]) * SEG,
-                              sAlpha[h2], nVotes);
+                    g2.e0        = sLo[h2] + (uint32_t)(t + 1 - sPre[h2]) * SEG;
+                    g2.cnt       = min(SEG, (int)(sLo[h2] + sCnt[h2] - g2.e0));
+                    g2.tailStart = sTail[h2];
+                    g2.offBody   = sOffB[h2];
+                    g2.offTail   = sOffT[h2];
+                }
+                nv += a.nTiles > 1 ? vote_run2<true>(a, acc, r0A, r1A, g1, g2)
+                                   : vote_run2<false>(a, acc, r0A, r1A, g1, g2);
             }
         }
+        nVotes += (unsigned long long)nv;
         __syncthreads();
     }
     if (k < a.dumpCount && a.accDump) {
@@ -894,6 +927,28 @@ static void launch_vote(const VoteArgs &a, int blocks, size_t lds, hipStream_t s
     vote_kernel<BLOCK><<<blocks, BLOCK, lds, s>>>(a);
 }
 
+// split points of every bucket for tiles of tileRefs references: the first
+// entry of tile t is the first entry whose ref >= t * tileRefs (refs ascend)
+__global__ void bucket_split_kernel(const uint2 *buckets, int K, const uint2 *entries, int tileRefs,
+                                    int nTiles, uint32_t *splits) {
+    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= K)
+        return;
+    const uint2 bk = buckets[b];
+    for (int t = 0; t <= nTiles; t++) {
+        uint32_t lo = bk.x, hi = bk.x + bk.y;  // first e in [lo, hi) with ref >= t * tileRefs
+        const uint32_t r = (uint32_t)t * (uint32_t)tileRefs;
+        while (lo < hi) {
+            uint32_t mid = (lo + hi) >> 1;
+            if (entries[mid].x < r)
+                lo = mid + 1;
+            else
+                hi = mid;
+        }
+        splits[(size_t)b * (nTiles + 1) + t] = lo;
+    }
+}
+
 template <int BLOCK>
 static void launch_vote_hits(const VoteArgs &a, int blocks, size_t lds, hipStream_t s) {
     PPF_HIP_CHECK(hipFuncSetAttribute((const void *)vote_hits_kernel<BLOCK>,
@@ -909,8 +964,9 @@ int vote_and_poses(Workspace &ws, const DevModel &m, const GridView &g, const fl
     if (nk <= 0 || m.nSampled <= 0)
         return 0;
     int tileRefs = std::min(m.nSampled, vote_max_tile_refs(m.angleNum, cfg.block));
-    int nTiles   = div_up(m.nSampled, tileRefs);
+    int nTiles   = std::max(div_up(m.nSampled, tileRefs), std::max(cfg.tiles, 1));
     tileRefs     = div_up(m.nSampled, nTiles);
+    nTiles       = div_up(m.nSampled, tileRefs);
     float cell   = g.cell;
     float D      = m.diameter;
     float searchR = D * 1.0001f + 1e-3f * cell;
@@ -931,6 +987,7 @@ int vote_and_poses(Workspace &ws, const DevModel &m, const GridView &g, const fl
     a.slots    = m.slots.get<uint4>();
     a.tmask    = m.tmask;
     a.entries  = m.entries.get<uint2>();
+    a.ventries = m.ventries.get<uint2>();
     a.refNum   = m.nSampled;
     a.angleNum = m.angleNum;
     a.tileRefs = tileRefs;
@@ -953,8 +1010,9 @@ int vote_and_poses(Workspace &ws, const DevModel &m, const GridView &g, const fl
     a.kpSel      = nullptr;
     a.countPairs = 1;
     a.hitCap     = 0;
-    a.hitStart = a.hitLen = nullptr;
+    a.hitStart = a.hitLen = a.hitBucket = nullptr;
     a.hitAlpha = nullptr;
+    a.splits   = nullptr;
     a.nHit = a.nbr = a.overflow = a.nOverflow = nullptr;
     size_t lds      = (size_t)tileRefs * m.angleNum * 4;
     auto   padded   = [&](int n) { return cfg.xcdRemap ? 8 * ((n + 7) / 8) : n; };
@@ -976,6 +1034,20 @@ int vote_and_poses(Workspace &ws, const DevModel &m, const GridView &g, const fl
         a.hitStart  = ws.hitStart.as<uint32_t>((size_t)nk * a.hitCap);
         a.hitLen    = ws.hitLen.as<uint32_t>((size_t)nk * a.hitCap);
         a.hitAlpha  = ws.hitAlpha.as<float>((size_t)nk * a.hitCap);
+        a.hitBucket = ws.hitBucket.as<uint32_t>((size_t)nk * a.hitCap);
+        if (nTiles > 1 && m.refsSorted && m.nKeys > 0) {
+            if (ws.splitGen != m.generation || ws.splitRefs != tileRefs ||
+                ws.splitTiles != nTiles) {
+                uint32_t *sp = ws.splits.as<uint32_t>((size_t)m.nKeys * (nTiles + 1));
+                bucket_split_kernel<<<div_up(m.nKeys, 256), 256, 0, s>>>(
+                    m.buckets.get<uint2>(), m.nKeys, m.entries.get<uint2>(), tileRefs, nTiles, sp);
+                PPF_HIP_CHECK(hipGetLastError());
+                ws.splitGen   = m.generation;
+                ws.splitRefs  = tileRefs;
+                ws.splitTiles = nTiles;
+            }
+            a.splits = ws.splits.get<uint32_t>();
+        }
         a.nHit      = ws.hitCount.as<int>((size_t)3 * nk + 1);
         a.nbr       = a.nHit + nk;
         a.overflow  = a.nHit + 2 * (size_t)nk;
