@@ -59,9 +59,10 @@ struct DevModel {
     DevBuf   rx, ry, rz, rnx, rny, rnz;      // resampled model (SoA)
     DevBuf   frames;                         // nSampled x 12 (3x4 row-major)
     DevBuf   entries;                        // uint2 {ref, alpha bits}, bucket-major
-    DevBuf   ventries;                       // uint2 {ref x angleNum, alpha bits}, the same order
-    DevBuf   slots;                          // uint4 {key, start, len, bucket index}
-    DevBuf   buckets;                        // uint2 {start, len} per bucket index
+    DevBuf   ventries;                       // uint2 {ref x angleNum, alpha bits}, same order,
+                                             // each bucket padded to a multiple of 8 entries
+    DevBuf   slots;                          // uint4 {key, start in ventries, len, bucket index}
+    DevBuf   buckets;                        // uint2 {start in ventries, len} per bucket index
     DevBuf   cosSin;                         // 2 * angleNum floats
     int      refsSorted = 0;                 // every bucket's refs ascending
     unsigned generation = 0;                 // bumped by every upload

@@ -250,11 +250,19 @@ void model_upload(DevModel &m, int nSampled, const float *sp, const float *sn, i
     }
     PPF_HIP_CHECK(hipMemcpyAsync(m.entries.as<uint2>(std::max(E, 1)), ent.data(), sizeof(uint2) * E,
                                  hipMemcpyHostToDevice, s));
-    std::vector<uint2> vent(E);
-    for (int e = 0; e < E; e++)
-        vent[e] = make_uint2(refs[e] * (uint32_t)angleNum, ent[e].y);
-    PPF_HIP_CHECK(hipMemcpyAsync(m.ventries.as<uint2>(std::max(E, 1)), vent.data(),
-                                 sizeof(uint2) * E, hipMemcpyHostToDevice, s));
+    // the voting copy: refs premultiplied by angleNum, every bucket starting
+    // on an 8-entry (64-byte) boundary so a segment of 8 entries is four
+    // aligned 16-byte loads; padding entries are never read as votes
+    std::vector<uint32_t> vstart(std::max(K, 1) + 1, 0);
+    for (int k = 0; k < K; k++)
+        vstart[k + 1] = vstart[k] + (uint32_t)((offs[k + 1] - offs[k] + 7) & ~7);
+    const size_t      VE = std::max<size_t>(vstart[K], 8);
+    std::vector<uint2> vent(VE, make_uint2(0xFFFFFFFFu, 0u));
+    for (int k = 0; k < K; k++)
+        for (int e = offs[k]; e < offs[k + 1]; e++)
+            vent[vstart[k] + (e - offs[k])] = make_uint2(refs[e] * (uint32_t)angleNum, ent[e].y);
+    PPF_HIP_CHECK(hipMemcpyAsync(m.ventries.as<uint2>(VE), vent.data(), sizeof(uint2) * VE,
+                                 hipMemcpyHostToDevice, s));
     // open-addressing key table, load factor <= 1/2; an empty slot has len 0
     uint32_t T = 64;
     while (T < 2u * (uint32_t)std::max(K, 1))
@@ -264,8 +272,7 @@ void model_upload(DevModel &m, int nSampled, const float *sp, const float *sn, i
         uint32_t h = keys[k] & (T - 1);
         while (slots[h].z != 0)
             h = (h + 1) & (T - 1);
-        slots[h] = make_uint4(keys[k], (uint32_t)offs[k], (uint32_t)(offs[k + 1] - offs[k]),
-                              (uint32_t)k);
+        slots[h] = make_uint4(keys[k], vstart[k], (uint32_t)(offs[k + 1] - offs[k]), (uint32_t)k);
     }
     m.tmask = T - 1;
     // buckets by index, and whether every bucket lists its references in
@@ -274,7 +281,7 @@ void model_upload(DevModel &m, int nSampled, const float *sp, const float *sn, i
     std::vector<uint2> bk(std::max(K, 1));
     m.refsSorted = 1;
     for (int k = 0; k < K; k++) {
-        bk[k] = make_uint2((uint32_t)offs[k], (uint32_t)(offs[k + 1] - offs[k]));
+        bk[k] = make_uint2(vstart[k], (uint32_t)(offs[k + 1] - offs[k]));
         for (int e = offs[k] + 1; e < offs[k + 1]; e++)
             if (refs[e] < refs[e - 1])
                 m.refsSorted = 0;

@@ -45,8 +45,7 @@ struct VoteArgs {
     int          nk;
     const uint4 *slots;
     uint32_t     tmask;
-    const uint2 *entries;
-    const uint2 *ventries;      // the same entries with ref x angleNum (voting)
+    const uint2 *ventries;      // bucket entries {ref x angleNum, alpha}, buckets 64 B aligned
     int          refNum, angleNum, tileRefs, nTiles;
     float        angleStep, distStep, sMaxId, D2, searchR, voteThreshold;
     unsigned long long *cand;   // [nk * nTiles * 3]
@@ -97,26 +96,37 @@ __device__ __forceinline__ Top3 top3_add(Top3 t, unsigned long long x) {
     return Top3{hi1, hi2, max(t.c, lo2)};
 }
 
-// Votes of one run of cnt (<= SEG) bucket entries from entry index e0 into
-// the accumulator tile whose first reference row starts at r0A (= r0 x
-// angleNum); entries from tailStart on take the bucket-tail bin rule.  The
-// vote entries hold ref x angleNum, so the accumulator index is one add.
+// Votes of one segment: the 8 bucket entries from entry index e0 (a
+// multiple of 8: buckets start on 64-byte boundaries, so the segment is four
+// aligned 16-byte loads) of which entries [e0 + vlo, e0 + vhi) are voted,
+// into the accumulator tile whose first reference row starts at r0A
+// (= r0 x angleNum); entries from tailStart on take the bucket-tail bin rule.
+// The vote entries hold ref x angleNum, so the accumulator index is one add.
 // Returns the votes cast.
 struct SegRun {
     uint32_t e0, tailStart;
-    int      cnt;
+    int      vlo, vhi;
     float    offBody, offTail;
 };
+
+__device__ __forceinline__ void load_segment(const VoteArgs &a, uint32_t e0, uint2 (&v)[SEG]) {
+    const uint4 *p = reinterpret_cast<const uint4 *>(a.ventries + e0);
+#pragma unroll
+    for (int q = 0; q < SEG / 2; q++) {
+        const uint4 w = p[q];
+        v[2 * q]      = make_uint2(w.x, w.y);
+        v[2 * q + 1]  = make_uint2(w.z, w.w);
+    }
+}
 
 template <bool RANGE>
 __device__ __forceinline__ int vote_run_loaded(const VoteArgs &a, uint32_t *acc, int r0A, int r1A,
                                                const SegRun &g, const uint2 (&v)[SEG]) {
-    const int tailFrom = (int)min(max((long long)g.tailStart - (long long)g.e0, 0ll),
-                                  (long long)g.cnt);
-    int nv = 0;
+    const int tailFrom = (int)min(max((long long)g.tailStart - (long long)g.e0, 0ll), (long long)SEG);
+    int       nv       = 0;
 #pragma unroll
     for (int kk = 0; kk < SEG; kk++) {
-        if (kk < g.cnt) {
+        if (kk >= g.vlo && kk < g.vhi) {
             const int refA = (int)v[kk].x;
             if (!RANGE || (refA >= r0A && refA < r1A)) {
                 const float am  = __uint_as_float(v[kk].y);
@@ -133,30 +143,24 @@ __device__ __forceinline__ int vote_run_loaded(const VoteArgs &a, uint32_t *acc,
 template <bool RANGE>
 __device__ __forceinline__ int vote_run(const VoteArgs &a, uint32_t *acc, int r0A, int r1A,
                                         const SegRun &g) {
-    const uint2 *ent = a.ventries + g.e0;
-    uint2        v[SEG];
-#pragma unroll
-    for (int kk = 0; kk < SEG; kk++)
-        if (kk < g.cnt)
-            v[kk] = ent[kk];
+    uint2 v[SEG];
+    load_segment(a, g.e0, v);
     return vote_run_loaded<RANGE>(a, acc, r0A, r1A, g, v);
 }
 
-// two runs with their entry loads issued together
+// two segments with their entry loads issued together (the second only
+// when it holds entries)
 template <bool RANGE>
 __device__ __forceinline__ int vote_run2(const VoteArgs &a, uint32_t *acc, int r0A, int r1A,
                                          const SegRun &g1, const SegRun &g2) {
-    const uint2 *p1 = a.ventries + g1.e0, *p2 = a.ventries + g2.e0;
-    uint2        v1[SEG], v2[SEG];
-#pragma unroll
-    for (int kk = 0; kk < SEG; kk++) {
-        if (kk < g1.cnt)
-            v1[kk] = p1[kk];
-        if (kk < g2.cnt)
-            v2[kk] = p2[kk];
-    }
-    return vote_run_loaded<RANGE>(a, acc, r0A, r1A, g1, v1) +
-           vote_run_loaded<RANGE>(a, acc, r0A, r1A, g2, v2);
+    uint2 v1[SEG], v2[SEG];
+    load_segment(a, g1.e0, v1);
+    if (g2.vhi > g2.vlo)
+        load_segment(a, g2.e0, v2);
+    int nv = vote_run_loaded<RANGE>(a, acc, r0A, r1A, g1, v1);
+    if (g2.vhi > g2.vlo)
+        nv += vote_run_loaded<RANGE>(a, acc, r0A, r1A, g2, v2);
+    return nv;
 }
 
 // orders this wave's LDS writes before its later LDS reads (a wave runs in
@@ -213,7 +217,8 @@ __device__ __forceinline__ void wave_vote(const VoteArgs &a, uint32_t *acc, int 
         const uint32_t st = qStart[lo], L = qLen[lo];
         SegRun         g;
         g.e0        = st + (uint32_t)(t - qPre[lo]) * SEG;
-        g.cnt       = (int)min((uint32_t)SEG, st + L - g.e0);
+        g.vlo       = 0;
+        g.vhi       = (int)min((uint32_t)SEG, st + L - g.e0);
         g.tailStart = st + L - (L & 7u);
         g.offBody   = ppfc::bin_offset_body(qAlpha[lo]);
         g.offTail   = ppfc::bin_offset_tail(qAlpha[lo]);
@@ -534,7 +539,8 @@ __global__ void __launch_bounds__(BLOCK) vote_kernel(VoteArgs a) {
             const uint32_t st = qStart[ew][ei], L = qLen[ew][ei];
             SegRun         gr;
             gr.e0        = st + (uint32_t)(t - flat[lo]) * SEG;
-            gr.cnt       = (int)min((uint32_t)SEG, st + L - gr.e0);
+            gr.vlo       = 0;
+            gr.vhi       = (int)min((uint32_t)SEG, st + L - gr.e0);
             gr.tailStart = st + L - (L & 7u);
             gr.offBody   = ppfc::bin_offset_body(qAlpha[ew][ei]);
             gr.offTail   = ppfc::bin_offset_tail(qAlpha[ew][ei]);
@@ -740,8 +746,7 @@ template <int BLOCK>
 __global__ void __launch_bounds__(BLOCK) vote_hits_kernel(VoteArgs a) {
     extern __shared__ uint32_t acc[];
     __shared__ int      sPre[BLOCK];
-    __shared__ uint32_t sLo[BLOCK], sTail[BLOCK];
-    __shared__ int      sCnt[BLOCK];
+    __shared__ uint32_t sLo[BLOCK], sHi[BLOCK], sTail[BLOCK];
     __shared__ float    sOffB[BLOCK], sOffT[BLOCK];
     __shared__ int      sScan[BLOCK / 64 + 1];
     __shared__ unsigned long long sTop[(BLOCK / 64) * 3];
@@ -781,17 +786,20 @@ __global__ void __launch_bounds__(BLOCK) vote_hits_kernel(VoteArgs a) {
                 lo                 = sp[tile];
                 hi                 = sp[tile + 1];
             }
-            cnt            = (int)(hi - lo);
+            // aligned segments [lo & ~7, roundup8(hi)) with entries outside
+            // [lo, hi) masked
+            const uint32_t al = lo & ~7u, ah = (hi + 7u) & ~7u;
+            cnt            = hi > lo ? (int)((ah - al) / SEG) : 0;
             sLo[tid]       = lo;
+            sHi[tid]       = hi;
             sTail[tid]     = st + L - (L & 7u);
             const float aS = a.hitAlpha[base + i];
             sOffB[tid]     = ppfc::bin_offset_body(aS);
             sOffT[tid]     = ppfc::bin_offset_tail(aS);
         }
         int       totSeg;
-        const int pre = block_excl_scan<BLOCK>((cnt + SEG - 1) / SEG, sScan, totSeg);
+        const int pre = block_excl_scan<BLOCK>(cnt, sScan, totSeg);
         sPre[tid]     = pre;
-        sCnt[tid]     = cnt;
         __syncthreads();
         // each thread votes a contiguous run of segments (SEG entries of one
         // bucket each; different lanes, different buckets, so the LDS atomics
@@ -816,23 +824,24 @@ __global__ void __launch_bounds__(BLOCK) vote_hits_kernel(VoteArgs a) {
             for (int t = t0; t < t1; t += 2) {
                 while (h + 1 < nb && sPre[h + 1] <= t)
                     h++;
-                SegRun g1, g2;
-                g1.e0        = sLo[h] + (uint32_t)(t - sPre[h]) * SEG;
-                g1.cnt       = min(SEG, (int)(sLo[h] + sCnt[h] - g1.e0));
-                g1.tailStart = sTail[h];
-                g1.offBody   = sOffB[h];
-                g1.offTail   = sOffT[h];
-                g2           = g1;
-                g2.cnt       = 0;
+                auto seg = [&](int hq, int tq) {
+                    SegRun g;
+                    g.e0        = (sLo[hq] & ~7u) + (uint32_t)(tq - sPre[hq]) * SEG;
+                    g.vlo       = (int)max((long long)sLo[hq] - (long long)g.e0, 0ll);
+                    g.vhi       = (int)min((long long)sHi[hq] - (long long)g.e0, (long long)SEG);
+                    g.tailStart = sTail[hq];
+                    g.offBody   = sOffB[hq];
+                    g.offTail   = sOffT[hq];
+                    return g;
+                };
+                const SegRun g1 = seg(h, t);
+                SegRun       g2 = g1;
+                g2.vhi          = g2.vlo;  // empty unless a second segment follows
                 if (t + 1 < t1) {
                     int h2 = h;
                     while (h2 + 1 < nb && sPre[h2 + 1] <= t + 1)
                         h2++;
-                    g2.e0        = sLo[h2] + (uint32_t)(t + 1 - sPre[h2]) * SEG;
-                    g2.cnt       = min(SEG, (int)(sLo[h2] + sCnt[h2] - g2.e0));
-                    g2.tailStart = sTail[h2];
-                    g2.offBody   = sOffB[h2];
-                    g2.offTail   = sOffT[h2];
+                    g2 = seg(h2, t + 1);
                 }
                 nv += a.nTiles > 1 ? vote_run2<true>(a, acc, r0A, r1A, g1, g2)
                                    : vote_run2<false>(a, acc, r0A, r1A, g1, g2);
@@ -929,15 +938,16 @@ static void launch_vote(const VoteArgs &a, int blocks, size_t lds, hipStream_t s
 
 // split points of every bucket for tiles of tileRefs references: the first
 // entry of tile t is the first entry whose ref >= t * tileRefs (refs ascend)
-__global__ void bucket_split_kernel(const uint2 *buckets, int K, const uint2 *entries, int tileRefs,
-                                    int nTiles, uint32_t *splits) {
+__global__ void bucket_split_kernel(const uint2 *buckets, int K, const uint2 *ventries,
+                                    int tileRefs, int angleNum, int nTiles, uint32_t *splits) {
     int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= K)
         return;
     const uint2 bk = buckets[b];
     for (int t = 0; t <= nTiles; t++) {
         uint32_t lo = bk.x, hi = bk.x + bk.y;  // first e in [lo, hi) with ref >= t * tileRefs
-        const uint32_t r = (uint32_t)t * (uint32_t)tileRefs;
+        const uint32_t r = (uint32_t)t * (uint32_t)tileRefs * (uint32_t)angleNum;
+        const uint2   *entries = ventries;
         while (lo < hi) {
             uint32_t mid = (lo + hi) >> 1;
             if (entries[mid].x < r)
@@ -986,7 +996,6 @@ int vote_and_poses(Workspace &ws, const DevModel &m, const GridView &g, const fl
     a.nk = nk;
     a.slots    = m.slots.get<uint4>();
     a.tmask    = m.tmask;
-    a.entries  = m.entries.get<uint2>();
     a.ventries = m.ventries.get<uint2>();
     a.refNum   = m.nSampled;
     a.angleNum = m.angleNum;
@@ -1040,7 +1049,8 @@ int vote_and_poses(Workspace &ws, const DevModel &m, const GridView &g, const fl
                 ws.splitTiles != nTiles) {
                 uint32_t *sp = ws.splits.as<uint32_t>((size_t)m.nKeys * (nTiles + 1));
                 bucket_split_kernel<<<div_up(m.nKeys, 256), 256, 0, s>>>(
-                    m.buckets.get<uint2>(), m.nKeys, m.entries.get<uint2>(), tileRefs, nTiles, sp);
+                    m.buckets.get<uint2>(), m.nKeys, m.ventries.get<uint2>(), tileRefs,
+                    m.angleNum, nTiles, sp);
                 PPF_HIP_CHECK(hipGetLastError());
                 ws.splitGen   = m.generation;
                 ws.splitRefs  = tileRefs;
