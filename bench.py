@@ -138,6 +138,49 @@ def icp_batching(det, H_per_instance=64, n_inst=8, seed=0x5EED0003):
             "max_batched_vs_plain_deg": round(float(diff), 6)}
 
 
+def scene_batching(seed=0x5EED0003, n_scenes=64):
+    """Many scenes of uneven size matched together (one vote launch for all
+    key points, one batched refinement per stage) vs one matchScene each,
+    with the box+cylinder model of 20k points."""
+    from surface_match_amd import Detector, PointCloud, MatchParam, ResidentScene, synth
+    mp, mn = synth.make_model(20000, 0x5EED0001)
+    det = Detector()
+    det.trainModel(PointCloud(mp, mn), 0.04)
+    D = det.model_info().diameter
+    scenes = synth.batch_scenes(seed, n_scenes)
+    rs = []
+    for sc in scenes:
+        c = PointCloud(sc.points)
+        c.viewPoint = np.array(synth.VIEWPOINT, np.float32)
+        rs.append(ResidentScene(det, c))
+    param = MatchParam(numMatches=8)
+    ResidentScene.match_many(rs, 0.04, 0.1, 0.2, param)       # warm-up
+    t0 = time.perf_counter()
+    many, st = ResidentScene.match_many(rs, 0.04, 0.1, 0.2, param)
+    wb = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    single = [r.match(0.04, 0.1, 0.2, param) for r in rs]
+    ws = time.perf_counter() - t0
+    same = all(np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+               for a, b in zip(many, single))
+    found = tot = 0
+    for sc, (P, S) in zip(scenes, many):
+        tot += len(sc.gt_poses)
+        hit = set()
+        for T in P:
+            k, ang, dt = synth.pose_error(T, sc.gt_poses, D)
+            if ang <= 1.0 and dt <= 0.01 * D:
+                hit.add(k)
+        found += len(hit)
+    return {"scenes": n_scenes, "points": int(sum(len(s.points) for s in scenes)),
+            "instances": tot, "keypoints": int(st["nKeypoints"]),
+            "hypotheses_refined": int(st["nSparseIcp"]), "dense_refined": int(st["nDenseIcp"]),
+            "batched_ms": round(wb * 1e3, 2), "one_by_one_ms": round(ws * 1e3, 2),
+            "speedup": round(ws / wb, 2), "scenes_per_s": round(n_scenes / wb, 1),
+            "results_identical": bool(same),
+            "instances_found_within_1deg_1pctD": found}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -146,6 +189,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-icp-batching", action="store_true")
     ap.add_argument("--autotune", action="store_true")
+    ap.add_argument("--no-scene-batching", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the PPF path runs on a single MI355X "
@@ -240,6 +284,8 @@ def main():
     }
     if not args.no_icp_batching:
         out["icp_batching"] = icp_batching(det)
+    if not args.no_scene_batching:
+        out["scene_batching"] = scene_batching()
     if not args.no_cpu_baseline:
         threads = min(os.cpu_count() or 1, 8)
         cms, cst = cpu_baseline(w, threads)

@@ -134,6 +134,19 @@ PPF_API int ppf_match_scene_resident(ppf_detector d, ppf_scene s, float sampling
                                      float keyPointFraction, float minScore,
                                      const ppf_match_param *param, float *poses16, float *scores,
                                      size_t cap, size_t *nOut, ppf_match_stats *stats);
+/* -- several resident scenes in one pass (no reference counterpart: the
+ * reference matches one scene per call).  Each scene is sampled and given
+ * normals on its own; the key points of all scenes are voted in one launch
+ * (each against its own scene) and the hypotheses of all scenes are refined
+ * in one batched refinement per stage.  Scene i's results (at most cap) go
+ * to poses16[16 * cap * i ..] and scores[cap * i ..], their count to
+ * nOut[i]; they equal ppf_match_scene_resident's for that scene.  stats
+ * (optional) sums the counts over the scenes and times the whole batch. -- */
+PPF_API int ppf_match_scenes_resident(ppf_detector d, const ppf_scene *scenes, size_t nScenes,
+                                      float samplingDistanceRel, float keyPointFraction,
+                                      float minScore, const ppf_match_param *param,
+                                      float *poses16, float *scores, size_t cap, size_t *nOut,
+                                      ppf_match_stats *stats);
 
 /* -- launch tuning (no reference counterpart).  ppf_autotune times the
  * vote stage over {split, fused} kernels, then workgroup sizes {256, 512,

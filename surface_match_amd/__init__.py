@@ -235,6 +235,35 @@ class ResidentScene:
         k = nout.value
         return poses[:k].reshape(k, 4, 4).transpose(0, 2, 1).copy(), scores[:k].copy(), st.as_dict()
 
+    @staticmethod
+    def match_many(scenes, samplingDistanceRel=0.04, keyPointFraction=0.2, minScore=0.2,
+                   param=None):
+        """Match several resident scenes of one detector together
+        (ppf_match_scenes_resident): returns [(poses [k,4,4], scores [k])] per
+        scene, and the summed stats."""
+        if not scenes:
+            return [], {}
+        det = scenes[0]._det
+        if any(sc._det is not det for sc in scenes):
+            raise ValueError("all scenes must belong to one detector")
+        param = param or MatchParam()
+        cap = max(int(param.numMatches), 1)
+        n = len(scenes)
+        poses = np.zeros((n, cap, 16), np.float32)
+        scores = np.zeros((n, cap), np.float32)
+        nout = np.zeros(n, np.uint64)
+        hs = (C.c_void_p * n)(*[sc._h.value for sc in scenes])
+        st = N.MatchStats()
+        N.check(N.lib().ppf_match_scenes_resident(det._h, C.cast(hs, C.c_void_p), n,
+                                                  samplingDistanceRel, keyPointFraction, minScore,
+                                                  C.byref(param), N.ptr(poses), N.ptr(scores), cap,
+                                                  N.ptr(nout), C.byref(st)))
+        out = []
+        for i in range(n):
+            k = int(nout[i])
+            out.append((poses[i, :k].reshape(k, 4, 4).transpose(0, 2, 1).copy(), scores[i, :k].copy()))
+        return out, st.as_dict()
+
     def autotune(self, samplingDistanceRel=0.04, keyPointFraction=0.2, minScore=0.2, param=None):
         """Pick the vote/ICP launch shapes that run fastest on this scene."""
         param = param or MatchParam()

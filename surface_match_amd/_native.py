@@ -96,7 +96,7 @@ EXPORTS = [
     "ppf_model_get_table", "ppf_pair_features", "ppf_vote", "ppf_voxel_sample",
     "ppf_radius_search", "ppf_knn_search", "ppf_estimate_normals", "ppf_device_math",
     "ppf_icp_target_create", "ppf_icp_target_destroy", "ppf_icp_refine", "ppf_icp_adjoint",
-    "ppf_autotune", "ppf_set_tuning", "ppf_get_tuning",
+    "ppf_autotune", "ppf_set_tuning", "ppf_get_tuning", "ppf_match_scenes_resident",
 ]
 
 
@@ -143,6 +143,8 @@ def lib() -> C.CDLL:
         "ppf_autotune": (i, [vp, vp, f, f, f, C.POINTER(MatchParam), C.POINTER(TuneReport)]),
         "ppf_set_tuning": (i, [vp, C.POINTER(Tuning)]),
         "ppf_get_tuning": (i, [vp, C.POINTER(Tuning)]),
+        "ppf_match_scenes_resident": (i, [vp, vp, sz, f, f, f, C.POINTER(MatchParam), vp, vp, sz,
+                                          vp, C.POINTER(MatchStats)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

@@ -100,6 +100,12 @@ __global__ void split_kernel(const float *aos, int n, float *x, float *y, float 
     y[i] = aos[3 * (size_t)i + 1];
     z[i] = aos[3 * (size_t)i + 2];
 }
+__global__ void fill_int_kernel(int *a, int n, int v) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        a[i] = v;
+}
+
 __global__ void fill_nan_kernel(float *a, float *b, float *c, int n) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n)
@@ -231,6 +237,14 @@ struct ppf_detector_s {
     DevCloud     tmpScene;
     DevBuf       idxA, idxB, idxC, idxD;  // sampled / keypoints / dense / scratch
     hipEvent_t   ev[13] = {};
+    // per-scene state of a batched match (grown on demand, reused)
+    struct SceneWork {
+        DevBuf samp, kp, dense;
+        Grid   gVote, gSparse, gFull, gDense;
+        int    ns = 0, nk = 0, nd = 0;
+    };
+    std::vector<std::unique_ptr<SceneWork>> sceneWork;
+    DevBuf batchKpScene, batchKpId, batchGrids, batchHypScene;
 
     ppf_detector_s() {
         PPF_HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
@@ -616,6 +630,339 @@ void match_impl(ppf_detector_s &D, DevCloud &c, float rel, float kpFrac, float m
     st->msTotal         = elapsed(D.ev[0], D.ev[8]);
     st->msIcpSparse     = nSparse ? elapsed(D.ev[9], D.ev[10]) : 0.0f;
     st->msIcpDense      = nDenseIcp ? elapsed(D.ev[11], D.ev[12]) : 0.0f;
+}
+
+// Several scenes matched together (same parameters): each scene is sampled,
+// given normals and key points on its own, then the key points of all scenes
+// are voted in one launch (each key point searches its own scene's grid),
+// the poses are clustered per scene, and the hypotheses of all scenes are
+// refined in one batched refinement per stage (each against its own scene).
+// Every per-scene result equals match_impl's on that scene.
+void match_batch_impl(ppf_detector_s &D, const std::vector<DevCloud *> &clouds, float rel,
+                      float kpFrac, float minScore, const ppf_match_param &p,
+                      std::vector<MatchOut> &outs, ppf_match_stats *st) {
+    if (!D.trained)
+        throw std::runtime_error("No trained model in matchScene");
+    if (rel > 1 || rel < 0)
+        throw std::range_error("Invalid Input: samplingDistanceRel range mismatch in matchScene");
+    if (kpFrac > 1 || kpFrac < 0)
+        throw std::range_error("Invalid Input: keyPointFraction range mismatch in matchScene");
+    if (minScore > 1 || minScore < 0)
+        throw std::range_error("Invalid Input: minScore range mismatch in matchScene");
+    for (DevCloud *c : clouds)
+        if (c->n <= 0)
+            throw std::runtime_error("Invalid Input: empty scene in matchScene");
+    check_match_param(p);
+    const int   n  = (int)clouds.size();
+    hipStream_t s  = D.stream;
+    HostModel  &hm = D.hm;
+    DevModel   &dm = D.dm;
+    ppf_match_stats stl{};
+    if (!st)
+        st = &stl;
+    std::memset(st, 0, sizeof *st);
+    outs.assign(n, MatchOut{});
+    while ((int)D.sceneWork.size() < n)
+        D.sceneWork.emplace_back(new ppf_detector_s::SceneWork());
+    // ppf.cpp:183-240 (the same derived parameters for every scene)
+    const float Dm           = box_diameter(hm.sampled.boxMin, hm.sampled.boxMax);
+    const float angleStep    = ppfc::kTwoPi / (float)hm.param.featAngleResolution;
+    const int   refNum       = hm.sampled.size();
+    const float reSampleStep = Dm * hm.param.poseRefRelSamplingDistance;
+    const float sampleStep   = Dm * rel;
+    const float normCell     = sampleStep * (float)D.normCell4 * 0.25f;
+    const float voteThreshold = (float)refNum * minScore;
+    float maxOverlapDist = 0;
+    if (p.maxOverlapDistRel > 0) maxOverlapDist = Dm * p.maxOverlapDistRel;
+    if (p.maxOverlapDistAbs > 0) maxOverlapDist = p.maxOverlapDistAbs;
+    float poseRefDist = 0;
+    if (p.poseRefDistThresholdRel > 0) poseRefDist = Dm * p.poseRefDistThresholdRel;
+    if (p.poseRefDistThresholdAbs > 0) poseRefDist = p.poseRefDistThresholdAbs;
+    float scoringDist = 0;
+    if (p.poseRefScoringDistRel > 0) scoringDist = Dm * p.poseRefScoringDistRel;
+    if (p.poseRefScoringDistAbs > 0) scoringDist = p.poseRefScoringDistAbs;
+    const float keySampleStep = std::sqrt(1.0f / kpFrac) * sampleStep;
+    int        *cnt           = D.ws.smallCnt.as<int>(8);
+
+    PPF_HIP_CHECK(hipEventRecord(D.ev[0], s));
+    // ---- per scene: sampling, normals, key points, vote grid
+    std::vector<int> ko(n + 1, 0);
+    float            minCell = FLT_MAX;
+    for (int i = 0; i < n; i++) {
+        DevCloud &c  = *clouds[i];
+        auto     &sw = *D.sceneWork[i];
+        c.reset(s);
+        float *PX = c.px.get<float>(), *PY = c.py.get<float>(), *PZ = c.pz.get<float>();
+        float *NX = c.nx.get<float>(), *NY = c.ny.get<float>(), *NZ = c.nz.get<float>();
+        int   *samp = sw.samp.as<int>(std::max(c.nValid, 1));
+        sw.ns = voxel_sample(D.ws, PX, PY, PZ, c.valid.get<int>(), c.nValid, sampleStep, c.boxMin,
+                             samp, cnt, s);
+        if (!c.hasNormal)
+            estimate_normals(D.ws, c.n, PX, PY, PZ, NX, NY, NZ, c.valid.get<int>(), c.nValid, samp,
+                             sw.ns, p.knnNormal, p.smoothNormal, p.invertNormal, c.vp, normCell, s);
+        int *kp = sw.kp.as<int>(std::max(sw.ns, 1));
+        sw.nk   = voxel_sample(D.ws, PX, PY, PZ, samp, sw.ns, keySampleStep, c.boxMin, kp, cnt, s);
+        grid_build(sw.gVote, PX, PY, PZ, NX, NY, NZ, samp, sw.ns, Dm / (float)D.vcfg.cellDiv, s);
+        minCell   = std::min(minCell, sw.gVote.view.cell);
+        ko[i + 1] = ko[i] + sw.nk;
+    }
+    PPF_HIP_CHECK(hipEventRecord(D.ev[3], s));
+    // ---- every scene's key points in one vote
+    const int    K = ko[n];
+    VoteStats    vs{};
+    int          np = 0;
+    std::vector<int> pOff(n + 1, 0);
+    if (K > 0) {
+        float *kx = D.ws.kpX.as<float>(K), *ky = D.ws.kpY.as<float>(K), *kz = D.ws.kpZ.as<float>(K);
+        float *knx = D.ws.kpNX.as<float>(K), *kny = D.ws.kpNY.as<float>(K),
+              *knz = D.ws.kpNZ.as<float>(K);
+        int   *kpScene = D.batchKpScene.as<int>(K), *kpId = D.batchKpId.as<int>(K);
+        std::vector<GridView> views(n);
+        for (int i = 0; i < n; i++) {
+            DevCloud &c  = *clouds[i];
+            auto     &sw = *D.sceneWork[i];
+            views[i]     = sw.gVote.view;
+            if (sw.nk == 0)
+                continue;
+            const int o = ko[i];
+            gather_keypoints(sw.kp.get<int>(), sw.nk, c.px.get<float>(), c.py.get<float>(),
+                             c.pz.get<float>(), c.nx.get<float>(), c.ny.get<float>(),
+                             c.nz.get<float>(), kx + o, ky + o, kz + o, knx + o, kny + o, knz + o,
+                             s);
+            fill_int_kernel<<<div_up(sw.nk, 256), 256, 0, s>>>(kpScene + o, sw.nk, i);
+            PPF_HIP_CHECK(hipMemcpyAsync(kpId + o, sw.kp.get<int>(), sizeof(int) * sw.nk,
+                                         hipMemcpyDeviceToDevice, s));
+        }
+        GridView *gv = D.batchGrids.as<GridView>(n);
+        PPF_HIP_CHECK(hipMemcpyAsync(gv, views.data(), sizeof(GridView) * n,
+                                     hipMemcpyHostToDevice, s));
+        VoteKeypoints in;
+        in.g        = views[0];
+        in.gridsDev = gv;
+        in.kpScene  = kpScene;
+        in.minCell  = minCell;
+        in.kx = kx;
+        in.ky = ky;
+        in.kz = kz;
+        in.knx = knx;
+        in.kny = kny;
+        in.knz = knz;
+        in.kp  = kpId;
+        in.nk  = K;
+        np = vote_keypoints(D.ws, dm, in, voteThreshold, D.vcfg, &vs, nullptr, 0, s);
+        // poses are compacted in key point order: scene i's are those of its key points
+        std::vector<int> offs(K + 1);
+        PPF_HIP_CHECK(hipMemcpyAsync(offs.data(), D.ws.voteCnt.get<int>() + K + 1,
+                                     sizeof(int) * (K + 1), hipMemcpyDeviceToHost, s));
+        PPF_HIP_CHECK(hipStreamSynchronize(s));
+        for (int i = 0; i <= n; i++)
+            pOff[i] = offs[ko[i]];
+    }
+    PPF_HIP_CHECK(hipEventRecord(D.ev[5], s));
+    // ---- clustering per scene (ppf.cpp:413-420)
+    float center[3];
+    for (int a = 0; a < 3; a++)
+        center[a] = (hm.sampled.boxMin[a] + hm.sampled.boxMax[a]) / 2.0f;
+    std::vector<std::vector<Hypo>> hyp(n);
+    int                            nClustersTot = 0;
+    for (int i = 0; i < n; i++) {
+        int nc = 0;
+        cluster_poses(D.ws, D.ws.poses.get<PoseRec>() + pOff[i], pOff[i + 1] - pOff[i], 0.1f * Dm,
+                      angleStep, center, maxOverlapDist, hyp[i], &nc, s);
+        nClustersTot += nc;
+    }
+    PPF_HIP_CHECK(hipEventRecord(D.ev[6], s));
+    // ---- refinement (ppf.cpp:423-495), every scene's hypotheses together
+    struct Cand {
+        int scene, h;
+    };
+    std::vector<Cand>               cand;
+    std::vector<std::vector<float>> poseNow(n), scoreNow(n);
+    std::vector<std::vector<char>>  alive(n);
+    for (int i = 0; i < n; i++) {
+        poseNow[i].assign(hyp[i].size() * 12, 0.0f);
+        scoreNow[i].assign(hyp[i].size(), 0.0f);
+        alive[i].assign(hyp[i].size(), 0);
+        for (int h = 0; h < (int)hyp[i].size(); h++)
+            if (!(hyp[i][h].votes < voteThreshold)) {
+                cand.push_back({i, h});
+                std::memcpy(&poseNow[i][12 * (size_t)h], hyp[i][h].pose, sizeof(float) * 12);
+                scoreNow[i][h] = hyp[i][h].votes;
+                alive[i][h]    = 1;
+            }
+    }
+    // dense samples (ppf.cpp:427-434) of the scenes that need them
+    std::vector<char> sceneHasCand(n, 0);
+    for (auto &c : cand)
+        sceneHasCand[c.scene] = 1;
+    int nSparse = 0, nDenseIcp = 0, nDenseTot = 0;
+    if (p.sparsePoseRefinement && !cand.empty()) {
+        std::vector<GridView> vs1(n), vf(n);
+        for (int i = 0; i < n; i++) {
+            if (!sceneHasCand[i])
+                continue;
+            DevCloud &c  = *clouds[i];
+            auto     &sw = *D.sceneWork[i];
+            float *PX = c.px.get<float>(), *PY = c.py.get<float>(), *PZ = c.pz.get<float>();
+            float *NX = c.nx.get<float>(), *NY = c.ny.get<float>(), *NZ = c.nz.get<float>();
+            if (p.densePoseRefinement) {
+                // ppf.cpp:427-434: the dense samples' normals overwrite the scene's
+                // before either refinement reads them
+                int *dense = sw.dense.as<int>(std::max(c.nValid, 1));
+                sw.nd = voxel_sample(D.ws, PX, PY, PZ, c.valid.get<int>(), c.nValid, reSampleStep,
+                                     c.boxMin, dense, cnt, s);
+                if (!c.hasNormal)
+                    estimate_normals(D.ws, c.n, PX, PY, PZ, NX, NY, NZ, c.valid.get<int>(),
+                                     c.nValid, dense, sw.nd, p.knnNormal, 0, p.invertNormal, c.vp,
+                                     normCell, s);
+                nDenseTot += sw.nd;
+            }
+            grid_build(sw.gSparse, PX, PY, PZ, NX, NY, NZ, sw.samp.get<int>(), sw.ns, sampleStep, s);
+            grid_build(sw.gFull, PX, PY, PZ, nullptr, nullptr, nullptr, c.valid.get<int>(),
+                       c.nValid, std::max(scoringDist * 2.0f, sampleStep * 0.25f), s);
+            vs1[i] = sw.gSparse.view;
+            vf[i]  = sw.gFull.view;
+        }
+        GridView *gS = D.batchGrids.as<GridView>((size_t)2 * n);
+        PPF_HIP_CHECK(hipMemcpyAsync(gS, vs1.data(), sizeof(GridView) * n, hipMemcpyHostToDevice, s));
+        PPF_HIP_CHECK(hipMemcpyAsync(gS + n, vf.data(), sizeof(GridView) * n, hipMemcpyHostToDevice,
+                                     s));
+        std::vector<float> init(cand.size() * 12);
+        std::vector<int>   hs(cand.size());
+        for (size_t q = 0; q < cand.size(); q++) {
+            std::memcpy(&init[12 * q], hyp[cand[q].scene][cand[q].h].pose, sizeof(float) * 12);
+            hs[q] = cand[q].scene;
+        }
+        int *hsD = D.batchHypScene.as<int>(cand.size());
+        PPF_HIP_CHECK(hipMemcpyAsync(hsD, hs.data(), sizeof(int) * hs.size(), hipMemcpyHostToDevice,
+                                     s));
+        const int first = cand[0].scene;
+        IcpCriteria sc{5, poseRefDist, sampleStep * 0.5f, sampleStep, 0.05f};  // ppf.cpp:423
+        std::vector<IcpResultDev> res(cand.size());
+        icp_batch(D.ws, refNum, dm.sx.get<float>(), dm.sy.get<float>(), dm.sz.get<float>(),
+                  dm.snx.get<float>(), dm.sny.get<float>(), dm.snz.get<float>(), vs1[first],
+                  vf[first], init.data(), (int)cand.size(), sc, scoringDist, D.icfg, res.data(), s,
+                  gS, gS + n, hsD);
+        nSparse = (int)cand.size();
+        std::vector<Cand> survivors;
+        for (size_t q = 0; q < cand.size(); q++) {
+            const int i = cand[q].scene, h = cand[q].h;
+            if (!res[q].converged) {
+                alive[i][h] = 0;
+                continue;
+            }
+            std::memcpy(&poseNow[i][12 * (size_t)h], res[q].pose, sizeof(float) * 12);
+            float score = (float)res[q].inliers / (float)refNum;
+            if (score > 1.0f)
+                score = 1.0f;
+            scoreNow[i][h] = score;
+            if (score < minScore) {
+                alive[i][h] = 0;
+                continue;
+            }
+            survivors.push_back(cand[q]);
+        }
+        if (p.densePoseRefinement && !survivors.empty()) {
+            std::vector<char> needDense(n, 0);
+            for (auto &c : survivors)
+                needDense[c.scene] = 1;
+            std::vector<GridView> vd(n);
+            for (int i = 0; i < n; i++) {
+                if (!needDense[i])
+                    continue;
+                DevCloud &c  = *clouds[i];
+                auto     &sw = *D.sceneWork[i];
+                float *PX = c.px.get<float>(), *PY = c.py.get<float>(), *PZ = c.pz.get<float>();
+                float *NX = c.nx.get<float>(), *NY = c.ny.get<float>(), *NZ = c.nz.get<float>();
+                grid_build(sw.gDense, PX, PY, PZ, NX, NY, NZ, sw.dense.get<int>(), sw.nd,
+                           reSampleStep * 2.0f, s);
+                vd[i] = sw.gDense.view;
+            }
+            GridView *gD = D.batchGrids.as<GridView>((size_t)3 * n);
+            PPF_HIP_CHECK(hipMemcpyAsync(gD + 2 * n, vd.data(), sizeof(GridView) * n,
+                                         hipMemcpyHostToDevice, s));
+            PPF_HIP_CHECK(hipMemcpyAsync(gD + n, vf.data(), sizeof(GridView) * n,
+                                         hipMemcpyHostToDevice, s));
+            std::vector<float> init2(survivors.size() * 12);
+            std::vector<int>   hs2(survivors.size());
+            for (size_t q = 0; q < survivors.size(); q++) {
+                std::memcpy(&init2[12 * q], &poseNow[survivors[q].scene][12 * (size_t)survivors[q].h],
+                            sizeof(float) * 12);
+                hs2[q] = survivors[q].scene;
+            }
+            int *hsD2 = D.batchHypScene.as<int>(survivors.size());
+            PPF_HIP_CHECK(hipMemcpyAsync(hsD2, hs2.data(), sizeof(int) * hs2.size(),
+                                         hipMemcpyHostToDevice, s));
+            const int first2 = survivors[0].scene;
+            IcpCriteria dc{p.poseRefNumSteps, poseRefDist, reSampleStep * 0.5f, sampleStep, 0.05f};
+            std::vector<IcpResultDev> res2(survivors.size());
+            icp_batch(D.ws, hm.reSampled.size(), dm.rx.get<float>(), dm.ry.get<float>(),
+                      dm.rz.get<float>(), dm.rnx.get<float>(), dm.rny.get<float>(),
+                      dm.rnz.get<float>(), vd[first2], vf[first2], init2.data(),
+                      (int)survivors.size(), dc, scoringDist, D.icfg, res2.data(), s, gD + 2 * n,
+                      gD + n, hsD2);
+            nDenseIcp = (int)survivors.size();
+            for (size_t q = 0; q < survivors.size(); q++) {
+                const int i = survivors[q].scene, h = survivors[q].h;
+                if (!res2[q].converged) {
+                    alive[i][h] = 0;
+                    continue;
+                }
+                std::memcpy(&poseNow[i][12 * (size_t)h], res2[q].pose, sizeof(float) * 12);
+                float score = (float)res2[q].inliers / (float)hm.reSampled.size();
+                if (score > 1.0f)
+                    score = 1.0f;
+                scoreNow[i][h] = score;
+            }
+        }
+    }
+    // ordered accept walk per scene (ppf.cpp:439-495), stable sort by score
+    for (int i = 0; i < n; i++) {
+        for (int h = 0; h < (int)hyp[i].size(); h++) {
+            if (!alive[i][h])
+                continue;
+            float score = scoreNow[i][h];
+            if ((p.sparsePoseRefinement || p.densePoseRefinement) && score < minScore)
+                continue;
+            outs[i].results.emplace_back(score, std::vector<float>(&poseNow[i][12 * (size_t)h],
+                                                                   &poseNow[i][12 * (size_t)h] + 12));
+            if ((int)outs[i].results.size() >= p.numMatches)
+                break;
+        }
+        std::stable_sort(outs[i].results.begin(), outs[i].results.end(),
+                         [](const auto &a, const auto &b) { return a.first > b.first; });
+    }
+    PPF_HIP_CHECK(hipEventRecord(D.ev[8], s));
+    PPF_HIP_CHECK(hipEventSynchronize(D.ev[8]));
+    for (int i = 0; i < n; i++) {
+        st->nScene += clouds[i]->n;
+        st->nValid += clouds[i]->nValid;
+        st->nSampled += D.sceneWork[i]->ns;
+        st->nResults += (int)outs[i].results.size();
+    }
+    st->nKeypoints      = K;
+    st->nVotedKeypoints = vs.voted;
+    st->nPoses          = np;
+    st->nClusters       = nClustersTot;
+    int nh = 0;
+    for (auto &v : hyp)
+        nh += (int)v.size();
+    st->nHypotheses   = nh;
+    st->nSparseIcp    = nSparse;
+    st->nDenseIcp     = nDenseIcp;
+    st->nDense        = nDenseTot;
+    st->nPairs        = vs.pairs;
+    st->nKeyHits      = vs.hits;
+    st->nVotes        = vs.votes;
+    st->msSample      = elapsed(D.ev[0], D.ev[3]);  // sampling + normals + key points
+    st->msVote        = elapsed(D.ev[3], D.ev[5]);
+    st->msVoteKernel  = vs.kernelMs;
+    st->msPairKernel  = vs.pairMs;
+    st->msHitsKernel  = vs.hitsMs;
+    st->nVoteFallback = vs.overflow;
+    st->msCluster     = elapsed(D.ev[5], D.ev[6]);
+    st->msIcp         = elapsed(D.ev[6], D.ev[8]);
+    st->msTotal       = elapsed(D.ev[0], D.ev[8]);
 }
 
 void write_poses(const MatchOut &o, float *poses16, float *scores, size_t cap, size_t *nOut) {
@@ -1011,6 +1358,31 @@ PPF_API int ppf_match_scene_resident(ppf_detector d, ppf_scene sc, float rel, fl
         MatchOut o;
         match_impl(*d, sc->cloud, rel, kpFrac, minScore, p, o, nullptr, stats);
         write_poses(o, poses16, scores, cap, nOut);
+    });
+}
+
+PPF_API int ppf_match_scenes_resident(ppf_detector d, const ppf_scene *scenes, size_t nScenes,
+                                      float rel, float kpFrac, float minScore,
+                                      const ppf_match_param *param, float *poses16, float *scores,
+                                      size_t cap, size_t *nOut, ppf_match_stats *stats) {
+    PPF_REQUIRE(d && (scenes || nScenes == 0) && nOut, "null argument");
+    for (size_t i = 0; i < nScenes; i++)
+        PPF_REQUIRE(scenes[i], "null scene");
+    return guarded([&] {
+        ppf_match_param p;
+        if (param)
+            p = *param;
+        else
+            ppf_match_param_default(&p);
+        std::vector<DevCloud *> clouds(nScenes);
+        for (size_t i = 0; i < nScenes; i++)
+            clouds[i] = &scenes[i]->cloud;
+        std::vector<MatchOut> outs;
+        if (nScenes > 0)
+            match_batch_impl(*d, clouds, rel, kpFrac, minScore, p, outs, stats);
+        for (size_t i = 0; i < nScenes; i++)
+            write_poses(outs[i], poses16 ? poses16 + 16 * cap * i : nullptr,
+                        scores ? scores + cap * i : nullptr, cap, nOut + i);
     });
 }
 

@@ -94,7 +94,22 @@ struct IcpArgs {
     int         *hslot;  // H x nSrc: hash slot used
     float       *nnD;    // H x nSrc
     IcpResultDev *out;
+    // several scenes: the target grids of hypothesis h are dstG[hypScene[h]]
+    // and fullG[hypScene[h]] (nullptr: dst and full for every hypothesis)
+    const GridView *dstG, *fullG;
+    const int      *hypScene;
 };
+
+// the arguments with hypothesis h's target grids in dst / full
+__device__ __forceinline__ IcpArgs args_for(const IcpArgs &a, int h) {
+    IcpArgs b = a;
+    if (a.dstG) {
+        const int sc = a.hypScene[h];
+        b.dst        = a.dstG[sc];
+        b.full       = a.fullG[sc];
+    }
+    return b;
+}
 
 // u = pinv(A) rhs for symmetric PSD A (FP64 Jacobi), dropping eigenvalues
 // below 6 * eps_f32 * max |eigenvalue| as JacobiSVD::solve does.
@@ -513,10 +528,11 @@ __device__ __forceinline__ bool inlier_point(const IcpArgs &a, int i, const floa
 // ---------------------------------------------------------------- batched
 
 template <int BLOCK>
-__global__ void __launch_bounds__(BLOCK) icp_kernel(IcpArgs a) {
+__global__ void __launch_bounds__(BLOCK) icp_kernel(IcpArgs a0) {
     const int h = blockIdx.x;
-    if (h >= a.H)
+    if (h >= a0.H)
         return;
+    const IcpArgs a = args_for(a0, h);
     __shared__ IcpState st;
     __shared__ double   sRed[40];
     __shared__ CF       sLds[(BLOCK / 64) * 27];
@@ -692,10 +708,11 @@ __global__ void spread_init_kernel(SpreadArgs w) {
 template <int BLOCK>
 __global__ void __launch_bounds__(BLOCK) spread_corr_kernel(SpreadArgs w) {
     const int h = blockIdx.x / w.nChunkG, chunk = blockIdx.x - h * w.nChunkG;
+    const IcpArgs b = args_for(w.b, h);
     __shared__ float sCur[12];
     __shared__ int   sTab;
     const IcpState  &st = w.st[h];
-    if (!hypo_active(st, w.b.c))
+    if (!hypo_active(st, b.c))
         return;
     if (threadIdx.x < 12)
         sCur[threadIdx.x] = st.curF[threadIdx.x];
@@ -703,20 +720,21 @@ __global__ void __launch_bounds__(BLOCK) spread_corr_kernel(SpreadArgs w) {
         sTab = st.iters & 1;
     __syncthreads();
     const int i = chunk * (BLOCK / ICP_G) + (int)threadIdx.x / ICP_G;
-    if (i < w.b.nSrc)
-        corr_point_group<ICP_G>(w.b, scratch_of(w.b, h, sTab), i, sCur,
-                                w.b.c.rejectDist * w.b.c.rejectDist);
+    if (i < b.nSrc)
+        corr_point_group<ICP_G>(b, scratch_of(b, h, sTab), i, sCur,
+                                b.c.rejectDist * b.c.rejectDist);
 }
 
 template <int BLOCK>
 __global__ void __launch_bounds__(BLOCK) spread_means_kernel(SpreadArgs w) {
     const int h = blockIdx.x / w.nChunk, chunk = blockIdx.x - h * w.nChunk;
+    const IcpArgs b = args_for(w.b, h);
     __shared__ float  sCur[12];
     __shared__ int    sTab, sLast;
     __shared__ CF     sLds[(BLOCK / 64) * 7];
     __shared__ double sRed[7];
     IcpState &st = w.st[h];
-    if (!hypo_active(st, w.b.c))
+    if (!hypo_active(st, b.c))
         return;
     if (threadIdx.x < 12)
         sCur[threadIdx.x] = st.curF[threadIdx.x];
@@ -726,9 +744,9 @@ __global__ void __launch_bounds__(BLOCK) spread_means_kernel(SpreadArgs w) {
     // the other table was last used by the previous iteration: clear this
     // chunk's share of it for the next one
     {
-        const HypoScratch o   = scratch_of(w.b, h, sTab ^ 1);
-        const uint32_t    per = (w.b.T + w.nChunk - 1) / w.nChunk;
-        const uint32_t    lo = per * chunk, hi = min(w.b.T, lo + per);
+        const HypoScratch o   = scratch_of(b, h, sTab ^ 1);
+        const uint32_t    per = (b.T + w.nChunk - 1) / w.nChunk;
+        const uint32_t    lo = per * chunk, hi = min(b.T, lo + per);
         for (uint32_t t = lo + threadIdx.x; t < hi; t += BLOCK) {
             o.hkey[t]  = 0u;
             o.hbest[t] = ~0ull;
@@ -738,8 +756,8 @@ __global__ void __launch_bounds__(BLOCK) spread_means_kernel(SpreadArgs w) {
     for (int k = 0; k < 7; k++)
         m7[k] = CF{0.0f, 0.0f};
     const int i = chunk * BLOCK + threadIdx.x;
-    if (i < w.b.nSrc)
-        add_means(w.b, scratch_of(w.b, h, sTab), i, sCur, m7);
+    if (i < b.nSrc)
+        add_means(b, scratch_of(b, h, sTab), i, sCur, m7);
     if (!arrive_reduce<BLOCK, 7>(w, h, chunk, w.nChunk, m7, sLds, sRed, &sLast))
         return;
     if (threadIdx.x == 0) {
@@ -758,12 +776,13 @@ __global__ void __launch_bounds__(BLOCK) spread_means_kernel(SpreadArgs w) {
 template <int BLOCK>
 __global__ void __launch_bounds__(BLOCK) spread_eq_kernel(SpreadArgs w) {
     const int h = blockIdx.x / w.nChunk, chunk = blockIdx.x - h * w.nChunk;
+    const IcpArgs b = args_for(w.b, h);
     __shared__ float  sCur[12], sMeans[6];
     __shared__ int    sTab, sLast;
     __shared__ CF     sLds[(BLOCK / 64) * 27];
     __shared__ double sRed[27];
     IcpState &st = w.st[h];
-    if (!hypo_active(st, w.b.c))
+    if (!hypo_active(st, b.c))
         return;
     if (threadIdx.x < 12)
         sCur[threadIdx.x] = st.curF[threadIdx.x];
@@ -776,23 +795,24 @@ __global__ void __launch_bounds__(BLOCK) spread_eq_kernel(SpreadArgs w) {
     for (int k = 0; k < 27; k++)
         e27[k] = CF{0.0f, 0.0f};
     const int i = chunk * BLOCK + threadIdx.x;
-    if (i < w.b.nSrc)
-        add_eq(w.b, scratch_of(w.b, h, sTab), i, sCur, sMeans, w.b.mode, e27);
+    if (i < b.nSrc)
+        add_eq(b, scratch_of(b, h, sTab), i, sCur, sMeans, b.mode, e27);
     if (!arrive_reduce<BLOCK, 27>(w, h, chunk, w.nChunk, e27, sLds, sRed, &sLast))
         return;
     if (threadIdx.x == 0)
-        solve_update(sRed, sMeans, w.b.mode, st);
+        solve_update(sRed, sMeans, b.mode, st);
 }
 
 template <int BLOCK>
 __global__ void __launch_bounds__(BLOCK) spread_mse_kernel(SpreadArgs w) {
     const int h = blockIdx.x / w.nChunkG, chunk = blockIdx.x - h * w.nChunkG;
+    const IcpArgs b = args_for(w.b, h);
     __shared__ float  sCur[12], sU[12];
     __shared__ int    sTab, sLast;
     __shared__ CF     sLds[BLOCK / 64];
     __shared__ double sRed[1];
     IcpState &st = w.st[h];
-    if (!hypo_active(st, w.b.c))
+    if (!hypo_active(st, b.c))
         return;
     if (threadIdx.x < 12) {
         sCur[threadIdx.x] = st.curF[threadIdx.x];
@@ -803,17 +823,18 @@ __global__ void __launch_bounds__(BLOCK) spread_mse_kernel(SpreadArgs w) {
     __syncthreads();
     CF        m1[1] = {CF{0.0f, 0.0f}};
     const int i     = chunk * (BLOCK / ICP_G) + (int)threadIdx.x / ICP_G;
-    if (i < w.b.nSrc)
-        add_mse_group<ICP_G>(w.b, scratch_of(w.b, h, sTab), i, sCur, sU, m1);
+    if (i < b.nSrc)
+        add_mse_group<ICP_G>(b, scratch_of(b, h, sTab), i, sCur, sU, m1);
     if (!arrive_reduce<BLOCK, 1>(w, h, chunk, w.nChunkG, m1, sLds, sRed, &sLast))
         return;
     if (threadIdx.x == 0)
-        converge(st, (float)(sRed[0] / (double)st.valid), w.b.c);
+        converge(st, (float)(sRed[0] / (double)st.valid), b.c);
 }
 
 template <int BLOCK>
 __global__ void __launch_bounds__(BLOCK) spread_inlier_kernel(SpreadArgs w) {
     const int h = blockIdx.x / w.nChunk, chunk = blockIdx.x - h * w.nChunk;
+    const IcpArgs b = args_for(w.b, h);
     __shared__ float sPF[12];
     __shared__ int   sCnt[BLOCK / 64];
     const IcpState  &st = w.st[h];
@@ -821,8 +842,8 @@ __global__ void __launch_bounds__(BLOCK) spread_inlier_kernel(SpreadArgs w) {
         sPF[threadIdx.x] = (float)st.pose[threadIdx.x];
     __syncthreads();
     const int i = chunk * BLOCK + threadIdx.x;
-    int       c = (i < w.b.nSrc &&
-             inlier_point(w.b, i, sPF, w.b.scoringDist * w.b.scoringDist)) ? 1 : 0;
+    int       c = (i < b.nSrc &&
+             inlier_point(b, i, sPF, b.scoringDist * b.scoringDist)) ? 1 : 0;
     for (int o = 32; o > 0; o >>= 1)
         c += __shfl_xor(c, o, 64);
     if ((threadIdx.x & 63) == 0)
@@ -838,7 +859,7 @@ __global__ void __launch_bounds__(BLOCK) spread_inlier_kernel(SpreadArgs w) {
         sLast = (atomicAdd(&w.arrive[h], 1u) == (uint32_t)(w.nChunk - 1)) ? 1 : 0;
         if (sLast) {
             __threadfence();
-            write_result(w.b.out[h], st, atomicAdd(&w.inl[h], 0));
+            write_result(b.out[h], st, atomicAdd(&w.inl[h], 0));
             w.inl[h]    = 0;
             w.arrive[h] = 0u;
         }
@@ -1045,12 +1066,16 @@ static int norm_block(int b) {
 void icp_batch(Workspace &ws, int nSrc, const float *sx, const float *sy, const float *sz,
                const float *snx, const float *sny, const float *snz, const GridView &dst,
                const GridView &full, const float *init12, int H, const IcpCriteria &c,
-               float scoringDist, const IcpConfig &cfg, IcpResultDev *out, hipStream_t s) {
+               float scoringDist, const IcpConfig &cfg, IcpResultDev *out, hipStream_t s,
+               const GridView *dstG, const GridView *fullG, const int *hypScene) {
     if (H <= 0)
         return;
     const int block = norm_block(cfg.block);
     IcpArgs a = make_args(ws, nSrc, sx, sy, sz, snx, sny, snz, dst, full, H, c, scoringDist,
                           cfg.mode);
+    a.dstG     = dstG;
+    a.fullG    = fullG;
+    a.hypScene = hypScene;
     float *init = ws.icpPose.as<float>((size_t)H * 12);
     PPF_HIP_CHECK(hipMemcpyAsync(init, init12, sizeof(float) * 12 * H, hipMemcpyHostToDevice, s));
     a.init = init;

@@ -140,6 +140,26 @@ struct VoteConfig {
 // Vote every keypoint (indices into the sampled-scene grid's clouds),
 // extract peaks and build poses.  Returns the number of poses; the poses
 // are in (keypoint, rank) order in ws.poses.
+// Key points of one or several scenes for the vote: flat per-key-point
+// position / normal / point id, and (several scenes) each key point's scene
+// index into a device array of the scenes' vote grids.
+struct VoteKeypoints {
+    GridView        g;         // the grid when there is one scene
+    const GridView *gridsDev;  // device array, one per scene (nullptr: g)
+    const int      *kpScene;   // device, per key point (with gridsDev)
+    float           minCell;   // smallest grid cell (host check of the cell bound)
+    const float    *kx, *ky, *kz, *knx, *kny, *knz;
+    const int      *kp;        // point id within its scene
+    int             nk;
+};
+// position / normal of points kp[0..nk) of a cloud into flat arrays
+void gather_keypoints(const int *kp, int nk, const float *PX, const float *PY, const float *PZ,
+                      const float *NX, const float *NY, const float *NZ, float *kx, float *ky,
+                      float *kz, float *knx, float *kny, float *knz, hipStream_t s);
+int vote_keypoints(Workspace &ws, const DevModel &m, const VoteKeypoints &in, float voteThreshold,
+                   const VoteConfig &cfg, VoteStats *stats, int32_t *accDump, int dumpCount,
+                   hipStream_t s);
+
 int vote_and_poses(Workspace &ws, const DevModel &m, const GridView &g, const float *PX,
                    const float *PY, const float *PZ, const float *NX, const float *NY,
                    const float *NZ, const int *keypoints, int nk, float voteThreshold,
@@ -182,7 +202,11 @@ struct IcpConfig {
 void icp_batch(Workspace &ws, int nSrc, const float *sx, const float *sy, const float *sz,
                const float *snx, const float *sny, const float *snz, const GridView &dst,
                const GridView &full, const float *init12, int H, const IcpCriteria &c,
-               float scoringDist, const IcpConfig &cfg, IcpResultDev *out, hipStream_t s);
+               float scoringDist, const IcpConfig &cfg, IcpResultDev *out, hipStream_t s,
+               const GridView *dstG = nullptr, const GridView *fullG = nullptr,
+               const int *hypScene = nullptr);
+// (with dstG: hypothesis h refines against the device grids dstG[hypScene[h]]
+// and fullG[hypScene[h]]; dst / full then only name the target kind)
 // The conventional per-hypothesis GPU refiner (grid over source points per
 // step, host stop check per iteration), kept for comparison with icp_batch.
 void icp_plain(Workspace &ws, int nSrc, const float *sx, const float *sy, const float *sz,

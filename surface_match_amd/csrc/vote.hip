@@ -39,15 +39,20 @@ int vote_max_tile_refs(int angleNum, int block) {
 }
 
 struct VoteArgs {
-    GridView     g;  // sampled scene, sorted, with normals
-    const float *PX, *PY, *PZ, *NX, *NY, *NZ;
+    GridView        g;      // sampled scene, sorted, with normals (one scene)
+    const GridView *grids;  // per scene (several scenes; nullptr: g)
+    const int      *kpScene;  // scene of each key point (several scenes)
+    // key point k: position / normal, and its point id in its scene (self
+    // exclusion)
+    const float *kpX, *kpY, *kpZ, *kpNX, *kpNY, *kpNZ;
     const int   *kp;
     int          nk;
+    float        D;  // model diameter (vote radius)
     const uint4 *slots;
     uint32_t     tmask;
     const uint2 *ventries;      // bucket entries {ref x angleNum, alpha}, buckets 64 B aligned
     int          refNum, angleNum, tileRefs, nTiles;
-    float        angleStep, distStep, sMaxId, D2, searchR, voteThreshold;
+    float        angleStep, distStep, sMaxId, D2, voteThreshold;
     unsigned long long *cand;   // [nk * nTiles * 3]
     unsigned long long *stats;  // pairs, hits, votes (optional)
     int32_t     *accDump;       // [dumpCount * refNum * angleNum] (optional)
@@ -254,14 +259,15 @@ __device__ __forceinline__ int block_keypoint(const VoteArgs &a, int nTiles, int
 // candidates below VOTE_MAX_CHUNKS, the cell holding its first candidate.
 // Returns the candidate count; the caller synchronises before use.
 template <int BLOCK>
-__device__ int vote_cells(const VoteArgs &a, float qx, float qy, float qz, int *sCellStart,
-                          int *sCellPre, int *sChunkCell, int *sScan, int &nCells) {
-    constexpr int   CPT = (VOTE_MAX_CELLS + BLOCK - 1) / BLOCK;
-    const GridView &g   = a.g;
-    const int       tid = threadIdx.x;
-    int xlo = cell_coord(qx - a.searchR, g.ox, g.cell, g.dx), xhi = cell_coord(qx + a.searchR, g.ox, g.cell, g.dx);
-    int ylo = cell_coord(qy - a.searchR, g.oy, g.cell, g.dy), yhi = cell_coord(qy + a.searchR, g.oy, g.cell, g.dy);
-    int zlo = cell_coord(qz - a.searchR, g.oz, g.cell, g.dz), zhi = cell_coord(qz + a.searchR, g.oz, g.cell, g.dz);
+__device__ int vote_cells(const VoteArgs &a, const GridView &g, float qx, float qy, float qz,
+                          int *sCellStart, int *sCellPre, int *sChunkCell, int *sScan,
+                          int &nCells) {
+    constexpr int CPT = (VOTE_MAX_CELLS + BLOCK - 1) / BLOCK;
+    const int     tid = threadIdx.x;
+    const float   R   = a.D * 1.0001f + 1e-3f * g.cell;  // the sphere's cells, with a margin
+    int xlo = cell_coord(qx - R, g.ox, g.cell, g.dx), xhi = cell_coord(qx + R, g.ox, g.cell, g.dx);
+    int ylo = cell_coord(qy - R, g.oy, g.cell, g.dy), yhi = cell_coord(qy + R, g.oy, g.cell, g.dy);
+    int zlo = cell_coord(qz - R, g.oz, g.cell, g.dz), zhi = cell_coord(qz + R, g.oz, g.cell, g.dz);
     int ncx = xhi - xlo + 1, ncy = yhi - ylo + 1, ncz = zhi - zlo + 1;
     nCells  = ncx * ncy * ncz;  // <= VOTE_MAX_CELLS: the host bounds cells per axis
     int cStart[CPT], cCnt[CPT], mySum = 0;
@@ -379,7 +385,7 @@ __device__ void peaks_top3(const VoteArgs &a, const uint32_t *acc, int nrTile, i
 // ---- fused kernel: pairs and votes in one workgroup per key point (the
 // path for key points whose hit list overflows the split path's capacity,
 // and a tuning choice on its own)
-template <int BLOCK>
+template <int BLOCK, bool MULTI>
 __global__ void __launch_bounds__(BLOCK) vote_kernel(VoteArgs a) {
     constexpr int NW   = BLOCK / 64;
     constexpr int QCAP = 128;  // >= 63 queued + 64 new hits
@@ -409,8 +415,8 @@ __global__ void __launch_bounds__(BLOCK) vote_kernel(VoteArgs a) {
     unsigned long long *candOut = a.cand + ((size_t)k * a.nTiles + tile) * 3;
 
     const int   kpId = a.kp[k];
-    const float qx = a.PX[kpId], qy = a.PY[kpId], qz = a.PZ[kpId];
-    const float nx1 = a.NX[kpId], ny1 = a.NY[kpId], nz1 = a.NZ[kpId];
+    const float qx = a.kpX[k], qy = a.kpY[k], qz = a.kpZ[k];
+    const float nx1 = a.kpNX[k], ny1 = a.kpNY[k], nz1 = a.kpNZ[k];
     if (ppfc::is_nan(nx1) || ppfc::is_nan(ny1) || ppfc::is_nan(nz1)) {
         if (tid < 3)
             candOut[tid] = 0ull;
@@ -421,9 +427,9 @@ __global__ void __launch_bounds__(BLOCK) vote_kernel(VoteArgs a) {
 
     for (int i = tid; i < nrTile * a.angleNum; i += BLOCK)
         acc[i] = 0u;
-    const GridView &g = a.g;
+    const GridView &g = MULTI ? a.grids[a.kpScene[k]] : a.g;
     int             nCells;
-    const int total = vote_cells<BLOCK>(a, qx, qy, qz, sCellStart, sCellPre, sChunkCell, sScan,
+    const int total = vote_cells<BLOCK>(a, g, qx, qy, qz, sCellStart, sCellPre, sChunkCell, sScan,
                                         nCells);
     if (tid == 0)
         sNext = 0;
@@ -598,6 +604,7 @@ __global__ void __launch_bounds__(BLOCK) vote_kernel(VoteArgs a) {
 // nHit = -1 and is voted by the fused kernel instead.
 constexpr int PAIR_BLOCK = 256;
 
+template <bool MULTI>
 __global__ void __launch_bounds__(PAIR_BLOCK) pair_kernel(VoteArgs a) {
     constexpr int NW   = PAIR_BLOCK / 64;
     constexpr int QCAP = 128;
@@ -615,8 +622,8 @@ __global__ void __launch_bounds__(PAIR_BLOCK) pair_kernel(VoteArgs a) {
     if (k < 0)
         return;
     const int   kpId = a.kp[k];
-    const float qx = a.PX[kpId], qy = a.PY[kpId], qz = a.PZ[kpId];
-    const float nx1 = a.NX[kpId], ny1 = a.NY[kpId], nz1 = a.NZ[kpId];
+    const float qx = a.kpX[k], qy = a.kpY[k], qz = a.kpZ[k];
+    const float nx1 = a.kpNX[k], ny1 = a.kpNY[k], nz1 = a.kpNZ[k];
     if (ppfc::is_nan(nx1) || ppfc::is_nan(ny1) || ppfc::is_nan(nz1)) {
         if (tid == 0) {
             a.nHit[k] = 0;
@@ -626,9 +633,9 @@ __global__ void __launch_bounds__(PAIR_BLOCK) pair_kernel(VoteArgs a) {
     }
     float rt[12];
     ppfc::frame(qx, qy, qz, nx1, ny1, nz1, rt);
-    const GridView &g = a.g;
+    const GridView &g = MULTI ? a.grids[a.kpScene[k]] : a.g;
     int             nCells;
-    const int total = vote_cells<PAIR_BLOCK>(a, qx, qy, qz, sCellStart, sCellPre, sChunkCell,
+    const int total = vote_cells<PAIR_BLOCK>(a, g, qx, qy, qz, sCellStart, sCellPre, sChunkCell,
                                              sScan, nCells);
     if (tid == 0) {
         sNext = 0;
@@ -872,10 +879,10 @@ __global__ void __launch_bounds__(BLOCK) vote_hits_kernel(VoteArgs a) {
 
 // Merge tiles, apply the 0.95 filter (ppf.cpp:393-397) and build the poses
 // T = inv(Ts) * Rx(alpha) * Tm (ppf.cpp:399-405).  One thread per keypoint.
-__global__ void poses_kernel(const unsigned long long *cand, int nk, int nTiles, const int *kp,
-                             const float *PX, const float *PY, const float *PZ, const float *NX,
-                             const float *NY, const float *NZ, const float *frames,
-                             const float *cosSin, int angleNum, PoseRec *slots, int *counts) {
+__global__ void poses_kernel(const unsigned long long *cand, int nk, int nTiles, const float *KX,
+                             const float *KY, const float *KZ, const float *KNX, const float *KNY,
+                             const float *KNZ, const float *frames, const float *cosSin,
+                             int angleNum, PoseRec *slots, int *counts) {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nk)
         return;
@@ -888,9 +895,8 @@ __global__ void poses_kernel(const unsigned long long *cand, int nk, int nTiles,
     if (b0) {
         float bestVote = (float)(uint32_t)(b0 >> 32);
         float thr      = (float)((double)bestVote * 0.95);
-        int   id       = kp[k];
         float Ts[12];
-        ppfc::frame(PX[id], PY[id], PZ[id], NX[id], NY[id], NZ[id], Ts);
+        ppfc::frame(KX[k], KY[k], KZ[k], KNX[k], KNY[k], KNZ[k], Ts);
         for (int r = 0; r < 3; r++) {
             unsigned long long c = bs[r];
             if (!c)
@@ -931,9 +937,15 @@ __global__ void compact_poses_kernel(const PoseRec *slots, const int *counts, co
 
 template <int BLOCK>
 static void launch_vote(const VoteArgs &a, int blocks, size_t lds, hipStream_t s) {
-    PPF_HIP_CHECK(hipFuncSetAttribute((const void *)vote_kernel<BLOCK>,
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    vote_kernel<BLOCK><<<blocks, BLOCK, lds, s>>>(a);
+    if (a.grids) {
+        PPF_HIP_CHECK(hipFuncSetAttribute((const void *)vote_kernel<BLOCK, true>,
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        vote_kernel<BLOCK, true><<<blocks, BLOCK, lds, s>>>(a);
+    } else {
+        PPF_HIP_CHECK(hipFuncSetAttribute((const void *)vote_kernel<BLOCK, false>,
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        vote_kernel<BLOCK, false><<<blocks, BLOCK, lds, s>>>(a);
+    }
 }
 
 // split points of every bucket for tiles of tileRefs references: the first
@@ -966,6 +978,31 @@ static void launch_vote_hits(const VoteArgs &a, int blocks, size_t lds, hipStrea
     vote_hits_kernel<BLOCK><<<blocks, BLOCK, lds, s>>>(a);
 }
 
+// key point data (position, normal) gathered into flat arrays
+__global__ void kp_gather_kernel(const int *kp, int nk, const float *PX, const float *PY,
+                                 const float *PZ, const float *NX, const float *NY, const float *NZ,
+                                 float *kx, float *ky, float *kz, float *knx, float *kny,
+                                 float *knz) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nk)
+        return;
+    int i  = kp[k];
+    kx[k]  = PX[i];
+    ky[k]  = PY[i];
+    kz[k]  = PZ[i];
+    knx[k] = NX[i];
+    kny[k] = NY[i];
+    knz[k] = NZ[i];
+}
+
+void gather_keypoints(const int *kp, int nk, const float *PX, const float *PY, const float *PZ,
+                      const float *NX, const float *NY, const float *NZ, float *kx, float *ky,
+                      float *kz, float *knx, float *kny, float *knz, hipStream_t s) {
+    if (nk > 0)
+        kp_gather_kernel<<<div_up(nk, 256), 256, 0, s>>>(kp, nk, PX, PY, PZ, NX, NY, NZ, kx, ky,
+                                                         kz, knx, kny, knz);
+}
+
 int vote_and_poses(Workspace &ws, const DevModel &m, const GridView &g, const float *PX,
                    const float *PY, const float *PZ, const float *NX, const float *NY,
                    const float *NZ, const int *keypoints, int nk, float voteThreshold,
@@ -973,11 +1010,37 @@ int vote_and_poses(Workspace &ws, const DevModel &m, const GridView &g, const fl
                    hipStream_t s) {
     if (nk <= 0 || m.nSampled <= 0)
         return 0;
+    VoteKeypoints in;
+    in.g       = g;
+    in.gridsDev = nullptr;
+    in.kpScene = nullptr;
+    in.minCell = g.cell;
+    in.kx  = ws.kpX.as<float>(nk);
+    in.ky  = ws.kpY.as<float>(nk);
+    in.kz  = ws.kpZ.as<float>(nk);
+    in.knx = ws.kpNX.as<float>(nk);
+    in.kny = ws.kpNY.as<float>(nk);
+    in.knz = ws.kpNZ.as<float>(nk);
+    in.kp  = keypoints;
+    in.nk  = nk;
+    kp_gather_kernel<<<div_up(nk, 256), 256, 0, s>>>(keypoints, nk, PX, PY, PZ, NX, NY, NZ,
+                                                     ws.kpX.get<float>(), ws.kpY.get<float>(),
+                                                     ws.kpZ.get<float>(), ws.kpNX.get<float>(),
+                                                     ws.kpNY.get<float>(), ws.kpNZ.get<float>());
+    return vote_keypoints(ws, m, in, voteThreshold, cfg, stats, accDump, dumpCount, s);
+}
+
+int vote_keypoints(Workspace &ws, const DevModel &m, const VoteKeypoints &in, float voteThreshold,
+                   const VoteConfig &cfg, VoteStats *stats, int32_t *accDump, int dumpCount,
+                   hipStream_t s) {
+    const int nk = in.nk;
+    if (nk <= 0 || m.nSampled <= 0)
+        return 0;
     int tileRefs = std::min(m.nSampled, vote_max_tile_refs(m.angleNum, cfg.block));
     int nTiles   = std::max(div_up(m.nSampled, tileRefs), std::max(cfg.tiles, 1));
     tileRefs     = div_up(m.nSampled, nTiles);
     nTiles       = div_up(m.nSampled, tileRefs);
-    float cell   = g.cell;
+    float cell   = in.minCell;
     float D      = m.diameter;
     float searchR = D * 1.0001f + 1e-3f * cell;
     // cells per axis over [q - R, q + R]: floor(2R / cell) + 2, plus one for
@@ -985,15 +1048,18 @@ int vote_and_poses(Workspace &ws, const DevModel &m, const GridView &g, const fl
     if ((int)std::floor(2.0 * searchR / cell) + 3 > VOTE_AXIS_CELLS)
         throw std::runtime_error("vote grid cell is smaller than a quarter of the model diameter");
     VoteArgs a;
-    a.g  = g;
-    a.PX = PX;
-    a.PY = PY;
-    a.PZ = PZ;
-    a.NX = NX;
-    a.NY = NY;
-    a.NZ = NZ;
-    a.kp = keypoints;
-    a.nk = nk;
+    a.g       = in.g;
+    a.grids   = in.gridsDev;
+    a.kpScene = in.kpScene;
+    a.kpX  = in.kx;
+    a.kpY  = in.ky;
+    a.kpZ  = in.kz;
+    a.kpNX = in.knx;
+    a.kpNY = in.kny;
+    a.kpNZ = in.knz;
+    a.kp   = in.kp;
+    a.nk   = nk;
+    a.D    = D;
     a.slots    = m.slots.get<uint4>();
     a.tmask    = m.tmask;
     a.ventries = m.ventries.get<uint2>();
@@ -1005,7 +1071,6 @@ int vote_and_poses(Workspace &ws, const DevModel &m, const GridView &g, const fl
     a.distStep  = m.distStep;
     a.sMaxId    = (float)(m.angleNum - 1) / ppfc::kTwoPi;
     a.D2        = D * D;            // ppf.cpp:191
-    a.searchR   = searchR;
     a.voteThreshold = voteThreshold;
     a.cand  = ws.cand.as<unsigned long long>((size_t)nk * nTiles * 3);
     a.stats = nullptr;
@@ -1063,7 +1128,10 @@ int vote_and_poses(Workspace &ws, const DevModel &m, const GridView &g, const fl
         a.overflow  = a.nHit + 2 * (size_t)nk;
         a.nOverflow = a.nHit + 3 * (size_t)nk;
         PPF_HIP_CHECK(hipMemsetAsync(a.nOverflow, 0, sizeof(int), s));
-        pair_kernel<<<padded(nk), PAIR_BLOCK, 0, s>>>(a);
+        if (a.grids)
+            pair_kernel<true><<<padded(nk), PAIR_BLOCK, 0, s>>>(a);
+        else
+            pair_kernel<false><<<padded(nk), PAIR_BLOCK, 0, s>>>(a);
         PPF_HIP_CHECK(hipGetLastError());
         PPF_HIP_CHECK(hipEventRecord(ws.voteEv[2], s));
         switch (cfg.block) {
@@ -1091,8 +1159,8 @@ int vote_and_poses(Workspace &ws, const DevModel &m, const GridView &g, const fl
     PoseRec *pslots = reinterpret_cast<PoseRec *>(ws.candTile.as<PoseRec>((size_t)nk * 3));
     int     *counts = ws.voteCnt.as<int>((size_t)nk * 2 + 2);
     int     *offs   = counts + nk + 1;
-    poses_kernel<<<div_up(nk, 128), 128, 0, s>>>(a.cand, nk, nTiles, keypoints, PX, PY, PZ, NX,
-                                                  NY, NZ, m.frames.get<float>(),
+    poses_kernel<<<div_up(nk, 128), 128, 0, s>>>(a.cand, nk, nTiles, in.kx, in.ky, in.kz, in.knx,
+                                                  in.kny, in.knz, m.frames.get<float>(),
                                                   m.cosSin.get<float>(), m.angleNum, pslots,
                                                   counts);
     size_t tb = 0;

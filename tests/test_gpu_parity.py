@@ -324,3 +324,26 @@ def test_autotune_keeps_results(trained, workload1):
         print("autotune", rep)
     finally:
         trained.set_tuning(**base)
+
+
+def test_batched_scenes_equal_individual(trained):
+    """Several scenes matched together (one vote launch, one batched
+    refinement per stage) give every scene exactly its single-scene result."""
+    from surface_match_amd import PointCloud, MatchParam, ResidentScene, synth
+    scenes = synth.batch_scenes(0x5EED0003, n_scenes=4, lo=20000, hi=80000)
+    rs = []
+    for sc in scenes:
+        c = PointCloud(sc.points)
+        c.viewPoint = np.array(synth.VIEWPOINT, np.float32)
+        rs.append(ResidentScene(trained, c))
+    param = MatchParam(numMatches=3)
+    many, st = ResidentScene.match_many(rs, 0.04, 0.1, 0.2, param)
+    assert len(many) == len(rs)
+    total = 0
+    for r, (P, S) in zip(rs, many):
+        P1, S1, st1 = r.match(0.04, 0.1, 0.2, param)
+        assert np.array_equal(S, S1)
+        assert np.array_equal(P, P1)
+        total += st1["nKeypoints"]
+    assert st["nKeypoints"] == total
+    print("batched scenes", [len(S) for _, S in many], st["msTotal"])
