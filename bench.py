@@ -31,6 +31,21 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "vote_hits_kernel_traffic_matchScene_1M.md")
+
+
+def measured_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` on this workload, as measured by the
+    two rocprofv3 PMC passes (FETCH_SIZE, WRITE_SIZE, copy-calibrated) whose
+    summary is kept with the profiles; None when no such measurement is kept."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                if line.startswith(f"traffic_bytes_per_launch {kernel}:"):
+                    return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
 
 
 def vote_algorithmic_bytes(st: dict) -> float:
@@ -178,7 +193,10 @@ def scene_batching(seed=0x5EED0003, n_scenes=64):
             "batched_ms": round(wb * 1e3, 2), "one_by_one_ms": round(ws * 1e3, 2),
             "speedup": round(ws / wb, 2), "scenes_per_s": round(n_scenes / wb, 1),
             "results_identical": bool(same),
-            "instances_found_within_1deg_1pctD": found}
+            "instances_found_within_1deg_1pctD": found,
+            "batched_stage_ms": {k: round(float(st[k]), 3) for k in
+                                 ("msSample", "msVote", "msVoteKernel", "msCluster", "msIcp",
+                                  "msTotal")}}
 
 
 def main():
@@ -274,7 +292,8 @@ def main():
         "vote_throughput": {"keypoints_per_s": round(st["nKeypoints"] / (vote_ms * 1e-3)),
                             "votes_per_s": round(st["nVotes"] / (vote_ms * 1e-3))},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": measured_traffic(kname),
                      "kernel": kname, "algorithmic_bytes": alg, "kernel_ms": round(kms, 4),
                      "vote_stage": {"algorithmic_bytes": vote_algorithmic_bytes(st),
                                     "ms": round(vote_ms, 4),
