@@ -134,6 +134,15 @@ PPF_API int ppf_match_scene_resident(ppf_detector d, ppf_scene s, float sampling
                                      float keyPointFraction, float minScore,
                                      const ppf_match_param *param, float *poses16, float *scores,
                                      size_t cap, size_t *nOut, ppf_match_stats *stats);
+/* inliner (util.cpp:628-641) as matchScene scores a refined pose
+ * (ppf.cpp:456-459, 479-482): for each pose (column-major 4x4), the number
+ * of sampled (dense = 0) or resampled (dense = 1) model points whose
+ * transformed position has a valid point of the scene within dist (dist2 <
+ * dist^2).  The scene is taken as the last match left it (points that got a
+ * fresh normal sit at their neighbourhood centroid, estimateNormal's
+ * write-back), so this scores poses exactly as that match did. */
+PPF_API int ppf_scene_inliers(ppf_detector d, ppf_scene s, const float *poses16, size_t nPoses,
+                              int dense, float dist, int *inliers);
 /* -- several resident scenes in one pass (no reference counterpart: the
  * reference matches one scene per call).  Each scene is sampled and given
  * normals on its own; the key points of all scenes are voted in one launch
@@ -151,7 +160,8 @@ PPF_API int ppf_match_scenes_resident(ppf_detector d, const ppf_scene *scenes, s
 /* -- launch tuning (no reference counterpart).  ppf_autotune times the
  * vote stage over {split, fused} kernels, then workgroup sizes {256, 512,
  * 1024}, then the vote-grid cell {D/2, D/3, D/4}, then reference tiles
- * {1, 2, 3, 4} (one knob at a time), then the batched ICP
+ * {1, 2, 3, 4}, then the key point order {Morton, plain}, then the XCD
+ * mapping {round-robin, contiguous} (one knob at a time), then the batched ICP
  * over {spread, persistent} x {64, 128, 256, 512, 1024} threads per
  * workgroup, then the normals grid cell over {1/4, 1/2, 1} sample step, on
  * the given resident scene, and keeps the fastest in the detector.  None of
@@ -168,6 +178,8 @@ typedef struct {
                          with more are voted by the fused kernel */
     int voteTiles;    /* at least this many reference tiles per key point (1 .. 16): smaller
                          LDS accumulators, more workgroups per CU */
+    int voteOrder;    /* 1: workgroups take key points in (scene, Morton code of position)
+                         order, 0: in key point order */
 } ppf_tuning;
 typedef struct {
     ppf_tuning best;

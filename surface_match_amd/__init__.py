@@ -235,6 +235,16 @@ class ResidentScene:
         k = nout.value
         return poses[:k].reshape(k, 4, 4).transpose(0, 2, 1).copy(), scores[:k].copy(), st.as_dict()
 
+    def inliers(self, poses, dense: bool, dist: float) -> np.ndarray:
+        """inliner (util.cpp:628-641) of the sampled (dense=False) or
+        resampled (dense=True) model at each pose (k x 4 x 4, row-major)
+        against this scene as the last match left it."""
+        P = np.ascontiguousarray(np.asarray(poses, np.float32).reshape(-1, 4, 4).transpose(0, 2, 1))
+        out = np.zeros(len(P), np.int32)
+        N.check(N.lib().ppf_scene_inliers(self._det._h, self._h, N.ptr(P), len(P), int(dense),
+                                          dist, N.ptr(out)))
+        return out
+
     @staticmethod
     def match_many(scenes, samplingDistanceRel=0.04, keyPointFraction=0.2, minScore=0.2,
                    param=None):

@@ -74,7 +74,7 @@ class IcpResult(C.Structure):
 class Tuning(C.Structure):
     _fields_ = [(n, C.c_int) for n in ("voteBlock", "voteXcdRemap", "voteCellDiv", "icpBlock",
                                        "icpSpread", "normCell4", "voteSplit",
-                                       "voteHitCap", "voteTiles")]
+                                       "voteHitCap", "voteTiles", "voteOrder")]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -97,6 +97,7 @@ EXPORTS = [
     "ppf_radius_search", "ppf_knn_search", "ppf_estimate_normals", "ppf_device_math",
     "ppf_icp_target_create", "ppf_icp_target_destroy", "ppf_icp_refine", "ppf_icp_adjoint",
     "ppf_autotune", "ppf_set_tuning", "ppf_get_tuning", "ppf_match_scenes_resident",
+    "ppf_scene_inliers",
 ]
 
 
@@ -145,6 +146,7 @@ def lib() -> C.CDLL:
         "ppf_get_tuning": (i, [vp, C.POINTER(Tuning)]),
         "ppf_match_scenes_resident": (i, [vp, vp, sz, f, f, f, C.POINTER(MatchParam), vp, vp, sz,
                                           vp, C.POINTER(MatchStats)]),
+        "ppf_scene_inliers": (i, [vp, vp, vp, sz, i, f, vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

@@ -749,6 +749,7 @@ void match_batch_impl(ppf_detector_s &D, const std::vector<DevCloud *> &clouds, 
         in.knz = knz;
         in.kp  = kpId;
         in.nk  = K;
+        in.nScenes = n;
         np = vote_keypoints(D.ws, dm, in, voteThreshold, D.vcfg, &vs, nullptr, 0, s);
         // poses are compacted in key point order: scene i's are those of its key points
         std::vector<int> offs(K + 1);
@@ -1361,6 +1362,40 @@ PPF_API int ppf_match_scene_resident(ppf_detector d, ppf_scene sc, float rel, fl
     });
 }
 
+static void pose16_to_12(const float *p16, float *p12) {
+    for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 4; c++)
+            p12[r * 4 + c] = p16[c * 4 + r];
+}
+
+PPF_API int ppf_scene_inliers(ppf_detector d, ppf_scene sc, const float *poses16, size_t nPoses,
+                              int dense, float dist, int *inliers) {
+    PPF_REQUIRE(d && sc && (poses16 || nPoses == 0) && (inliers || nPoses == 0),
+                "null argument");
+    return guarded([&] {
+        if (!d->trained)
+            throw std::runtime_error("No trained model in matchScene");
+        DevCloud   &c  = sc->cloud;
+        hipStream_t s  = d->stream;
+        DevModel   &dm = d->dm;
+        const float Dm = box_diameter(d->hm.sampled.boxMin, d->hm.sampled.boxMax);
+        grid_build(d->ws.gFull, c.px.get<float>(), c.py.get<float>(), c.pz.get<float>(), nullptr,
+                   nullptr, nullptr, c.valid.get<int>(), c.nValid,
+                   std::max(dist * 2.0f, Dm * 1e-3f), s);
+        std::vector<float> P(12 * nPoses);
+        for (size_t h = 0; h < nPoses; h++)
+            pose16_to_12(poses16 + 16 * h, &P[12 * h]);
+        if (dense)
+            icp_inliers(d->ws, d->ws.gFull.view, dm.nReSampled, dm.rx.get<float>(),
+                        dm.ry.get<float>(), dm.rz.get<float>(), P.data(), (int)nPoses, dist,
+                        inliers, s);
+        else
+            icp_inliers(d->ws, d->ws.gFull.view, dm.nSampled, dm.sx.get<float>(),
+                        dm.sy.get<float>(), dm.sz.get<float>(), P.data(), (int)nPoses, dist,
+                        inliers, s);
+    });
+}
+
 PPF_API int ppf_match_scenes_resident(ppf_detector d, const ppf_scene *scenes, size_t nScenes,
                                       float rel, float kpFrac, float minScore,
                                       const ppf_match_param *param, float *poses16, float *scores,
@@ -1398,6 +1433,7 @@ ppf_tuning tuning_of(const ppf_detector_s &d) {
     t.voteSplit    = d.vcfg.split;
     t.voteHitCap   = d.vcfg.hitCap;
     t.voteTiles    = d.vcfg.tiles;
+    t.voteOrder    = d.vcfg.order;
     return t;
 }
 }  // namespace
@@ -1430,6 +1466,7 @@ PPF_API int ppf_set_tuning(ppf_detector d, const ppf_tuning *t) {
     d->vcfg.split    = t->voteSplit ? 1 : 0;
     d->vcfg.hitCap   = t->voteHitCap;
     d->vcfg.tiles    = t->voteTiles;
+    d->vcfg.order    = t->voteOrder ? 1 : 0;
     return PPF_OK;
 }
 
@@ -1474,6 +1511,8 @@ PPF_API int ppf_autotune(ppf_detector d, ppf_scene sc, float rel, float kpFrac, 
         tryVote(&VoteConfig::block, {256, 512, 1024});
         tryVote(&VoteConfig::cellDiv, {2, 3, 4});
         tryVote(&VoteConfig::tiles, {1, 2, 3, 4});
+        tryVote(&VoteConfig::order, {1, 0});
+        tryVote(&VoteConfig::xcdRemap, {0, 1});
         d->vcfg = bestV;
         for (int spread : {1, 0})
             for (int block : {64, 128, 256, 512, 1024}) {
@@ -1778,12 +1817,6 @@ PPF_API int ppf_estimate_normals(float *pts, float *nrm, size_t n, const int *va
         }
         PPF_HIP_CHECK(hipStreamSynchronize(s));
     });
-}
-
-static void pose16_to_12(const float *p16, float *p12) {
-    for (int r = 0; r < 3; r++)
-        for (int c = 0; c < 4; c++)
-            p12[r * 4 + c] = p16[c * 4 + r];
 }
 
 PPF_API int ppf_icp_target_create(const float *pts, const float *nrm, size_t n, float cell,

@@ -1187,6 +1187,55 @@ void icp_plain(Workspace &ws, int nSrc, const float *sx, const float *sy, const 
     }
 }
 
+// ---------------------------------------------------------------- scoring
+
+// The refiner's scoring step (inlier_point) for given poses: blockIdx.y is
+// the pose, x the chunk of source points.
+__global__ void __launch_bounds__(256) inliers_kernel(GridView full, const float *sx,
+                                                      const float *sy, const float *sz, int nSrc,
+                                                      const float *poses12, float r2, int *out) {
+    const int   h = blockIdx.y;
+    __shared__ float sPF[12];
+    __shared__ int   sCnt[4];
+    if (threadIdx.x < 12)
+        sPF[threadIdx.x] = poses12[12 * h + threadIdx.x];
+    __syncthreads();
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    int       c = 0;
+    if (i < nSrc) {
+        float px, py, pz;
+        xform(sPF, sx[i], sy[i], sz[i], px, py, pz);
+        c = grid_any_within(full, px, py, pz, r2) ? 1 : 0;
+    }
+    for (int o = 32; o > 0; o >>= 1)
+        c += __shfl_xor(c, o, 64);
+    if ((threadIdx.x & 63) == 0)
+        sCnt[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        atomicAdd(&out[h], sCnt[0] + sCnt[1] + sCnt[2] + sCnt[3]);
+}
+
+void icp_inliers(Workspace &ws, const GridView &full, int nSrc, const float *sx, const float *sy,
+                 const float *sz, const float *poses12Host, int H, float dist, int *inliersHost,
+                 hipStream_t s) {
+    if (H <= 0)
+        return;
+    if (H > 65535)
+        throw std::runtime_error("at most 65535 poses per scoring call");
+    float *P   = ws.icpPose.as<float>((size_t)H * 12);
+    int   *cnt = ws.icpInl.as<int>(H);
+    PPF_HIP_CHECK(hipMemcpyAsync(P, poses12Host, sizeof(float) * 12 * H, hipMemcpyHostToDevice, s));
+    PPF_HIP_CHECK(hipMemsetAsync(cnt, 0, sizeof(int) * H, s));
+    if (nSrc > 0) {
+        dim3 grid(div_up(nSrc, 256), H);
+        inliers_kernel<<<grid, 256, 0, s>>>(full, sx, sy, sz, nSrc, P, dist * dist, cnt);
+        PPF_HIP_CHECK(hipGetLastError());
+    }
+    PPF_HIP_CHECK(hipMemcpyAsync(inliersHost, cnt, sizeof(int) * H, hipMemcpyDeviceToHost, s));
+    PPF_HIP_CHECK(hipStreamSynchronize(s));
+}
+
 // ---------------------------------------------------------------- adjoint
 //
 // Point-to-plane cost E = 1/2 sum_i r_i^2, r_i = n_q.(x_i - q_i), over the

@@ -25,7 +25,8 @@ struct Workspace {
     Grid   gNormal, gVote, gSparse, gDense, gFull;
     // vote
     DevBuf kpX, kpY, kpZ, kpNX, kpNY, kpNZ, cand, candTile, poses, voteStats, accDump, voteCnt,
-        smallCnt, hitStart, hitLen, hitAlpha, hitBucket, hitCount, splits;
+        smallCnt, hitStart, hitLen, hitAlpha, hitBucket, hitCount, splits, kpKey, kpKey2, kpOrd,
+        kpOrd2;
     // the bucket split points in `splits` were computed for this model
     // generation, tile size and tile count
     unsigned splitGen = ~0u;
@@ -135,6 +136,7 @@ struct VoteConfig {
     int split    = 1;     // 1: pair kernel + hit voting kernel; 0: fused kernel
     int hitCap   = 2048;  // hits kept per key point on the split path
     int tiles    = 1;     // at least this many reference tiles per key point
+    int order    = 1;     // visit key points in (scene, Morton code of position) order
 };
 
 // Vote every keypoint (indices into the sampled-scene grid's clouds),
@@ -151,6 +153,7 @@ struct VoteKeypoints {
     const float    *kx, *ky, *kz, *knx, *kny, *knz;
     const int      *kp;        // point id within its scene
     int             nk;
+    int             nScenes = 1;
 };
 // position / normal of points kp[0..nk) of a cloud into flat arrays
 void gather_keypoints(const int *kp, int nk, const float *PX, const float *PY, const float *PZ,
@@ -218,6 +221,12 @@ void icp_plain(Workspace &ws, int nSrc, const float *sx, const float *sy, const 
 // (T = U(xi) T0) against dst, with the refiner's correspondence rule.  out:
 // H x 36 doubles {cost, count, centre[3], grad[6], JtJ[21], 0 x4}; corr
 // (optional): H x nSrc dst ids of the winners (-1 = none).
+// inliner (util.cpp:628-641) at given poses: per pose (H x 12 host, row-major
+// 3x4) the count of src points whose transformed position has a point of
+// `full` within dist (dist2 < dist^2) -- the refiner's scoring step alone.
+void icp_inliers(Workspace &ws, const GridView &full, int nSrc, const float *sx, const float *sy,
+                 const float *sz, const float *poses12Host, int H, float dist, int *inliersHost,
+                 hipStream_t s);
 void icp_adjoint(Workspace &ws, int nSrc, const float *sx, const float *sy, const float *sz,
                  const GridView &dst, const float *T0host, const double *xiHost, int H,
                  float rejectDist, int center, double *outHost, int *corrHost, hipStream_t s);

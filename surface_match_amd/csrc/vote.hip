@@ -995,6 +995,43 @@ __global__ void kp_gather_kernel(const int *kp, int nk, const float *PX, const f
     knz[k] = NZ[i];
 }
 
+__device__ __forceinline__ uint32_t morton_spread10(uint32_t v) {
+    v &= 0x3FFu;
+    v = (v | (v << 16)) & 0x030000FFu;
+    v = (v | (v << 8)) & 0x0300F00Fu;
+    v = (v | (v << 4)) & 0x030C30C3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+
+// Visiting order of the key points: sort key = scene index (high bits), then
+// the Morton code of the position on a 1024^3 lattice over the scene's vote
+// grid.  Workgroups in flight then hold spatially close key points, whose
+// vote spheres cover the same scene cells, so the candidates' positions and
+// normals are reused from L2 instead of refetched.  The order only permutes
+// which workgroup takes which key point; every output stays indexed by key
+// point.
+__global__ void kp_order_kernel(const float *KX, const float *KY, const float *KZ,
+                                const int *kpScene, const GridView *grids, GridView g, int nk,
+                                unsigned long long *keys, int *idx) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nk)
+        return;
+    const int       sc  = kpScene ? kpScene[k] : 0;
+    const GridView &v   = grids ? grids[sc] : g;
+    const float     ext = v.cell * (float)max(v.dx, max(v.dy, v.dz));
+    const float     inv = 1024.0f / ext;
+    auto q = [&](float x, float o) {
+        float f = floorf((x - o) * inv);
+        f       = fminf(fmaxf(f, 0.0f), 1023.0f);
+        return (uint32_t)f;
+    };
+    const uint32_t m = morton_spread10(q(KX[k], v.ox)) | (morton_spread10(q(KY[k], v.oy)) << 1) |
+                       (morton_spread10(q(KZ[k], v.oz)) << 2);
+    keys[k] = ((unsigned long long)sc << 30) | m;
+    idx[k]  = k;
+}
+
 void gather_keypoints(const int *kp, int nk, const float *PX, const float *PY, const float *PZ,
                       const float *NX, const float *NY, const float *NZ, float *kx, float *ky,
                       float *kz, float *knx, float *kny, float *knz, hipStream_t s) {
@@ -1101,6 +1138,25 @@ int vote_keypoints(Workspace &ws, const DevModel &m, const VoteKeypoints &in, fl
             default: launch_vote<1024>(x, nb, lds, s); break;
         }
     };
+    const int *order = nullptr;
+    if (cfg.order && nk > 1) {
+        unsigned long long *k0 = ws.kpKey.as<unsigned long long>(nk);
+        unsigned long long *k1 = ws.kpKey2.as<unsigned long long>(nk);
+        int                *i0 = ws.kpOrd.as<int>(nk), *i1 = ws.kpOrd2.as<int>(nk);
+        kp_order_kernel<<<div_up(nk, 256), 256, 0, s>>>(in.kx, in.ky, in.kz, in.kpScene,
+                                                        in.gridsDev, in.g, nk, k0, i0);
+        PPF_HIP_CHECK(hipGetLastError());
+        int endBit = 30;
+        while ((1 << (endBit - 30)) < std::max(in.nScenes, 1))
+            endBit++;
+        size_t tb = 0;
+        PPF_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, k0, k1, i0, i1, nk, 0,
+                                                         endBit, s));
+        PPF_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(ws.cubTmp.reserve(tb), tb, k0, k1, i0, i1,
+                                                         nk, 0, endBit, s));
+        order = i1;
+    }
+    a.kpSel = order;
     int nOverflow = 0;
     PPF_HIP_CHECK(hipEventRecord(ws.voteEv[0], s));
     if (cfg.split) {

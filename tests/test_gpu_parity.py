@@ -176,18 +176,19 @@ def test_l4_vote_accumulators_bitwise(trained, oracle_mod, workload1):
     try:
         # split and fused kernels; a hit capacity of 40 sends part of the key
         # points through the fused fallback
-        for vb, remap, div, split, cap, tiles in ((1024, 0, 4, 1, 2048, 1), (256, 1, 2, 0, 2048, 2),
-                                                  (512, 0, 3, 1, 40, 3), (1024, 1, 2, 0, 2048, 1),
-                                                  (256, 0, 4, 1, 2048, 4), (512, 0, 3, 1, 2048, 2)):
+        for vb, remap, div, split, cap, tiles, order in (
+                (1024, 0, 4, 1, 2048, 1, 0), (256, 1, 2, 0, 2048, 2, 1), (512, 0, 3, 1, 40, 3, 1),
+                (1024, 1, 2, 0, 2048, 1, 0), (256, 0, 4, 1, 2048, 4, 1), (512, 1, 3, 1, 2048, 2, 1),
+                (512, 1, 3, 1, 40, 1, 1)):
             trained.set_tuning(voteBlock=vb, voteXcdRemap=remap, voteCellDiv=div,
-                               voteSplit=split, voteHitCap=cap, voteTiles=tiles)
+                               voteSplit=split, voteHitCap=cap, voteTiles=tiles, voteOrder=order)
             cand, poses, acc = trained.vote(pts, nrm, kps, minScore=0.2, dump=len(kps))
             for t, k in enumerate(kps):
                 if expect[t] is None:
                     assert (cand[t] == -1).all()
                     continue
                 oacc, pk = expect[t]
-                assert np.array_equal(acc[t], oacc[:, 1:]), f"keypoint {k} tuning {vb, remap, div, split, cap, tiles}"
+                assert np.array_equal(acc[t], oacc[:, 1:]), f"keypoint {k} tuning {vb, remap, div, split, cap, tiles, order}"
                 assert np.array_equal(acc[t].sum(1), oacc[:, 0])
                 got = [tuple(c) for c in cand[t] if c[0] >= 0]
                 assert got == [(a, b, int(v)) for a, b, v in pk], f"keypoint {k}"
@@ -310,7 +311,7 @@ def test_autotune_keeps_results(trained, workload1):
     try:
         p0, s0, st0 = rs.match(0.04, 0.1, 0.2, MatchParam(numMatches=3))
         rep = rs.autotune(0.04, 0.1, 0.2, MatchParam(numMatches=3))
-        assert rep["configsTried"] == (2 + 3 + 3 + 4) + 2 * 5 + 3
+        assert rep["configsTried"] == (2 + 3 + 3 + 4 + 2 + 2) + 2 * 5 + 3
         assert trained.get_tuning() == rep["best"]
         p1, s1, st1 = rs.match(0.04, 0.1, 0.2, MatchParam(numMatches=3))
         for k in ("nPairs", "nKeyHits", "nVotes", "nPoses", "nHypotheses"):
