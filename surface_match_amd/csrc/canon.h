@@ -218,6 +218,20 @@ PPF_HD int bin_tail(float alphaModel, float offTail, float sMaxId, int angleNum)
     return bin_of(a, sMaxId, angleNum);
 }
 
+// bin_body and bin_tail in one branch-free form, given the rule's offset:
+// with |alphaModel| <= pi and the offset in [-2pi, 0], a = alphaModel - off
+// lies in [-pi, 3pi], so at most one of the two wraps applies (a > 2pi
+// leaves a - 2pi > 0; a < 0 gives a + 2pi <= 2pi) and the rules' different
+// wrap orders agree.  After the wrap a >= 0, where truncation is floor;
+// non-finite inputs clamp as bin_of clamps them.
+PPF_HD int bin_wrapped(float alphaModel, float off, float sMaxId, int angleNum) {
+    float a = alphaModel - off;
+    a       = a > kTwoPi ? a - kTwoPi : (a < 0.0f ? a + kTwoPi : a);
+    int bin = (int)(sMaxId * a);
+    bin     = bin < 0 ? 0 : bin;
+    return bin > angleNum - 1 ? angleNum - 1 : bin;
+}
+
 // angle bin of a bucket entry (ppf.cpp:316-340); tail = the entry is among
 // the bucket's last size%8.
 PPF_HD int angle_bin(float alphaModel, float alphaScene, float sMaxId, int angleNum, bool tail) {

@@ -72,18 +72,23 @@ struct GridView {
     const float    *nx, *ny, *nz;  // optional (nullptr)
     const int      *id;            // original point id of each sorted slot
     int             n;
-    // occupancy of coarse cells (4 x 4 x 4 cells each), one bit per coarse
-    // cell, x fastest: lets searches skip empty space without table probes
+    // occupancy of blocks of 2^occShift cells per axis (occShift 0: of every
+    // cell), one bit per block, x fastest: lets searches skip empty space
+    // without table probes
     const uint32_t *occ;
-    int             odx, ody;
+    int             odx, ody, occShift;
+    int             unused;  // no padding bytes: views are compared bytewise as graph keys
 };
 
-constexpr int OCC_SHIFT = 2;  // coarse cell = 2^2 cells per axis
+// occupancy bitmaps up to this many bits use one bit per cell; larger grids
+// one bit per 2x2x2 or 4x4x4 block
+constexpr size_t OCC_MAX_BITS = size_t(1) << 26;
 
 __device__ __forceinline__ bool coarse_empty(const GridView &g, int cx, int cy, int cz) {
-    const uint32_t b = ((uint32_t)(cz >> OCC_SHIFT) * (uint32_t)g.ody + (uint32_t)(cy >> OCC_SHIFT)) *
+    const int      sh = g.occShift;
+    const uint32_t b  = ((uint32_t)(cz >> sh) * (uint32_t)g.ody + (uint32_t)(cy >> sh)) *
                            (uint32_t)g.odx +
-                       (uint32_t)(cx >> OCC_SHIFT);
+                       (uint32_t)(cx >> sh);
     return ((g.occ[b >> 5] >> (b & 31u)) & 1u) == 0u;
 }
 
@@ -197,8 +202,8 @@ __device__ __forceinline__ void visit_ring(const GridView &g, int c0x, int c0y, 
         const bool face = (cz == c0z - r) || (cz == c0z + r) || (cy == c0y - r) || (cy == c0y + r);
         // on a row off the z/y faces only the cells x = c0x -+ r are on the shell
         const bool on = face || cx == c0x - r || cx == c0x + r;
-        if (on && !coarse_empty(g, cx, cy, cz) &&
-            !(cell_min_d2(g, cx, cy, cz, qx, qy, qz) > bound2)) {
+        if (on && !(cell_min_d2(g, cx, cy, cz, qx, qy, qz) > bound2) &&
+            !coarse_empty(g, cx, cy, cz)) {
             int s0, c0;
             if (cell_lookup(g, cx, cy, cz, s0, c0))
                 f(s0, c0);

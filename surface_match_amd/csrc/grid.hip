@@ -103,16 +103,15 @@ __global__ void grid_keys_kernel(const float *X, const float *Y, const float *Z,
 
 // coarse occupancy bit of every occupied cell (first slot of each cell run)
 __global__ void grid_occ_kernel(const uint32_t *keys, int m, int dx, int dy, int odx, int ody,
-                                uint32_t *occ) {
+                                int sh, uint32_t *occ) {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= m || (s > 0 && keys[s] == keys[s - 1]))
         return;
     uint32_t k  = keys[s];
     int      cx = (int)(k % (uint32_t)dx), cy = (int)((k / (uint32_t)dx) % (uint32_t)dy),
         cz      = (int)(k / ((uint32_t)dx * (uint32_t)dy));
-    uint32_t b  = ((uint32_t)(cz >> OCC_SHIFT) * (uint32_t)ody + (uint32_t)(cy >> OCC_SHIFT)) *
-                     (uint32_t)odx +
-                 (uint32_t)(cx >> OCC_SHIFT);
+    uint32_t b  = ((uint32_t)(cz >> sh) * (uint32_t)ody + (uint32_t)(cy >> sh)) * (uint32_t)odx +
+                 (uint32_t)(cx >> sh);
     atomicOr(&occ[b >> 5], 1u << (b & 31u));
 }
 
@@ -194,8 +193,12 @@ void grid_build(Grid &g, const float *X, const float *Y, const float *Z, const f
     float    *y     = g.y.as<float>(std::max(m, 1));
     float    *z     = g.z.as<float>(std::max(m, 1));
     PPF_HIP_CHECK(hipMemsetAsync(htab, 0, sizeof(uint4) * hsize, s));
-    const int odx = ((dims[0] - 1) >> OCC_SHIFT) + 1, ody = ((dims[1] - 1) >> OCC_SHIFT) + 1,
-              odz = ((dims[2] - 1) >> OCC_SHIFT) + 1;
+    int sh = 0;
+    while (sh < 2 && ((size_t)(((dims[0] - 1) >> sh) + 1) * (size_t)(((dims[1] - 1) >> sh) + 1) *
+                      (size_t)(((dims[2] - 1) >> sh) + 1)) > OCC_MAX_BITS)
+        sh++;
+    const int odx = ((dims[0] - 1) >> sh) + 1, ody = ((dims[1] - 1) >> sh) + 1,
+              odz = ((dims[2] - 1) >> sh) + 1;
     const size_t occWords = ((size_t)odx * ody * odz + 31) / 32;
     uint32_t    *occ      = g.occ.as<uint32_t>(occWords);
     PPF_HIP_CHECK(hipMemsetAsync(occ, 0, sizeof(uint32_t) * occWords, s));
@@ -213,7 +216,8 @@ void grid_build(Grid &g, const float *X, const float *Y, const float *Z, const f
         PPF_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(tmp, tb, keys, keys2, vals, vals2, m, 0,
                                                          endBit, s));
         grid_insert_kernel<<<div_up(m, 256), 256, 0, s>>>(keys2, m, htab, hsize - 1);
-        grid_occ_kernel<<<div_up(m, 256), 256, 0, s>>>(keys2, m, dims[0], dims[1], odx, ody, occ);
+        grid_occ_kernel<<<div_up(m, 256), 256, 0, s>>>(keys2, m, dims[0], dims[1], odx, ody, sh,
+                                                        occ);
         grid_count_kernel<<<div_up(m, 256), 256, 0, s>>>(keys2, m, htab, hsize - 1);
         gather3_kernel<<<div_up(m, 256), 256, 0, s>>>(vals2, m, X, Y, Z, x, y, z);
     }
@@ -235,6 +239,7 @@ void grid_build(Grid &g, const float *X, const float *Y, const float *Z, const f
     v.occ = occ;
     v.odx = odx;
     v.ody = ody;
+    v.occShift = sh;
     v.nx = v.ny = v.nz = nullptr;
     if (NX && m > 0) {
         float *nx = g.nx.as<float>(m), *ny = g.ny.as<float>(m), *nz = g.nz.as<float>(m);
@@ -472,29 +477,56 @@ __device__ void store_estimate(const NormArgs &a, int id, int want, const KnnLis
     }
 }
 
-// queries with a finite normal keep it; the others are marked for estimation
-__global__ void normals_mark_kernel(NormArgs a) {
+// queries with a finite normal keep it; the others are marked for
+// estimation and counted
+__global__ void normals_mark_kernel(NormArgs a, int *nMarked) {
+    int  t    = blockIdx.x * blockDim.x + threadIdx.x;
+    bool mark = false;
+    if (t < a.nq) {
+        int id     = a.query[t];
+        a.qidx[id] = t;
+        if (finite3(a.N0x[id], a.N0y[id], a.N0z[id]))
+            a.cntq[t] = 0;
+        else {
+            a.mark[id] = 1;
+            mark       = true;
+        }
+    }
+    const unsigned long long m = __ballot(mark);
+    if ((threadIdx.x & 63) == 0 && m)
+        atomicAdd(nMarked, __popcll(m));
+}
+
+// The grid's points (in cell order) whose mark equals want: estimating them
+// in this order lets neighbouring threads search the same cells.  With
+// minCount > 0 they are split by the population of their own cell: points
+// in cells of fewer than minCount points (sparse regions, whose k nearest
+// neighbours lie several cells away) go to the second list, searched on a
+// coarser grid.  Either grid gives the same exact neighbours.
+__global__ void normals_order_flags_kernel(GridView g, int n, const int *mark, int want,
+                                           int minCount, char *fineF, char *coarseF) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= a.nq)
+    if (t >= n)
         return;
-    int id     = a.query[t];
-    a.qidx[id] = t;
-    if (finite3(a.N0x[id], a.N0y[id], a.N0z[id]))
-        a.cntq[t] = 0;
-    else
-        a.mark[id] = 1;
+    const bool m    = mark[g.id[t]] == want;
+    bool       fine = true;
+    if (m && minCount > 0) {
+        const int cx = cell_coord(g.x[t], g.ox, g.cell, g.dx);
+        const int cy = cell_coord(g.y[t], g.oy, g.cell, g.dy);
+        const int cz = cell_coord(g.z[t], g.oz, g.cell, g.dz);
+        int       s0 = 0, c0 = 0;
+        fine = cell_lookup(g, cx, cy, cz, s0, c0) && c0 >= minCount;
+    }
+    fineF[t] = (m && fine) ? 1 : 0;
+    if (coarseF)
+        coarseF[t] = (m && !fine) ? 1 : 0;
 }
 
 // neighbours that the smoothing needs and that have no estimate yet
-__global__ void normals_touch_mark_kernel(NormArgs a, int n, char *flags, int *iota) {
+__global__ void normals_touch_mark_kernel(NormArgs a, int n) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n)
-        return;
-    const bool need = a.touch[i] && !a.have[i];
-    if (need)
+    if (i < n && a.touch[i] && !a.have[i])
         a.mark[i] = 2;
-    flags[i] = need ? 1 : 0;
-    iota[i]  = i;
 }
 
 // Estimation of one point by the general grid kNN (ring expansion)
@@ -606,7 +638,8 @@ __global__ void normals_orient_kernel(NormArgs a, const float *outN, float *PX, 
 
 void estimate_normals(Workspace &ws, int n, float *PX, float *PY, float *PZ, float *NX, float *NY,
                       float *NZ, const int *valid, int nValid, const int *query, int nq, int k,
-                      int smooth, int invert, const float vp[3], float cellHint, hipStream_t s) {
+                      int smooth, int invert, const float vp[3], float cellHint, hipStream_t s,
+                      float coarseCell) {
     if (k < 3)
         k = 3;
     if (k > KNN_MAX)
@@ -621,6 +654,10 @@ void estimate_normals(Workspace &ws, int n, float *PX, float *PY, float *PZ, flo
     PPF_HIP_CHECK(hipMemcpyAsync(N0y, NY, sizeof(float) * n, hipMemcpyDeviceToDevice, s));
     PPF_HIP_CHECK(hipMemcpyAsync(N0z, NZ, sizeof(float) * n, hipMemcpyDeviceToDevice, s));
     grid_build(ws.gNormal, P0x, P0y, P0z, nullptr, nullptr, nullptr, valid, nValid, cellHint, s);
+    const bool twoGrids = coarseCell > cellHint;
+    if (twoGrids)
+        grid_build(ws.gNormalCoarse, P0x, P0y, P0z, nullptr, nullptr, nullptr, valid, nValid,
+                   coarseCell, s);
 
     NormArgs a;
     a.g    = ws.gNormal.view;
@@ -650,34 +687,72 @@ void estimate_normals(Workspace &ws, int n, float *PX, float *PY, float *PZ, flo
     PPF_HIP_CHECK(hipMemsetAsync(a.have, 0, sizeof(int) * n, s));
     PPF_HIP_CHECK(hipMemsetAsync(a.touch, 0, sizeof(int) * n, s));
     PPF_HIP_CHECK(hipMemsetAsync(a.mark, 0, sizeof(int) * n, s));
+    int *cntD = ws.counter.as<int>(4);
+    PPF_HIP_CHECK(hipMemsetAsync(cntD, 0, sizeof(int) * 4, s));
     if (nq > 0)
-        normals_mark_kernel<<<div_up(nq, 256), 256, 0, s>>>(a);
+        normals_mark_kernel<<<div_up(nq, 256), 256, 0, s>>>(a, cntD + 2);
+    // the same arguments over the coarse grid
+    NormArgs ac = a;
+    if (twoGrids) {
+        ac.g    = ws.gNormalCoarse.view;
+        ac.g.nx = P0x;
+        ac.g.ny = P0y;
+        ac.g.nz = P0z;
+    }
+    // ids with mark == want in the grid's cell order (counts to cntD[0] and,
+    // split off to the coarse grid, cntD[1])
+    const int ng     = a.g.n;
+    int      *olist  = ws.touchList.as<int>(std::max(ng, 1));
+    int      *olist2 = ws.nlist2.as<int>(std::max(ng, 1));
+    const int minCount = twoGrids ? std::max(2, (k + 3) / 4) : 0;
+    auto ordered = [&](int want) {
+        char *flags  = ws.voxFlag.as<char>(std::max(ng, 1));
+        char *flags2 = twoGrids ? ws.nflag2.as<char>(std::max(ng, 1)) : nullptr;
+        if (ng > 0)
+            normals_order_flags_kernel<<<div_up(ng, 256), 256, 0, s>>>(a.g, ng, a.mark, want,
+                                                                       minCount, flags, flags2);
+        size_t tb = 0;
+        PPF_HIP_CHECK(hipcub::DeviceSelect::Flagged(nullptr, tb, a.g.id, flags, olist, cntD, ng, s));
+        void *tmp = ws.cubTmp.reserve(tb);
+        PPF_HIP_CHECK(hipcub::DeviceSelect::Flagged(tmp, tb, a.g.id, flags, olist, cntD, ng, s));
+        if (twoGrids)
+            PPF_HIP_CHECK(hipcub::DeviceSelect::Flagged(tmp, tb, a.g.id, flags2, olist2, cntD + 1,
+                                                        ng, s));
+    };
 
     const bool small = k <= 16;  // kNN list bound: 16 or 32 entries
-    auto estimate = [&](const int *list, int nl, int want) {
+    auto estimate = [&](const NormArgs &x, const int *list, int nl, int want) {
         if (nl <= 0)
             return;
         if (small)
-            normals_point_kernel<16><<<div_up(nl, 128), 128, 0, s>>>(a, list, nl, want);
+            normals_point_kernel<16><<<div_up(nl, 128), 128, 0, s>>>(x, list, nl, want);
         else
-            normals_point_kernel<KNN_MAX><<<div_up(nl, 128), 128, 0, s>>>(a, list, nl, want);
+            normals_point_kernel<KNN_MAX><<<div_up(nl, 128), 128, 0, s>>>(x, list, nl, want);
     };
-    estimate(query, nq, 1);
-    if (smooth) {
-        // the neighbours the smoothing needs that have no estimate yet
-        char *flags = ws.voxFlag.as<char>(n);
-        int  *io    = ws.voxIota.as<int>(n);
-        int  *list  = ws.touchList.as<int>(n);
-        int  *cntD  = ws.counter.as<int>(4);
-        normals_touch_mark_kernel<<<div_up(n, 256), 256, 0, s>>>(a, n, flags, io);
-        size_t tb = 0;
-        PPF_HIP_CHECK(hipcub::DeviceSelect::Flagged(nullptr, tb, io, flags, list, cntD, n, s));
-        PPF_HIP_CHECK(hipcub::DeviceSelect::Flagged(ws.cubTmp.reserve(tb), tb, io, flags, list,
-                                                    cntD, n, s));
-        int nl = 0;
-        PPF_HIP_CHECK(hipMemcpyAsync(&nl, cntD, sizeof(int), hipMemcpyDeviceToHost, s));
+    // the queries to estimate, in cell order when every one of them is a
+    // point of the grid (a query outside the valid set keeps list order)
+    if (nq > 0) {
+        ordered(1);
+        int c[3] = {0, 0, 0};
+        PPF_HIP_CHECK(hipMemcpyAsync(c, cntD, sizeof(int) * 3, hipMemcpyDeviceToHost, s));
         PPF_HIP_CHECK(hipStreamSynchronize(s));
-        estimate(list, nl, 2);
+        if (c[0] + c[1] == c[2]) {
+            estimate(a, olist, c[0], 1);
+            estimate(ac, olist2, c[1], 1);
+        } else {
+            estimate(a, query, nq, 1);
+        }
+    }
+    if (smooth) {
+        // the neighbours the smoothing needs that have no estimate yet (all
+        // of them are grid points: they come from kNN lists)
+        normals_touch_mark_kernel<<<div_up(n, 256), 256, 0, s>>>(a, n);
+        ordered(2);
+        int c[2] = {0, 0};
+        PPF_HIP_CHECK(hipMemcpyAsync(c, cntD, sizeof(int) * 2, hipMemcpyDeviceToHost, s));
+        PPF_HIP_CHECK(hipStreamSynchronize(s));
+        estimate(a, olist, c[0], 2);
+        estimate(ac, olist2, c[1], 2);
     }
     if (nq > 0)
         normals_final_kernel<<<div_up(nq, 256), 256, 0, s>>>(a, outN);

@@ -252,12 +252,13 @@ void model_upload(DevModel &m, int nSampled, const float *sp, const float *sn, i
                                  hipMemcpyHostToDevice, s));
     // the voting copy: refs premultiplied by angleNum, every bucket starting
     // on an 8-entry (64-byte) boundary so a segment of 8 entries is four
-    // aligned 16-byte loads; padding entries are never read as votes
+    // aligned 16-byte loads; padding entries name the spare accumulator row
+    // past the last reference (voted into, never read)
     std::vector<uint32_t> vstart(std::max(K, 1) + 1, 0);
     for (int k = 0; k < K; k++)
         vstart[k + 1] = vstart[k] + (uint32_t)((offs[k + 1] - offs[k] + 7) & ~7);
     const size_t      VE = std::max<size_t>(vstart[K], 8);
-    std::vector<uint2> vent(VE, make_uint2(0xFFFFFFFFu, 0u));
+    std::vector<uint2> vent(VE, make_uint2((uint32_t)nSampled * (uint32_t)angleNum, 0u));
     for (int k = 0; k < K; k++)
         for (int e = offs[k]; e < offs[k + 1]; e++)
             vent[vstart[k] + (e - offs[k])] = make_uint2(refs[e] * (uint32_t)angleNum, ent[e].y);

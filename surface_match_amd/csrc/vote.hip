@@ -33,9 +33,11 @@ constexpr int VOTE_STATIC_LDS = 55 * 1024;   // bound on the kernel's static LDS
 constexpr int VOTE_MAX_CHUNKS = 512;         // chunks of 64 candidates with a first-cell entry
 constexpr int LDS_LIMIT       = 160 * 1024;  // gfx950 LDS per workgroup
 
+// references per accumulator tile: the LDS left beside the kernels' own
+// arrays, less the spare row that single-tile voting sends padding entries to
 int vote_max_tile_refs(int angleNum, int block) {
     (void)block;
-    return (LDS_LIMIT - VOTE_STATIC_LDS) / (angleNum * 4);
+    return (LDS_LIMIT - VOTE_STATIC_LDS) / (angleNum * 4) - 1;
 }
 
 struct VoteArgs {
@@ -124,22 +126,32 @@ __device__ __forceinline__ void load_segment(const VoteArgs &a, uint32_t e0, uin
     }
 }
 
+//
+// A segment is all body or all tail: buckets and their tails start on
+// multiples of 8 entries.  With one reference tile (RANGE false) the run is
+// [0, vhi) and the entries past vhi are the bucket's padding, whose
+// reference is the accumulator's spare row (refNum x angleNum), so every
+// entry is voted without a per-entry mask.
 template <bool RANGE>
 __device__ __forceinline__ int vote_run_loaded(const VoteArgs &a, uint32_t *acc, int r0A, int r1A,
                                                const SegRun &g, const uint2 (&v)[SEG]) {
-    const int tailFrom = (int)min(max((long long)g.tailStart - (long long)g.e0, 0ll), (long long)SEG);
-    int       nv       = 0;
+    const float off = g.e0 >= g.tailStart ? g.offTail : g.offBody;
+    if (!RANGE) {
+#pragma unroll
+        for (int kk = 0; kk < SEG; kk++) {
+            const int bin = ppfc::bin_wrapped(__uint_as_float(v[kk].y), off, a.sMaxId, a.angleNum);
+            atomicAdd(&acc[(int)v[kk].x + bin], 1u);
+        }
+        return g.vhi - g.vlo;
+    }
+    int nv = 0;
 #pragma unroll
     for (int kk = 0; kk < SEG; kk++) {
-        if (kk >= g.vlo && kk < g.vhi) {
-            const int refA = (int)v[kk].x;
-            if (!RANGE || (refA >= r0A && refA < r1A)) {
-                const float am  = __uint_as_float(v[kk].y);
-                const int   bin = kk < tailFrom ? ppfc::bin_body(am, g.offBody, a.sMaxId, a.angleNum)
-                                                : ppfc::bin_tail(am, g.offTail, a.sMaxId, a.angleNum);
-                atomicAdd(&acc[refA - r0A + bin], 1u);
-                nv++;
-            }
+        const int refA = (int)v[kk].x;
+        if (kk >= g.vlo && kk < g.vhi && refA >= r0A && refA < r1A) {
+            const int bin = ppfc::bin_wrapped(__uint_as_float(v[kk].y), off, a.sMaxId, a.angleNum);
+            atomicAdd(&acc[refA - r0A + bin], 1u);
+            nv++;
         }
     }
     return nv;
@@ -1125,7 +1137,7 @@ int vote_keypoints(Workspace &ws, const DevModel &m, const VoteKeypoints &in, fl
     a.hitAlpha = nullptr;
     a.splits   = nullptr;
     a.nHit = a.nbr = a.overflow = a.nOverflow = nullptr;
-    size_t lds      = (size_t)tileRefs * m.angleNum * 4;
+    size_t lds      = (size_t)(tileRefs + 1) * m.angleNum * 4;  // + the spare row
     auto   padded   = [&](int n) { return cfg.xcdRemap ? 8 * ((n + 7) / 8) : n; };
     int    blocks   = padded(nk) * nTiles;
     for (auto &e : ws.voteEv)

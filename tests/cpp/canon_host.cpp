@@ -14,6 +14,10 @@ float ch_alpha(const float *rt, const float *p) { return ppfc::alpha_of(rt, p[0]
 int   ch_bin(float am, float as, int n, int tail) {
     return ppfc::angle_bin(am, as, (float)(n - 1) / ppfc::kTwoPi, n, tail != 0);
 }
+int ch_bin_wrapped(float am, float as, int n, int tail) {
+    const float off = tail ? ppfc::bin_offset_tail(as) : ppfc::bin_offset_body(as);
+    return ppfc::bin_wrapped(am, off, (float)(n - 1) / ppfc::kTwoPi, n);
+}
 void ch_pose(const float *Ts, const float *Tm, float c, float s, float *out) {
     ppfc::compose_pose(Ts, Tm, c, s, out);
 }

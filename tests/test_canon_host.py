@@ -31,6 +31,8 @@ def H(tmp_path_factory):
     L.ch_alpha.argtypes = [fp, fp]
     L.ch_bin.restype = C.c_int
     L.ch_bin.argtypes = [C.c_float, C.c_float, C.c_int, C.c_int]
+    L.ch_bin_wrapped.restype = C.c_int
+    L.ch_bin_wrapped.argtypes = [C.c_float, C.c_float, C.c_int, C.c_int]
     L.ch_pose.argtypes = [fp, fp, C.c_float, C.c_float, fp]
     L.ch_quat.argtypes = [fp, fp]
     return L
@@ -74,3 +76,28 @@ def test_canon_matches_oracle(H, oracle_mod):
         H.ch_quat(np.ascontiguousarray(R.ravel()), q1)
         O.lib().orc_quat_from_rot(np.ascontiguousarray(R.ravel()), q2)
         assert np.array_equal(q1, q2)
+
+
+def test_branch_free_bin_equals_both_rules(H, oracle_mod):
+    """The voting kernels' branch-free bin (one wrap chosen from the
+    unwrapped angle) equals the reference's body and tail rules, each with
+    its own wrap order, on random angles and on the edges where a wrap
+    result lands on 0 or 2pi."""
+    O = oracle_mod
+    r = rng(91)
+    pi = np.float32(math.pi)
+    am = np.concatenate([r.uniform(-math.pi, math.pi, 20000).astype(np.float32),
+                         np.array([pi, -pi, 0.0, -0.0, np.nextafter(pi, 0), np.nextafter(-pi, 0)],
+                                  np.float32)])
+    as_ = np.concatenate([r.uniform(-math.pi, math.pi, 20000).astype(np.float32),
+                          np.array([-pi, pi, pi, -pi, 0.0, np.float32(1e-7)], np.float32)])
+    # pairs that make alphaModel - offset hit the wrap boundaries exactly
+    edge_m = np.array([pi, -pi, pi, -pi, 0.0], np.float32)
+    edge_s = np.array([pi, -pi, -pi, pi, -pi], np.float32)
+    am = np.concatenate([am, edge_m, as_[:500]])
+    as_ = np.concatenate([as_, edge_s, am[:500]])
+    for n in (30, 29, 64):
+        for tail in (0, 1):
+            for x, y in zip(am, as_):
+                assert H.ch_bin_wrapped(x, y, n, tail) == O.lib().orc_angle_bin(
+                    float(x), float(y), n, tail), (x, y, n, tail)
