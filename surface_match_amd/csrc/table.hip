@@ -258,6 +258,8 @@ void model_upload(DevModel &m, int nSampled, const float *sp, const float *sn, i
     for (int k = 0; k < K; k++)
         vstart[k + 1] = vstart[k] + (uint32_t)((offs[k + 1] - offs[k] + 7) & ~7);
     const size_t      VE = std::max<size_t>(vstart[K], 8);
+    if (VE >= (size_t(1) << 31))  // the voting kernels index entries with 32-bit offsets
+        throw std::runtime_error("model table too large for the voting kernels");
     std::vector<uint2> vent(VE, make_uint2((uint32_t)nSampled * (uint32_t)angleNum, 0u));
     for (int k = 0; k < K; k++)
         for (int e = offs[k]; e < offs[k + 1]; e++)

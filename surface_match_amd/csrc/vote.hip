@@ -615,6 +615,7 @@ __global__ void __launch_bounds__(BLOCK) vote_kernel(VoteArgs a) {
 // each other's latency.  A key point with more than hitCap hits gets
 // nHit = -1 and is voted by the fused kernel instead.
 constexpr int PAIR_BLOCK = 256;
+constexpr int HITS_INFLIGHT = 2;  // segments whose loads a voting lane issues together
 
 template <bool MULTI>
 __global__ void __launch_bounds__(PAIR_BLOCK) pair_kernel(VoteArgs a) {
@@ -823,8 +824,10 @@ __global__ void __launch_bounds__(BLOCK) vote_hits_kernel(VoteArgs a) {
         // each thread votes a contiguous run of segments (SEG entries of one
         // bucket each; different lanes, different buckets, so the LDS atomics
         // of a wave scatter over the accumulator): one search for the run's
-        // first hit, then a walk (hits with no segment are skipped); two
-        // segments in flight per step
+        // first hit, then a walk (hits with no segment are skipped); the
+        // loads of HITS_INFLIGHT segments are issued before any of them is
+        // voted (the accumulator leaves room for two workgroups of waves per
+        // CU at most, so each lane keeps several loads in flight instead)
         const int nb  = min(BLOCK, n - h0);
         const int spt = (totSeg + BLOCK - 1) / BLOCK;
         const int t0  = tid * spt, t1 = min(t0 + spt, totSeg);
@@ -839,31 +842,51 @@ __global__ void __launch_bounds__(BLOCK) vote_hits_kernel(VoteArgs a) {
                 else
                     hi = mid - 1;
             }
-            int h = lo;
-            for (int t = t0; t < t1; t += 2) {
-                while (h + 1 < nb && sPre[h + 1] <= t)
-                    h++;
-                auto seg = [&](int hq, int tq) {
-                    SegRun g;
-                    g.e0        = (sLo[hq] & ~7u) + (uint32_t)(tq - sPre[hq]) * SEG;
-                    g.vlo       = (int)max((long long)sLo[hq] - (long long)g.e0, 0ll);
-                    g.vhi       = (int)min((long long)sHi[hq] - (long long)g.e0, (long long)SEG);
-                    g.tailStart = sTail[hq];
-                    g.offBody   = sOffB[hq];
-                    g.offTail   = sOffT[hq];
-                    return g;
-                };
-                const SegRun g1 = seg(h, t);
-                SegRun       g2 = g1;
-                g2.vhi          = g2.vlo;  // empty unless a second segment follows
-                if (t + 1 < t1) {
-                    int h2 = h;
-                    while (h2 + 1 < nb && sPre[h2 + 1] <= t + 1)
-                        h2++;
-                    g2 = seg(h2, t + 1);
+            // the current hit's descriptor stays in registers; LDS is read
+            // again only when the run crosses into the next hit
+            int      h       = lo;
+            int      hPre    = sPre[h];
+            int      nextPre = h + 1 < nb ? sPre[h + 1] : 0x7fffffff;
+            uint32_t hLo = sLo[h], hHi = sHi[h], hTail = sTail[h];
+            float    hOffB = sOffB[h], hOffT = sOffT[h];
+            for (int t = t0; t < t1; t += HITS_INFLIGHT) {
+                SegRun g[HITS_INFLIGHT];
+                uint2  ent[HITS_INFLIGHT][SEG];
+#pragma unroll
+                for (int j = 0; j < HITS_INFLIGHT; j++) {
+                    const int tq = t + j;
+                    g[j].vlo     = 0;
+                    g[j].vhi     = 0;
+                    if (tq < t1) {
+                        if (tq >= nextPre) {
+                            // hits without segments share their prefix with
+                            // the next hit and are stepped over
+                            do {
+                                h++;
+                                hPre    = nextPre;
+                                nextPre = h + 1 < nb ? sPre[h + 1] : 0x7fffffff;
+                            } while (tq >= nextPre);
+                            hLo   = sLo[h];
+                            hHi   = sHi[h];
+                            hTail = sTail[h];
+                            hOffB = sOffB[h];
+                            hOffT = sOffT[h];
+                        }
+                        // entry offsets stay below 2^31: 32-bit differences
+                        g[j].e0        = (hLo & ~7u) + (uint32_t)(tq - hPre) * SEG;
+                        g[j].vlo       = max((int)(hLo - g[j].e0), 0);
+                        g[j].vhi       = min((int)(hHi - g[j].e0), SEG);
+                        g[j].tailStart = hTail;
+                        g[j].offBody   = hOffB;
+                        g[j].offTail   = hOffT;
+                        load_segment(a, g[j].e0, ent[j]);
+                    }
                 }
-                nv += a.nTiles > 1 ? vote_run2<true>(a, acc, r0A, r1A, g1, g2)
-                                   : vote_run2<false>(a, acc, r0A, r1A, g1, g2);
+#pragma unroll
+                for (int j = 0; j < HITS_INFLIGHT; j++)
+                    if (t + j < t1)
+                        nv += a.nTiles > 1 ? vote_run_loaded<true>(a, acc, r0A, r1A, g[j], ent[j])
+                                           : vote_run_loaded<false>(a, acc, r0A, r1A, g[j], ent[j]);
             }
         }
         nVotes += (unsigned long long)nv;
