@@ -163,8 +163,7 @@ PPF_API int ppf_match_scenes_resident(ppf_detector d, const ppf_scene *scenes, s
  * {1, 2, 3, 4}, then the key point order {Morton, plain}, then the XCD
  * mapping {round-robin, contiguous} (one knob at a time), then the batched ICP
  * over {spread, persistent} x {64, 128, 256, 512, 1024} threads per
- * workgroup, then the normals grid cell over {1/4, 1/2, 1} sample step,
- * then the coarse normals grid over {none, 2, 3, 4} sample steps, on
+ * workgroup, then the normals grid cell over {1/4, 1/2, 1} sample step, on
  * the given resident scene, and keeps the fastest in the detector.  None of
  * these changes any result. -- */
 typedef struct {
@@ -181,9 +180,6 @@ typedef struct {
                          LDS accumulators, more workgroups per CU */
     int voteOrder;    /* 1: workgroups take key points in (scene, Morton code of position)
                          order, 0: in key point order */
-    int normCoarse4;  /* points in sparse regions search a second normals grid of cell =
-                         sample step * normCoarse4 / 4 (0 .. 64; 0 or not above the normals
-                         cell: one grid) */
 } ppf_tuning;
 typedef struct {
     ppf_tuning best;

@@ -74,8 +74,7 @@ class IcpResult(C.Structure):
 class Tuning(C.Structure):
     _fields_ = [(n, C.c_int) for n in ("voteBlock", "voteXcdRemap", "voteCellDiv", "icpBlock",
                                        "icpSpread", "normCell4", "voteSplit",
-                                       "voteHitCap", "voteTiles", "voteOrder",
-                                       "normCoarse4")]
+                                       "voteHitCap", "voteTiles", "voteOrder")]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

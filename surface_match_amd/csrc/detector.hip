@@ -233,9 +233,6 @@ struct ppf_detector_s {
     VoteConfig   vcfg;
     IcpConfig    icfg;
     int          normCell4 = 4;  // normals grid cell = sample step * normCell4 / 4
-    // second normals grid for points in sparse regions: cell = sample step *
-    // normCoarse4 / 4 (0 = one grid)
-    int          normCoarse4 = 0;
     int          refTail = 1;
     DevCloud     tmpScene;
     DevBuf       idxA, idxB, idxC, idxD;  // sampled / keypoints / dense / scratch
@@ -448,7 +445,6 @@ void match_impl(ppf_detector_s &D, DevCloud &c, float rel, float kpFrac, float m
     const float reSampleStep = Dm * hm.param.poseRefRelSamplingDistance;
     const float sampleStep   = Dm * rel;  // :200
     const float normCell     = sampleStep * (float)D.normCell4 * 0.25f;
-    const float normCoarse   = sampleStep * (float)D.normCoarse4 * 0.25f;
     // :223-240
     const float voteThreshold = (float)refNum * minScore;
     float maxOverlapDist = 0;
@@ -472,8 +468,7 @@ void match_impl(ppf_detector_s &D, DevCloud &c, float rel, float kpFrac, float m
     // normals (ppf.cpp:204-211)
     if (!c.hasNormal)
         estimate_normals(D.ws, c.n, PX, PY, PZ, NX, NY, NZ, c.valid.get<int>(), c.nValid, samp, ns,
-                         p.knnNormal, p.smoothNormal, p.invertNormal, c.vp, normCell, s,
-                         normCoarse);
+                         p.knnNormal, p.smoothNormal, p.invertNormal, c.vp, normCell, s);
     PPF_HIP_CHECK(hipEventRecord(D.ev[2], s));
     // key points (ppf.cpp:212-216)
     const float keySampleStep = std::sqrt(1.0f / kpFrac) * sampleStep;
@@ -509,8 +504,7 @@ void match_impl(ppf_detector_s &D, DevCloud &c, float rel, float kpFrac, float m
                           dense, cnt, s);
         if (!c.hasNormal)
             estimate_normals(D.ws, c.n, PX, PY, PZ, NX, NY, NZ, c.valid.get<int>(), c.nValid,
-                             dense, nd, p.knnNormal, 0, p.invertNormal, c.vp, normCell, s,
-                             normCoarse);
+                             dense, nd, p.knnNormal, 0, p.invertNormal, c.vp, normCell, s);
     }
     PPF_HIP_CHECK(hipEventRecord(D.ev[7], s));
     // icp (ppf.cpp:436-495)
@@ -677,7 +671,6 @@ void match_batch_impl(ppf_detector_s &D, const std::vector<DevCloud *> &clouds, 
     const float reSampleStep = Dm * hm.param.poseRefRelSamplingDistance;
     const float sampleStep   = Dm * rel;
     const float normCell     = sampleStep * (float)D.normCell4 * 0.25f;
-    const float normCoarse   = sampleStep * (float)D.normCoarse4 * 0.25f;
     const float voteThreshold = (float)refNum * minScore;
     float maxOverlapDist = 0;
     if (p.maxOverlapDistRel > 0) maxOverlapDist = Dm * p.maxOverlapDistRel;
@@ -706,8 +699,7 @@ void match_batch_impl(ppf_detector_s &D, const std::vector<DevCloud *> &clouds, 
                              samp, cnt, s);
         if (!c.hasNormal)
             estimate_normals(D.ws, c.n, PX, PY, PZ, NX, NY, NZ, c.valid.get<int>(), c.nValid, samp,
-                             sw.ns, p.knnNormal, p.smoothNormal, p.invertNormal, c.vp, normCell, s,
-                         normCoarse);
+                             sw.ns, p.knnNormal, p.smoothNormal, p.invertNormal, c.vp, normCell, s);
         int *kp = sw.kp.as<int>(std::max(sw.ns, 1));
         sw.nk   = voxel_sample(D.ws, PX, PY, PZ, samp, sw.ns, keySampleStep, c.boxMin, kp, cnt, s);
         grid_build(sw.gVote, PX, PY, PZ, NX, NY, NZ, samp, sw.ns, Dm / (float)D.vcfg.cellDiv, s);
@@ -823,7 +815,7 @@ void match_batch_impl(ppf_detector_s &D, const std::vector<DevCloud *> &clouds, 
                 if (!c.hasNormal)
                     estimate_normals(D.ws, c.n, PX, PY, PZ, NX, NY, NZ, c.valid.get<int>(),
                                      c.nValid, dense, sw.nd, p.knnNormal, 0, p.invertNormal, c.vp,
-                                     normCell, s, normCoarse);
+                                     normCell, s);
                 nDenseTot += sw.nd;
             }
             grid_build(sw.gSparse, PX, PY, PZ, NX, NY, NZ, sw.samp.get<int>(), sw.ns, sampleStep, s);
@@ -1442,7 +1434,6 @@ ppf_tuning tuning_of(const ppf_detector_s &d) {
     t.voteHitCap   = d.vcfg.hitCap;
     t.voteTiles    = d.vcfg.tiles;
     t.voteOrder    = d.vcfg.order;
-    t.normCoarse4  = d.normCoarse4;
     return t;
 }
 }  // namespace
@@ -1469,8 +1460,6 @@ PPF_API int ppf_set_tuning(ppf_detector d, const ppf_tuning *t) {
     PPF_REQUIRE(t->voteTiles >= 1 && t->voteTiles <= 16, "vote tiles must be 1 .. 16");
     PPF_REQUIRE(t->normCell4 == 1 || t->normCell4 == 2 || t->normCell4 == 4,
                 "normals cell must be 1, 2 or 4 quarters of the sample step");
-    PPF_REQUIRE(t->normCoarse4 >= 0 && t->normCoarse4 <= 64,
-                "coarse normals cell must be 0 .. 64 quarters of the sample step");
     d->icfg.block    = t->icpBlock;
     d->icfg.spread   = t->icpSpread ? 1 : 0;
     d->normCell4     = t->normCell4;
@@ -1478,7 +1467,6 @@ PPF_API int ppf_set_tuning(ppf_detector d, const ppf_tuning *t) {
     d->vcfg.hitCap   = t->voteHitCap;
     d->vcfg.tiles    = t->voteTiles;
     d->vcfg.order    = t->voteOrder ? 1 : 0;
-    d->normCoarse4   = t->normCoarse4;
     return PPF_OK;
 }
 
@@ -1539,7 +1527,7 @@ PPF_API int ppf_autotune(ppf_detector d, ppf_scene sc, float rel, float kpFrac, 
                 }
             }
         d->icfg               = bestI;
-        // normals: the grid cell, then the coarse grid for sparse regions
+        // normals: the grid cell
         float bestNormMs = 1e30f;
         auto  tryNormals = [&](int ppf_detector_s::*knob, std::initializer_list<int> values) {
             int best = d->*knob;
@@ -1561,7 +1549,6 @@ PPF_API int ppf_autotune(ppf_detector d, ppf_scene sc, float rel, float kpFrac, 
             d->*knob = best;
         };
         tryNormals(&ppf_detector_s::normCell4, {1, 2, 4});
-        tryNormals(&ppf_detector_s::normCoarse4, {0, 8, 12, 16});
         rep->normalsMs        = bestNormMs;
         rep->best             = tuning_of(*d);
         rep->voteMs           = bestVoteMs;
@@ -1814,7 +1801,7 @@ PPF_API int ppf_estimate_normals(float *pts, float *nrm, size_t n, const int *va
         float cell = (float)(2.0 * std::cbrt(vol / std::max<size_t>(nValid, 1)));
         estimate_normals(ws, (int)n, P.x.get<float>(), P.y.get<float>(), P.z.get<float>(),
                          N.x.get<float>(), N.y.get<float>(), N.z.get<float>(), v, (int)nValid, qq,
-                         (int)nq, k, smooth, invert, viewPoint, cell, s, 4.0f * cell);
+                         (int)nq, k, smooth, invert, viewPoint, cell, s);
         DevBuf outb;
         float *o = outb.as<float>(std::max<size_t>(n, 1) * 3);
         int    m = std::max<int>((int)n, 1);

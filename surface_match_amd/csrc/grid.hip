@@ -497,29 +497,13 @@ __global__ void normals_mark_kernel(NormArgs a, int *nMarked) {
         atomicAdd(nMarked, __popcll(m));
 }
 
-// The grid's points (in cell order) whose mark equals want: estimating them
-// in this order lets neighbouring threads search the same cells.  With
-// minCount > 0 they are split by the population of their own cell: points
-// in cells of fewer than minCount points (sparse regions, whose k nearest
-// neighbours lie several cells away) go to the second list, searched on a
-// coarser grid.  Either grid gives the same exact neighbours.
-__global__ void normals_order_flags_kernel(GridView g, int n, const int *mark, int want,
-                                           int minCount, char *fineF, char *coarseF) {
+// the grid's points (in cell order) whose mark equals want: estimating them
+// in this order lets neighbouring threads search the same cells
+__global__ void normals_order_flags_kernel(const int *gid, int n, const int *mark, int want,
+                                           char *flags) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n)
-        return;
-    const bool m    = mark[g.id[t]] == want;
-    bool       fine = true;
-    if (m && minCount > 0) {
-        const int cx = cell_coord(g.x[t], g.ox, g.cell, g.dx);
-        const int cy = cell_coord(g.y[t], g.oy, g.cell, g.dy);
-        const int cz = cell_coord(g.z[t], g.oz, g.cell, g.dz);
-        int       s0 = 0, c0 = 0;
-        fine = cell_lookup(g, cx, cy, cz, s0, c0) && c0 >= minCount;
-    }
-    fineF[t] = (m && fine) ? 1 : 0;
-    if (coarseF)
-        coarseF[t] = (m && !fine) ? 1 : 0;
+    if (t < n)
+        flags[t] = mark[gid[t]] == want ? 1 : 0;
 }
 
 // neighbours that the smoothing needs and that have no estimate yet
@@ -638,8 +622,7 @@ __global__ void normals_orient_kernel(NormArgs a, const float *outN, float *PX, 
 
 void estimate_normals(Workspace &ws, int n, float *PX, float *PY, float *PZ, float *NX, float *NY,
                       float *NZ, const int *valid, int nValid, const int *query, int nq, int k,
-                      int smooth, int invert, const float vp[3], float cellHint, hipStream_t s,
-                      float coarseCell) {
+                      int smooth, int invert, const float vp[3], float cellHint, hipStream_t s) {
     if (k < 3)
         k = 3;
     if (k > KNN_MAX)
@@ -654,10 +637,6 @@ void estimate_normals(Workspace &ws, int n, float *PX, float *PY, float *PZ, flo
     PPF_HIP_CHECK(hipMemcpyAsync(N0y, NY, sizeof(float) * n, hipMemcpyDeviceToDevice, s));
     PPF_HIP_CHECK(hipMemcpyAsync(N0z, NZ, sizeof(float) * n, hipMemcpyDeviceToDevice, s));
     grid_build(ws.gNormal, P0x, P0y, P0z, nullptr, nullptr, nullptr, valid, nValid, cellHint, s);
-    const bool twoGrids = coarseCell > cellHint;
-    if (twoGrids)
-        grid_build(ws.gNormalCoarse, P0x, P0y, P0z, nullptr, nullptr, nullptr, valid, nValid,
-                   coarseCell, s);
 
     NormArgs a;
     a.g    = ws.gNormal.view;
@@ -691,43 +670,28 @@ void estimate_normals(Workspace &ws, int n, float *PX, float *PY, float *PZ, flo
     PPF_HIP_CHECK(hipMemsetAsync(cntD, 0, sizeof(int) * 4, s));
     if (nq > 0)
         normals_mark_kernel<<<div_up(nq, 256), 256, 0, s>>>(a, cntD + 2);
-    // the same arguments over the coarse grid
-    NormArgs ac = a;
-    if (twoGrids) {
-        ac.g    = ws.gNormalCoarse.view;
-        ac.g.nx = P0x;
-        ac.g.ny = P0y;
-        ac.g.nz = P0z;
-    }
-    // ids with mark == want in the grid's cell order (counts to cntD[0] and,
-    // split off to the coarse grid, cntD[1])
-    const int ng     = a.g.n;
-    int      *olist  = ws.touchList.as<int>(std::max(ng, 1));
-    int      *olist2 = ws.nlist2.as<int>(std::max(ng, 1));
-    const int minCount = twoGrids ? std::max(2, (k + 3) / 4) : 0;
+    // ids with mark == want in the grid's cell order (their count to cntD[0])
+    const int ng    = a.g.n;
+    int      *olist = ws.touchList.as<int>(std::max(ng, 1));
     auto ordered = [&](int want) {
-        char *flags  = ws.voxFlag.as<char>(std::max(ng, 1));
-        char *flags2 = twoGrids ? ws.nflag2.as<char>(std::max(ng, 1)) : nullptr;
+        char *flags = ws.voxFlag.as<char>(std::max(ng, 1));
         if (ng > 0)
-            normals_order_flags_kernel<<<div_up(ng, 256), 256, 0, s>>>(a.g, ng, a.mark, want,
-                                                                       minCount, flags, flags2);
+            normals_order_flags_kernel<<<div_up(ng, 256), 256, 0, s>>>(a.g.id, ng, a.mark, want,
+                                                                       flags);
         size_t tb = 0;
         PPF_HIP_CHECK(hipcub::DeviceSelect::Flagged(nullptr, tb, a.g.id, flags, olist, cntD, ng, s));
-        void *tmp = ws.cubTmp.reserve(tb);
-        PPF_HIP_CHECK(hipcub::DeviceSelect::Flagged(tmp, tb, a.g.id, flags, olist, cntD, ng, s));
-        if (twoGrids)
-            PPF_HIP_CHECK(hipcub::DeviceSelect::Flagged(tmp, tb, a.g.id, flags2, olist2, cntD + 1,
-                                                        ng, s));
+        PPF_HIP_CHECK(hipcub::DeviceSelect::Flagged(ws.cubTmp.reserve(tb), tb, a.g.id, flags,
+                                                    olist, cntD, ng, s));
     };
 
     const bool small = k <= 16;  // kNN list bound: 16 or 32 entries
-    auto estimate = [&](const NormArgs &x, const int *list, int nl, int want) {
+    auto estimate = [&](const int *list, int nl, int want) {
         if (nl <= 0)
             return;
         if (small)
-            normals_point_kernel<16><<<div_up(nl, 128), 128, 0, s>>>(x, list, nl, want);
+            normals_point_kernel<16><<<div_up(nl, 128), 128, 0, s>>>(a, list, nl, want);
         else
-            normals_point_kernel<KNN_MAX><<<div_up(nl, 128), 128, 0, s>>>(x, list, nl, want);
+            normals_point_kernel<KNN_MAX><<<div_up(nl, 128), 128, 0, s>>>(a, list, nl, want);
     };
     // the queries to estimate, in cell order when every one of them is a
     // point of the grid (a query outside the valid set keeps list order)
@@ -736,23 +700,20 @@ void estimate_normals(Workspace &ws, int n, float *PX, float *PY, float *PZ, flo
         int c[3] = {0, 0, 0};
         PPF_HIP_CHECK(hipMemcpyAsync(c, cntD, sizeof(int) * 3, hipMemcpyDeviceToHost, s));
         PPF_HIP_CHECK(hipStreamSynchronize(s));
-        if (c[0] + c[1] == c[2]) {
-            estimate(a, olist, c[0], 1);
-            estimate(ac, olist2, c[1], 1);
-        } else {
-            estimate(a, query, nq, 1);
-        }
+        if (c[0] == c[2])
+            estimate(olist, c[0], 1);
+        else
+            estimate(query, nq, 1);
     }
     if (smooth) {
         // the neighbours the smoothing needs that have no estimate yet (all
         // of them are grid points: they come from kNN lists)
         normals_touch_mark_kernel<<<div_up(n, 256), 256, 0, s>>>(a, n);
         ordered(2);
-        int c[2] = {0, 0};
-        PPF_HIP_CHECK(hipMemcpyAsync(c, cntD, sizeof(int) * 2, hipMemcpyDeviceToHost, s));
+        int nl = 0;
+        PPF_HIP_CHECK(hipMemcpyAsync(&nl, cntD, sizeof(int), hipMemcpyDeviceToHost, s));
         PPF_HIP_CHECK(hipStreamSynchronize(s));
-        estimate(a, olist, c[0], 2);
-        estimate(ac, olist2, c[1], 2);
+        estimate(olist, nl, 2);
     }
     if (nq > 0)
         normals_final_kernel<<<div_up(nq, 256), 256, 0, s>>>(a, outN);

@@ -22,8 +22,7 @@ struct Workspace {
     DevBuf voxKey, voxBest, voxFlag, voxIota, cubTmp, counter;
     DevBuf p0x, p0y, p0z, n0x, n0y, n0z, rawX, rawY, rawZ, cenX, cenY, cenZ, have, touch,
         touchList, nbq, cntq, outN, mark, qidx;
-    Grid   gNormal, gNormalCoarse, gVote, gSparse, gDense, gFull;
-    DevBuf nlist2, nflag2;
+    Grid   gNormal, gVote, gSparse, gDense, gFull;
     // vote
     DevBuf kpX, kpY, kpZ, kpNX, kpNY, kpNZ, cand, candTile, poses, voteStats, accDump, voteCnt,
         smallCnt, hitStart, hitLen, hitAlpha, hitBucket, hitCount, splits, kpKey, kpKey2, kpOrd,
@@ -105,7 +104,7 @@ int  voxel_sample(Workspace &ws, const float *X, const float *Y, const float *Z,
 void estimate_normals(Workspace &ws, int n, float *PX, float *PY, float *PZ, float *NX,
                       float *NY, float *NZ, const int *valid, int nValid, const int *query,
                       int nq, int k, int smooth, int invert, const float vp[3], float cellHint,
-                      hipStream_t s, float coarseCell = 0.0f);
+                      hipStream_t s);
 
 // ---------------------------------------------------------------- vote
 
