@@ -67,6 +67,13 @@ struct DevModel {
     DevBuf   cosSin;                         // 2 * angleNum floats
     int      refsSorted = 0;                 // every bucket's refs ascending
     unsigned generation = 0;                 // bumped by every upload
+    // pair-stage pre-filters over quantized features (filterA == 0: none):
+    // bitmaps of the (q4), (q3, q4) and (q1, q3, q4) projections of every
+    // feature tuple whose key is in the table; angle indices 0 .. filterA - 2
+    // and distance indices 0 .. filterD - 2 are values, the last index of
+    // each stands for any other value and is always marked
+    DevBuf   keyFilter;
+    int      filterA = 0, filterD = 0;
 };
 
 // Records of the model pair stage before grouping (train on device).

@@ -218,6 +218,61 @@ static void upload_soa(DevBuf &bx, DevBuf &by, DevBuf &bz, const float *aos, int
     PPF_HIP_CHECK(hipStreamSynchronize(s));
 }
 
+// Pre-filters for the pair stage.  A scene pair's key is murmur(q1, q2, q3,
+// q4) of its quantized features; angles lie in [0, pi] and the distance of
+// an in-sphere pair below the model diameter, so every finite quantized
+// value of a scene pair is at most Qa = ceil(pi / angleStep) (angles) or
+// Qd = ceil(1.001 D / distStep) (distance).  That tuple space is small, so
+// every tuple is hashed here and those whose key is in the table are
+// projected onto (q4), (q3, q4) and (q1, q3, q4).  A pair whose partial tuple
+// is in none of them cannot hit the table, and the pair kernel drops it
+// before computing the remaining features.  Values outside the ranges (the
+// NaN code of a degenerate pair) map to an index that always passes.
+void build_key_filter(DevModel &m, const std::vector<uint32_t> &keys, hipStream_t s) {
+    m.filterA = m.filterD = 0;
+    if (keys.empty())
+        return;
+    const int Qa = (int)std::ceil(ppfc::kPi / m.angleStep) + 1;
+    const int Qd = (int)std::ceil(m.diameter * 1.001f / m.distStep) + 1;
+    const int A = Qa + 2, Dd = Qd + 2;  // values 0 .. Q, then "other"
+    // the pair kernel packs quantized values in 10 bits
+    if (Qa > 1000 || Qd > 1000 || (double)(Qa + 1) * (Qa + 1) * (Qa + 1) * (Qd + 1) > 3.4e7)
+        return;
+    std::vector<uint32_t> sorted(keys);
+    std::sort(sorted.begin(), sorted.end());
+    const size_t n4 = Dd, n34 = (size_t)A * Dd, n134 = (size_t)A * A * Dd;
+    const size_t w4 = (n4 + 31) / 32, w34 = (n34 + 31) / 32, w134 = (n134 + 31) / 32;
+    std::vector<uint32_t> bits(w4 + w34 + w134, 0u);
+    auto set = [&](size_t base, size_t i) { bits[base + i / 32] |= 1u << (i % 32); };
+    for (int q1 = 0; q1 <= Qa; q1++)
+        for (int q2 = 0; q2 <= Qa; q2++)
+            for (int q3 = 0; q3 <= Qa; q3++)
+                for (int q4 = 0; q4 <= Qd; q4++) {
+                    const uint32_t k = ppfc::murmur_key4((uint32_t)q1, (uint32_t)q2, (uint32_t)q3,
+                                                         (uint32_t)q4);
+                    if (!std::binary_search(sorted.begin(), sorted.end(), k))
+                        continue;
+                    set(0, (size_t)q4);
+                    set(w4, (size_t)q3 * Dd + q4);
+                    set(w4 + w34, ((size_t)q1 * A + q3) * Dd + q4);
+                }
+    // any "other" index passes
+    set(0, (size_t)(Dd - 1));
+    for (int q3 = 0; q3 < A; q3++)
+        for (int q4 = 0; q4 < Dd; q4++) {
+            if (q3 == A - 1 || q4 == Dd - 1)
+                set(w4, (size_t)q3 * Dd + q4);
+            for (int q1 = 0; q1 < A; q1++)
+                if (q1 == A - 1 || q3 == A - 1 || q4 == Dd - 1)
+                    set(w4 + w34, ((size_t)q1 * A + q3) * Dd + q4);
+        }
+    PPF_HIP_CHECK(hipMemcpyAsync(m.keyFilter.as<uint32_t>(bits.size()), bits.data(),
+                                 sizeof(uint32_t) * bits.size(), hipMemcpyHostToDevice, s));
+    PPF_HIP_CHECK(hipStreamSynchronize(s));
+    m.filterA = A;
+    m.filterD = Dd;
+}
+
 void model_upload(DevModel &m, int nSampled, const float *sp, const float *sn, int nRe,
                   const float *rp, const float *rn, const std::vector<uint32_t> &keys,
                   const std::vector<int> &offs, const std::vector<uint32_t> &refs,
@@ -294,6 +349,7 @@ void model_upload(DevModel &m, int nSampled, const float *sp, const float *sn, i
     m.generation++;
     PPF_HIP_CHECK(hipMemcpyAsync(m.slots.as<uint4>(T), slots.data(), sizeof(uint4) * T,
                                  hipMemcpyHostToDevice, s));
+    build_key_filter(m, keys, s);
     // pose angle of each bin: (float)((double)(fl(2pi) j / maxIdx) - pi), ppf.cpp:402
     std::vector<float> cs(2 * angleNum);
     for (int j = 0; j < angleNum; j++) {

@@ -71,6 +71,9 @@ struct VoteArgs {
     int         *nHit, *nbr;    // hit count (-1: overflow), neighbour count incl. self
     int         *overflow, *nOverflow;
     int          countPairs;    // fused kernel adds its pairs and hits to stats
+    // quantized-feature pre-filters of the pair kernel (nullptr: none)
+    const uint32_t *filter;
+    int          fA, fD;
 };
 
 __device__ __forceinline__ bool table_find(const uint4 *slots, uint32_t tmask, uint32_t key,
@@ -614,8 +617,25 @@ __global__ void __launch_bounds__(BLOCK) vote_kernel(VoteArgs a) {
 // HBM.  No accumulator, so several small workgroups share a CU and hide
 // each other's latency.  A key point with more than hitCap hits gets
 // nHit = -1 and is voted by the fused kernel instead.
+//
+// The features of a pair are computed in three stages, each on full waves
+// drawn from a per-wave LDS queue: (A) the distance and the normal angle,
+// checked against the model's (q4) and (q3, q4) pre-filters; (B) the angle
+// of the first normal to the pair direction, checked against (q1, q3, q4);
+// (C) the last angle, the key and the table probe, and the scene alpha for
+// hits only.  Most pairs leave after stage A or B, so the expensive tail of
+// the feature sequence runs on a fraction of them.  The features are the
+// canonical ones (ppfc::ppf_key's operations, in pieces).
 constexpr int PAIR_BLOCK = 256;
 constexpr int HITS_INFLIGHT = 2;  // segments whose loads a voting lane issues together
+
+// quantized feature values packed 10 bits each; 1023 stands for the NaN
+// code 0x80000000 (finite values stay below 1000 when the pre-filters are on)
+__device__ __forceinline__ uint32_t qpack(uint32_t q) { return q > 1022u ? 1023u : q; }
+__device__ __forceinline__ uint32_t qunpack(uint32_t c) { return c == 1023u ? 0x80000000u : c; }
+__device__ __forceinline__ bool filter_bit(const uint32_t *w, uint32_t i) {
+    return (w[i >> 5] >> (i & 31u)) & 1u;
+}
 
 template <bool MULTI>
 __global__ void __launch_bounds__(PAIR_BLOCK) pair_kernel(VoteArgs a) {
@@ -624,7 +644,15 @@ __global__ void __launch_bounds__(PAIR_BLOCK) pair_kernel(VoteArgs a) {
     __shared__ int sCellStart[VOTE_MAX_CELLS];
     __shared__ int sCellPre[VOTE_MAX_CELLS + 1];
     __shared__ int sChunkCell[VOTE_MAX_CHUNKS];
-    __shared__ int pQueue[NW][QCAP];
+    __shared__ int pQueue[NW][QCAP];  // in-sphere pairs: candidate slot
+    // stage B queue: slot, |d|, packed (q3, q4)
+    __shared__ int      bSlot[NW][QCAP];
+    __shared__ float    bDn[NW][QCAP];
+    __shared__ uint32_t bCode[NW][QCAP];
+    // stage C queue: slot, packed (q1, q3, q4), unit direction
+    __shared__ int      cSlot[NW][QCAP];
+    __shared__ uint32_t cCode[NW][QCAP];
+    __shared__ float    cU[NW][3][QCAP];
     __shared__ int sScan[NW + 1];
     __shared__ int sNext, sHits, sIn;
     __shared__ unsigned long long sPairs[NW];
@@ -659,23 +687,36 @@ __global__ void __launch_bounds__(PAIR_BLOCK) pair_kernel(VoteArgs a) {
 
     const int          nChunks = (total + 63) >> 6;
     const size_t       base    = (size_t)k * a.hitCap;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    const bool         filt    = a.filter != nullptr;
+    const uint32_t     fA = (uint32_t)a.fA, fD = (uint32_t)a.fD;
+    const uint32_t    *f4 = a.filter, *f34 = f4 + (fD + 31) / 32,
+                   *f134 = f34 + (fA * fD + 31) / 32;
+    auto aidx = [&](uint32_t q) { return q < fA - 1 ? q : fA - 1; };
+    auto didx = [&](uint32_t q) { return q < fD - 1 ? q : fD - 1; };
     unsigned long long nPairs  = 0;
-    int                inSphere = 0, pn = 0;
+    int                inSphere = 0, pn = 0, bn = 0, cn = 0;
     int               *pq = pQueue[wv];
-    auto pair_stage = [&](int cnt) {
+
+    // (C) last angle, key, probe; alpha of the hits; hits to HBM
+    auto stageC = [&](int cnt) {
         int      hit    = 0;
         uint32_t bStart = 0, bLen = 0, bucket = 0;
         float    alphaS = 0.0f;
         if (lane < cnt) {
-            const int s  = pq[lane];
-            float     px = g.x[s], py = g.y[s], pz = g.z[s];
-            alphaS       = ppfc::alpha_of(rt, px, py, pz);
-            uint32_t key = ppfc::ppf_key(qx, qy, qz, nx1, ny1, nz1, px, py, pz, g.nx[s], g.ny[s],
-                                         g.nz[s], a.angleStep, a.distStep, false);
-            if (table_find(a.slots, a.tmask, key, bStart, bLen, bucket) && !ppfc::is_nan(alphaS))
-                hit = 1;
+            const int      s    = cSlot[wv][lane];
+            const uint32_t code = cCode[wv][lane];
+            const float    f2 = ppfc::angle_a2b(cU[wv][0][lane], cU[wv][1][lane], cU[wv][2][lane],
+                                                g.nx[s], g.ny[s], g.nz[s]);
+            const uint32_t key = ppfc::murmur_key4(
+                qunpack(code >> 20), ppfc::quant(f2, a.angleStep, false),
+                qunpack((code >> 10) & 1023u), qunpack(code & 1023u));
+            if (table_find(a.slots, a.tmask, key, bStart, bLen, bucket)) {
+                alphaS = ppfc::alpha_of(rt, g.x[s], g.y[s], g.z[s]);
+                hit    = ppfc::is_nan(alphaS) ? 0 : 1;
+            }
         }
-        const unsigned long long m = __ballot(hit);
+        const unsigned long long m  = __ballot(hit);
         const int                nh = __popcll(m);
         if (nh == 0)
             return;
@@ -684,7 +725,7 @@ __global__ void __launch_bounds__(PAIR_BLOCK) pair_kernel(VoteArgs a) {
             at = atomicAdd(&sHits, nh);
         at = __shfl(at, 0, 64);
         if (hit) {
-            const int pos = at + __popcll(m & ((1ull << lane) - 1ull));
+            const int pos = at + __popcll(m & below);
             if (pos < a.hitCap) {
                 a.hitStart[base + pos]  = bStart;
                 a.hitLen[base + pos]    = bLen;
@@ -692,6 +733,112 @@ __global__ void __launch_bounds__(PAIR_BLOCK) pair_kernel(VoteArgs a) {
                 a.hitBucket[base + pos] = bucket;
             }
         }
+    };
+    // 64 entries of the C queue voted, the rest moved to its front
+    auto drainC = [&](bool all) {
+        while (cn >= 64 || (all && cn > 0)) {
+            wave_sync();
+            stageC(min(cn, 64));
+            const int rest = max(cn - 64, 0);
+            int       s0 = 0;
+            uint32_t  c0 = 0;
+            float     u0 = 0.0f, u1 = 0.0f, u2 = 0.0f;
+            if (lane < rest) {
+                s0 = cSlot[wv][64 + lane];
+                c0 = cCode[wv][64 + lane];
+                u0 = cU[wv][0][64 + lane];
+                u1 = cU[wv][1][64 + lane];
+                u2 = cU[wv][2][64 + lane];
+            }
+            wave_sync();
+            if (lane < rest) {
+                cSlot[wv][lane]    = s0;
+                cCode[wv][lane]    = c0;
+                cU[wv][0][lane]    = u0;
+                cU[wv][1][lane]    = u1;
+                cU[wv][2][lane]    = u2;
+            }
+            cn = rest;
+        }
+    };
+    // (B) angle of the first normal to the pair direction
+    auto stageB = [&](int cnt) {
+        bool     pass = false;
+        int      s    = 0;
+        uint32_t code = 0;
+        float    ux = 0.0f, uy = 0.0f, uz = 0.0f;
+        if (lane < cnt) {
+            s              = bSlot[wv][lane];
+            const float dn = bDn[wv][lane];
+            code           = bCode[wv][lane];
+            ux             = (g.x[s] - qx) / dn;
+            uy             = (g.y[s] - qy) / dn;
+            uz             = (g.z[s] - qz) / dn;
+            const float    f1 = ppfc::angle_a2b(ux, uy, uz, nx1, ny1, nz1);
+            const uint32_t q1 = qpack(ppfc::quant(f1, a.angleStep, false));
+            code |= q1 << 20;
+            pass = !filt || filter_bit(f134, (aidx(q1) * fA + aidx((code >> 10) & 1023u)) * fD +
+                                                 didx(code & 1023u));
+        }
+        const unsigned long long m = __ballot(pass);
+        if (pass) {
+            const int pos      = cn + __popcll(m & below);
+            cSlot[wv][pos]     = s;
+            cCode[wv][pos]     = code;
+            cU[wv][0][pos]     = ux;
+            cU[wv][1][pos]     = uy;
+            cU[wv][2][pos]     = uz;
+        }
+        cn += __popcll(m);
+        drainC(false);
+    };
+    auto drainB = [&](bool all) {
+        while (bn >= 64 || (all && bn > 0)) {
+            wave_sync();
+            stageB(min(bn, 64));
+            const int rest = max(bn - 64, 0);
+            int       s0 = 0;
+            float     d0 = 0.0f;
+            uint32_t  c0 = 0;
+            if (lane < rest) {
+                s0 = bSlot[wv][64 + lane];
+                d0 = bDn[wv][64 + lane];
+                c0 = bCode[wv][64 + lane];
+            }
+            wave_sync();
+            if (lane < rest) {
+                bSlot[wv][lane] = s0;
+                bDn[wv][lane]   = d0;
+                bCode[wv][lane] = c0;
+            }
+            bn = rest;
+        }
+    };
+    // (A) distance and normal angle
+    auto stageA = [&](int cnt) {
+        bool     pass = false;
+        int      s    = 0;
+        float    dn   = 0.0f;
+        uint32_t code = 0;
+        if (lane < cnt) {
+            s              = pq[lane];
+            const float dx = g.x[s] - qx, dy = g.y[s] - qy, dz = g.z[s] - qz;
+            dn             = sqrtf(dx * dx + dy * dy + dz * dz);
+            const uint32_t q4 = qpack(ppfc::quant(dn, a.distStep, false));
+            const float    f3 = ppfc::angle_a2b(nx1, ny1, nz1, g.nx[s], g.ny[s], g.nz[s]);
+            const uint32_t q3 = qpack(ppfc::quant(f3, a.angleStep, false));
+            code              = (q3 << 10) | q4;
+            pass = !filt || (filter_bit(f4, didx(q4)) && filter_bit(f34, aidx(q3) * fD + didx(q4)));
+        }
+        const unsigned long long m = __ballot(pass);
+        if (pass) {
+            const int pos   = bn + __popcll(m & below);
+            bSlot[wv][pos]  = s;
+            bDn[wv][pos]    = dn;
+            bCode[wv][pos]  = code;
+        }
+        bn += __popcll(m);
+        drainB(false);
     };
     for (;;) {
         int t = 0;
@@ -714,11 +861,11 @@ __global__ void __launch_bounds__(PAIR_BLOCK) pair_kernel(VoteArgs a) {
         }
         const unsigned long long m = __ballot(pc);
         if (pc)
-            pq[pn + __popcll(m & ((1ull << lane) - 1ull))] = s;
+            pq[pn + __popcll(m & below)] = s;
         pn += __popcll(m);
         if (pn >= 64) {
             wave_sync();
-            pair_stage(64);
+            stageA(64);
             const int rest = pn - 64;
             const int v    = lane < rest ? pq[64 + lane] : 0;
             wave_sync();
@@ -731,8 +878,10 @@ __global__ void __launch_bounds__(PAIR_BLOCK) pair_kernel(VoteArgs a) {
     }
     if (pn > 0) {
         wave_sync();
-        pair_stage(pn);
+        stageA(pn);
     }
+    drainB(true);
+    drainC(true);
     for (int o = 32; o > 0; o >>= 1) {
         inSphere += __shfl_xor(inSphere, o, 64);
         nPairs += __shfl_xor(nPairs, o, 64);
@@ -1155,6 +1304,9 @@ int vote_keypoints(Workspace &ws, const DevModel &m, const VoteKeypoints &in, fl
     a.xcdRemap   = cfg.xcdRemap;
     a.kpSel      = nullptr;
     a.countPairs = 1;
+    a.filter     = m.filterA ? m.keyFilter.get<uint32_t>() : nullptr;
+    a.fA         = m.filterA;
+    a.fD         = m.filterD;
     a.hitCap     = 0;
     a.hitStart = a.hitLen = a.hitBucket = nullptr;
     a.hitAlpha = nullptr;
