@@ -78,7 +78,14 @@ struct GridView {
     const uint32_t *occ;
     int             odx, ody, occShift;
     int             unused;  // no padding bytes: views are compared bytewise as graph keys
+    // grids of at most GRID_DENSE_MAX cells: the first sorted slot of every
+    // cell (linear key), cstart[key + 1] - cstart[key] points; neighbouring
+    // cells of a row are then neighbouring words instead of scattered table
+    // slots.  nullptr: only the hash table.
+    const int      *cstart;
 };
+
+constexpr size_t GRID_DENSE_MAX = size_t(1) << 24;
 
 // occupancy bitmaps up to this many bits use one bit per cell; larger grids
 // one bit per 2x2x2 or 4x4x4 block
@@ -108,6 +115,12 @@ __device__ __forceinline__ uint32_t cell_hash(uint32_t key) {
 }
 __device__ __forceinline__ bool cell_lookup(const GridView &g, int cx, int cy, int cz, int &start,
                                             int &count) {
+    if (g.cstart) {
+        const uint32_t c = (uint32_t)((cz * g.dy + cy) * g.dx + cx);
+        start            = g.cstart[c];
+        count            = g.cstart[c + 1] - start;
+        return count > 0;
+    }
     uint32_t key = (uint32_t)((cz * g.dy + cy) * g.dx + cx) + 1u;
     uint32_t h   = cell_hash(key) & g.hmask;
     for (;;) {
@@ -186,9 +199,12 @@ struct KnnList {
 };
 
 // Visit the cells of Chebyshev ring r around (c0x,c0y,c0z), calling f(start,count)
-// for occupied cells whose box is within bound2 of q.  One loop with a single
-// call site: rows off the shell's z/y faces step from the low x end straight
-// to the high x end.
+// for occupied cells whose box is within bound2 of q.  Row by row: a row on
+// the shell's z/y faces is visited over its whole x range, a row off them at
+// its two ends.  With a per-cell occupancy bitmap (occShift 0) a row's cells
+// are read 32 at a time from the bitmap words and only the occupied ones
+// are visited, so empty space costs one load per word rather than work per
+// cell.  A single call site of f keeps the register use down.
 template <class F>
 __device__ __forceinline__ void visit_ring(const GridView &g, int c0x, int c0y, int c0z, int r,
                                            float qx, float qy, float qz, float bound2, F &&f) {
@@ -197,29 +213,47 @@ __device__ __forceinline__ void visit_ring(const GridView &g, int c0x, int c0y, 
     const int xlo = max(c0x - r, 0), xhi = min(c0x + r, g.dx - 1);
     if (zlo > zhi || ylo > yhi || xlo > xhi)
         return;
-    int cz = zlo, cy = ylo, cx = xlo;
-    for (;;) {
-        const bool face = (cz == c0z - r) || (cz == c0z + r) || (cy == c0y - r) || (cy == c0y + r);
-        // on a row off the z/y faces only the cells x = c0x -+ r are on the shell
-        const bool on = face || cx == c0x - r || cx == c0x + r;
-        if (on && !(cell_min_d2(g, cx, cy, cz, qx, qy, qz) > bound2) &&
-            !coarse_empty(g, cx, cy, cz)) {
-            int s0, c0;
-            if (cell_lookup(g, cx, cy, cz, s0, c0))
-                f(s0, c0);
-        }
-        // next cell (cx >= xlo >= c0x - r always); off the faces, jump from
-        // the low end (or a clamped interior start) straight to the high end
-        cx = (!face && cx < c0x + r) ? c0x + r : cx + 1;
-        if (cx > xhi) {
-            cx = xlo;
-            if (++cy > yhi) {
-                cy = ylo;
-                if (++cz > zhi)
-                    break;
+    for (int cz = zlo; cz <= zhi; cz++)
+        for (int cy = ylo; cy <= yhi; cy++) {
+            const bool face = (cz == c0z - r) || (cz == c0z + r) || (cy == c0y - r) ||
+                              (cy == c0y + r);
+            const uint32_t rowBase = ((uint32_t)cz * (uint32_t)g.ody + (uint32_t)cy) *
+                                     (uint32_t)g.odx;
+            for (int seg = 0; seg < (face ? 1 : 2); seg++) {
+                // the segment [a0, a1] of this row: the whole row on a face,
+                // else one end of it
+                int a0 = xlo, a1 = xhi;
+                if (!face) {
+                    a0 = a1 = seg == 0 ? c0x - r : c0x + r;
+                    if (a0 < xlo || a0 > xhi)
+                        continue;
+                }
+                for (int x = a0; x <= a1;) {
+                    uint32_t bits;
+                    const int x0 = x;
+                    if (g.occShift == 0) {
+                        const uint32_t b   = rowBase + (uint32_t)x;
+                        const int      off = (int)(b & 31u);
+                        const int      n   = min(32 - off, a1 - x + 1);
+                        bits               = g.occ[b >> 5] >> off;
+                        bits &= n == 32 ? ~0u : ((1u << n) - 1u);
+                        x += n;
+                    } else {
+                        bits = coarse_empty(g, x, cy, cz) ? 0u : 1u;
+                        x += 1;
+                    }
+                    while (bits) {
+                        const int cx = x0 + __ffs(bits) - 1;
+                        bits &= bits - 1u;
+                        if (cell_min_d2(g, cx, cy, cz, qx, qy, qz) > bound2)
+                            continue;
+                        int s0, c0;
+                        if (cell_lookup(g, cx, cy, cz, s0, c0))
+                            f(s0, c0);
+                    }
+                }
             }
         }
-    }
 }
 
 // distance from q to the boundary of the (2r+1)^3 block of cells around c0;
@@ -433,7 +467,7 @@ __device__ inline bool grid_any_within(const GridView &g, float qx, float qy, fl
 // host side of the grid
 // --------------------------------------------------------------------------
 struct Grid {
-    DevBuf   keys, keys2, vals, vals2, htab, x, y, z, nx, ny, nz, tmp, bbox, occ;
+    DevBuf   keys, keys2, vals, vals2, htab, x, y, z, nx, ny, nz, tmp, bbox, occ, cstart;
     GridView view{};
     int      capN = 0;
 };

@@ -115,6 +115,23 @@ __global__ void grid_occ_kernel(const uint32_t *keys, int m, int dx, int dy, int
     atomicOr(&occ[b >> 5], 1u << (b & 31u));
 }
 
+// first sorted slot of every cell c in [0, P]: the lower bound of c in the
+// sorted keys (P: m)
+__global__ void grid_cstart_kernel(const uint32_t *keys, int m, uint32_t P, int *cstart) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c > P)
+        return;
+    int lo = 0, hi = m;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (keys[mid] < c)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    cstart[c] = lo;
+}
+
 __global__ void grid_insert_kernel(const uint32_t *keys, int m, uint4 *htab, uint32_t hmask) {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= m)
@@ -192,7 +209,10 @@ void grid_build(Grid &g, const float *X, const float *Y, const float *Z, const f
     float    *x     = g.x.as<float>(std::max(m, 1));
     float    *y     = g.y.as<float>(std::max(m, 1));
     float    *z     = g.z.as<float>(std::max(m, 1));
-    PPF_HIP_CHECK(hipMemsetAsync(htab, 0, sizeof(uint4) * hsize, s));
+    const size_t cells = (size_t)dims[0] * dims[1] * dims[2];
+    const bool   dense = cells <= GRID_DENSE_MAX;  // cell lookups by a dense start array
+    if (!dense)
+        PPF_HIP_CHECK(hipMemsetAsync(htab, 0, sizeof(uint4) * hsize, s));
     int sh = 0;
     while (sh < 2 && ((size_t)(((dims[0] - 1) >> sh) + 1) * (size_t)(((dims[1] - 1) >> sh) + 1) *
                       (size_t)(((dims[2] - 1) >> sh) + 1)) > OCC_MAX_BITS)
@@ -201,6 +221,7 @@ void grid_build(Grid &g, const float *X, const float *Y, const float *Z, const f
               odz = ((dims[2] - 1) >> sh) + 1;
     const size_t occWords = ((size_t)odx * ody * odz + 31) / 32;
     uint32_t    *occ      = g.occ.as<uint32_t>(occWords);
+    int         *cstart   = nullptr;
     PPF_HIP_CHECK(hipMemsetAsync(occ, 0, sizeof(uint32_t) * occWords, s));
     if (m > 0) {
         grid_keys_kernel<<<div_up(m, 256), 256, 0, s>>>(X, Y, Z, subset, m, hb[0], hb[1], hb[2],
@@ -215,10 +236,16 @@ void grid_build(Grid &g, const float *X, const float *Y, const float *Z, const f
         void *tmp = g.tmp.reserve(tb);
         PPF_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(tmp, tb, keys, keys2, vals, vals2, m, 0,
                                                          endBit, s));
-        grid_insert_kernel<<<div_up(m, 256), 256, 0, s>>>(keys2, m, htab, hsize - 1);
+        if (dense) {
+            cstart = g.cstart.as<int>(cells + 1);
+            grid_cstart_kernel<<<div_up((int)cells + 1, 256), 256, 0, s>>>(keys2, m,
+                                                                            (uint32_t)cells, cstart);
+        } else {
+            grid_insert_kernel<<<div_up(m, 256), 256, 0, s>>>(keys2, m, htab, hsize - 1);
+            grid_count_kernel<<<div_up(m, 256), 256, 0, s>>>(keys2, m, htab, hsize - 1);
+        }
         grid_occ_kernel<<<div_up(m, 256), 256, 0, s>>>(keys2, m, dims[0], dims[1], odx, ody, sh,
                                                         occ);
-        grid_count_kernel<<<div_up(m, 256), 256, 0, s>>>(keys2, m, htab, hsize - 1);
         gather3_kernel<<<div_up(m, 256), 256, 0, s>>>(vals2, m, X, Y, Z, x, y, z);
     }
     GridView &v = g.view;
@@ -240,6 +267,7 @@ void grid_build(Grid &g, const float *X, const float *Y, const float *Z, const f
     v.odx = odx;
     v.ody = ody;
     v.occShift = sh;
+    v.cstart   = cstart;
     v.nx = v.ny = v.nz = nullptr;
     if (NX && m > 0) {
         float *nx = g.nx.as<float>(m), *ny = g.ny.as<float>(m), *nz = g.nz.as<float>(m);
