@@ -70,6 +70,28 @@ def hits_algorithmic_bytes(st: dict) -> float:
     return 12.0 * st["nKeyHits"] + 8.0 * st["nVotes"] + 24.0 * st["nVotedKeypoints"]
 
 
+def measured_ceilings() -> dict:
+    """This GPU's streaming-copy bandwidth and LDS atomic rate, measured by
+    the library's own microbenchmarks (the roofline's peak stays the
+    datasheet's 8 TB/s)."""
+    import ctypes as C
+    from surface_match_amd import _native as N
+    gbps, ats = C.c_float(0), C.c_float(0)
+    N.check(N.lib().ppf_measure_ceilings(C.byref(gbps), C.byref(ats)))
+    return {"copy_GBps": round(gbps.value, 1), "lds_atomics_per_s": float("%.4g" % ats.value)}
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(w, threads: int):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
@@ -290,7 +312,9 @@ def main():
                       "msIcpSparse", "msIcpDense", "msTotal")},
         "vote_fallback_keypoints": int(st["nVoteFallback"]),
         "vote_throughput": {"keypoints_per_s": round(st["nKeypoints"] / (vote_ms * 1e-3)),
-                            "votes_per_s": round(st["nVotes"] / (vote_ms * 1e-3))},
+                            "votes_per_s": round(st["nVotes"] / (vote_ms * 1e-3)),
+                            # one LDS atomic add per vote (the count slot is derived)
+                            "lds_atomics_per_s": round(st["nVotes"] / ((hits_ms or vote_ms) * 1e-3))},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": measured_traffic(kname),
@@ -298,6 +322,7 @@ def main():
                      "vote_stage": {"algorithmic_bytes": vote_algorithmic_bytes(st),
                                     "ms": round(vote_ms, 4),
                                     "GBps": round(vote_algorithmic_bytes(st) / (vote_ms * 1e-3) / 1e9, 1)}},
+        "measured_ceilings": measured_ceilings(),
         "results": found,
         "tuning": det.get_tuning(),
     }
@@ -311,7 +336,12 @@ def main():
         out["cpu_baseline"] = {"value": round(cms, 1), "unit": "ms", "cores": threads,
                                "kind": "port",
                                "sample": "the full workload, one matchScene by the C oracle "
-                                         "(OpenMP, like the reference's Detector)"}
+                                         "(OpenMP over min(cores, 8) threads, like the "
+                                         "reference's Detector)",
+                               "cpu_model": cpu_model(), "host_cpus": os.cpu_count(),
+                               "stage_ms": {k: round(float(cst.timeMs[i]), 1) for i, k in
+                                            enumerate(("sample_and_normals", "vote", "cluster",
+                                                       "refine"))}}
     print(json.dumps(out))
 
 

@@ -292,6 +292,12 @@ PPF_API int ppf_icp_adjoint(ppf_icp_target t, const float *src, size_t nSrc, con
  * 7 double sqrt((double)a*b) rounded to float bits of the double's high word */
 PPF_API int ppf_device_math(int op, const float *a, const float *b, size_t n, uint32_t *out);
 
+/* measured ceilings of this GPU for the rooflines (no reference
+ * counterpart): a 1 GiB device-to-device streaming copy (bytes read +
+ * written per second, best of 4) and LDS atomic adds at pseudo-random
+ * addresses over 4096 workgroups of 1024 threads (best of 3) */
+PPF_API int ppf_measure_ceilings(float *copyGBps, float *ldsAtomicsPerSec);
+
 #ifdef __cplusplus
 }
 #endif

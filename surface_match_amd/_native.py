@@ -97,7 +97,7 @@ EXPORTS = [
     "ppf_radius_search", "ppf_knn_search", "ppf_estimate_normals", "ppf_device_math",
     "ppf_icp_target_create", "ppf_icp_target_destroy", "ppf_icp_refine", "ppf_icp_adjoint",
     "ppf_autotune", "ppf_set_tuning", "ppf_get_tuning", "ppf_match_scenes_resident",
-    "ppf_scene_inliers",
+    "ppf_scene_inliers", "ppf_measure_ceilings",
 ]
 
 
@@ -147,6 +147,7 @@ def lib() -> C.CDLL:
         "ppf_match_scenes_resident": (i, [vp, vp, sz, f, f, f, C.POINTER(MatchParam), vp, vp, sz,
                                           vp, C.POINTER(MatchStats)]),
         "ppf_scene_inliers": (i, [vp, vp, vp, sz, i, f, vp]),
+        "ppf_measure_ceilings": (i, [C.POINTER(f), C.POINTER(f)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

@@ -1196,6 +1196,32 @@ __global__ void knn_kernel(GridView g, const float *q, int nq, int k, int *ids, 
     }
 }
 
+// ceilings of this GPU for the roofline: a streaming copy (16-byte loads and
+// stores, grid-stride) and LDS atomic adds at pseudo-random addresses
+__global__ void ceiling_copy_kernel(const uint4 *src, uint4 *dst, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+__global__ void __launch_bounds__(1024) ceiling_lds_kernel(int iters, uint32_t *sink) {
+    __shared__ uint32_t acc[16384];
+    for (int i = threadIdx.x; i < 16384; i += blockDim.x)
+        acc[i] = 0u;
+    __syncthreads();
+    uint32_t x = (blockIdx.x * 1024u + threadIdx.x) * 2654435761u + 1u;
+    for (int it = 0; it < iters; it++) {
+        x = x * 1664525u + 1013904223u;  // Numerical Recipes LCG
+        atomicAdd(&acc[x >> 18], 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t s = 0;
+        for (int i = 0; i < 16384; i += 997)
+            s += acc[i];
+        atomicAdd(sink, s);
+    }
+}
+
 __global__ void math_kernel(int op, const float *a, const float *b, int n, uint32_t *out) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n)
@@ -1965,6 +1991,45 @@ PPF_API int ppf_device_math(int op, const float *a, const float *b, size_t n, ui
             math_kernel<<<div_up(n, 256), 256, 0, s>>>(op, A, B, (int)n, O);
         PPF_HIP_CHECK(hipMemcpyAsync(out, O, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, s));
         PPF_HIP_CHECK(hipStreamSynchronize(s));
+    });
+}
+
+PPF_API int ppf_measure_ceilings(float *copyGBps, float *ldsAtomicsPerSec) {
+    PPF_REQUIRE(copyGBps && ldsAtomicsPerSec, "null argument");
+    return guarded([&] {
+        hipStream_t s = util_stream();
+        hipEvent_t  e0, e1;
+        PPF_HIP_CHECK(hipEventCreate(&e0));
+        PPF_HIP_CHECK(hipEventCreate(&e1));
+        const size_t bytes = size_t(1) << 30;
+        DevBuf       a, b, sink;
+        uint4       *A = a.as<uint4>(bytes / 16), *B = b.as<uint4>(bytes / 16);
+        PPF_HIP_CHECK(hipMemsetAsync(A, 1, bytes, s));
+        float best = 1e30f, ms = 0.0f;
+        for (int rep = 0; rep < 4; rep++) {
+            PPF_HIP_CHECK(hipEventRecord(e0, s));
+            ceiling_copy_kernel<<<8192, 256, 0, s>>>(A, B, bytes / 16);
+            PPF_HIP_CHECK(hipEventRecord(e1, s));
+            PPF_HIP_CHECK(hipEventSynchronize(e1));
+            PPF_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+            best = std::min(best, ms);
+        }
+        *copyGBps = (float)(2.0 * (double)bytes / (best * 1e-3) / 1e9);
+        uint32_t *S = sink.as<uint32_t>(1);
+        PPF_HIP_CHECK(hipMemsetAsync(S, 0, 4, s));
+        const int blocks = 4096, iters = 4096;
+        best             = 1e30f;
+        for (int rep = 0; rep < 3; rep++) {
+            PPF_HIP_CHECK(hipEventRecord(e0, s));
+            ceiling_lds_kernel<<<blocks, 1024, 0, s>>>(iters, S);
+            PPF_HIP_CHECK(hipEventRecord(e1, s));
+            PPF_HIP_CHECK(hipEventSynchronize(e1));
+            PPF_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+            best = std::min(best, ms);
+        }
+        *ldsAtomicsPerSec = (float)((double)blocks * 1024.0 * iters / (best * 1e-3));
+        PPF_HIP_CHECK(hipEventDestroy(e0));
+        PPF_HIP_CHECK(hipEventDestroy(e1));
     });
 }
 
