@@ -41,6 +41,7 @@ enum {
 
 typedef struct ppf_detector_s *ppf_detector;
 typedef struct ppf_scene_s    *ppf_scene;
+typedef struct ppf_ply_s      *ppf_ply;
 
 /* TrainParam, type.h:39-74 (smoothNormal is the reference's bool). */
 typedef struct {
@@ -291,6 +292,29 @@ PPF_API int ppf_icp_adjoint(ppf_icp_target t, const float *src, size_t nSrc, con
  * (a,b bit patterns as words 0,1 and 2,3 = 0), 5 quant ceil(a/b), 6 quant trunc,
  * 7 double sqrt((double)a*b) rounded to float bits of the double's high word */
 PPF_API int ppf_device_math(int op, const float *a, const float *b, size_t n, uint32_t *out);
+
+/* -- PLY files (filePLY.h:6-10).  ppf_ply_read replaces readPLY
+ * (filePLY.cpp:199-257): vertex x/y/z, vertex nx/ny/nz when nx is present,
+ * and the polygons of face "vertex_indices" (else "vertex_index") split
+ * into triangles by ear clipping (filePLY.cpp:45-107); ascii and binary
+ * little / big endian, every scalar type.  The counts come from
+ * ppf_ply_info, the data (n x 3 floats, faces x 3 ints) from ppf_ply_copy.
+ * Failures return PPF_ERR_RUNTIME with readPLY's message (the reference
+ * prints it and returns false). -- */
+PPF_API int  ppf_ply_read(const char *path, ppf_ply *out);
+PPF_API int  ppf_ply_info(ppf_ply p, size_t *nPoints, size_t *nNormals, size_t *nFaces);
+PPF_API int  ppf_ply_copy(ppf_ply p, float *points, float *normals, int *faces);
+PPF_API void ppf_ply_free(ppf_ply p);
+/* writePLY (filePLY.cpp:259-333): binary little endian (ascii != 0: ascii),
+ * "comment Created by Open3D", float x/y/z (and nx/ny/nz when normals is
+ * given), triangles as uchar-counted uint lists. */
+PPF_API int ppf_ply_write(const char *path, const float *points, const float *normals, size_t n,
+                          const int *faces, size_t nFaces, int ascii);
+/* a PLY file read and uploaded as a resident scene (no reference
+ * counterpart: readPLY followed by ppf_scene_upload); normals are used when
+ * the file has one per point (type.cpp:36-38) */
+PPF_API int ppf_scene_upload_ply(ppf_detector d, const char *path, const float *viewPoint,
+                                 ppf_scene *out);
 
 /* measured ceilings of this GPU for the rooflines (no reference
  * counterpart): a 1 GiB device-to-device streaming copy (bytes read +

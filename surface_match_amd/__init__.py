@@ -41,6 +41,7 @@ class PointCloud:
     present when their count equals the point count (type.cpp:36-38)."""
     point: np.ndarray = field(default_factory=lambda: np.zeros((0, 3), np.float32))
     normal: np.ndarray = field(default_factory=lambda: np.zeros((0, 3), np.float32))
+    face: np.ndarray = field(default_factory=lambda: np.zeros((0, 3), np.int32))
     box: BoundingBox = field(default_factory=BoundingBox)
     viewPoint: np.ndarray = field(default_factory=lambda: np.full(3, np.nan, np.float32))
 
@@ -83,6 +84,41 @@ def MatchParam(numMatches=1, knnNormal=10, smoothNormal=True, invertNormal=False
                      poseRefDistThresholdAbs, poseRefScoringDistRel, poseRefScoringDistAbs)
     N.check(N.lib().ppf_match_param_check(C.byref(p)))
     return p
+
+
+def readPLY(filename: str, mesh: PointCloud) -> bool:
+    """filePLY.h:6: fills mesh.point / normal / face from a PLY file; False
+    (after printing the reason, as the reference does) when it cannot."""
+    h = C.c_void_p()
+    rc = N.lib().ppf_ply_read(str(filename).encode(), C.byref(h))
+    if rc != 0:
+        print(N.lib().ppf_last_error().decode())
+        return False
+    try:
+        nv, nn, nf = C.c_size_t(), C.c_size_t(), C.c_size_t()
+        N.check(N.lib().ppf_ply_info(h, C.byref(nv), C.byref(nn), C.byref(nf)))
+        pts = np.zeros((nv.value, 3), np.float32)
+        nrm = np.zeros((nn.value, 3), np.float32)
+        fac = np.zeros((nf.value, 3), np.int32)
+        N.check(N.lib().ppf_ply_copy(h, N.ptr(pts), N.ptr(nrm), N.ptr(fac)))
+    finally:
+        N.lib().ppf_ply_free(h)
+    mesh.point, mesh.normal, mesh.face = pts, nrm, fac
+    return True
+
+
+def writePLY(filename: str, mesh: PointCloud, write_ascii: bool = False) -> bool:
+    """filePLY.h:8-9: binary little endian (or ASCII) PLY of the points,
+    normals when present, and triangles."""
+    p = N.f32(np.reshape(mesh.point, (-1, 3)))
+    n = N.f32(np.reshape(mesh.normal, (-1, 3))) if mesh.hasNormal() else None
+    f = N.i32(np.reshape(mesh.face, (-1, 3)))
+    rc = N.lib().ppf_ply_write(str(filename).encode(), N.ptr(p), N.ptr(n), len(p), N.ptr(f),
+                               len(f), int(bool(write_ascii)))
+    if rc != 0:
+        print(N.lib().ppf_last_error().decode())
+        return False
+    return True
 
 
 @dataclass
@@ -213,11 +249,17 @@ class Detector:
 class ResidentScene:
     """A scene kept in HBM for repeated matches (ppf_scene_upload)."""
 
-    def __init__(self, det: Detector, scene: PointCloud):
-        p, n, box, vp = scene._c_args()
+    def __init__(self, det: Detector, scene: PointCloud | None = None, ply: str | None = None,
+                 viewPoint=None):
         h = C.c_void_p()
-        N.check(N.lib().ppf_scene_upload(det._h, N.ptr(p), N.ptr(n), len(p), N.ptr(box),
-                                         N.ptr(vp), C.byref(h)))
+        if ply is not None:
+            vp = None if viewPoint is None else N.f32(np.asarray(viewPoint, np.float32).ravel())
+            N.check(N.lib().ppf_scene_upload_ply(det._h, str(ply).encode(), N.ptr(vp),
+                                                 C.byref(h)))
+        else:
+            p, n, box, vp = scene._c_args()
+            N.check(N.lib().ppf_scene_upload(det._h, N.ptr(p), N.ptr(n), len(p), N.ptr(box),
+                                             N.ptr(vp), C.byref(h)))
         self._h = h
         self._det = det
 

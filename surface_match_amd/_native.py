@@ -97,7 +97,8 @@ EXPORTS = [
     "ppf_radius_search", "ppf_knn_search", "ppf_estimate_normals", "ppf_device_math",
     "ppf_icp_target_create", "ppf_icp_target_destroy", "ppf_icp_refine", "ppf_icp_adjoint",
     "ppf_autotune", "ppf_set_tuning", "ppf_get_tuning", "ppf_match_scenes_resident",
-    "ppf_scene_inliers", "ppf_measure_ceilings",
+    "ppf_scene_inliers", "ppf_measure_ceilings", "ppf_ply_read", "ppf_ply_info", "ppf_ply_copy",
+    "ppf_ply_free", "ppf_ply_write", "ppf_scene_upload_ply",
 ]
 
 
@@ -148,6 +149,12 @@ def lib() -> C.CDLL:
                                           vp, C.POINTER(MatchStats)]),
         "ppf_scene_inliers": (i, [vp, vp, vp, sz, i, f, vp]),
         "ppf_measure_ceilings": (i, [C.POINTER(f), C.POINTER(f)]),
+        "ppf_ply_read": (i, [C.c_char_p, C.POINTER(vp)]),
+        "ppf_ply_info": (i, [vp, C.POINTER(sz), C.POINTER(sz), C.POINTER(sz)]),
+        "ppf_ply_copy": (i, [vp, vp, vp, vp]),
+        "ppf_ply_free": (None, [vp]),
+        "ppf_ply_write": (i, [C.c_char_p, vp, vp, sz, vp, sz, i]),
+        "ppf_scene_upload_ply": (i, [vp, C.c_char_p, vp, C.POINTER(vp)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

@@ -1994,6 +1994,66 @@ PPF_API int ppf_device_math(int op, const float *a, const float *b, size_t n, ui
     });
 }
 
+struct ppf_ply_s {
+    PlyCloud c;
+};
+
+PPF_API int ppf_ply_read(const char *path, ppf_ply *out) {
+    PPF_REQUIRE(path && out, "null argument");
+    *out = nullptr;
+    return guarded([&] {
+        auto p = std::make_unique<ppf_ply_s>();
+        ply_read(path, p->c);
+        *out = p.release();
+    });
+}
+
+PPF_API int ppf_ply_info(ppf_ply p, size_t *nPoints, size_t *nNormals, size_t *nFaces) {
+    PPF_REQUIRE(p, "null argument");
+    if (nPoints)
+        *nPoints = p->c.points.size() / 3;
+    if (nNormals)
+        *nNormals = p->c.normals.size() / 3;
+    if (nFaces)
+        *nFaces = p->c.faces.size() / 3;
+    return PPF_OK;
+}
+
+PPF_API int ppf_ply_copy(ppf_ply p, float *points, float *normals, int *faces) {
+    PPF_REQUIRE(p, "null argument");
+    if (points && !p->c.points.empty())
+        std::memcpy(points, p->c.points.data(), sizeof(float) * p->c.points.size());
+    if (normals && !p->c.normals.empty())
+        std::memcpy(normals, p->c.normals.data(), sizeof(float) * p->c.normals.size());
+    if (faces && !p->c.faces.empty())
+        std::memcpy(faces, p->c.faces.data(), sizeof(int) * p->c.faces.size());
+    return PPF_OK;
+}
+
+PPF_API void ppf_ply_free(ppf_ply p) { delete p; }
+
+PPF_API int ppf_ply_write(const char *path, const float *points, const float *normals, size_t n,
+                          const int *faces, size_t nFaces, int ascii) {
+    PPF_REQUIRE(path && (points || n == 0) && (faces || nFaces == 0), "null argument");
+    return guarded([&] { ply_write(path, points, normals, n, faces, nFaces, ascii != 0); });
+}
+
+PPF_API int ppf_scene_upload_ply(ppf_detector d, const char *path, const float *viewPoint,
+                                 ppf_scene *out) {
+    PPF_REQUIRE(d && path && out, "null argument");
+    *out = nullptr;
+    return guarded([&] {
+        PlyCloud c;
+        ply_read(path, c);
+        const size_t n  = c.points.size() / 3;
+        const bool   hn = c.normals.size() == c.points.size();  // type.cpp:36-38
+        auto         sc = std::make_unique<ppf_scene_s>();
+        sc->cloud.upload(c.points.data(), hn ? c.normals.data() : nullptr, (int)n, nullptr,
+                         viewPoint, d->stream);
+        *out = sc.release();
+    });
+}
+
 PPF_API int ppf_measure_ceilings(float *copyGBps, float *ldsAtomicsPerSec) {
     PPF_REQUIRE(copyGBps && ldsAtomicsPerSec, "null argument");
     return guarded([&] {

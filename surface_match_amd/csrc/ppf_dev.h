@@ -238,4 +238,17 @@ void icp_adjoint(Workspace &ws, int nSrc, const float *sx, const float *sy, cons
                  const GridView &dst, const float *T0host, const double *xiHost, int H,
                  float rejectDist, int center, double *outHost, int *corrHost, hipStream_t s);
 
+// ---------------------------------------------------------------- PLY
+
+// a PLY file's points and normals (n x 3 floats) and triangles (3 ints each)
+struct PlyCloud {
+    std::vector<float> points, normals;
+    std::vector<int>   faces;
+};
+// readPLY / writePLY (filePLY.cpp:199-333); throw std::runtime_error with
+// the reference's messages
+void ply_read(const char *path, PlyCloud &out);
+void ply_write(const char *path, const float *pts, const float *nrm, size_t n, const int *faces,
+               size_t nFaces, bool ascii);
+
 }  // namespace ppfd

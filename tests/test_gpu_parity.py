@@ -348,3 +348,19 @@ def test_batched_scenes_equal_individual(trained):
         total += st1["nKeypoints"]
     assert st["nKeypoints"] == total
     print("batched scenes", [len(S) for _, S in many], st["msTotal"])
+
+
+def test_scene_from_ply_equals_scene_from_arrays(trained, workload1, tmp_path):
+    """A scene read from a PLY file (ppf_scene_upload_ply) matches exactly as
+    the same points handed over as arrays."""
+    from surface_match_amd import PointCloud, MatchParam, ResidentScene, synth, writePLY
+    pts = workload1["scene"].points
+    path = tmp_path / "scene.ply"
+    assert writePLY(str(path), PointCloud(pts))
+    cloud = PointCloud(pts)
+    cloud.viewPoint = np.array(synth.VIEWPOINT, np.float32)
+    param = MatchParam(numMatches=3)
+    P1, S1, _ = ResidentScene(trained, cloud).match(0.04, 0.1, 0.2, param)
+    P2, S2, _ = ResidentScene(trained, ply=str(path), viewPoint=synth.VIEWPOINT).match(
+        0.04, 0.1, 0.2, param)
+    assert np.array_equal(P1, P2) and np.array_equal(S1, S2)

@@ -1,0 +1,242 @@
+"""PLY input/output (filePLY.cpp) against the reference's own PLY code path:
+readPLY / writePLY restated over the reference's rply library, built from
+the reference tree into oracle/_ref (`make -C oracle ref`).  CPU only.
+
+Files: the sample that ships with rply (tests/golden/rply_input.ply, ASCII
+with CRLF line endings and leading zeros) and files written here in every
+storage format and property type, with extra properties, comments,
+obj_info lines, polygons and broken contents."""
+import ctypes as C
+import os
+import struct
+
+import numpy as np
+import pytest
+
+from conftest import rng
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+REF = os.path.join(ROOT, "oracle", "_ref", "libply_ref.so")
+GOLD = os.path.join(ROOT, "tests", "golden", "rply_input.ply")
+
+
+@pytest.fixture(scope="module")
+def R():
+    if not os.path.exists(REF):
+        pytest.skip("oracle/_ref not built")
+    L = C.CDLL(REF)
+    L.plyref_read.argtypes = [C.c_char_p] + [C.POINTER(C.c_long)] * 3 + \
+        [C.POINTER(C.POINTER(C.c_float))] * 2 + [C.POINTER(C.POINTER(C.c_int))]
+    L.plyref_free.argtypes = [C.c_void_p]
+    L.plyref_write.argtypes = [C.c_char_p, C.c_void_p, C.c_void_p, C.c_long, C.c_void_p,
+                               C.c_long, C.c_int]
+    return L
+
+
+def ref_read(R, path):
+    nv, nn, nt = C.c_long(), C.c_long(), C.c_long()
+    p, n = C.POINTER(C.c_float)(), C.POINTER(C.c_float)()
+    t = C.POINTER(C.c_int)()
+    if not R.plyref_read(str(path).encode(), C.byref(nv), C.byref(nn), C.byref(nt), C.byref(p),
+                         C.byref(n), C.byref(t)):
+        return None
+    pts = np.ctypeslib.as_array(p, (nv.value, 3)).copy()
+    nrm = np.ctypeslib.as_array(n, (max(nn.value, 1), 3))[:nn.value].copy()
+    tri = np.ctypeslib.as_array(t, (nt.value, 3)).copy() if nt.value else np.zeros((0, 3), np.int32)
+    for a in (p, n, t):
+        R.plyref_free(C.cast(a, C.c_void_p))
+    return pts, nrm, tri
+
+
+def lib_read(path):
+    from surface_match_amd import PointCloud, readPLY
+    m = PointCloud()
+    if not readPLY(str(path), m):
+        return None
+    return m.point, m.normal, m.face
+
+
+def same(a, b):
+    if a is None or b is None:
+        return a is None and b is None
+    return all(x.shape == y.shape and np.array_equal(x.view(np.uint32) if x.dtype == np.float32
+                                                     else x, y.view(np.uint32)
+                                                     if y.dtype == np.float32 else y)
+               for x, y in zip(a, b))
+
+
+FMT = {"int8": "b", "char": "b", "uint8": "B", "uchar": "B", "int16": "h", "short": "h",
+       "uint16": "H", "ushort": "H", "int32": "i", "int": "i", "uint32": "I", "uint": "I",
+       "float32": "f", "float": "f", "float64": "d", "double": "d"}
+
+
+def write_ply(path, fmt, vprops, rows, faces=None, face_prop=("uchar", "int", "vertex_indices"),
+              eol="\n", extra_header=(), extra_elements=()):
+    """A PLY file: vprops = [(type, name)], rows = per-vertex value lists,
+    faces = list of index lists; ascii values are written with repr()."""
+    h = ["ply", f"format {fmt} 1.0", "comment written by the test"] + list(extra_header)
+    for name, count, props, _ in extra_elements:
+        h.append(f"element {name} {count}")
+        h += [f"property {t} {n}" for t, n in props]
+    h.append(f"element vertex {len(rows)}")
+    h += [f"property {t} {n}" for t, n in vprops]
+    if faces is not None:
+        h.append(f"element face {len(faces)}")
+        h.append("property list %s %s %s" % face_prop)
+    h.append("end_header")
+    head = eol.join(h) + eol
+    body = b""
+    end = "<" if fmt == "binary_little_endian" else ">"
+    elems = [(props, data) for _, _, props, data in extra_elements] + [(vprops, rows)]
+    for props, data in elems:
+        for r in data:
+            if fmt == "ascii":
+                body += (" ".join(repr(v) if isinstance(v, float) else str(v) for v in r) + eol).encode()
+            else:
+                body += b"".join(struct.pack(end + FMT[t], v) for (t, _), v in zip(props, r))
+    if faces is not None:
+        lt, vt, _ = face_prop
+        for f in faces:
+            if fmt == "ascii":
+                body += (" ".join(str(v) for v in [len(f)] + list(f)) + eol).encode()
+            else:
+                body += struct.pack(end + FMT[lt], len(f)) + b"".join(
+                    struct.pack(end + FMT[vt], v) for v in f)
+    with open(path, "wb") as fh:
+        fh.write(head.encode() + body)
+
+
+def cloud_rows(n, seed, normals=True):
+    r = rng(seed)
+    p = r.normal(scale=40.0, size=(n, 3))
+    q = r.normal(size=(n, 3))
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    return [list(map(float, p[i])) + (list(map(float, q[i])) if normals else []) for i in range(n)]
+
+
+def test_rply_sample_file(R):
+    a, b = lib_read(GOLD), ref_read(R, GOLD)
+    assert same(a, b)
+    assert a[0].tolist() == [[-1, 0, 0], [0, 1, 0], [1, 0, 0]] and a[1].shape == (0, 3)
+    assert a[2].tolist() == [[1, 0, 2]]
+
+
+@pytest.mark.parametrize("fmt", ["ascii", "binary_little_endian", "binary_big_endian"])
+@pytest.mark.parametrize("eol", ["\n", "\r\n"])
+def test_formats_types_and_extras(R, tmp_path, fmt, eol):
+    rows = cloud_rows(300, 11)
+    # coordinates as double, normals as float, and an interleaved colour and
+    # an int16 / uint32 property the reader must step over
+    vprops = [("double", "x"), ("uchar", "red"), ("double", "y"), ("double", "z"),
+              ("float", "nx"), ("short", "tag"), ("float", "ny"), ("float", "nz"),
+              ("uint", "id")]
+    r = rng(12)
+    full = [[x, int(r.integers(0, 256)), y, z, nx, int(r.integers(-30000, 30000)), ny, nz,
+             int(r.integers(0, 2**32 - 1))] for x, y, z, nx, ny, nz in rows]
+    if fmt == "ascii":
+        # float properties as text: short, exponent and zero-padded forms
+        for k, row in enumerate(full):
+            if k % 3 == 0:
+                row[4] = float("%.3e" % row[4])
+    faces = [list(r.choice(300, 3, replace=False)) for _ in range(40)]
+    extra = [("material", 2, [("float", "shine"), ("int", "kind")], [[0.5, 1], [1.5, 2]])]
+    path = tmp_path / "a.ply"
+    write_ply(path, fmt, vprops, full, faces, eol=eol, extra_elements=extra,
+              extra_header=("obj_info made by the test",))
+    a, b = lib_read(path), ref_read(R, path)
+    assert b is not None and same(a, b)
+    assert a[0].shape == (300, 3) and a[1].shape == (300, 3) and a[2].shape == (40, 3)
+
+
+@pytest.mark.parametrize("fmt", ["ascii", "binary_little_endian", "binary_big_endian"])
+def test_integer_coordinates_no_normals_and_vertex_index(R, tmp_path, fmt):
+    r = rng(13)
+    rows = [[int(v) for v in r.integers(-2000, 2000, 3)] for _ in range(64)]
+    for ty in ("int8", "uint16", "int32", "float64"):
+        rr = [[max(-128, min(127, v)) for v in row] for row in rows] if ty == "int8" else \
+             [[abs(v) for v in row] for row in rows] if ty == "uint16" else rows
+        if ty == "float64":
+            rr = [[v * 0.37 for v in row] for row in rows]
+        path = tmp_path / f"{ty}.ply"
+        write_ply(path, fmt, [(ty, "x"), (ty, "y"), (ty, "z")], rr, [[0, 1, 2], [3, 4, 5]],
+                  face_prop=("uint8", "uint32", "vertex_index"))
+        a, b = lib_read(path), ref_read(R, path)
+        assert b is not None and same(a, b), ty
+        assert a[1].shape == (0, 3)
+        # a random (non-planar) quad: both give up on the same polygons
+        write_ply(path, fmt, [(ty, "x"), (ty, "y"), (ty, "z")], rr,
+                  [[0, 1, 2], [3, 4, 5, 6], [7, 8, 9, 10]],
+                  face_prop=("uint8", "uint32", "vertex_index"))
+        a, b = lib_read(path), ref_read(R, path)
+        assert same(a, b), ty
+
+
+def test_polygons_split_by_ear_clipping(R, tmp_path):
+    """Polygons become triangles as AddTrianglesByEarClipping splits them:
+    a convex quad into two, a concave (L-shaped) hexagon into four that tile
+    it (areas add up to the polygon's shoelace area)."""
+    pts = [[0, 0, 0], [2, 0, 0], [2, 1, 0], [0, 1, 0],                       # quad
+           [0, 0, 1], [3, 0, 1], [3, 1, 1], [1, 1, 1], [1, 3, 1], [0, 3, 1]]  # L hexagon
+    rows = [list(map(float, p)) for p in pts]
+    faces = [[0, 1, 2, 3], [4, 5, 6, 7, 8, 9]]
+    for fmt in ("ascii", "binary_little_endian"):
+        path = tmp_path / f"poly_{fmt}.ply"
+        write_ply(path, fmt, [("float", "x"), ("float", "y"), ("float", "z")], rows, faces)
+        a, b = lib_read(path), ref_read(R, path)
+        assert same(a, b)
+        tri = a[2]
+        assert tri.tolist()[:2] == [[0, 1, 2], [0, 2, 3]]
+        P = np.array(pts, float)
+        area = sum(0.5 * abs(np.cross(P[t[1]] - P[t[0]], P[t[2]] - P[t[0]])[2]) for t in tri[2:])
+        assert len(tri) == 6 and abs(area - 5.0) < 1e-9
+
+
+@pytest.mark.parametrize("case", ["missing", "magic", "no_x", "truncated", "bad_token",
+                                  "range", "bad_type"])
+def test_broken_files_fail_on_both(R, tmp_path, case):
+    path = tmp_path / "bad.ply"
+    rows = cloud_rows(20, 14, normals=False)
+    vp = [("float", "x"), ("float", "y"), ("float", "z")]
+    if case == "magic":
+        path.write_bytes(b"plx\nformat ascii 1.0\nend_header\n")
+    elif case == "no_x":
+        write_ply(path, "ascii", [("float", "u"), ("float", "y"), ("float", "z")], rows)
+    elif case == "truncated":
+        write_ply(path, "binary_little_endian", vp, rows)
+        path.write_bytes(path.read_bytes()[:-5])
+    elif case == "bad_token":
+        write_ply(path, "ascii", vp, rows)
+        path.write_bytes(path.read_bytes().replace(b"end_header\n", b"end_header\n1.5x "))
+    elif case == "range":
+        write_ply(path, "ascii", [("uchar", "x"), ("float", "y"), ("float", "z")],
+                  [[300, 1.0, 2.0]])
+    elif case == "bad_type":
+        write_ply(path, "ascii", vp, rows)
+        path.write_bytes(path.read_bytes().replace(b"property float y", b"property flot y"))
+    assert ref_read(R, path) is None
+    assert lib_read(path) is None
+
+
+@pytest.mark.parametrize("ascii", [0, 1])
+@pytest.mark.parametrize("normals", [True, False])
+def test_writer_bytes_equal_reference(R, tmp_path, ascii, normals):
+    from surface_match_amd import PointCloud, writePLY
+    r = rng(15)
+    p = (r.normal(scale=50.0, size=(500, 3))).astype(np.float32)
+    n = r.normal(size=(500, 3)).astype(np.float32)
+    t = r.integers(0, 500, (70, 3)).astype(np.int32)
+    m = PointCloud(p, n if normals else np.zeros((0, 3), np.float32), t)
+    ours, theirs = tmp_path / "ours.ply", tmp_path / "theirs.ply"
+    assert writePLY(str(ours), m, bool(ascii))
+    assert R.plyref_write(str(theirs).encode(), p.ctypes.data, n.ctypes.data if normals else None,
+                          len(p), t.ctypes.data, len(t), ascii)
+    assert ours.read_bytes() == theirs.read_bytes()
+    back = lib_read(ours)
+    if not ascii:  # binary round trip is exact
+        assert np.array_equal(back[0], p) and np.array_equal(back[2], t)
+        assert np.array_equal(back[1], n if normals else np.zeros((0, 3), np.float32))
+
+
+def test_empty_mesh_is_not_written(tmp_path):
+    from surface_match_amd import PointCloud, writePLY
+    assert not writePLY(str(tmp_path / "e.ply"), PointCloud())
