@@ -77,6 +77,19 @@ struct PlyFile {
             pos++;  // the delimiter after the token
         return t;
     }
+    // the next token of the data, NUL-terminated in place (the buffer
+    // carries one spare NUL at its end); empty at the end of the file
+    const char *data_token() {
+        const size_t size = buf.size() - 1;
+        while (pos < size && blank(buf[pos]))
+            pos++;
+        const size_t b = pos;
+        while (pos < size && !blank(buf[pos]))
+            pos++;
+        if (pos < size)
+            buf[pos++] = '\0';
+        return buf.data() + b;
+    }
     // from the cursor (past the delimiter of the token just read) through
     // the next newline, as rply reads a comment line
     void skip_line() {
@@ -93,7 +106,7 @@ struct PlyFile {
 
 void read_header(PlyFile &f) {
     const std::vector<char> &b = f.buf;
-    if (b.size() < 4 || b[0] != 'p' || b[1] != 'l' || b[2] != 'y' ||
+    if (b.size() < 5 || b[0] != 'p' || b[1] != 'l' || b[2] != 'y' ||
         !(b[3] == ' ' || b[3] == '\n' || b[3] == '\r' || b[3] == '\t' || b[3] == '\v' ||
           b[3] == '\f'))
         ply_fail("unable to parse header.");
@@ -163,18 +176,18 @@ void read_header(PlyFile &f) {
 // one value of type t, as a double
 double read_value(PlyFile &f, PlyType t) {
     if (f.format == 0) {
-        const std::string tok = f.token();
-        if (tok.empty())
+        const char *tok = f.data_token();
+        if (!*tok)
             ply_fail("unable to read file.");
         char *end = nullptr;
         if (t == T_F32 || t == T_F64) {
-            const double v = std::strtod(tok.c_str(), &end);
+            const double v = std::strtod(tok, &end);
             const double m = t == T_F32 ? FLT_MAX : DBL_MAX;
             if (*end || v < -m || v > m)
                 ply_fail("unable to read file.");
             return v;
         }
-        const double v = (double)std::strtol(tok.c_str(), &end, 10);
+        const double v = (double)std::strtol(tok, &end, 10);
         static const double lo[] = {-128, 0, -32768, 0, -2147483648.0, 0};
         static const double hi[] = {127, 255, 32767, 65535, 2147483647.0, 4294967295.0};
         if (*end || v < lo[t] || v > hi[t])
@@ -182,7 +195,7 @@ double read_value(PlyFile &f, PlyType t) {
         return v;
     }
     const int n = ply_type_size(t);
-    if (f.pos + n > f.buf.size())
+    if (f.pos + n > f.buf.size() - 1)
         ply_fail("unable to read file.");
     unsigned char raw[8];
     std::memcpy(raw, f.buf.data() + f.pos, n);
@@ -297,6 +310,7 @@ void ply_read(const char *path, PlyCloud &out) {
         f.buf.resize(size > 0 ? (size_t)size : 0);
         if (size > 0 && std::fread(f.buf.data(), 1, (size_t)size, fp.get()) != (size_t)size)
             ply_fail(std::string("unable to read file: ") + path);
+        f.buf.push_back('\0');  // terminates the last ASCII token
     }
     read_header(f);
     // the properties readPLY asks for (filePLY.cpp:207-225)
@@ -327,9 +341,67 @@ void ply_read(const char *path, PlyCloud &out) {
             faceProp = has(&e, "vertex_indices") ? "vertex_indices"
                        : has(&e, "vertex_index") ? "vertex_index"
                                                  : nullptr;
+        // where each scalar property of the vertex element goes: 0-2 the
+        // point, 3-5 the normal, -1 nowhere
+        std::vector<int> slot(e.props.size(), -1);
+        bool             lists = false;
+        for (size_t q = 0; q < e.props.size(); q++) {
+            const std::string &nm = e.props[q].name;
+            lists                 = lists || e.props[q].type == T_LIST;
+            if (!isVertex || e.props[q].type == T_LIST)
+                continue;
+            if (nm == "x") slot[q] = 0;
+            else if (nm == "y") slot[q] = 1;
+            else if (nm == "z") slot[q] = 2;
+            else if (nn && nm == "nx") slot[q] = 3;
+            else if (nn && nm == "ny") slot[q] = 4;
+            else if (nn && nm == "nz") slot[q] = 5;
+        }
+        if (f.format != 0 && !lists) {
+            // binary rows of scalars: fixed stride, values converted in place
+            size_t stride = 0;
+            std::vector<size_t> off(e.props.size());
+            for (size_t q = 0; q < e.props.size(); q++) {
+                off[q] = stride;
+                stride += (size_t)ply_type_size(e.props[q].type);
+            }
+            if (f.pos + stride * (size_t)e.count > f.buf.size() - 1)
+                ply_fail("unable to read file.");
+            if (isVertex) {
+                for (long i = 0; i < e.count; i++) {
+                    const char *row = f.buf.data() + f.pos + stride * (size_t)i;
+                    for (size_t q = 0; q < e.props.size(); q++) {
+                        if (slot[q] < 0)
+                            continue;
+                        const int n = ply_type_size(e.props[q].type);
+                        unsigned char raw[8];
+                        std::memcpy(raw, row + off[q], n);
+                        if (f.format == 2)
+                            for (int b = 0; b < n / 2; b++)
+                                std::swap(raw[b], raw[n - 1 - b]);
+                        double v;
+                        switch (e.props[q].type) {
+                            case T_I8: { int8_t w; std::memcpy(&w, raw, 1); v = w; break; }
+                            case T_U8: { uint8_t w; std::memcpy(&w, raw, 1); v = w; break; }
+                            case T_I16: { int16_t w; std::memcpy(&w, raw, 2); v = w; break; }
+                            case T_U16: { uint16_t w; std::memcpy(&w, raw, 2); v = w; break; }
+                            case T_I32: { int32_t w; std::memcpy(&w, raw, 4); v = w; break; }
+                            case T_U32: { uint32_t w; std::memcpy(&w, raw, 4); v = w; break; }
+                            case T_F32: { float w; std::memcpy(&w, raw, 4); v = w; break; }
+                            default: { double w; std::memcpy(&w, raw, 8); v = w; break; }
+                        }
+                        (slot[q] < 3 ? out.points : out.normals)[3 * (size_t)i + slot[q] % 3] =
+                            (float)v;
+                    }
+                }
+            }
+            f.pos += stride * (size_t)e.count;
+            continue;
+        }
         std::vector<unsigned> poly;
         for (long i = 0; i < e.count; i++) {
-            for (const PlyProperty &p : e.props) {
+            for (size_t q = 0; q < e.props.size(); q++) {
+                const PlyProperty &p = e.props[q];
                 if (p.type == T_LIST) {
                     const double len  = read_value(f, p.lengthType);
                     const bool   keep = isFace && faceProp && p.name == faceProp;
@@ -354,15 +426,9 @@ void ply_read(const char *path, PlyCloud &out) {
                     continue;
                 }
                 const double v = read_value(f, p.type);
-                if (!isVertex)
-                    continue;
-                const char *nm = p.name.c_str();
-                if (!std::strcmp(nm, "x")) out.points[3 * i + 0] = (float)v;
-                else if (!std::strcmp(nm, "y")) out.points[3 * i + 1] = (float)v;
-                else if (!std::strcmp(nm, "z")) out.points[3 * i + 2] = (float)v;
-                else if (nn && !std::strcmp(nm, "nx")) out.normals[3 * i + 0] = (float)v;
-                else if (nn && !std::strcmp(nm, "ny")) out.normals[3 * i + 1] = (float)v;
-                else if (nn && !std::strcmp(nm, "nz")) out.normals[3 * i + 2] = (float)v;
+                if (slot[q] >= 0)
+                    (slot[q] < 3 ? out.points : out.normals)[3 * (size_t)i + slot[q] % 3] =
+                        (float)v;
             }
         }
     }

@@ -240,3 +240,26 @@ def test_writer_bytes_equal_reference(R, tmp_path, ascii, normals):
 def test_empty_mesh_is_not_written(tmp_path):
     from surface_match_amd import PointCloud, writePLY
     assert not writePLY(str(tmp_path / "e.ply"), PointCloud())
+
+
+def test_ingest_speed_against_rply(R, tmp_path):
+    """Throughput of the library's reader and of the rply-based reference
+    path on a 1M-point binary file with normals, and a 200k-point ASCII one
+    (reported; the results must agree)."""
+    import time
+    from surface_match_amd import PointCloud, writePLY
+    r = rng(16)
+    for n, ascii in ((1_000_000, False), (200_000, True)):
+        p = r.normal(scale=100.0, size=(n, 3)).astype(np.float32)
+        q = r.normal(size=(n, 3)).astype(np.float32)
+        path = tmp_path / f"big_{int(ascii)}.ply"
+        assert writePLY(str(path), PointCloud(p, q), ascii)
+        t0 = time.perf_counter()
+        a = lib_read(path)
+        t1 = time.perf_counter()
+        b = ref_read(R, path)
+        t2 = time.perf_counter()
+        assert same(a, b)
+        mb = path.stat().st_size / 1e6
+        print(f"{'ascii' if ascii else 'binary'} {n} points, {mb:.1f} MB: library "
+              f"{mb / (t1 - t0):.0f} MB/s, rply path {mb / (t2 - t1):.0f} MB/s")
