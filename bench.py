@@ -96,6 +96,49 @@ def ply_ingest(det, points) -> dict:
             "arrays_to_resident_ms": round(best_m * 1e3, 2)}
 
 
+MESH_GRID, MESH_RADIUS, MESH_SEED = 1001, 0.0023, 0x5EED
+
+
+def mesh_sampling() -> dict:
+    """sampleMesh on a 2M-triangle mesh (a 1001 x 1001 height field) at a
+    radius that yields about 1M points, host arrays in and out: one
+    ppf_sample_mesh call into buffers of the known size (copies in, area
+    pass, host CDF, sampling kernel, copies out; best of 3), and the Python
+    sampleMesh, which first asks for the size."""
+    import ctypes as C
+    from surface_match_amd import PointCloud, sampleMesh, synth, _native as N
+    P, F = synth.height_field_mesh(MESH_GRID, 5)
+    pc = PointCloud(P)
+    pc.face = F
+    t0 = time.perf_counter()
+    out = sampleMesh(pc, MESH_RADIUS, MESH_SEED)
+    t_py = time.perf_counter() - t0
+    n = len(out.point)
+    OP, ON = np.empty((n, 3), np.float32), np.empty((n, 3), np.float32)
+    got = C.c_size_t()
+    best = 1e9
+    for _ in range(3):
+        t0 = time.perf_counter()
+        N.check(N.lib().ppf_sample_mesh(N.ptr(P), len(P), N.ptr(F), len(F), MESH_RADIUS,
+                                        MESH_SEED, N.ptr(OP), N.ptr(ON), n, C.byref(got)))
+        best = min(best, time.perf_counter() - t0)
+    assert got.value == n and np.array_equal(OP, out.point)
+    return {"faces": int(len(F)), "points": n, "ms": round(best * 1e3, 2),
+            "points_per_s": float("%.4g" % (n / best)),
+            "python_sampleMesh_ms": round(t_py * 1e3, 2)}
+
+
+def cpu_mesh_sampling() -> float:
+    """the same sampleMesh call by the C oracle (one thread), ms"""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    from surface_match_amd import synth
+    P, F = synth.height_field_mesh(MESH_GRID, 5)
+    t0 = time.perf_counter()
+    O.sample_mesh(P, F, MESH_RADIUS, MESH_SEED)
+    return (time.perf_counter() - t0) * 1e3
+
+
 def measured_ceilings() -> dict:
     """This GPU's streaming-copy bandwidth and LDS atomic rate, measured by
     the library's own microbenchmarks (the roofline's peak stays the
@@ -350,6 +393,7 @@ def main():
                                     "GBps": round(vote_algorithmic_bytes(st) / (vote_ms * 1e-3) / 1e9, 1)}},
         "measured_ceilings": measured_ceilings(),
         "ply_ingest": ply_ingest(det, w["scene"].points),
+        "mesh_sampling": mesh_sampling(),
         "results": found,
         "tuning": det.get_tuning(),
     }
@@ -368,7 +412,8 @@ def main():
                                "cpu_model": cpu_model(), "host_cpus": os.cpu_count(),
                                "stage_ms": {k: round(float(cst.timeMs[i]), 1) for i, k in
                                             enumerate(("sample_and_normals", "vote", "cluster",
-                                                       "refine"))}}
+                                                       "refine"))},
+                               "mesh_sampling_ms": round(cpu_mesh_sampling(), 1)}
     print(json.dumps(out))
 
 

@@ -46,6 +46,12 @@ struct Vec3f {
 };
 struct Vec3i {
     int v[3] = {0, 0, 0};
+    Vec3i() = default;
+    Vec3i(int a, int b, int c) : v{a, b, c} {}
+    int       &operator[](int i) { return v[i]; }
+    int        operator[](int i) const { return v[i]; }
+    int       *data() { return v; }
+    const int *data() const { return v; }
 };
 // 4x4 column-major, the memory layout of Eigen::Matrix4f
 struct Mat4f {
@@ -89,7 +95,7 @@ struct BoundingBox {
 struct PointCloud {
     std::vector<Vec3f> point;
     std::vector<Vec3f> normal;
-    std::vector<Vec3i> face;  // accepted for source compatibility; meshes are not sampled
+    std::vector<Vec3i> face;  // a model with faces is sampled by trainModel (ppf.cpp:63-70)
     BoundingBox        box;
     Vec3f              viewPoint = Vec3f(NAN, NAN, NAN);
 
@@ -179,11 +185,25 @@ public:
     }
     ~Detector() = default;
 
+    // key of the mesh sampler's Philox stream (trainModel of a model with faces)
+    uint64_t meshSeed = 0x5EED;
+
     void trainModel(const PointCloud &model, float samplingDistanceRel = 0.04f,
                     TrainParam param = TrainParam()) {
         float           box[6];
         ppf_train_param p = param.c();
         pack_box(model, box);
+        if (!model.face.empty()) {
+            // the mesh sampler's uniforms are seeded (the reference draws its
+            // seed from std::random_device)
+            static_assert(sizeof(Vec3i) == 3 * sizeof(int), "faces are packed int triples");
+            detail::throw_status(ppf_train_model_mesh(
+                h_.get(), model.point.empty() ? nullptr : model.point[0].data(),
+                model.hasNormal() ? model.normal[0].data() : nullptr, model.point.size(),
+                model.face[0].data(), model.face.size(), box, model.viewPoint.data(),
+                samplingDistanceRel, &p, meshSeed));
+            return;
+        }
         detail::throw_status(ppf_train_model(
             h_.get(), model.point.empty() ? nullptr : model.point[0].data(),
             model.hasNormal() ? model.normal[0].data() : nullptr, model.point.size(), box,

@@ -108,6 +108,18 @@ PPF_API int ppf_train_model(ppf_detector d, const float *points, const float *no
                             const float *box6, const float *viewPoint, float samplingDistanceRel,
                             const ppf_train_param *param);
 
+/* trainModel for a model with faces (ppf.cpp:63-70): the mesh is sampled
+ * (sampleMesh, util.cpp:20-84) at radius reSampleStep / 4 with area-weighted
+ * points and the triangles' normals, and the sample trained on with the
+ * given model's box.  faces: nFaces x 3 point indices.  The reference seeds
+ * its sampler from std::random_device; here the uniforms come from
+ * Philox4x32-10 keyed by seed, so a seed fixes the model.  nFaces == 0 is
+ * ppf_train_model. */
+PPF_API int ppf_train_model_mesh(ppf_detector d, const float *points, const float *normals,
+                                 size_t n, const int *faces, size_t nFaces, const float *box6,
+                                 const float *viewPoint, float rel, const ppf_train_param *param,
+                                 uint64_t seed);
+
 /* Detector::matchScene, ppf.h:39-42 / ppf.cpp:149-510.  Writes up to cap
  * poses (16 floats each, column-major 4x4) and scores; *nOut = number of
  * matches (<= numMatches).  result and stats may be NULL. */
@@ -315,6 +327,18 @@ PPF_API int ppf_ply_write(const char *path, const float *points, const float *no
  * the file has one per point (type.cpp:36-38) */
 PPF_API int ppf_scene_upload_ply(ppf_detector d, const char *path, const float *viewPoint,
                                  ppf_scene *out);
+
+/* sampleMesh (util.h:8 / util.cpp:20-84): int(area / radius^2) points,
+ * area-weighted over the triangles (faces: nFaces x 3 point indices), each
+ * with its triangle's unit normal.  The uniforms of point j come from
+ * Philox4x32-10 at counter (j, 0, 0, 0) under key seed (the reference
+ * seeds std::mt19937 from std::random_device).  *nOut = the number of
+ * points, which does not depend on seed; with points or normals NULL (a
+ * size query) or cap < *nOut nothing is written.  nFaces == 0 gives 0
+ * points (the reference's empty cloud). */
+PPF_API int ppf_sample_mesh(const float *pts, size_t n, const int *faces, size_t nFaces,
+                            float radius, uint64_t seed, float *points, float *normals,
+                            size_t cap, size_t *nOut);
 
 /* measured ceilings of this GPU for the rooflines (no reference
  * counterpart): a 1 GiB device-to-device streaming copy (bytes read +

@@ -155,6 +155,10 @@ def lib():
         L.orc_voxel_sample.argtypes = [C.c_int, fp, C.c_void_p, C.c_int, C.c_float, fp, ip]
         L.orc_greedy_sample.restype = C.c_int
         L.orc_greedy_sample.argtypes = [C.c_int, fp, ip, C.c_int, C.c_float, ip]
+        L.orc_philox4x32_10.argtypes = [up, up]
+        L.orc_sample_mesh.restype = C.c_int
+        L.orc_sample_mesh.argtypes = [C.c_int, fp, C.c_int, C.c_void_p, C.c_float, C.c_uint64,
+                                      C.c_void_p, C.c_void_p, C.c_int]
         L.orc_estimate_normals.argtypes = [C.c_int, fp, fp, ip, C.c_int, ip, C.c_int, C.c_int,
                                            C.c_int, C.c_int, C.c_void_p]
         L.orc_jacobi_eig.restype = C.c_int
@@ -285,6 +289,53 @@ def voxel_sample(pts, step, box_min, subset=None):
     n = lib().orc_voxel_sample(len(pts), pts, _ptr(sub), 0 if sub is None else len(sub), step,
                                f32(box_min), out)
     return out[:n]
+
+
+def philox4x32_10(ctr, key):
+    c = np.ascontiguousarray(ctr, np.uint32).copy()
+    lib().orc_philox4x32_10(c, np.ascontiguousarray(key, np.uint32))
+    return c
+
+
+def sample_mesh(pts, faces, radius, seed):
+    """sampleMesh (util.cpp:20-84) with Philox uniforms: (points, normals)."""
+    pts = f32(pts).reshape(-1, 3)
+    fc = np.ascontiguousarray(faces, np.int32).reshape(-1, 3)
+    L = lib()
+    n = L.orc_sample_mesh(len(pts), pts, len(fc), _ptr(fc), radius, seed, None, None, 0)
+    if n < 0:
+        raise ValueError("orc_sample_mesh failed (%d)" % n)
+    P = np.zeros((n, 3), np.float32)
+    N = np.zeros((n, 3), np.float32)
+    if n:
+        L.orc_sample_mesh(len(pts), pts, len(fc), _ptr(fc), radius, seed, _ptr(P), _ptr(N), n)
+    return P, N
+
+
+def mesh_box(pts, nrm=None):
+    """computeBoundingBox over removeNan's valid points (util.cpp:86-105,178-207)."""
+    pts = f32(pts)
+    ok = np.isfinite(pts).all(1)
+    if nrm is not None:
+        ok &= np.isfinite(f32(nrm)).all(1)
+    v = pts[ok]
+    return np.concatenate([v.min(0), v.max(0)]).astype(np.float32)
+
+
+def train_mesh(pts, faces, nrm=None, rel=0.04, param=None, box6=None, vp=None, seed=0x5EED,
+               ref_tail=1):
+    """trainModel of a model with faces (ppf.cpp:46-70): the box of the given
+    model, the mesh sampled at radius reSampleStep / 4, the sample trained on
+    with that box."""
+    param = param or default_train_param()
+    box = f32(box6) if box6 is not None else None
+    if box is None or np.float32(np.linalg.norm(box[3:] - box[:3])) == 0:
+        box = mesh_box(pts, nrm)
+    d = box[3:] - box[:3]
+    D = np.sqrt(np.float32((d[0] * d[0] + d[1] * d[1]) + d[2] * d[2]), dtype=np.float32)
+    reStep = np.float32(D * np.float32(param.poseRefRelSamplingDistance))
+    P, N = sample_mesh(pts, faces, np.float32(np.float64(reStep) / 4.0), seed)
+    return train(P, N, rel=rel, param=param, box6=box, vp=vp, ref_tail=ref_tail)
 
 
 def estimate_normals(pts, nrm, valid, query, k=10, smooth=1, invert=0, vp=None):

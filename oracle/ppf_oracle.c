@@ -1726,6 +1726,137 @@ static void gather(const float *src, const int *idx, int m, float *dst, int norm
     }
 }
 
+/* ====================================================================== */
+/* mesh sampling                                                          */
+/* ====================================================================== */
+
+/* Philox4x32-10: ten rounds of two 32x32->64 multiplies (multipliers
+ * 0xD2511F53, 0xCD9E8D57) mixing the counter, with the key bumped by the
+ * Weyl constants 0x9E3779B9, 0xBB67AE85 between rounds (J. Salmon et al.,
+ * "Parallel random numbers: as easy as 1, 2, 3", SC'11, and its Random123
+ * known-answer tests). */
+static void orc_mulhilo32(uint32_t a, uint32_t b, uint32_t *hi, uint32_t *lo) {
+    uint64_t p = (uint64_t)a * (uint64_t)b;
+    *hi        = (uint32_t)(p >> 32);
+    *lo        = (uint32_t)p;
+}
+
+void orc_philox4x32_10(uint32_t ctr[4], const uint32_t key[2]) {
+    uint32_t k[2] = {key[0], key[1]};
+    for (int round = 0; round < 10; round++) {
+        uint32_t hi0, lo0, hi1, lo1;
+        orc_mulhilo32(0xD2511F53u, ctr[0], &hi0, &lo0);
+        orc_mulhilo32(0xCD9E8D57u, ctr[2], &hi1, &lo1);
+        uint32_t x[4] = {hi1 ^ ctr[1] ^ k[0], lo1, hi0 ^ ctr[3] ^ k[1], lo0};
+        memcpy(ctr, x, sizeof x);
+        k[0] += 0x9E3779B9u;
+        k[1] += 0xBB67AE85u;
+    }
+}
+
+/* sampleMesh, util.cpp:20-84.  Per triangle (util.cpp:29-42): u = p1-p0,
+ * v = p2-p0, k = u x v, area = |k| / 2 with |k| = sqrt((k0^2+k1^2)+k2^2),
+ * normal = k / |k| (k itself when |k|^2 == 0, Eigen's normalized()), the
+ * total area summed in triangle order.  CDF (:45-48): c0 = a0/A,
+ * ci = ai/A + c(i-1), the last set to 1.  nPoints = int(A / r^2) (:50).
+ * Triangle i fills outputs up to round(ci * nPoints) (:61-62; clamped to
+ * nPoints where the reference would write past its buffer).  Point j
+ * (:63-74): r1, r2 = top 24 bits of Philox words 0 and 1 times 2^-24;
+ * a = 1 - sqrt(r1), b = sqrt(r1)(1 - r2) in double, c = sqrt(r1) r2 in
+ * float; a and b rounded to float, point = (a p0 + b p1) + c p2. */
+int orc_sample_mesh(int nPts, const float *pts, int nFaces, const int *faces, float radius,
+                    uint64_t seed, float *outP, float *outN, int cap) {
+    if (nFaces <= 0)
+        return 0;
+    float *cdf = (float *)malloc(sizeof(float) * (size_t)nFaces);
+    float *tn  = (float *)malloc(sizeof(float) * 3 * (size_t)nFaces);
+    float  A   = 0.0f;
+    for (int i = 0; i < nFaces; i++) {
+        const int *f = faces + 3 * (size_t)i;
+        if (f[0] < 0 || f[1] < 0 || f[2] < 0 || f[0] >= nPts || f[1] >= nPts || f[2] >= nPts) {
+            free(cdf);
+            free(tn);
+            return -1;
+        }
+        const float *p0 = pts + 3 * (size_t)f[0], *p1 = pts + 3 * (size_t)f[1],
+                    *p2 = pts + 3 * (size_t)f[2];
+        float u0 = p1[0] - p0[0], u1 = p1[1] - p0[1], u2 = p1[2] - p0[2];
+        float v0 = p2[0] - p0[0], v1 = p2[1] - p0[1], v2 = p2[2] - p0[2];
+        float k0 = u1 * v2 - u2 * v1;
+        float k1 = u2 * v0 - u0 * v2;
+        float k2 = u0 * v1 - u1 * v0;
+        float sq = k0 * k0 + k1 * k1;
+        sq       = sq + k2 * k2;
+        float l  = sqrtf(sq);
+        cdf[i]   = l / 2.0f;
+        if (sq > 0.0f) {
+            tn[3 * i] = k0 / l, tn[3 * i + 1] = k1 / l, tn[3 * i + 2] = k2 / l;
+        } else {
+            tn[3 * i] = k0, tn[3 * i + 1] = k1, tn[3 * i + 2] = k2;
+        }
+        A += cdf[i];
+    }
+    float q = A / (radius * radius);
+    if (!(q >= 0.0f && q < 2147483648.0f)) {
+        free(cdf);
+        free(tn);
+        return -2;
+    }
+    int nPoints = (int)q;
+    if (nPoints == 0) {
+        free(cdf);
+        free(tn);
+        return 0;
+    }
+    cdf[0] = cdf[0] / A;
+    for (int i = 1; i < nFaces; i++)
+        cdf[i] = cdf[i] / A + cdf[i - 1];
+    cdf[nFaces - 1] = 1.0f;
+    /* first pass: the count */
+    long total = 0;
+    for (int i = 0; i < nFaces; i++) {
+        long e = (long)roundf(cdf[i] * (float)nPoints);
+        if (e > nPoints)
+            e = nPoints;
+        if (e > total)
+            total = e;
+    }
+    if (!outP || !outN || cap < total) {
+        free(cdf);
+        free(tn);
+        return (int)total;
+    }
+    const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    long           j      = 0;
+    for (int i = 0; i < nFaces; i++) {
+        long e = (long)roundf(cdf[i] * (float)nPoints);
+        if (e > nPoints)
+            e = nPoints;
+        const int   *f  = faces + 3 * (size_t)i;
+        const float *p0 = pts + 3 * (size_t)f[0], *p1 = pts + 3 * (size_t)f[1],
+                    *p2 = pts + 3 * (size_t)f[2];
+        for (; j < e; j++) {
+            uint32_t ctr[4] = {(uint32_t)j, 0u, 0u, 0u};
+            orc_philox4x32_10(ctr, key);
+            float  r1 = (float)(ctr[0] >> 8) * 0x1p-24f;
+            float  r2 = (float)(ctr[1] >> 8) * 0x1p-24f;
+            float  s  = sqrtf(r1);
+            double a  = 1.0 - (double)s;
+            double b  = (double)s * (1.0 - (double)r2);
+            float  c  = s * r2;
+            float  fa = (float)a, fb = (float)b;
+            for (int d = 0; d < 3; d++) {
+                float t              = fa * p0[d] + fb * p1[d];
+                outP[3 * (size_t)j + d] = t + c * p2[d];
+                outN[3 * (size_t)j + d] = tn[3 * i + d];
+            }
+        }
+    }
+    free(cdf);
+    free(tn);
+    return (int)total;
+}
+
 /* Detector::trainModel (ppf.cpp:36-147) with voxel sampling. */
 int orc_train(int n, const float *pts, const float *nrm, const float *box6, const float vp[3],
               float rel, const orc_train_param *p, int refTail, orc_model *out) {

@@ -114,6 +114,16 @@ int orc_voxel_sample(int n, const float *pts, const int *subset, int nSubset, fl
 /* greedy Poisson-disk sampling in a given visiting order (util.cpp:107-139) */
 int orc_greedy_sample(int n, const float *pts, const int *order, int nOrder, float step, int *out);
 
+/* ---- mesh sampling (util.cpp:20-84) --------------------------------- */
+/* Philox4x32-10 (Salmon et al., SC'11) in place on ctr under key */
+void orc_philox4x32_10(uint32_t ctr[4], const uint32_t key[2]);
+/* sampleMesh with point j's uniforms from Philox at counter (j,0,0,0),
+ * key (seed low word, high word).  Returns the number of points and writes
+ * them (with their triangles' normals) when cap allows; -1 for a face
+ * index out of range, -2 for an area or radius out of range. */
+int orc_sample_mesh(int nPts, const float *pts, int nFaces, const int *faces, float radius,
+                    uint64_t seed, float *outP, float *outN, int cap);
+
 /* ---- normals (util.cpp:234-334), snapshot semantics ------------------ */
 int orc_estimate_normals(int n, float *pts, float *nrm, const int *valid, int nValid,
                          const int *query, int nQuery, int k, int smooth, int invert,

@@ -15,7 +15,8 @@ import numpy as np
 
 from . import _native as N
 
-__all__ = ["PointCloud", "TrainParam", "MatchParam", "MatchResult", "Detector", "BoundingBox"]
+__all__ = ["PointCloud", "TrainParam", "MatchParam", "MatchResult", "Detector", "BoundingBox",
+           "ResidentScene", "readPLY", "writePLY", "sampleMesh"]
 
 
 @dataclass
@@ -121,6 +122,29 @@ def writePLY(filename: str, mesh: PointCloud, write_ascii: bool = False) -> bool
     return True
 
 
+def sampleMesh(pc: PointCloud, radius: float, seed: int = 0x5EED) -> PointCloud:
+    """util.h:8 / util.cpp:20-84: int(area / radius^2) points spread over the
+    triangles by area, each with its triangle's unit normal; the box is
+    carried over.  The uniforms come from Philox4x32-10 keyed by seed (the
+    reference seeds from std::random_device).  No faces: an empty cloud."""
+    p = N.f32(np.reshape(pc.point, (-1, 3)))
+    f = N.i32(np.reshape(pc.face, (-1, 3)))
+    out = PointCloud()
+    if len(f) == 0:
+        return out
+    n = C.c_size_t()
+    L = N.lib()
+    N.check(L.ppf_sample_mesh(N.ptr(p), len(p), N.ptr(f), len(f), radius, seed, None, None, 0,
+                              C.byref(n)))
+    P = np.zeros((n.value, 3), np.float32)
+    Q = np.zeros((n.value, 3), np.float32)
+    if n.value:
+        N.check(L.ppf_sample_mesh(N.ptr(p), len(p), N.ptr(f), len(f), radius, seed, N.ptr(P),
+                                  N.ptr(Q), n.value, C.byref(n)))
+    out.point, out.normal, out.box = P, Q, pc.box
+    return out
+
+
 @dataclass
 class MatchResult:
     """type.h:178-181"""
@@ -144,11 +168,19 @@ class Detector:
             self._h = None
 
     def trainModel(self, model: PointCloud, samplingDistanceRel: float = 0.04,
-                   param: N.TrainParam | None = None):
+                   param: N.TrainParam | None = None, mesh_seed: int = 0x5EED):
+        """ppf.cpp:36-147.  A model with faces is sampled first (sampleMesh,
+        seeded by mesh_seed where the reference draws from random_device)."""
         p, n, box, vp = model._c_args()
         param = param or TrainParam()
-        N.check(N.lib().ppf_train_model(self._h, N.ptr(p), N.ptr(n), len(p), N.ptr(box),
-                                        N.ptr(vp), samplingDistanceRel, C.byref(param)))
+        f = N.i32(np.reshape(model.face, (-1, 3)))
+        if len(f):
+            N.check(N.lib().ppf_train_model_mesh(self._h, N.ptr(p), N.ptr(n), len(p), N.ptr(f),
+                                                 len(f), N.ptr(box), N.ptr(vp),
+                                                 samplingDistanceRel, C.byref(param), mesh_seed))
+        else:
+            N.check(N.lib().ppf_train_model(self._h, N.ptr(p), N.ptr(n), len(p), N.ptr(box),
+                                            N.ptr(vp), samplingDistanceRel, C.byref(param)))
 
     def matchScene(self, scene: PointCloud, samplingDistanceRel: float = 0.04,
                    keyPointFraction: float = 0.2, minScore: float = 0.2,

@@ -98,7 +98,8 @@ EXPORTS = [
     "ppf_icp_target_create", "ppf_icp_target_destroy", "ppf_icp_refine", "ppf_icp_adjoint",
     "ppf_autotune", "ppf_set_tuning", "ppf_get_tuning", "ppf_match_scenes_resident",
     "ppf_scene_inliers", "ppf_measure_ceilings", "ppf_ply_read", "ppf_ply_info", "ppf_ply_copy",
-    "ppf_ply_free", "ppf_ply_write", "ppf_scene_upload_ply",
+    "ppf_ply_free", "ppf_ply_write", "ppf_scene_upload_ply", "ppf_train_model_mesh",
+    "ppf_sample_mesh",
 ]
 
 
@@ -155,6 +156,9 @@ def lib() -> C.CDLL:
         "ppf_ply_free": (None, [vp]),
         "ppf_ply_write": (i, [C.c_char_p, vp, vp, sz, vp, sz, i]),
         "ppf_scene_upload_ply": (i, [vp, C.c_char_p, vp, C.POINTER(vp)]),
+        "ppf_train_model_mesh": (i, [vp, vp, vp, sz, vp, sz, vp, vp, f, C.POINTER(TrainParam),
+                                     C.c_uint64]),
+        "ppf_sample_mesh": (i, [vp, sz, vp, sz, f, C.c_uint64, vp, vp, sz, C.POINTER(sz)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

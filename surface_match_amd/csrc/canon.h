@@ -239,6 +239,68 @@ PPF_HD int angle_bin(float alphaModel, float alphaScene, float sMaxId, int angle
                 : bin_body(alphaModel, bin_offset_body(alphaScene), sMaxId, angleNum);
 }
 
+// Philox4x32-10 counter-based generator (Salmon, Moraes, Dror, Shaw,
+// "Parallel random numbers: as easy as 1, 2, 3", SC 2011): ten rounds of
+// two 32x32->64 multiplies, the key bumped by the Weyl constants between
+// rounds.  c (4 words) is replaced by the output block.
+PPF_HD void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1) {
+#if defined(__HIP_DEVICE_COMPILE__)
+#pragma unroll
+#endif
+    for (int r = 0; r < 10; r++) {
+        if (r > 0) {
+            k0 += 0x9E3779B9u;
+            k1 += 0xBB67AE85u;
+        }
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c[0];
+        const uint64_t p1 = (uint64_t)0xCD9E8D57u * c[2];
+        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c[1] ^ k0, n1 = (uint32_t)p1;
+        const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c[3] ^ k1, n3 = (uint32_t)p0;
+        c[0] = n0;
+        c[1] = n1;
+        c[2] = n2;
+        c[3] = n3;
+    }
+}
+// a float in [0, 1) from the top 24 bits of a random word
+PPF_HD float uniform01(uint32_t u) { return (float)(u >> 8) * (1.0f / 16777216.0f); }
+
+// a triangle's area |u x v| / 2 and unit normal (u x v) / |u x v|, u = p1 - p0,
+// v = p2 - p0, as sampleMesh forms them (util.cpp:29-42): Eigen's cross,
+// |k| = sqrt((k0^2 + k1^2) + k2^2), and normalized() leaving a zero k as is
+PPF_HD float tri_area_normal(const float *p0, const float *p1, const float *p2, float nrm[3]) {
+    const float u0 = p1[0] - p0[0], u1 = p1[1] - p0[1], u2 = p1[2] - p0[2];
+    const float v0 = p2[0] - p0[0], v1 = p2[1] - p0[1], v2 = p2[2] - p0[2];
+    const float k0 = u1 * v2 - u2 * v1, k1 = u2 * v0 - u0 * v2, k2 = u0 * v1 - u1 * v0;
+    const float n2 = (k0 * k0 + k1 * k1) + k2 * k2;
+    const float l  = sqrtf(n2);
+    if (n2 > 0.0f) {
+        nrm[0] = k0 / l;
+        nrm[1] = k1 / l;
+        nrm[2] = k2 / l;
+    } else {
+        nrm[0] = k0;
+        nrm[1] = k1;
+        nrm[2] = k2;
+    }
+    return l / 2.f;
+}
+
+// one point of triangle (p0, p1, p2) from two uniforms, as sampleMesh
+// forms it (util.cpp:60-71): a = 1 - sqrt(r1) and b = sqrt(r1) (1 - r2) in
+// double, c = sqrt(r1) r2 in float, each weight rounded to float before it
+// scales its vertex, summed left to right in float
+PPF_HD void mesh_point(const float *p0, const float *p1, const float *p2, float r1, float r2,
+                       float out[3]) {
+    const float  s = sqrtf(r1);
+    const double a = 1. - (double)s;
+    const double b = (double)s * (1. - (double)r2);
+    const float  c = s * r2;
+    const float  fa = (float)a, fb = (float)b;
+    for (int k = 0; k < 3; k++)
+        out[k] = (fa * p0[k] + fb * p1[k]) + c * p2[k];
+}
+
 // nanoflann L2 metric for DIM = 3 (nanoflann.hpp:545-581): ((0+dx^2)+dy^2)+dz^2.
 PPF_HD float dist2(float qx, float qy, float qz, float px, float py, float pz) {
     float dx = qx - px, dy = qy - py, dz = qz - pz;

@@ -1323,6 +1323,46 @@ PPF_API int ppf_train_model(ppf_detector d, const float *points, const float *no
     });
 }
 
+PPF_API int ppf_train_model_mesh(ppf_detector d, const float *points, const float *normals,
+                                 size_t n, const int *faces, size_t nFaces, const float *box6,
+                                 const float *viewPoint, float rel, const ppf_train_param *param,
+                                 uint64_t seed) {
+    PPF_REQUIRE(d && (faces || nFaces == 0), "null argument");
+    return guarded([&] {
+        ppf_train_param p;
+        if (param)
+            p = *param;
+        else
+            ppf_train_param_default(&p);
+        if (nFaces == 0) {
+            train_impl(*d, points, normals, (int)n, box6, viewPoint, rel, p);
+            return;
+        }
+        // ppf.cpp:39-52: the checks, then the box of the given model, whose
+        // diameter sets the sampling radius reSampleStep / 4 (:57, :66)
+        if (rel > 1 || rel < 0)
+            throw std::range_error("Invalid Input: samplingDistanceRel range mismatch in trainModel");
+        if (n == 0 || !points)
+            throw std::runtime_error("Invalid Input: empty model in trainModel");
+        check_train_param(p);
+        float box[6];
+        if (box6 && box_diameter(box6, box6 + 3) != 0.0f) {
+            std::memcpy(box, box6, sizeof box);
+        } else {
+            std::vector<int> v = remove_nan(points, normals, (int)n);
+            bbox_host(points, v, box, box + 3);
+        }
+        const float Dm           = box_diameter(box, box + 3);
+        const float reSampleStep = Dm * p.poseRefRelSamplingDistance;
+        std::vector<float> sp, sn;
+        sample_mesh(points, (int)n, faces, (int)nFaces, (float)(reSampleStep / 4.), seed, sp, sn,
+                    d->stream);
+        if (sp.empty())
+            throw std::runtime_error("Invalid Input: mesh too small to sample in trainModel");
+        train_impl(*d, sp.data(), sn.data(), (int)(sp.size() / 3), box, viewPoint, rel, p);
+    });
+}
+
 PPF_API int ppf_match_scene(ppf_detector d, const float *points, const float *normals, size_t n,
                             const float *box6, const float *viewPoint, float rel, float kpFrac,
                             float minScore, const ppf_match_param *param, float *poses16,
@@ -1711,6 +1751,26 @@ PPF_API int ppf_vote(ppf_detector d, const float *pts, const float *nrm, size_t 
         }
         if (nPoses)
             *nPoses = np;
+    });
+}
+
+PPF_API int ppf_sample_mesh(const float *pts, size_t n, const int *faces, size_t nFaces,
+                            float radius, uint64_t seed, float *points, float *normals,
+                            size_t cap, size_t *nOut) {
+    PPF_REQUIRE(nOut && (pts || n == 0) && (faces || nFaces == 0), "null argument");
+    return guarded([&] {
+        hipStream_t s = util_stream();
+        if (!points || !normals) {
+            *nOut = (size_t)sample_mesh_count(pts, (int)n, faces, (int)nFaces, radius, s);
+            return;
+        }
+        std::vector<float> sp, sn;
+        sample_mesh(pts, (int)n, faces, (int)nFaces, radius, seed, sp, sn, s);
+        *nOut = sp.size() / 3;
+        if (cap < *nOut)
+            return;
+        std::memcpy(points, sp.data(), sizeof(float) * sp.size());
+        std::memcpy(normals, sn.data(), sizeof(float) * sn.size());
     });
 }
 

@@ -238,6 +238,17 @@ void icp_adjoint(Workspace &ws, int nSrc, const float *sx, const float *sy, cons
                  const GridView &dst, const float *T0host, const double *xiHost, int H,
                  float rejectDist, int center, double *outHost, int *corrHost, hipStream_t s);
 
+// ---------------------------------------------------------------- mesh
+
+// sampleMesh (util.cpp:20-84) with Philox4x32-10 uniforms keyed by seed:
+// points and normals (n x 3 each) of the sample
+void sample_mesh(const float *pts, int nPts, const int *faces, int nFaces, float radius,
+                 uint64_t seed, std::vector<float> &outP, std::vector<float> &outN,
+                 hipStream_t s);
+// the number of points sample_mesh returns (the same for every seed)
+int sample_mesh_count(const float *pts, int nPts, const int *faces, int nFaces, float radius,
+                      hipStream_t s);
+
 // ---------------------------------------------------------------- PLY
 
 // a PLY file's points and normals (n x 3 floats) and triangles (3 ints each)

@@ -218,6 +218,22 @@ def batch_scenes(seed: int, n_scenes: int = 64, lo: int = 50000, hi: int = 50000
     return out
 
 
+def height_field_mesh(n: int = 24, seed: int = 3):
+    """An open, asymmetric triangle mesh z = f(x, y) over the unit square:
+    an n x n vertex grid, two counter-clockwise triangles per cell (normals
+    towards +z).  Returns (points n^2 x 3 float32, faces 2(n-1)^2 x 3 int32)."""
+    rng = _rng(seed)
+    g = np.linspace(0.0, 1.0, n, dtype=np.float64)
+    X, Y = np.meshgrid(g, g, indexing="ij")
+    Z = 0.2 * np.sin(3 * X) * np.cos(2 * Y) + 0.1 * X * X + 0.003 * rng.standard_normal(X.shape)
+    P = np.stack([X, Y, Z], -1).reshape(-1, 3).astype(np.float32)
+    i, j = np.meshgrid(np.arange(n - 1), np.arange(n - 1), indexing="ij")
+    a = (i * n + j).ravel()
+    b, c, d = a + n, a + n + 1, a + 1
+    F = np.stack([np.stack([a, b, c], 1), np.stack([a, c, d], 1)], 1).reshape(-1, 3)
+    return P, F.astype(np.int32)
+
+
 def symmetries() -> list:
     """Rigid self-maps of the model (identity and the 180 degree turn about
     the cylinder axis x=50, y=30)."""

@@ -1,5 +1,7 @@
 """Shared test setup: the `gpu` marker, import paths, seeded workloads."""
+import ctypes as C
 import os
+import subprocess
 import sys
 
 import numpy as np
@@ -28,3 +30,33 @@ def workload1():
 
 def rng(seed):
     return np.random.Generator(np.random.PCG64(seed))
+
+
+@pytest.fixture(scope="session")
+def H(tmp_path_factory):
+    """canon.h compiled for the host (the device arithmetic, run on the CPU)."""
+    so = str(tmp_path_factory.mktemp("canon") / "canon_host.so")
+    subprocess.check_call(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",
+                           "-fno-fast-math", os.path.join(ROOT, "tests", "cpp", "canon_host.cpp"),
+                           "-o", so])
+    L = C.CDLL(so)
+    fp = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+    L.ch_atan2f.restype = C.c_float
+    L.ch_atan2f.argtypes = [C.c_float, C.c_float]
+    L.ch_ppf_key.restype = C.c_uint32
+    L.ch_ppf_key.argtypes = [fp, fp, fp, fp, C.c_float, C.c_float, C.c_int]
+    L.ch_frame.argtypes = [fp, fp, fp]
+    L.ch_alpha.restype = C.c_float
+    L.ch_alpha.argtypes = [fp, fp]
+    L.ch_bin.restype = C.c_int
+    L.ch_bin.argtypes = [C.c_float, C.c_float, C.c_int, C.c_int]
+    L.ch_bin_wrapped.restype = C.c_int
+    L.ch_bin_wrapped.argtypes = [C.c_float, C.c_float, C.c_int, C.c_int]
+    L.ch_pose.argtypes = [fp, fp, C.c_float, C.c_float, fp]
+    L.ch_quat.argtypes = [fp, fp]
+    up = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
+    L.ch_philox.argtypes = [up, C.c_uint32, C.c_uint32]
+    L.ch_uniform01.restype = C.c_float
+    L.ch_uniform01.argtypes = [C.c_uint32]
+    L.ch_mesh_point.argtypes = [fp, fp, fp, C.c_float, C.c_float, fp]
+    return L

@@ -22,4 +22,10 @@ void ch_pose(const float *Ts, const float *Tm, float c, float s, float *out) {
     ppfc::compose_pose(Ts, Tm, c, s, out);
 }
 void ch_quat(const float *R, float *q) { ppfc::quat_from_rot(R, q); }
+void ch_philox(uint32_t *c, uint32_t k0, uint32_t k1) { ppfc::philox4x32_10(c, k0, k1); }
+float ch_uniform01(uint32_t u) { return ppfc::uniform01(u); }
+void ch_mesh_point(const float *p0, const float *p1, const float *p2, float r1, float r2,
+                   float *out) {
+    ppfc::mesh_point(p0, p1, p2, r1, r2, out);
+}
 }

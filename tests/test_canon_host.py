@@ -1,41 +1,13 @@
 """The device arithmetic header (canon.h) compiled for the host agrees bit for
 bit with the oracle -- the same FP32 sequences, written twice.  CPU only."""
-import ctypes as C
 import math
-import os
-import subprocess
 
 import numpy as np
 import pytest
 
 from conftest import rng
 
-ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 TWO_PI_F = np.float32(2.0) * np.float32(math.pi)
-
-
-@pytest.fixture(scope="module")
-def H(tmp_path_factory):
-    so = str(tmp_path_factory.mktemp("canon") / "canon_host.so")
-    subprocess.check_call(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",
-                           "-fno-fast-math", os.path.join(ROOT, "tests", "cpp", "canon_host.cpp"),
-                           "-o", so])
-    L = C.CDLL(so)
-    fp = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
-    L.ch_atan2f.restype = C.c_float
-    L.ch_atan2f.argtypes = [C.c_float, C.c_float]
-    L.ch_ppf_key.restype = C.c_uint32
-    L.ch_ppf_key.argtypes = [fp, fp, fp, fp, C.c_float, C.c_float, C.c_int]
-    L.ch_frame.argtypes = [fp, fp, fp]
-    L.ch_alpha.restype = C.c_float
-    L.ch_alpha.argtypes = [fp, fp]
-    L.ch_bin.restype = C.c_int
-    L.ch_bin.argtypes = [C.c_float, C.c_float, C.c_int, C.c_int]
-    L.ch_bin_wrapped.restype = C.c_int
-    L.ch_bin_wrapped.argtypes = [C.c_float, C.c_float, C.c_int, C.c_int]
-    L.ch_pose.argtypes = [fp, fp, C.c_float, C.c_float, fp]
-    L.ch_quat.argtypes = [fp, fp]
-    return L
 
 
 def test_canon_matches_oracle(H, oracle_mod):
