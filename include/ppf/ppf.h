@@ -10,7 +10,10 @@
 
 #include "../ppf_c.h"
 
+#include <chrono>
 #include <cmath>
+#include <cstdint>
+#include <iostream>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -271,6 +274,182 @@ private:
             out.normal[i] = Vec3f(n[3 * i], n[3 * i + 1], n[3 * i + 2]);
         }
     }
+};
+
+static_assert(sizeof(Vec3i) == 3 * sizeof(int), "face layout must be 3 packed ints");
+
+// ---- filePLY.h:7-9 ----------------------------------------------------
+// readPLY: points, normals (when the file has nx) and ear-clipped faces; on
+// failure prints the reason and returns false, as the reference does
+inline bool readPLY(const std::string &filename, PointCloud &mesh) {
+    ppf_ply h = nullptr;
+    if (ppf_ply_read(filename.c_str(), &h) != PPF_OK) {
+        std::cout << ppf_last_error() << std::endl;
+        return false;
+    }
+    size_t nv = 0, nn = 0, nf = 0;
+    ppf_ply_info(h, &nv, &nn, &nf);
+    mesh.point.resize(nv);
+    mesh.normal.resize(nn);
+    mesh.face.resize(nf);
+    ppf_ply_copy(h, nv ? mesh.point[0].data() : nullptr, nn ? mesh.normal[0].data() : nullptr,
+                 nf ? reinterpret_cast<int *>(mesh.face.data()) : nullptr);
+    ppf_ply_free(h);
+    return true;
+}
+
+inline bool writePLY(const std::string &filename, const PointCloud &mesh,
+                     bool write_ascii = false) {
+    const int rc = ppf_ply_write(
+        filename.c_str(), mesh.point.empty() ? nullptr : mesh.point[0].data(),
+        mesh.hasNormal() ? mesh.normal[0].data() : nullptr, mesh.point.size(),
+        mesh.face.empty() ? nullptr : reinterpret_cast<const int *>(mesh.face.data()),
+        mesh.face.size(), write_ascii ? 1 : 0);
+    if (rc != PPF_OK) {
+        std::cout << ppf_last_error() << std::endl;
+        return false;
+    }
+    return true;
+}
+
+// ---- util.h:8-21 ------------------------------------------------------
+// sampleMesh (util.cpp:20-84) on the GPU; the uniforms come from a Philox
+// stream keyed by seed where the reference draws from std::random_device
+inline PointCloud sampleMesh(const PointCloud &pc, float radius, uint64_t seed = 0x5EED) {
+    PointCloud result;
+    if (pc.face.empty())
+        return result;
+    const float *pts   = pc.point.empty() ? nullptr : pc.point[0].data();
+    const int   *faces = reinterpret_cast<const int *>(pc.face.data());
+    size_t       n     = 0;
+    detail::throw_status(ppf_sample_mesh(pts, pc.point.size(), faces, pc.face.size(), radius,
+                                         seed, nullptr, nullptr, 0, &n));
+    result.point.resize(n);
+    result.normal.resize(n);
+    if (n)
+        detail::throw_status(ppf_sample_mesh(pts, pc.point.size(), faces, pc.face.size(), radius,
+                                             seed, result.point[0].data(),
+                                             result.normal[0].data(), n, &n));
+    result.box = pc.box;
+    return result;
+}
+
+namespace detail {
+inline bool finite3(const Vec3f &v) {
+    return std::isfinite(v[0]) && std::isfinite(v[1]) && std::isfinite(v[2]);
+}
+// Eigen's normalized(): v / |v|, v itself when |v|^2 is 0
+inline Vec3f normalized(const Vec3f &v) {
+    const float s = (v[0] * v[0] + v[1] * v[1]) + v[2] * v[2];
+    if (!(s > 0.0f))
+        return v;
+    const float l = std::sqrt(s);
+    return Vec3f(v[0] / l, v[1] / l, v[2] / l);
+}
+}  // namespace detail
+
+// removeNan (util.cpp:86-105): indices of the points (and normals, when
+// asked and present) with all coordinates finite
+inline std::vector<int> removeNan(const PointCloud &pc, bool checkNormal = false) {
+    checkNormal = checkNormal && pc.hasNormal();
+    std::vector<int> result;
+    result.reserve(pc.size());
+    for (size_t i = 0; i < pc.size(); i++)
+        if (detail::finite3(pc.point[i]) && (!checkNormal || detail::finite3(pc.normal[i])))
+            result.push_back((int)i);
+    return result;
+}
+
+// computeBoundingBox (util.cpp:178-207), over validIndices when given
+inline BoundingBox computeBoundingBox(const PointCloud &pc,
+                                      const std::vector<int> &validIndices = {}) {
+    Vec3f mn = validIndices.empty() ? pc.point[0] : pc.point[validIndices[0]];
+    Vec3f mx = mn;
+    const size_t m = validIndices.empty() ? pc.size() : validIndices.size();
+    for (int dim = 0; dim < 3; dim++)
+        for (size_t t = 0; t < m; t++) {
+            const Vec3f &p = pc.point[validIndices.empty() ? t : (size_t)validIndices[t]];
+            if (p[dim] > mx[dim])
+                mx[dim] = p[dim];
+            else if (p[dim] < mn[dim])
+                mn[dim] = p[dim];
+        }
+    return BoundingBox(mn, mx);
+}
+
+// extraIndices (util.cpp:141-162): the listed points, normals normalized
+inline PointCloud extraIndices(const PointCloud &pc, const std::vector<std::size_t> &indices) {
+    PointCloud result;
+    const bool hasNormal = pc.hasNormal();
+    result.point.resize(indices.size());
+    if (hasNormal)
+        result.normal.resize(indices.size());
+    for (size_t i = 0; i < indices.size(); i++) {
+        result.point[i] = pc.point[indices[i]];
+        if (hasNormal)
+            result.normal[i] = detail::normalized(pc.normal[indices[i]]);
+    }
+    result.box = pc.box;
+    if (result.box.diameter() == 0)
+        result.box = computeBoundingBox(result);
+    return result;
+}
+
+// normalizeNormal (util.cpp:164-176)
+inline void normalizeNormal(PointCloud &pc, bool invert = false) {
+    for (auto &n : pc.normal) {
+        const Vec3f u = detail::normalized(n);
+        n             = invert ? Vec3f(-u[0], -u[1], -u[2]) : u;
+    }
+}
+
+// transformPointCloud (util.cpp:209-232): R p + t and R n, faces kept, box
+// recomputed
+inline PointCloud transformPointCloud(const PointCloud &pc, const Mat4f &pose,
+                                      bool useNormal = true) {
+    const bool doNormal = pc.hasNormal() && useNormal;
+    PointCloud result;
+    result.point.resize(pc.size());
+    if (doNormal)
+        result.normal.resize(pc.size());
+    auto rot = [&](const Vec3f &v) {
+        return Vec3f((pose(0, 0) * v[0] + pose(0, 1) * v[1]) + pose(0, 2) * v[2],
+                     (pose(1, 0) * v[0] + pose(1, 1) * v[1]) + pose(1, 2) * v[2],
+                     (pose(2, 0) * v[0] + pose(2, 1) * v[1]) + pose(2, 2) * v[2]);
+    };
+    for (size_t i = 0; i < pc.size(); i++) {
+        const Vec3f r   = rot(pc.point[i]);
+        result.point[i] = Vec3f(r[0] + pose(0, 3), r[1] + pose(1, 3), r[2] + pose(2, 3));
+        if (doNormal)
+            result.normal[i] = rot(pc.normal[i]);
+    }
+    result.face = pc.face;
+    if (!result.point.empty())
+        result.box = computeBoundingBox(result);
+    return result;
+}
+
+// ---- helper.h ---------------------------------------------------------
+// prints "<content> cost(ms): <ms>" when released or destroyed
+class Timer {
+public:
+    explicit Timer(std::string content)
+        : content_(std::move(content)), start_(std::chrono::steady_clock::now()) {}
+    ~Timer() {
+        if (!released_)
+            release();
+    }
+    void release() {
+        released_ = true;
+        const auto ms = std::chrono::duration_cast<std::chrono::milliseconds>(
+            std::chrono::steady_clock::now() - start_);
+        std::cout << content_ << " cost(ms): " << ms.count() << std::endl;
+    }
+
+private:
+    std::string                           content_;
+    std::chrono::steady_clock::time_point start_;
+    bool                                  released_ = false;
 };
 
 }  // namespace ppf

@@ -1,34 +1,22 @@
 // Drives the C++ drop-in facade the way the reference's example does
-// (main.cpp:6-47): train -> save -> load into a second detector -> match.
-// Usage: facade_main model.bin scene.bin out.model
-// Files: int32 n, then n*3 float points, then (model only) n*3 float normals.
+// (main.cpp:6-47): readPLY -> train -> save -> load into a second detector
+// -> match -> transformPointCloud / writePLY of the results, then
+// sampleMesh on a mesh file.
+// Usage: facade_main model.ply scene.ply out.model outdir mesh.ply
 #include <ppf/ppf.h>
 
 #include <cstdio>
-#include <fstream>
 #include <iostream>
 
-static ppf::PointCloud read_cloud(const char *path, bool normals) {
-    std::ifstream f(path, std::ios::binary);
-    int           n = 0;
-    f.read(reinterpret_cast<char *>(&n), 4);
-    ppf::PointCloud c;
-    c.point.resize(n);
-    f.read(reinterpret_cast<char *>(c.point.data()), 12 * (size_t)n);
-    if (normals) {
-        c.normal.resize(n);
-        f.read(reinterpret_cast<char *>(c.normal.data()), 12 * (size_t)n);
-    }
-    return c;
-}
-
 int main(int argc, char **argv) {
-    if (argc < 4)
+    if (argc < 6)
         return 2;
-    ppf::PointCloud model = read_cloud(argv[1], true);
-    ppf::PointCloud scene = read_cloud(argv[2], false);
-    scene.viewPoint       = ppf::Vec3f(620, 100, 500);
+    ppf::PointCloud model, scene;
+    if (!ppf::readPLY(argv[1], model) || !ppf::readPLY(argv[2], scene))
+        return 5;
+    scene.viewPoint = ppf::Vec3f(620, 100, 500);
     {
+        ppf::Timer    t("train model");
         ppf::Detector detector;
         detector.trainModel(model, 0.04f);
         detector.save(argv[3]);
@@ -57,11 +45,30 @@ int main(int argc, char **argv) {
     detector.matchScene(scene, pose, score, 0.04f, 0.1f, 0.2f, ppf::MatchParam(3), &result);
     std::cout << "matches " << pose.size() << " sampled " << result.sampledScene.size()
               << " keypoints " << result.keyPoint.size() << "\n";
+    const std::string dir = argv[4];
     for (size_t i = 0; i < pose.size(); i++) {
         std::cout << "score " << score[i] << " pose";
         for (int k = 0; k < 16; k++)
             std::cout << " " << pose[i].data()[k];
         std::cout << "\n";
+        auto pc = ppf::transformPointCloud(model, pose[i]);
+        if (!ppf::writePLY(dir + "/out" + std::to_string(i) + ".ply", pc))
+            return 6;
     }
+    if (!ppf::writePLY(dir + "/sampledScene.ply", result.sampledScene) ||
+        !ppf::writePLY(dir + "/sampledKeypoint.ply", result.keyPoint, true))
+        return 7;
+    // a missing file: false and the reason on stdout
+    ppf::PointCloud none;
+    if (ppf::readPLY(dir + "/missing.ply", none))
+        return 8;
+    ppf::PointCloud mesh;
+    if (!ppf::readPLY(argv[5], mesh))
+        return 9;
+    ppf::PointCloud samples = ppf::sampleMesh(mesh, 0.1f, 42);
+    std::cout << "mesh faces " << mesh.face.size() << " samples " << samples.size() << "\n";
+    auto valid = ppf::removeNan(samples, true);
+    auto box   = ppf::computeBoundingBox(samples, valid);
+    std::cout << "valid " << valid.size() << " box " << box.min[0] << " " << box.max[0] << "\n";
     return 0;
 }
