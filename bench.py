@@ -128,6 +128,53 @@ def mesh_sampling() -> dict:
             "python_sampleMesh_ms": round(t_py * 1e3, 2)}
 
 
+MLS_N, MLS_STRIDE, MLS_RADIUS, MLS_SIGMA = 1_000_000, 4, 0.03, 0.004
+
+
+def mls_cloud():
+    """1M points on a unit sphere with radial noise sigma 0.004 (seeded)"""
+    r = np.random.Generator(np.random.PCG64(8))
+    d = r.standard_normal((MLS_N, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    return (d * (1.0 + MLS_SIGMA * r.standard_normal((MLS_N, 1)))).astype(np.float32)
+
+
+def mls_normals() -> dict:
+    """estimateNormalMLS, order 2, radius 0.03 on the 1M-point noisy sphere
+    for every 4th point (250k queries); device time of search + sort + fit
+    (best of 3), and the whole call with host arrays in and out."""
+    from surface_match_amd import PointCloud, estimateNormalMLS
+    P = mls_cloud()
+    q = np.arange(0, MLS_N, MLS_STRIDE)
+    best, best_call, st = 1e9, 1e9, None
+    for _ in range(3):
+        pc = PointCloud(P.copy())
+        pc.viewPoint = np.zeros(3, np.float32)
+        t0 = time.perf_counter()
+        st = estimateNormalMLS(pc, q, MLS_RADIUS, 2, invert=True)
+        best_call = min(best_call, time.perf_counter() - t0)
+        best = min(best, st["device_ms"])
+    return {"points": MLS_N, "queries": int(len(q)), "radius": MLS_RADIUS, "order": 2,
+            "neighbours": st["neighbours"], "device_ms": round(best, 3),
+            "call_ms": round(best_call * 1e3, 2),
+            "neighbours_per_s": float("%.4g" % (st["neighbours"] / (best * 1e-3)))}
+
+
+def cpu_mls_normals(n_queries: int = 5000) -> dict:
+    """the oracle's estimateNormalMLS (one thread) on the first queries of
+    the same workload"""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    P = mls_cloud()
+    q = np.arange(0, MLS_N, MLS_STRIDE)[:n_queries]
+    t0 = time.perf_counter()
+    O.estimate_normals_mls(P, np.full_like(P, np.nan), np.arange(MLS_N), q, MLS_RADIUS, 2,
+                           invert=1, vp=np.zeros(3, np.float32))
+    ms = (time.perf_counter() - t0) * 1e3
+    return {"queries": n_queries, "ms": round(ms, 1),
+            "ms_per_250k_queries": round(ms * (MLS_N // MLS_STRIDE) / n_queries, 1)}
+
+
 def cpu_mesh_sampling() -> float:
     """the same sampleMesh call by the C oracle (one thread), ms"""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -394,6 +441,7 @@ def main():
         "measured_ceilings": measured_ceilings(),
         "ply_ingest": ply_ingest(det, w["scene"].points),
         "mesh_sampling": mesh_sampling(),
+        "mls_normals": mls_normals(),
         "results": found,
         "tuning": det.get_tuning(),
     }
@@ -413,7 +461,8 @@ def main():
                                "stage_ms": {k: round(float(cst.timeMs[i]), 1) for i, k in
                                             enumerate(("sample_and_normals", "vote", "cluster",
                                                        "refine"))},
-                               "mesh_sampling_ms": round(cpu_mesh_sampling(), 1)}
+                               "mesh_sampling_ms": round(cpu_mesh_sampling(), 1),
+                               "mls_normals": cpu_mls_normals()}
     print(json.dumps(out))
 
 

@@ -257,6 +257,24 @@ PPF_API int ppf_estimate_normals(float *pts, float *nrm, size_t n, const int *va
                                  size_t nValid, const int *query, size_t nq, int k, int smooth,
                                  int invert, const float viewPoint[3]);
 
+/* estimateNormalMLS (privateUtil.h:13-14, util.cpp:336-457): for each
+ * query, PCA of its neighbours within radius among the valid points, then
+ * a weighted polynomial fit (order 1..3) of their heights in the PCA
+ * plane's Darboux frame; the query moves onto the fitted surface and its
+ * normal follows the fit's slopes, oriented towards viewPoint (or +z when
+ * it is NULL or not finite; away when invert).  pts/nrm (n x 3) in-out;
+ * neighbours are read from the positions before any point moves.  stats
+ * may be NULL. */
+typedef struct {
+    long long neighbours; /* sum of the queries' neighbourhood sizes */
+    int       queries;
+    float     deviceMs;   /* search + sort + fit on the device */
+} ppf_mls_stats;
+PPF_API int ppf_estimate_normals_mls(float *pts, float *nrm, size_t n, const int *valid,
+                                     size_t nValid, const int *query, size_t nq, float radius,
+                                     int order, int invert, const float viewPoint[3],
+                                     ppf_mls_stats *stats);
+
 /* -- ICP on explicit hypotheses (ICP::regist with a prebuilt search
  * structure and several initial poses, icp.h:86-90 / icp.cpp:214-278) -- */
 typedef struct ppf_icp_target_s *ppf_icp_target;

@@ -155,6 +155,12 @@ def lib():
         L.orc_voxel_sample.argtypes = [C.c_int, fp, C.c_void_p, C.c_int, C.c_float, fp, ip]
         L.orc_greedy_sample.restype = C.c_int
         L.orc_greedy_sample.argtypes = [C.c_int, fp, ip, C.c_int, C.c_float, ip]
+        L.orc_expf.restype = C.c_float
+        L.orc_expf.argtypes = [C.c_float]
+        L.orc_unit_orthogonal.argtypes = [fp, fp]
+        L.orc_estimate_normals_mls.restype = C.c_long
+        L.orc_estimate_normals_mls.argtypes = [C.c_int, fp, fp, ip, C.c_int, ip, C.c_int,
+                                               C.c_float, C.c_int, C.c_int, C.c_void_p]
         L.orc_philox4x32_10.argtypes = [up, up]
         L.orc_sample_mesh.restype = C.c_int
         L.orc_sample_mesh.argtypes = [C.c_int, fp, C.c_int, C.c_void_p, C.c_float, C.c_uint64,
@@ -289,6 +295,20 @@ def voxel_sample(pts, step, box_min, subset=None):
     n = lib().orc_voxel_sample(len(pts), pts, _ptr(sub), 0 if sub is None else len(sub), step,
                                f32(box_min), out)
     return out[:n]
+
+
+def estimate_normals_mls(pts, nrm, valid, query, radius, order=2, invert=0, vp=None):
+    """estimateNormalMLS (util.cpp:336-457), snapshot semantics:
+    (points, normals, total neighbourhood size)."""
+    pts = f32(pts).copy()
+    nrm = f32(nrm).copy()
+    vpa = None if vp is None else f32(vp)
+    tot = lib().orc_estimate_normals_mls(len(pts), pts, nrm, np.ascontiguousarray(valid, np.int32),
+                                         len(valid), np.ascontiguousarray(query, np.int32),
+                                         len(query), radius, order, invert, _ptr(vpa))
+    if tot < 0:
+        raise ValueError("orc_estimate_normals_mls: order or radius out of range")
+    return pts, nrm, tot
 
 
 def philox4x32_10(ctr, key):

@@ -878,6 +878,254 @@ static int pca_normal(const orc_grid *g, const float *pos, int id, int k, float 
     return cnt;
 }
 
+/* ====================================================================== */
+/* MLS normals                                                            */
+/* ====================================================================== */
+
+/* e^x after Cephes expf.c (S. L. Moshier): n = floor(x log2 e + 1/2),
+ * g = (x - n*0.693359375) - n*(-2.12194440e-4), e^g by a degree-5
+ * polynomial, times 2^n applied as two exact power-of-two factors. */
+float orc_expf(float xx) {
+    if (isnan(xx))
+        return xx;
+    if (xx > 88.72283905206835f)
+        return INFINITY;
+    if (xx < -103.278929903431851103f)
+        return 0.0f;
+    float t = 1.44269504088896341f * xx;
+    float z = floorf(t + 0.5f);
+    float x = xx - z * 0.693359375f;
+    x       = x - z * -2.12194440e-4f;
+    int   n = (int)z;
+    float xx2 = x * x;
+    float p   = 1.9875691500e-4f;
+    p = p * x + 1.3981999507e-3f;
+    p = p * x + 8.3334519073e-3f;
+    p = p * x + 4.1665795894e-2f;
+    p = p * x + 1.6666665459e-1f;
+    p = p * x + 5.0000001201e-1f;
+    float r  = p * xx2 + x;
+    r        = r + 1.0f;
+    int   na = n / 2, nb = n - na;
+    union { uint32_t u; float f; } fa, fb;
+    fa.u = (uint32_t)(na + 127) << 23;
+    fb.u = (uint32_t)(nb + 127) << 23;
+    return (r * fa.f) * fb.f;
+}
+
+/* Eigen's unitOrthogonal for 3-vectors: unless |x| and |y| are both
+ * <= 1e-5 |z| (NumTraits<float>::dummy_precision), (-y, x, 0) times the
+ * reciprocal of |(x, y)|; else (0, -z, y) times that of |(y, z)|. */
+void orc_unit_orthogonal(const float n[3], float o[3]) {
+    float lim = fabsf(n[2]) * 1e-5f;
+    if (!(fabsf(n[0]) <= lim) || !(fabsf(n[1]) <= lim)) {
+        float inv = 1.0f / sqrtf(n[0] * n[0] + n[1] * n[1]);
+        o[0] = -n[1] * inv;
+        o[1] = n[0] * inv;
+        o[2] = 0.0f;
+    } else {
+        float inv = 1.0f / sqrtf(n[1] * n[1] + n[2] * n[2]);
+        o[0] = 0.0f;
+        o[1] = -n[2] * inv;
+        o[2] = n[1] * inv;
+    }
+}
+
+static float dot3f(const float a[3], const float b[3]) {
+    float s = a[0] * b[0] + a[1] * b[1];
+    return s + a[2] * b[2];
+}
+
+/* Cholesky solve of the packed lower triangle A (A[i(i+1)/2 + j], j <= i)
+ * in double; 0 when a pivot is not positive. */
+static int chol_solve(int nc, const double *A, double *x) {
+    double L[64];
+#define LL(i, j) L[(i) * ((i) + 1) / 2 + (j)]
+    for (int j = 0; j < nc; j++) {
+        double d = A[j * (j + 1) / 2 + j];
+        for (int k = 0; k < j; k++)
+            d -= LL(j, k) * LL(j, k);
+        if (!(d > 0.0))
+            return 0;
+        LL(j, j) = sqrt(d);
+        for (int i = j + 1; i < nc; i++) {
+            double s = A[i * (i + 1) / 2 + j];
+            for (int k = 0; k < j; k++)
+                s -= LL(i, k) * LL(j, k);
+            LL(i, j) = s / LL(j, j);
+        }
+    }
+    for (int i = 0; i < nc; i++) {
+        double s = x[i];
+        for (int k = 0; k < i; k++)
+            s -= LL(i, k) * x[k];
+        x[i] = s / LL(i, i);
+    }
+    for (int i = nc - 1; i >= 0; i--) {
+        double s = x[i];
+        for (int k = i + 1; k < nc; k++)
+            s -= LL(k, i) * x[k];
+        x[i] = s / LL(i, i);
+    }
+#undef LL
+    return 1;
+}
+
+/* estimateNormalMLS, util.cpp:336-457, per query (single thread, snapshot
+ * positions; sums in ascending neighbour id order):
+ *   neighbours: valid ids with dist2 < r^2 (RadiusResultSet); < 3: skip
+ *   centroid = (sum p) / k, cov = sum (p - c)(p - c)^T in float (:364-367)
+ *   n = eigenvector of the smallest |eigenvalue| (FP64 Jacobi; :369-374),
+ *       written at once; non-finite: stop (:375-376)
+ *   d = -(n.c); dist = p.n + d; mean = p - dist n (:377-381)
+ *   v = unitOrthogonal(n), u = n x v (:383-385)
+ *   for each neighbour (:400-426): e = q - mean, w = exp(-|e|^2 / r^2),
+ *       (uc, vc, f) = (e.u, e.v, e.n), terms T = u^i v^j in the order
+ *       i = 0..order, j = 0..order-i
+ *   M[i][k] = sum (T_i w) T_k (k <= i), b[i] = sum (T_i w) f (:429-431)
+ *   c = M^-1 b by FP64 Cholesky (:432); pivot <= 0 or c not finite: stop
+ *   p' = mean + c0 n; n' = normalize((n - c[order+1] u) - c[1] v)
+ *   flip n' when n'.(vp - p') < 0 (> 0 with invert), n'.z without a
+ *   finite view point (:443-455). */
+long orc_estimate_normals_mls(int n, float *pts, float *nrm, const int *valid, int nValid,
+                              const int *query, int nQuery, float radius, int order, int invert,
+                              const float vp[3]) {
+    if (order < 1 || order > 3 || !(radius > 0.0f) || isinf(radius))
+        return -1;
+    const int nc = (order + 1) * (order + 2) / 2;
+    float     r2 = radius * radius;
+    float    *P0 = (float *)malloc(sizeof(float) * 3 * (size_t)(n ? n : 1));
+    memcpy(P0, pts, sizeof(float) * 3 * (size_t)n);
+    orc_grid g;
+    orc_grid_build(&g, n, P0, valid, nValid, radius);
+    int *nb = (int *)malloc(sizeof(int) * (size_t)(nValid ? nValid : 1));
+    int  hasVp = vp && isfinite(vp[0]) && isfinite(vp[1]) && isfinite(vp[2]);
+    long total = 0;
+    for (int t = 0; t < nQuery; t++) {
+        int          q  = query[t];
+        const float *p  = P0 + 3 * (size_t)q;
+        float       *on = nrm + 3 * (size_t)q;
+        int cnt = 0;
+        if (isfinite(p[0]) && isfinite(p[1]) && isfinite(p[2]))
+            cnt = orc_grid_radius(&g, p, r2, nb, nValid);
+        total += cnt;
+        if (cnt < 3)
+            continue;
+        float s[3] = {0.0f, 0.0f, 0.0f};
+        for (int j = 0; j < cnt; j++)
+            for (int a = 0; a < 3; a++)
+                s[a] = s[a] + P0[3 * (size_t)nb[j] + a];
+        float fk     = (float)cnt;
+        float cen[3] = {s[0] / fk, s[1] / fk, s[2] / fk};
+        float C[6]   = {0, 0, 0, 0, 0, 0}; /* 00 01 02 11 12 22 */
+        for (int j = 0; j < cnt; j++) {
+            const float *w = P0 + 3 * (size_t)nb[j];
+            float x = w[0] - cen[0], y = w[1] - cen[1], z = w[2] - cen[2];
+            C[0] = C[0] + x * x;
+            C[1] = C[1] + x * y;
+            C[2] = C[2] + x * z;
+            C[3] = C[3] + y * y;
+            C[4] = C[4] + y * z;
+            C[5] = C[5] + z * z;
+        }
+        double A[9] = {C[0], C[1], C[2], C[1], C[3], C[4], C[2], C[4], C[5]}, V[9], W[3];
+        orc_jacobi_eig(3, A, V, W);
+        int mi = 0;
+        for (int k = 1; k < 3; k++)
+            if (fabs(W[k]) < fabs(W[mi]))
+                mi = k;
+        double vl = sqrt(V[mi] * V[mi] + V[3 + mi] * V[3 + mi] + V[6 + mi] * V[6 + mi]);
+        float  nn[3] = {(float)(V[mi] / vl), (float)(V[3 + mi] / vl), (float)(V[6 + mi] / vl)};
+        memcpy(on, nn, sizeof nn);
+        if (!(isfinite(nn[0]) && isfinite(nn[1]) && isfinite(nn[2])))
+            continue;
+        float d    = -dot3f(nn, cen);
+        float dist = dot3f(p, nn) + d;
+        float mean[3];
+        for (int a = 0; a < 3; a++)
+            mean[a] = p[a] - dist * nn[a];
+        float va[3], ua[3];
+        orc_unit_orthogonal(nn, va);
+        ua[0] = nn[1] * va[2] - nn[2] * va[1];
+        ua[1] = nn[2] * va[0] - nn[0] * va[2];
+        ua[2] = nn[0] * va[1] - nn[1] * va[0];
+        if (order <= 1 && nc > cnt)
+            continue;
+        float M[55], B[10];
+        memset(M, 0, sizeof M);
+        memset(B, 0, sizeof B);
+        for (int j = 0; j < cnt; j++) {
+            const float *w = P0 + 3 * (size_t)nb[j];
+            float e[3]     = {w[0] - mean[0], w[1] - mean[1], w[2] - mean[2]};
+            float sq       = e[0] * e[0] + e[1] * e[1];
+            sq             = sq + e[2] * e[2];
+            float wt       = orc_expf(-sq / r2);
+            float uc = dot3f(e, ua), vc = dot3f(e, va), f = dot3f(e, nn);
+            float T[10];
+            int   m    = 0;
+            float upow = 1.0f;
+            for (int ui = 0; ui <= order; ui++) {
+                float vpow = 1.0f;
+                for (int vi = 0; vi <= order - ui; vi++) {
+                    T[m++] = upow * vpow;
+                    vpow   = vpow * vc;
+                }
+                upow = upow * uc;
+            }
+            for (int i = 0; i < nc; i++) {
+                float pw = T[i] * wt;
+                B[i]     = B[i] + pw * f;
+                for (int k = 0; k <= i; k++)
+                    M[i * (i + 1) / 2 + k] = M[i * (i + 1) / 2 + k] + pw * T[k];
+            }
+        }
+        double Md[55], c[10];
+        for (int i = 0; i < nc * (nc + 1) / 2; i++)
+            Md[i] = M[i];
+        for (int i = 0; i < nc; i++)
+            c[i] = B[i];
+        if (!chol_solve(nc, Md, c))
+            continue;
+        float cf[10];
+        int   ok = 1;
+        for (int i = 0; i < nc; i++) {
+            cf[i] = (float)c[i];
+            ok    = ok && isfinite(cf[i]);
+        }
+        if (!ok)
+            continue;
+        float np[3], n2[3];
+        for (int a = 0; a < 3; a++) {
+            np[a] = mean[a] + cf[0] * nn[a];
+            n2[a] = (nn[a] - cf[order + 1] * ua[a]) - cf[1] * va[a];
+        }
+        float l2 = n2[0] * n2[0] + n2[1] * n2[1];
+        l2       = l2 + n2[2] * n2[2];
+        if (l2 > 0.0f) {
+            float l = sqrtf(l2);
+            for (int a = 0; a < 3; a++)
+                n2[a] = n2[a] / l;
+        }
+        float val;
+        if (hasVp) {
+            float to[3] = {vp[0] - np[0], vp[1] - np[1], vp[2] - np[2]};
+            val         = dot3f(n2, to);
+        } else {
+            float zz[3] = {0.0f, 0.0f, 1.0f};
+            val         = dot3f(n2, zz);
+        }
+        if (invert ? val > 0.0f : val < 0.0f)
+            for (int a = 0; a < 3; a++)
+                n2[a] = -n2[a];
+        memcpy(pts + 3 * (size_t)q, np, sizeof np);
+        memcpy(on, n2, sizeof n2);
+    }
+    orc_grid_free(&g);
+    free(nb);
+    free(P0);
+    return total;
+}
+
 static void normalize3(float *n) {
     float l = sqrtf(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
     n[0]    = n[0] / l;

@@ -129,6 +129,18 @@ int orc_estimate_normals(int n, float *pts, float *nrm, const int *valid, int nV
                          const int *query, int nQuery, int k, int smooth, int invert,
                          const float viewPoint[3]);
 
+/* ---- MLS normals (util.cpp:336-457), snapshot semantics --------------- */
+/* e^x, Cephes expf method (Cody-Waite reduction, degree-5 polynomial) */
+float orc_expf(float x);
+/* Eigen's unitOrthogonal() of a 3-vector */
+void orc_unit_orthogonal(const float n[3], float o[3]);
+/* estimateNormalMLS over `query` (indices into pts), neighbours within
+ * radius among `valid`, polynomial order 1..3; pts/nrm in-out.  Returns
+ * the total neighbourhood size, or -1 for an order or radius out of range. */
+long orc_estimate_normals_mls(int n, float *pts, float *nrm, const int *valid, int nValid,
+                              const int *query, int nQuery, float radius, int order, int invert,
+                              const float vp[3]);
+
 /* ---- small linear algebra ------------------------------------------- */
 int  orc_jacobi_eig(int n, double *a, double *evec, double *eval);
 void orc_quat_from_rot(const float R[9], float q[4]); /* q = (w,x,y,z) */

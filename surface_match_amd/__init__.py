@@ -16,7 +16,7 @@ import numpy as np
 from . import _native as N
 
 __all__ = ["PointCloud", "TrainParam", "MatchParam", "MatchResult", "Detector", "BoundingBox",
-           "ResidentScene", "readPLY", "writePLY", "sampleMesh"]
+           "ResidentScene", "readPLY", "writePLY", "sampleMesh", "estimateNormalMLS"]
 
 
 @dataclass
@@ -143,6 +143,32 @@ def sampleMesh(pc: PointCloud, radius: float, seed: int = 0x5EED) -> PointCloud:
                                   N.ptr(Q), n.value, C.byref(n)))
     out.point, out.normal, out.box = P, Q, pc.box
     return out
+
+
+def estimateNormalMLS(pc: PointCloud, indices, radius: float, order: int = 2,
+                      invert: bool = False, validIndices=None) -> dict:
+    """estimateNormalMLS (privateUtil.h:13-14, util.cpp:336-457) on the GPU:
+    the points listed in `indices` move onto a weighted polynomial fit
+    (order 1..3) of their neighbours within `radius` and take its normal,
+    oriented towards pc.viewPoint (or +z).  The reference's kd-tree holds
+    the cloud's valid points; here `validIndices` (default: removeNan).
+    Updates pc.point / pc.normal in place (normals are created as NaN when
+    absent); returns the neighbour total and the device time."""
+    p = N.f32(np.reshape(pc.point, (-1, 3))).copy()
+    n = (N.f32(np.reshape(pc.normal, (-1, 3))).copy() if pc.hasNormal()
+         else np.full_like(p, np.nan))
+    if validIndices is None:
+        validIndices = np.flatnonzero(np.isfinite(p).all(1))
+    v = N.i32(validIndices)
+    q = N.i32(indices)
+    vp = N.f32(np.asarray(pc.viewPoint, np.float32).ravel())
+    st = N.MlsStats()
+    N.check(N.lib().ppf_estimate_normals_mls(N.ptr(p), N.ptr(n), len(p), N.ptr(v), len(v),
+                                             N.ptr(q), len(q), radius, order, int(bool(invert)),
+                                             N.ptr(vp), C.byref(st)))
+    pc.point, pc.normal = p, n
+    return {"neighbours": int(st.neighbours), "queries": int(st.queries),
+            "device_ms": float(st.deviceMs)}
 
 
 @dataclass

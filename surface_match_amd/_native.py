@@ -80,6 +80,10 @@ class Tuning(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class MlsStats(C.Structure):
+    _fields_ = [("neighbours", C.c_longlong), ("queries", C.c_int), ("deviceMs", C.c_float)]
+
+
 class TuneReport(C.Structure):
     _fields_ = [("best", Tuning), ("voteMs", C.c_float), ("icpMs", C.c_float),
                 ("configsTried", C.c_int), ("normalsMs", C.c_float)]
@@ -99,7 +103,7 @@ EXPORTS = [
     "ppf_autotune", "ppf_set_tuning", "ppf_get_tuning", "ppf_match_scenes_resident",
     "ppf_scene_inliers", "ppf_measure_ceilings", "ppf_ply_read", "ppf_ply_info", "ppf_ply_copy",
     "ppf_ply_free", "ppf_ply_write", "ppf_scene_upload_ply", "ppf_train_model_mesh",
-    "ppf_sample_mesh",
+    "ppf_sample_mesh", "ppf_estimate_normals_mls",
 ]
 
 
@@ -159,6 +163,8 @@ def lib() -> C.CDLL:
         "ppf_train_model_mesh": (i, [vp, vp, vp, sz, vp, sz, vp, vp, f, C.POINTER(TrainParam),
                                      C.c_uint64]),
         "ppf_sample_mesh": (i, [vp, sz, vp, sz, f, C.c_uint64, vp, vp, sz, C.POINTER(sz)]),
+        "ppf_estimate_normals_mls": (i, [vp, vp, sz, vp, sz, vp, sz, f, i, i, vp,
+                                         C.POINTER(MlsStats)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

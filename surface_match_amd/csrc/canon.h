@@ -382,6 +382,101 @@ PPF_HD void compose_pose(const float Ts[12], const float Tm[12], float c, float 
     }
 }
 
+// e^x by the method of the Cephes library's expf.c (S. L. Moshier): x =
+// g + n ln2 with n = floor(x log2(e) + 1/2) and ln2 split into
+// 0.693359375 - 2.12194440e-4 (Cody-Waite), a degree-5 polynomial for
+// e^g on [-ln2/2, ln2/2], then the exact scaling by 2^n.  Fixed operation
+// order, so host and device agree bit for bit.
+PPF_HD float expf_c(float xx) {
+    if (xx != xx)
+        return xx;
+    if (xx > 88.72283905206835f)
+        return INFINITY;
+    if (xx < -103.278929903431851103f)
+        return 0.0f;
+    float       x = xx;
+    const float z = floorf(1.44269504088896341f * x + 0.5f);
+    x             = x - z * 0.693359375f;
+    x             = x - z * -2.12194440e-4f;
+    const int   n = (int)z;
+    const float q = x * x;
+    float       p = 1.9875691500e-4f * x + 1.3981999507e-3f;
+    p             = p * x + 8.3334519073e-3f;
+    p             = p * x + 4.1665795894e-2f;
+    p             = p * x + 1.6666665459e-1f;
+    p             = p * x + 5.0000001201e-1f;
+    float r       = (p * q + x) + 1.0f;
+    // r * 2^n in two exact power-of-two steps (n in [-150, 128])
+    const int n1 = n / 2, n2 = n - n / 2;
+    union { uint32_t u; float f; } s1{(uint32_t)(n1 + 127) << 23}, s2{(uint32_t)(n2 + 127) << 23};
+    return (r * s1.f) * s2.f;
+}
+
+// Eigen's unitOrthogonal() of a 3-vector: (-y, x, 0) / |(x, y)| unless both
+// |x| and |y| are at most 1e-5 |z|, else (0, -z, y) / |(y, z)|; the
+// division is a multiplication by the reciprocal of the norm
+PPF_HD void unit_orthogonal(const float n[3], float o[3]) {
+    const float az = fabsf(n[2]) * 1e-5f;
+    if (!(fabsf(n[0]) <= az) || !(fabsf(n[1]) <= az)) {
+        const float inv = 1.0f / sqrtf(n[0] * n[0] + n[1] * n[1]);
+        o[0]            = -n[1] * inv;
+        o[1]            = n[0] * inv;
+        o[2]            = 0.0f;
+    } else {
+        const float inv = 1.0f / sqrtf(n[1] * n[1] + n[2] * n[2]);
+        o[0]            = 0.0f;
+        o[1]            = -n[2] * inv;
+        o[2]            = n[1] * inv;
+    }
+}
+
+// a x b as Eigen's cross() forms it
+PPF_HD void cross3(const float a[3], const float b[3], float o[3]) {
+    o[0] = a[1] * b[2] - a[2] * b[1];
+    o[1] = a[2] * b[0] - a[0] * b[2];
+    o[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+PPF_HD float dot3(const float a[3], const float b[3]) {
+    return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2];
+}
+
+// Solve A x = b for a symmetric positive definite N x N matrix given by its
+// lower triangle (row-major packed: A[i*(i+1)/2 + j], j <= i) by Cholesky in
+// FP64, sums in ascending index order.  False when a pivot is not positive.
+template <int N>
+PPF_HD bool cholesky_solve(const double *A, double *x) {
+    double L[N * (N + 1) / 2];
+    for (int j = 0; j < N; j++) {
+        double d = A[j * (j + 1) / 2 + j];
+        for (int k = 0; k < j; k++)
+            d = d - L[j * (j + 1) / 2 + k] * L[j * (j + 1) / 2 + k];
+        if (!(d > 0.0))
+            return false;
+        const double ljj       = sqrt(d);
+        L[j * (j + 1) / 2 + j] = ljj;
+        for (int i = j + 1; i < N; i++) {
+            double s = A[i * (i + 1) / 2 + j];
+            for (int k = 0; k < j; k++)
+                s = s - L[i * (i + 1) / 2 + k] * L[j * (j + 1) / 2 + k];
+            L[i * (i + 1) / 2 + j] = s / ljj;
+        }
+    }
+    for (int i = 0; i < N; i++) {  // L y = b
+        double s = x[i];
+        for (int k = 0; k < i; k++)
+            s = s - L[i * (i + 1) / 2 + k] * x[k];
+        x[i] = s / L[i * (i + 1) / 2 + i];
+    }
+    for (int i = N - 1; i >= 0; i--) {  // L^T x = y
+        double s = x[i];
+        for (int k = i + 1; k < N; k++)
+            s = s - L[k * (k + 1) / 2 + i] * x[k];
+        x[i] = s / L[i * (i + 1) / 2 + i];
+    }
+    return true;
+}
+
 // Cyclic Jacobi eigen-decomposition of a symmetric n x n FP64 matrix (Golub
 // & Van Loan, Matrix Computations, sec. 8.5).  a is destroyed; v receives the
 // eigenvectors as columns, w the eigenvalues.

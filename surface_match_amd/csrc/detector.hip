@@ -1909,6 +1909,68 @@ PPF_API int ppf_estimate_normals(float *pts, float *nrm, size_t n, const int *va
     });
 }
 
+PPF_API int ppf_estimate_normals_mls(float *pts, float *nrm, size_t n, const int *valid,
+                                     size_t nValid, const int *query, size_t nq, float radius,
+                                     int order, int invert, const float viewPoint[3],
+                                     ppf_mls_stats *stats) {
+    PPF_REQUIRE(pts && nrm && (valid || nValid == 0) && (query || nq == 0), "null argument");
+    return guarded([&] {
+        for (size_t t = 0; t < nq; t++)
+            if (query[t] < 0 || (size_t)query[t] >= n)
+                throw std::range_error("Invalid Input: MLS query index out of range");
+        for (size_t t = 0; t < nValid; t++)
+            if (valid[t] < 0 || (size_t)valid[t] >= n)
+                throw std::range_error("Invalid Input: MLS valid index out of range");
+        hipStream_t s = util_stream();
+        static Workspace *wsp = new Workspace();  // lives until exit, never destroyed
+        Workspace        &ws  = *wsp;
+        TmpCloud P, O, N;
+        P.up(pts, (int)n, s);
+        O.up(pts, (int)n, s);
+        N.up(nrm, (int)n, s);
+        DevBuf vd, qd;
+        int   *v  = vd.as<int>(std::max<size_t>(nValid, 1));
+        int   *qq = qd.as<int>(std::max<size_t>(nq, 1));
+        PPF_HIP_CHECK(hipMemcpyAsync(v, valid, sizeof(int) * nValid, hipMemcpyHostToDevice, s));
+        PPF_HIP_CHECK(hipMemcpyAsync(qq, query, sizeof(int) * nq, hipMemcpyHostToDevice, s));
+        hipEvent_t ev[2];
+        PPF_HIP_CHECK(hipEventCreate(&ev[0]));
+        PPF_HIP_CHECK(hipEventCreate(&ev[1]));
+        PPF_HIP_CHECK(hipEventRecord(ev[0], s));
+        MlsStats st;
+        estimate_normals_mls(ws, (int)n, P.x.get<float>(), P.y.get<float>(), P.z.get<float>(),
+                             O.x.get<float>(), O.y.get<float>(), O.z.get<float>(),
+                             N.x.get<float>(), N.y.get<float>(), N.z.get<float>(), v, (int)nValid,
+                             qq, (int)nq, radius, order, invert, viewPoint, &st, s);
+        PPF_HIP_CHECK(hipEventRecord(ev[1], s));
+        DevBuf           outb, ad;  // outlive the final synchronisation
+        std::vector<int> all(n);
+        if (n) {
+            float *o    = outb.as<float>(n * 3);
+            int   *alld = ad.as<int>(n);
+            std::iota(all.begin(), all.end(), 0);
+            PPF_HIP_CHECK(hipMemcpyAsync(alld, all.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
+            gather_aos_kernel<<<div_up(n, 256), 256, 0, s>>>(alld, (int)n, O.x.get<float>(),
+                                                              O.y.get<float>(), O.z.get<float>(), o);
+            PPF_HIP_CHECK(hipMemcpyAsync(pts, o, sizeof(float) * 3 * n, hipMemcpyDeviceToHost, s));
+            PPF_HIP_CHECK(hipStreamSynchronize(s));
+            gather_aos_kernel<<<div_up(n, 256), 256, 0, s>>>(alld, (int)n, N.x.get<float>(),
+                                                              N.y.get<float>(), N.z.get<float>(), o);
+            PPF_HIP_CHECK(hipMemcpyAsync(nrm, o, sizeof(float) * 3 * n, hipMemcpyDeviceToHost, s));
+        }
+        PPF_HIP_CHECK(hipStreamSynchronize(s));
+        float ms = 0.0f;
+        PPF_HIP_CHECK(hipEventElapsedTime(&ms, ev[0], ev[1]));
+        (void)hipEventDestroy(ev[0]);
+        (void)hipEventDestroy(ev[1]);
+        if (stats) {
+            stats->neighbours = st.neighbours;
+            stats->queries    = st.queries;
+            stats->deviceMs   = ms;
+        }
+    });
+}
+
 PPF_API int ppf_icp_target_create(const float *pts, const float *nrm, size_t n, float cell,
                                   ppf_icp_target *out) {
     PPF_REQUIRE(pts && nrm && out && n > 0, "bad argument");

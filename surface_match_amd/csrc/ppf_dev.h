@@ -22,7 +22,9 @@ struct Workspace {
     DevBuf voxKey, voxBest, voxFlag, voxIota, cubTmp, counter;
     DevBuf p0x, p0y, p0z, n0x, n0y, n0z, rawX, rawY, rawZ, cenX, cenY, cenZ, have, touch,
         touchList, nbq, cntq, outN, mark, qidx;
-    Grid   gNormal, gVote, gSparse, gDense, gFull;
+    Grid   gNormal, gVote, gSparse, gDense, gFull, gMls;
+    // MLS normals: per-query neighbour counts, 64-bit offsets, ids (twice)
+    DevBuf mlsCnt, mlsOff, mlsIds, mlsIds2;
     // vote
     DevBuf kpX, kpY, kpZ, kpNX, kpNY, kpNZ, cand, candTile, poses, voteStats, accDump, voteCnt,
         smallCnt, hitStart, hitLen, hitAlpha, hitBucket, hitCount, splits, kpKey, kpKey2, kpOrd,
@@ -112,6 +114,21 @@ void estimate_normals(Workspace &ws, int n, float *PX, float *PY, float *PZ, flo
                       float *NY, float *NZ, const int *valid, int nValid, const int *query,
                       int nq, int k, int smooth, int invert, const float vp[3], float cellHint,
                       hipStream_t s);
+
+// estimateNormalMLS (util.cpp:336-457) for the queries (indices into the
+// n-point cloud PX/PY/PZ, which is not modified): moved points into OX/OY/OZ
+// (which must hold copies of the inputs), normals updated in NX/NY/NZ.
+// Neighbours come from the valid points.  Asynchronous on s after its one
+// read-back of the neighbour total.
+constexpr int MLS_MAX_ORDER = 3;
+struct MlsStats {
+    long long neighbours = 0;
+    int       queries    = 0;
+};
+void estimate_normals_mls(Workspace &ws, int n, const float *PX, const float *PY, const float *PZ,
+                          float *OX, float *OY, float *OZ, float *NX, float *NY, float *NZ,
+                          const int *valid, int nValid, const int *query, int nq, float radius,
+                          int order, int invert, const float vp[3], MlsStats *st, hipStream_t s);
 
 // ---------------------------------------------------------------- vote
 

@@ -59,4 +59,7 @@ def H(tmp_path_factory):
     L.ch_uniform01.restype = C.c_float
     L.ch_uniform01.argtypes = [C.c_uint32]
     L.ch_mesh_point.argtypes = [fp, fp, fp, C.c_float, C.c_float, fp]
+    L.ch_expf.restype = C.c_float
+    L.ch_expf.argtypes = [C.c_float]
+    L.ch_unit_orthogonal.argtypes = [fp, fp]
     return L

@@ -23,6 +23,8 @@ void ch_pose(const float *Ts, const float *Tm, float c, float s, float *out) {
 }
 void ch_quat(const float *R, float *q) { ppfc::quat_from_rot(R, q); }
 void ch_philox(uint32_t *c, uint32_t k0, uint32_t k1) { ppfc::philox4x32_10(c, k0, k1); }
+float ch_expf(float x) { return ppfc::expf_c(x); }
+void  ch_unit_orthogonal(const float *n, float *o) { ppfc::unit_orthogonal(n, o); }
 float ch_uniform01(uint32_t u) { return ppfc::uniform01(u); }
 void ch_mesh_point(const float *p0, const float *p1, const float *p2, float r1, float r2,
                    float *out) {
