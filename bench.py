@@ -346,12 +346,18 @@ def main():
     ap.add_argument("--no-icp-batching", action="store_true")
     ap.add_argument("--autotune", action="store_true")
     ap.add_argument("--no-scene-batching", action="store_true")
+    ap.add_argument("--only", choices=["mls", "mesh"],
+                    help="run one side measurement (MLS normals or mesh sampling) and print "
+                         "its JSON, e.g. to profile it")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the PPF path runs on a single MI355X "
               "(multi-GPU sharding is a design note, not a measured configuration)",
               file=sys.stderr)
         sys.exit(2)
+    if args.only:
+        print(json.dumps(mls_normals() if args.only == "mls" else mesh_sampling()))
+        return
 
     from surface_match_amd import Detector, PointCloud, MatchParam, ResidentScene, synth
 

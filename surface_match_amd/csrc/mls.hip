@@ -44,12 +44,22 @@ __device__ __forceinline__ void cell_range(const GridView &g, float qx, float qy
 }
 
 // FILL = false: counts[t] = neighbours of query t; true: their ids into
-// ids[offs[t] ..]
+// ids[offs[t] ..] (in no particular order; they are sorted afterwards).
+// MLS_GROUP lanes share a query: they test consecutive slots of each run of
+// candidate slots, and a wave ballot gives every hit its output position,
+// so loads and stores of a group are contiguous.
+constexpr int MLS_GROUP = 16;
+
 template <bool FILL>
-__global__ void mls_neighbours_kernel(GridView g, const float *X, const float *Y, const float *Z,
-                                      const int *query, int nq, float r2, int *counts,
-                                      const long long *offs, int *ids) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(256) mls_neighbours_kernel(GridView g, const float *X,
+                                                             const float *Y, const float *Z,
+                                                             const int *query, int nq, float r2,
+                                                             int *counts, const long long *offs,
+                                                             int *ids) {
+    const int t    = (int)((blockIdx.x * blockDim.x + threadIdx.x) / MLS_GROUP);
+    const int lane = (int)(threadIdx.x & 63u);
+    const int l    = lane & (MLS_GROUP - 1);
+    const int gsh  = lane & ~(MLS_GROUP - 1);
     if (t >= nq)
         return;
     const int   q  = query[t];
@@ -59,26 +69,74 @@ __global__ void mls_neighbours_kernel(GridView g, const float *X, const float *Y
     if (g.n > 0 && isfinite(qx) && isfinite(qy) && isfinite(qz)) {
         int lo[3], hi[3];
         cell_range(g, qx, qy, qz, r2, lo, hi);
+        auto scan = [&](int s0, int e0) {
+            for (int base = s0; base < e0; base += MLS_GROUP) {
+                const int  k   = base + l;
+                const bool hit = k < e0 && ppfc::dist2(qx, qy, qz, g.x[k], g.y[k], g.z[k]) < r2;
+                const uint64_t m  = __ballot(hit);
+                const uint32_t gm = (uint32_t)(m >> gsh) & ((1u << MLS_GROUP) - 1u);
+                if (FILL && hit)
+                    ids[o + __popc(gm & ((1u << l) - 1u))] = g.id[k];
+                o += __popc(gm);
+                c += __popc(gm);
+            }
+        };
         for (int cz = lo[2]; cz <= hi[2]; cz++)
-            for (int cy = lo[1]; cy <= hi[1]; cy++)
-                for (int cx = lo[0]; cx <= hi[0]; cx++) {
-                    int s0, c0;
-                    if (!cell_lookup(g, cx, cy, cz, s0, c0))
-                        continue;
-                    for (int k = s0; k < s0 + c0; k++)
-                        if (ppfc::dist2(qx, qy, qz, g.x[k], g.y[k], g.z[k]) < r2) {
-                            if (FILL)
-                                ids[o++] = g.id[k];
-                            c++;
-                        }
+            for (int cy = lo[1]; cy <= hi[1]; cy++) {
+                if (g.cstart) {  // a row of cells is one run of sorted slots
+                    const int row = (cz * g.dy + cy) * g.dx;
+                    scan(g.cstart[row + lo[0]], g.cstart[row + hi[0] + 1]);
+                } else {
+                    for (int cx = lo[0]; cx <= hi[0]; cx++) {
+                        int s0, c0;
+                        if (cell_lookup(g, cx, cy, cz, s0, c0))
+                            scan(s0, s0 + c0);
+                    }
                 }
+            }
     }
-    if (!FILL)
+    if (!FILL && l == 0)
         counts[t] = c;
+}
+
+// Morton code of each query's grid cell (queries at non-finite positions
+// last), so that the queries a wave processes are close in space and share
+// neighbours in cache
+__device__ __forceinline__ uint32_t spread10(uint32_t v) {
+    v &= 0x3FFu;
+    v = (v | (v << 16)) & 0x030000FFu;
+    v = (v | (v << 8)) & 0x0300F00Fu;
+    v = (v | (v << 4)) & 0x030C30C3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+
+__global__ void mls_query_key_kernel(GridView g, const float *X, const float *Y, const float *Z,
+                                     const int *query, int nq, uint32_t *key, int *val) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nq)
+        return;
+    const int   q  = query[t];
+    const float qx = X[q], qy = Y[q], qz = Z[q];
+    uint32_t    k  = 0xFFFFFFFFu;
+    if (isfinite(qx) && isfinite(qy) && isfinite(qz))
+        k = spread10((uint32_t)cell_coord(qx, g.ox, g.cell, g.dx)) |
+            (spread10((uint32_t)cell_coord(qy, g.oy, g.cell, g.dy)) << 1) |
+            (spread10((uint32_t)cell_coord(qz, g.oz, g.cell, g.dz)) << 2);
+    key[t] = k;
+    val[t] = q;
+}
+
+__global__ void mls_pack_kernel(const float *X, const float *Y, const float *Z, int n,
+                                float4 *P) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        P[i] = make_float4(X[i], Y[i], Z[i], 0.0f);
 }
 
 struct MlsArgs {
     const float     *X, *Y, *Z;        // snapshot positions (all points)
+    const float4    *P4;               // the same, one 16-byte load per point
     const int       *query;
     const long long *offs;             // nq + 1
     const int       *nb;               // neighbour ids, ascending per query
@@ -87,6 +145,25 @@ struct MlsArgs {
     float           *OX, *OY, *OZ;     // output positions (start as copies)
     float           *NX, *NY, *NZ;     // normals, in place (never read as neighbours)
 };
+
+// body(p) for the neighbours b..e-1 in order, with the loads of four
+// neighbours in flight at a time
+template <class F>
+__device__ __forceinline__ void for_each_nb(const int *__restrict__ nb,
+                                            const float4 *__restrict__ P4, long long b,
+                                            long long e, F &&body) {
+    long long j = b;
+    for (; j + 4 <= e; j += 4) {
+        const int    i0 = nb[j], i1 = nb[j + 1], i2 = nb[j + 2], i3 = nb[j + 3];
+        const float4 w0 = P4[i0], w1 = P4[i1], w2 = P4[i2], w3 = P4[i3];
+        body(w0);
+        body(w1);
+        body(w2);
+        body(w3);
+    }
+    for (; j < e; j++)
+        body(P4[nb[j]]);
+}
 
 template <int ORDER>
 __global__ void __launch_bounds__(128) mls_kernel(MlsArgs a) {
@@ -102,25 +179,23 @@ __global__ void __launch_bounds__(128) mls_kernel(MlsArgs a) {
         return;
     // centroid and covariance (util.cpp:364-367)
     float sx = 0.0f, sy = 0.0f, sz = 0.0f;
-    for (long long j = b; j < e; j++) {
-        const int id = a.nb[j];
-        sx           = sx + a.X[id];
-        sy           = sy + a.Y[id];
-        sz           = sz + a.Z[id];
-    }
+    for_each_nb(a.nb, a.P4, b, e, [&](const float4 &w) {
+        sx = sx + w.x;
+        sy = sy + w.y;
+        sz = sz + w.z;
+    });
     const float fk     = (float)cnt;
     const float cen[3] = {sx / fk, sy / fk, sz / fk};
     float       c00 = 0.0f, c01 = 0.0f, c02 = 0.0f, c11 = 0.0f, c12 = 0.0f, c22 = 0.0f;
-    for (long long j = b; j < e; j++) {
-        const int   id = a.nb[j];
-        const float x = a.X[id] - cen[0], y = a.Y[id] - cen[1], z = a.Z[id] - cen[2];
+    for_each_nb(a.nb, a.P4, b, e, [&](const float4 &w) {
+        const float x = w.x - cen[0], y = w.y - cen[1], z = w.z - cen[2];
         c00           = c00 + x * x;
         c01           = c01 + x * y;
         c02           = c02 + x * z;
         c11           = c11 + y * y;
         c12           = c12 + y * z;
         c22           = c22 + z * z;
-    }
+    });
     double A3[9] = {c00, c01, c02, c01, c11, c12, c02, c12, c22}, V[9], W[3];
     ppfc::jacobi_eig<3>(A3, V, W);
     int mi = 0;
@@ -152,9 +227,8 @@ __global__ void __launch_bounds__(128) mls_kernel(MlsArgs a) {
 #pragma unroll
     for (int i = 0; i < NC; i++)
         R[i] = 0.0f;
-    for (long long j = b; j < e; j++) {
-        const int   id    = a.nb[j];
-        const float de[3] = {a.X[id] - mean[0], a.Y[id] - mean[1], a.Z[id] - mean[2]};
+    for_each_nb(a.nb, a.P4, b, e, [&](const float4 &pw4) {
+        const float de[3] = {pw4.x - mean[0], pw4.y - mean[1], pw4.z - mean[2]};
         const float sq    = (de[0] * de[0] + de[1] * de[1]) + de[2] * de[2];
         const float w     = ppfc::expf_c(-sq / a.r2);
         const float uc = ppfc::dot3(de, uA), vc = ppfc::dot3(de, vA), f = ppfc::dot3(de, n);
@@ -179,7 +253,7 @@ __global__ void __launch_bounds__(128) mls_kernel(MlsArgs a) {
             for (int k = 0; k <= i; k++)
                 M[i * (i + 1) / 2 + k] = M[i * (i + 1) / 2 + k] + pw * T[k];
         }
-    }
+    });
     double Md[NA], c[NC];
 #pragma unroll
     for (int i = 0; i < NA; i++)
@@ -241,11 +315,26 @@ void estimate_normals_mls(Workspace &ws, int n, const float *PX, const float *PY
     const float r2 = radius * radius;
     Grid       &g  = ws.gMls;
     grid_build(g, PX, PY, PZ, nullptr, nullptr, nullptr, valid, nValid, radius, s);
+    // process the queries in Morton order of their cells (each query's result
+    // depends only on itself, so the order changes no value)
+    {
+        uint32_t *k1 = ws.mlsKey.as<uint32_t>(nq), *k2 = ws.mlsKey2.as<uint32_t>(nq);
+        int      *v1 = ws.mlsQ.as<int>(nq), *v2 = ws.mlsQ2.as<int>(nq);
+        mls_query_key_kernel<<<div_up(nq, 256), 256, 0, s>>>(g.view, PX, PY, PZ, query, nq, k1,
+                                                             v1);
+        PPF_HIP_CHECK(hipGetLastError());
+        size_t tb = 0;
+        PPF_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, k1, k2, v1, v2, nq, 0, 32, s));
+        PPF_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(ws.cubTmp.reserve(tb), tb, k1, k2, v1, v2,
+                                                         nq, 0, 32, s));
+        query = v2;
+    }
     int       *cnt  = ws.mlsCnt.as<int>((size_t)nq + 1);
     long long *offs = ws.mlsOff.as<long long>((size_t)nq + 1);
-    const int  blocks = div_up(nq, 128);
-    mls_neighbours_kernel<false><<<blocks, 128, 0, s>>>(g.view, PX, PY, PZ, query, nq, r2, cnt,
-                                                         nullptr, nullptr);
+    const int  blocks  = div_up(nq, 128);
+    const int  nblocks = div_up(nq, 256 / MLS_GROUP);
+    mls_neighbours_kernel<false><<<nblocks, 256, 0, s>>>(g.view, PX, PY, PZ, query, nq, r2, cnt,
+                                                          nullptr, nullptr);
     PPF_HIP_CHECK(hipGetLastError());
     // exclusive scan of the counts in 64-bit, total in offs[nq]
     PPF_HIP_CHECK(hipMemsetAsync(cnt + nq, 0, sizeof(int), s));
@@ -262,8 +351,8 @@ void estimate_normals_mls(Workspace &ws, int n, const float *PX, const float *PY
     int *ids  = ws.mlsIds.as<int>((size_t)std::max<long long>(total, 1));
     int *ids2 = ws.mlsIds2.as<int>((size_t)std::max<long long>(total, 1));
     if (total > 0) {
-        mls_neighbours_kernel<true><<<blocks, 128, 0, s>>>(g.view, PX, PY, PZ, query, nq, r2,
-                                                            nullptr, offs, ids);
+        mls_neighbours_kernel<true><<<nblocks, 256, 0, s>>>(g.view, PX, PY, PZ, query, nq, r2,
+                                                             nullptr, offs, ids);
         PPF_HIP_CHECK(hipGetLastError());
         int bits = 1;
         while (bits < 31 && (1ll << bits) < (long long)n)
@@ -277,8 +366,11 @@ void estimate_normals_mls(Workspace &ws, int n, const float *PX, const float *PY
                                                                  k2, (int)total, nq, offs,
                                                                  offs + 1, 0, bits, s));
     }
+    float4 *P4 = ws.mlsP4.as<float4>((size_t)std::max(n, 1));
+    mls_pack_kernel<<<div_up(std::max(n, 1), 256), 256, 0, s>>>(PX, PY, PZ, n, P4);
+    PPF_HIP_CHECK(hipGetLastError());
     MlsArgs a{};
-    a.X = PX, a.Y = PY, a.Z = PZ;
+    a.X = PX, a.Y = PY, a.Z = PZ, a.P4 = P4;
     a.query = query, a.offs = offs, a.nb = ids2, a.nq = nq, a.invert = invert;
     a.hasVp = vp && std::isfinite(vp[0]) && std::isfinite(vp[1]) && std::isfinite(vp[2]);
     a.r2    = r2;

@@ -24,7 +24,7 @@ struct Workspace {
         touchList, nbq, cntq, outN, mark, qidx;
     Grid   gNormal, gVote, gSparse, gDense, gFull, gMls;
     // MLS normals: per-query neighbour counts, 64-bit offsets, ids (twice)
-    DevBuf mlsCnt, mlsOff, mlsIds, mlsIds2;
+    DevBuf mlsCnt, mlsOff, mlsIds, mlsIds2, mlsP4, mlsKey, mlsKey2, mlsQ, mlsQ2;
     // vote
     DevBuf kpX, kpY, kpZ, kpNX, kpNY, kpNZ, cand, candTile, poses, voteStats, accDump, voteCnt,
         smallCnt, hitStart, hitLen, hitAlpha, hitBucket, hitCount, splits, kpKey, kpKey2, kpOrd,
