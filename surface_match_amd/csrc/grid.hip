@@ -556,6 +556,128 @@ __global__ void __launch_bounds__(128) normals_point_kernel(NormArgs a, const in
     store_estimate<KMAX>(a, id, want, L);
 }
 
+// ---- cooperative kNN: KNN_GROUP lanes per point
+//
+// The k-best list (k <= KNN_GROUP) is spread over the group, lane i holding
+// entry i.  Each batch of KNN_GROUP consecutive slots of a visited cell is
+// tested by the group at once (contiguous loads); the candidates that beat
+// the current k-th entry are inserted one by one: a ballot finds the suffix
+// of entries the candidate precedes, entries shift one lane up.  The list
+// ends as the k smallest by (dist2, id), the same set and order as
+// grid_knn's, since both visit every cell that can hold a better point.
+constexpr int  KNN_GROUP     = 16;
+constexpr bool NORMALS_COOP  = true;
+
+__device__ __forceinline__ void coop_knn(const GridView &g, float qx, float qy, float qz, int k,
+                                         int l, int gsh, float &myD, int &myId, int &cnt) {
+    constexpr uint32_t GMASK = (1u << KNN_GROUP) - 1u;
+    myD      = FLT_MAX;
+    myId     = 0x7fffffff;
+    cnt      = 0;
+    float wd = FLT_MAX;
+    int   wi = 0x7fffffff;
+    if (g.n == 0 || !(isfinite(qx) && isfinite(qy) && isfinite(qz)))
+        return;
+    const int c0x  = cell_coord_unclamped(qx, g.ox, g.cell);
+    const int c0y  = cell_coord_unclamped(qy, g.oy, g.cell);
+    const int c0z  = cell_coord_unclamped(qz, g.oz, g.cell);
+    const int maxR = max(max(g.dx, g.dy), g.dz) + max(max(abs(c0x), abs(c0y)), abs(c0z)) + 1;
+    for (int r = first_ring(g, c0x, c0y, c0z); r <= maxR; r++) {
+        visit_ring(g, c0x, c0y, c0z, r, qx, qy, qz, wd, [&](int s, int c) {
+            for (int base = s; base < s + c; base += KNN_GROUP) {
+                const int t    = base + l;
+                float     dd   = FLT_MAX;
+                int       ii   = 0x7fffffff;
+                bool      pass = false;
+                if (t < s + c) {
+                    dd   = ppfc::dist2(qx, qy, qz, g.x[t], g.y[t], g.z[t]);
+                    ii   = g.id[t];
+                    pass = cnt < k || dd < wd || (dd == wd && ii < wi);
+                }
+                uint32_t m = (uint32_t)(__ballot(pass) >> gsh) & GMASK;
+                while (m) {
+                    const int src = __ffs(m) - 1;
+                    m &= m - 1u;
+                    const float cd = __shfl(dd, src, KNN_GROUP);
+                    const int   ci = __shfl(ii, src, KNN_GROUP);
+                    if (cnt == k && !(cd < wd || (cd == wd && ci < wi)))
+                        continue;
+                    const bool before = l < cnt && (cd < myD || (cd == myD && ci < myId));
+                    const uint32_t bm = (uint32_t)(__ballot(before) >> gsh) & GMASK;
+                    const int      pos = bm ? __ffs(bm) - 1 : cnt;
+                    const float    upD = __shfl_up(myD, 1, KNN_GROUP);
+                    const int      upI = __shfl_up(myId, 1, KNN_GROUP);
+                    if (l > pos && l < k) {
+                        myD  = upD;
+                        myId = upI;
+                    }
+                    if (l == pos) {
+                        myD  = cd;
+                        myId = ci;
+                    }
+                    cnt = cnt < k ? cnt + 1 : k;
+                    if (cnt == k) {
+                        wd = __shfl(myD, k - 1, KNN_GROUP);
+                        wi = __shfl(myId, k - 1, KNN_GROUP);
+                    }
+                }
+            }
+        });
+        const float safe = ring_safe(g, c0x, c0y, c0z, r, qx, qy, qz);
+        if (safe == FLT_MAX)
+            break;
+        if (cnt == k && wd < safe * safe)
+            break;
+    }
+}
+
+// the kNN lists of the points list[0..nl) with mark == want: ids
+// (k per point, ascending by (dist2, id)) and counts (-1: not estimated)
+__global__ void __launch_bounds__(256) normals_knn_coop_kernel(NormArgs a, const int *list,
+                                                               int nl, int want, int *ids,
+                                                               int *cnts) {
+    const int t    = (int)((blockIdx.x * blockDim.x + threadIdx.x) / KNN_GROUP);
+    const int lane = (int)(threadIdx.x & 63u);
+    const int l    = lane & (KNN_GROUP - 1);
+    const int gsh  = lane & ~(KNN_GROUP - 1);
+    if (t >= nl)
+        return;
+    const int id = list[t];
+    if (a.mark[id] != want) {
+        if (l == 0)
+            cnts[t] = -1;
+        return;
+    }
+    float d;
+    int   i, cnt;
+    coop_knn(a.g, a.g.nx[id], a.g.ny[id], a.g.nz[id], a.k, l, gsh, d, i, cnt);
+    if (l < cnt)
+        ids[(size_t)t * a.k + l] = i;
+    if (l == 0)
+        cnts[t] = cnt;
+}
+
+// PCA of each point from its kNN list (one thread per point)
+template <int KMAX>
+__global__ void __launch_bounds__(128) normals_pca_kernel(NormArgs a, const int *list, int nl,
+                                                          int want, const int *ids,
+                                                          const int *cnts) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nl)
+        return;
+    const int cnt = cnts[t];
+    if (cnt < 0)
+        return;
+    KnnList<KMAX> L;
+    L.init();
+    L.cnt = cnt;
+#pragma unroll
+    for (int x = 0; x < KMAX; x++)
+        if (x < cnt)
+            L.id[x] = ids[(size_t)t * a.k + x];
+    store_estimate<KMAX>(a, list[t], want, L);
+}
+
 // final normal of each query (smoothing, util.cpp:292-306)
 __global__ void normals_final_kernel(NormArgs a, float *outN /* nq*3 */) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -716,7 +838,13 @@ void estimate_normals(Workspace &ws, int n, float *PX, float *PY, float *PZ, flo
     auto estimate = [&](const int *list, int nl, int want) {
         if (nl <= 0)
             return;
-        if (small)
+        if (NORMALS_COOP && k <= KNN_GROUP) {
+            int *ids  = ws.knnIds.as<int>((size_t)nl * k);
+            int *cnts = ws.knnCnt.as<int>(nl);
+            normals_knn_coop_kernel<<<div_up(nl, 256 / KNN_GROUP), 256, 0, s>>>(a, list, nl, want,
+                                                                               ids, cnts);
+            normals_pca_kernel<16><<<div_up(nl, 128), 128, 0, s>>>(a, list, nl, want, ids, cnts);
+        } else if (small)
             normals_point_kernel<16><<<div_up(nl, 128), 128, 0, s>>>(a, list, nl, want);
         else
             normals_point_kernel<KNN_MAX><<<div_up(nl, 128), 128, 0, s>>>(a, list, nl, want);

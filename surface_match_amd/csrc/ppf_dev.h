@@ -25,6 +25,8 @@ struct Workspace {
     Grid   gNormal, gVote, gSparse, gDense, gFull, gMls;
     // MLS normals: per-query neighbour counts, 64-bit offsets, ids (twice)
     DevBuf mlsCnt, mlsOff, mlsIds, mlsIds2, mlsP4, mlsKey, mlsKey2, mlsQ, mlsQ2;
+    // cooperative kNN normals: each point's list (k ids) and count
+    DevBuf knnIds, knnCnt;
     // vote
     DevBuf kpX, kpY, kpZ, kpNX, kpNY, kpNZ, cand, candTile, poses, voteStats, accDump, voteCnt,
         smallCnt, hitStart, hitLen, hitAlpha, hitBucket, hitCount, splits, kpKey, kpKey2, kpOrd,
