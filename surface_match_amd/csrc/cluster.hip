@@ -287,7 +287,8 @@ __global__ void __launch_bounds__(64) nms_seq_kernel(const Hypo *h, const int *o
         pz[j]          = P[8] * c0 + P[9] * c1 + P[10] * c2 + P[11];
     }
     __syncthreads();
-    int nk = 0;  // wave-uniform
+    int  nk    = 0;  // wave-uniform
+    int *sStat = reinterpret_cast<int *>(px);  // px[j] is dead once pose j is walked
     for (int j = 0; j < C; j++) {
         const float tx = px[j], ty = py[j], tz = pz[j];
         int         first = -1;
@@ -311,6 +312,7 @@ __global__ void __launch_bounds__(64) nms_seq_kernel(const Hypo *h, const int *o
                 kz[nk]    = tz;
                 ki[nk]    = j;
                 status[j] = CENTER;
+                sStat[j]  = CENTER;
             }
             nk++;
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -318,19 +320,23 @@ __global__ void __launch_bounds__(64) nms_seq_kernel(const Hypo *h, const int *o
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         } else if (lane == 0) {
             status[j] = ki[first];
+            sStat[j]  = ki[first];
         }
     }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    __syncthreads();
+    // every pose's votes next to its status in LDS (py is dead too)
+    float *sVotes = py;
+    for (int j = lane; j < C; j += 64)
+        sVotes[j] = h[ord[j]].votes;
+    __syncthreads();
     // kept poses in order, each with its members' votes added in ascending
     // member order (the reference's accumulation order)
     for (int t = lane; t < nk; t += 64) {
         const int i = ki[t];
         Hypo      o = h[ord[i]];
         for (int j = i + 1; j < C; j++)
-            if (status[j] == i)
-                o.votes = o.votes + h[ord[j]].votes;
+            if (sStat[j] == i)
+                o.votes = o.votes + sVotes[j];
         out[t] = o;
     }
     if (lane == 0)
