@@ -265,8 +265,12 @@ __global__ void __launch_bounds__(1024) nms_kernel(const Hypo *h, const int *ord
 }
 
 // The same rule walked in order by one wave: pose j takes the first kept
-// pose within the overlap distance (a ballot over the kept list held in
-// LDS), else it is kept.  Used while the kept list fits in LDS.
+// pose within the overlap distance, else it is kept.  The wave walks 64
+// poses at a time: each lane first looks for its pose's first kept pose
+// among those kept before the block (those precede any kept in the
+// block); the lanes left over are then settled in order, each against the
+// poses of the block kept so far (one ballot), becoming kept when none is
+// near.  Used while the kept list fits in LDS.
 constexpr int NMS_SEQ_MAX = 4096;
 
 __global__ void __launch_bounds__(64) nms_seq_kernel(const Hypo *h, const int *ord, int C,
@@ -288,40 +292,65 @@ __global__ void __launch_bounds__(64) nms_seq_kernel(const Hypo *h, const int *o
     }
     __syncthreads();
     int  nk    = 0;  // wave-uniform
-    int *sStat = reinterpret_cast<int *>(px);  // px[j] is dead once pose j is walked
-    for (int j = 0; j < C; j++) {
-        const float tx = px[j], ty = py[j], tz = pz[j];
-        int         first = -1;
-        for (int b = 0; b < nk; b += 64) {
-            const int t   = b + lane;
-            bool      hit = false;
-            if (t < nk) {
+    int *sStat = reinterpret_cast<int *>(px);  // px[j] is dead once its block is walked
+    for (int b0 = 0; b0 < C; b0 += 64) {
+        const int  j     = b0 + lane;
+        const bool valid = j < C;
+        float      tx = 0.0f, ty = 0.0f, tz = 0.0f;
+        if (valid) {
+            tx = px[j];
+            ty = py[j];
+            tz = pz[j];
+        }
+        // the first pose kept before this block within the distance
+        int       st  = UNDECIDED;
+        const int nk0 = nk;
+        if (valid)
+            for (int t = 0; t < nk0; t++) {
                 const float dx = kx[t] - tx, dy = ky[t] - ty, dz = kz[t] - tz;
-                hit            = dx * dx + dy * dy + dz * dz < sq;
+                if (dx * dx + dy * dy + dz * dz < sq) {
+                    st = ki[t];
+                    break;
+                }
             }
-            const unsigned long long m = __ballot(hit);
+        // the rest in order, against the poses kept in this block so far
+        bool               keptHere = false;
+        unsigned long long rest     = __ballot(valid && st == UNDECIDED);
+        while (rest) {
+            const int u = __ffsll((long long)rest) - 1;
+            rest &= rest - 1ull;
+            const float ux = __shfl(tx, u, 64), uy = __shfl(ty, u, 64), uz = __shfl(tz, u, 64);
+            bool        near = false;
+            if (keptHere) {
+                const float dx = tx - ux, dy = ty - uy, dz = tz - uz;
+                near           = dx * dx + dy * dy + dz * dz < sq;
+            }
+            const unsigned long long m = __ballot(near);
             if (m) {
-                first = b + __ffsll((long long)m) - 1;
-                break;
+                if (lane == u)
+                    st = b0 + __ffsll((long long)m) - 1;
+            } else {
+                if (lane == u) {
+                    st       = CENTER;
+                    keptHere = true;
+                    kx[nk]   = tx;
+                    ky[nk]   = ty;
+                    kz[nk]   = tz;
+                    ki[nk]   = j;
+                }
+                nk++;
             }
         }
-        if (first < 0) {
-            if (lane == 0) {
-                kx[nk]    = tx;
-                ky[nk]    = ty;
-                kz[nk]    = tz;
-                ki[nk]    = j;
-                status[j] = CENTER;
-                sStat[j]  = CENTER;
-            }
-            nk++;
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        } else if (lane == 0) {
-            status[j] = ki[first];
-            sStat[j]  = ki[first];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (valid) {
+            status[j] = st;
+            sStat[j]  = st;
         }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
     __syncthreads();
     // every pose's votes next to its status in LDS (py is dead too)
