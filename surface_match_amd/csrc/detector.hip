@@ -13,12 +13,15 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
+#include <exception>
 #include <fstream>
 #include <memory>
 #include <numeric>
 #include <string>
+#include <thread>
 #include <vector>
 
 using namespace ppfd;
@@ -245,6 +248,19 @@ struct ppf_detector_s {
     };
     std::vector<std::unique_ptr<SceneWork>> sceneWork;
     DevBuf batchKpScene, batchKpId, batchGrids, batchHypScene;
+    // the per-scene stages of a batched match run on up to kLanes host
+    // threads, each with its own stream (default priority) and workspace,
+    // so the small kernels and read-backs of different scenes overlap
+    static constexpr int kLanes = 4;
+    struct Lane {
+        hipStream_t stream = nullptr;
+        Workspace   ws;
+        ~Lane() {
+            if (stream)
+                (void)hipStreamDestroy(stream);
+        }
+    };
+    std::vector<std::unique_ptr<Lane>> lanes;
 
     ppf_detector_s() {
         PPF_HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
@@ -252,6 +268,7 @@ struct ppf_detector_s {
             PPF_HIP_CHECK(hipEventCreate(&e));
     }
     ~ppf_detector_s() {
+        lanes.clear();
         for (auto &e : ev)
             if (e)
                 (void)hipEventDestroy(e);
@@ -638,6 +655,49 @@ void match_impl(ppf_detector_s &D, DevCloud &c, float rel, float kpFrac, float m
 // the poses are clustered per scene, and the hypotheses of all scenes are
 // refined in one batched refinement per stage (each against its own scene).
 // Every per-scene result equals match_impl's on that scene.
+// f(i, workspace, stream) for every scene i < n, spread over the detector's
+// lanes; returns once all of them finished (their streams synchronised).
+// Work queued on the detector's stream before the call completes first.
+// Each scene's work is independent of the lane it lands on.
+template <class F>
+void for_scenes(ppf_detector_s &D, int n, F &&f) {
+    PPF_HIP_CHECK(hipStreamSynchronize(D.stream));
+    const int L = std::min(n, ppf_detector_s::kLanes);
+    if (L <= 1) {
+        for (int i = 0; i < n; i++)
+            f(i, D.ws, D.stream);
+        PPF_HIP_CHECK(hipStreamSynchronize(D.stream));
+        return;
+    }
+    while ((int)D.lanes.size() < L) {
+        auto ln = std::make_unique<ppf_detector_s::Lane>();
+        PPF_HIP_CHECK(hipStreamCreateWithFlags(&ln->stream, hipStreamNonBlocking));
+        D.lanes.push_back(std::move(ln));
+    }
+    int dev = 0;
+    PPF_HIP_CHECK(hipGetDevice(&dev));
+    std::atomic<int>                next{0};
+    std::vector<std::exception_ptr> err(L);
+    std::vector<std::thread>        th;
+    for (int l = 0; l < L; l++)
+        th.emplace_back([&, l] {
+            try {
+                PPF_HIP_CHECK(hipSetDevice(dev));
+                ppf_detector_s::Lane &ln = *D.lanes[l];
+                for (int i = next++; i < n; i = next++)
+                    f(i, ln.ws, ln.stream);
+                PPF_HIP_CHECK(hipStreamSynchronize(ln.stream));
+            } catch (...) {
+                err[l] = std::current_exception();
+            }
+        });
+    for (auto &t : th)
+        t.join();
+    for (auto &e : err)
+        if (e)
+            std::rethrow_exception(e);
+}
+
 void match_batch_impl(ppf_detector_s &D, const std::vector<DevCloud *> &clouds, float rel,
                       float kpFrac, float minScore, const ppf_match_param &p,
                       std::vector<MatchOut> &outs, ppf_match_stats *st) {
@@ -682,29 +742,31 @@ void match_batch_impl(ppf_detector_s &D, const std::vector<DevCloud *> &clouds, 
     if (p.poseRefScoringDistRel > 0) scoringDist = Dm * p.poseRefScoringDistRel;
     if (p.poseRefScoringDistAbs > 0) scoringDist = p.poseRefScoringDistAbs;
     const float keySampleStep = std::sqrt(1.0f / kpFrac) * sampleStep;
-    int        *cnt           = D.ws.smallCnt.as<int>(8);
 
     PPF_HIP_CHECK(hipEventRecord(D.ev[0], s));
     // ---- per scene: sampling, normals, key points, vote grid
-    std::vector<int> ko(n + 1, 0);
-    float            minCell = FLT_MAX;
-    for (int i = 0; i < n; i++) {
+    for_scenes(D, n, [&](int i, Workspace &ws, hipStream_t ls) {
         DevCloud &c  = *clouds[i];
         auto     &sw = *D.sceneWork[i];
-        c.reset(s);
+        int      *cn = ws.smallCnt.as<int>(8);
+        c.reset(ls);
         float *PX = c.px.get<float>(), *PY = c.py.get<float>(), *PZ = c.pz.get<float>();
         float *NX = c.nx.get<float>(), *NY = c.ny.get<float>(), *NZ = c.nz.get<float>();
         int   *samp = sw.samp.as<int>(std::max(c.nValid, 1));
-        sw.ns = voxel_sample(D.ws, PX, PY, PZ, c.valid.get<int>(), c.nValid, sampleStep, c.boxMin,
-                             samp, cnt, s);
+        sw.ns = voxel_sample(ws, PX, PY, PZ, c.valid.get<int>(), c.nValid, sampleStep, c.boxMin,
+                             samp, cn, ls);
         if (!c.hasNormal)
-            estimate_normals(D.ws, c.n, PX, PY, PZ, NX, NY, NZ, c.valid.get<int>(), c.nValid, samp,
-                             sw.ns, p.knnNormal, p.smoothNormal, p.invertNormal, c.vp, normCell, s);
+            estimate_normals(ws, c.n, PX, PY, PZ, NX, NY, NZ, c.valid.get<int>(), c.nValid, samp,
+                             sw.ns, p.knnNormal, p.smoothNormal, p.invertNormal, c.vp, normCell, ls);
         int *kp = sw.kp.as<int>(std::max(sw.ns, 1));
-        sw.nk   = voxel_sample(D.ws, PX, PY, PZ, samp, sw.ns, keySampleStep, c.boxMin, kp, cnt, s);
-        grid_build(sw.gVote, PX, PY, PZ, NX, NY, NZ, samp, sw.ns, Dm / (float)D.vcfg.cellDiv, s);
-        minCell   = std::min(minCell, sw.gVote.view.cell);
-        ko[i + 1] = ko[i] + sw.nk;
+        sw.nk   = voxel_sample(ws, PX, PY, PZ, samp, sw.ns, keySampleStep, c.boxMin, kp, cn, ls);
+        grid_build(sw.gVote, PX, PY, PZ, NX, NY, NZ, samp, sw.ns, Dm / (float)D.vcfg.cellDiv, ls);
+    });
+    std::vector<int> ko(n + 1, 0);
+    float            minCell = FLT_MAX;
+    for (int i = 0; i < n; i++) {
+        minCell   = std::min(minCell, D.sceneWork[i]->gVote.view.cell);
+        ko[i + 1] = ko[i] + D.sceneWork[i]->nk;
     }
     PPF_HIP_CHECK(hipEventRecord(D.ev[3], s));
     // ---- every scene's key points in one vote
@@ -765,13 +827,15 @@ void match_batch_impl(ppf_detector_s &D, const std::vector<DevCloud *> &clouds, 
     for (int a = 0; a < 3; a++)
         center[a] = (hm.sampled.boxMin[a] + hm.sampled.boxMax[a]) / 2.0f;
     std::vector<std::vector<Hypo>> hyp(n);
-    int                            nClustersTot = 0;
-    for (int i = 0; i < n; i++) {
-        int nc = 0;
-        cluster_poses(D.ws, D.ws.poses.get<PoseRec>() + pOff[i], pOff[i + 1] - pOff[i], 0.1f * Dm,
-                      angleStep, center, maxOverlapDist, hyp[i], &nc, s);
-        nClustersTot += nc;
-    }
+    std::vector<int>               ncs(n, 0);
+    const PoseRec                 *posesAll = D.ws.poses.get<PoseRec>();
+    for_scenes(D, n, [&](int i, Workspace &ws, hipStream_t ls) {
+        cluster_poses(ws, posesAll + pOff[i], pOff[i + 1] - pOff[i], 0.1f * Dm, angleStep, center,
+                      maxOverlapDist, hyp[i], &ncs[i], ls);
+    });
+    int nClustersTot = 0;
+    for (int i = 0; i < n; i++)
+        nClustersTot += ncs[i];
     PPF_HIP_CHECK(hipEventRecord(D.ev[6], s));
     // ---- refinement (ppf.cpp:423-495), every scene's hypotheses together
     struct Cand {
@@ -799,9 +863,9 @@ void match_batch_impl(ppf_detector_s &D, const std::vector<DevCloud *> &clouds, 
     int nSparse = 0, nDenseIcp = 0, nDenseTot = 0;
     if (p.sparsePoseRefinement && !cand.empty()) {
         std::vector<GridView> vs1(n), vf(n);
-        for (int i = 0; i < n; i++) {
+        for_scenes(D, n, [&](int i, Workspace &ws, hipStream_t ls) {
             if (!sceneHasCand[i])
-                continue;
+                return;
             DevCloud &c  = *clouds[i];
             auto     &sw = *D.sceneWork[i];
             float *PX = c.px.get<float>(), *PY = c.py.get<float>(), *PZ = c.pz.get<float>();
@@ -810,20 +874,24 @@ void match_batch_impl(ppf_detector_s &D, const std::vector<DevCloud *> &clouds, 
                 // ppf.cpp:427-434: the dense samples' normals overwrite the scene's
                 // before either refinement reads them
                 int *dense = sw.dense.as<int>(std::max(c.nValid, 1));
-                sw.nd = voxel_sample(D.ws, PX, PY, PZ, c.valid.get<int>(), c.nValid, reSampleStep,
-                                     c.boxMin, dense, cnt, s);
+                sw.nd = voxel_sample(ws, PX, PY, PZ, c.valid.get<int>(), c.nValid, reSampleStep,
+                                     c.boxMin, dense, ws.smallCnt.as<int>(8), ls);
                 if (!c.hasNormal)
-                    estimate_normals(D.ws, c.n, PX, PY, PZ, NX, NY, NZ, c.valid.get<int>(),
+                    estimate_normals(ws, c.n, PX, PY, PZ, NX, NY, NZ, c.valid.get<int>(),
                                      c.nValid, dense, sw.nd, p.knnNormal, 0, p.invertNormal, c.vp,
-                                     normCell, s);
-                nDenseTot += sw.nd;
+                                     normCell, ls);
             }
-            grid_build(sw.gSparse, PX, PY, PZ, NX, NY, NZ, sw.samp.get<int>(), sw.ns, sampleStep, s);
+            grid_build(sw.gSparse, PX, PY, PZ, NX, NY, NZ, sw.samp.get<int>(), sw.ns, sampleStep,
+                       ls);
             grid_build(sw.gFull, PX, PY, PZ, nullptr, nullptr, nullptr, c.valid.get<int>(),
-                       c.nValid, std::max(scoringDist * 2.0f, sampleStep * 0.25f), s);
+                       c.nValid, std::max(scoringDist * 2.0f, sampleStep * 0.25f), ls);
             vs1[i] = sw.gSparse.view;
             vf[i]  = sw.gFull.view;
-        }
+        });
+        if (p.densePoseRefinement)
+            for (int i = 0; i < n; i++)
+                if (sceneHasCand[i])
+                    nDenseTot += D.sceneWork[i]->nd;
         GridView *gS = D.batchGrids.as<GridView>((size_t)2 * n);
         PPF_HIP_CHECK(hipMemcpyAsync(gS, vs1.data(), sizeof(GridView) * n, hipMemcpyHostToDevice, s));
         PPF_HIP_CHECK(hipMemcpyAsync(gS + n, vf.data(), sizeof(GridView) * n, hipMemcpyHostToDevice,
@@ -868,17 +936,17 @@ void match_batch_impl(ppf_detector_s &D, const std::vector<DevCloud *> &clouds, 
             for (auto &c : survivors)
                 needDense[c.scene] = 1;
             std::vector<GridView> vd(n);
-            for (int i = 0; i < n; i++) {
+            for_scenes(D, n, [&](int i, Workspace &, hipStream_t ls) {
                 if (!needDense[i])
-                    continue;
+                    return;
                 DevCloud &c  = *clouds[i];
                 auto     &sw = *D.sceneWork[i];
                 float *PX = c.px.get<float>(), *PY = c.py.get<float>(), *PZ = c.pz.get<float>();
                 float *NX = c.nx.get<float>(), *NY = c.ny.get<float>(), *NZ = c.nz.get<float>();
                 grid_build(sw.gDense, PX, PY, PZ, NX, NY, NZ, sw.dense.get<int>(), sw.nd,
-                           reSampleStep * 2.0f, s);
+                           reSampleStep * 2.0f, ls);
                 vd[i] = sw.gDense.view;
-            }
+            });
             GridView *gD = D.batchGrids.as<GridView>((size_t)3 * n);
             PPF_HIP_CHECK(hipMemcpyAsync(gD + 2 * n, vd.data(), sizeof(GridView) * n,
                                          hipMemcpyHostToDevice, s));
