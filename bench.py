@@ -215,11 +215,29 @@ def cpu_baseline(w, threads: int):
     mp, mn = w["model"]
     m = O.train(mp, mn, 0.04)
     t0 = time.perf_counter()
-    _, _, st = O.match(m, w["scene"].points, None, 0.04, 0.1, 0.2, vp=synth.VIEWPOINT,
-                       param=O.default_match_param(numMatches=10), threads=threads)
+    opos, osc, st = O.match(m, w["scene"].points, None, 0.04, 0.1, 0.2, vp=synth.VIEWPOINT,
+                            param=O.default_match_param(numMatches=10), threads=threads)
     ms = (time.perf_counter() - t0) * 1e3
     O.lib().orc_free_model(__import__("ctypes").byref(m))
-    return ms, st
+    return ms, st, opos, osc
+
+
+def vs_oracle(poses, scores, opos, osc, diameter):
+    """each GPU result against the oracle's result for the same instance
+    (the oracle refines in FP64): rotation and translation difference and
+    score difference"""
+    from surface_match_amd import synth
+    out = []
+    for P, s in zip(poses, scores):
+        best = None
+        for Q, t in zip(opos, osc):
+            _, ang, dt = synth.pose_error(P, [np.vstack([Q, [0, 0, 0, 1]])], diameter)
+            if best is None or ang / 0.1 + dt / (0.001 * diameter) < best[0] / 0.1 + best[1] / (0.001 * diameter):
+                best = (ang, dt, abs(float(s) - float(t)))
+        if best is not None:
+            out.append(dict(rot_deg=float("%.4g" % best[0]), trans_pct_D=float("%.4g" % (100 * best[1] / diameter)),
+                            score_diff=float("%.3g" % best[2])))
+    return out
 
 
 def icp_batching(det, H_per_instance=64, n_inst=8, seed=0x5EED0003):
@@ -346,9 +364,9 @@ def main():
     ap.add_argument("--no-icp-batching", action="store_true")
     ap.add_argument("--autotune", action="store_true")
     ap.add_argument("--no-scene-batching", action="store_true")
-    ap.add_argument("--only", choices=["mls", "mesh"],
-                    help="run one side measurement (MLS normals or mesh sampling) and print "
-                         "its JSON, e.g. to profile it")
+    ap.add_argument("--only", choices=["mls", "mesh", "batch"],
+                    help="run one side measurement (MLS normals, mesh sampling or the 64-scene "
+                         "batch) and print its JSON, e.g. to profile it")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the PPF path runs on a single MI355X "
@@ -356,7 +374,8 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     if args.only:
-        print(json.dumps(mls_normals() if args.only == "mls" else mesh_sampling()))
+        side = {"mls": mls_normals, "mesh": mesh_sampling, "batch": scene_batching}
+        print(json.dumps(side[args.only]()))
         return
 
     from surface_match_amd import Detector, PointCloud, MatchParam, ResidentScene, synth
@@ -457,7 +476,7 @@ def main():
         out["scene_batching"] = scene_batching()
     if not args.no_cpu_baseline:
         threads = min(os.cpu_count() or 1, 8)
-        cms, cst = cpu_baseline(w, threads)
+        cms, cst, opos, osc = cpu_baseline(w, threads)
         out["cpu_baseline"] = {"value": round(cms, 1), "unit": "ms", "cores": threads,
                                "kind": "port",
                                "sample": "the full workload, one matchScene by the C oracle "
@@ -468,7 +487,10 @@ def main():
                                             enumerate(("sample_and_normals", "vote", "cluster",
                                                        "refine"))},
                                "mesh_sampling_ms": round(cpu_mesh_sampling(), 1),
-                               "mls_normals": cpu_mls_normals()}
+                               "mls_normals": cpu_mls_normals(),
+                               # every GPU result against the oracle's (FP64
+                               # refinement) result for the same instance
+                               "results_vs_oracle": vs_oracle(poses, scores, opos, osc, D)}
     print(json.dumps(out))
 
 

@@ -156,7 +156,7 @@ struct VoteStats {
 };
 
 struct VoteConfig {
-    int block    = 512;   // threads per voting workgroup
+    int block    = 256;   // threads per voting workgroup
     int xcdRemap = 0;     // keep spatially adjacent keypoints on one XCD
     int cellDiv  = 3;     // vote grid cell = model diameter / cellDiv (2..4)
     int split    = 1;     // 1: pair kernel + hit voting kernel; 0: fused kernel

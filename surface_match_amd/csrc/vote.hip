@@ -160,6 +160,32 @@ __device__ __forceinline__ int vote_run_loaded(const VoteArgs &a, uint32_t *acc,
     return nv;
 }
 
+// The same for one reference tile with 16-bit counters, two per word: the
+// vote adds 1 to the cell's half word and the returned old word tells
+// whether that half was already at 0xFFFF (a carry into the neighbour).
+// Such a carry from a real cell (c < realCells) sets ovf, and the key point
+// is then voted again with 32-bit counters.  Cells of the spare row never
+// carry into real cells: a carry goes from the low half (even cell) to the
+// high half of the same word, and every cell above a spare one is spare.
+__device__ __forceinline__ int vote_run_packed(const VoteArgs &a, uint32_t *acc,
+                                               const SegRun &g, const uint2 (&v)[SEG],
+                                               int realCells, uint32_t &ovf) {
+    const float off = g.e0 >= g.tailStart ? g.offTail : g.offBody;
+    uint32_t    old[SEG];
+    int         cell[SEG];
+#pragma unroll
+    for (int kk = 0; kk < SEG; kk++) {
+        const int bin = ppfc::bin_wrapped(__uint_as_float(v[kk].y), off, a.sMaxId, a.angleNum);
+        cell[kk]      = (int)v[kk].x + bin;
+        old[kk]       = atomicAdd(&acc[cell[kk] >> 1], 1u << ((cell[kk] & 1) << 4));
+    }
+#pragma unroll
+    for (int kk = 0; kk < SEG; kk++)
+        ovf |= (cell[kk] < realCells &&
+                ((old[kk] >> ((cell[kk] & 1) << 4)) & 0xFFFFu) == 0xFFFFu) ? 1u : 0u;
+    return g.vhi - g.vlo;
+}
+
 template <bool RANGE>
 __device__ __forceinline__ int vote_run(const VoteArgs &a, uint32_t *acc, int r0A, int r1A,
                                         const SegRun &g) {
@@ -346,7 +372,15 @@ __device__ __forceinline__ int candidate_slot(int t, int c, int nCells, const in
 
 // peaks of a voted accumulator tile (ppf.cpp:363-391): the best three
 // (vote, ref, bin) codes of the tile
-template <int BLOCK>
+// accumulator cell c: a 32-bit word, or with PACKED a 16-bit half word
+template <bool PACKED>
+__device__ __forceinline__ uint32_t acc_get(const uint32_t *acc, int c) {
+    if (!PACKED)
+        return acc[c];
+    return (acc[c >> 1] >> ((c & 1) << 4)) & 0xFFFFu;
+}
+
+template <int BLOCK, bool PACKED = false>
 __device__ void peaks_top3(const VoteArgs &a, const uint32_t *acc, int nrTile, int r0,
                            unsigned long long *sTop, unsigned long long *candOut) {
     const int   tid  = threadIdx.x;
@@ -354,10 +388,10 @@ __device__ void peaks_top3(const VoteArgs &a, const uint32_t *acc, int nrTile, i
     const int   A    = a.angleNum, maxI = A - 1;
     Top3        top{0ull, 0ull, 0ull};
     for (int r = tid; r < nrTile; r += BLOCK) {
-        const uint32_t *bins = acc + r * A;
+        auto bins = [&](int x) { return acc_get<PACKED>(acc, r * A + x); };
         int sum = 0, j = 0, best = -1;
         for (int x = 0; x < A; x++) {
-            int v = (int)bins[x];
+            int v = (int)bins(x);
             sum += v;
             if (v > best) {
                 best = v;
@@ -366,8 +400,8 @@ __device__ void peaks_top3(const VoteArgs &a, const uint32_t *acc, int nrTile, i
         }
         if ((float)sum < thre || (float)best < thre)
             continue;
-        int v = best + (int)((j == 0) ? bins[maxI] : bins[j - 1]) +
-                (int)((j == maxI) ? bins[0] : bins[j + 1]);
+        int v = best + (int)((j == 0) ? bins(maxI) : bins(j - 1)) +
+                (int)((j == maxI) ? bins(0) : bins(j + 1));
         unsigned long long code = ((unsigned long long)(uint32_t)v << 32) |
                                   ((unsigned long long)(0xFFFFFu - (uint32_t)(r0 + r)) << 11) |
                                   (uint32_t)j;
@@ -911,9 +945,10 @@ __global__ void __launch_bounds__(PAIR_BLOCK) pair_kernel(VoteArgs a) {
 // ---- split path, stage 2: vote the hits of one key point (and tile) into
 // the LDS accumulator, all lanes striped over the buckets' segments, then
 // the peaks.
-template <int BLOCK>
+template <int BLOCK, bool PACKED>
 __global__ void __launch_bounds__(BLOCK) vote_hits_kernel(VoteArgs a) {
     extern __shared__ uint32_t acc[];
+    __shared__ uint32_t sOvf;
     __shared__ int      sPre[BLOCK];
     __shared__ uint32_t sLo[BLOCK], sHi[BLOCK], sTail[BLOCK];
     __shared__ float    sOffB[BLOCK], sOffT[BLOCK];
@@ -938,8 +973,12 @@ __global__ void __launch_bounds__(BLOCK) vote_hits_kernel(VoteArgs a) {
             candOut[tid] = 0ull;
         return;
     }
-    for (int i = tid; i < nrTile * a.angleNum; i += BLOCK)
+    const int realCells = nrTile * a.angleNum;
+    for (int i = tid; i < (PACKED ? (realCells + 1) >> 1 : realCells); i += BLOCK)
         acc[i] = 0u;
+    if (tid == 0)
+        sOvf = 0u;
+    uint32_t           ovf    = 0u;
     unsigned long long nVotes = 0;
     const size_t       base   = (size_t)k * a.hitCap;
     for (int h0 = 0; h0 < n; h0 += BLOCK) {
@@ -1033,17 +1072,35 @@ __global__ void __launch_bounds__(BLOCK) vote_hits_kernel(VoteArgs a) {
                 }
 #pragma unroll
                 for (int j = 0; j < HITS_INFLIGHT; j++)
-                    if (t + j < t1)
-                        nv += a.nTiles > 1 ? vote_run_loaded<true>(a, acc, r0A, r1A, g[j], ent[j])
-                                           : vote_run_loaded<false>(a, acc, r0A, r1A, g[j], ent[j]);
+                    if (t + j < t1) {
+                        if (PACKED)
+                            nv += vote_run_packed(a, acc, g[j], ent[j], realCells, ovf);
+                        else
+                            nv += a.nTiles > 1
+                                      ? vote_run_loaded<true>(a, acc, r0A, r1A, g[j], ent[j])
+                                      : vote_run_loaded<false>(a, acc, r0A, r1A, g[j], ent[j]);
+                    }
             }
         }
         nVotes += (unsigned long long)nv;
         __syncthreads();
     }
+    if (PACKED) {
+        // a 16-bit counter overflowed: the fused kernel votes this key point
+        // again (and writes its candidates, dump and counts)
+        if (ovf)
+            atomicOr(&sOvf, 1u);
+        __syncthreads();
+        if (sOvf) {
+            if (tid == 0)
+                a.overflow[atomicAdd(a.nOverflow, 1)] = k;
+            return;
+        }
+    }
     if (k < a.dumpCount && a.accDump) {
         for (int i = tid; i < nrTile * a.angleNum; i += BLOCK)
-            a.accDump[((size_t)k * a.refNum + r0) * a.angleNum + i] = (int32_t)acc[i];
+            a.accDump[((size_t)k * a.refNum + r0) * a.angleNum + i] =
+                (int32_t)acc_get<PACKED>(acc, i);
     }
     if (!voted) {
         if (tid < 3)
@@ -1058,7 +1115,7 @@ __global__ void __launch_bounds__(BLOCK) vote_hits_kernel(VoteArgs a) {
         if (tid == 0 && tile == 0)
             atomicAdd(a.stats + 3, 1ull);
     }
-    peaks_top3<BLOCK>(a, acc, nrTile, r0, sTop, candOut);
+    peaks_top3<BLOCK, PACKED>(a, acc, nrTile, r0, sTop, candOut);
 }
 
 // Merge tiles, apply the 0.95 filter (ppf.cpp:393-397) and build the poses
@@ -1155,11 +1212,22 @@ __global__ void bucket_split_kernel(const uint2 *buckets, int K, const uint2 *ve
     }
 }
 
+// 16-bit packed accumulators for the split path when one tile holds every
+// reference (half the LDS, so twice the workgroups per CU)
+constexpr bool VOTE_PACKED = true;
+
 template <int BLOCK>
-static void launch_vote_hits(const VoteArgs &a, int blocks, size_t lds, hipStream_t s) {
-    PPF_HIP_CHECK(hipFuncSetAttribute((const void *)vote_hits_kernel<BLOCK>,
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    vote_hits_kernel<BLOCK><<<blocks, BLOCK, lds, s>>>(a);
+static void launch_vote_hits(const VoteArgs &a, int blocks, size_t lds, bool packed,
+                             hipStream_t s) {
+    if (packed) {
+        PPF_HIP_CHECK(hipFuncSetAttribute((const void *)vote_hits_kernel<BLOCK, true>,
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        vote_hits_kernel<BLOCK, true><<<blocks, BLOCK, lds, s>>>(a);
+    } else {
+        PPF_HIP_CHECK(hipFuncSetAttribute((const void *)vote_hits_kernel<BLOCK, false>,
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        vote_hits_kernel<BLOCK, false><<<blocks, BLOCK, lds, s>>>(a);
+    }
 }
 
 // key point data (position, normal) gathered into flat arrays
@@ -1377,10 +1445,12 @@ int vote_keypoints(Workspace &ws, const DevModel &m, const VoteKeypoints &in, fl
             pair_kernel<false><<<padded(nk), PAIR_BLOCK, 0, s>>>(a);
         PPF_HIP_CHECK(hipGetLastError());
         PPF_HIP_CHECK(hipEventRecord(ws.voteEv[2], s));
+        const bool   packed = VOTE_PACKED && nTiles == 1;
+        const size_t ldsH   = packed ? ((size_t)(tileRefs + 1) * m.angleNum + 1) / 2 * 4 : lds;
         switch (cfg.block) {
-            case 256: launch_vote_hits<256>(a, blocks, lds, s); break;
-            case 512: launch_vote_hits<512>(a, blocks, lds, s); break;
-            default: launch_vote_hits<1024>(a, blocks, lds, s); break;
+            case 256: launch_vote_hits<256>(a, blocks, ldsH, packed, s); break;
+            case 512: launch_vote_hits<512>(a, blocks, ldsH, packed, s); break;
+            default: launch_vote_hits<1024>(a, blocks, ldsH, packed, s); break;
         }
         PPF_HIP_CHECK(hipGetLastError());
         PPF_HIP_CHECK(hipMemcpyAsync(&nOverflow, a.nOverflow, sizeof(int), hipMemcpyDeviceToHost, s));

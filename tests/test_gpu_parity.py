@@ -196,6 +196,52 @@ def test_l4_vote_accumulators_bitwise(trained, oracle_mod, workload1):
         trained.set_tuning(**base)
 
 
+def test_l4_vote_counts_past_16_bits(N, oracle_mod):
+    """A flat model voted from a dense flat scene: the centre key point's
+    busiest accumulator cells pass 65535 votes, which the 16-bit counters of
+    the one-tile split path cannot hold; the overflow is detected and the key
+    point voted again with 32-bit counters, so every tuning still equals the
+    oracle's accumulators (for a corner key point as well)."""
+    from surface_match_amd import Detector, PointCloud
+    O = oracle_mod
+    g = np.arange(0, 100.01, 2.0)
+    X, Y = np.meshgrid(g, g, indexing="ij")
+    mp = np.stack([X.ravel(), Y.ravel(), np.zeros(X.size)], 1).astype(np.float32)
+    mn = np.tile(np.array([0, 0, 1], np.float32), (len(mp), 1))
+    det = Detector()
+    det.trainModel(PointCloud(mp, mn), 0.04)
+    sp, sn, _, _ = det.sampled_model()
+    info = det.model_info()
+    D = np.float32(info.diameter)
+    ang = TWO_PI_F / np.float32(30)
+    dist = D * np.float32(0.04)
+    ot = O.OracleTable(sp, sn, ang, dist)
+    r = rng(3)
+    ns = 60000
+    pts = np.concatenate([[[50, 50, 0], [2, 3, 0]],
+                          np.stack([r.uniform(0, 100, ns), r.uniform(0, 100, ns),
+                                    np.zeros(ns)], 1)]).astype(np.float32)
+    nrm = np.tile(np.array([0, 0, 1], np.float32), (len(pts), 1))
+    kps = np.array([0, 1], np.int32)
+    refNum = info.nSampled
+    expect = []
+    for k in kps:
+        nb = np.array([j for j in range(len(pts)) if j != k], np.int64)
+        expect.append(O.vote_keypoint(ot, refNum, 30, pts[k], nrm[k], pts[nb], nrm[nb], ang,
+                                      dist))
+    print("largest cells", [int(e[:, 1:].max()) for e in expect], flush=True)
+    assert expect[0][:, 1:].max() > 65535
+    base = det.get_tuning()
+    try:
+        for split, tiles, cap in ((1, 1, 1 << 17), (1, 2, 1 << 17), (0, 1, 2048), (1, 1, 2048)):
+            det.set_tuning(voteSplit=split, voteTiles=tiles, voteHitCap=cap)
+            cand, poses, acc = det.vote(pts, nrm, kps, minScore=0.2, dump=len(kps))
+            for t in range(len(kps)):
+                assert np.array_equal(acc[t], expect[t][:, 1:]), (t, split, tiles, cap)
+    finally:
+        det.set_tuning(**base)
+
+
 def test_l2_voxel_and_search(N, oracle_mod, workload1):
     O = oracle_mod
     sc = workload1["scene"].points
