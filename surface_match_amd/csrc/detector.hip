@@ -144,6 +144,10 @@ struct DevCloud {
     DevBuf aos, ox, oy, oz, onx, ony, onz, px, py, pz, nx, ny, nz, valid;
     int    nValid = 0;
     float  boxMin[3], boxMax[3], vp[3];
+    // bounding box of the valid points (min xyz, max xyz): every working
+    // position stays inside it (normal estimation moves points to
+    // centroids of valid points), so grids over them need no device box
+    float  ext[6];
 
     void upload(const float *pts, const float *nrm, int count, const float *box6,
                 const float *viewPoint, hipStream_t s) {
@@ -151,11 +155,16 @@ struct DevCloud {
         hasNormal = nrm != nullptr;
         std::vector<int> v = remove_nan(pts, hasNormal ? nrm : nullptr, n);
         nValid   = (int)v.size();
+        if (v.empty())
+            std::fill(ext, ext + 6, 0.0f);
+        else
+            bbox_host(pts, v, ext, ext + 3);
         if (box6 && box_diameter(box6, box6 + 3) != 0.0f) {
             std::memcpy(boxMin, box6, sizeof boxMin);
             std::memcpy(boxMax, box6 + 3, sizeof boxMax);
         } else {
-            bbox_host(pts, v, boxMin, boxMax);
+            std::memcpy(boxMin, ext, sizeof boxMin);
+            std::memcpy(boxMax, ext + 3, sizeof boxMax);
         }
         for (int a = 0; a < 3; a++)
             vp[a] = viewPoint ? viewPoint[a] : NAN;
@@ -485,7 +494,7 @@ void match_impl(ppf_detector_s &D, DevCloud &c, float rel, float kpFrac, float m
     // normals (ppf.cpp:204-211)
     if (!c.hasNormal)
         estimate_normals(D.ws, c.n, PX, PY, PZ, NX, NY, NZ, c.valid.get<int>(), c.nValid, samp, ns,
-                         p.knnNormal, p.smoothNormal, p.invertNormal, c.vp, normCell, s);
+                         p.knnNormal, p.smoothNormal, p.invertNormal, c.vp, normCell, s, c.ext);
     PPF_HIP_CHECK(hipEventRecord(D.ev[2], s));
     // key points (ppf.cpp:212-216)
     const float keySampleStep = std::sqrt(1.0f / kpFrac) * sampleStep;
@@ -498,7 +507,7 @@ void match_impl(ppf_detector_s &D, DevCloud &c, float rel, float kpFrac, float m
                            mr->nSampled);
     }
     // vote (ppf.cpp:247-411)
-    grid_build(D.ws.gVote, PX, PY, PZ, NX, NY, NZ, samp, ns, Dm / (float)D.vcfg.cellDiv, s);
+    grid_build(D.ws.gVote, PX, PY, PZ, NX, NY, NZ, samp, ns, Dm / (float)D.vcfg.cellDiv, s, c.ext);
     VoteStats vs{};
     PPF_HIP_CHECK(hipEventRecord(D.ev[4], s));
     int np = vote_and_poses(D.ws, dm, D.ws.gVote.view, PX, PY, PZ, NX, NY, NZ, kp, nk,
@@ -521,7 +530,7 @@ void match_impl(ppf_detector_s &D, DevCloud &c, float rel, float kpFrac, float m
                           dense, cnt, s);
         if (!c.hasNormal)
             estimate_normals(D.ws, c.n, PX, PY, PZ, NX, NY, NZ, c.valid.get<int>(), c.nValid,
-                             dense, nd, p.knnNormal, 0, p.invertNormal, c.vp, normCell, s);
+                             dense, nd, p.knnNormal, 0, p.invertNormal, c.vp, normCell, s, c.ext);
     }
     PPF_HIP_CHECK(hipEventRecord(D.ev[7], s));
     // icp (ppf.cpp:436-495)
@@ -539,9 +548,9 @@ void match_impl(ppf_detector_s &D, DevCloud &c, float rel, float kpFrac, float m
     int nSparse = 0, nDenseIcp = 0;
     bool needGrids = p.sparsePoseRefinement && !cand.empty();
     if (needGrids) {
-        grid_build(D.ws.gSparse, PX, PY, PZ, NX, NY, NZ, samp, ns, sampleStep, s);
+        grid_build(D.ws.gSparse, PX, PY, PZ, NX, NY, NZ, samp, ns, sampleStep, s, c.ext);
         grid_build(D.ws.gFull, PX, PY, PZ, nullptr, nullptr, nullptr, c.valid.get<int>(),
-                   c.nValid, std::max(scoringDist * 2.0f, sampleStep * 0.25f), s);
+                   c.nValid, std::max(scoringDist * 2.0f, sampleStep * 0.25f), s, c.ext);
         IcpCriteria sc{5, poseRefDist, sampleStep * 0.5f, sampleStep, 0.05f};  // ppf.cpp:423
         std::vector<float>        init(cand.size() * 12);
         std::vector<IcpResultDev> res(cand.size());
@@ -573,7 +582,8 @@ void match_impl(ppf_detector_s &D, DevCloud &c, float rel, float kpFrac, float m
             survivors.push_back(i);
         }
         if (p.densePoseRefinement && !survivors.empty()) {
-            grid_build(D.ws.gDense, PX, PY, PZ, NX, NY, NZ, dense, nd, reSampleStep * 2.0f, s);
+            grid_build(D.ws.gDense, PX, PY, PZ, NX, NY, NZ, dense, nd, reSampleStep * 2.0f, s,
+                       c.ext);
             IcpCriteria dc{p.poseRefNumSteps, poseRefDist, reSampleStep * 0.5f, sampleStep, 0.05f};
             std::vector<float>        init2(survivors.size() * 12);
             std::vector<IcpResultDev> res2(survivors.size());
@@ -757,10 +767,12 @@ void match_batch_impl(ppf_detector_s &D, const std::vector<DevCloud *> &clouds, 
                              samp, cn, ls);
         if (!c.hasNormal)
             estimate_normals(ws, c.n, PX, PY, PZ, NX, NY, NZ, c.valid.get<int>(), c.nValid, samp,
-                             sw.ns, p.knnNormal, p.smoothNormal, p.invertNormal, c.vp, normCell, ls);
+                             sw.ns, p.knnNormal, p.smoothNormal, p.invertNormal, c.vp, normCell, ls,
+                             c.ext);
         int *kp = sw.kp.as<int>(std::max(sw.ns, 1));
         sw.nk   = voxel_sample(ws, PX, PY, PZ, samp, sw.ns, keySampleStep, c.boxMin, kp, cn, ls);
-        grid_build(sw.gVote, PX, PY, PZ, NX, NY, NZ, samp, sw.ns, Dm / (float)D.vcfg.cellDiv, ls);
+        grid_build(sw.gVote, PX, PY, PZ, NX, NY, NZ, samp, sw.ns, Dm / (float)D.vcfg.cellDiv, ls,
+                   c.ext);
     });
     std::vector<int> ko(n + 1, 0);
     float            minCell = FLT_MAX;
@@ -879,12 +891,12 @@ void match_batch_impl(ppf_detector_s &D, const std::vector<DevCloud *> &clouds, 
                 if (!c.hasNormal)
                     estimate_normals(ws, c.n, PX, PY, PZ, NX, NY, NZ, c.valid.get<int>(),
                                      c.nValid, dense, sw.nd, p.knnNormal, 0, p.invertNormal, c.vp,
-                                     normCell, ls);
+                                     normCell, ls, c.ext);
             }
             grid_build(sw.gSparse, PX, PY, PZ, NX, NY, NZ, sw.samp.get<int>(), sw.ns, sampleStep,
-                       ls);
+                       ls, c.ext);
             grid_build(sw.gFull, PX, PY, PZ, nullptr, nullptr, nullptr, c.valid.get<int>(),
-                       c.nValid, std::max(scoringDist * 2.0f, sampleStep * 0.25f), ls);
+                       c.nValid, std::max(scoringDist * 2.0f, sampleStep * 0.25f), ls, c.ext);
             vs1[i] = sw.gSparse.view;
             vf[i]  = sw.gFull.view;
         });
@@ -944,7 +956,7 @@ void match_batch_impl(ppf_detector_s &D, const std::vector<DevCloud *> &clouds, 
                 float *PX = c.px.get<float>(), *PY = c.py.get<float>(), *PZ = c.pz.get<float>();
                 float *NX = c.nx.get<float>(), *NY = c.ny.get<float>(), *NZ = c.nz.get<float>();
                 grid_build(sw.gDense, PX, PY, PZ, NX, NY, NZ, sw.dense.get<int>(), sw.nd,
-                           reSampleStep * 2.0f, ls);
+                           reSampleStep * 2.0f, ls, c.ext);
                 vd[i] = sw.gDense.view;
             });
             GridView *gD = D.batchGrids.as<GridView>((size_t)3 * n);

@@ -476,9 +476,11 @@ struct Grid {
 // (device int list of length m; nullptr = all m points).  cellHint is the
 // requested cell edge; it grows if the grid would exceed 1023 cells per
 // axis.  withNormals gathers NX/NY/NZ too.  Synchronises `s` once (bbox).
+// hostBox6 (min xyz, max xyz), when given, is a box known to contain the
+// points; it saves the device bounding box and its read-back.
 void grid_build(Grid &g, const float *X, const float *Y, const float *Z, const float *NX,
                 const float *NY, const float *NZ, const int *subset, int m, float cellHint,
-                hipStream_t s);
+                hipStream_t s, const float *hostBox6 = nullptr);
 
 // block-wide exclusive scan of one int per thread (blockDim a multiple of 64, <= 1024)
 template <int BLOCK>

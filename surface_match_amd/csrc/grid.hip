@@ -180,12 +180,22 @@ __global__ void gather3_kernel(const int *idx, int m, const float *A, const floa
 
 void grid_build(Grid &g, const float *X, const float *Y, const float *Z, const float *NX,
                 const float *NY, const float *NZ, const int *subset, int m, float cellHint,
-                hipStream_t s) {
-    float *bb = g.bbox.as<float>(8);
-    device_bbox(X, Y, Z, subset, m, bb, s);
+                hipStream_t s, const float *hostBox6) {
     float hb[6];
-    PPF_HIP_CHECK(hipMemcpyAsync(hb, bb, sizeof hb, hipMemcpyDeviceToHost, s));
-    PPF_HIP_CHECK(hipStreamSynchronize(s));
+    if (hostBox6) {
+        // a box known to hold the points, widened by 1e-4 of its extent (a
+        // centroid can land a few ulps outside its neighbours' box)
+        for (int a = 0; a < 3; a++) {
+            const float pad = 1e-4f * (hostBox6[3 + a] - hostBox6[a]) + 1e-6f;
+            hb[a]           = hostBox6[a] - pad;
+            hb[3 + a]       = hostBox6[3 + a] + pad;
+        }
+    } else {
+        float *bb = g.bbox.as<float>(8);
+        device_bbox(X, Y, Z, subset, m, bb, s);
+        PPF_HIP_CHECK(hipMemcpyAsync(hb, bb, sizeof hb, hipMemcpyDeviceToHost, s));
+        PPF_HIP_CHECK(hipStreamSynchronize(s));
+    }
     if (m == 0)
         for (int a = 0; a < 6; a++)
             hb[a] = 0.0f;
@@ -772,7 +782,8 @@ __global__ void normals_orient_kernel(NormArgs a, const float *outN, float *PX, 
 
 void estimate_normals(Workspace &ws, int n, float *PX, float *PY, float *PZ, float *NX, float *NY,
                       float *NZ, const int *valid, int nValid, const int *query, int nq, int k,
-                      int smooth, int invert, const float vp[3], float cellHint, hipStream_t s) {
+                      int smooth, int invert, const float vp[3], float cellHint, hipStream_t s,
+                      const float *hostBox6) {
     if (k < 3)
         k = 3;
     if (k > KNN_MAX)
@@ -786,7 +797,8 @@ void estimate_normals(Workspace &ws, int n, float *PX, float *PY, float *PZ, flo
     PPF_HIP_CHECK(hipMemcpyAsync(N0x, NX, sizeof(float) * n, hipMemcpyDeviceToDevice, s));
     PPF_HIP_CHECK(hipMemcpyAsync(N0y, NY, sizeof(float) * n, hipMemcpyDeviceToDevice, s));
     PPF_HIP_CHECK(hipMemcpyAsync(N0z, NZ, sizeof(float) * n, hipMemcpyDeviceToDevice, s));
-    grid_build(ws.gNormal, P0x, P0y, P0z, nullptr, nullptr, nullptr, valid, nValid, cellHint, s);
+    grid_build(ws.gNormal, P0x, P0y, P0z, nullptr, nullptr, nullptr, valid, nValid, cellHint, s,
+               hostBox6);
 
     NormArgs a;
     a.g    = ws.gNormal.view;

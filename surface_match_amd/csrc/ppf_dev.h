@@ -115,7 +115,7 @@ int  voxel_sample(Workspace &ws, const float *X, const float *Y, const float *Z,
 void estimate_normals(Workspace &ws, int n, float *PX, float *PY, float *PZ, float *NX,
                       float *NY, float *NZ, const int *valid, int nValid, const int *query,
                       int nq, int k, int smooth, int invert, const float vp[3], float cellHint,
-                      hipStream_t s);
+                      hipStream_t s, const float *hostBox6 = nullptr);
 
 // estimateNormalMLS (util.cpp:336-457) for the queries (indices into the
 // n-point cloud PX/PY/PZ, which is not modified): moved points into OX/OY/OZ
