@@ -185,7 +185,7 @@ typedef struct {
     int voteCellDiv;  /* vote grid cell = model diameter / voteCellDiv: 2, 3 or 4 */
     int icpBlock;     /* threads per ICP workgroup: 64 .. 1024, a power of two */
     int icpSpread;    /* 1: each hypothesis spread over workgroups, 0: persistent */
-    int normCell4;    /* normals grid cell = sample step * normCell4 / 4: 1, 2 or 4 */
+    int normCell4;    /* scene normals grid cell = sample step * normCell4 / 4: 1 .. 16 */
     int voteSplit;    /* 1: pair kernel + hit voting kernel, 0: one fused kernel */
     int voteHitCap;   /* key hits kept per key point on the split path (>= 1); key points
                          with more are voted by the fused kernel */

@@ -244,7 +244,7 @@ struct ppf_detector_s {
     TrainScratch ts;
     VoteConfig   vcfg;
     IcpConfig    icfg;
-    int          normCell4 = 4;  // normals grid cell = sample step * normCell4 / 4
+    int          normCell4 = 6;  // scene normals grid cell = sample step * normCell4 / 4
     int          refTail = 1;
     DevCloud     tmpScene;
     DevBuf       idxA, idxB, idxC, idxD;  // sampled / keypoints / dense / scratch
@@ -478,6 +478,9 @@ void match_impl(ppf_detector_s &D, DevCloud &c, float rel, float kpFrac, float m
     const float reSampleStep = Dm * hm.param.poseRefRelSamplingDistance;
     const float sampleStep   = Dm * rel;  // :200
     const float normCell     = sampleStep * (float)D.normCell4 * 0.25f;
+    // the dense samples' normals (queries 16x denser) measured best with a
+    // cell of one sample step whatever the scene pass uses
+    const float denseNormCell = sampleStep;
     // :223-240
     const float voteThreshold = (float)refNum * minScore;
     float maxOverlapDist = 0;
@@ -537,7 +540,8 @@ void match_impl(ppf_detector_s &D, DevCloud &c, float rel, float kpFrac, float m
                           dense, cnt, s);
         if (!c.hasNormal)
             estimate_normals(D.ws, c.n, PX, PY, PZ, NX, NY, NZ, c.valid.get<int>(), c.nValid,
-                             dense, nd, p.knnNormal, 0, p.invertNormal, c.vp, normCell, s, c.ext);
+                             dense, nd, p.knnNormal, 0, p.invertNormal, c.vp, denseNormCell, s,
+                             c.ext);
     }
     PPF_HIP_CHECK(hipEventRecord(D.ev[7], s));
     // icp (ppf.cpp:436-495)
@@ -749,6 +753,9 @@ void match_batch_impl(ppf_detector_s &D, const std::vector<DevCloud *> &clouds, 
     const float reSampleStep = Dm * hm.param.poseRefRelSamplingDistance;
     const float sampleStep   = Dm * rel;
     const float normCell     = sampleStep * (float)D.normCell4 * 0.25f;
+    // the dense samples' normals (queries 16x denser) measured best with a
+    // cell of one sample step whatever the scene pass uses
+    const float denseNormCell = sampleStep;
     const float voteThreshold = (float)refNum * minScore;
     float maxOverlapDist = 0;
     if (p.maxOverlapDistRel > 0) maxOverlapDist = Dm * p.maxOverlapDistRel;
@@ -899,7 +906,7 @@ void match_batch_impl(ppf_detector_s &D, const std::vector<DevCloud *> &clouds, 
                 if (!c.hasNormal)
                     estimate_normals(ws, c.n, PX, PY, PZ, NX, NY, NZ, c.valid.get<int>(),
                                      c.nValid, dense, sw.nd, p.knnNormal, 0, p.invertNormal, c.vp,
-                                     normCell, ls, c.ext);
+                                     denseNormCell, ls, c.ext);
             }
             grid_build(sw.gSparse, PX, PY, PZ, NX, NY, NZ, sw.samp.get<int>(), sw.ns,
                        sampleStep * kSparseCell, ls, c.ext);
@@ -1612,8 +1619,8 @@ PPF_API int ppf_set_tuning(ppf_detector d, const ppf_tuning *t) {
     PPF_REQUIRE(t->voteHitCap >= 1 && t->voteHitCap <= (1 << 20),
                 "vote hit capacity must be 1 .. 2^20");
     PPF_REQUIRE(t->voteTiles >= 1 && t->voteTiles <= 16, "vote tiles must be 1 .. 16");
-    PPF_REQUIRE(t->normCell4 == 1 || t->normCell4 == 2 || t->normCell4 == 4,
-                "normals cell must be 1, 2 or 4 quarters of the sample step");
+    PPF_REQUIRE(t->normCell4 >= 1 && t->normCell4 <= 16,
+                "normals cell must be 1 .. 16 quarters of the sample step");
     d->icfg.block    = t->icpBlock;
     d->icfg.spread   = t->icpSpread ? 1 : 0;
     d->normCell4     = t->normCell4;
@@ -1702,7 +1709,7 @@ PPF_API int ppf_autotune(ppf_detector d, ppf_scene sc, float rel, float kpFrac, 
             }
             d->*knob = best;
         };
-        tryNormals(&ppf_detector_s::normCell4, {1, 2, 4});
+        tryNormals(&ppf_detector_s::normCell4, {4, 5, 6, 8});
         rep->normalsMs        = bestNormMs;
         rep->best             = tuning_of(*d);
         rep->voteMs           = bestVoteMs;

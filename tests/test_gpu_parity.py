@@ -357,7 +357,7 @@ def test_autotune_keeps_results(trained, workload1):
     try:
         p0, s0, st0 = rs.match(0.04, 0.1, 0.2, MatchParam(numMatches=3))
         rep = rs.autotune(0.04, 0.1, 0.2, MatchParam(numMatches=3))
-        assert rep["configsTried"] == (2 + 3 + 3 + 4 + 2 + 2) + 2 * 5 + 3
+        assert rep["configsTried"] == (2 + 3 + 3 + 4 + 2 + 2) + 2 * 5 + 4
         assert trained.get_tuning() == rep["best"]
         p1, s1, st1 = rs.match(0.04, 0.1, 0.2, MatchParam(numMatches=3))
         for k in ("nPairs", "nKeyHits", "nVotes", "nPoses", "nHypotheses"):
