@@ -31,8 +31,21 @@
  *   jacobi eig         numpy eigh
  *   ICP                exact-correspondence recovery; ground-truth pose
  *                      recovery on seeded scenes
+ *   orc_philox4x32_10  Random123's published known-answer vectors
+ *   orc_sample_mesh    closed forms of the point weights, per-triangle
+ *                      counts worked by hand, points on their triangles,
+ *                      right-hand normals, uniform-over-area statistics
+ *                      (tests/test_mesh.py)
+ *   orc_expf           below one ulp of numpy's float64 exp; IEEE cases
+ *   orc_unit_orthogonal  Eigen's two branches by closed form; orthonormality
+ *   MLS normals        plane and paraboloid-apex fixed points, the order-1
+ *                      lift in closed form, sphere denoising, an independent
+ *                      float64 formulation (eigh + lstsq) (tests/test_mls.py)
  * Bitwise parity against the reference's xsimd atan2/sin and Eigen
- * AngleAxis/EigenSolver/JacobiSVD is unpinned (those libraries are absent).
+ * AngleAxis/EigenSolver/JacobiSVD/SelfAdjointEigenSolver/LLT and glibc's
+ * expf is unpinned (those libraries are absent), and sampleMesh's stream
+ * is seeded Philox where the reference's std::mt19937 is seeded from
+ * std::random_device (no fixed reference output exists).
  */
 #include "ppf_oracle.h"
 
