@@ -14,6 +14,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <exception>
@@ -447,6 +448,25 @@ void write_result_cloud(ppf_detector_s &D, DevCloud &c, const int *idxDev, int m
     }
 }
 
+// the detector's lane l (created on first use)
+ppf_detector_s::Lane &lane_of(ppf_detector_s &D, int l) {
+    while ((int)D.lanes.size() <= l) {
+        auto ln = std::make_unique<ppf_detector_s::Lane>();
+        PPF_HIP_CHECK(hipStreamCreateWithFlags(&ln->stream, hipStreamNonBlocking));
+        D.lanes.push_back(std::move(ln));
+    }
+    return *D.lanes[l];
+}
+
+// joins a helper thread however the scope is left
+struct JoinGuard {
+    std::thread &t;
+    ~JoinGuard() {
+        if (t.joinable())
+            t.join();
+    }
+};
+
 void match_impl(ppf_detector_s &D, DevCloud &c, float rel, float kpFrac, float minScore,
                 const ppf_match_param &p, MatchOut &out, ppf_match_result *mr,
                 ppf_match_stats *st) {
@@ -518,6 +538,41 @@ void match_impl(ppf_detector_s &D, DevCloud &c, float rel, float kpFrac, float m
     }
     // vote (ppf.cpp:247-411)
     grid_build(D.ws.gVote, PX, PY, PZ, NX, NY, NZ, samp, ns, Dm / (float)D.vcfg.cellDiv, s, c.ext);
+    // dense prep (ppf.cpp:427-434) on a lane, overlapping the vote and the
+    // clustering: it samples the scene again and estimates normals only for
+    // points without one (so never a sampled point, which all have one by
+    // now) and writes only those points, while the vote reads its grid's
+    // own copies and the key points, all sampled points
+    int        *dense = D.idxC.as<int>(std::max(c.nValid, 1));
+    int         nd    = 0;
+    float       prepMs = 0.0f;
+    std::thread prep;
+    std::exception_ptr prepErr;
+    JoinGuard   prepJoin{prep};
+    if (p.densePoseRefinement) {
+        ppf_detector_s::Lane &ln = lane_of(D, 0);
+        PPF_HIP_CHECK(hipEventRecord(D.ev[12], s));
+        PPF_HIP_CHECK(hipStreamWaitEvent(ln.stream, D.ev[12], 0));
+        int dev = 0;
+        PPF_HIP_CHECK(hipGetDevice(&dev));
+        prep = std::thread([&, dev] {
+            try {
+                PPF_HIP_CHECK(hipSetDevice(dev));
+                const auto t0 = std::chrono::steady_clock::now();
+                nd = voxel_sample(ln.ws, PX, PY, PZ, c.valid.get<int>(), c.nValid, reSampleStep,
+                                  c.boxMin, dense, ln.ws.smallCnt.as<int>(8), ln.stream);
+                if (!c.hasNormal)
+                    estimate_normals(ln.ws, c.n, PX, PY, PZ, NX, NY, NZ, c.valid.get<int>(),
+                                     c.nValid, dense, nd, p.knnNormal, 0, p.invertNormal, c.vp,
+                                     denseNormCell, ln.stream, c.ext);
+                PPF_HIP_CHECK(hipStreamSynchronize(ln.stream));
+                prepMs = std::chrono::duration<float, std::milli>(
+                             std::chrono::steady_clock::now() - t0).count();
+            } catch (...) {
+                prepErr = std::current_exception();
+            }
+        });
+    }
     VoteStats vs{};
     PPF_HIP_CHECK(hipEventRecord(D.ev[4], s));
     int np = vote_and_poses(D.ws, dm, D.ws.gVote.view, PX, PY, PZ, NX, NY, NZ, kp, nk,
@@ -532,17 +587,11 @@ void match_impl(ppf_detector_s &D, DevCloud &c, float rel, float kpFrac, float m
     cluster_poses(D.ws, D.ws.poses.get<PoseRec>(), np, 0.1f * Dm, angleStep, center,
                   maxOverlapDist, hyp, &nClusters, s);
     PPF_HIP_CHECK(hipEventRecord(D.ev[6], s));
-    // dense prep (ppf.cpp:427-434)
-    int *dense = D.idxC.as<int>(std::max(c.nValid, 1));
-    int  nd    = 0;
-    if (p.densePoseRefinement) {
-        nd = voxel_sample(D.ws, PX, PY, PZ, c.valid.get<int>(), c.nValid, reSampleStep, c.boxMin,
-                          dense, cnt, s);
-        if (!c.hasNormal)
-            estimate_normals(D.ws, c.n, PX, PY, PZ, NX, NY, NZ, c.valid.get<int>(), c.nValid,
-                             dense, nd, p.knnNormal, 0, p.invertNormal, c.vp, denseNormCell, s,
-                             c.ext);
-    }
+    // the dense prep has finished before the refinement reads the scene
+    if (prep.joinable())
+        prep.join();
+    if (prepErr)
+        std::rethrow_exception(prepErr);
     PPF_HIP_CHECK(hipEventRecord(D.ev[7], s));
     // icp (ppf.cpp:436-495)
     std::vector<int> cand;
@@ -664,7 +713,7 @@ void match_impl(ppf_detector_s &D, DevCloud &c, float rel, float kpFrac, float m
     st->msHitsKernel    = vs.hitsMs;
     st->nVoteFallback   = vs.overflow;
     st->msCluster       = elapsed(D.ev[5], D.ev[6]);
-    st->msIcpPrepare    = elapsed(D.ev[6], D.ev[7]);
+    st->msIcpPrepare    = prepMs;  // overlapped with the vote and the clustering
     st->msIcp           = elapsed(D.ev[7], D.ev[8]);
     st->msTotal         = elapsed(D.ev[0], D.ev[8]);
     st->msIcpSparse     = nSparse ? elapsed(D.ev[9], D.ev[10]) : 0.0f;
@@ -691,11 +740,7 @@ void for_scenes(ppf_detector_s &D, int n, F &&f) {
         PPF_HIP_CHECK(hipStreamSynchronize(D.stream));
         return;
     }
-    while ((int)D.lanes.size() < L) {
-        auto ln = std::make_unique<ppf_detector_s::Lane>();
-        PPF_HIP_CHECK(hipStreamCreateWithFlags(&ln->stream, hipStreamNonBlocking));
-        D.lanes.push_back(std::move(ln));
-    }
+    lane_of(D, L - 1);
     int dev = 0;
     PPF_HIP_CHECK(hipGetDevice(&dev));
     std::atomic<int>                next{0};
