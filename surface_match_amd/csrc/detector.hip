@@ -542,14 +542,17 @@ void match_impl(ppf_detector_s &D, DevCloud &c, float rel, float kpFrac, float m
     // clustering: it samples the scene again and estimates normals only for
     // points without one (so never a sampled point, which all have one by
     // now) and writes only those points, while the vote reads its grid's
-    // own copies and the key points, all sampled points
+    // own copies and the key points, all sampled points.  The refinement's
+    // grids (their own buffers, over the final positions) follow on the
+    // same lane.
     int        *dense = D.idxC.as<int>(std::max(c.nValid, 1));
     int         nd    = 0;
     float       prepMs = 0.0f;
     std::thread prep;
     std::exception_ptr prepErr;
     JoinGuard   prepJoin{prep};
-    if (p.densePoseRefinement) {
+    const bool laneGrids = p.sparsePoseRefinement != 0;
+    if (p.densePoseRefinement || laneGrids) {
         ppf_detector_s::Lane &ln = lane_of(D, 0);
         PPF_HIP_CHECK(hipEventRecord(D.ev[12], s));
         PPF_HIP_CHECK(hipStreamWaitEvent(ln.stream, D.ev[12], 0));
@@ -559,12 +562,25 @@ void match_impl(ppf_detector_s &D, DevCloud &c, float rel, float kpFrac, float m
             try {
                 PPF_HIP_CHECK(hipSetDevice(dev));
                 const auto t0 = std::chrono::steady_clock::now();
-                nd = voxel_sample(ln.ws, PX, PY, PZ, c.valid.get<int>(), c.nValid, reSampleStep,
-                                  c.boxMin, dense, ln.ws.smallCnt.as<int>(8), ln.stream);
-                if (!c.hasNormal)
-                    estimate_normals(ln.ws, c.n, PX, PY, PZ, NX, NY, NZ, c.valid.get<int>(),
-                                     c.nValid, dense, nd, p.knnNormal, 0, p.invertNormal, c.vp,
-                                     denseNormCell, ln.stream, c.ext);
+                if (p.densePoseRefinement) {
+                    nd = voxel_sample(ln.ws, PX, PY, PZ, c.valid.get<int>(), c.nValid,
+                                      reSampleStep, c.boxMin, dense, ln.ws.smallCnt.as<int>(8),
+                                      ln.stream);
+                    if (!c.hasNormal)
+                        estimate_normals(ln.ws, c.n, PX, PY, PZ, NX, NY, NZ, c.valid.get<int>(),
+                                         c.nValid, dense, nd, p.knnNormal, 0, p.invertNormal,
+                                         c.vp, denseNormCell, ln.stream, c.ext);
+                }
+                if (laneGrids) {
+                    grid_build(D.ws.gSparse, PX, PY, PZ, NX, NY, NZ, samp, ns,
+                               sampleStep * kSparseCell, ln.stream, c.ext);
+                    grid_build(D.ws.gFull, PX, PY, PZ, nullptr, nullptr, nullptr,
+                               c.valid.get<int>(), c.nValid,
+                               std::max(scoringDist * 2.0f, sampleStep * 0.25f), ln.stream, c.ext);
+                    if (p.densePoseRefinement)
+                        grid_build(D.ws.gDense, PX, PY, PZ, NX, NY, NZ, dense, nd,
+                                   reSampleStep * kDenseCell, ln.stream, c.ext);
+                }
                 PPF_HIP_CHECK(hipStreamSynchronize(ln.stream));
                 prepMs = std::chrono::duration<float, std::milli>(
                              std::chrono::steady_clock::now() - t0).count();
@@ -608,10 +624,7 @@ void match_impl(ppf_detector_s &D, DevCloud &c, float rel, float kpFrac, float m
     int nSparse = 0, nDenseIcp = 0;
     bool needGrids = p.sparsePoseRefinement && !cand.empty();
     if (needGrids) {
-        grid_build(D.ws.gSparse, PX, PY, PZ, NX, NY, NZ, samp, ns, sampleStep * kSparseCell, s,
-                   c.ext);
-        grid_build(D.ws.gFull, PX, PY, PZ, nullptr, nullptr, nullptr, c.valid.get<int>(),
-                   c.nValid, std::max(scoringDist * 2.0f, sampleStep * 0.25f), s, c.ext);
+        // gSparse and gFull were built on the lane (laneGrids)
         IcpCriteria sc{5, poseRefDist, sampleStep * 0.5f, sampleStep, 0.05f};  // ppf.cpp:423
         std::vector<float>        init(cand.size() * 12);
         std::vector<IcpResultDev> res(cand.size());
@@ -643,8 +656,7 @@ void match_impl(ppf_detector_s &D, DevCloud &c, float rel, float kpFrac, float m
             survivors.push_back(i);
         }
         if (p.densePoseRefinement && !survivors.empty()) {
-            grid_build(D.ws.gDense, PX, PY, PZ, NX, NY, NZ, dense, nd, reSampleStep * kDenseCell, s,
-                       c.ext);
+            // gDense was built on the lane
             IcpCriteria dc{p.poseRefNumSteps, poseRefDist, reSampleStep * 0.5f, sampleStep, 0.05f};
             std::vector<float>        init2(survivors.size() * 12);
             std::vector<IcpResultDev> res2(survivors.size());
