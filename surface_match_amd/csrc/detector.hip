@@ -14,7 +14,6 @@
 
 #include <algorithm>
 #include <atomic>
-#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <exception>
@@ -264,8 +263,12 @@ struct ppf_detector_s {
     static constexpr int kLanes = 4;
     struct Lane {
         hipStream_t stream = nullptr;
+        hipEvent_t  ev[2]  = {nullptr, nullptr};  // brackets timed lane work
         Workspace   ws;
         ~Lane() {
+            for (auto e : ev)
+                if (e)
+                    (void)hipEventDestroy(e);
             if (stream)
                 (void)hipStreamDestroy(stream);
         }
@@ -453,10 +456,15 @@ ppf_detector_s::Lane &lane_of(ppf_detector_s &D, int l) {
     while ((int)D.lanes.size() <= l) {
         auto ln = std::make_unique<ppf_detector_s::Lane>();
         PPF_HIP_CHECK(hipStreamCreateWithFlags(&ln->stream, hipStreamNonBlocking));
+        for (auto &e : ln->ev)
+            PPF_HIP_CHECK(hipEventCreate(&e));
         D.lanes.push_back(std::move(ln));
     }
     return *D.lanes[l];
 }
+
+// the dense prep waits for the vote's pair kernel
+constexpr bool PREP_AFTER_PAIRS = true;
 
 // joins a helper thread however the scope is left
 struct JoinGuard {
@@ -548,6 +556,7 @@ void match_impl(ppf_detector_s &D, DevCloud &c, float rel, float kpFrac, float m
     int        *dense = D.idxC.as<int>(std::max(c.nValid, 1));
     int         nd    = 0;
     float       prepMs = 0.0f;
+    int         np    = 0;
     std::thread prep;
     std::exception_ptr prepErr;
     JoinGuard   prepJoin{prep};
@@ -556,12 +565,22 @@ void match_impl(ppf_detector_s &D, DevCloud &c, float rel, float kpFrac, float m
         ppf_detector_s::Lane &ln = lane_of(D, 0);
         PPF_HIP_CHECK(hipEventRecord(D.ev[12], s));
         PPF_HIP_CHECK(hipStreamWaitEvent(ln.stream, D.ev[12], 0));
+        D.ws.pairQueued.store(0, std::memory_order_relaxed);
         int dev = 0;
         PPF_HIP_CHECK(hipGetDevice(&dev));
         prep = std::thread([&, dev] {
             try {
                 PPF_HIP_CHECK(hipSetDevice(dev));
-                const auto t0 = std::chrono::steady_clock::now();
+                // start after the pair kernel (the VALU-bound stage), beside
+                // the LDS-bound hit voting and the clustering
+                if (PREP_AFTER_PAIRS) {
+                    int q;
+                    while ((q = D.ws.pairQueued.load(std::memory_order_acquire)) == 0)
+                        std::this_thread::yield();
+                    if (q == 1)
+                        PPF_HIP_CHECK(hipStreamWaitEvent(ln.stream, D.ws.voteEv[2], 0));
+                }
+                PPF_HIP_CHECK(hipEventRecord(ln.ev[0], ln.stream));
                 if (p.densePoseRefinement) {
                     nd = voxel_sample(ln.ws, PX, PY, PZ, c.valid.get<int>(), c.nValid,
                                       reSampleStep, c.boxMin, dense, ln.ws.smallCnt.as<int>(8),
@@ -581,9 +600,9 @@ void match_impl(ppf_detector_s &D, DevCloud &c, float rel, float kpFrac, float m
                         grid_build(D.ws.gDense, PX, PY, PZ, NX, NY, NZ, dense, nd,
                                    reSampleStep * kDenseCell, ln.stream, c.ext);
                 }
+                PPF_HIP_CHECK(hipEventRecord(ln.ev[1], ln.stream));
                 PPF_HIP_CHECK(hipStreamSynchronize(ln.stream));
-                prepMs = std::chrono::duration<float, std::milli>(
-                             std::chrono::steady_clock::now() - t0).count();
+                PPF_HIP_CHECK(hipEventElapsedTime(&prepMs, ln.ev[0], ln.ev[1]));
             } catch (...) {
                 prepErr = std::current_exception();
             }
@@ -591,8 +610,19 @@ void match_impl(ppf_detector_s &D, DevCloud &c, float rel, float kpFrac, float m
     }
     VoteStats vs{};
     PPF_HIP_CHECK(hipEventRecord(D.ev[4], s));
-    int np = vote_and_poses(D.ws, dm, D.ws.gVote.view, PX, PY, PZ, NX, NY, NZ, kp, nk,
+    {
+        // however the vote ends (no pair kernel, an error), the helper's
+        // wait for the pair kernel ends with it
+        struct PairRelease {
+            std::atomic<int> &f;
+            ~PairRelease() {
+                int z = 0;
+                f.compare_exchange_strong(z, 2, std::memory_order_acq_rel);
+            }
+        } release{D.ws.pairQueued};
+        np = vote_and_poses(D.ws, dm, D.ws.gVote.view, PX, PY, PZ, NX, NY, NZ, kp, nk,
                             voteThreshold, D.vcfg, &vs, nullptr, 0, s);
+    }
     PPF_HIP_CHECK(hipEventRecord(D.ev[5], s));
     // cluster (ppf.cpp:413-420)
     std::vector<Hypo> hyp;

@@ -4,6 +4,8 @@
 
 #include "dev_common.h"
 
+#include <atomic>
+
 namespace ppfd {
 
 // A captured launch and the argument bytes it was captured with.
@@ -45,6 +47,10 @@ struct Workspace {
     int        icpGraphNext = 0;
     // events bracketing the vote kernel (created on first use)
     hipEvent_t voteEv[3] = {nullptr, nullptr, nullptr};
+    // set once the pair kernel and voteEv[2] after it are queued (split
+    // path): lets a helper thread start work that should follow the pair
+    // kernel on another stream
+    std::atomic<int> pairQueued{0};
     ~Workspace() {
         for (auto &e : voteEv)
             if (e)

@@ -1445,6 +1445,7 @@ int vote_keypoints(Workspace &ws, const DevModel &m, const VoteKeypoints &in, fl
             pair_kernel<false><<<padded(nk), PAIR_BLOCK, 0, s>>>(a);
         PPF_HIP_CHECK(hipGetLastError());
         PPF_HIP_CHECK(hipEventRecord(ws.voteEv[2], s));
+        ws.pairQueued.store(1, std::memory_order_release);
         const bool   packed = VOTE_PACKED && nTiles == 1;
         const size_t ldsH   = packed ? ((size_t)(tileRefs + 1) * m.angleNum + 1) / 2 * 4 : lds;
         switch (cfg.block) {
