@@ -466,6 +466,17 @@ ppf_detector_s::Lane &lane_of(ppf_detector_s &D, int l) {
 // the dense prep waits for the vote's pair kernel
 constexpr bool PREP_AFTER_PAIRS = true;
 
+// ends a helper's wait for the pair kernel however the vote ends (no pair
+// kernel queued: 2), without overwriting the vote's own 1
+struct PairRelease {
+    std::atomic<int> &f;
+    void operator()() {
+        int z = 0;
+        f.compare_exchange_strong(z, 2, std::memory_order_acq_rel);
+    }
+    ~PairRelease() { (*this)(); }
+};
+
 // joins a helper thread however the scope is left
 struct JoinGuard {
     std::thread &t;
@@ -560,6 +571,9 @@ void match_impl(ppf_detector_s &D, DevCloud &c, float rel, float kpFrac, float m
     std::thread prep;
     std::exception_ptr prepErr;
     JoinGuard   prepJoin{prep};
+    // released after the vote, or on the way out if anything throws first
+    // (before the join above)
+    PairRelease pairRelease{D.ws.pairQueued};
     const bool laneGrids = p.sparsePoseRefinement != 0;
     if (p.densePoseRefinement || laneGrids) {
         ppf_detector_s::Lane &ln = lane_of(D, 0);
@@ -610,19 +624,9 @@ void match_impl(ppf_detector_s &D, DevCloud &c, float rel, float kpFrac, float m
     }
     VoteStats vs{};
     PPF_HIP_CHECK(hipEventRecord(D.ev[4], s));
-    {
-        // however the vote ends (no pair kernel, an error), the helper's
-        // wait for the pair kernel ends with it
-        struct PairRelease {
-            std::atomic<int> &f;
-            ~PairRelease() {
-                int z = 0;
-                f.compare_exchange_strong(z, 2, std::memory_order_acq_rel);
-            }
-        } release{D.ws.pairQueued};
-        np = vote_and_poses(D.ws, dm, D.ws.gVote.view, PX, PY, PZ, NX, NY, NZ, kp, nk,
-                            voteThreshold, D.vcfg, &vs, nullptr, 0, s);
-    }
+    np = vote_and_poses(D.ws, dm, D.ws.gVote.view, PX, PY, PZ, NX, NY, NZ, kp, nk,
+                        voteThreshold, D.vcfg, &vs, nullptr, 0, s);
+    pairRelease();
     PPF_HIP_CHECK(hipEventRecord(D.ev[5], s));
     // cluster (ppf.cpp:413-420)
     std::vector<Hypo> hyp;

@@ -327,6 +327,43 @@ def test_match_workload1_poses(trained, oracle_mod, workload1):
     print("gpu scores", scores, "oracle scores", osc, trained.last_stats)
 
 
+@pytest.mark.parametrize("sparse,dense", [(1, 0), (0, 1), (0, 0), (1, 1)])
+def test_match_refinement_switches(trained, oracle_mod, workload1, sparse, dense):
+    """sparsePoseRefinement / densePoseRefinement on and off (ppf.cpp:423-495):
+    without the sparse refinement the results are the clustered poses scored
+    by their votes (equal to the oracle's bit for bit), the dense refinement
+    only follows a sparse one; refined poses within 0.1 deg / 0.1 % D of the
+    FP64 oracle refinement and scores within 1e-5.  The detector's helper
+    stream prepares the refinement's inputs for every combination with
+    refinement, so each case also runs that path."""
+    from surface_match_amd import PointCloud, MatchParam, synth
+    O = oracle_mod
+    sc = workload1["scene"]
+    D = trained.model_info().diameter
+    cloud = PointCloud(sc.points)
+    cloud.viewPoint = np.array(synth.VIEWPOINT, np.float32)
+    poses, scores = trained.matchScene(
+        cloud, 0.04, 0.1, 0.2,
+        MatchParam(numMatches=3, sparsePoseRefinement=sparse, densePoseRefinement=dense))
+    mp, mn = workload1["model"]
+    om = O.train(mp, mn, 0.04)
+    op = O.default_match_param(numMatches=3)
+    op.sparsePoseRefinement, op.densePoseRefinement = sparse, dense
+    opos, osc, _ = O.match(om, sc.points, None, 0.04, 0.1, 0.2, vp=synth.VIEWPOINT, param=op,
+                           threads=8)
+    assert len(poses) == len(opos) == 3
+    if sparse:
+        assert np.abs(np.asarray(scores) - osc).max() <= 1e-5, (scores, osc)
+        assert np.all(np.asarray(scores) <= 1.0)
+    else:
+        assert np.array_equal(np.asarray(scores, np.float32), osc), (scores, osc)
+        assert scores[0] > 1.0  # vote counts, not inlier fractions
+    for P in poses:
+        errs = [synth.pose_error(P, [np.vstack([Q, [0, 0, 0, 1]])], D) for Q in opos]
+        _, ang, dt = min(errs, key=lambda e: e[1] + e[2])
+        assert ang < 0.1 and dt < 0.001 * D, (sparse, dense, ang, dt)
+
+
 def test_save_load_roundtrip(trained, tmp_path):
     from surface_match_amd import Detector
     f = tmp_path / "m.model"
