@@ -364,6 +364,46 @@ def test_match_refinement_switches(trained, oracle_mod, workload1, sparse, dense
         assert ang < 0.1 and dt < 0.001 * D, (sparse, dense, ang, dt)
 
 
+def test_match_degenerate_and_translated_scenes(trained, oracle_mod, workload1):
+    """ppf.cpp:167-168: an empty scene is an error; scenes with no valid
+    point, or too few to give a key point, end with no result (the oracle
+    too).  The detector still matches afterwards: the scene moved by 1e4 mm
+    along every axis gives the oracle's results (same float arithmetic at
+    that magnitude), poses within 0.1 deg / 0.1 % D and scores within 1e-5."""
+    from surface_match_amd import PointCloud, MatchParam, synth
+    O = oracle_mod
+    sc = workload1["scene"]
+    D = trained.model_info().diameter
+    mp, mn = workload1["model"]
+    om = O.train(mp, mn, 0.04)
+    prm = MatchParam(numMatches=3)
+    with pytest.raises(RuntimeError, match="empty scene in matchScene"):
+        trained.matchScene(PointCloud(np.zeros((0, 3), np.float32)), 0.04, 0.1, 0.2, prm)
+    for pts in (np.full((1000, 3), np.nan, np.float32), sc.points[:5].copy(),
+                sc.points[:1].copy()):
+        poses, scores = trained.matchScene(PointCloud(pts), 0.04, 0.1, 0.2, prm)
+        _, osc, _ = O.match(om, pts, None, 0.04, 0.1, 0.2, vp=synth.VIEWPOINT,
+                            param=O.default_match_param(numMatches=3), threads=8)
+        assert len(poses) == len(scores) == len(osc) == 0
+    moved = (sc.points + np.float32(1e4)).astype(np.float32)
+    cloud = PointCloud(moved)
+    cloud.viewPoint = np.array(synth.VIEWPOINT, np.float32)
+    poses, scores = trained.matchScene(cloud, 0.04, 0.1, 0.2, prm)
+    opos, osc, _ = O.match(om, moved, None, 0.04, 0.1, 0.2, vp=synth.VIEWPOINT,
+                           param=O.default_match_param(numMatches=3), threads=8)
+    assert len(poses) == len(opos) == 3
+    assert np.abs(np.asarray(scores) - osc).max() <= 1e-5, (scores, osc)
+    shift = np.eye(4)
+    shift[:3, 3] = 1e4
+    for P in poses:
+        errs = [synth.pose_error(P, [np.vstack([Q, [0, 0, 0, 1]])], D) for Q in opos]
+        _, ang, dt = min(errs, key=lambda e: e[1] + e[2])
+        assert ang < 0.1 and dt < 0.001 * D, (ang, dt)
+        # and they are poses of the scene's instances, moved
+        _, ang, dt = synth.pose_error(P, [shift @ G for G in sc.gt_poses], D)
+        assert ang < 1.0 and dt < 0.01 * D, (ang, dt)
+
+
 def test_save_load_roundtrip(trained, tmp_path):
     from surface_match_amd import Detector
     f = tmp_path / "m.model"
