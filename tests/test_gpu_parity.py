@@ -452,12 +452,16 @@ def test_autotune_keeps_results(trained, workload1):
 
 def test_batched_scenes_equal_individual(trained):
     """Several scenes matched together (one vote launch, one batched
-    refinement per stage) give every scene exactly its single-scene result."""
+    refinement per stage) give every scene exactly its single-scene result,
+    also with a scene of no valid point first and a one-point scene between
+    (no key point: no result)."""
     from surface_match_amd import PointCloud, MatchParam, ResidentScene, synth
     scenes = synth.batch_scenes(0x5EED0003, n_scenes=4, lo=20000, hi=80000)
+    clouds = [np.full((500, 3), np.nan, np.float32)] + [sc.points for sc in scenes]
+    clouds.insert(3, scenes[0].points[:1].copy())
     rs = []
-    for sc in scenes:
-        c = PointCloud(sc.points)
+    for pts in clouds:
+        c = PointCloud(pts)
         c.viewPoint = np.array(synth.VIEWPOINT, np.float32)
         rs.append(ResidentScene(trained, c))
     param = MatchParam(numMatches=3)
@@ -470,6 +474,8 @@ def test_batched_scenes_equal_individual(trained):
         assert np.array_equal(P, P1)
         total += st1["nKeypoints"]
     assert st["nKeypoints"] == total
+    assert len(many[0][1]) == 0 and len(many[3][1]) == 0
+    assert all(len(S) > 0 for i, (_, S) in enumerate(many) if i not in (0, 3))
     print("batched scenes", [len(S) for _, S in many], st["msTotal"])
 
 
