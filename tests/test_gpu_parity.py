@@ -404,6 +404,50 @@ def test_match_degenerate_and_translated_scenes(trained, oracle_mod, workload1):
         assert ang < 1.0 and dt < 0.01 * D, (ang, dt)
 
 
+def test_match_scene_with_normals(trained, oracle_mod, workload1):
+    """A scene that brings its own normals (ppf.cpp:170, 208-211): points
+    with a NaN normal are dropped, the rest normalised instead of estimated,
+    and the dense refinement keeps them.  Normals: the oracle's kNN
+    estimates scaled by 0.5..3, NaN on 1 % of the points.  Same results as
+    the oracle: poses within 0.1 deg / 0.1 % D; the score gate is the GPU's
+    scoring at the oracle's pose, equal to the oracle's score within 1e-5
+    (at the GPU's own pose, a rounding-level different one, a point at the
+    scoring radius may flip, so the own-pose scores are held to one
+    inlier)."""
+    from surface_match_amd import PointCloud, MatchParam, ResidentScene, synth
+    O = oracle_mod
+    sc = workload1["scene"]
+    D = trained.model_info().diameter
+    pts = sc.points
+    valid = np.arange(len(pts), dtype=np.int32)
+    _, nrm = O.estimate_normals(pts, np.full_like(pts, np.nan), valid, valid, 10, 0, 0,
+                                synth.VIEWPOINT)
+    r = rng(11)
+    nrm = (nrm * r.uniform(0.5, 3.0, (len(nrm), 1))).astype(np.float32)
+    nrm[r.choice(len(nrm), len(nrm) // 100, replace=False)] = np.nan
+    cloud = PointCloud(pts, nrm)
+    cloud.viewPoint = np.array(synth.VIEWPOINT, np.float32)
+    rs = ResidentScene(trained, cloud)
+    poses, scores, _ = rs.match(0.04, 0.1, 0.2, MatchParam(numMatches=3))
+    om = O.train(*workload1["model"], 0.04)
+    opos, osc, _ = O.match(om, pts, nrm, 0.04, 0.1, 0.2, vp=synth.VIEWPOINT,
+                           param=O.default_match_param(numMatches=3), threads=8)
+    assert len(poses) == len(opos) == 3
+    info = trained.model_info()
+    OP4 = np.array([np.vstack([Q, [0, 0, 0, 1]]) for Q in opos], np.float32)
+    inl = rs.inliers(OP4, dense=True, dist=np.float32(D) * np.float32(0.01))
+    at_oracle = np.minimum(inl.astype(np.float32) / np.float32(info.nReSampled), np.float32(1))
+    assert np.abs(at_oracle - osc).max() <= 1e-5, (at_oracle, osc)
+    one = 1.0 / info.nReSampled
+    assert np.abs(np.asarray(scores) - osc).max() <= one + 1e-6, (scores, osc)
+    for P in poses:
+        errs = [synth.pose_error(P, [np.vstack([Q, [0, 0, 0, 1]])], D) for Q in opos]
+        _, ang, dt = min(errs, key=lambda e: e[1] + e[2])
+        assert ang < 0.1 and dt < 0.001 * D, (ang, dt)
+        _, ang, dt = synth.pose_error(P, sc.gt_poses, D)
+        assert ang < 1.0 and dt < 0.01 * D, (ang, dt)
+
+
 def test_save_load_roundtrip(trained, tmp_path):
     from surface_match_amd import Detector
     f = tmp_path / "m.model"
