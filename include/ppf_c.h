@@ -307,12 +307,16 @@ PPF_API int ppf_icp_refine(ppf_icp_target t, const float *src, const float *srcN
                            int mode, int batched, int block, float scoringDist,
                            ppf_icp_result *out, float *deviceMs);
 /* point-to-plane cost adjoint (no reference counterpart; its Jacobian rows
- * are icp.cpp:318-320's): for each pose T = U(xi) T0, U(xi) = [R(w),
- * c - R(w)c + v], the correspondences of the refiner, and out[36] per pose =
- * {cost, count, c[3], grad[6] (= J^T r at xi = 0), J^T J[21] (upper
- * triangle, row-major), 0 x4}; center 0 = origin, 1 = winners' centroid.
- * corr (optional, nPoses x nSrc) receives each source point's target id
- * (-1 = none). */
+ * are icp.cpp:318-320's): for each pose T = U(xi) T0 with U(xi) = [R(w), v]
+ * (xi = (w, v), R(w) = exp([w]x) by Rodrigues' formula), the refiner's
+ * correspondences at T and out[36] per pose = {cost, count, c[3], g[6],
+ * J^T J[21] (upper triangle, row-major), 0 x4}.  g = J^T r is the gradient
+ * with respect to a perturbation d = (om, u) applied on the left of T as
+ * [R(om), c - R(om) c + u], about the centre c: the origin (center 0) or
+ * the winners' centroid (center 1); at xi = 0 it is the gradient in xi
+ * when c = 0, and d cost / d xi = (Jl(w)^T (g_om + g_u x (v - c)), g_u)
+ * in general (Jl = SO(3)'s left Jacobian).  corr (optional, nPoses x nSrc)
+ * receives each source point's target id (-1 = none). */
 PPF_API int ppf_icp_adjoint(ppf_icp_target t, const float *src, size_t nSrc, const float *poses16,
                             const double *xi6, size_t nPoses, float rejectDist, int center,
                             double *out, int *corr);

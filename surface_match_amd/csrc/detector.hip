@@ -1192,17 +1192,39 @@ void write_cloud(std::ofstream &f, const HostCloud &c) {  // serialize.cpp:59-86
     f.write(reinterpret_cast<const char *>(c.vp), sizeof(float) * 3);
 }
 
-void read_cloud(std::ifstream &f, HostCloud &c) {
-    uint32_t np = 0, nn = 0;
-    rd(f, np);
+// bytes of the file not read yet (0 once the stream has failed)
+size_t remaining(std::ifstream &f, size_t fileSize) {
+    if (!f.good())
+        return 0;
+    std::streamoff at = f.tellg();
+    return at < 0 || (size_t)at > fileSize ? 0 : fileSize - (size_t)at;
+}
+
+// an element count read from the file, checked against what is left of it
+// before anything is allocated for it (a corrupt count would otherwise ask
+// for gigabytes or wrap a size computation)
+uint32_t read_count(std::ifstream &f, size_t fileSize, size_t elemBytes, const char *path) {
+    uint32_t n = 0;
+    rd(f, n);
+    if (!f.good() || (size_t)n > remaining(f, fileSize) / elemBytes)
+        throw std::runtime_error(std::string("truncated model file:") + path);
+    return n;
+}
+
+void read_cloud(std::ifstream &f, size_t fileSize, HostCloud &c, const char *path) {
+    uint32_t np = read_count(f, fileSize, 12, path);
     c.p.resize((size_t)np * 3);
     f.read(reinterpret_cast<char *>(c.p.data()), sizeof(float) * c.p.size());
-    rd(f, nn);
+    uint32_t nn = read_count(f, fileSize, 12, path);
     c.n.resize((size_t)nn * 3);
     f.read(reinterpret_cast<char *>(c.n.data()), sizeof(float) * c.n.size());
     f.read(reinterpret_cast<char *>(c.boxMin), sizeof(float) * 3);
     f.read(reinterpret_cast<char *>(c.boxMax), sizeof(float) * 3);
     f.read(reinterpret_cast<char *>(c.vp), sizeof(float) * 3);
+    // a trained model has a normal per point (trainModel estimates the
+    // missing ones); the voting frames and the refinement read both arrays
+    if (f.good() && nn != np)
+        throw std::runtime_error(std::string("corrupt point cloud in:") + path);
 }
 
 void save_impl(const ppf_detector_s &D, const char *path) {
@@ -1242,6 +1264,9 @@ void load_impl(ppf_detector_s &D, const char *path) {
     if (!f.is_open())
         throw std::runtime_error(std::string("failed to open file:") + path);
     D.trained = false;
+    f.seekg(0, std::ios::end);
+    const size_t fileSize = f.tellg() < 0 ? 0 : (size_t)f.tellg();
+    f.seekg(0, std::ios::beg);
     int magic = 0, version = 0;
     rd(f, magic);
     if (magic != kMagic)
@@ -1256,30 +1281,47 @@ void load_impl(ppf_detector_s &D, const char *path) {
     uint8_t smooth = 0;
     rd(f, smooth);
     m.param.smoothNormal = smooth ? 1 : 0;
-    read_cloud(f, m.sampled);
-    read_cloud(f, m.reSampled);
-    uint32_t nk = 0;
-    rd(f, nk);
+    if (!f.good())
+        throw std::runtime_error(std::string("truncated model file:") + path);
+    // the stored TrainParam must be one the constructor accepts: the angle
+    // resolution sizes the vote accumulator and the steps the quantisation
+    check_train_param(m.param);
+    read_cloud(f, fileSize, m.sampled, path);
+    read_cloud(f, fileSize, m.reSampled, path);
+    if (m.sampled.size() == 0)
+        throw std::runtime_error(std::string("corrupt point cloud in:") + path);
+    // a bucket takes at least 12 bytes (key and two counts)
+    uint32_t nk = read_count(f, fileSize, 12, path);
     // buckets keep the file's intra-bucket order (it decides which entries
     // take the bin tail rule); buckets are re-ordered by key for the CSR.
     std::vector<std::pair<uint32_t, std::pair<std::vector<uint32_t>, std::vector<float>>>> b(nk);
+    const uint32_t nRef = (uint32_t)m.sampled.size();
     for (uint32_t k = 0; k < nk && f.good(); k++) {
-        uint32_t len = 0, lenA = 0;
         rd(f, b[k].first);
-        rd(f, len);
+        uint32_t len = read_count(f, fileSize, 4, path);
         b[k].second.first.resize(len);
         f.read(reinterpret_cast<char *>(b[k].second.first.data()), sizeof(uint32_t) * len);
-        rd(f, lenA);
-        b[k].second.second.resize(lenA);
-        f.read(reinterpret_cast<char *>(b[k].second.second.data()), sizeof(float) * lenA);
+        uint32_t lenA = read_count(f, fileSize, 4, path);
         if (lenA != len)
             throw std::runtime_error(std::string("corrupt hash table in:") + path);
+        b[k].second.second.resize(lenA);
+        f.read(reinterpret_cast<char *>(b[k].second.second.data()), sizeof(float) * lenA);
+        // every entry names a sampled model point (its frame and its row of
+        // the vote accumulator)
+        for (uint32_t r : b[k].second.first)
+            if (r >= nRef)
+                throw std::runtime_error(std::string("corrupt hash table in:") + path);
     }
     if (!f.good())
         throw std::runtime_error(std::string("truncated model file:") + path);
     std::stable_sort(b.begin(), b.end(), [](const auto &x, const auto &y) { return x.first < y.first; });
     m.offs.assign(1, 0);
-    for (auto &e : b) {
+    for (size_t k = 0; k < b.size(); k++) {
+        // a key stored twice keeps its first bucket, as the reference's
+        // flat_hash_map::emplace does (serialize.cpp:150-159)
+        if (k > 0 && b[k].first == b[k - 1].first)
+            continue;
+        auto &e = b[k];
         m.keys.push_back(e.first);
         m.refs.insert(m.refs.end(), e.second.first.begin(), e.second.first.end());
         m.alphas.insert(m.alphas.end(), e.second.second.begin(), e.second.second.end());
@@ -1661,14 +1703,27 @@ PPF_API int ppf_match_scenes_resident(ppf_detector d, const ppf_scene *scenes, s
             p = *param;
         else
             ppf_match_param_default(&p);
-        std::vector<DevCloud *> clouds(nScenes);
-        for (size_t i = 0; i < nScenes; i++)
-            clouds[i] = &scenes[i]->cloud;
+        // a scene named more than once is matched once (its stages move its
+        // points in place, so two lanes must never work on one scene) and
+        // its result is written to every entry that names it
+        std::vector<DevCloud *> clouds;
+        std::vector<size_t>     slot(nScenes);
+        for (size_t i = 0; i < nScenes; i++) {
+            size_t j = 0;
+            while (j < i && scenes[j] != scenes[i])
+                j++;
+            if (j == i) {
+                slot[i] = clouds.size();
+                clouds.push_back(&scenes[i]->cloud);
+            } else {
+                slot[i] = slot[j];
+            }
+        }
         std::vector<MatchOut> outs;
-        if (nScenes > 0)
+        if (!clouds.empty())
             match_batch_impl(*d, clouds, rel, kpFrac, minScore, p, outs, stats);
         for (size_t i = 0; i < nScenes; i++)
-            write_poses(outs[i], poses16 ? poses16 + 16 * cap * i : nullptr,
+            write_poses(outs[slot[i]], poses16 ? poses16 + 16 * cap * i : nullptr,
                         scores ? scores + cap * i : nullptr, cap, nOut + i);
     });
 }

@@ -1240,10 +1240,11 @@ void icp_inliers(Workspace &ws, const GridView &full, int nSrc, const float *sx,
 //
 // Point-to-plane cost E = 1/2 sum_i r_i^2, r_i = n_q.(x_i - q_i), over the
 // refiner's correspondences at the pose T = U(xi) T0, where
-// U(xi) = [R(w), c - R(w) c + v] (Rodrigues, xi = (w, v), c = the rotation
-// centre: the origin or the winners' centroid).  Its gradient at xi = 0 is
-// J^T r with J_i = [(x_i - c) x n_q ; n_q] -- the Jacobian rows of the
-// reference's point-to-plane solver (icp.cpp:318-320) -- and J^T J is the
+// U(xi) = [R(w), v] (Rodrigues, xi = (w, v)).  The kernel returns J^T r with
+// J_i = [(x_i - c) x n_q ; n_q] -- the Jacobian rows of the reference's
+// point-to-plane solver (icp.cpp:318-320) -- the gradient with respect to a
+// perturbation [R(om), c - R(om) c + u] applied on the left of T about the
+// centre c (the origin or the winners' centroid), and J^T J, the
 // Gauss-Newton matrix the refiner solves with (mode 1).
 
 struct AdjArgs {
@@ -1267,7 +1268,8 @@ __global__ void __launch_bounds__(BLOCK) adjoint_kernel(AdjArgs a) {
     const int         n   = a.b.nSrc;
     const HypoScratch x   = scratch_of(a.b, h);
     if (tid == 0) {
-        // T = U(xi) T0 about the origin; the centred form is applied below
+        // T = U(xi) T0 (U about the origin; the centre c only enters the
+        // perturbation the gradient is taken in)
         double w[3] = {a.xi[h * 6 + 0], a.xi[h * 6 + 1], a.xi[h * 6 + 2]}, R[9];
         rodrigues(w, R);
         double U[12], T0[12], T[12];

@@ -96,19 +96,52 @@ def _batched_code(b):
     raise ValueError(f"batched must be True/'spread', 'persistent' or False, not {b!r}")
 
 
-def point_to_plane_cost():
-    """torch.autograd.Function: cost(xi) for H poses, gradient = J^T r.
+def _skew(a):
+    return np.array([[0.0, -a[2], a[1]], [a[2], 0.0, -a[0]], [-a[1], a[0], 0.0]])
 
-    Gradients are exact at xi = 0 (the Gauss-Newton increment about the
-    current pose), which is how the refiner uses them."""
+
+def left_jacobian_so3(w):
+    """Jl(w) of SO(3): R(w + dw) R(w)^T = exp([Jl(w) dw]x) to first order,
+    Jl = I + (1 - cos t) / t^2 [w]x + (t - sin t) / t^3 [w]x^2, t = |w|."""
+    w = np.asarray(w, np.float64)
+    t = float(np.linalg.norm(w))
+    K = _skew(w)
+    if t < 1e-6:
+        return np.eye(3) + 0.5 * K + K @ K / 6.0
+    return np.eye(3) + (1.0 - np.cos(t)) / t ** 2 * K + (t - np.sin(t)) / t ** 3 * (K @ K)
+
+
+def twist_gradient(xi, g, center):
+    """d cost / d xi for T = [R(w), v] T0 from the adjoint kernel's g = J^T r,
+    the gradient in a left perturbation [R(om), c - R(om) c + u] of T.  With
+    R(w + dw) R(w)^T = I + [phi]x, phi = Jl(w) dw, the change of xi is that
+    perturbation with om = phi and u = dv + (v - c) x phi, so
+    d/dw = Jl(w)^T (g_om + g_u x (v - c)) and d/dv = g_u."""
+    xi = np.asarray(xi, np.float64).reshape(-1, 6)
+    g = np.asarray(g, np.float64).reshape(-1, 6)
+    c = np.asarray(center, np.float64).reshape(-1, 3)
+    out = np.empty_like(g)
+    for h in range(len(xi)):
+        w, v = xi[h, :3], xi[h, 3:]
+        out[h, :3] = left_jacobian_so3(w).T @ (g[h, :3] + np.cross(g[h, 3:], v - c[h]))
+        out[h, 3:] = g[h, 3:]
+    return out
+
+
+def point_to_plane_cost():
+    """torch.autograd.Function: cost(xi) for H poses T = [R(w), v] T0 and its
+    gradient d cost / d xi (the adjoint kernel's J^T r carried through the
+    exponential map's Jacobian; exact for the correspondences at T)."""
     import torch
 
     class PointToPlaneCost(torch.autograd.Function):
         @staticmethod
         def forward(ctx, xi, target: IcpTarget, src, poses, reject_dist=10.0, center=1):
-            r = target.adjoint(src, poses, xi.detach().cpu().numpy(), reject_dist, center)
-            ctx.save_for_backward(torch.from_numpy(r["grad"]).to(xi.dtype))
-            return torch.from_numpy(r["cost"]).to(xi.dtype)
+            x = xi.detach().cpu().numpy().astype(np.float64)
+            r = target.adjoint(src, poses, x, reject_dist, center)
+            grad = twist_gradient(x, r["grad"], r["center"])
+            ctx.save_for_backward(torch.from_numpy(grad).to(device=xi.device, dtype=xi.dtype))
+            return torch.from_numpy(r["cost"]).to(device=xi.device, dtype=xi.dtype)
 
         @staticmethod
         def backward(ctx, g):

@@ -121,7 +121,11 @@ def test_adjoint_vs_fd_and_icp_jacobian(icp_case, oracle_mod):
 
 def test_torch_autograd_binding(tmp_path):
     """The optional torch binding, in its own process (torch brings its own
-    HIP runtime; it only ever sees host tensors here)."""
+    HIP runtime; it only ever sees host tensors here).  The gradient of the
+    cost at T = [R(w), v] T0 with respect to xi = (w, v), at xi = 0 and at a
+    nonzero twist, about the origin and about the winners' centroid, against
+    central finite differences (h = 1e-6, FP64) of the oracle's cost on the
+    same correspondences."""
     try:
         import importlib.util
         if importlib.util.find_spec("torch") is None:
@@ -132,7 +136,9 @@ def test_torch_autograd_binding(tmp_path):
     script.write_text(f"""
 import sys, numpy as np
 sys.path.insert(0, {ROOT!r})
+sys.path.insert(0, {os.path.join(ROOT, "oracle")!r})
 import torch
+import oracle as O
 from surface_match_amd import synth
 from surface_match_amd.icp import IcpTarget, point_to_plane_cost
 src, _ = synth.make_model(1500, 80)
@@ -140,13 +146,24 @@ dst, dn = synth.make_model(20000, 81)
 T0 = np.eye(4); T0[:3, 3] = [0.5, -0.3, 0.2]
 tgt = IcpTarget(dst, dn, 2.0)
 F = point_to_plane_cost()
-xi = torch.zeros(1, 6, dtype=torch.float64, requires_grad=True)
-cost = F.apply(xi, tgt, src, T0[None], 10.0, 1)
-cost.sum().backward()
-r = tgt.adjoint(src, T0[None], reject_dist=10.0, center=1)
-assert np.allclose(xi.grad.numpy(), r["grad"]), (xi.grad, r["grad"])
-assert abs(cost.item() - r["cost"][0]) < 1e-9
-print("torch binding ok", cost.item(), xi.grad.numpy())
+for center in (0, 1):
+    for x0 in (np.zeros(6), np.array([0.02, -0.015, 0.01, 0.3, -0.2, 0.25])):
+        xi = torch.tensor(x0[None], dtype=torch.float64, requires_grad=True)
+        cost = F.apply(xi, tgt, src, T0[None], 10.0, center)
+        cost.sum().backward()
+        r = tgt.adjoint(src, T0[None], x0[None], reject_dist=10.0, center=center, with_corr=True)
+        assert abs(cost.item() - r["cost"][0]) < 1e-9
+        fd = np.zeros(6)
+        for k in range(6):
+            e = np.zeros(6); e[k] = 1e-6
+            fp = O.p2pl_system(src, T0[:3].ravel(), r["corr"][0], dst, dn, np.zeros(3), x0 + e)[0]
+            fm = O.p2pl_system(src, T0[:3].ravel(), r["corr"][0], dst, dn, np.zeros(3), x0 - e)[0]
+            fd[k] = (fp - fm) / 2e-6
+        g = xi.grad.numpy()[0]
+        rel = np.linalg.norm(g - fd) / np.linalg.norm(fd)
+        print("center", center, "xi", x0, "grad", g, "fd", fd, "rel", rel)
+        assert rel < 1e-3, (center, x0, g, fd)
+print("torch binding ok")
 """)
     out = subprocess.run([sys.executable, str(script)], capture_output=True, text=True, timeout=600)
     assert out.returncode == 0, out.stdout[-3000:] + out.stderr[-3000:]

@@ -5,50 +5,21 @@ point-to-plane adjoint over 1024 hypotheses of a 20k-point model in a
 icp.cpp's Jacobian.
 """
 import ctypes as C
-import hashlib
-import json
 import math
-import os
 
 import numpy as np
 import pytest
 
 from conftest import rng
+import parity
 
 pytestmark = pytest.mark.gpu
-HERE = os.path.dirname(os.path.abspath(__file__))
-TWO_PI_F = np.float32(2.0) * np.float32(math.pi)
-
-
-def _digest(a):
-    return hashlib.sha256(np.ascontiguousarray(a, np.float32).tobytes()).hexdigest()
 
 
 @pytest.fixture(scope="module")
 def workload5():
     from surface_match_amd import synth
     return synth.workload(5)
-
-
-def _oracle_workload5(w, oracle_mod):
-    """The oracle's result for the 4M-point scene: the committed fixture when
-    this run generated the same clouds (SHA-256), else computed here."""
-    from surface_match_amd import synth
-    with open(os.path.join(HERE, "golden", "workload5_oracle.json")) as f:
-        g = json.load(f)
-    if (g["scene_sha256"] == _digest(w["scene"].points) and
-            g["model_sha256"] == _digest(w["model"][0])):
-        S = np.array(g["scores_f32_bits"], np.uint32).view(np.float32)
-        P = np.array(g["poses_3x4"], np.float32).reshape(-1, 3, 4)
-        return P, S, g["match"]["numMatches"], "fixture"
-    O = oracle_mod
-    mp, mn = w["model"]
-    m = O.train(mp, mn, 0.04)
-    P, S, _ = O.match(m, w["scene"].points, None, 0.04, 0.1, 0.2,
-                      param=O.default_match_param(numMatches=g["match"]["numMatches"]),
-                      vp=np.array(synth.VIEWPOINT, np.float32),
-                      threads=min(os.cpu_count() or 1, 16))
-    return P, S, g["match"]["numMatches"], "computed"
 
 
 def test_workload5_matches_oracle(workload5, oracle_mod):
@@ -68,8 +39,15 @@ def test_workload5_matches_oracle(workload5, oracle_mod):
     det = Detector()
     det.trainModel(PointCloud(mp, mn), 0.04)
     info = det.model_info()
-    D = float(info.diameter)
-    OP, OS, nm, src = _oracle_workload5(w, O)
+    g = parity.load_fixture("workload5_oracle.json")
+    nm = g["match"]["numMatches"]
+    if (g["scene_sha256"] == parity.digest(w["scene"].points) and
+            g["model_sha256"] == parity.digest(mp)):
+        OP, OS = parity.fixture_result(g)
+        src = "fixture"
+    else:
+        OP, OS = parity.oracle_match(O, O.train(mp, mn, 0.04), w["scene"].points, nm)
+        src = "computed"
     cloud = PointCloud(w["scene"].points)
     cloud.viewPoint = np.array(synth.VIEWPOINT, np.float32)
     param = MatchParam(numMatches=nm)
@@ -77,49 +55,15 @@ def test_workload5_matches_oracle(workload5, oracle_mod):
     P, S, st = rs.match(0.04, 0.1, 0.2, param)
     print(f"oracle ({src}) scores {OS}; gpu scores {S}; gpu {st['msTotal']:.1f} ms, "
           f"{st['nSampled']} sampled, {st['nKeypoints']} key points, {st['nPairs']} pairs")
-    assert len(S) == len(OS)
-    for T, Q in zip(P, OP):
-        Q4 = np.vstack([Q.astype(np.float64), [0, 0, 0, 1]])
-        ang = synth.rotation_angle_deg(Q4[:3, :3].T @ T[:3, :3])
-        dt = float(np.linalg.norm(Q4[:3, 3] - T[:3, 3]))
-        assert ang < 0.1 and dt < 0.001 * D, (ang, dt)
-    OP4 = np.array([np.vstack([Q, [0, 0, 0, 1]]) for Q in OP], np.float32)
-    inl = rs.inliers(OP4, dense=True, dist=np.float32(D) * np.float32(0.01))
-    at_oracle = np.minimum(inl.astype(np.float32) / np.float32(info.nReSampled), np.float32(1))
-    assert np.abs(at_oracle - OS).max(initial=0.0) <= 1e-5, (at_oracle, OS)
+    at_oracle = parity.oracle_pose_scores(rs, OP, info.nReSampled)
+    a, t, s = parity.check_results(P, S, OP, OS, info.diameter, at_oracle)
     print(f"score at the oracle pose == oracle score; at the GPU's own pose max |diff| "
-          f"{np.abs(S - OS).max(initial=0.0):.2e} (one inlier = {1 / info.nReSampled:.2e})")
+          f"{s:.2e} (one inlier = {1 / info.nReSampled:.2e}); poses within {a:.2e} deg, "
+          f"{t:.2e} mm")
     mr = MatchResult()
     det.matchScene(cloud, 0.04, 0.1, 0.2, param, matchResult=mr)
-
-    # votes: sampled key points of the GPU's sampled scene, GPU vote kernel vs
-    # the oracle's vote loop on identical inputs
-    pts, nrm = mr.sampledScene.point, mr.sampledScene.normal
-    sp, sn, _, _ = det.sampled_model()
-    ang = TWO_PI_F / np.float32(30)
-    dist = np.float32(D) * np.float32(0.04)
-    ot = O.OracleTable(sp, sn, ang, dist)
-    refNum = info.nSampled
-    thr = np.float32(refNum) * np.float32(0.2)
-    Df = np.float32(D)
-    r = rng(55)
-    kps = np.sort(r.choice(len(pts), 12, replace=False)).astype(np.int32)
-    cand, _, acc = det.vote(pts, nrm, kps, minScore=0.2, dump=len(kps))
-    voted = 0
-    for t, k in enumerate(kps):
-        d = pts - pts[k]
-        d2 = ((np.float32(0) + d[:, 0] * d[:, 0]) + d[:, 1] * d[:, 1]) + d[:, 2] * d[:, 2]
-        nb = np.nonzero(d2 < Df * Df)[0]
-        if np.float32(len(nb)) < thr:
-            assert (cand[t] == -1).all()
-            continue
-        nb = nb[nb != k]
-        oacc = O.vote_keypoint(ot, refNum, 30, pts[k], nrm[k], pts[nb], nrm[nb], ang, dist)
-        assert np.array_equal(acc[t], oacc[:, 1:]), f"key point {k}"
-        got = [tuple(c) for c in cand[t] if c[0] >= 0]
-        assert got == [(a, b, int(v)) for a, b, v in O.peaks(oacc, refNum, 30, thr)]
-        voted += 1
-    print(f"{voted} of {len(kps)} sampled key points voted, accumulators bit-exact")
+    voted, n = parity.check_votes(det, O, mr.sampledScene.point, mr.sampledScene.normal, 12, 55)
+    print(f"{voted} of {n} sampled key points voted, accumulators bit-exact")
 
 
 def _perturb(T, r, deg=2.0, mm=2.0):
