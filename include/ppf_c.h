@@ -91,6 +91,8 @@ typedef struct {
     float  msPairKernel;  /* of msVoteKernel: the pair/key-hit kernel (split vote path) */
     float  msHitsKernel;  /* of msVoteKernel: hit voting + peaks (split vote path) */
     int    nVoteFallback; /* key points voted by the fused kernel for lack of hit capacity */
+    float  msIngest;      /* ppf_match_scene: host wall time of the scene's upload (copy to
+                             HBM, removeNan, bounding box; ppf.cpp:153,170-176) */
 } ppf_match_stats;
 
 /* -- parameters (TrainParam/MatchParam constructors, type.cpp:71-127) -- */
@@ -143,6 +145,16 @@ PPF_API const char *ppf_last_error(void);
 PPF_API int ppf_scene_upload(ppf_detector d, const float *points, const float *normals, size_t n,
                              const float *box6, const float *viewPoint, ppf_scene *out);
 PPF_API void ppf_scene_free(ppf_scene s);
+/* what an upload kept, as matchScene's first steps see it (ppf.cpp:153,
+ * 170-176): the points and normals as given (n x 3 each; normals only when
+ * the scene has them), the ascending indices of the valid points (finite
+ * point and, with normals, finite normal: removeNan, util.cpp:86-105) and
+ * the box {min xyz, max xyz} (the caller's, else computeBoundingBox of the
+ * valid points, util.cpp:178-207).  Any output may be NULL; points / normals
+ * need room for n, valid for *nValid points (sizes from a first call with
+ * NULL buffers). */
+PPF_API int ppf_scene_download(ppf_scene s, size_t *n, size_t *nValid, int *hasNormals,
+                               float *points, float *normals, int *valid, float box6[6]);
 PPF_API int ppf_match_scene_resident(ppf_detector d, ppf_scene s, float samplingDistanceRel,
                                      float keyPointFraction, float minScore,
                                      const ppf_match_param *param, float *poses16, float *scores,

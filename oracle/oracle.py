@@ -187,6 +187,9 @@ def lib():
         L.orc_train.argtypes = [C.c_int, fp, C.c_void_p, C.c_void_p, C.c_void_p, C.c_float,
                                 C.POINTER(TrainParam), C.c_int, C.POINTER(Model)]
         L.orc_free_model.argtypes = [C.POINTER(Model)]
+        L.orc_remove_nan.restype = C.c_int
+        L.orc_remove_nan.argtypes = [C.c_int, fp, C.c_void_p, ip]
+        L.orc_bounding_box.argtypes = [C.c_int, fp, ip, C.c_int, fp]
         L.orc_match.restype = C.c_int
         L.orc_match.argtypes = [C.POINTER(Model), C.c_int, fp, C.c_void_p, C.c_void_p,
                                 C.c_void_p, C.c_float, C.c_float, C.c_float,
@@ -375,6 +378,25 @@ def p2pl_system(src, T12, corr, dst, dstn, center, xi, tukey=0):
                           np.ascontiguousarray(center, np.float64),
                           np.ascontiguousarray(xi, np.float64), tukey, out)
     return out
+
+
+def remove_nan(pts, nrm=None):
+    """removeNan (util.cpp:86-105): ascending valid indices"""
+    pts = f32(pts)
+    nrm = None if nrm is None else f32(nrm)
+    out = np.zeros(max(len(pts), 1), np.int32)
+    m = lib().orc_remove_nan(len(pts), pts, _ptr(nrm), out)
+    return out[:m]
+
+
+def bounding_box(pts, idx):
+    """computeBoundingBox (util.cpp:178-207) over idx: {min xyz, max xyz}"""
+    pts = f32(pts)
+    idx = np.ascontiguousarray(idx, np.int32)
+    box = np.zeros(6, np.float32)
+    lib().orc_bounding_box(len(pts), pts, idx if len(idx) else np.zeros(1, np.int32), len(idx),
+                           box)
+    return box
 
 
 def train(pts, nrm=None, rel=0.04, param=None, box6=None, vp=None, ref_tail=1):

@@ -1979,6 +1979,22 @@ static int *valid_indices(int n, const float *pts, const float *nrm, int *m) {
     return v;
 }
 
+/* removeNan (util.cpp:86-105): ascending indices of the points with a finite
+ * position and, when nrm != NULL, a finite normal; returns their count */
+int orc_remove_nan(int n, const float *pts, const float *nrm, int *out) {
+    int  m;
+    int *v = valid_indices(n, pts, nrm, &m);
+    memcpy(out, v, sizeof(int) * (size_t)m);
+    free(v);
+    return m;
+}
+
+/* computeBoundingBox (util.cpp:178-207) over the given indices:
+ * box6 = {min xyz, max xyz} */
+void orc_bounding_box(int n, const float *pts, const int *idx, int m, float box6[6]) {
+    bbox_of(n, pts, idx, m, box6, box6 + 3);
+}
+
 static void gather(const float *src, const int *idx, int m, float *dst, int normalize) {
     for (int t = 0; t < m; t++) {
         memcpy(dst + 3 * t, src + 3 * idx[t], sizeof(float) * 3);

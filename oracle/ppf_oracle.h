@@ -99,6 +99,10 @@ typedef struct {
 int  orc_grid_build(orc_grid *g, int n, const float *pts, const int *subset, int nSubset,
                     float cell);
 void orc_grid_free(orc_grid *g);
+/* removeNan (util.cpp:86-105) and computeBoundingBox (util.cpp:178-207) */
+int  orc_remove_nan(int n, const float *pts, const float *nrm, int *out);
+void orc_bounding_box(int n, const float *pts, const int *idx, int m, float box6[6]);
+
 /* nanoflann L2 metric for DIM=3: ((0+dx^2)+dy^2)+dz^2, dx = q - p. */
 float orc_dist2(const float q[3], const float p[3]);
 /* all ids with dist2 < r2 (nanoflann RadiusResultSet::addPoint), ascending id */

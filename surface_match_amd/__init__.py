@@ -335,6 +335,21 @@ class ResidentScene:
         k = nout.value
         return poses[:k].reshape(k, 4, 4).transpose(0, 2, 1).copy(), scores[:k].copy(), st.as_dict()
 
+    def download(self):
+        """What the upload kept: (points [n,3], normals [n,3] or None, valid
+        indices, box [6]) -- the cloud as given, removeNan's indices and the
+        box the match uses (ppf.cpp:153,170-176)."""
+        n, nv, hn = C.c_size_t(), C.c_size_t(), C.c_int()
+        N.check(N.lib().ppf_scene_download(self._h, C.byref(n), C.byref(nv), C.byref(hn), None,
+                                           None, None, None))
+        pts = np.zeros((n.value, 3), np.float32)
+        nrm = np.zeros((n.value, 3), np.float32) if hn.value else None
+        valid = np.zeros(nv.value, np.int32)
+        box = np.zeros(6, np.float32)
+        N.check(N.lib().ppf_scene_download(self._h, None, None, None, N.ptr(pts), N.ptr(nrm),
+                                           N.ptr(valid), N.ptr(box)))
+        return pts, nrm, valid, box
+
     def inliers(self, poses, dense: bool, dist: float) -> np.ndarray:
         """inliner (util.cpp:628-641) of the sampled (dense=False) or
         resampled (dense=True) model at each pose (k x 4 x 4, row-major)

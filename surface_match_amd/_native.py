@@ -47,7 +47,7 @@ class MatchStats(C.Structure):
                                          "msCluster", "msIcpPrepare", "msIcp", "msTotal",
                                          "msVoteKernel", "msIcpSparse", "msIcpDense",
                                          "msPairKernel", "msHitsKernel")] + \
-               [("nVoteFallback", C.c_int)]
+               [("nVoteFallback", C.c_int), ("msIngest", C.c_float)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -103,7 +103,7 @@ EXPORTS = [
     "ppf_autotune", "ppf_set_tuning", "ppf_get_tuning", "ppf_match_scenes_resident",
     "ppf_scene_inliers", "ppf_measure_ceilings", "ppf_ply_read", "ppf_ply_info", "ppf_ply_copy",
     "ppf_ply_free", "ppf_ply_write", "ppf_scene_upload_ply", "ppf_train_model_mesh",
-    "ppf_sample_mesh", "ppf_estimate_normals_mls",
+    "ppf_sample_mesh", "ppf_estimate_normals_mls", "ppf_scene_download",
 ]
 
 
@@ -130,6 +130,8 @@ def lib() -> C.CDLL:
         "ppf_last_error": (C.c_char_p, []),
         "ppf_scene_upload": (i, [vp, vp, vp, sz, vp, vp, C.POINTER(vp)]),
         "ppf_scene_free": (None, [vp]),
+        "ppf_scene_download": (i, [vp, C.POINTER(sz), C.POINTER(sz), C.POINTER(i), vp, vp, vp,
+                                   vp]),
         "ppf_match_scene_resident": (i, [vp, vp, f, f, f, C.POINTER(MatchParam), vp, vp, sz,
                                          C.POINTER(sz), vp]),
         "ppf_model_get_info": (i, [vp, C.POINTER(ModelInfo)]),

@@ -16,7 +16,7 @@ CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "_lib")
 LIB = os.path.join(OUT_DIR, "libppf_amd.so")
 SOURCES = ["grid.hip", "table.hip", "vote.hip", "cluster.hip", "icp.hip", "detector.hip", "ply.hip",
-           "mesh.hip", "mls.hip"]
+           "mesh.hip", "mls.hip", "ingest.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("PPF_OFFLOAD_ARCH", "gfx950")
 

@@ -14,12 +14,15 @@
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <exception>
 #include <fstream>
 #include <memory>
+#include <mutex>
 #include <numeric>
+#include <shared_mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -141,7 +144,7 @@ __global__ void gather_aos_kernel(const int *idx, int m, const float *x, const f
 struct DevCloud {
     int    n = 0;
     bool   hasNormal = false;
-    DevBuf aos, ox, oy, oz, onx, ony, onz, px, py, pz, nx, ny, nz, valid;
+    DevBuf aos, aosN, ox, oy, oz, onx, ony, onz, px, py, pz, nx, ny, nz, valid;
     int    nValid = 0;
     float  boxMin[3], boxMax[3], vp[3];
     // bounding box of the valid points (min xyz, max xyz): every working
@@ -149,16 +152,30 @@ struct DevCloud {
     // centroids of valid points), so grids over them need no device box
     float  ext[6];
 
+    // the caller's AoS arrays to HBM, then removeNan + computeBoundingBox
+    // (util.cpp:86-105,178-207) on the device: SoA originals, the ascending
+    // valid indices and the box of the valid points (ppf.cpp:170-176)
     void upload(const float *pts, const float *nrm, int count, const float *box6,
-                const float *viewPoint, hipStream_t s) {
+                const float *viewPoint, hipStream_t s, IngestScratch &is) {
         n         = count;
         hasNormal = nrm != nullptr;
-        std::vector<int> v = remove_nan(pts, hasNormal ? nrm : nullptr, n);
-        nValid   = (int)v.size();
-        if (v.empty())
-            std::fill(ext, ext + 6, 0.0f);
-        else
-            bbox_host(pts, v, ext, ext + 3);
+        int    m  = std::max(n, 1);
+        float *A  = aos.as<float>((size_t)m * 3);
+        float *B  = hasNormal ? aosN.as<float>((size_t)m * 3) : nullptr;
+        PPF_HIP_CHECK(hipMemcpyAsync(A, pts, sizeof(float) * 3 * n, hipMemcpyHostToDevice, s));
+        if (hasNormal)
+            PPF_HIP_CHECK(hipMemcpyAsync(B, nrm, sizeof(float) * 3 * n, hipMemcpyHostToDevice, s));
+        float *X = ox.as<float>(m), *Y = oy.as<float>(m), *Z = oz.as<float>(m);
+        float *NX = onx.as<float>(m), *NY = ony.as<float>(m), *NZ = onz.as<float>(m);
+        px.as<float>(m);
+        py.as<float>(m);
+        pz.as<float>(m);
+        nx.as<float>(m);
+        ny.as<float>(m);
+        nz.as<float>(m);
+        nValid = ingest_cloud(is, A, B, n, X, Y, Z, hasNormal ? NX : nullptr,
+                              hasNormal ? NY : nullptr, hasNormal ? NZ : nullptr,
+                              valid.as<int>(m), ext, s);
         if (box6 && box_diameter(box6, box6 + 3) != 0.0f) {
             std::memcpy(boxMin, box6, sizeof boxMin);
             std::memcpy(boxMax, box6 + 3, sizeof boxMax);
@@ -168,28 +185,6 @@ struct DevCloud {
         }
         for (int a = 0; a < 3; a++)
             vp[a] = viewPoint ? viewPoint[a] : NAN;
-        int m = std::max(n, 1);
-        float *A = aos.as<float>((size_t)m * 3);
-        PPF_HIP_CHECK(hipMemcpyAsync(A, pts, sizeof(float) * 3 * n, hipMemcpyHostToDevice, s));
-        split_kernel<<<div_up(m, 256), 256, 0, s>>>(A, n, ox.as<float>(m), oy.as<float>(m),
-                                                     oz.as<float>(m));
-        onx.as<float>(m);
-        ony.as<float>(m);
-        onz.as<float>(m);
-        if (hasNormal) {
-            PPF_HIP_CHECK(hipMemcpyAsync(A, nrm, sizeof(float) * 3 * n, hipMemcpyHostToDevice, s));
-            split_kernel<<<div_up(m, 256), 256, 0, s>>>(A, n, onx.get<float>(), ony.get<float>(),
-                                                         onz.get<float>());
-        }
-        PPF_HIP_CHECK(hipMemcpyAsync(valid.as<int>(std::max(nValid, 1)), v.data(),
-                                     sizeof(int) * nValid, hipMemcpyHostToDevice, s));
-        px.as<float>(m);
-        py.as<float>(m);
-        pz.as<float>(m);
-        nx.as<float>(m);
-        ny.as<float>(m);
-        nz.as<float>(m);
-        PPF_HIP_CHECK(hipStreamSynchronize(s));
     }
 
     // working copies := originals
@@ -215,7 +210,8 @@ struct DevCloud {
 }  // namespace
 
 struct ppf_scene_s {
-    DevCloud cloud;
+    DevCloud   cloud;
+    std::mutex busy;  // a match moves the scene's working points: one at a time
 };
 
 struct ppf_icp_target_s {
@@ -235,20 +231,18 @@ struct ppf_icp_target_s {
     }
 };
 
-struct ppf_detector_s {
-    hipStream_t  stream = nullptr;
-    bool         trained = false;
-    HostModel    hm;
-    DevModel     dm;
-    Workspace    ws;
-    TrainScratch ts;
-    VoteConfig   vcfg;
-    IcpConfig    icfg;
-    int          normCell4 = 6;  // scene normals grid cell = sample step * normCell4 / 4
-    int          refTail = 1;
-    DevCloud     tmpScene;
-    DevBuf       idxA, idxB, idxC, idxD;  // sampled / keypoints / dense / scratch
-    hipEvent_t   ev[13] = {};
+// Everything one match call writes: a stream, scratch, index buffers,
+// events, the host-array scene copy and the helper lanes.  A detector keeps
+// a pool of these; each call takes one for its duration, so calls on one
+// detector run concurrently (the reference's matchScene only reads the
+// trained model, ppf.cpp:149-510).
+struct MatchCtx {
+    hipStream_t   stream = nullptr;
+    Workspace     ws;
+    IngestScratch ingest;
+    DevCloud      tmpScene;
+    DevBuf        idxA, idxB, idxC, idxD;  // sampled / keypoints / dense / scratch
+    hipEvent_t    ev[13] = {};
     // per-scene state of a batched match (grown on demand, reused)
     struct SceneWork {
         DevBuf samp, kp, dense;
@@ -275,12 +269,12 @@ struct ppf_detector_s {
     };
     std::vector<std::unique_ptr<Lane>> lanes;
 
-    ppf_detector_s() {
+    MatchCtx() {
         PPF_HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
         for (auto &e : ev)
             PPF_HIP_CHECK(hipEventCreate(&e));
     }
-    ~ppf_detector_s() {
+    ~MatchCtx() {
         lanes.clear();
         for (auto &e : ev)
             if (e)
@@ -290,7 +284,50 @@ struct ppf_detector_s {
     }
 };
 
+// The trained model and the launch shapes (read by every call, written by
+// train / load / set_tuning / autotune under the exclusive lock) and the
+// pool of call contexts.
+struct ppf_detector_s {
+    bool         trained = false;
+    HostModel    hm;
+    DevModel     dm;
+    TrainScratch ts;
+    VoteConfig   vcfg;
+    IcpConfig    icfg;
+    int          normCell4 = 6;  // scene normals grid cell = sample step * normCell4 / 4
+    int          refTail = 1;
+    std::shared_mutex                      model;   // shared: readers, unique: writers
+    std::mutex                             poolMu;  // guards the two lists below
+    std::vector<std::unique_ptr<MatchCtx>> ctxs;
+    std::vector<MatchCtx *>                freeCtx;
+};
+
 namespace {
+
+// a call context from the detector's pool for the duration of one call
+struct Lease {
+    ppf_detector_s &D;
+    MatchCtx       *X = nullptr;
+    explicit Lease(ppf_detector_s &d) : D(d) {
+        std::lock_guard<std::mutex> g(D.poolMu);
+        if (D.freeCtx.empty()) {
+            D.ctxs.push_back(std::make_unique<MatchCtx>());
+            X = D.ctxs.back().get();
+        } else {
+            X = D.freeCtx.back();
+            D.freeCtx.pop_back();
+        }
+    }
+    Lease(const Lease &) = delete;
+    Lease &operator=(const Lease &) = delete;
+    ~Lease() {
+        std::lock_guard<std::mutex> g(D.poolMu);
+        D.freeCtx.push_back(X);
+    }
+    MatchCtx &operator*() const { return *X; }
+};
+using ReadLock  = std::shared_lock<std::shared_mutex>;
+using WriteLock = std::unique_lock<std::shared_mutex>;
 
 // cells of the refinement's target grids, in units of the sample step
 // (sparse stage) and of the resample step (dense stage): measured on the
@@ -352,7 +389,7 @@ void download_aos(const int *idxDev, int m, const DevCloud &c, bool normals, std
     PPF_HIP_CHECK(hipStreamSynchronize(s));
 }
 
-void train_impl(ppf_detector_s &D, const float *pts, const float *nrm, int n, const float *box6,
+void train_impl(ppf_detector_s &D, MatchCtx &X, const float *pts, const float *nrm, int n, const float *box6,
                 const float *vp, float rel, const ppf_train_param &param) {
     // ppf.cpp:39-44
     if (rel > 1 || rel < 0)
@@ -360,28 +397,28 @@ void train_impl(ppf_detector_s &D, const float *pts, const float *nrm, int n, co
     if (n <= 0 || !pts)
         throw std::runtime_error("Invalid Input: empty model in trainModel");
     check_train_param(param);
-    hipStream_t s = D.stream;
-    DevCloud   &c = D.tmpScene;
-    c.upload(pts, nrm, n, box6, vp, s);
+    hipStream_t s = X.stream;
+    DevCloud   &c = X.tmpScene;
+    c.upload(pts, nrm, n, box6, vp, s, X.ingest);
     c.reset(s);
     float Dm           = box_diameter(c.boxMin, c.boxMax);             // ppf.cpp:54
     float sampleStep   = Dm * rel;                                     // :56
     float reSampleStep = Dm * param.poseRefRelSamplingDistance;        // :57
     float distanceStep = Dm * param.featDistanceStepRel;               // :58
     float angleStep    = ppfc::kTwoPi / (float)param.featAngleResolution;  // :59
-    int  *cnt          = D.ws.smallCnt.as<int>(8);
-    int  *s1 = D.idxA.as<int>(std::max(c.nValid, 1)), *s2 = D.idxB.as<int>(std::max(c.nValid, 1));
-    int   n1 = voxel_sample(D.ws, c.px.get<float>(), c.py.get<float>(), c.pz.get<float>(),
+    int  *cnt          = X.ws.smallCnt.as<int>(8);
+    int  *s1 = X.idxA.as<int>(std::max(c.nValid, 1)), *s2 = X.idxB.as<int>(std::max(c.nValid, 1));
+    int   n1 = voxel_sample(X.ws, c.px.get<float>(), c.py.get<float>(), c.pz.get<float>(),
                             c.valid.get<int>(), c.nValid, sampleStep, c.boxMin, s1, cnt, s);
-    int   n2 = voxel_sample(D.ws, c.px.get<float>(), c.py.get<float>(), c.pz.get<float>(),
+    int   n2 = voxel_sample(X.ws, c.px.get<float>(), c.py.get<float>(), c.pz.get<float>(),
                             c.valid.get<int>(), c.nValid, reSampleStep, c.boxMin, s2, cnt, s);
     if (!c.hasNormal) {
         float cellHint = sampleStep * 0.5f;
-        estimate_normals(D.ws, n, c.px.get<float>(), c.py.get<float>(), c.pz.get<float>(),
+        estimate_normals(X.ws, n, c.px.get<float>(), c.py.get<float>(), c.pz.get<float>(),
                          c.nx.get<float>(), c.ny.get<float>(), c.nz.get<float>(),
                          c.valid.get<int>(), c.nValid, s1, n1, param.knnNormal, param.smoothNormal,
                          0, c.vp, cellHint, s);
-        estimate_normals(D.ws, n, c.px.get<float>(), c.py.get<float>(), c.pz.get<float>(),
+        estimate_normals(X.ws, n, c.px.get<float>(), c.py.get<float>(), c.pz.get<float>(),
                          c.nx.get<float>(), c.ny.get<float>(), c.nz.get<float>(),
                          c.valid.get<int>(), c.nValid, s2, n2, param.knnNormal, param.smoothNormal,
                          0, c.vp, cellHint, s);
@@ -418,20 +455,20 @@ void train_impl(ppf_detector_s &D, const float *pts, const float *nrm, int n, co
     D.trained = true;
 }
 
-void upload_model(ppf_detector_s &D) {
+void upload_model(ppf_detector_s &D, MatchCtx &X) {
     HostModel &hm = D.hm;
     float      Dm = box_diameter(hm.sampled.boxMin, hm.sampled.boxMax);  // ppf.cpp:183
     model_upload(D.dm, hm.sampled.size(), hm.sampled.p.data(), hm.sampled.n.data(),
                  hm.reSampled.size(), hm.reSampled.p.data(), hm.reSampled.n.data(), hm.keys,
                  hm.offs, hm.refs, hm.alphas, hm.param.featAngleResolution, Dm,
-                 hm.param.featDistanceStepRel, D.stream);
+                 hm.param.featDistanceStepRel, X.stream);
 }
 
 struct MatchOut {
     std::vector<std::pair<float, std::vector<float>>> results;  // score, pose 3x4
 };
 
-void write_result_cloud(ppf_detector_s &D, DevCloud &c, const int *idxDev, int m, float *pOut,
+void write_result_cloud(MatchCtx &X, DevCloud &c, const int *idxDev, int m, float *pOut,
                         float *nOut, size_t cap, size_t &countOut) {
     countOut = (size_t)m;
     size_t w = std::min((size_t)m, cap);
@@ -440,11 +477,11 @@ void write_result_cloud(ppf_detector_s &D, DevCloud &c, const int *idxDev, int m
     std::vector<float> tmp;
     DevBuf             scratch;
     if (pOut) {
-        download_aos(idxDev, (int)w, c, false, tmp, scratch, D.stream);
+        download_aos(idxDev, (int)w, c, false, tmp, scratch, X.stream);
         std::memcpy(pOut, tmp.data(), sizeof(float) * 3 * w);
     }
     if (nOut) {
-        download_aos(idxDev, (int)w, c, true, tmp, scratch, D.stream);
+        download_aos(idxDev, (int)w, c, true, tmp, scratch, X.stream);
         for (size_t i = 0; i < w; i++)
             normalize3(&tmp[3 * i]);
         std::memcpy(nOut, tmp.data(), sizeof(float) * 3 * w);
@@ -452,15 +489,15 @@ void write_result_cloud(ppf_detector_s &D, DevCloud &c, const int *idxDev, int m
 }
 
 // the detector's lane l (created on first use)
-ppf_detector_s::Lane &lane_of(ppf_detector_s &D, int l) {
-    while ((int)D.lanes.size() <= l) {
-        auto ln = std::make_unique<ppf_detector_s::Lane>();
+MatchCtx::Lane &lane_of(MatchCtx &X, int l) {
+    while ((int)X.lanes.size() <= l) {
+        auto ln = std::make_unique<MatchCtx::Lane>();
         PPF_HIP_CHECK(hipStreamCreateWithFlags(&ln->stream, hipStreamNonBlocking));
         for (auto &e : ln->ev)
             PPF_HIP_CHECK(hipEventCreate(&e));
-        D.lanes.push_back(std::move(ln));
+        X.lanes.push_back(std::move(ln));
     }
-    return *D.lanes[l];
+    return *X.lanes[l];
 }
 
 // the dense prep waits for the vote's pair kernel
@@ -486,7 +523,7 @@ struct JoinGuard {
     }
 };
 
-void match_impl(ppf_detector_s &D, DevCloud &c, float rel, float kpFrac, float minScore,
+void match_impl(ppf_detector_s &D, MatchCtx &X, DevCloud &c, float rel, float kpFrac, float minScore,
                 const ppf_match_param &p, MatchOut &out, ppf_match_result *mr,
                 ppf_match_stats *st) {
     // ppf.cpp:155-168
@@ -501,14 +538,14 @@ void match_impl(ppf_detector_s &D, DevCloud &c, float rel, float kpFrac, float m
     if (c.n <= 0)
         throw std::runtime_error("Invalid Input: empty scene in matchScene");
     check_match_param(p);
-    hipStream_t s   = D.stream;
+    hipStream_t s   = X.stream;
     HostModel  &hm  = D.hm;
     DevModel   &dm  = D.dm;
     ppf_match_stats stl{};
     if (!st)
         st = &stl;
     std::memset(st, 0, sizeof *st);
-    PPF_HIP_CHECK(hipEventRecord(D.ev[0], s));
+    PPF_HIP_CHECK(hipEventRecord(X.ev[0], s));
     c.reset(s);
     // ppf.cpp:183-193
     const float Dm           = box_diameter(hm.sampled.boxMin, hm.sampled.boxMax);
@@ -534,29 +571,29 @@ void match_impl(ppf_detector_s &D, DevCloud &c, float rel, float kpFrac, float m
 
     float *PX = c.px.get<float>(), *PY = c.py.get<float>(), *PZ = c.pz.get<float>();
     float *NX = c.nx.get<float>(), *NY = c.ny.get<float>(), *NZ = c.nz.get<float>();
-    int   *cnt = D.ws.smallCnt.as<int>(8);
-    int   *samp = D.idxA.as<int>(std::max(c.nValid, 1));
+    int   *cnt = X.ws.smallCnt.as<int>(8);
+    int   *samp = X.idxA.as<int>(std::max(c.nValid, 1));
     // scene sample1 (ppf.cpp:199-203)
-    int ns = voxel_sample(D.ws, PX, PY, PZ, c.valid.get<int>(), c.nValid, sampleStep, c.boxMin,
+    int ns = voxel_sample(X.ws, PX, PY, PZ, c.valid.get<int>(), c.nValid, sampleStep, c.boxMin,
                           samp, cnt, s);
-    PPF_HIP_CHECK(hipEventRecord(D.ev[1], s));
+    PPF_HIP_CHECK(hipEventRecord(X.ev[1], s));
     // normals (ppf.cpp:204-211)
     if (!c.hasNormal)
-        estimate_normals(D.ws, c.n, PX, PY, PZ, NX, NY, NZ, c.valid.get<int>(), c.nValid, samp, ns,
+        estimate_normals(X.ws, c.n, PX, PY, PZ, NX, NY, NZ, c.valid.get<int>(), c.nValid, samp, ns,
                          p.knnNormal, p.smoothNormal, p.invertNormal, c.vp, normCell, s, c.ext);
-    PPF_HIP_CHECK(hipEventRecord(D.ev[2], s));
+    PPF_HIP_CHECK(hipEventRecord(X.ev[2], s));
     // key points (ppf.cpp:212-216)
     const float keySampleStep = std::sqrt(1.0f / kpFrac) * sampleStep;
-    int *kp = D.idxB.as<int>(std::max(ns, 1));
-    int  nk = voxel_sample(D.ws, PX, PY, PZ, samp, ns, keySampleStep, c.boxMin, kp, cnt, s);
-    PPF_HIP_CHECK(hipEventRecord(D.ev[3], s));
+    int *kp = X.idxB.as<int>(std::max(ns, 1));
+    int  nk = voxel_sample(X.ws, PX, PY, PZ, samp, ns, keySampleStep, c.boxMin, kp, cnt, s);
+    PPF_HIP_CHECK(hipEventRecord(X.ev[3], s));
     if (mr) {  // ppf.cpp:242-245
-        write_result_cloud(D, c, kp, nk, mr->keyPoints, mr->keyNormals, mr->keyCap, mr->nKey);
-        write_result_cloud(D, c, samp, ns, mr->sampledPoints, mr->sampledNormals, mr->sampledCap,
+        write_result_cloud(X, c, kp, nk, mr->keyPoints, mr->keyNormals, mr->keyCap, mr->nKey);
+        write_result_cloud(X, c, samp, ns, mr->sampledPoints, mr->sampledNormals, mr->sampledCap,
                            mr->nSampled);
     }
     // vote (ppf.cpp:247-411)
-    grid_build(D.ws.gVote, PX, PY, PZ, NX, NY, NZ, samp, ns, Dm / (float)D.vcfg.cellDiv, s, c.ext);
+    grid_build(X.ws.gVote, PX, PY, PZ, NX, NY, NZ, samp, ns, Dm / (float)D.vcfg.cellDiv, s, c.ext);
     // dense prep (ppf.cpp:427-434) on a lane, overlapping the vote and the
     // clustering: it samples the scene again and estimates normals only for
     // points without one (so never a sampled point, which all have one by
@@ -564,7 +601,7 @@ void match_impl(ppf_detector_s &D, DevCloud &c, float rel, float kpFrac, float m
     // own copies and the key points, all sampled points.  The refinement's
     // grids (their own buffers, over the final positions) follow on the
     // same lane.
-    int        *dense = D.idxC.as<int>(std::max(c.nValid, 1));
+    int        *dense = X.idxC.as<int>(std::max(c.nValid, 1));
     int         nd    = 0;
     float       prepMs = 0.0f;
     int         np    = 0;
@@ -573,13 +610,13 @@ void match_impl(ppf_detector_s &D, DevCloud &c, float rel, float kpFrac, float m
     JoinGuard   prepJoin{prep};
     // released after the vote, or on the way out if anything throws first
     // (before the join above)
-    PairRelease pairRelease{D.ws.pairQueued};
+    PairRelease pairRelease{X.ws.pairQueued};
     const bool laneGrids = p.sparsePoseRefinement != 0;
     if (p.densePoseRefinement || laneGrids) {
-        ppf_detector_s::Lane &ln = lane_of(D, 0);
-        PPF_HIP_CHECK(hipEventRecord(D.ev[12], s));
-        PPF_HIP_CHECK(hipStreamWaitEvent(ln.stream, D.ev[12], 0));
-        D.ws.pairQueued.store(0, std::memory_order_relaxed);
+        MatchCtx::Lane &ln = lane_of(X, 0);
+        PPF_HIP_CHECK(hipEventRecord(X.ev[12], s));
+        PPF_HIP_CHECK(hipStreamWaitEvent(ln.stream, X.ev[12], 0));
+        X.ws.pairQueued.store(0, std::memory_order_relaxed);
         int dev = 0;
         PPF_HIP_CHECK(hipGetDevice(&dev));
         prep = std::thread([&, dev] {
@@ -589,10 +626,10 @@ void match_impl(ppf_detector_s &D, DevCloud &c, float rel, float kpFrac, float m
                 // the LDS-bound hit voting and the clustering
                 if (PREP_AFTER_PAIRS) {
                     int q;
-                    while ((q = D.ws.pairQueued.load(std::memory_order_acquire)) == 0)
+                    while ((q = X.ws.pairQueued.load(std::memory_order_acquire)) == 0)
                         std::this_thread::yield();
                     if (q == 1)
-                        PPF_HIP_CHECK(hipStreamWaitEvent(ln.stream, D.ws.voteEv[2], 0));
+                        PPF_HIP_CHECK(hipStreamWaitEvent(ln.stream, X.ws.voteEv[2], 0));
                 }
                 PPF_HIP_CHECK(hipEventRecord(ln.ev[0], ln.stream));
                 if (p.densePoseRefinement) {
@@ -605,13 +642,13 @@ void match_impl(ppf_detector_s &D, DevCloud &c, float rel, float kpFrac, float m
                                          c.vp, denseNormCell, ln.stream, c.ext);
                 }
                 if (laneGrids) {
-                    grid_build(D.ws.gSparse, PX, PY, PZ, NX, NY, NZ, samp, ns,
+                    grid_build(X.ws.gSparse, PX, PY, PZ, NX, NY, NZ, samp, ns,
                                sampleStep * kSparseCell, ln.stream, c.ext);
-                    grid_build(D.ws.gFull, PX, PY, PZ, nullptr, nullptr, nullptr,
+                    grid_build(X.ws.gFull, PX, PY, PZ, nullptr, nullptr, nullptr,
                                c.valid.get<int>(), c.nValid,
                                std::max(scoringDist * 2.0f, sampleStep * 0.25f), ln.stream, c.ext);
                     if (p.densePoseRefinement)
-                        grid_build(D.ws.gDense, PX, PY, PZ, NX, NY, NZ, dense, nd,
+                        grid_build(X.ws.gDense, PX, PY, PZ, NX, NY, NZ, dense, nd,
                                    reSampleStep * kDenseCell, ln.stream, c.ext);
                 }
                 PPF_HIP_CHECK(hipEventRecord(ln.ev[1], ln.stream));
@@ -623,26 +660,26 @@ void match_impl(ppf_detector_s &D, DevCloud &c, float rel, float kpFrac, float m
         });
     }
     VoteStats vs{};
-    PPF_HIP_CHECK(hipEventRecord(D.ev[4], s));
-    np = vote_and_poses(D.ws, dm, D.ws.gVote.view, PX, PY, PZ, NX, NY, NZ, kp, nk,
+    PPF_HIP_CHECK(hipEventRecord(X.ev[4], s));
+    np = vote_and_poses(X.ws, dm, X.ws.gVote.view, PX, PY, PZ, NX, NY, NZ, kp, nk,
                         voteThreshold, D.vcfg, &vs, nullptr, 0, s);
     pairRelease();
-    PPF_HIP_CHECK(hipEventRecord(D.ev[5], s));
+    PPF_HIP_CHECK(hipEventRecord(X.ev[5], s));
     // cluster (ppf.cpp:413-420)
     std::vector<Hypo> hyp;
     float center[3];
     for (int a = 0; a < 3; a++)
         center[a] = (hm.sampled.boxMin[a] + hm.sampled.boxMax[a]) / 2.0f;
     int nClusters = 0;
-    cluster_poses(D.ws, D.ws.poses.get<PoseRec>(), np, 0.1f * Dm, angleStep, center,
+    cluster_poses(X.ws, X.ws.poses.get<PoseRec>(), np, 0.1f * Dm, angleStep, center,
                   maxOverlapDist, hyp, &nClusters, s);
-    PPF_HIP_CHECK(hipEventRecord(D.ev[6], s));
+    PPF_HIP_CHECK(hipEventRecord(X.ev[6], s));
     // the dense prep has finished before the refinement reads the scene
     if (prep.joinable())
         prep.join();
     if (prepErr)
         std::rethrow_exception(prepErr);
-    PPF_HIP_CHECK(hipEventRecord(D.ev[7], s));
+    PPF_HIP_CHECK(hipEventRecord(X.ev[7], s));
     // icp (ppf.cpp:436-495)
     std::vector<int> cand;
     for (int i = 0; i < (int)hyp.size(); i++)
@@ -664,12 +701,12 @@ void match_impl(ppf_detector_s &D, DevCloud &c, float rel, float kpFrac, float m
         std::vector<IcpResultDev> res(cand.size());
         for (size_t k = 0; k < cand.size(); k++)
             std::memcpy(&init[12 * k], hyp[cand[k]].pose, sizeof(float) * 12);
-        PPF_HIP_CHECK(hipEventRecord(D.ev[9], s));
-        icp_batch(D.ws, refNum, dm.sx.get<float>(), dm.sy.get<float>(), dm.sz.get<float>(),
+        PPF_HIP_CHECK(hipEventRecord(X.ev[9], s));
+        icp_batch(X.ws, refNum, dm.sx.get<float>(), dm.sy.get<float>(), dm.sz.get<float>(),
                   dm.snx.get<float>(), dm.sny.get<float>(), dm.snz.get<float>(),
-                  D.ws.gSparse.view, D.ws.gFull.view, init.data(), (int)cand.size(), sc,
+                  X.ws.gSparse.view, X.ws.gFull.view, init.data(), (int)cand.size(), sc,
                   scoringDist, D.icfg, res.data(), s);
-        PPF_HIP_CHECK(hipEventRecord(D.ev[10], s));
+        PPF_HIP_CHECK(hipEventRecord(X.ev[10], s));
         nSparse = (int)cand.size();
         std::vector<int> survivors;
         for (size_t k = 0; k < cand.size(); k++) {
@@ -697,12 +734,12 @@ void match_impl(ppf_detector_s &D, DevCloud &c, float rel, float kpFrac, float m
             for (size_t k = 0; k < survivors.size(); k++)
                 std::memcpy(&init2[12 * k], &poseNow[12 * (size_t)survivors[k]], sizeof(float) * 12);
             IcpConfig dcfg = D.icfg;
-            PPF_HIP_CHECK(hipEventRecord(D.ev[11], s));
-            icp_batch(D.ws, hm.reSampled.size(), dm.rx.get<float>(), dm.ry.get<float>(),
+            PPF_HIP_CHECK(hipEventRecord(X.ev[11], s));
+            icp_batch(X.ws, hm.reSampled.size(), dm.rx.get<float>(), dm.ry.get<float>(),
                       dm.rz.get<float>(), dm.rnx.get<float>(), dm.rny.get<float>(),
-                      dm.rnz.get<float>(), D.ws.gDense.view, D.ws.gFull.view, init2.data(),
+                      dm.rnz.get<float>(), X.ws.gDense.view, X.ws.gFull.view, init2.data(),
                       (int)survivors.size(), dc, scoringDist, dcfg, res2.data(), s);
-            PPF_HIP_CHECK(hipEventRecord(D.ev[12], s));
+            PPF_HIP_CHECK(hipEventRecord(X.ev[12], s));
             nDenseIcp = (int)survivors.size();
             for (size_t k = 0; k < survivors.size(); k++) {
                 int i = survivors[k];
@@ -733,8 +770,8 @@ void match_impl(ppf_detector_s &D, DevCloud &c, float rel, float kpFrac, float m
     // ppf.cpp:498-499 (stable on ties)
     std::stable_sort(out.results.begin(), out.results.end(),
                      [](const auto &a, const auto &b) { return a.first > b.first; });
-    PPF_HIP_CHECK(hipEventRecord(D.ev[8], s));
-    PPF_HIP_CHECK(hipEventSynchronize(D.ev[8]));
+    PPF_HIP_CHECK(hipEventRecord(X.ev[8], s));
+    PPF_HIP_CHECK(hipEventSynchronize(X.ev[8]));
     st->nScene          = c.n;
     st->nValid          = c.nValid;
     st->nSampled        = ns;
@@ -750,20 +787,20 @@ void match_impl(ppf_detector_s &D, DevCloud &c, float rel, float kpFrac, float m
     st->nPairs          = vs.pairs;
     st->nKeyHits        = vs.hits;
     st->nVotes          = vs.votes;
-    st->msSample        = elapsed(D.ev[0], D.ev[1]);
-    st->msNormals       = elapsed(D.ev[1], D.ev[2]);
-    st->msKeypoints     = elapsed(D.ev[2], D.ev[3]);
-    st->msVote          = elapsed(D.ev[3], D.ev[5]);
+    st->msSample        = elapsed(X.ev[0], X.ev[1]);
+    st->msNormals       = elapsed(X.ev[1], X.ev[2]);
+    st->msKeypoints     = elapsed(X.ev[2], X.ev[3]);
+    st->msVote          = elapsed(X.ev[3], X.ev[5]);
     st->msVoteKernel    = vs.kernelMs;
     st->msPairKernel    = vs.pairMs;
     st->msHitsKernel    = vs.hitsMs;
     st->nVoteFallback   = vs.overflow;
-    st->msCluster       = elapsed(D.ev[5], D.ev[6]);
+    st->msCluster       = elapsed(X.ev[5], X.ev[6]);
     st->msIcpPrepare    = prepMs;  // overlapped with the vote and the clustering
-    st->msIcp           = elapsed(D.ev[7], D.ev[8]);
-    st->msTotal         = elapsed(D.ev[0], D.ev[8]);
-    st->msIcpSparse     = nSparse ? elapsed(D.ev[9], D.ev[10]) : 0.0f;
-    st->msIcpDense      = nDenseIcp ? elapsed(D.ev[11], D.ev[12]) : 0.0f;
+    st->msIcp           = elapsed(X.ev[7], X.ev[8]);
+    st->msTotal         = elapsed(X.ev[0], X.ev[8]);
+    st->msIcpSparse     = nSparse ? elapsed(X.ev[9], X.ev[10]) : 0.0f;
+    st->msIcpDense      = nDenseIcp ? elapsed(X.ev[11], X.ev[12]) : 0.0f;
 }
 
 // Several scenes matched together (same parameters): each scene is sampled,
@@ -777,16 +814,16 @@ void match_impl(ppf_detector_s &D, DevCloud &c, float rel, float kpFrac, float m
 // Work queued on the detector's stream before the call completes first.
 // Each scene's work is independent of the lane it lands on.
 template <class F>
-void for_scenes(ppf_detector_s &D, int n, F &&f) {
-    PPF_HIP_CHECK(hipStreamSynchronize(D.stream));
-    const int L = std::min(n, ppf_detector_s::kLanes);
+void for_scenes(MatchCtx &X, int n, F &&f) {
+    PPF_HIP_CHECK(hipStreamSynchronize(X.stream));
+    const int L = std::min(n, MatchCtx::kLanes);
     if (L <= 1) {
         for (int i = 0; i < n; i++)
-            f(i, D.ws, D.stream);
-        PPF_HIP_CHECK(hipStreamSynchronize(D.stream));
+            f(i, X.ws, X.stream);
+        PPF_HIP_CHECK(hipStreamSynchronize(X.stream));
         return;
     }
-    lane_of(D, L - 1);
+    lane_of(X, L - 1);
     int dev = 0;
     PPF_HIP_CHECK(hipGetDevice(&dev));
     std::atomic<int>                next{0};
@@ -796,7 +833,7 @@ void for_scenes(ppf_detector_s &D, int n, F &&f) {
         th.emplace_back([&, l] {
             try {
                 PPF_HIP_CHECK(hipSetDevice(dev));
-                ppf_detector_s::Lane &ln = *D.lanes[l];
+                MatchCtx::Lane &ln = *X.lanes[l];
                 for (int i = next++; i < n; i = next++)
                     f(i, ln.ws, ln.stream);
                 PPF_HIP_CHECK(hipStreamSynchronize(ln.stream));
@@ -811,7 +848,7 @@ void for_scenes(ppf_detector_s &D, int n, F &&f) {
             std::rethrow_exception(e);
 }
 
-void match_batch_impl(ppf_detector_s &D, const std::vector<DevCloud *> &clouds, float rel,
+void match_batch_impl(ppf_detector_s &D, MatchCtx &X, const std::vector<DevCloud *> &clouds, float rel,
                       float kpFrac, float minScore, const ppf_match_param &p,
                       std::vector<MatchOut> &outs, ppf_match_stats *st) {
     if (!D.trained)
@@ -827,7 +864,7 @@ void match_batch_impl(ppf_detector_s &D, const std::vector<DevCloud *> &clouds, 
             throw std::runtime_error("Invalid Input: empty scene in matchScene");
     check_match_param(p);
     const int   n  = (int)clouds.size();
-    hipStream_t s  = D.stream;
+    hipStream_t s  = X.stream;
     HostModel  &hm = D.hm;
     DevModel   &dm = D.dm;
     ppf_match_stats stl{};
@@ -835,8 +872,8 @@ void match_batch_impl(ppf_detector_s &D, const std::vector<DevCloud *> &clouds, 
         st = &stl;
     std::memset(st, 0, sizeof *st);
     outs.assign(n, MatchOut{});
-    while ((int)D.sceneWork.size() < n)
-        D.sceneWork.emplace_back(new ppf_detector_s::SceneWork());
+    while ((int)X.sceneWork.size() < n)
+        X.sceneWork.emplace_back(new MatchCtx::SceneWork());
     // ppf.cpp:183-240 (the same derived parameters for every scene)
     const float Dm           = box_diameter(hm.sampled.boxMin, hm.sampled.boxMax);
     const float angleStep    = ppfc::kTwoPi / (float)hm.param.featAngleResolution;
@@ -859,11 +896,11 @@ void match_batch_impl(ppf_detector_s &D, const std::vector<DevCloud *> &clouds, 
     if (p.poseRefScoringDistAbs > 0) scoringDist = p.poseRefScoringDistAbs;
     const float keySampleStep = std::sqrt(1.0f / kpFrac) * sampleStep;
 
-    PPF_HIP_CHECK(hipEventRecord(D.ev[0], s));
+    PPF_HIP_CHECK(hipEventRecord(X.ev[0], s));
     // ---- per scene: sampling, normals, key points, vote grid
-    for_scenes(D, n, [&](int i, Workspace &ws, hipStream_t ls) {
+    for_scenes(X, n, [&](int i, Workspace &ws, hipStream_t ls) {
         DevCloud &c  = *clouds[i];
-        auto     &sw = *D.sceneWork[i];
+        auto     &sw = *X.sceneWork[i];
         int      *cn = ws.smallCnt.as<int>(8);
         c.reset(ls);
         float *PX = c.px.get<float>(), *PY = c.py.get<float>(), *PZ = c.pz.get<float>();
@@ -883,24 +920,24 @@ void match_batch_impl(ppf_detector_s &D, const std::vector<DevCloud *> &clouds, 
     std::vector<int> ko(n + 1, 0);
     float            minCell = FLT_MAX;
     for (int i = 0; i < n; i++) {
-        minCell   = std::min(minCell, D.sceneWork[i]->gVote.view.cell);
-        ko[i + 1] = ko[i] + D.sceneWork[i]->nk;
+        minCell   = std::min(minCell, X.sceneWork[i]->gVote.view.cell);
+        ko[i + 1] = ko[i] + X.sceneWork[i]->nk;
     }
-    PPF_HIP_CHECK(hipEventRecord(D.ev[3], s));
+    PPF_HIP_CHECK(hipEventRecord(X.ev[3], s));
     // ---- every scene's key points in one vote
     const int    K = ko[n];
     VoteStats    vs{};
     int          np = 0;
     std::vector<int> pOff(n + 1, 0);
     if (K > 0) {
-        float *kx = D.ws.kpX.as<float>(K), *ky = D.ws.kpY.as<float>(K), *kz = D.ws.kpZ.as<float>(K);
-        float *knx = D.ws.kpNX.as<float>(K), *kny = D.ws.kpNY.as<float>(K),
-              *knz = D.ws.kpNZ.as<float>(K);
-        int   *kpScene = D.batchKpScene.as<int>(K), *kpId = D.batchKpId.as<int>(K);
+        float *kx = X.ws.kpX.as<float>(K), *ky = X.ws.kpY.as<float>(K), *kz = X.ws.kpZ.as<float>(K);
+        float *knx = X.ws.kpNX.as<float>(K), *kny = X.ws.kpNY.as<float>(K),
+              *knz = X.ws.kpNZ.as<float>(K);
+        int   *kpScene = X.batchKpScene.as<int>(K), *kpId = X.batchKpId.as<int>(K);
         std::vector<GridView> views(n);
         for (int i = 0; i < n; i++) {
             DevCloud &c  = *clouds[i];
-            auto     &sw = *D.sceneWork[i];
+            auto     &sw = *X.sceneWork[i];
             views[i]     = sw.gVote.view;
             if (sw.nk == 0)
                 continue;
@@ -913,7 +950,7 @@ void match_batch_impl(ppf_detector_s &D, const std::vector<DevCloud *> &clouds, 
             PPF_HIP_CHECK(hipMemcpyAsync(kpId + o, sw.kp.get<int>(), sizeof(int) * sw.nk,
                                          hipMemcpyDeviceToDevice, s));
         }
-        GridView *gv = D.batchGrids.as<GridView>(n);
+        GridView *gv = X.batchGrids.as<GridView>(n);
         PPF_HIP_CHECK(hipMemcpyAsync(gv, views.data(), sizeof(GridView) * n,
                                      hipMemcpyHostToDevice, s));
         VoteKeypoints in;
@@ -930,31 +967,31 @@ void match_batch_impl(ppf_detector_s &D, const std::vector<DevCloud *> &clouds, 
         in.kp  = kpId;
         in.nk  = K;
         in.nScenes = n;
-        np = vote_keypoints(D.ws, dm, in, voteThreshold, D.vcfg, &vs, nullptr, 0, s);
+        np = vote_keypoints(X.ws, dm, in, voteThreshold, D.vcfg, &vs, nullptr, 0, s);
         // poses are compacted in key point order: scene i's are those of its key points
         std::vector<int> offs(K + 1);
-        PPF_HIP_CHECK(hipMemcpyAsync(offs.data(), D.ws.voteCnt.get<int>() + K + 1,
+        PPF_HIP_CHECK(hipMemcpyAsync(offs.data(), X.ws.voteCnt.get<int>() + K + 1,
                                      sizeof(int) * (K + 1), hipMemcpyDeviceToHost, s));
         PPF_HIP_CHECK(hipStreamSynchronize(s));
         for (int i = 0; i <= n; i++)
             pOff[i] = offs[ko[i]];
     }
-    PPF_HIP_CHECK(hipEventRecord(D.ev[5], s));
+    PPF_HIP_CHECK(hipEventRecord(X.ev[5], s));
     // ---- clustering per scene (ppf.cpp:413-420)
     float center[3];
     for (int a = 0; a < 3; a++)
         center[a] = (hm.sampled.boxMin[a] + hm.sampled.boxMax[a]) / 2.0f;
     std::vector<std::vector<Hypo>> hyp(n);
     std::vector<int>               ncs(n, 0);
-    const PoseRec                 *posesAll = D.ws.poses.get<PoseRec>();
-    for_scenes(D, n, [&](int i, Workspace &ws, hipStream_t ls) {
+    const PoseRec                 *posesAll = X.ws.poses.get<PoseRec>();
+    for_scenes(X, n, [&](int i, Workspace &ws, hipStream_t ls) {
         cluster_poses(ws, posesAll + pOff[i], pOff[i + 1] - pOff[i], 0.1f * Dm, angleStep, center,
                       maxOverlapDist, hyp[i], &ncs[i], ls);
     });
     int nClustersTot = 0;
     for (int i = 0; i < n; i++)
         nClustersTot += ncs[i];
-    PPF_HIP_CHECK(hipEventRecord(D.ev[6], s));
+    PPF_HIP_CHECK(hipEventRecord(X.ev[6], s));
     // ---- refinement (ppf.cpp:423-495), every scene's hypotheses together
     struct Cand {
         int scene, h;
@@ -981,11 +1018,11 @@ void match_batch_impl(ppf_detector_s &D, const std::vector<DevCloud *> &clouds, 
     int nSparse = 0, nDenseIcp = 0, nDenseTot = 0;
     if (p.sparsePoseRefinement && !cand.empty()) {
         std::vector<GridView> vs1(n), vf(n);
-        for_scenes(D, n, [&](int i, Workspace &ws, hipStream_t ls) {
+        for_scenes(X, n, [&](int i, Workspace &ws, hipStream_t ls) {
             if (!sceneHasCand[i])
                 return;
             DevCloud &c  = *clouds[i];
-            auto     &sw = *D.sceneWork[i];
+            auto     &sw = *X.sceneWork[i];
             float *PX = c.px.get<float>(), *PY = c.py.get<float>(), *PZ = c.pz.get<float>();
             float *NX = c.nx.get<float>(), *NY = c.ny.get<float>(), *NZ = c.nz.get<float>();
             if (p.densePoseRefinement) {
@@ -1009,8 +1046,8 @@ void match_batch_impl(ppf_detector_s &D, const std::vector<DevCloud *> &clouds, 
         if (p.densePoseRefinement)
             for (int i = 0; i < n; i++)
                 if (sceneHasCand[i])
-                    nDenseTot += D.sceneWork[i]->nd;
-        GridView *gS = D.batchGrids.as<GridView>((size_t)2 * n);
+                    nDenseTot += X.sceneWork[i]->nd;
+        GridView *gS = X.batchGrids.as<GridView>((size_t)2 * n);
         PPF_HIP_CHECK(hipMemcpyAsync(gS, vs1.data(), sizeof(GridView) * n, hipMemcpyHostToDevice, s));
         PPF_HIP_CHECK(hipMemcpyAsync(gS + n, vf.data(), sizeof(GridView) * n, hipMemcpyHostToDevice,
                                      s));
@@ -1020,13 +1057,13 @@ void match_batch_impl(ppf_detector_s &D, const std::vector<DevCloud *> &clouds, 
             std::memcpy(&init[12 * q], hyp[cand[q].scene][cand[q].h].pose, sizeof(float) * 12);
             hs[q] = cand[q].scene;
         }
-        int *hsD = D.batchHypScene.as<int>(cand.size());
+        int *hsD = X.batchHypScene.as<int>(cand.size());
         PPF_HIP_CHECK(hipMemcpyAsync(hsD, hs.data(), sizeof(int) * hs.size(), hipMemcpyHostToDevice,
                                      s));
         const int first = cand[0].scene;
         IcpCriteria sc{5, poseRefDist, sampleStep * 0.5f, sampleStep, 0.05f};  // ppf.cpp:423
         std::vector<IcpResultDev> res(cand.size());
-        icp_batch(D.ws, refNum, dm.sx.get<float>(), dm.sy.get<float>(), dm.sz.get<float>(),
+        icp_batch(X.ws, refNum, dm.sx.get<float>(), dm.sy.get<float>(), dm.sz.get<float>(),
                   dm.snx.get<float>(), dm.sny.get<float>(), dm.snz.get<float>(), vs1[first],
                   vf[first], init.data(), (int)cand.size(), sc, scoringDist, D.icfg, res.data(), s,
                   gS, gS + n, hsD);
@@ -1054,18 +1091,18 @@ void match_batch_impl(ppf_detector_s &D, const std::vector<DevCloud *> &clouds, 
             for (auto &c : survivors)
                 needDense[c.scene] = 1;
             std::vector<GridView> vd(n);
-            for_scenes(D, n, [&](int i, Workspace &, hipStream_t ls) {
+            for_scenes(X, n, [&](int i, Workspace &, hipStream_t ls) {
                 if (!needDense[i])
                     return;
                 DevCloud &c  = *clouds[i];
-                auto     &sw = *D.sceneWork[i];
+                auto     &sw = *X.sceneWork[i];
                 float *PX = c.px.get<float>(), *PY = c.py.get<float>(), *PZ = c.pz.get<float>();
                 float *NX = c.nx.get<float>(), *NY = c.ny.get<float>(), *NZ = c.nz.get<float>();
                 grid_build(sw.gDense, PX, PY, PZ, NX, NY, NZ, sw.dense.get<int>(), sw.nd,
                            reSampleStep * kDenseCell, ls, c.ext);
                 vd[i] = sw.gDense.view;
             });
-            GridView *gD = D.batchGrids.as<GridView>((size_t)3 * n);
+            GridView *gD = X.batchGrids.as<GridView>((size_t)3 * n);
             PPF_HIP_CHECK(hipMemcpyAsync(gD + 2 * n, vd.data(), sizeof(GridView) * n,
                                          hipMemcpyHostToDevice, s));
             PPF_HIP_CHECK(hipMemcpyAsync(gD + n, vf.data(), sizeof(GridView) * n,
@@ -1077,13 +1114,13 @@ void match_batch_impl(ppf_detector_s &D, const std::vector<DevCloud *> &clouds, 
                             sizeof(float) * 12);
                 hs2[q] = survivors[q].scene;
             }
-            int *hsD2 = D.batchHypScene.as<int>(survivors.size());
+            int *hsD2 = X.batchHypScene.as<int>(survivors.size());
             PPF_HIP_CHECK(hipMemcpyAsync(hsD2, hs2.data(), sizeof(int) * hs2.size(),
                                          hipMemcpyHostToDevice, s));
             const int first2 = survivors[0].scene;
             IcpCriteria dc{p.poseRefNumSteps, poseRefDist, reSampleStep * 0.5f, sampleStep, 0.05f};
             std::vector<IcpResultDev> res2(survivors.size());
-            icp_batch(D.ws, hm.reSampled.size(), dm.rx.get<float>(), dm.ry.get<float>(),
+            icp_batch(X.ws, hm.reSampled.size(), dm.rx.get<float>(), dm.ry.get<float>(),
                       dm.rz.get<float>(), dm.rnx.get<float>(), dm.rny.get<float>(),
                       dm.rnz.get<float>(), vd[first2], vf[first2], init2.data(),
                       (int)survivors.size(), dc, scoringDist, D.icfg, res2.data(), s, gD + 2 * n,
@@ -1119,12 +1156,12 @@ void match_batch_impl(ppf_detector_s &D, const std::vector<DevCloud *> &clouds, 
         std::stable_sort(outs[i].results.begin(), outs[i].results.end(),
                          [](const auto &a, const auto &b) { return a.first > b.first; });
     }
-    PPF_HIP_CHECK(hipEventRecord(D.ev[8], s));
-    PPF_HIP_CHECK(hipEventSynchronize(D.ev[8]));
+    PPF_HIP_CHECK(hipEventRecord(X.ev[8], s));
+    PPF_HIP_CHECK(hipEventSynchronize(X.ev[8]));
     for (int i = 0; i < n; i++) {
         st->nScene += clouds[i]->n;
         st->nValid += clouds[i]->nValid;
-        st->nSampled += D.sceneWork[i]->ns;
+        st->nSampled += X.sceneWork[i]->ns;
         st->nResults += (int)outs[i].results.size();
     }
     st->nKeypoints      = K;
@@ -1141,15 +1178,15 @@ void match_batch_impl(ppf_detector_s &D, const std::vector<DevCloud *> &clouds, 
     st->nPairs        = vs.pairs;
     st->nKeyHits      = vs.hits;
     st->nVotes        = vs.votes;
-    st->msSample      = elapsed(D.ev[0], D.ev[3]);  // sampling + normals + key points
-    st->msVote        = elapsed(D.ev[3], D.ev[5]);
+    st->msSample      = elapsed(X.ev[0], X.ev[3]);  // sampling + normals + key points
+    st->msVote        = elapsed(X.ev[3], X.ev[5]);
     st->msVoteKernel  = vs.kernelMs;
     st->msPairKernel  = vs.pairMs;
     st->msHitsKernel  = vs.hitsMs;
     st->nVoteFallback = vs.overflow;
-    st->msCluster     = elapsed(D.ev[5], D.ev[6]);
-    st->msIcp         = elapsed(D.ev[6], D.ev[8]);
-    st->msTotal       = elapsed(D.ev[0], D.ev[8]);
+    st->msCluster     = elapsed(X.ev[5], X.ev[6]);
+    st->msIcp         = elapsed(X.ev[6], X.ev[8]);
+    st->msTotal       = elapsed(X.ev[0], X.ev[8]);
 }
 
 void write_poses(const MatchOut &o, float *poses16, float *scores, size_t cap, size_t *nOut) {
@@ -1259,7 +1296,7 @@ void save_impl(const ppf_detector_s &D, const char *path) {
     f.close();
 }
 
-void load_impl(ppf_detector_s &D, const char *path) {
+void load_impl(ppf_detector_s &D, MatchCtx &X, const char *path) {
     std::ifstream f(path, std::ios::binary);  // ppf.cpp:529-547
     if (!f.is_open())
         throw std::runtime_error(std::string("failed to open file:") + path);
@@ -1328,7 +1365,7 @@ void load_impl(ppf_detector_s &D, const char *path) {
         m.offs.push_back((int)m.refs.size());
     }
     D.hm = std::move(m);
-    upload_model(D);
+    upload_model(D, X);
     D.trained = true;
 }
 
@@ -1547,7 +1584,9 @@ PPF_API int ppf_train_model(ppf_detector d, const float *points, const float *no
             p = *param;
         else
             ppf_train_param_default(&p);
-        train_impl(*d, points, normals, (int)n, box6, viewPoint, rel, p);
+        WriteLock w(d->model);
+        Lease     L(*d);
+        train_impl(*d, *L, points, normals, (int)n, box6, viewPoint, rel, p);
     });
 }
 
@@ -1562,8 +1601,10 @@ PPF_API int ppf_train_model_mesh(ppf_detector d, const float *points, const floa
             p = *param;
         else
             ppf_train_param_default(&p);
+        WriteLock w(d->model);
+        Lease     L(*d);
         if (nFaces == 0) {
-            train_impl(*d, points, normals, (int)n, box6, viewPoint, rel, p);
+            train_impl(*d, *L, points, normals, (int)n, box6, viewPoint, rel, p);
             return;
         }
         // ppf.cpp:39-52: the checks, then the box of the given model, whose
@@ -1584,10 +1625,10 @@ PPF_API int ppf_train_model_mesh(ppf_detector d, const float *points, const floa
         const float reSampleStep = Dm * p.poseRefRelSamplingDistance;
         std::vector<float> sp, sn;
         sample_mesh(points, (int)n, faces, (int)nFaces, (float)(reSampleStep / 4.), seed, sp, sn,
-                    d->stream);
+                    (*L).stream);
         if (sp.empty())
             throw std::runtime_error("Invalid Input: mesh too small to sample in trainModel");
-        train_impl(*d, sp.data(), sn.data(), (int)(sp.size() / 3), box, viewPoint, rel, p);
+        train_impl(*d, *L, sp.data(), sn.data(), (int)(sp.size() / 3), box, viewPoint, rel, p);
     });
 }
 
@@ -1598,6 +1639,7 @@ PPF_API int ppf_match_scene(ppf_detector d, const float *points, const float *no
                             ppf_match_stats *stats) {
     PPF_REQUIRE(d, "null detector");
     return guarded([&] {
+        ReadLock r(d->model);
         if (!d->trained)
             throw std::runtime_error("No trained model in matchScene");
         ppf_match_param p;
@@ -1607,21 +1649,35 @@ PPF_API int ppf_match_scene(ppf_detector d, const float *points, const float *no
             ppf_match_param_default(&p);
         if (n == 0 || !points)
             throw std::runtime_error("Invalid Input: empty scene in matchScene");
-        d->tmpScene.upload(points, normals, (int)n, box6, viewPoint, d->stream);
+        Lease     L(*d);
+        MatchCtx &X = *L;
+        const auto t0 = std::chrono::steady_clock::now();
+        X.tmpScene.upload(points, normals, (int)n, box6, viewPoint, X.stream, X.ingest);
+        const float ingestMs =
+            std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
         MatchOut o;
-        match_impl(*d, d->tmpScene, rel, kpFrac, minScore, p, o, result, stats);
+        match_impl(*d, X, X.tmpScene, rel, kpFrac, minScore, p, o, result, stats);
+        if (stats)
+            stats->msIngest = ingestMs;
         write_poses(o, poses16, scores, cap, nOut);
     });
 }
 
 PPF_API int ppf_save(ppf_detector d, const char *path) {
     PPF_REQUIRE(d && path, "null argument");
-    return guarded([&] { save_impl(*d, path); });
+    return guarded([&] {
+        ReadLock r(d->model);
+        save_impl(*d, path);
+    });
 }
 
 PPF_API int ppf_load(ppf_detector d, const char *path) {
     PPF_REQUIRE(d && path, "null argument");
-    return guarded([&] { load_impl(*d, path); });
+    return guarded([&] {
+        WriteLock w(d->model);
+        Lease     L(*d);
+        load_impl(*d, *L, path);
+    });
 }
 
 PPF_API const char *ppf_last_error(void) { return g_lastError.c_str(); }
@@ -1631,13 +1687,42 @@ PPF_API int ppf_scene_upload(ppf_detector d, const float *points, const float *n
     PPF_REQUIRE(d && out, "null argument");
     *out = nullptr;
     return guarded([&] {
-        auto sc = std::make_unique<ppf_scene_s>();
-        sc->cloud.upload(points, normals, (int)n, box6, viewPoint, d->stream);
+        auto  sc = std::make_unique<ppf_scene_s>();
+        Lease L(*d);
+        sc->cloud.upload(points, normals, (int)n, box6, viewPoint, (*L).stream, (*L).ingest);
         *out = sc.release();
     });
 }
 
 PPF_API void ppf_scene_free(ppf_scene s) { delete s; }
+
+PPF_API int ppf_scene_download(ppf_scene sc, size_t *n, size_t *nValid, int *hasNormals,
+                               float *points, float *normals, int *valid, float box6[6]) {
+    PPF_REQUIRE(sc, "null scene");
+    return guarded([&] {
+        std::lock_guard<std::mutex> busy(sc->busy);
+        const DevCloud &c = sc->cloud;
+        if (n)
+            *n = (size_t)c.n;
+        if (nValid)
+            *nValid = (size_t)c.nValid;
+        if (hasNormals)
+            *hasNormals = c.hasNormal ? 1 : 0;
+        if (box6) {
+            std::memcpy(box6, c.boxMin, sizeof c.boxMin);
+            std::memcpy(box6 + 3, c.boxMax, sizeof c.boxMax);
+        }
+        if (c.n > 0 && points)
+            PPF_HIP_CHECK(hipMemcpy(points, c.aos.get<float>(), sizeof(float) * 3 * c.n,
+                                    hipMemcpyDeviceToHost));
+        if (c.n > 0 && normals && c.hasNormal)
+            PPF_HIP_CHECK(hipMemcpy(normals, c.aosN.get<float>(), sizeof(float) * 3 * c.n,
+                                    hipMemcpyDeviceToHost));
+        if (c.nValid > 0 && valid)
+            PPF_HIP_CHECK(hipMemcpy(valid, c.valid.get<int>(), sizeof(int) * c.nValid,
+                                    hipMemcpyDeviceToHost));
+    });
+}
 
 PPF_API int ppf_match_scene_resident(ppf_detector d, ppf_scene sc, float rel, float kpFrac,
                                      float minScore, const ppf_match_param *param, float *poses16,
@@ -1650,8 +1735,11 @@ PPF_API int ppf_match_scene_resident(ppf_detector d, ppf_scene sc, float rel, fl
             p = *param;
         else
             ppf_match_param_default(&p);
-        MatchOut o;
-        match_impl(*d, sc->cloud, rel, kpFrac, minScore, p, o, nullptr, stats);
+        ReadLock                    r(d->model);
+        std::lock_guard<std::mutex> busy(sc->busy);
+        Lease                       L(*d);
+        MatchOut                    o;
+        match_impl(*d, *L, sc->cloud, rel, kpFrac, minScore, p, o, nullptr, stats);
         write_poses(o, poses16, scores, cap, nOut);
     });
 }
@@ -1667,24 +1755,28 @@ PPF_API int ppf_scene_inliers(ppf_detector d, ppf_scene sc, const float *poses16
     PPF_REQUIRE(d && sc && (poses16 || nPoses == 0) && (inliers || nPoses == 0),
                 "null argument");
     return guarded([&] {
+        ReadLock                    r(d->model);
+        std::lock_guard<std::mutex> busy(sc->busy);
+        Lease                       L(*d);
+        MatchCtx                   &X = *L;
         if (!d->trained)
             throw std::runtime_error("No trained model in matchScene");
         DevCloud   &c  = sc->cloud;
-        hipStream_t s  = d->stream;
+        hipStream_t s  = X.stream;
         DevModel   &dm = d->dm;
         const float Dm = box_diameter(d->hm.sampled.boxMin, d->hm.sampled.boxMax);
-        grid_build(d->ws.gFull, c.px.get<float>(), c.py.get<float>(), c.pz.get<float>(), nullptr,
+        grid_build(X.ws.gFull, c.px.get<float>(), c.py.get<float>(), c.pz.get<float>(), nullptr,
                    nullptr, nullptr, c.valid.get<int>(), c.nValid,
                    std::max(dist * 2.0f, Dm * 1e-3f), s);
         std::vector<float> P(12 * nPoses);
         for (size_t h = 0; h < nPoses; h++)
             pose16_to_12(poses16 + 16 * h, &P[12 * h]);
         if (dense)
-            icp_inliers(d->ws, d->ws.gFull.view, dm.nReSampled, dm.rx.get<float>(),
+            icp_inliers(X.ws, X.ws.gFull.view, dm.nReSampled, dm.rx.get<float>(),
                         dm.ry.get<float>(), dm.rz.get<float>(), P.data(), (int)nPoses, dist,
                         inliers, s);
         else
-            icp_inliers(d->ws, d->ws.gFull.view, dm.nSampled, dm.sx.get<float>(),
+            icp_inliers(X.ws, X.ws.gFull.view, dm.nSampled, dm.sx.get<float>(),
                         dm.sy.get<float>(), dm.sz.get<float>(), P.data(), (int)nPoses, dist,
                         inliers, s);
     });
@@ -1719,9 +1811,22 @@ PPF_API int ppf_match_scenes_resident(ppf_detector d, const ppf_scene *scenes, s
                 slot[i] = slot[j];
             }
         }
+        // each distinct scene is held for the call, in address order (no
+        // deadlock between calls naming the same scenes in other orders)
+        std::vector<ppf_scene_s *> held;
+        for (size_t i = 0; i < nScenes; i++)
+            if (slot[i] == held.size())
+                held.push_back(scenes[i]);
+        std::vector<ppf_scene_s *> order(held);
+        std::sort(order.begin(), order.end());
+        std::vector<std::unique_lock<std::mutex>> busy;
+        for (ppf_scene_s *h : order)
+            busy.emplace_back(h->busy);
+        ReadLock              r(d->model);
+        Lease                 L(*d);
         std::vector<MatchOut> outs;
         if (!clouds.empty())
-            match_batch_impl(*d, clouds, rel, kpFrac, minScore, p, outs, stats);
+            match_batch_impl(*d, *L, clouds, rel, kpFrac, minScore, p, outs, stats);
         for (size_t i = 0; i < nScenes; i++)
             write_poses(outs[slot[i]], poses16 ? poses16 + 16 * cap * i : nullptr,
                         scores ? scores + cap * i : nullptr, cap, nOut + i);
@@ -1747,6 +1852,7 @@ ppf_tuning tuning_of(const ppf_detector_s &d) {
 
 PPF_API int ppf_get_tuning(ppf_detector d, ppf_tuning *out) {
     PPF_REQUIRE(d && out, "null argument");
+    ReadLock r(d->model);
     *out = tuning_of(*d);
     return PPF_OK;
 }
@@ -1759,14 +1865,15 @@ PPF_API int ppf_set_tuning(ppf_detector d, const ppf_tuning *t) {
     PPF_REQUIRE(t->icpBlock == 64 || t->icpBlock == 128 || t->icpBlock == 256 ||
                     t->icpBlock == 512 || t->icpBlock == 1024,
                 "icp block must be 64..1024, a power of two");
-    d->vcfg.block    = t->voteBlock;
-    d->vcfg.xcdRemap = t->voteXcdRemap ? 1 : 0;
-    d->vcfg.cellDiv  = t->voteCellDiv;
     PPF_REQUIRE(t->voteHitCap >= 1 && t->voteHitCap <= (1 << 20),
                 "vote hit capacity must be 1 .. 2^20");
     PPF_REQUIRE(t->voteTiles >= 1 && t->voteTiles <= 16, "vote tiles must be 1 .. 16");
     PPF_REQUIRE(t->normCell4 >= 1 && t->normCell4 <= 16,
                 "normals cell must be 1 .. 16 quarters of the sample step");
+    WriteLock w(d->model);
+    d->vcfg.block    = t->voteBlock;
+    d->vcfg.xcdRemap = t->voteXcdRemap ? 1 : 0;
+    d->vcfg.cellDiv  = t->voteCellDiv;
     d->icfg.block    = t->icpBlock;
     d->icfg.spread   = t->icpSpread ? 1 : 0;
     d->normCell4     = t->normCell4;
@@ -1786,6 +1893,9 @@ PPF_API int ppf_autotune(ppf_detector d, ppf_scene sc, float rel, float kpFrac, 
             p = *param;
         else
             ppf_match_param_default(&p);
+        WriteLock                   w(d->model);
+        std::lock_guard<std::mutex> busy(sc->busy);
+        Lease                       L(*d);
         VoteConfig bestV = d->vcfg;
         IcpConfig  bestI = d->icfg;
         float      bestVoteMs = 1e30f, bestIcpMs = 1e30f;
@@ -1795,7 +1905,7 @@ PPF_API int ppf_autotune(ppf_detector d, ppf_scene sc, float rel, float kpFrac, 
             for (int rep2 = 0; rep2 < 2; rep2++) {
                 MatchOut        o;
                 ppf_match_stats st{};
-                match_impl(*d, sc->cloud, rel, kpFrac, minScore, p, o, nullptr, &st);
+                match_impl(*d, *L, sc->cloud, rel, kpFrac, minScore, p, o, nullptr, &st);
                 voteMs = std::min(voteMs, st.msVoteKernel);
                 icpMs  = std::min(icpMs, st.msIcp);
             }
@@ -1844,7 +1954,7 @@ PPF_API int ppf_autotune(ppf_detector d, ppf_scene sc, float rel, float kpFrac, 
                 for (int rep2 = 0; rep2 < 2; rep2++) {
                     MatchOut        o;
                     ppf_match_stats st{};
-                    match_impl(*d, sc->cloud, rel, kpFrac, minScore, p, o, nullptr, &st);
+                    match_impl(*d, *L, sc->cloud, rel, kpFrac, minScore, p, o, nullptr, &st);
                     ms = std::min(ms, st.msNormals + st.msIcpPrepare);
                 }
                 tried++;
@@ -1867,6 +1977,7 @@ PPF_API int ppf_autotune(ppf_detector d, ppf_scene sc, float rel, float kpFrac, 
 PPF_API int ppf_model_get_info(ppf_detector d, ppf_model_info *info) {
     PPF_REQUIRE(d && info, "null argument");
     return guarded([&] {
+        ReadLock r(d->model);
         if (!d->trained)
             throw std::runtime_error("No trained model");
         const HostModel &m     = d->hm;
@@ -1886,6 +1997,7 @@ PPF_API int ppf_model_get_sampled(ppf_detector d, float *pts, float *nrm, float 
                                   float *reNrm) {
     PPF_REQUIRE(d, "null detector");
     return guarded([&] {
+        ReadLock r(d->model);
         if (!d->trained)
             throw std::runtime_error("No trained model");
         const HostModel &m = d->hm;
@@ -1900,6 +2012,7 @@ PPF_API int ppf_model_get_table(ppf_detector d, uint32_t *keys, int *offs, uint3
                                 float *alphas) {
     PPF_REQUIRE(d, "null detector");
     return guarded([&] {
+        ReadLock r(d->model);
         if (!d->trained)
             throw std::runtime_error("No trained model");
         const HostModel &m = d->hm;
@@ -1936,11 +2049,14 @@ PPF_API int ppf_vote(ppf_detector d, const float *pts, const float *nrm, size_t 
                      size_t dumpCount, float *poses12, int *nPoses) {
     PPF_REQUIRE(d && pts && nrm && keypoints && cand3, "null argument");
     return guarded([&] {
+        ReadLock  r(d->model);
+        Lease     L(*d);
+        MatchCtx &X = *L;
         if (!d->trained)
             throw std::runtime_error("No trained model in matchScene");
-        hipStream_t s = d->stream;
-        DevCloud   &c = d->tmpScene;
-        c.upload(pts, nrm, (int)n, nullptr, nullptr, s);
+        hipStream_t s = X.stream;
+        DevCloud   &c = X.tmpScene;
+        c.upload(pts, nrm, (int)n, nullptr, nullptr, s, X.ingest);
         // the vote stage reads positions and normals as given
         size_t b = sizeof(float) * n;
         PPF_HIP_CHECK(hipMemcpyAsync(c.px.get<float>(), c.ox.get<float>(), b, hipMemcpyDeviceToDevice, s));
@@ -1954,11 +2070,11 @@ PPF_API int ppf_vote(ppf_detector d, const float *pts, const float *nrm, size_t 
         // every point is a sampled point
         std::vector<int> all(n);
         std::iota(all.begin(), all.end(), 0);
-        int *samp = d->idxA.as<int>(std::max<size_t>(n, 1));
-        int *kp   = d->idxB.as<int>(std::max<size_t>(nk, 1));
+        int *samp = X.idxA.as<int>(std::max<size_t>(n, 1));
+        int *kp   = X.idxB.as<int>(std::max<size_t>(nk, 1));
         PPF_HIP_CHECK(hipMemcpyAsync(samp, all.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
         PPF_HIP_CHECK(hipMemcpyAsync(kp, keypoints, sizeof(int) * nk, hipMemcpyHostToDevice, s));
-        grid_build(d->ws.gVote, c.px.get<float>(), c.py.get<float>(), c.pz.get<float>(),
+        grid_build(X.ws.gVote, c.px.get<float>(), c.py.get<float>(), c.pz.get<float>(),
                    c.nx.get<float>(), c.ny.get<float>(), c.nz.get<float>(), samp, (int)n,
                    Dm / (float)d->vcfg.cellDiv, s);
         DevBuf dump;
@@ -1968,12 +2084,12 @@ PPF_API int ppf_vote(ppf_detector d, const float *pts, const float *nrm, size_t 
             PPF_HIP_CHECK(hipMemsetAsync(dd, 0, sizeof(int32_t) * dumpCount * d->dm.nSampled * d->dm.angleNum, s));
         }
         float voteThreshold = (float)hm.sampled.size() * minScore;
-        int   np = vote_and_poses(d->ws, d->dm, d->ws.gVote.view, c.px.get<float>(),
+        int   np = vote_and_poses(X.ws, d->dm, X.ws.gVote.view, c.px.get<float>(),
                                   c.py.get<float>(), c.pz.get<float>(), c.nx.get<float>(),
                                   c.ny.get<float>(), c.nz.get<float>(), kp, (int)nk, voteThreshold,
                                   d->vcfg, nullptr, dd, (int)dumpCount, s);
         std::vector<PoseRec> pr(np);
-        PPF_HIP_CHECK(hipMemcpyAsync(pr.data(), d->ws.poses.get<PoseRec>(), sizeof(PoseRec) * np,
+        PPF_HIP_CHECK(hipMemcpyAsync(pr.data(), X.ws.poses.get<PoseRec>(), sizeof(PoseRec) * np,
                                      hipMemcpyDeviceToHost, s));
         if (dd)
             PPF_HIP_CHECK(hipMemcpyAsync(accDump, dd,
@@ -2411,8 +2527,9 @@ PPF_API int ppf_scene_upload_ply(ppf_detector d, const char *path, const float *
         const size_t n  = c.points.size() / 3;
         const bool   hn = c.normals.size() == c.points.size();  // type.cpp:36-38
         auto         sc = std::make_unique<ppf_scene_s>();
+        Lease        L(*d);
         sc->cloud.upload(c.points.data(), hn ? c.normals.data() : nullptr, (int)n, nullptr,
-                         viewPoint, d->stream);
+                         viewPoint, (*L).stream, (*L).ingest);
         *out = sc.release();
     });
 }

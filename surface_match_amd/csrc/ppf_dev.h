@@ -112,6 +112,24 @@ void model_upload(DevModel &m, int nSampled, const float *sp, const float *sn, i
                   const std::vector<float> &alphas, int angleNum, float diameter,
                   float distStepRel, hipStream_t s);
 
+// ---------------------------------------------------------------- ingest
+// removeNan + computeBoundingBox (util.cpp:86-105,178-207) of a cloud whose
+// AoS arrays are already on the device (ingest.hip).
+struct IngestScratch {
+    DevBuf    masks, blockCnt, blockOff, keys, devInfo;
+    uint32_t *info = nullptr;  // pinned read-back of {count, box bits}
+    IngestScratch() = default;
+    IngestScratch(const IngestScratch &) = delete;
+    IngestScratch &operator=(const IngestScratch &) = delete;
+    ~IngestScratch();
+};
+// SoA originals (normals only when aosN != NULL), the ascending indices of
+// the valid points into valid[], and the box of the valid points (zeros when
+// none); returns the number of valid points.  Synchronises s.
+int ingest_cloud(IngestScratch &is, const float *aosP, const float *aosN, int n, float *ox,
+                 float *oy, float *oz, float *onx, float *ony, float *onz, int *valid,
+                 float ext[6], hipStream_t s);
+
 // ---------------------------------------------------------------- grid/sample/normals
 void device_bbox(const float *X, const float *Y, const float *Z, const int *subset, int m,
                  float *bb6dev, hipStream_t s);

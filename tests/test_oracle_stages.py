@@ -347,3 +347,35 @@ def test_match_end_to_end_small(oracle_mod):
         found.add(k)
     assert len(found) == len(poses)
     assert (np.diff(scores) <= 0).all()
+
+
+def test_remove_nan_and_bounding_box_hand_cases(oracle_mod):
+    """removeNan (util.cpp:86-105) and computeBoundingBox (util.cpp:178-207)
+    on hand-made clouds: non-finite coordinates drop a point, non-finite
+    normals drop it only when the cloud has normals; the box starts at the
+    first valid point and a coordinate replaces a bound only when strictly
+    beyond it, so among equal extremes (+0 and -0 included) the earliest
+    point's bits stay."""
+    O = oracle_mod
+    nan, inf = np.float32(np.nan), np.float32(np.inf)
+    p = np.array([[1, 2, 3], [nan, 0, 0], [0, inf, 0], [-1, 5, 2], [4, -7, 9]], np.float32)
+    n = np.array([[0, 0, 1], [0, 0, 1], [0, 0, 1], [nan, 0, 1], [1, 0, 0]], np.float32)
+    assert O.remove_nan(p).tolist() == [0, 3, 4]
+    assert O.remove_nan(p, n).tolist() == [0, 4]
+    assert O.bounding_box(p, [0, 3, 4]).tolist() == [-1, -7, 2, 4, 5, 9]
+    # ties of +0 and -0: the earliest holder of the extreme keeps its sign
+    z = np.float32(0.0)
+    mz = np.float32(-0.0)
+    q = np.array([[z, 1, 1], [mz, 2, 2], [3, 3, 3]], np.float32)
+    b = O.bounding_box(q, [0, 1, 2])
+    assert b[0] == 0 and not np.signbit(b[0])          # +0 first, -0 later: +0
+    b = O.bounding_box(q, [1, 0, 2])
+    assert b[0] == 0 and np.signbit(b[0])              # -0 first: -0
+    q = np.array([[-1, -1, -1], [mz, 0, 0], [z, 0, 0]], np.float32)
+    b = O.bounding_box(q, [0, 1, 2])
+    assert b[3] == 0 and np.signbit(b[3])              # max: -0 reached first
+    # a repeated extreme: the first index (visible through -0/+0 only, so
+    # check that later equal values do not move the bound)
+    q = np.array([[5, 0, 0], [mz, 0, 0], [z, 0, 0], [mz, 0, 0]], np.float32)
+    b = O.bounding_box(q, [0, 1, 2, 3])
+    assert b[0] == 0 and np.signbit(b[0]) and b[3] == 5
