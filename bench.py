@@ -479,8 +479,8 @@ def main():
         "results": found,
         "tuning": det.get_tuning(),
     }
+    out["measured_ceilings"] = measured_ceilings()
     if not args.lean:
-        out["measured_ceilings"] = measured_ceilings()
         out["ply_ingest"] = ply_ingest(det, w["scene"].points)
         out["mesh_sampling"] = mesh_sampling()
         out["mls_normals"] = mls_normals()

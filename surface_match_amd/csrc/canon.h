@@ -15,6 +15,9 @@
 
 #include <stdint.h>
 #include <math.h>
+#ifndef __HIP_DEVICE_COMPILE__
+#include <cmath>
+#endif
 
 #if defined(__HIPCC__)
 #include <hip/hip_runtime.h>
@@ -230,6 +233,25 @@ PPF_HD int bin_wrapped(float alphaModel, float off, float sMaxId, int angleNum) 
     int bin = (int)(sMaxId * a);
     bin     = bin < 0 ? 0 : bin;
     return bin > angleNum - 1 ? angleNum - 1 : bin;
+}
+
+// bin_wrapped for finite inputs in fewer instructions (the voting loop's
+// form).  The wrap adds one selected constant: a - 2pi is a + (-2pi) in
+// IEEE arithmetic, and a + 0 == a except for a = -0, whose bin is 0 either
+// way.  The clamp acts on the product in float: for finite x,
+// trunc(clamp(x, 0, maxBin)) == clamp(trunc(x), 0, maxBin) because trunc is
+// monotone and maps [0, maxBin] onto integers within the same bounds.
+// maxBin = angleNum - 1 as a float.
+PPF_HD int bin_wrapped_fast(float alphaModel, float off, float sMaxId, float maxBin) {
+    float       a = alphaModel - off;
+    const float w = a > kTwoPi ? -kTwoPi : (a < 0.0f ? kTwoPi : 0.0f);
+    a             = a + w;
+#if defined(__HIP_DEVICE_COMPILE__)
+    const float x = __builtin_amdgcn_fmed3f(sMaxId * a, 0.0f, maxBin);
+#else
+    const float x = std::fmin(std::fmax(sMaxId * a, 0.0f), maxBin);
+#endif
+    return (int)x;
 }
 
 // angle bin of a bucket entry (ppf.cpp:316-340); tail = the entry is among

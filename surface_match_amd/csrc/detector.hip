@@ -1348,6 +1348,11 @@ void load_impl(ppf_detector_s &D, MatchCtx &X, const char *path) {
         for (uint32_t r : b[k].second.first)
             if (r >= nRef)
                 throw std::runtime_error(std::string("corrupt hash table in:") + path);
+        // training never stores a NaN alpha (ppf.cpp:131-139); the voting
+        // kernels rely on finite ones
+        for (float al : b[k].second.second)
+            if (!std::isfinite(al))
+                throw std::runtime_error(std::string("corrupt hash table in:") + path);
     }
     if (!f.good())
         throw std::runtime_error(std::string("truncated model file:") + path);

@@ -76,6 +76,9 @@ struct DevModel {
     DevBuf   buckets;                        // uint2 {start in ventries, len} per bucket index
     DevBuf   cosSin;                         // 2 * angleNum floats
     int      refsSorted = 0;                 // every bucket's refs ascending
+    // the most entries any bucket has for one reference: a key point with n
+    // hits adds at most n * maxMult votes to any accumulator cell
+    uint32_t maxMult = 1;
     unsigned generation = 0;                 // bumped by every upload
     // pair-stage pre-filters over quantized features (filterA == 0: none):
     // bitmaps of the (q4), (q3, q4) and (q1, q3, q4) projections of every

@@ -338,11 +338,21 @@ void model_upload(DevModel &m, int nSampled, const float *sp, const float *sn, i
     // ascending; a loaded table may differ)
     std::vector<uint2> bk(std::max(K, 1));
     m.refsSorted = 1;
+    m.maxMult    = 1;
+    std::vector<uint32_t> seen((size_t)std::max(nSampled, 1), 0u), mult((size_t)std::max(nSampled, 1), 0u);
     for (int k = 0; k < K; k++) {
         bk[k] = make_uint2(vstart[k], (uint32_t)(offs[k + 1] - offs[k]));
         for (int e = offs[k] + 1; e < offs[k + 1]; e++)
             if (refs[e] < refs[e - 1])
                 m.refsSorted = 0;
+        for (int e = offs[k]; e < offs[k + 1]; e++) {
+            const uint32_t r = refs[e] < (uint32_t)nSampled ? refs[e] : 0u;
+            if (seen[r] != (uint32_t)k + 1u) {
+                seen[r] = (uint32_t)k + 1u;
+                mult[r] = 0u;
+            }
+            m.maxMult = std::max(m.maxMult, ++mult[r]);
+        }
     }
     PPF_HIP_CHECK(hipMemcpyAsync(m.buckets.as<uint2>(bk.size()), bk.data(), sizeof(uint2) * bk.size(),
                                  hipMemcpyHostToDevice, s));

@@ -20,6 +20,8 @@
 // accumulator does not depend on the order of any of this.
 #include "ppf_dev.h"
 
+#include <type_traits>
+
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
@@ -74,6 +76,8 @@ struct VoteArgs {
     // quantized-feature pre-filters of the pair kernel (nullptr: none)
     const uint32_t *filter;
     int          fA, fD;
+    float        maxBin;   // angleNum - 1
+    uint32_t     maxMult;  // most entries of one reference in one bucket
 };
 
 __device__ __forceinline__ bool table_find(const uint4 *slots, uint32_t tmask, uint32_t key,
@@ -161,28 +165,39 @@ __device__ __forceinline__ int vote_run_loaded(const VoteArgs &a, uint32_t *acc,
 }
 
 // The same for one reference tile with 16-bit counters, two per word: the
-// vote adds 1 to the cell's half word and the returned old word tells
-// whether that half was already at 0xFFFF (a carry into the neighbour).
-// Such a carry from a real cell (c < realCells) sets ovf, and the key point
-// is then voted again with 32-bit counters.  Cells of the spare row never
-// carry into real cells: a carry goes from the low half (even cell) to the
-// high half of the same word, and every cell above a spare one is spare.
+// vote adds 1 to the cell's half word.  The returned old words are OR-ed
+// into ovf: a counter that ends above 0x8000 was incremented from 0x8000 at
+// some point, and that increment returned a word with bit 15 or 31 set.  So
+// when no returned word had either bit set, every counter stayed at or below
+// 0x8000 and no half ever carried into its neighbour; otherwise (checked
+// once, after the votes) the key point is voted again with 32-bit counters.
+// A set bit from a counter that stopped at exactly 0x8000, or from the
+// spare row, only sends the key point to the 32-bit kernel unnecessarily.
+//
+// RET false: the key point's hits cannot bring any real cell past 0xFFFF
+// (n hits x maxMult <= 0xFFFF), so the adds return nothing; the spare row
+// may wrap, which only moves counts between spare cells.
+// Entry alphas and scene alphas are finite (training skips NaN alphas,
+// ppf.cpp:131-139, loading rejects them; hits have finite scene alphas), so
+// the bin is bin_wrapped_fast's.  The word index is masked to 18 bits (the
+// accumulator is far smaller) so the compiler can fold the LDS base into
+// the instruction's offset.
+template <bool RET>
 __device__ __forceinline__ int vote_run_packed(const VoteArgs &a, uint32_t *acc,
                                                const SegRun &g, const uint2 (&v)[SEG],
-                                               int realCells, uint32_t &ovf) {
+                                               uint32_t &ovf) {
     const float off = g.e0 >= g.tailStart ? g.offTail : g.offBody;
-    uint32_t    old[SEG];
-    int         cell[SEG];
 #pragma unroll
     for (int kk = 0; kk < SEG; kk++) {
-        const int bin = ppfc::bin_wrapped(__uint_as_float(v[kk].y), off, a.sMaxId, a.angleNum);
-        cell[kk]      = (int)v[kk].x + bin;
-        old[kk]       = atomicAdd(&acc[cell[kk] >> 1], 1u << ((cell[kk] & 1) << 4));
+        const uint32_t bin =
+            (uint32_t)ppfc::bin_wrapped_fast(__uint_as_float(v[kk].y), off, a.sMaxId, a.maxBin);
+        const uint32_t cell = v[kk].x + bin;
+        const uint32_t inc  = 1u << ((cell & 1u) << 4);
+        if (RET)
+            ovf |= atomicAdd(&acc[(cell >> 1) & 0x3FFFFu], inc);
+        else
+            atomicAdd(&acc[(cell >> 1) & 0x3FFFFu], inc);
     }
-#pragma unroll
-    for (int kk = 0; kk < SEG; kk++)
-        ovf |= (cell[kk] < realCells &&
-                ((old[kk] >> ((cell[kk] & 1) << 4)) & 0xFFFFu) == 0xFFFFu) ? 1u : 0u;
     return g.vhi - g.vlo;
 }
 
@@ -950,8 +965,13 @@ __global__ void __launch_bounds__(BLOCK) vote_hits_kernel(VoteArgs a) {
     extern __shared__ uint32_t acc[];
     __shared__ uint32_t sOvf;
     __shared__ int      sPre[BLOCK];
-    __shared__ uint32_t sLo[BLOCK], sHi[BLOCK], sTail[BLOCK];
-    __shared__ float    sOffB[BLOCK], sOffT[BLOCK];
+    // the current batch of hits: with one tile (PACKED) each bucket is
+    // voted whole and a hit needs only {start, length, scene alpha} (12 B;
+    // the static LDS then leaves room for three workgroups per CU beside a
+    // 50 KB accumulator); with reference tiles {lo, hi, tail start, body
+    // offset, tail offset} of the tile's slice
+    constexpr int       HW = PACKED ? 3 : 5;
+    __shared__ uint32_t sHit[HW * BLOCK];
     __shared__ int      sScan[BLOCK / 64 + 1];
     __shared__ unsigned long long sTop[(BLOCK / 64) * 3];
 
@@ -974,121 +994,154 @@ __global__ void __launch_bounds__(BLOCK) vote_hits_kernel(VoteArgs a) {
         return;
     }
     const int realCells = nrTile * a.angleNum;
-    for (int i = tid; i < (PACKED ? (realCells + 1) >> 1 : realCells); i += BLOCK)
+    // packed: the spare row too, whose returned words take part in the
+    // overflow test
+    for (int i = tid; i < (PACKED ? (realCells + a.angleNum + 1) >> 1 : realCells); i += BLOCK)
         acc[i] = 0u;
     if (tid == 0)
         sOvf = 0u;
     uint32_t           ovf    = 0u;
     unsigned long long nVotes = 0;
     const size_t       base   = (size_t)k * a.hitCap;
-    for (int h0 = 0; h0 < n; h0 += BLOCK) {
-        // this tile's entries of each hit's bucket: the whole bucket, or with
-        // ref-sorted buckets only the slice holding the tile's references
-        const int i   = h0 + tid;
-        int       cnt = 0;
-        if (i < n) {
-            const uint32_t st = a.hitStart[base + i], L = a.hitLen[base + i];
-            uint32_t       lo = st, hi = st + L;
-            if (a.splits) {
-                const uint32_t *sp = a.splits + (size_t)a.hitBucket[base + i] * (a.nTiles + 1);
-                lo                 = sp[tile];
-                hi                 = sp[tile + 1];
-            }
-            // aligned segments [lo & ~7, roundup8(hi)) with entries outside
-            // [lo, hi) masked
-            const uint32_t al = lo & ~7u, ah = (hi + 7u) & ~7u;
-            cnt            = hi > lo ? (int)((ah - al) / SEG) : 0;
-            sLo[tid]       = lo;
-            sHi[tid]       = hi;
-            sTail[tid]     = st + L - (L & 7u);
-            const float aS = a.hitAlpha[base + i];
-            sOffB[tid]     = ppfc::bin_offset_body(aS);
-            sOffT[tid]     = ppfc::bin_offset_tail(aS);
-        }
-        int       totSeg;
-        const int pre = block_excl_scan<BLOCK>(cnt, sScan, totSeg);
-        sPre[tid]     = pre;
-        __syncthreads();
-        // each thread votes a contiguous run of segments (SEG entries of one
-        // bucket each; different lanes, different buckets, so the LDS atomics
-        // of a wave scatter over the accumulator): one search for the run's
-        // first hit, then a walk (hits with no segment are skipped); the
-        // loads of HITS_INFLIGHT segments are issued before any of them is
-        // voted (the accumulator leaves room for two workgroups of waves per
-        // CU at most, so each lane keeps several loads in flight instead)
-        const int nb  = min(BLOCK, n - h0);
-        const int spt = (totSeg + BLOCK - 1) / BLOCK;
-        const int t0  = tid * spt, t1 = min(t0 + spt, totSeg);
-        const int r0A = r0 * a.angleNum, r1A = r1 * a.angleNum;
-        int       nv  = 0;
-        if (t0 < t1) {
-            int lo = 0, hi = nb - 1;  // last hit whose prefix <= t0
-            while (lo < hi) {
-                int mid = (lo + hi + 1) >> 1;
-                if (sPre[mid] <= t0)
-                    lo = mid;
-                else
-                    hi = mid - 1;
-            }
-            // the current hit's descriptor stays in registers; LDS is read
-            // again only when the run crosses into the next hit
-            int      h       = lo;
-            int      hPre    = sPre[h];
-            int      nextPre = h + 1 < nb ? sPre[h + 1] : 0x7fffffff;
-            uint32_t hLo = sLo[h], hHi = sHi[h], hTail = sTail[h];
-            float    hOffB = sOffB[h], hOffT = sOffT[h];
-            for (int t = t0; t < t1; t += HITS_INFLIGHT) {
-                SegRun g[HITS_INFLIGHT];
-                uint2  ent[HITS_INFLIGHT][SEG];
-#pragma unroll
-                for (int j = 0; j < HITS_INFLIGHT; j++) {
-                    const int tq = t + j;
-                    g[j].vlo     = 0;
-                    g[j].vhi     = 0;
-                    if (tq < t1) {
-                        if (tq >= nextPre) {
-                            // hits without segments share their prefix with
-                            // the next hit and are stepped over
-                            do {
-                                h++;
-                                hPre    = nextPre;
-                                nextPre = h + 1 < nb ? sPre[h + 1] : 0x7fffffff;
-                            } while (tq >= nextPre);
-                            hLo   = sLo[h];
-                            hHi   = sHi[h];
-                            hTail = sTail[h];
-                            hOffB = sOffB[h];
-                            hOffT = sOffT[h];
-                        }
-                        // entry offsets stay below 2^31: 32-bit differences
-                        g[j].e0        = (hLo & ~7u) + (uint32_t)(tq - hPre) * SEG;
-                        g[j].vlo       = max((int)(hLo - g[j].e0), 0);
-                        g[j].vhi       = min((int)(hHi - g[j].e0), SEG);
-                        g[j].tailStart = hTail;
-                        g[j].offBody   = hOffB;
-                        g[j].offTail   = hOffT;
-                        load_segment(a, g[j].e0, ent[j]);
-                    }
+    // the voting loop, with returning adds (RET) or without: a key point
+    // whose hits cannot lift a 16-bit counter past 0xFFFF (n * maxMult <=
+    // 0xFFFF) needs no overflow test
+    auto voteAll = [&](auto retTag) {
+        constexpr bool RET = decltype(retTag)::value;
+        for (int h0 = 0; h0 < n; h0 += BLOCK) {
+            // this tile's entries of each hit's bucket: the whole bucket, or with
+            // ref-sorted buckets only the slice holding the tile's references
+            const int i   = h0 + tid;
+            int       cnt = 0;
+            if (i < n) {
+                const uint32_t st = a.hitStart[base + i], L = a.hitLen[base + i];
+                uint32_t       lo = st, hi = st + L;
+                if (a.splits) {
+                    const uint32_t *sp = a.splits + (size_t)a.hitBucket[base + i] * (a.nTiles + 1);
+                    lo                 = sp[tile];
+                    hi                 = sp[tile + 1];
                 }
-#pragma unroll
-                for (int j = 0; j < HITS_INFLIGHT; j++)
-                    if (t + j < t1) {
-                        if (PACKED)
-                            nv += vote_run_packed(a, acc, g[j], ent[j], realCells, ovf);
-                        else
-                            nv += a.nTiles > 1
-                                      ? vote_run_loaded<true>(a, acc, r0A, r1A, g[j], ent[j])
-                                      : vote_run_loaded<false>(a, acc, r0A, r1A, g[j], ent[j]);
-                    }
+                // aligned segments [lo & ~7, roundup8(hi)) with entries outside
+                // [lo, hi) masked
+                const uint32_t al = lo & ~7u, ah = (hi + 7u) & ~7u;
+                cnt            = hi > lo ? (int)((ah - al) / SEG) : 0;
+                const float aS = a.hitAlpha[base + i];
+                if (PACKED) {
+                    sHit[tid]             = st;
+                    sHit[BLOCK + tid]     = L;
+                    sHit[2 * BLOCK + tid] = __float_as_uint(aS);
+                } else {
+                    sHit[tid]             = lo;
+                    sHit[BLOCK + tid]     = hi;
+                    sHit[2 * BLOCK + tid] = st + L - (L & 7u);
+                    sHit[3 * BLOCK + tid] = __float_as_uint(ppfc::bin_offset_body(aS));
+                    sHit[4 * BLOCK + tid] = __float_as_uint(ppfc::bin_offset_tail(aS));
+                }
             }
+            int       totSeg;
+            const int pre = block_excl_scan<BLOCK>(cnt, sScan, totSeg);
+            sPre[tid]     = pre;
+            __syncthreads();
+            // each thread votes a contiguous run of segments (SEG entries of one
+            // bucket each; different lanes, different buckets, so the LDS atomics
+            // of a wave scatter over the accumulator): one search for the run's
+            // first hit, then a walk (hits with no segment are skipped); the
+            // loads of HITS_INFLIGHT segments are issued before any of them is
+            // voted (the accumulator leaves room for two workgroups of waves per
+            // CU at most, so each lane keeps several loads in flight instead)
+            const int nb  = min(BLOCK, n - h0);
+            const int spt = (totSeg + BLOCK - 1) / BLOCK;
+            const int t0  = tid * spt, t1 = min(t0 + spt, totSeg);
+            const int r0A = r0 * a.angleNum, r1A = r1 * a.angleNum;
+            int       nv  = 0;
+            if (t0 < t1) {
+                int lo = 0, hi = nb - 1;  // last hit whose prefix <= t0
+                while (lo < hi) {
+                    int mid = (lo + hi + 1) >> 1;
+                    if (sPre[mid] <= t0)
+                        lo = mid;
+                    else
+                        hi = mid - 1;
+                }
+                // the current hit's descriptor stays in registers; LDS is read
+                // again only when the run crosses into the next hit
+                int      h       = lo;
+                int      hPre    = sPre[h];
+                int      nextPre = h + 1 < nb ? sPre[h + 1] : 0x7fffffff;
+                uint32_t hLo, hHi, hTail;
+                float    hOffB, hOffT;
+                auto     loadHit = [&](int q) {
+                    if (PACKED) {
+                        const uint32_t st = sHit[q], L = sHit[BLOCK + q];
+                        const float    aS = __uint_as_float(sHit[2 * BLOCK + q]);
+                        hLo                = st;
+                        hHi                = st + L;
+                        hTail              = st + L - (L & 7u);
+                        hOffB              = ppfc::bin_offset_body(aS);
+                        hOffT              = ppfc::bin_offset_tail(aS);
+                    } else {
+                        hLo   = sHit[q];
+                        hHi   = sHit[BLOCK + q];
+                        hTail = sHit[2 * BLOCK + q];
+                        hOffB = __uint_as_float(sHit[3 * BLOCK + q]);
+                        hOffT = __uint_as_float(sHit[4 * BLOCK + q]);
+                    }
+                };
+                loadHit(h);
+                for (int t = t0; t < t1; t += HITS_INFLIGHT) {
+                    SegRun g[HITS_INFLIGHT];
+                    uint2  ent[HITS_INFLIGHT][SEG];
+#pragma unroll
+                    for (int j = 0; j < HITS_INFLIGHT; j++) {
+                        const int tq = t + j;
+                        g[j].vlo     = 0;
+                        g[j].vhi     = 0;
+                        if (tq < t1) {
+                            if (tq >= nextPre) {
+                                // hits without segments share their prefix with
+                                // the next hit and are stepped over
+                                do {
+                                    h++;
+                                    hPre    = nextPre;
+                                    nextPre = h + 1 < nb ? sPre[h + 1] : 0x7fffffff;
+                                } while (tq >= nextPre);
+                                loadHit(h);
+                            }
+                            // entry offsets stay below 2^31: 32-bit differences
+                            g[j].e0        = (hLo & ~7u) + (uint32_t)(tq - hPre) * SEG;
+                            g[j].vlo       = max((int)(hLo - g[j].e0), 0);
+                            g[j].vhi       = min((int)(hHi - g[j].e0), SEG);
+                            g[j].tailStart = hTail;
+                            g[j].offBody   = hOffB;
+                            g[j].offTail   = hOffT;
+                            load_segment(a, g[j].e0, ent[j]);
+                        }
+                    }
+#pragma unroll
+                    for (int j = 0; j < HITS_INFLIGHT; j++)
+                        if (t + j < t1) {
+                            if (PACKED)
+                                nv += vote_run_packed<RET>(a, acc, g[j], ent[j], ovf);
+                            else
+                                nv += a.nTiles > 1
+                                          ? vote_run_loaded<true>(a, acc, r0A, r1A, g[j], ent[j])
+                                          : vote_run_loaded<false>(a, acc, r0A, r1A, g[j], ent[j]);
+                        }
+                }
+            }
+            nVotes += (unsigned long long)nv;
+            __syncthreads();
         }
-        nVotes += (unsigned long long)nv;
-        __syncthreads();
-    }
+    };
+    const bool ret = PACKED && (unsigned long long)n * a.maxMult > 0xFFFFull;
+    if (ret)
+        voteAll(std::true_type{});
+    else
+        voteAll(std::false_type{});
     if (PACKED) {
         // a 16-bit counter overflowed: the fused kernel votes this key point
         // again (and writes its candidates, dump and counts)
-        if (ovf)
+        if (ovf & 0x80008000u)
             atomicOr(&sOvf, 1u);
         __syncthreads();
         if (sOvf) {
@@ -1359,6 +1412,8 @@ int vote_keypoints(Workspace &ws, const DevModel &m, const VoteKeypoints &in, fl
     a.angleStep = m.angleStep;
     a.distStep  = m.distStep;
     a.sMaxId    = (float)(m.angleNum - 1) / ppfc::kTwoPi;
+    a.maxBin    = (float)(m.angleNum - 1);
+    a.maxMult   = m.maxMult;
     a.D2        = D * D;            // ppf.cpp:191
     a.voteThreshold = voteThreshold;
     a.cand  = ws.cand.as<unsigned long long>((size_t)nk * nTiles * 3);

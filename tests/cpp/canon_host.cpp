@@ -18,6 +18,10 @@ int ch_bin_wrapped(float am, float as, int n, int tail) {
     const float off = tail ? ppfc::bin_offset_tail(as) : ppfc::bin_offset_body(as);
     return ppfc::bin_wrapped(am, off, (float)(n - 1) / ppfc::kTwoPi, n);
 }
+int ch_bin_wrapped_fast(float am, float as, int n, int tail) {
+    const float off = tail ? ppfc::bin_offset_tail(as) : ppfc::bin_offset_body(as);
+    return ppfc::bin_wrapped_fast(am, off, (float)(n - 1) / ppfc::kTwoPi, (float)(n - 1));
+}
 void ch_pose(const float *Ts, const float *Tm, float c, float s, float *out) {
     ppfc::compose_pose(Ts, Tm, c, s, out);
 }

@@ -71,5 +71,7 @@ def test_branch_free_bin_equals_both_rules(H, oracle_mod):
     for n in (30, 29, 64):
         for tail in (0, 1):
             for x, y in zip(am, as_):
-                assert H.ch_bin_wrapped(x, y, n, tail) == O.lib().orc_angle_bin(
-                    float(x), float(y), n, tail), (x, y, n, tail)
+                ref = O.lib().orc_angle_bin(float(x), float(y), n, tail)
+                assert H.ch_bin_wrapped(x, y, n, tail) == ref, (x, y, n, tail)
+                # the voting loop's form (added wrap constant, float clamp)
+                assert H.ch_bin_wrapped_fast(x, y, n, tail) == ref, (x, y, n, tail)
