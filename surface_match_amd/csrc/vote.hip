@@ -132,6 +132,21 @@ __device__ __forceinline__ void load_segment(const VoteArgs &a, uint32_t e0, uin
         v[2 * q + 1]  = make_uint2(w.z, w.w);
     }
 }
+// N consecutive entries from e (a multiple of N; N = 1, 2, 4 or 8)
+template <int N>
+__device__ __forceinline__ void load_entries(const VoteArgs &a, uint32_t e, uint2 (&v)[N]) {
+    if (N == 1) {
+        v[0] = a.ventries[e];
+    } else {
+        const uint4 *p = reinterpret_cast<const uint4 *>(a.ventries + e);
+#pragma unroll
+        for (int q = 0; q < N / 2; q++) {
+            const uint4 w = p[q];
+            v[2 * q]      = make_uint2(w.x, w.y);
+            v[2 * q + 1]  = make_uint2(w.z, w.w);
+        }
+    }
+}
 
 //
 // A segment is all body or all tail: buckets and their tails start on
@@ -179,24 +194,30 @@ __device__ __forceinline__ int vote_run_loaded(const VoteArgs &a, uint32_t *acc,
 // may wrap, which only moves counts between spare cells.
 // Entry alphas and scene alphas are finite (training skips NaN alphas,
 // ppf.cpp:131-139, loading rejects them; hits have finite scene alphas), so
-// the bin is bin_wrapped_fast's.  The word index is masked to 18 bits (the
-// accumulator is far smaller) so the compiler can fold the LDS base into
-// the instruction's offset.
-template <bool RET>
-__device__ __forceinline__ int vote_run_packed(const VoteArgs &a, uint32_t *acc,
-                                               const SegRun &g, const uint2 (&v)[SEG],
-                                               uint32_t &ovf) {
+// the bin is bin_wrapped_fast's.  The accumulator is the kernel's only LDS
+// and starts at LDS address 0 (checked on the host before the first
+// launch), so the adds go through an LDS pointer of known value 0 and the
+// word index, masked to 18 bits (the accumulator is far smaller), is the
+// whole address computation.
+// With N < SEG the segment is shared by SEG / N lanes, each voting N of its
+// entries (the caller counts the segment's votes once).
+typedef __attribute__((address_space(3))) uint32_t lds_u32;
+template <bool RET, int N>
+__device__ __forceinline__ int vote_run_packed(const VoteArgs &a, const SegRun &g,
+                                               const uint2 (&v)[N], uint32_t &ovf) {
+    lds_u32    *A   = (lds_u32 *)(size_t)0;
     const float off = g.e0 >= g.tailStart ? g.offTail : g.offBody;
 #pragma unroll
-    for (int kk = 0; kk < SEG; kk++) {
+    for (int kk = 0; kk < N; kk++) {
         const uint32_t bin =
             (uint32_t)ppfc::bin_wrapped_fast(__uint_as_float(v[kk].y), off, a.sMaxId, a.maxBin);
         const uint32_t cell = v[kk].x + bin;
         const uint32_t inc  = 1u << ((cell & 1u) << 4);
+        lds_u32       *w    = A + ((cell >> 1) & 0x3FFFFu);
         if (RET)
-            ovf |= atomicAdd(&acc[(cell >> 1) & 0x3FFFFu], inc);
+            ovf |= __hip_atomic_fetch_add(w, inc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         else
-            atomicAdd(&acc[(cell >> 1) & 0x3FFFFu], inc);
+            __hip_atomic_fetch_add(w, inc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     }
     return g.vhi - g.vlo;
 }
@@ -676,7 +697,8 @@ __global__ void __launch_bounds__(BLOCK) vote_kernel(VoteArgs a) {
 // the feature sequence runs on a fraction of them.  The features are the
 // canonical ones (ppfc::ppf_key's operations, in pieces).
 constexpr int PAIR_BLOCK = 256;
-constexpr int HITS_INFLIGHT = 2;  // segments whose loads a voting lane issues together
+constexpr int HITS_INFLIGHT      = 2;  // segments whose loads a voting lane issues together
+constexpr int VOTE_LANES_PER_SEG = 2;  // lanes sharing one 8-entry segment in packed voting
 
 // quantized feature values packed 10 bits each; 1023 stands for the NaN
 // code 0x80000000 (finite values stay below 1000 when the pre-filters are on)
@@ -960,20 +982,39 @@ __global__ void __launch_bounds__(PAIR_BLOCK) pair_kernel(VoteArgs a) {
 // ---- split path, stage 2: vote the hits of one key point (and tile) into
 // the LDS accumulator, all lanes striped over the buckets' segments, then
 // the peaks.
+// The kernel's LDS is one dynamic block: the accumulator at offset 0 (so
+// the voting loop's LDS addresses need no base added), then its own arrays
+// (HitsLds).  The current batch of hits: with one tile (PACKED) each bucket
+// is voted whole and a hit needs only {start, length, scene alpha}, and a
+// batch holds BLOCK / 2 hits (2.2 KB at 256 threads, which leaves room for
+// three workgroups per CU beside the 50 KB accumulator of an 832-point
+// model; LDS is allocated in 512-byte granules); with reference tiles
+// {lo, hi, tail start, body offset, tail offset} of the tile's slice, BLOCK
+// hits a batch.
+__host__ __device__ inline int hits_acc_words(int tileRefs, int angleNum, bool packed) {
+    const int cells = (tileRefs + 1) * angleNum;  // + the spare row
+    const int w     = packed ? (cells + 1) / 2 : cells;
+    return (w + 1) & ~1;  // 8-byte alignment for what follows
+}
+template <int BLOCK, bool PACKED> struct HitsLds {
+    static constexpr int HB = PACKED ? BLOCK / 2 : BLOCK;
+    static constexpr int HW = PACKED ? 3 : 5;
+    // words after the accumulator: sTop (u64), sOvf, sScan, sPre, sHit
+    static constexpr int words = 2 * (BLOCK / 64) * 3 + 1 + (BLOCK / 64 + 1) + HB + HW * HB;
+};
+
 template <int BLOCK, bool PACKED>
 __global__ void __launch_bounds__(BLOCK) vote_hits_kernel(VoteArgs a) {
     extern __shared__ uint32_t acc[];
-    __shared__ uint32_t sOvf;
-    __shared__ int      sPre[BLOCK];
-    // the current batch of hits: with one tile (PACKED) each bucket is
-    // voted whole and a hit needs only {start, length, scene alpha} (12 B;
-    // the static LDS then leaves room for three workgroups per CU beside a
-    // 50 KB accumulator); with reference tiles {lo, hi, tail start, body
-    // offset, tail offset} of the tile's slice
-    constexpr int       HW = PACKED ? 3 : 5;
-    __shared__ uint32_t sHit[HW * BLOCK];
-    __shared__ int      sScan[BLOCK / 64 + 1];
-    __shared__ unsigned long long sTop[(BLOCK / 64) * 3];
+    using LL                 = HitsLds<BLOCK, PACKED>;
+    constexpr int HB         = LL::HB;
+    constexpr int HW         = LL::HW;
+    uint32_t     *extra      = acc + hits_acc_words(a.tileRefs, a.angleNum, PACKED);
+    unsigned long long *sTop = reinterpret_cast<unsigned long long *>(extra);
+    uint32_t           &sOvf = extra[2 * (BLOCK / 64) * 3];
+    int *sScan               = reinterpret_cast<int *>(extra + 2 * (BLOCK / 64) * 3 + 1);
+    int *sPre                = sScan + (BLOCK / 64 + 1);
+    uint32_t *sHit           = reinterpret_cast<uint32_t *>(sPre + HB);
 
     const int tid = threadIdx.x;
     int       tile;
@@ -1008,12 +1049,12 @@ __global__ void __launch_bounds__(BLOCK) vote_hits_kernel(VoteArgs a) {
     // 0xFFFF) needs no overflow test
     auto voteAll = [&](auto retTag) {
         constexpr bool RET = decltype(retTag)::value;
-        for (int h0 = 0; h0 < n; h0 += BLOCK) {
+        for (int h0 = 0; h0 < n; h0 += HB) {
             // this tile's entries of each hit's bucket: the whole bucket, or with
             // ref-sorted buckets only the slice holding the tile's references
             const int i   = h0 + tid;
             int       cnt = 0;
-            if (i < n) {
+            if (tid < HB && i < n) {
                 const uint32_t st = a.hitStart[base + i], L = a.hitLen[base + i];
                 uint32_t       lo = st, hi = st + L;
                 if (a.splits) {
@@ -1028,19 +1069,20 @@ __global__ void __launch_bounds__(BLOCK) vote_hits_kernel(VoteArgs a) {
                 const float aS = a.hitAlpha[base + i];
                 if (PACKED) {
                     sHit[tid]             = st;
-                    sHit[BLOCK + tid]     = L;
-                    sHit[2 * BLOCK + tid] = __float_as_uint(aS);
+                    sHit[HB + tid]     = L;
+                    sHit[2 * HB + tid] = __float_as_uint(aS);
                 } else {
                     sHit[tid]             = lo;
-                    sHit[BLOCK + tid]     = hi;
-                    sHit[2 * BLOCK + tid] = st + L - (L & 7u);
-                    sHit[3 * BLOCK + tid] = __float_as_uint(ppfc::bin_offset_body(aS));
-                    sHit[4 * BLOCK + tid] = __float_as_uint(ppfc::bin_offset_tail(aS));
+                    sHit[HB + tid]     = hi;
+                    sHit[2 * HB + tid] = st + L - (L & 7u);
+                    sHit[3 * HB + tid] = __float_as_uint(ppfc::bin_offset_body(aS));
+                    sHit[4 * HB + tid] = __float_as_uint(ppfc::bin_offset_tail(aS));
                 }
             }
             int       totSeg;
             const int pre = block_excl_scan<BLOCK>(cnt, sScan, totSeg);
-            sPre[tid]     = pre;
+            if (tid < HB)
+                sPre[tid] = pre;
             __syncthreads();
             // each thread votes a contiguous run of segments (SEG entries of one
             // bucket each; different lanes, different buckets, so the LDS atomics
@@ -1049,9 +1091,16 @@ __global__ void __launch_bounds__(BLOCK) vote_hits_kernel(VoteArgs a) {
             // loads of HITS_INFLIGHT segments are issued before any of them is
             // voted (the accumulator leaves room for two workgroups of waves per
             // CU at most, so each lane keeps several loads in flight instead)
-            const int nb  = min(BLOCK, n - h0);
-            const int spt = (totSeg + BLOCK - 1) / BLOCK;
-            const int t0  = tid * spt, t1 = min(t0 + spt, totSeg);
+            const int nb  = min(HB, n - h0);
+            // packed: LPS lanes share a segment, each loading and voting EPL
+            // of its entries, so one load instruction of a wave touches
+            // 64 / LPS segments (64-byte lines) instead of 64: the L1's tag
+            // lookups, not the ALUs, bound this loop
+            constexpr int LPS = PACKED ? VOTE_LANES_PER_SEG : 1;
+            constexpr int EPL = SEG / LPS;
+            const int     grp = tid / LPS, part = tid % LPS;
+            const int spt = (totSeg + BLOCK / LPS - 1) / (BLOCK / LPS);
+            const int t0  = grp * spt, t1 = min(t0 + spt, totSeg);
             const int r0A = r0 * a.angleNum, r1A = r1 * a.angleNum;
             int       nv  = 0;
             if (t0 < t1) {
@@ -1072,8 +1121,8 @@ __global__ void __launch_bounds__(BLOCK) vote_hits_kernel(VoteArgs a) {
                 float    hOffB, hOffT;
                 auto     loadHit = [&](int q) {
                     if (PACKED) {
-                        const uint32_t st = sHit[q], L = sHit[BLOCK + q];
-                        const float    aS = __uint_as_float(sHit[2 * BLOCK + q]);
+                        const uint32_t st = sHit[q], L = sHit[HB + q];
+                        const float    aS = __uint_as_float(sHit[2 * HB + q]);
                         hLo                = st;
                         hHi                = st + L;
                         hTail              = st + L - (L & 7u);
@@ -1081,16 +1130,16 @@ __global__ void __launch_bounds__(BLOCK) vote_hits_kernel(VoteArgs a) {
                         hOffT              = ppfc::bin_offset_tail(aS);
                     } else {
                         hLo   = sHit[q];
-                        hHi   = sHit[BLOCK + q];
-                        hTail = sHit[2 * BLOCK + q];
-                        hOffB = __uint_as_float(sHit[3 * BLOCK + q]);
-                        hOffT = __uint_as_float(sHit[4 * BLOCK + q]);
+                        hHi   = sHit[HB + q];
+                        hTail = sHit[2 * HB + q];
+                        hOffB = __uint_as_float(sHit[3 * HB + q]);
+                        hOffT = __uint_as_float(sHit[4 * HB + q]);
                     }
                 };
                 loadHit(h);
                 for (int t = t0; t < t1; t += HITS_INFLIGHT) {
                     SegRun g[HITS_INFLIGHT];
-                    uint2  ent[HITS_INFLIGHT][SEG];
+                    uint2  ent[HITS_INFLIGHT][EPL];
 #pragma unroll
                     for (int j = 0; j < HITS_INFLIGHT; j++) {
                         const int tq = t + j;
@@ -1114,15 +1163,16 @@ __global__ void __launch_bounds__(BLOCK) vote_hits_kernel(VoteArgs a) {
                             g[j].tailStart = hTail;
                             g[j].offBody   = hOffB;
                             g[j].offTail   = hOffT;
-                            load_segment(a, g[j].e0, ent[j]);
+                            load_entries<EPL>(a, g[j].e0 + (uint32_t)(part * EPL), ent[j]);
                         }
                     }
 #pragma unroll
                     for (int j = 0; j < HITS_INFLIGHT; j++)
                         if (t + j < t1) {
-                            if (PACKED)
-                                nv += vote_run_packed<RET>(a, acc, g[j], ent[j], ovf);
-                            else
+                            if constexpr (PACKED) {
+                                const int c = vote_run_packed<RET, EPL>(a, g[j], ent[j], ovf);
+                                nv += part == 0 ? c : 0;
+                            } else
                                 nv += a.nTiles > 1
                                           ? vote_run_loaded<true>(a, acc, r0A, r1A, g[j], ent[j])
                                           : vote_run_loaded<false>(a, acc, r0A, r1A, g[j], ent[j]);
@@ -1270,8 +1320,19 @@ __global__ void bucket_split_kernel(const uint2 *buckets, int K, const uint2 *ve
 constexpr bool VOTE_PACKED = true;
 
 template <int BLOCK>
-static void launch_vote_hits(const VoteArgs &a, int blocks, size_t lds, bool packed,
-                             hipStream_t s) {
+static void launch_vote_hits(const VoteArgs &a, int blocks, bool packed, hipStream_t s) {
+    const size_t lds =
+        4 * ((size_t)hits_acc_words(a.tileRefs, a.angleNum, packed) +
+             (packed ? HitsLds<BLOCK, true>::words : HitsLds<BLOCK, false>::words));
+    // the packed voting addresses its accumulator from LDS address 0: the
+    // kernel must have no static LDS ahead of its dynamic block
+    static const bool noStatic = [] {
+        hipFuncAttributes at{};
+        PPF_HIP_CHECK(hipFuncGetAttributes(&at, (const void *)vote_hits_kernel<BLOCK, true>));
+        return at.sharedSizeBytes == 0;
+    }();
+    if (packed && !noStatic)
+        throw HipError("vote_hits_kernel has static LDS ahead of its accumulator");
     if (packed) {
         PPF_HIP_CHECK(hipFuncSetAttribute((const void *)vote_hits_kernel<BLOCK, true>,
                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
@@ -1501,12 +1562,11 @@ int vote_keypoints(Workspace &ws, const DevModel &m, const VoteKeypoints &in, fl
         PPF_HIP_CHECK(hipGetLastError());
         PPF_HIP_CHECK(hipEventRecord(ws.voteEv[2], s));
         ws.pairQueued.store(1, std::memory_order_release);
-        const bool   packed = VOTE_PACKED && nTiles == 1;
-        const size_t ldsH   = packed ? ((size_t)(tileRefs + 1) * m.angleNum + 1) / 2 * 4 : lds;
+        const bool packed = VOTE_PACKED && nTiles == 1;
         switch (cfg.block) {
-            case 256: launch_vote_hits<256>(a, blocks, ldsH, packed, s); break;
-            case 512: launch_vote_hits<512>(a, blocks, ldsH, packed, s); break;
-            default: launch_vote_hits<1024>(a, blocks, ldsH, packed, s); break;
+            case 256: launch_vote_hits<256>(a, blocks, packed, s); break;
+            case 512: launch_vote_hits<512>(a, blocks, packed, s); break;
+            default: launch_vote_hits<1024>(a, blocks, packed, s); break;
         }
         PPF_HIP_CHECK(hipGetLastError());
         PPF_HIP_CHECK(hipMemcpyAsync(&nOverflow, a.nOverflow, sizeof(int), hipMemcpyDeviceToHost, s));
