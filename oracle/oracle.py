@@ -429,3 +429,69 @@ def match(model: Model, pts, nrm=None, rel=0.04, kp_frac=0.1, min_score=0.2, par
     if n < 0:
         raise ValueError("orc_match failed")
     return poses[:n].reshape(-1, 3, 4), scores[:n], st
+
+
+# ---- the CPU baseline's timing mode (oracle/cpu_ref.cpp, built into
+# oracle/_ref by `make -C oracle ref` from the reference tree)
+CPUREF_STAGES = ("scene remove nan", "scene kdtree", "scene sample1", "scene compute normal",
+                 "scene sample2", "scene ppf", "cluster", "icp prepare", "icp", "total")
+
+
+class CpuRefStats(C.Structure):
+    _fields_ = [("ms", C.c_float * 10), ("nSampled", C.c_int), ("nKeypoints", C.c_int),
+                ("nVotedKeypoints", C.c_int), ("nPoses", C.c_int), ("nResults", C.c_int),
+                ("threads", C.c_int), ("nPairs", C.c_long), ("nVotes", C.c_long)]
+
+    def as_dict(self):
+        d = {k: getattr(self, k) for k, _ in self._fields_ if k != "ms"}
+        d["stage_ms"] = {name: round(float(self.ms[i]), 2) for i, name in enumerate(CPUREF_STAGES)}
+        return d
+
+
+_cpuref = None
+
+
+def cpuref_lib():
+    global _cpuref
+    if _cpuref is None:
+        path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_ref", "libcpu_ref.so")
+        if not os.path.exists(path):
+            raise ImportError(f"{path} is missing: build it with `make -C oracle ref`")
+        L = C.CDLL(path)
+        L.cpuref_model.restype = C.c_void_p
+        L.cpuref_model.argtypes = [C.POINTER(Model)]
+        L.cpuref_model_free.argtypes = [C.c_void_p]
+        L.cpuref_match.restype = C.c_int
+        L.cpuref_match.argtypes = [C.c_void_p, C.c_int, fp, C.c_void_p, C.c_float, C.c_float,
+                                   C.c_float, C.POINTER(MatchParam), C.c_int, C.c_int, fp, fp,
+                                   C.c_int, C.POINTER(CpuRefStats)]
+        _cpuref = L
+    return _cpuref
+
+
+class CpuRefModel:
+    """A trained oracle model prepared for the timing mode (hash map,
+    frames); keeps the oracle model alive."""
+
+    def __init__(self, model: Model):
+        self.model = model
+        self.h = cpuref_lib().cpuref_model(C.byref(model))
+
+    def match(self, pts, rel=0.04, kp_frac=0.1, min_score=0.2, param=None, vp=None,
+              threads=1, kp_stride=1, cap=64):
+        param = param or default_match_param()
+        pts = f32(pts)
+        vpa = None if vp is None else f32(vp)
+        poses = np.zeros((cap, 12), np.float32)
+        scores = np.zeros(cap, np.float32)
+        st = CpuRefStats()
+        n = cpuref_lib().cpuref_match(self.h, len(pts), pts, _ptr(vpa), rel, kp_frac, min_score,
+                                      C.byref(param), threads, kp_stride, poses, scores, cap,
+                                      C.byref(st))
+        k = min(n, cap)
+        return poses[:k].reshape(-1, 3, 4), scores[:k], st.as_dict()
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            cpuref_lib().cpuref_model_free(self.h)
+            self.h = None
