@@ -256,6 +256,20 @@ PPF_API int ppf_vote(ppf_detector d, const float *pts, const float *nrm, size_t 
 /* voxel-grid downsampling (replaces samplePointCloud, util.cpp:107-139) */
 PPF_API int ppf_voxel_sample(const float *pts, size_t n, const int *subset, size_t nSubset,
                              float step, const float boxMin[3], int *out, size_t *nOut);
+/* ppf_voxel_sample without any allocation (no reference counterpart: for
+ * callers that cannot allocate device memory -- code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  x, y, z:
+ * device arrays of n points; subset (device, nullable) selects nSubset of
+ * them; out (device, room for every selected point) receives the sampled
+ * ids in the order of ppf_voxel_sample, *count (device) their number.  The
+ * work is queued on stream (a hipStream_t; NULL = the default stream) and
+ * not waited for.  At most 1048576 selected points (PPF_ERR_RANGE beyond):
+ * the voxel table lives in a fixed 33 MiB device array that every GPU the
+ * library runs on holds until the process ends, shared by all calls on
+ * that GPU -- calls that overlap on one GPU race. */
+PPF_API int ppf_voxel_sample_noalloc(const float *x, const float *y, const float *z, size_t n,
+                                     const int *subset, size_t nSubset, float step,
+                                     const float boxMin[3], int *out, int *count, void *stream);
 /* exact radius search: per query the ids with dist2 < r2, ascending
  * (RadiusResultSet, nanoflann.hpp:364-418); counts[nq], ids packed in query
  * order up to cap */

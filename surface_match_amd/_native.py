@@ -104,6 +104,7 @@ EXPORTS = [
     "ppf_scene_inliers", "ppf_measure_ceilings", "ppf_ply_read", "ppf_ply_info", "ppf_ply_copy",
     "ppf_ply_free", "ppf_ply_write", "ppf_scene_upload_ply", "ppf_train_model_mesh",
     "ppf_sample_mesh", "ppf_estimate_normals_mls", "ppf_scene_download",
+    "ppf_voxel_sample_noalloc",
 ]
 
 
@@ -140,6 +141,7 @@ def lib() -> C.CDLL:
         "ppf_pair_features": (i, [vp, vp, vp, sz, f, f, i, vp, vp]),
         "ppf_vote": (i, [vp, vp, vp, sz, vp, sz, f, vp, vp, sz, vp, vp]),
         "ppf_voxel_sample": (i, [vp, sz, vp, sz, f, vp, vp, C.POINTER(sz)]),
+        "ppf_voxel_sample_noalloc": (i, [vp, vp, vp, sz, vp, sz, f, vp, vp, vp, vp]),
         "ppf_radius_search": (i, [vp, sz, vp, sz, f, f, vp, vp, sz]),
         "ppf_knn_search": (i, [vp, sz, vp, sz, i, f, vp, vp]),
         "ppf_estimate_normals": (i, [vp, vp, sz, vp, sz, vp, sz, i, i, i, vp]),

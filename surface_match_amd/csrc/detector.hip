@@ -1539,6 +1539,39 @@ struct TmpCloud {
 
 }  // namespace
 
+// ---------------------------------------------------------------- allocation-free voxel sampling
+//
+// ppf_voxel_sample_noalloc's scratch: the voxel table, flags and block
+// counts of up to kNoAllocMaxPoints points, in a fixed device array that
+// each GPU holds from module load to process end.  Host code never passes
+// the array's name to a kernel (in host code it is a stand-in, not a device
+// address): the device address comes from hipGetSymbolAddress, looked up
+// once per device with that device current, and only that address is used.
+
+constexpr int    kNoAllocMaxPoints = 1 << 20;
+constexpr size_t kNoAllocScratch   = 34619392;  // voxel_scratch_bytes(kNoAllocMaxPoints)
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratch];
+
+namespace {
+constexpr int  kMaxDevices = 64;
+std::mutex     g_scratchMu;
+unsigned char *g_scratchPtr[kMaxDevices] = {};
+
+unsigned char *scratch_ptr() {
+    int dev = 0;
+    PPF_HIP_CHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices)
+        throw HipError("ppf_voxel_sample_noalloc: device index beyond the library's table");
+    std::lock_guard<std::mutex> g(g_scratchMu);
+    if (!g_scratchPtr[dev]) {
+        void *p = nullptr;
+        PPF_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratchPtr[dev] = static_cast<unsigned char *>(p);
+    }
+    return g_scratchPtr[dev];
+}
+}  // namespace
+
 // ======================================================================== C ABI
 
 extern "C" {
@@ -2158,6 +2191,24 @@ PPF_API int ppf_voxel_sample(const float *pts, size_t n, const int *subset, size
         PPF_HIP_CHECK(hipMemcpyAsync(out, od, sizeof(int) * c, hipMemcpyDeviceToHost, s));
         PPF_HIP_CHECK(hipStreamSynchronize(s));
         *nOut = (size_t)c;
+    });
+}
+
+PPF_API int ppf_voxel_sample_noalloc(const float *x, const float *y, const float *z, size_t n,
+                                     const int *subset, size_t nSubset, float step,
+                                     const float boxMin[3], int *out, int *count, void *stream) {
+    PPF_REQUIRE(x && y && z && boxMin && out && count && (subset || nSubset == 0),
+                "null argument");
+    return guarded([&] {
+        const size_t m = subset ? nSubset : n;
+        if (m > (size_t)kNoAllocMaxPoints)
+            throw std::range_error(
+                "ppf_voxel_sample_noalloc: at most 1048576 points (its fixed scratch); use "
+                "ppf_voxel_sample for more");
+        if (voxel_scratch_bytes((int)m) > kNoAllocScratch)
+            throw std::range_error("ppf_voxel_sample_noalloc: scratch too small");
+        voxel_sample_into(x, y, z, subset, (int)m, step, boxMin, out, count, scratch_ptr(),
+                          static_cast<hipStream_t>(stream));
     });
 }
 

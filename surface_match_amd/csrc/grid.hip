@@ -410,6 +410,101 @@ int voxel_sample(Workspace &ws, const float *X, const float *Y, const float *Z,
     return cnt;
 }
 
+// ---- voxel sampling into caller-provided scratch: no allocation, no
+// memset, no host synchronisation (the same kernels as voxel_sample; the
+// order-keeping compaction is the library's own: per-block counts, one
+// workgroup's scan, a scatter)
+namespace {
+constexpr int VS_BLOCK = 256;
+
+__global__ void vs_clear_kernel(unsigned long long *tkey, unsigned long long *tbest, uint32_t n) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        tkey[i]  = 0ull;
+        tbest[i] = ~0ull;
+    }
+}
+
+__global__ void __launch_bounds__(VS_BLOCK) vs_count_kernel(const char *flags, int m, int *blockCnt) {
+    __shared__ int sScan[VS_BLOCK / 64 + 1];
+    const int      t = blockIdx.x * VS_BLOCK + threadIdx.x;
+    int            total;
+    block_excl_scan<VS_BLOCK>(t < m ? (int)flags[t] : 0, sScan, total);
+    if (threadIdx.x == 0)
+        blockCnt[blockIdx.x] = total;
+}
+
+// one workgroup: exclusive scan of the block counts; the total to *count
+__global__ void __launch_bounds__(1024) vs_scan_kernel(int *blockCnt, int nb, int *count) {
+    __shared__ int sScan[1024 / 64 + 1];
+    int            carry = 0;
+    for (int base = 0; base < nb; base += 1024) {
+        const int b = base + threadIdx.x;
+        const int v = b < nb ? blockCnt[b] : 0;
+        int       total;
+        const int pre = block_excl_scan<1024>(v, sScan, total);
+        if (b < nb)
+            blockCnt[b] = carry + pre;
+        carry += total;
+    }
+    if (threadIdx.x == 0)
+        *count = carry;
+}
+
+__global__ void __launch_bounds__(VS_BLOCK)
+    vs_scatter_kernel(const char *flags, const int *subset, int m, const int *blockOff, int *out) {
+    __shared__ int sScan[VS_BLOCK / 64 + 1];
+    const int      t = blockIdx.x * VS_BLOCK + threadIdx.x;
+    const int      f = t < m ? (int)flags[t] : 0;
+    int            total;
+    const int      pre = block_excl_scan<VS_BLOCK>(f, sScan, total);
+    if (f)
+        out[blockOff[blockIdx.x] + pre] = subset ? subset[t] : t;
+}
+}  // namespace
+
+size_t voxel_scratch_bytes(int m) {
+    uint32_t hsize = 64;
+    while (hsize < 2u * (uint32_t)std::max(m, 1))
+        hsize <<= 1;
+    const size_t nb = (size_t)div_up(std::max(m, 1), VS_BLOCK);
+    return 16 * (size_t)hsize + (((size_t)std::max(m, 1) + 255) & ~(size_t)255) + 4 * nb;
+}
+
+void voxel_sample_into(const float *X, const float *Y, const float *Z, const int *subset, int m,
+                       float step, const float bmin[3], int *out, int *countDev,
+                       unsigned char *scratch, hipStream_t s) {
+    uint32_t hsize = 64;
+    while (hsize < 2u * (uint32_t)std::max(m, 1))
+        hsize <<= 1;
+    const int nb = div_up(std::max(m, 1), VS_BLOCK);
+    VoxArgs   a;
+    a.X      = X;
+    a.Y      = Y;
+    a.Z      = Z;
+    a.subset = subset;
+    a.m      = m;
+    a.step   = step;
+    for (int k = 0; k < 3; k++)
+        a.bmin[k] = bmin[k];
+    // scratch: keys, best codes, per-point flags, per-block counts
+    a.tkey          = reinterpret_cast<unsigned long long *>(scratch);
+    a.tbest         = a.tkey + hsize;
+    char *flags     = reinterpret_cast<char *>(a.tbest + hsize);
+    int  *blockCnt  = reinterpret_cast<int *>(flags + (((size_t)std::max(m, 1) + 255) & ~(size_t)255));
+    a.hmask         = hsize - 1;
+    vs_clear_kernel<<<std::min<uint32_t>(div_up(hsize, 256), 2048), 256, 0, s>>>(a.tkey, a.tbest,
+                                                                                  hsize);
+    if (m > 0) {
+        voxel_insert_kernel<<<div_up(m, 256), 256, 0, s>>>(a);
+        voxel_flag_kernel<<<div_up(m, 256), 256, 0, s>>>(a, flags);
+        vs_count_kernel<<<nb, VS_BLOCK, 0, s>>>(flags, m, blockCnt);
+    }
+    vs_scan_kernel<<<1, 1024, 0, s>>>(blockCnt, m > 0 ? nb : 0, countDev);
+    if (m > 0)
+        vs_scatter_kernel<<<nb, VS_BLOCK, 0, s>>>(flags, subset, m, blockCnt, out);
+    PPF_HIP_CHECK(hipGetLastError());
+}
+
 // ---------------------------------------------------------------- normals
 
 constexpr int KNN_MAX = 32;

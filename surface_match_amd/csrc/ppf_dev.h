@@ -139,6 +139,13 @@ void device_bbox(const float *X, const float *Y, const float *Z, const int *subs
 int  voxel_sample(Workspace &ws, const float *X, const float *Y, const float *Z,
                   const int *subset, int m, float step, const float bmin[3], int *out,
                   int *countDev, hipStream_t s);
+// voxel_sample without allocation, memset or host synchronisation: scratch
+// (device, at least voxel_scratch_bytes(m)) holds the voxel table, flags and
+// block counts; the count goes to countDev.  Same output as voxel_sample.
+size_t voxel_scratch_bytes(int m);
+void   voxel_sample_into(const float *X, const float *Y, const float *Z, const int *subset, int m,
+                         float step, const float bmin[3], int *out, int *countDev,
+                         unsigned char *scratch, hipStream_t s);
 void estimate_normals(Workspace &ws, int n, float *PX, float *PY, float *PZ, float *NX,
                       float *NY, float *NZ, const int *valid, int nValid, const int *query,
                       int nq, int k, int smooth, int invert, const float vp[3], float cellHint,
