@@ -52,6 +52,8 @@ def H(tmp_path_factory):
     L.ch_bin.argtypes = [C.c_float, C.c_float, C.c_int, C.c_int]
     L.ch_bin_wrapped.restype = C.c_int
     L.ch_bin_wrapped.argtypes = [C.c_float, C.c_float, C.c_int, C.c_int]
+    L.ch_libm_divergence.argtypes = [fp, fp, C.c_int, C.c_float, C.c_float, C.c_int,
+                                     np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")]
     L.ch_bin_wrapped_fast.restype = C.c_int
     L.ch_bin_wrapped_fast.argtypes = [C.c_float, C.c_float, C.c_int, C.c_int]
     L.ch_pose.argtypes = [fp, fp, C.c_float, C.c_float, fp]

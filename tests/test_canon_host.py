@@ -1,5 +1,6 @@
 """The device arithmetic header (canon.h) compiled for the host agrees bit for
 bit with the oracle -- the same FP32 sequences, written twice.  CPU only."""
+import ctypes as C
 import math
 
 import numpy as np
@@ -75,3 +76,30 @@ def test_branch_free_bin_equals_both_rules(H, oracle_mod):
                 assert H.ch_bin_wrapped(x, y, n, tail) == ref, (x, y, n, tail)
                 # the voting loop's form (added wrap constant, float clamp)
                 assert H.ch_bin_wrapped_fast(x, y, n, tail) == ref, (x, y, n, tail)
+
+
+def test_libm_atan2f_divergence(H, oracle_mod):
+    """How often glibc's atan2f (the reference's scalar-tail path,
+    util.cpp:698,712-730, and computeAlpha's scalar form, :477-484) would
+    change a key, an alpha or an angle bin against the canonical atan2f,
+    over every ordered pair of the config-1 model's sampled points (the
+    pairs trainModel hashes).  Reported, with a sanity bound: the two
+    functions differ by at most an ulp or two, so a key changes only when a
+    feature sits within an ulp of a quantisation step."""
+    O = oracle_mod
+    from surface_match_amd import synth
+    mp, mn = synth.workload(1)["model"]
+    m = O.train(mp, mn, 0.04)
+    sp, sn = m.sampled()
+    D = float(np.linalg.norm(np.asarray(m.boxMax) - np.asarray(m.boxMin)))
+    step_a = np.float32(2.0) * np.float32(math.pi) / np.float32(30)
+    step_d = np.float32(D) * np.float32(0.04)
+    for tail in (0, 1):
+        out = np.zeros(4, np.int64)
+        H.ch_libm_divergence(np.ascontiguousarray(sp), np.ascontiguousarray(sn), len(sp), step_a,
+                             step_d, tail, out)
+        pairs, keys, alphas, bins = (int(x) for x in out)
+        print(f"tail={tail}: {pairs} pairs; keys differ {keys} ({keys / pairs:.2e}), alphas "
+              f"{alphas} ({alphas / pairs:.2e}), bins {bins} ({bins / pairs:.2e})")
+        assert keys / pairs < 5e-3 and bins / pairs < 5e-3
+    O.lib().orc_free_model(C.byref(m))
