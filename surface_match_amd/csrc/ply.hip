@@ -139,7 +139,9 @@ void read_header(PlyFile &f) {
             const std::string n = f.token();
             char             *end = nullptr;
             e.count           = std::strtol(n.c_str(), &end, 10);
-            if (n.empty() || end == n.c_str())
+            // a negative count would size the reference's arrays from a
+            // negative long (filePLY.cpp:211-216); refused here
+            if (n.empty() || end == n.c_str() || e.count < 0)
                 ply_fail("unable to parse header.");
             t = f.token();
             while (t == "property" || t == "comment" || t == "obj_info") {
@@ -365,7 +367,10 @@ void ply_read(const char *path, PlyCloud &out) {
                 off[q] = stride;
                 stride += (size_t)ply_type_size(e.props[q].type);
             }
-            if (f.pos + stride * (size_t)e.count > f.buf.size() - 1)
+            // the rows must fit in what is left of the file (compared by
+            // division: stride * count may not fit a size_t)
+            const size_t left = f.buf.size() - 1 > f.pos ? f.buf.size() - 1 - f.pos : 0;
+            if (stride > 0 && (size_t)e.count > left / stride)
                 ply_fail("unable to read file.");
             if (isVertex) {
                 for (long i = 0; i < e.count; i++) {

@@ -263,3 +263,19 @@ def test_ingest_speed_against_rply(R, tmp_path):
         mb = path.stat().st_size / 1e6
         print(f"{'ascii' if ascii else 'binary'} {n} points, {mb:.1f} MB: library "
               f"{mb / (t1 - t0):.0f} MB/s, rply path {mb / (t2 - t1):.0f} MB/s")
+
+
+@pytest.mark.parametrize("case", ["negative", "huge_binary", "huge_ascii"])
+def test_bad_element_counts_are_refused(tmp_path, case):
+    """Element counts the file cannot hold: a negative count, and binary /
+    ascii counts near 2^63 whose byte size would wrap a size_t.  The library
+    refuses them with readPLY's failure (false), never reading past the
+    buffer."""
+    path = tmp_path / "bad.ply"
+    rows = cloud_rows(20, 16, normals=False)
+    vp = [("float", "x"), ("float", "y"), ("float", "z")]
+    fmt = "ascii" if case != "huge_binary" else "binary_little_endian"
+    write_ply(path, fmt, vp, rows)
+    count = b"-5" if case == "negative" else b"4611686018427387904"
+    path.write_bytes(path.read_bytes().replace(b"element vertex 20", b"element vertex " + count))
+    assert lib_read(path) is None
