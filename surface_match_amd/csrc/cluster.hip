@@ -12,7 +12,8 @@
 // same structure and is resolved the same way.
 #include "ppf_dev.h"
 
-#include <hipcub/hipcub.hpp>
+#include <cstring>
+#include <rocprim/rocprim.hpp>
 
 #include <algorithm>
 
@@ -383,9 +384,8 @@ __global__ void hypo_keys_kernel(const Hypo *h, int n, uint32_t *keys, int *vals
 static void sort_pairs(Workspace &ws, uint32_t *k, uint32_t *k2, int *v, int *v2, int n,
                        hipStream_t s) {
     size_t tb = 0;
-    PPF_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, k, k2, v, v2, n, 0, 32, s));
-    PPF_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(ws.cubTmp.reserve(tb), tb, k, k2, v, v2, n, 0,
-                                                     32, s));
+    PPF_HIP_CHECK(rocprim::radix_sort_pairs(nullptr, tb, k, k2, v, v2, n, 0, 32, s));
+    PPF_HIP_CHECK(rocprim::radix_sort_pairs(ws.cubTmp.reserve(tb), tb, k, k2, v, v2, n, 0, 32, s));
 }
 
 void cluster_poses(Workspace &ws, const PoseRec *poses, int np, float dThr, float aThr,
@@ -420,8 +420,8 @@ void cluster_poses(Workspace &ws, const PoseRec *poses, int np, float dThr, floa
     cluster_rounds_kernel<<<1, 1024, 0, s>>>(sp, np, ws.gNormal.view, dThr, aThr, status, misc);
     center_rank_kernel<<<div_up(np, 256), 256, 0, s>>>(status, np, isC);
     size_t tb = 0;
-    PPF_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, isC, rank, np, s));
-    PPF_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(ws.cubTmp.reserve(tb), tb, isC, rank, np, s));
+    PPF_HIP_CHECK(rocprim::exclusive_scan(nullptr, tb, isC, rank, 0, np, rocprim::plus<int>(), s));
+    PPF_HIP_CHECK(rocprim::exclusive_scan(ws.cubTmp.reserve(tb), tb, isC, rank, 0, np, rocprim::plus<int>(), s));
     int lastRank = 0, lastIs = 0;
     PPF_HIP_CHECK(hipMemcpyAsync(&lastRank, rank + np - 1, sizeof(int), hipMemcpyDeviceToHost, s));
     PPF_HIP_CHECK(hipMemcpyAsync(&lastIs, isC + np - 1, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -435,23 +435,20 @@ void cluster_poses(Workspace &ws, const PoseRec *poses, int np, float dThr, floa
     while ((1 << endBit) < nc + 1)
         endBit++;
     tb = 0;
-    PPF_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, k1, k3, v1, members, np, 0,
-                                                     endBit, s));
+    PPF_HIP_CHECK(rocprim::radix_sort_pairs(nullptr, tb, k1, k3, v1, members, np, 0, endBit, s));
     int *mOff    = members + np;
-    PPF_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(ws.cubTmp.reserve(tb), tb, k1, k3, v1, members,
-                                                     np, 0, endBit, s));
+    PPF_HIP_CHECK(rocprim::radix_sort_pairs(ws.cubTmp.reserve(tb), tb, k1, k3, v1, members, np, 0, endBit, s));
     // cluster offsets: counts by run-length of k3 (clusters are 0..nc-1, all non-empty)
     int *cnts = cntsBuf;
     uint32_t *uk = reinterpret_cast<uint32_t *>(k1);
     int      *nr = misc + 1;
     tb = 0;
-    PPF_HIP_CHECK(hipcub::DeviceRunLengthEncode::Encode(nullptr, tb, k3, uk, cnts, nr, np, s));
-    PPF_HIP_CHECK(hipcub::DeviceRunLengthEncode::Encode(ws.cubTmp.reserve(tb), tb, k3, uk, cnts,
-                                                        nr, np, s));
+    PPF_HIP_CHECK(rocprim::run_length_encode(nullptr, tb, k3, np, uk, cnts, nr, s));
+    PPF_HIP_CHECK(rocprim::run_length_encode(ws.cubTmp.reserve(tb), tb, k3, np, uk, cnts, nr, s));
     PPF_HIP_CHECK(hipMemsetAsync(cnts + nc, 0, sizeof(int), s));
     tb = 0;
-    PPF_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, cnts, mOff, nc + 1, s));
-    PPF_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(ws.cubTmp.reserve(tb), tb, cnts, mOff, nc + 1, s));
+    PPF_HIP_CHECK(rocprim::exclusive_scan(nullptr, tb, cnts, mOff, 0, nc + 1, rocprim::plus<int>(), s));
+    PPF_HIP_CHECK(rocprim::exclusive_scan(ws.cubTmp.reserve(tb), tb, cnts, mOff, 0, nc + 1, rocprim::plus<int>(), s));
     // v2 still holds ord (the second sort wrote to `members`)
     float *angles  = buf + (size_t)np * 5;
     cluster_avg_kernel<<<div_up(nc, 64), 64, 0, s>>>(poses, ord, members, mOff, nc, centers,

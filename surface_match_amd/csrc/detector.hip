@@ -10,7 +10,6 @@
 #include "ppf_dev.h"
 #include "../../include/ppf_c.h"
 
-#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 #include <atomic>

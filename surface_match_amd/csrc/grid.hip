@@ -7,7 +7,8 @@
 // all write-backs happen afterwards).
 #include "ppf_dev.h"
 
-#include <hipcub/hipcub.hpp>
+#include <cstring>
+#include <rocprim/rocprim.hpp>
 
 #include <algorithm>
 #include <cmath>
@@ -241,11 +242,9 @@ void grid_build(Grid &g, const float *X, const float *Y, const float *Z, const f
         while ((1ull << endBit) < (unsigned long long)dims[0] * dims[1] * dims[2])
             endBit++;
         size_t tb = 0;
-        PPF_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, keys, keys2, vals, vals2, m,
-                                                         0, endBit, s));
+        PPF_HIP_CHECK(rocprim::radix_sort_pairs(nullptr, tb, keys, keys2, vals, vals2, m, 0, endBit, s));
         void *tmp = g.tmp.reserve(tb);
-        PPF_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(tmp, tb, keys, keys2, vals, vals2, m, 0,
-                                                         endBit, s));
+        PPF_HIP_CHECK(rocprim::radix_sort_pairs(tmp, tb, keys, keys2, vals, vals2, m, 0, endBit, s));
         if (dense) {
             cstart = g.cstart.as<int>(cells + 1);
             grid_cstart_kernel<<<div_up((int)cells + 1, 256), 256, 0, s>>>(keys2, m,
@@ -401,9 +400,9 @@ int voxel_sample(Workspace &ws, const float *X, const float *Y, const float *Z,
         voxel_flag_kernel<<<div_up(m, 256), 256, 0, s>>>(a, flags);
     }
     size_t tb = 0;
-    PPF_HIP_CHECK(hipcub::DeviceSelect::Flagged(nullptr, tb, src, flags, out, countDev, m, s));
+    PPF_HIP_CHECK(rocprim::select(nullptr, tb, src, flags, out, countDev, m, s));
     void *tmp = ws.cubTmp.reserve(tb);
-    PPF_HIP_CHECK(hipcub::DeviceSelect::Flagged(tmp, tb, src, flags, out, countDev, m, s));
+    PPF_HIP_CHECK(rocprim::select(tmp, tb, src, flags, out, countDev, m, s));
     int cnt = 0;
     PPF_HIP_CHECK(hipMemcpyAsync(&cnt, countDev, sizeof(int), hipMemcpyDeviceToHost, s));
     PPF_HIP_CHECK(hipStreamSynchronize(s));
@@ -936,9 +935,8 @@ void estimate_normals(Workspace &ws, int n, float *PX, float *PY, float *PZ, flo
             normals_order_flags_kernel<<<div_up(ng, 256), 256, 0, s>>>(a.g.id, ng, a.mark, want,
                                                                        flags);
         size_t tb = 0;
-        PPF_HIP_CHECK(hipcub::DeviceSelect::Flagged(nullptr, tb, a.g.id, flags, olist, cntD, ng, s));
-        PPF_HIP_CHECK(hipcub::DeviceSelect::Flagged(ws.cubTmp.reserve(tb), tb, a.g.id, flags,
-                                                    olist, cntD, ng, s));
+        PPF_HIP_CHECK(rocprim::select(nullptr, tb, a.g.id, flags, olist, cntD, ng, s));
+        PPF_HIP_CHECK(rocprim::select(ws.cubTmp.reserve(tb), tb, a.g.id, flags, olist, cntD, ng, s));
     };
 
     const bool small = k <= 16;  // kNN list bound: 16 or 32 entries

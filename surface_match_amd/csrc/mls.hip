@@ -26,7 +26,8 @@
 // the ids -> one thread per query for the fit.
 #include "ppf_dev.h"
 
-#include <hipcub/hipcub.hpp>
+#include <cstring>
+#include <rocprim/rocprim.hpp>
 
 #include <algorithm>
 #include <cmath>
@@ -324,9 +325,8 @@ void estimate_normals_mls(Workspace &ws, int n, const float *PX, const float *PY
                                                              v1);
         PPF_HIP_CHECK(hipGetLastError());
         size_t tb = 0;
-        PPF_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, k1, k2, v1, v2, nq, 0, 32, s));
-        PPF_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(ws.cubTmp.reserve(tb), tb, k1, k2, v1, v2,
-                                                         nq, 0, 32, s));
+        PPF_HIP_CHECK(rocprim::radix_sort_pairs(nullptr, tb, k1, k2, v1, v2, nq, 0, 32, s));
+        PPF_HIP_CHECK(rocprim::radix_sort_pairs(ws.cubTmp.reserve(tb), tb, k1, k2, v1, v2, nq, 0, 32, s));
         query = v2;
     }
     int       *cnt  = ws.mlsCnt.as<int>((size_t)nq + 1);
@@ -339,10 +339,8 @@ void estimate_normals_mls(Workspace &ws, int n, const float *PX, const float *PY
     // exclusive scan of the counts in 64-bit, total in offs[nq]
     PPF_HIP_CHECK(hipMemsetAsync(cnt + nq, 0, sizeof(int), s));
     size_t tb = 0;
-    PPF_HIP_CHECK(hipcub::DeviceScan::ExclusiveScan(nullptr, tb, cnt, offs, hipcub::Sum(),
-                                                    (long long)0, nq + 1, s));
-    PPF_HIP_CHECK(hipcub::DeviceScan::ExclusiveScan(ws.cubTmp.reserve(tb), tb, cnt, offs,
-                                                    hipcub::Sum(), (long long)0, nq + 1, s));
+    PPF_HIP_CHECK(rocprim::exclusive_scan(nullptr, tb, cnt, offs, (long long)0, nq + 1, rocprim::plus<long long>(), s));
+    PPF_HIP_CHECK(rocprim::exclusive_scan(ws.cubTmp.reserve(tb), tb, cnt, offs, (long long)0, nq + 1, rocprim::plus<long long>(), s));
     long long total = 0;
     PPF_HIP_CHECK(hipMemcpyAsync(&total, offs + nq, sizeof total, hipMemcpyDeviceToHost, s));
     PPF_HIP_CHECK(hipStreamSynchronize(s));
@@ -360,11 +358,8 @@ void estimate_normals_mls(Workspace &ws, int n, const float *PX, const float *PY
         // ids are non-negative: sort them as unsigned words
         uint32_t *k1 = reinterpret_cast<uint32_t *>(ids), *k2 = reinterpret_cast<uint32_t *>(ids2);
         tb = 0;
-        PPF_HIP_CHECK(hipcub::DeviceSegmentedRadixSort::SortKeys(nullptr, tb, k1, k2, (int)total,
-                                                                 nq, offs, offs + 1, 0, bits, s));
-        PPF_HIP_CHECK(hipcub::DeviceSegmentedRadixSort::SortKeys(ws.cubTmp.reserve(tb), tb, k1,
-                                                                 k2, (int)total, nq, offs,
-                                                                 offs + 1, 0, bits, s));
+        PPF_HIP_CHECK(rocprim::segmented_radix_sort_keys(nullptr, tb, k1, k2, (int)total, nq, offs, offs + 1, 0, bits, s));
+        PPF_HIP_CHECK(rocprim::segmented_radix_sort_keys(ws.cubTmp.reserve(tb), tb, k1, k2, (int)total, nq, offs, offs + 1, 0, bits, s));
     }
     float4 *P4 = ws.mlsP4.as<float4>((size_t)std::max(n, 1));
     mls_pack_kernel<<<div_up(std::max(n, 1), 256), 256, 0, s>>>(PX, PY, PZ, n, P4);

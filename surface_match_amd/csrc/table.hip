@@ -8,7 +8,8 @@
 // single-thread insertion order (i ascending, j ascending) inside each one.
 #include "ppf_dev.h"
 
-#include <hipcub/hipcub.hpp>
+#include <cstring>
+#include <rocprim/rocprim.hpp>
 
 #include <algorithm>
 #include <cmath>
@@ -126,9 +127,8 @@ void train_table(TrainScratch &ts, int n, const float *px, const float *py, cons
     int *cntD = ts.cnt.as<int>(4);
     DevBuf cubTmp;
     size_t tb = 0;
-    PPF_HIP_CHECK(hipcub::DeviceSelect::Flagged(nullptr, tb, iota, a.flags, sel, cntD, (int)nn, s));
-    PPF_HIP_CHECK(hipcub::DeviceSelect::Flagged(cubTmp.reserve(tb), tb, iota, a.flags, sel, cntD,
-                                                (int)nn, s));
+    PPF_HIP_CHECK(rocprim::select(nullptr, tb, iota, a.flags, sel, cntD, (int)nn, s));
+    PPF_HIP_CHECK(rocprim::select(cubTmp.reserve(tb), tb, iota, a.flags, sel, cntD, (int)nn, s));
     int m = 0;
     PPF_HIP_CHECK(hipMemcpyAsync(&m, cntD, sizeof(int), hipMemcpyDeviceToHost, s));
     PPF_HIP_CHECK(hipStreamSynchronize(s));
@@ -141,17 +141,17 @@ void train_table(TrainScratch &ts, int n, const float *px, const float *py, cons
     int      *idx2 = iota;
     DevBuf    cubTmp2;
     tb = 0;
-    PPF_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, k1, k2, sel, idx2, m, 0, 32, s));
+    PPF_HIP_CHECK(rocprim::radix_sort_pairs(nullptr, tb, k1, k2, sel, idx2, m, 0, 32, s));
     PPF_HIP_CHECK(
-        hipcub::DeviceRadixSort::SortPairs(cubTmp2.reserve(tb), tb, k1, k2, sel, idx2, m, 0, 32, s));
+        rocprim::radix_sort_pairs(cubTmp2.reserve(tb), tb, k1, k2, sel, idx2, m, 0, 32, s));
     uint32_t *uk  = ts.ukeys.as<uint32_t>(m);
     int      *uc  = ts.ucount.as<int>(m);
     int      *nrD = ts.cnt.as<int>(4);
     DevBuf    cubTmp3;
     tb = 0;
-    PPF_HIP_CHECK(hipcub::DeviceRunLengthEncode::Encode(nullptr, tb, k2, uk, uc, nrD, m, s));
+    PPF_HIP_CHECK(rocprim::run_length_encode(nullptr, tb, k2, m, uk, uc, nrD, s));
     PPF_HIP_CHECK(
-        hipcub::DeviceRunLengthEncode::Encode(cubTmp3.reserve(tb), tb, k2, uk, uc, nrD, m, s));
+        rocprim::run_length_encode(cubTmp3.reserve(tb), tb, k2, m, uk, uc, nrD, s));
     uint32_t *rf = ts.refs.as<uint32_t>(m);
     float    *al = reinterpret_cast<float *>(k1);  // k1 no longer needed
     finish_entries_kernel<<<div_up(m, 256), 256, 0, s>>>(idx2, a.alphas, m, n, rf, al);

@@ -22,7 +22,8 @@
 
 #include <type_traits>
 
-#include <hipcub/hipcub.hpp>
+#include <cstring>
+#include <rocprim/rocprim.hpp>
 
 #include <algorithm>
 
@@ -1521,10 +1522,8 @@ int vote_keypoints(Workspace &ws, const DevModel &m, const VoteKeypoints &in, fl
         while ((1 << (endBit - 30)) < std::max(in.nScenes, 1))
             endBit++;
         size_t tb = 0;
-        PPF_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, k0, k1, i0, i1, nk, 0,
-                                                         endBit, s));
-        PPF_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(ws.cubTmp.reserve(tb), tb, k0, k1, i0, i1,
-                                                         nk, 0, endBit, s));
+        PPF_HIP_CHECK(rocprim::radix_sort_pairs(nullptr, tb, k0, k1, i0, i1, nk, 0, endBit, s));
+        PPF_HIP_CHECK(rocprim::radix_sort_pairs(ws.cubTmp.reserve(tb), tb, k0, k1, i0, i1, nk, 0, endBit, s));
         order = i1;
     }
     a.kpSel = order;
@@ -1593,11 +1592,10 @@ int vote_keypoints(Workspace &ws, const DevModel &m, const VoteKeypoints &in, fl
                                                   m.cosSin.get<float>(), m.angleNum, pslots,
                                                   counts);
     size_t tb = 0;
-    PPF_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, counts, offs, nk + 1, s));
+    PPF_HIP_CHECK(rocprim::exclusive_scan(nullptr, tb, counts, offs, 0, nk + 1, rocprim::plus<int>(), s));
     // counts[nk] must be 0 so offs[nk] is the total
     PPF_HIP_CHECK(hipMemsetAsync(counts + nk, 0, sizeof(int), s));
-    PPF_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(ws.cubTmp.reserve(tb), tb, counts, offs,
-                                                   nk + 1, s));
+    PPF_HIP_CHECK(rocprim::exclusive_scan(ws.cubTmp.reserve(tb), tb, counts, offs, 0, nk + 1, rocprim::plus<int>(), s));
     PoseRec *out = ws.poses.as<PoseRec>((size_t)nk * 3);
     compact_poses_kernel<<<div_up(nk, 128), 128, 0, s>>>(pslots, counts, offs, nk, out);
     int np = 0;
