@@ -52,6 +52,31 @@ def profiled(quantity: str, kernel: str):
     return None
 
 
+BINDING = {
+    # what the PMC counters of each vote kernel show binds it
+    "vote_hits_kernel": "L1-to-register data path (TD) and VALU issue, not HBM or LDS atomics",
+    "pair_kernel": "VALU issue (per-pair feature arithmetic: correctly rounded divisions, square "
+                   "roots, arctangents, MurmurHash3), not HBM",
+    "vote_kernel": "memory and LDS latency at one workgroup per CU",
+}
+
+
+def binding(kname: str, lds_rate: float, ceil: dict) -> dict:
+    """The resource that binds `kname`, with the figures that show it: the
+    counters kept in PROFILE_FILE for that kernel and, for the kernels that
+    cast votes, the LDS atomic rate against this GPU's measured ceiling."""
+    out = {"resource": BINDING[kname],
+           "l1_tag_lookups_per_cu_cycle": profiled("l1_tag_lookups_per_cu_cycle", kname),
+           "valu_busy_frac": profiled("valu_busy_frac", kname),
+           "alone_ms": profiled("alone_ms", kname),
+           "profile": os.path.relpath(PROFILE_FILE, ROOT)}
+    if kname != "pair_kernel":
+        out.update({"lds_atomics_per_s": float("%.4g" % lds_rate),
+                    "lds_atomic_ceiling_per_s": ceil["lds_atomics_per_s"],
+                    "lds_frac": round(lds_rate / ceil["lds_atomics_per_s"], 4)})
+    return out
+
+
 def vote_algorithmic_bytes(st: dict) -> float:
     """The whole vote stage, per launch set: 24 B per neighbour pair (position
     + normal), 12 B per voted key point, 8 B per key hit (bucket descriptor),
@@ -479,24 +504,12 @@ def main():
                             # one LDS atomic add per vote (the count slot is derived)
                             "lds_atomics_per_s": round(st["nVotes"] / ((hits_ms or vote_ms) * 1e-3))},
         # the dominant kernel against HBM (its algorithmic bytes; MFMA is
-        # unused), with what the PMC counters show binds it: the vector L1's
-        # address/tag lookups (one 16-byte bucket-entry load per lane touches
-        # a line of its own) and LDS atomics, against this GPU's measured
-        # LDS atomic rate
+        # unused), with what the PMC counters show binds it
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": profiled("traffic_bytes_per_launch", kname),
                      "kernel": kname, "algorithmic_bytes": alg, "kernel_ms": round(kms, 4),
-                     "binding": {
-                         "resource": "L1-to-register data path (TD) and VALU issue, not HBM "
-                                     "or LDS atomics",
-                         "lds_atomics_per_s": float("%.4g" % lds_rate),
-                         "lds_atomic_ceiling_per_s": ceil["lds_atomics_per_s"],
-                         "lds_frac": round(lds_rate / ceil["lds_atomics_per_s"], 4),
-                         "l1_tag_lookups_per_cu_cycle": profiled("l1_tag_lookups_per_cu_cycle", kname),
-                         "valu_busy_frac": profiled("valu_busy_frac", kname),
-                         "alone_ms": profiled("alone_ms", kname),
-                         "profile": os.path.relpath(PROFILE_FILE, ROOT)},
+                     "binding": binding(kname, lds_rate, ceil),
                      "vote_stage": {"algorithmic_bytes": vote_algorithmic_bytes(st),
                                     "ms": round(vote_ms, 4),
                                     "GBps": round(vote_algorithmic_bytes(st) / (vote_ms * 1e-3) / 1e9, 1)}},

@@ -27,10 +27,18 @@
  *   grid radius/knn    brute force over all points (test)
  *   voxel sample       numpy brute force (test)
  *   normals            numpy eigh on the same covariance; plane/sphere
+ *   normal smoothing   a hand case whose sum is known in closed form, with
+ *                      the sign rule, self term and perpendicular neighbour
+ *                      (tests/test_oracle_icp_rules.py)
+ *   remove_nan/bbox    hand cases: NaN and inf coordinates, -0, ties of the
+ *                      extreme point (tests/test_oracle_stages.py)
  *   clustering/NMS     hand cases + Python brute force (test)
  *   jacobi eig         numpy eigh
  *   ICP                exact-correspondence recovery; ground-truth pose
- *                      recovery on seeded scenes
+ *                      recovery on seeded scenes; ICP::regist's rules by
+ *                      hand cases (5/6/7 correspondences, the mse and rate
+ *                      stops, the iteration limit, a near-identity initial
+ *                      pose; tests/test_oracle_icp_rules.py)
  *   orc_philox4x32_10  Random123's published known-answer vectors
  *   orc_sample_mesh    closed forms of the point weights, per-triangle
  *                      counts worked by hand, points on their triangles,
