@@ -332,6 +332,20 @@ PPF_API int ppf_icp_refine(ppf_icp_target t, const float *src, const float *srcN
                            const float *initPoses16, size_t nPoses, const ppf_icp_criteria *c,
                            int mode, int batched, int block, float scoringDist,
                            ppf_icp_result *out, float *deviceMs);
+/* hypotheses of several scenes refined together (no reference counterpart:
+ * the batching extension, for callers with the pose hypotheses of many
+ * scenes).  Pose h is refined against targets[poseTarget[h]] exactly as
+ * ppf_icp_refine refines it against that target alone; batched == 1 or 2:
+ * every pose in one batched refinement (as ppf_icp_refine), batched == 0:
+ * the conventional refiner, one pose at a time.  The call runs on the first
+ * target's stream and scratch, so no other call may use that target
+ * meanwhile.  Errors: PPF_ERR_ARG for a null argument or target, or a
+ * poseTarget entry outside [0, nTargets). */
+PPF_API int ppf_icp_refine_multi(const ppf_icp_target *targets, size_t nTargets,
+                                 const int *poseTarget, const float *src, const float *srcNrm,
+                                 size_t nSrc, const float *initPoses16, size_t nPoses,
+                                 const ppf_icp_criteria *c, int mode, int batched, int block,
+                                 float scoringDist, ppf_icp_result *out, float *deviceMs);
 /* point-to-plane cost adjoint (no reference counterpart; its Jacobian rows
  * are icp.cpp:318-320's): for each pose T = U(xi) T0 with U(xi) = [R(w), v]
  * (xi = (w, v), R(w) = exp([w]x) by Rodrigues' formula), the refiner's

@@ -104,7 +104,7 @@ EXPORTS = [
     "ppf_scene_inliers", "ppf_measure_ceilings", "ppf_ply_read", "ppf_ply_info", "ppf_ply_copy",
     "ppf_ply_free", "ppf_ply_write", "ppf_scene_upload_ply", "ppf_train_model_mesh",
     "ppf_sample_mesh", "ppf_estimate_normals_mls", "ppf_scene_download",
-    "ppf_voxel_sample_noalloc",
+    "ppf_voxel_sample_noalloc", "ppf_icp_refine_multi",
 ]
 
 
@@ -151,6 +151,8 @@ def lib() -> C.CDLL:
         "ppf_icp_refine": (i, [vp, vp, vp, sz, vp, sz, C.POINTER(IcpCriteria), i, i, i, f,
                                vp, C.POINTER(f)]),
         "ppf_icp_adjoint": (i, [vp, vp, sz, vp, vp, sz, f, i, vp, vp]),
+        "ppf_icp_refine_multi": (i, [vp, sz, vp, vp, vp, sz, vp, sz, C.POINTER(IcpCriteria), i, i,
+                                     i, f, vp, C.POINTER(f)]),
         "ppf_autotune": (i, [vp, vp, f, f, f, C.POINTER(MatchParam), C.POINTER(TuneReport)]),
         "ppf_set_tuning": (i, [vp, C.POINTER(Tuning)]),
         "ppf_get_tuning": (i, [vp, C.POINTER(Tuning)]),

@@ -219,6 +219,7 @@ struct ppf_icp_target_s {
     Grid        grid, full;
     Workspace   ws;
     DevBuf      sx, sy, sz, snx, sny, snz, saos;
+    DevBuf      views, poseTarget;  // ppf_icp_refine_multi: every target's grids, each pose's target
     int         n = 0;
     hipEvent_t  ev[2] = {nullptr, nullptr};
     ~ppf_icp_target_s() {
@@ -2492,6 +2493,90 @@ PPF_API int ppf_icp_refine(ppf_icp_target t, const float *src, const float *srcN
             o.converged    = res[h].converged;
             o.inliers      = res[h].inliers;
         }
+    });
+}
+
+static void write_icp_result(const IcpResultDev &r, ppf_icp_result &o) {
+    const float *P = r.pose;
+    for (int row = 0; row < 3; row++)
+        for (int col = 0; col < 4; col++)
+            o.pose16[col * 4 + row] = P[row * 4 + col];
+    o.pose16[3] = o.pose16[7] = o.pose16[11] = 0.0f;
+    o.pose16[15]   = 1.0f;
+    o.type         = r.type;
+    o.mse          = r.mse;
+    o.convergeRate = r.convergeRate;
+    o.iterations   = r.iterations;
+    o.converged    = r.converged;
+    o.inliers      = r.inliers;
+}
+
+PPF_API int ppf_icp_refine_multi(const ppf_icp_target *targets, size_t nTargets,
+                                 const int *poseTarget, const float *src, const float *srcNrm,
+                                 size_t nSrc, const float *initPoses16, size_t nPoses,
+                                 const ppf_icp_criteria *c, int mode, int batched, int block,
+                                 float scoringDist, ppf_icp_result *out, float *deviceMs) {
+    PPF_REQUIRE(targets && nTargets > 0 && poseTarget && src && srcNrm && initPoses16 && c && out,
+                "null argument");
+    for (size_t i = 0; i < nTargets; i++)
+        PPF_REQUIRE(targets[i], "null target");
+    for (size_t h = 0; h < nPoses; h++)
+        PPF_REQUIRE(poseTarget[h] >= 0 && (size_t)poseTarget[h] < nTargets,
+                    "pose target index out of range");
+    if (nPoses == 0)
+        return PPF_OK;
+    return guarded([&] {
+        // the first target's stream, scratch and source buffers serve the call
+        ppf_icp_target_s &t0 = *targets[0];
+        hipStream_t       s  = t0.stream;
+        upload_src(t0, src, srcNrm, (int)nSrc);
+        std::vector<float> init(12 * nPoses);
+        for (size_t h = 0; h < nPoses; h++)
+            pose16_to_12(initPoses16 + 16 * h, &init[12 * h]);
+        IcpCriteria ic{c->iterations, c->rejectDist, c->mseMin, c->mseMax, c->tolerance};
+        std::vector<IcpResultDev> res(nPoses);
+        PPF_HIP_CHECK(hipEventRecord(t0.ev[0], s));
+        if (batched) {
+            std::vector<GridView> v(2 * nTargets);
+            for (size_t i = 0; i < nTargets; i++) {
+                v[i]            = targets[i]->grid.view;
+                v[nTargets + i] = targets[i]->full.view;
+            }
+            GridView *gv = t0.views.as<GridView>(2 * nTargets);
+            int      *pt = t0.poseTarget.as<int>(nPoses);
+            PPF_HIP_CHECK(hipMemcpyAsync(gv, v.data(), sizeof(GridView) * v.size(),
+                                         hipMemcpyHostToDevice, s));
+            PPF_HIP_CHECK(hipMemcpyAsync(pt, poseTarget, sizeof(int) * nPoses,
+                                         hipMemcpyHostToDevice, s));
+            IcpConfig cfg;
+            cfg.mode   = mode;
+            cfg.block  = block > 0 ? block : 256;
+            cfg.spread = batched == 2 ? 0 : 1;
+            const ppf_icp_target_s &tf = *targets[poseTarget[0]];
+            icp_batch(t0.ws, (int)nSrc, t0.sx.get<float>(), t0.sy.get<float>(), t0.sz.get<float>(),
+                      t0.snx.get<float>(), t0.sny.get<float>(), t0.snz.get<float>(), tf.grid.view,
+                      tf.full.view, init.data(), (int)nPoses, ic, scoringDist, cfg, nullptr, s, gv,
+                      gv + nTargets, pt);
+            PPF_HIP_CHECK(hipEventRecord(t0.ev[1], s));
+            PPF_HIP_CHECK(hipMemcpyAsync(res.data(), t0.ws.icpOut.get<IcpResultDev>(),
+                                         sizeof(IcpResultDev) * nPoses, hipMemcpyDeviceToHost, s));
+            PPF_HIP_CHECK(hipStreamSynchronize(s));
+        } else {
+            // the conventional refiner, one pose at a time against its target
+            for (size_t h = 0; h < nPoses; h++) {
+                const ppf_icp_target_s &t = *targets[poseTarget[h]];
+                icp_plain(t0.ws, (int)nSrc, t0.sx.get<float>(), t0.sy.get<float>(),
+                          t0.sz.get<float>(), t0.snx.get<float>(), t0.sny.get<float>(),
+                          t0.snz.get<float>(), t.grid.view, t.full.view, &init[12 * h], 1, ic,
+                          scoringDist, mode, &res[h], s);
+            }
+            PPF_HIP_CHECK(hipEventRecord(t0.ev[1], s));
+            PPF_HIP_CHECK(hipStreamSynchronize(s));
+        }
+        if (deviceMs)
+            *deviceMs = elapsed(t0.ev[0], t0.ev[1]);
+        for (size_t h = 0; h < nPoses; h++)
+            write_icp_result(res[h], out[h]);
     });
 }
 

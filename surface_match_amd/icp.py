@@ -86,6 +86,34 @@ class IcpTarget:
         return r
 
 
+def refine_many(targets, pose_target, src, src_nrm, init_poses, iterations=10, reject_dist=10.0,
+                mse_min=0.0, mse_max=1e30, tolerance=0.0, mode=1, batched=True, block=0,
+                scoring_dist=1.0):
+    """Pose h refined against targets[pose_target[h]] (IcpTargets of several
+    scenes), all in one batched refinement (batched True / "spread" /
+    "persistent") or one at a time (False); each result equals
+    targets[pose_target[h]].refine of that pose alone.  Returns what
+    IcpTarget.refine returns."""
+    src, src_nrm = N.f32(src), N.f32(src_nrm)
+    P = _pose16(init_poses)
+    H = len(P)
+    pt = np.ascontiguousarray(pose_target, np.int32).reshape(-1)
+    if len(pt) != H:
+        raise ValueError("one target index per pose")
+    hs = (C.c_void_p * len(targets))(*[t._h.value for t in targets])
+    crit = N.IcpCriteria(iterations, reject_dist, mse_min, mse_max, tolerance)
+    out = (N.IcpResult * max(H, 1))()
+    ms = C.c_float(0)
+    N.check(N.lib().ppf_icp_refine_multi(hs, len(targets), N.ptr(pt), N.ptr(src), N.ptr(src_nrm),
+                                         len(src), N.ptr(P), H, C.byref(crit), mode,
+                                         _batched_code(batched), block, scoring_dist, out,
+                                         C.byref(ms)))
+    poses = np.array([np.array(out[h].pose16, np.float32).reshape(4, 4).T for h in range(H)])
+    res = {k: np.array([getattr(out[h], k) for h in range(H)]) for k in
+           ("type", "mse", "convergeRate", "iterations", "converged", "inliers")}
+    return poses, res, ms.value
+
+
 def _batched_code(b):
     if b is False or b == 0:
         return 0

@@ -119,6 +119,44 @@ def test_adjoint_vs_fd_and_icp_jacobian(icp_case, oracle_mod):
             print(f"center {center} pose {h}: grad rel err vs FD {rel:.2e}")
 
 
+def test_refine_many_equals_per_target():
+    """Hypotheses of three scenes refined in one batched refinement
+    (ppf_icp_refine_multi), with their targets interleaved: each result
+    equals the refinement of that pose against its own target alone, bit for
+    bit, for the batched and the conventional refiner; a target index out of
+    range is rejected."""
+    from surface_match_amd import synth
+    from surface_match_amd._native import PPFError
+    from surface_match_amd.icp import IcpTarget, refine_many
+    r = rng(74)
+    src, srcn = synth.make_model(2500, 71)
+    targets, gts = [], []
+    for k in range(3):
+        p, n = synth.make_model(20000 + 5000 * k, 80 + k)
+        R = synth.random_rotation(r)
+        t = r.uniform(-150, 150, 3)
+        dst = (p @ R.T + t + r.normal(scale=0.05, size=p.shape)).astype(np.float32)
+        targets.append(IcpTarget(dst, (n @ R.T).astype(np.float32), 2.0))
+        T = np.eye(4)
+        T[:3, :3], T[:3, 3] = R, t
+        gts.append(T)
+    owner = np.array([h % 3 if h % 4 else 2 - h % 3 for h in range(21)], np.int32)
+    inits = np.array([_perturb(gts[o], r) for o in owner])
+    kw = dict(iterations=10, reject_dist=15.0, mode=1, scoring_dist=1.5)
+    for batched in (True, False):
+        pm, rm, _ = refine_many(targets, owner, src, srcn, inits, batched=batched, **kw)
+        for k in range(3):
+            sel = np.nonzero(owner == k)[0]
+            pk, rk, _ = targets[k].refine(src, srcn, inits[sel], batched=batched, **kw)
+            assert np.array_equal(pm[sel].view(np.uint32), pk.view(np.uint32)), (batched, k)
+            for key in rk:
+                assert np.array_equal(rm[key][sel], rk[key]), (batched, k, key)
+        ok = [synth.rotation_angle_deg(gts[o][:3, :3].T @ P[:3, :3]) < 0.1 for o, P in zip(owner, pm)]
+        print(f"batched={batched}: {sum(ok)} of {len(ok)} within 0.1 deg")
+    with pytest.raises(PPFError, match="pose target index"):
+        refine_many(targets, np.full(len(inits), 3, np.int32), src, srcn, inits, **kw)
+
+
 def test_torch_autograd_binding(tmp_path):
     """The optional torch binding, in its own process (torch brings its own
     HIP runtime; it only ever sees host tensors here).  The gradient of the
