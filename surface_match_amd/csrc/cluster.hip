@@ -117,6 +117,88 @@ __global__ void __launch_bounds__(1024) cluster_rounds_kernel(SortedPoses s, int
         *roundsOut = rounds;
 }
 
+// The same rule walked in order by one wave, for pose lists that fit in
+// LDS: 64 poses at a time, each lane first looks for its pose's first
+// matching centre among those opened before the block (centres keep their
+// creation order); the lanes left over are then settled in order, each
+// against the centres opened in the block so far (one ballot: the lowest
+// matching lane is the earliest), opening a centre when none matches.  No
+// rounds, so short lists finish in one pass instead of a chain of
+// workgroup-wide rounds.
+constexpr int CLUSTER_SEQ_MAX = 512;
+
+__global__ void __launch_bounds__(64) cluster_seq_kernel(SortedPoses s, int n, float dThr,
+                                                         float aThr, int *status) {
+    // centres in creation order: x, y, z, angle, sorted position
+    extern __shared__ float sC[];
+    float    *cx = sC, *cy = sC + n, *cz = sC + 2 * n, *ca = sC + 3 * n;
+    int      *ci = reinterpret_cast<int *>(sC + 4 * n);
+    const int lane = threadIdx.x;
+    int       nc   = 0;  // wave-uniform
+    for (int b0 = 0; b0 < n; b0 += 64) {
+        const int  i     = b0 + lane;
+        const bool valid = i < n;
+        float      tx = 0.0f, ty = 0.0f, tz = 0.0f, ta = 0.0f;
+        if (valid) {
+            tx = s.tx[i];
+            ty = s.ty[i];
+            tz = s.tz[i];
+            ta = s.angle[i];
+        }
+        // comparePose (util.cpp:493-497) with the later pose first, as
+        // pose_match(s, i, centre)
+        auto match = [&](float x, float y, float z, float a) {
+            const float dx = tx - x, dy = ty - y, dz = tz - z;
+            const float d  = sqrtf(dx * dx + dy * dy + dz * dz);
+            return d < dThr && fabsf(ta - a) < aThr;
+        };
+        int       st  = UNDECIDED;
+        const int nc0 = nc;
+        if (valid)
+            for (int t = 0; t < nc0; t++)
+                if (match(cx[t], cy[t], cz[t], ca[t])) {
+                    st = ci[t];
+                    break;
+                }
+        bool               opened = false;
+        unsigned long long rest   = __ballot(valid && st == UNDECIDED);
+        while (rest) {
+            const int u = __ffsll((long long)rest) - 1;
+            rest &= rest - 1ull;
+            const float ux = __shfl(tx, u, 64), uy = __shfl(ty, u, 64), uz = __shfl(tz, u, 64),
+                        ua = __shfl(ta, u, 64);
+            bool m = false;
+            if (opened) {
+                // the centre is this lane's pose, the later pose is u's
+                const float dx = ux - tx, dy = uy - ty, dz = uz - tz;
+                const float d  = sqrtf(dx * dx + dy * dy + dz * dz);
+                m              = d < dThr && fabsf(ua - ta) < aThr;
+            }
+            const unsigned long long mm = __ballot(m);
+            if (mm) {
+                if (lane == u)
+                    st = b0 + __ffsll((long long)mm) - 1;
+            } else {
+                if (lane == u) {
+                    st     = CENTER;
+                    opened = true;
+                    cx[nc] = tx;
+                    cy[nc] = ty;
+                    cz[nc] = tz;
+                    ca[nc] = ta;
+                    ci[nc] = i;
+                }
+                nc++;
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (valid)
+            status[i] = st;
+    }
+}
+
 // cluster index of each centre = its rank among centres (creation order)
 __global__ void center_rank_kernel(const int *status, int n, int *isCenter) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -412,12 +494,21 @@ void cluster_poses(Workspace &ws, const PoseRec *poses, int np, float dThr, floa
     SortedPoses sp{buf, buf + np, buf + 2 * (size_t)np, buf + 3 * (size_t)np, buf + 4 * (size_t)np,
                    ord};
     gather_sorted_kernel<<<div_up(np, 256), 256, 0, s>>>(poses, ord, np, sp);
-    // translation grid: ids are sorted positions
-    grid_build(ws.gNormal, sp.tx, sp.ty, sp.tz, nullptr, nullptr, nullptr, nullptr, np, dThr, s);
+    // long lists: a translation grid (ids are sorted positions) for the rounds
     int *rank   = status + np;
     int *isC    = status + 2 * (size_t)np;
     int *misc   = status + 3 * (size_t)np;
-    cluster_rounds_kernel<<<1, 1024, 0, s>>>(sp, np, ws.gNormal.view, dThr, aThr, status, misc);
+    if (np <= CLUSTER_SEQ_MAX) {
+        const size_t lds = (size_t)20 * np;
+        PPF_HIP_CHECK(hipFuncSetAttribute((const void *)cluster_seq_kernel,
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        cluster_seq_kernel<<<1, 64, lds, s>>>(sp, np, dThr, aThr, status);
+    } else {
+        grid_build(ws.gNormal, sp.tx, sp.ty, sp.tz, nullptr, nullptr, nullptr, nullptr, np, dThr,
+                   s);
+        cluster_rounds_kernel<<<1, 1024, 0, s>>>(sp, np, ws.gNormal.view, dThr, aThr, status,
+                                                 misc);
+    }
     center_rank_kernel<<<div_up(np, 256), 256, 0, s>>>(status, np, isC);
     size_t tb = 0;
     PPF_HIP_CHECK(rocprim::exclusive_scan(nullptr, tb, isC, rank, 0, np, rocprim::plus<int>(), s));

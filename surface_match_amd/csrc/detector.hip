@@ -18,6 +18,7 @@
 #include <cstring>
 #include <exception>
 #include <fstream>
+#include <functional>
 #include <memory>
 #include <mutex>
 #include <numeric>
@@ -848,6 +849,65 @@ void for_scenes(MatchCtx &X, int n, F &&f) {
             std::rethrow_exception(e);
 }
 
+// Two passes of the lanes over the scenes, in the background of the
+// calling thread: f(i, workspace, stream) for every scene, then, once
+// open() is called, g(i, ...) for every scene.  join() waits for both and
+// rethrows a lane's error; leaving the scope without open() skips g.
+struct LanePasses {
+    using Fn = std::function<void(int, Workspace &, hipStream_t)>;
+    MatchCtx                       &X;
+    int                             n;
+    Fn                              f, g;
+    std::atomic<int>                next1{0}, next2{0}, gate{0};  // gate: 0 wait, 1 go, -1 skip
+    std::vector<std::thread>        th;
+    std::vector<std::exception_ptr> err;
+
+    LanePasses(MatchCtx &X_, int n_, Fn f_, Fn g_) : X(X_), n(n_), f(std::move(f_)), g(std::move(g_)) {
+        PPF_HIP_CHECK(hipStreamSynchronize(X.stream));
+        const int L = std::max(1, std::min(n, MatchCtx::kLanes));
+        lane_of(X, L - 1);
+        int dev = 0;
+        PPF_HIP_CHECK(hipGetDevice(&dev));
+        err.resize(L);
+        for (int l = 0; l < L; l++)
+            th.emplace_back([this, l, dev] {
+                try {
+                    PPF_HIP_CHECK(hipSetDevice(dev));
+                    MatchCtx::Lane &ln = *X.lanes[l];
+                    for (int i = next1++; i < n; i = next1++)
+                        f(i, ln.ws, ln.stream);
+                    int q;
+                    while ((q = gate.load(std::memory_order_acquire)) == 0)
+                        std::this_thread::yield();
+                    if (q == 1)
+                        for (int i = next2++; i < n; i = next2++)
+                            g(i, ln.ws, ln.stream);
+                    PPF_HIP_CHECK(hipStreamSynchronize(ln.stream));
+                } catch (...) {
+                    err[l] = std::current_exception();
+                }
+            });
+    }
+    void open() { gate.store(1, std::memory_order_release); }
+    void join() {
+        int z = 0;
+        gate.compare_exchange_strong(z, -1);
+        for (auto &t : th)
+            if (t.joinable())
+                t.join();
+        for (auto &e : err)
+            if (e)
+                std::rethrow_exception(e);
+    }
+    ~LanePasses() {
+        int z = 0;
+        gate.compare_exchange_strong(z, -1);
+        for (auto &t : th)
+            if (t.joinable())
+                t.join();
+    }
+};
+
 void match_batch_impl(ppf_detector_s &D, MatchCtx &X, const std::vector<DevCloud *> &clouds, float rel,
                       float kpFrac, float minScore, const ppf_match_param &p,
                       std::vector<MatchOut> &outs, ppf_match_stats *st) {
@@ -923,18 +983,26 @@ void match_batch_impl(ppf_detector_s &D, MatchCtx &X, const std::vector<DevCloud
         minCell   = std::min(minCell, X.sceneWork[i]->gVote.view.cell);
         ko[i + 1] = ko[i] + X.sceneWork[i]->nk;
     }
-    PPF_HIP_CHECK(hipEventRecord(X.ev[3], s));
-    // ---- every scene's key points in one vote
+    // every scene's key points gathered before the lanes start (a point
+    // whose normal the sampled pass left undefined can be given one, and
+    // moved, by its scene's dense pass)
     const int    K = ko[n];
     VoteStats    vs{};
     int          np = 0;
-    std::vector<int> pOff(n + 1, 0);
+    float *kx = nullptr, *ky = nullptr, *kz = nullptr, *knx = nullptr, *kny = nullptr,
+          *knz = nullptr;
+    int   *kpScene = nullptr, *kpId = nullptr;
+    GridView *gv = nullptr;
+    std::vector<GridView> views(n);
     if (K > 0) {
-        float *kx = X.ws.kpX.as<float>(K), *ky = X.ws.kpY.as<float>(K), *kz = X.ws.kpZ.as<float>(K);
-        float *knx = X.ws.kpNX.as<float>(K), *kny = X.ws.kpNY.as<float>(K),
-              *knz = X.ws.kpNZ.as<float>(K);
-        int   *kpScene = X.batchKpScene.as<int>(K), *kpId = X.batchKpId.as<int>(K);
-        std::vector<GridView> views(n);
+        kx  = X.ws.kpX.as<float>(K);
+        ky  = X.ws.kpY.as<float>(K);
+        kz  = X.ws.kpZ.as<float>(K);
+        knx = X.ws.kpNX.as<float>(K);
+        kny = X.ws.kpNY.as<float>(K);
+        knz = X.ws.kpNZ.as<float>(K);
+        kpScene = X.batchKpScene.as<int>(K);
+        kpId    = X.batchKpId.as<int>(K);
         for (int i = 0; i < n; i++) {
             DevCloud &c  = *clouds[i];
             auto     &sw = *X.sceneWork[i];
@@ -950,9 +1018,60 @@ void match_batch_impl(ppf_detector_s &D, MatchCtx &X, const std::vector<DevCloud
             PPF_HIP_CHECK(hipMemcpyAsync(kpId + o, sw.kp.get<int>(), sizeof(int) * sw.nk,
                                          hipMemcpyDeviceToDevice, s));
         }
-        GridView *gv = X.batchGrids.as<GridView>(n);
+        gv = X.batchGrids.as<GridView>(n);
         PPF_HIP_CHECK(hipMemcpyAsync(gv, views.data(), sizeof(GridView) * n,
                                      hipMemcpyHostToDevice, s));
+    }
+    PPF_HIP_CHECK(hipEventRecord(X.ev[3], s));
+    // ---- per scene, on the lanes while the vote runs: the refinement's
+    // dense samples with their normals (ppf.cpp:427-434) and its grids over
+    // the final positions, as the single-scene match prepares them beside
+    // its vote (they write only points without a normal, and the vote reads
+    // the scene only through its grids' copies and the key points gathered
+    // above); then, once the vote is in, each scene's clustering
+    // (ppf.cpp:413-420)
+    const bool laneGrids = p.sparsePoseRefinement != 0;
+    float center[3];
+    for (int a = 0; a < 3; a++)
+        center[a] = (hm.sampled.boxMin[a] + hm.sampled.boxMax[a]) / 2.0f;
+    std::vector<std::vector<Hypo>> hyp(n);
+    std::vector<int>               ncs(n, 0);
+    std::vector<int>               pOff(n + 1, 0);
+    LanePasses lanes(
+        X, n,
+        [&](int i, Workspace &ws, hipStream_t ls) {
+            DevCloud &c  = *clouds[i];
+            auto     &sw = *X.sceneWork[i];
+            float *PX = c.px.get<float>(), *PY = c.py.get<float>(), *PZ = c.pz.get<float>();
+            float *NX = c.nx.get<float>(), *NY = c.ny.get<float>(), *NZ = c.nz.get<float>();
+            sw.nd = 0;
+            if (p.densePoseRefinement) {
+                int *dense = sw.dense.as<int>(std::max(c.nValid, 1));
+                sw.nd = voxel_sample(ws, PX, PY, PZ, c.valid.get<int>(), c.nValid, reSampleStep,
+                                     c.boxMin, dense, ws.smallCnt.as<int>(8), ls);
+                if (!c.hasNormal)
+                    estimate_normals(ws, c.n, PX, PY, PZ, NX, NY, NZ, c.valid.get<int>(),
+                                     c.nValid, dense, sw.nd, p.knnNormal, 0, p.invertNormal, c.vp,
+                                     denseNormCell, ls, c.ext);
+            }
+            if (laneGrids) {
+                grid_build(sw.gSparse, PX, PY, PZ, NX, NY, NZ, sw.samp.get<int>(), sw.ns,
+                           sampleStep * kSparseCell, ls, c.ext);
+                grid_build(sw.gFull, PX, PY, PZ, nullptr, nullptr, nullptr, c.valid.get<int>(),
+                           c.nValid, std::max(scoringDist * 2.0f, sampleStep * 0.25f), ls, c.ext);
+                if (p.densePoseRefinement)
+                    grid_build(sw.gDense, PX, PY, PZ, NX, NY, NZ, sw.dense.get<int>(), sw.nd,
+                               reSampleStep * kDenseCell, ls, c.ext);
+            }
+        },
+        [&](int i, Workspace &ws, hipStream_t ls) {
+            // the vote's poses (its buffer may have grown during the vote)
+            const PoseRec *posesAll = X.ws.poses.get<PoseRec>();
+            cluster_poses(ws, posesAll + pOff[i], pOff[i + 1] - pOff[i], 0.1f * Dm, angleStep,
+                          center, maxOverlapDist, hyp[i], &ncs[i], ls);
+        });
+    // ---- every scene's key points in one vote
+    if (K > 0) {
         VoteKeypoints in;
         in.g        = views[0];
         in.gridsDev = gv;
@@ -977,17 +1096,8 @@ void match_batch_impl(ppf_detector_s &D, MatchCtx &X, const std::vector<DevCloud
             pOff[i] = offs[ko[i]];
     }
     PPF_HIP_CHECK(hipEventRecord(X.ev[5], s));
-    // ---- clustering per scene (ppf.cpp:413-420)
-    float center[3];
-    for (int a = 0; a < 3; a++)
-        center[a] = (hm.sampled.boxMin[a] + hm.sampled.boxMax[a]) / 2.0f;
-    std::vector<std::vector<Hypo>> hyp(n);
-    std::vector<int>               ncs(n, 0);
-    const PoseRec                 *posesAll = X.ws.poses.get<PoseRec>();
-    for_scenes(X, n, [&](int i, Workspace &ws, hipStream_t ls) {
-        cluster_poses(ws, posesAll + pOff[i], pOff[i + 1] - pOff[i], 0.1f * Dm, angleStep, center,
-                      maxOverlapDist, hyp[i], &ncs[i], ls);
-    });
+    lanes.open();
+    lanes.join();
     int nClustersTot = 0;
     for (int i = 0; i < n; i++)
         nClustersTot += ncs[i];
@@ -1011,46 +1121,28 @@ void match_batch_impl(ppf_detector_s &D, MatchCtx &X, const std::vector<DevCloud
                 alive[i][h]    = 1;
             }
     }
-    // dense samples (ppf.cpp:427-434) of the scenes that need them
     std::vector<char> sceneHasCand(n, 0);
     for (auto &c : cand)
         sceneHasCand[c.scene] = 1;
     int nSparse = 0, nDenseIcp = 0, nDenseTot = 0;
     if (p.sparsePoseRefinement && !cand.empty()) {
-        std::vector<GridView> vs1(n), vf(n);
-        for_scenes(X, n, [&](int i, Workspace &ws, hipStream_t ls) {
-            if (!sceneHasCand[i])
-                return;
-            DevCloud &c  = *clouds[i];
-            auto     &sw = *X.sceneWork[i];
-            float *PX = c.px.get<float>(), *PY = c.py.get<float>(), *PZ = c.pz.get<float>();
-            float *NX = c.nx.get<float>(), *NY = c.ny.get<float>(), *NZ = c.nz.get<float>();
+        std::vector<GridView> vs1(n), vf(n), vd(n);
+        for (int i = 0; i < n; i++) {
+            vs1[i] = X.sceneWork[i]->gSparse.view;
+            vf[i]  = X.sceneWork[i]->gFull.view;
             if (p.densePoseRefinement) {
-                // ppf.cpp:427-434: the dense samples' normals overwrite the scene's
-                // before either refinement reads them
-                int *dense = sw.dense.as<int>(std::max(c.nValid, 1));
-                sw.nd = voxel_sample(ws, PX, PY, PZ, c.valid.get<int>(), c.nValid, reSampleStep,
-                                     c.boxMin, dense, ws.smallCnt.as<int>(8), ls);
-                if (!c.hasNormal)
-                    estimate_normals(ws, c.n, PX, PY, PZ, NX, NY, NZ, c.valid.get<int>(),
-                                     c.nValid, dense, sw.nd, p.knnNormal, 0, p.invertNormal, c.vp,
-                                     denseNormCell, ls, c.ext);
-            }
-            grid_build(sw.gSparse, PX, PY, PZ, NX, NY, NZ, sw.samp.get<int>(), sw.ns,
-                       sampleStep * kSparseCell, ls, c.ext);
-            grid_build(sw.gFull, PX, PY, PZ, nullptr, nullptr, nullptr, c.valid.get<int>(),
-                       c.nValid, std::max(scoringDist * 2.0f, sampleStep * 0.25f), ls, c.ext);
-            vs1[i] = sw.gSparse.view;
-            vf[i]  = sw.gFull.view;
-        });
-        if (p.densePoseRefinement)
-            for (int i = 0; i < n; i++)
+                vd[i] = X.sceneWork[i]->gDense.view;
                 if (sceneHasCand[i])
                     nDenseTot += X.sceneWork[i]->nd;
-        GridView *gS = X.batchGrids.as<GridView>((size_t)2 * n);
+            }
+        }
+        GridView *gS = X.batchGrids.as<GridView>((size_t)3 * n);
         PPF_HIP_CHECK(hipMemcpyAsync(gS, vs1.data(), sizeof(GridView) * n, hipMemcpyHostToDevice, s));
         PPF_HIP_CHECK(hipMemcpyAsync(gS + n, vf.data(), sizeof(GridView) * n, hipMemcpyHostToDevice,
                                      s));
+        if (p.densePoseRefinement)
+            PPF_HIP_CHECK(hipMemcpyAsync(gS + 2 * n, vd.data(), sizeof(GridView) * n,
+                                         hipMemcpyHostToDevice, s));
         std::vector<float> init(cand.size() * 12);
         std::vector<int>   hs(cand.size());
         for (size_t q = 0; q < cand.size(); q++) {
@@ -1087,26 +1179,6 @@ void match_batch_impl(ppf_detector_s &D, MatchCtx &X, const std::vector<DevCloud
             survivors.push_back(cand[q]);
         }
         if (p.densePoseRefinement && !survivors.empty()) {
-            std::vector<char> needDense(n, 0);
-            for (auto &c : survivors)
-                needDense[c.scene] = 1;
-            std::vector<GridView> vd(n);
-            for_scenes(X, n, [&](int i, Workspace &, hipStream_t ls) {
-                if (!needDense[i])
-                    return;
-                DevCloud &c  = *clouds[i];
-                auto     &sw = *X.sceneWork[i];
-                float *PX = c.px.get<float>(), *PY = c.py.get<float>(), *PZ = c.pz.get<float>();
-                float *NX = c.nx.get<float>(), *NY = c.ny.get<float>(), *NZ = c.nz.get<float>();
-                grid_build(sw.gDense, PX, PY, PZ, NX, NY, NZ, sw.dense.get<int>(), sw.nd,
-                           reSampleStep * kDenseCell, ls, c.ext);
-                vd[i] = sw.gDense.view;
-            });
-            GridView *gD = X.batchGrids.as<GridView>((size_t)3 * n);
-            PPF_HIP_CHECK(hipMemcpyAsync(gD + 2 * n, vd.data(), sizeof(GridView) * n,
-                                         hipMemcpyHostToDevice, s));
-            PPF_HIP_CHECK(hipMemcpyAsync(gD + n, vf.data(), sizeof(GridView) * n,
-                                         hipMemcpyHostToDevice, s));
             std::vector<float> init2(survivors.size() * 12);
             std::vector<int>   hs2(survivors.size());
             for (size_t q = 0; q < survivors.size(); q++) {
@@ -1123,8 +1195,8 @@ void match_batch_impl(ppf_detector_s &D, MatchCtx &X, const std::vector<DevCloud
             icp_batch(X.ws, hm.reSampled.size(), dm.rx.get<float>(), dm.ry.get<float>(),
                       dm.rz.get<float>(), dm.rnx.get<float>(), dm.rny.get<float>(),
                       dm.rnz.get<float>(), vd[first2], vf[first2], init2.data(),
-                      (int)survivors.size(), dc, scoringDist, D.icfg, res2.data(), s, gD + 2 * n,
-                      gD + n, hsD2);
+                      (int)survivors.size(), dc, scoringDist, D.icfg, res2.data(), s, gS + 2 * n,
+                      gS + n, hsD2);
             nDenseIcp = (int)survivors.size();
             for (size_t q = 0; q < survivors.size(); q++) {
                 const int i = survivors[q].scene, h = survivors[q].h;

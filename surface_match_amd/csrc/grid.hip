@@ -179,6 +179,16 @@ __global__ void gather3_kernel(const int *idx, int m, const float *A, const floa
     c[s]  = Cc[i];
 }
 
+__global__ void grid_clear_kernel(uint32_t *a, size_t na, uint32_t *b, size_t nb) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < na + nb; i += stride) {
+        if (i < na)
+            a[i] = 0u;
+        else
+            b[i - na] = 0u;
+    }
+}
+
 void grid_build(Grid &g, const float *X, const float *Y, const float *Z, const float *NX,
                 const float *NY, const float *NZ, const int *subset, int m, float cellHint,
                 hipStream_t s, const float *hostBox6) {
@@ -222,8 +232,6 @@ void grid_build(Grid &g, const float *X, const float *Y, const float *Z, const f
     float    *z     = g.z.as<float>(std::max(m, 1));
     const size_t cells = (size_t)dims[0] * dims[1] * dims[2];
     const bool   dense = cells <= GRID_DENSE_MAX;  // cell lookups by a dense start array
-    if (!dense)
-        PPF_HIP_CHECK(hipMemsetAsync(htab, 0, sizeof(uint4) * hsize, s));
     int sh = 0;
     while (sh < 2 && ((size_t)(((dims[0] - 1) >> sh) + 1) * (size_t)(((dims[1] - 1) >> sh) + 1) *
                       (size_t)(((dims[2] - 1) >> sh) + 1)) > OCC_MAX_BITS)
@@ -233,7 +241,12 @@ void grid_build(Grid &g, const float *X, const float *Y, const float *Z, const f
     const size_t occWords = ((size_t)odx * ody * odz + 31) / 32;
     uint32_t    *occ      = g.occ.as<uint32_t>(occWords);
     int         *cstart   = nullptr;
-    PPF_HIP_CHECK(hipMemsetAsync(occ, 0, sizeof(uint32_t) * occWords, s));
+    {
+        // the cell table (hashed cells only) and the occupancy bits, cleared in one launch
+        const size_t words = (dense ? 0 : (size_t)4 * hsize) + occWords;
+        grid_clear_kernel<<<(int)std::min<size_t>((words + 255) / 256, 2048), 256, 0, s>>>(
+            reinterpret_cast<uint32_t *>(htab), dense ? 0 : (size_t)4 * hsize, occ, occWords);
+    }
     if (m > 0) {
         grid_keys_kernel<<<div_up(m, 256), 256, 0, s>>>(X, Y, Z, subset, m, hb[0], hb[1], hb[2],
                                                          cell, dims[0], dims[1], dims[2], keys,
@@ -367,6 +380,14 @@ __global__ void iota_kernel(int *o, int m) {
         o[t] = t;
 }
 
+// an empty voxel table: no key, no best point
+__global__ void vs_clear_kernel(unsigned long long *tkey, unsigned long long *tbest, uint32_t n) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        tkey[i]  = 0ull;
+        tbest[i] = ~0ull;
+    }
+}
+
 int voxel_sample(Workspace &ws, const float *X, const float *Y, const float *Z,
                  const int *subset, int m, float step, const float bmin[3], int *out,
                  int *countDev, hipStream_t s) {
@@ -393,8 +414,8 @@ int voxel_sample(Workspace &ws, const float *X, const float *Y, const float *Z,
             iota_kernel<<<div_up(m, 256), 256, 0, s>>>(io, m);
         src = io;
     }
-    PPF_HIP_CHECK(hipMemsetAsync(a.tkey, 0, sizeof(unsigned long long) * hsize, s));
-    PPF_HIP_CHECK(hipMemsetAsync(a.tbest, 0xff, sizeof(unsigned long long) * hsize, s));
+    vs_clear_kernel<<<std::min<uint32_t>(div_up(hsize, 256), 2048), 256, 0, s>>>(a.tkey, a.tbest,
+                                                                                  hsize);
     if (m > 0) {
         voxel_insert_kernel<<<div_up(m, 256), 256, 0, s>>>(a);
         voxel_flag_kernel<<<div_up(m, 256), 256, 0, s>>>(a, flags);
@@ -415,13 +436,6 @@ int voxel_sample(Workspace &ws, const float *X, const float *Y, const float *Z,
 // workgroup's scan, a scatter)
 namespace {
 constexpr int VS_BLOCK = 256;
-
-__global__ void vs_clear_kernel(unsigned long long *tkey, unsigned long long *tbest, uint32_t n) {
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        tkey[i]  = 0ull;
-        tbest[i] = ~0ull;
-    }
-}
 
 __global__ void __launch_bounds__(VS_BLOCK) vs_count_kernel(const char *flags, int m, int *blockCnt) {
     __shared__ int sScan[VS_BLOCK / 64 + 1];
@@ -874,6 +888,34 @@ __global__ void normals_orient_kernel(NormArgs a, const float *outN, float *PX, 
 }
 
 
+// the call's snapshot of positions and normals
+__global__ void snapshot6_kernel(int n, const float *a0, const float *a1, const float *a2,
+                                 const float *a3, const float *a4, const float *a5, float *b0,
+                                 float *b1, float *b2, float *b3, float *b4, float *b5) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        b0[i] = a0[i];
+        b1[i] = a1[i];
+        b2[i] = a2[i];
+        b3[i] = a3[i];
+        b4[i] = a4[i];
+        b5[i] = a5[i];
+    }
+}
+
+// per-point flags and the four counters, zeroed
+__global__ void normals_clear_kernel(int n, int *have, int *touch, int *mark, int *cnt) {
+    const int m = n > 4 ? n : 4;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gridDim.x * blockDim.x) {
+        if (i < n) {
+            have[i]  = 0;
+            touch[i] = 0;
+            mark[i]  = 0;
+        }
+        if (i < 4)
+            cnt[i] = 0;
+    }
+}
+
 void estimate_normals(Workspace &ws, int n, float *PX, float *PY, float *PZ, float *NX, float *NY,
                       float *NZ, const int *valid, int nValid, const int *query, int nq, int k,
                       int smooth, int invert, const float vp[3], float cellHint, hipStream_t s,
@@ -885,12 +927,9 @@ void estimate_normals(Workspace &ws, int n, float *PX, float *PY, float *PZ, flo
     // snapshot copies
     float *P0x = ws.p0x.as<float>(n), *P0y = ws.p0y.as<float>(n), *P0z = ws.p0z.as<float>(n);
     float *N0x = ws.n0x.as<float>(n), *N0y = ws.n0y.as<float>(n), *N0z = ws.n0z.as<float>(n);
-    PPF_HIP_CHECK(hipMemcpyAsync(P0x, PX, sizeof(float) * n, hipMemcpyDeviceToDevice, s));
-    PPF_HIP_CHECK(hipMemcpyAsync(P0y, PY, sizeof(float) * n, hipMemcpyDeviceToDevice, s));
-    PPF_HIP_CHECK(hipMemcpyAsync(P0z, PZ, sizeof(float) * n, hipMemcpyDeviceToDevice, s));
-    PPF_HIP_CHECK(hipMemcpyAsync(N0x, NX, sizeof(float) * n, hipMemcpyDeviceToDevice, s));
-    PPF_HIP_CHECK(hipMemcpyAsync(N0y, NY, sizeof(float) * n, hipMemcpyDeviceToDevice, s));
-    PPF_HIP_CHECK(hipMemcpyAsync(N0z, NZ, sizeof(float) * n, hipMemcpyDeviceToDevice, s));
+    if (n > 0)
+        snapshot6_kernel<<<std::min(div_up(n, 256), 4096), 256, 0, s>>>(
+            n, PX, PY, PZ, NX, NY, NZ, P0x, P0y, P0z, N0x, N0y, N0z);
     grid_build(ws.gNormal, P0x, P0y, P0z, nullptr, nullptr, nullptr, valid, nValid, cellHint, s,
                hostBox6);
 
@@ -919,11 +958,9 @@ void estimate_normals(Workspace &ws, int n, float *PX, float *PY, float *PZ, flo
     a.mark  = ws.mark.as<int>(n);
     a.qidx  = ws.qidx.as<int>(n);
     float *outN = ws.outN.as<float>((size_t)std::max(nq, 1) * 3);
-    PPF_HIP_CHECK(hipMemsetAsync(a.have, 0, sizeof(int) * n, s));
-    PPF_HIP_CHECK(hipMemsetAsync(a.touch, 0, sizeof(int) * n, s));
-    PPF_HIP_CHECK(hipMemsetAsync(a.mark, 0, sizeof(int) * n, s));
     int *cntD = ws.counter.as<int>(4);
-    PPF_HIP_CHECK(hipMemsetAsync(cntD, 0, sizeof(int) * 4, s));
+    normals_clear_kernel<<<std::min(div_up(std::max(n, 4), 256), 4096), 256, 0, s>>>(
+        n, a.have, a.touch, a.mark, cntD);
     if (nq > 0)
         normals_mark_kernel<<<div_up(nq, 256), 256, 0, s>>>(a, cntD + 2);
     // ids with mark == want in the grid's cell order (their count to cntD[0])
