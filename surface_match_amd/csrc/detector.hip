@@ -869,6 +869,20 @@ struct LanePasses {
         int dev = 0;
         PPF_HIP_CHECK(hipGetDevice(&dev));
         err.resize(L);
+        try {
+            start(L, dev);
+        } catch (...) {
+            // a thread that could not be started: the started ones skip the
+            // second pass and are joined before the error leaves
+            int z = 0;
+            gate.compare_exchange_strong(z, -1);
+            for (auto &t : th)
+                if (t.joinable())
+                    t.join();
+            throw;
+        }
+    }
+    void start(int L, int dev) {
         for (int l = 0; l < L; l++)
             th.emplace_back([this, l, dev] {
                 try {
