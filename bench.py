@@ -571,7 +571,17 @@ def main():
                      "binding": binding(kname, lds_rate, ceil),
                      "vote_stage": {"algorithmic_bytes": vote_algorithmic_bytes(st),
                                     "ms": round(vote_ms, 4),
-                                    "GBps": round(vote_algorithmic_bytes(st) / (vote_ms * 1e-3) / 1e9, 1)}},
+                                    "GBps": round(vote_algorithmic_bytes(st) / (vote_ms * 1e-3) / 1e9, 1)},
+                     # both kernels of the split vote path (they take about the
+                     # same time, so either can be the dominant one in a run)
+                     "vote_kernels": {k: {"ms": round(ms_, 4), "algorithmic_bytes": b,
+                                          "GBps": round(b / (ms_ * 1e-3) / 1e9, 1),
+                                          "frac": round(b / (ms_ * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                                          "binding": BINDING[k]}
+                                      for k, ms_, b in (("pair_kernel", pair_ms, pair_algorithmic_bytes(st)),
+                                                        ("vote_hits_kernel", hits_ms,
+                                                         hits_algorithmic_bytes(st)))
+                                      if ms_ > 0}},
         "results": found,
         "tuning": det.get_tuning(),
     }
