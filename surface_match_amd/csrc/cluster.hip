@@ -126,18 +126,22 @@ __global__ void __launch_bounds__(1024) cluster_rounds_kernel(SortedPoses s, int
 // rounds, so short lists finish in one pass instead of a chain of
 // workgroup-wide rounds.
 constexpr int CLUSTER_SEQ_MAX = 512;
+// several scenes at once, one wave each: the walks run side by side, so
+// longer lists pay (80 KB of LDS at the limit)
+constexpr int CLUSTER_MULTI_MAX = 4096;
 
-__global__ void __launch_bounds__(64) cluster_seq_kernel(SortedPoses s, int n, float dThr,
-                                                         float aThr, int *status) {
+// poses [b, b + n) of the sorted list, status written at the same
+// positions; centres kept in sC (5 words each, capacity n)
+__device__ void cluster_seq_range(const SortedPoses &s, int b, int n, float dThr, float aThr,
+                                  int *status, float *sC) {
     // centres in creation order: x, y, z, angle, sorted position
-    extern __shared__ float sC[];
     float    *cx = sC, *cy = sC + n, *cz = sC + 2 * n, *ca = sC + 3 * n;
     int      *ci = reinterpret_cast<int *>(sC + 4 * n);
     const int lane = threadIdx.x;
     int       nc   = 0;  // wave-uniform
     for (int b0 = 0; b0 < n; b0 += 64) {
-        const int  i     = b0 + lane;
-        const bool valid = i < n;
+        const int  i     = b + b0 + lane;
+        const bool valid = b0 + lane < n;
         float      tx = 0.0f, ty = 0.0f, tz = 0.0f, ta = 0.0f;
         if (valid) {
             tx = s.tx[i];
@@ -177,7 +181,7 @@ __global__ void __launch_bounds__(64) cluster_seq_kernel(SortedPoses s, int n, f
             const unsigned long long mm = __ballot(m);
             if (mm) {
                 if (lane == u)
-                    st = b0 + __ffsll((long long)mm) - 1;
+                    st = b + b0 + __ffsll((long long)mm) - 1;
             } else {
                 if (lane == u) {
                     st     = CENTER;
@@ -197,6 +201,21 @@ __global__ void __launch_bounds__(64) cluster_seq_kernel(SortedPoses s, int n, f
         if (valid)
             status[i] = st;
     }
+}
+
+__global__ void __launch_bounds__(64) cluster_seq_kernel(SortedPoses s, int n, float dThr,
+                                                         float aThr, int *status) {
+    extern __shared__ float sC[];
+    cluster_seq_range(s, 0, n, dThr, aThr, status, sC);
+}
+
+// one wave per scene: scene w's poses are sorted positions [off[w], off[w + 1])
+__global__ void __launch_bounds__(64) cluster_seq_multi_kernel(SortedPoses s, const int *off,
+                                                               float dThr, float aThr,
+                                                               int *status) {
+    extern __shared__ float sC[];
+    const int b = off[blockIdx.x];
+    cluster_seq_range(s, b, off[blockIdx.x + 1] - b, dThr, aThr, status, sC);
 }
 
 // cluster index of each centre = its rank among centres (creation order)
@@ -356,11 +375,9 @@ __global__ void __launch_bounds__(1024) nms_kernel(const Hypo *h, const int *ord
 // near.  Used while the kept list fits in LDS.
 constexpr int NMS_SEQ_MAX = 4096;
 
-__global__ void __launch_bounds__(64) nms_seq_kernel(const Hypo *h, const int *ord, int C,
-                                                     const float *center, float thr, int *status,
-                                                     Hypo *out, int *nOut) {
+__device__ void nms_seq_range(const Hypo *h, const int *ord, int C, const float *center,
+                              float thr, int *status, Hypo *out, int *nOut, float *sKept) {
     // kept: x[C], y[C], z[C], pose index[C]; then every pose's centre x/y/z[C]
-    extern __shared__ float sKept[];
     float      *kx = sKept, *ky = sKept + C, *kz = sKept + 2 * C;
     int        *ki = reinterpret_cast<int *>(sKept + 3 * C);
     float      *px = sKept + 4 * C, *py = sKept + 5 * C, *pz = sKept + 6 * C;
@@ -453,6 +470,26 @@ __global__ void __launch_bounds__(64) nms_seq_kernel(const Hypo *h, const int *o
     }
     if (lane == 0)
         *nOut = nk;
+}
+
+__global__ void __launch_bounds__(64) nms_seq_kernel(const Hypo *h, const int *ord, int C,
+                                                     const float *center, float thr, int *status,
+                                                     Hypo *out, int *nOut) {
+    extern __shared__ float sKept[];
+    nms_seq_range(h, ord, C, center, thr, status, out, nOut, sKept);
+}
+
+// one wave per scene: scene w's clusters are the sorted positions
+// [off[w], off[w + 1]) of ord; its kept poses go to out + off[w], their
+// count to nOut[w]
+__global__ void __launch_bounds__(64) nms_seq_multi_kernel(const Hypo *h, const int *ord,
+                                                           const int *off, const float *center,
+                                                           float thr, int *status, Hypo *out,
+                                                           int *nOut) {
+    extern __shared__ float sKept[];
+    const int b = off[blockIdx.x];
+    nms_seq_range(h, ord + b, off[blockIdx.x + 1] - b, center, thr, status + b, out + b,
+                  nOut + blockIdx.x, sKept);
 }
 
 __global__ void hypo_keys_kernel(const Hypo *h, int n, uint32_t *keys, int *vals) {
@@ -572,6 +609,186 @@ void cluster_poses(Workspace &ws, const PoseRec *poses, int np, float dThr, floa
     PPF_HIP_CHECK(hipMemcpyAsync(out.data(), kept, sizeof(Hypo) * nk, hipMemcpyDeviceToHost, s));
     PPF_HIP_CHECK(hipStreamSynchronize(s));
     PPF_HIP_CHECK(hipGetLastError());
+}
+
+// ---- several scenes at once (the batched match): the same steps over the
+// concatenated pose list, every sort keyed by (scene, key) so each scene's
+// poses and clusters stay contiguous and in their own order, one wave per
+// scene for the ordered walks
+
+bool cluster_multi_fits(const std::vector<int> &pOff) {
+    for (size_t w = 0; w + 1 < pOff.size(); w++)
+        if (pOff[w + 1] - pOff[w] > CLUSTER_MULTI_MAX)
+            return false;
+    return true;
+}
+
+// scene w of item i: off[w] <= i < off[w + 1] (off non-decreasing)
+__device__ __forceinline__ int scene_of(const int *off, int S, int i) {
+    int lo = 0, hi = S;  // first w with off[w] > i, minus one
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (off[mid] <= i)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo - 1;
+}
+
+__global__ void vote_keys_multi_kernel(const PoseRec *p, int n, const int *off, int S,
+                                       unsigned long long *keys, int *vals) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    keys[i] = ((unsigned long long)scene_of(off, S, i) << 32) |
+              (0xFFFFFFFFu - __float_as_uint(p[i].votes));
+    vals[i] = i;
+}
+
+// each scene's first cluster index (centres are ranked scene by scene, in
+// creation order), and the total at off[S]
+__global__ void scene_cluster_off_kernel(const int *rank, const int *isC, const int *off, int S,
+                                         int np, int *cOff) {
+    int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w > S)
+        return;
+    const int total = np > 0 ? rank[np - 1] + isC[np - 1] : 0;
+    cOff[w]         = off[w] < np ? rank[off[w]] : total;
+}
+
+__global__ void hypo_keys_multi_kernel(const Hypo *h, int n, const int *cOff, int S,
+                                       unsigned long long *keys, int *vals) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    keys[i] = ((unsigned long long)scene_of(cOff, S, i) << 32) |
+              (0xFFFFFFFFu - __float_as_uint(h[i].votes));
+    vals[i] = i;
+}
+
+static int bits_for(long long v) {
+    int b = 1;
+    while ((1ll << b) < v)
+        b++;
+    return b;
+}
+
+void cluster_poses_multi(Workspace &ws, const PoseRec *poses, const std::vector<int> &pOff,
+                         float dThr, float aThr, const float center[3], float nmsThr,
+                         std::vector<std::vector<Hypo>> &out, std::vector<int> &nClusters,
+                         hipStream_t s) {
+    const int S  = (int)pOff.size() - 1;
+    const int np = S > 0 ? pOff[S] : 0;
+    out.assign(std::max(S, 0), {});
+    nClusters.assign(std::max(S, 0), 0);
+    if (np <= 0)
+        return;
+    int maxNp = 0;
+    for (int w = 0; w < S; w++)
+        maxNp = std::max(maxNp, pOff[w + 1] - pOff[w]);
+    if (maxNp > CLUSTER_MULTI_MAX)
+        throw std::logic_error("cluster_poses_multi: a scene holds more poses than its walk keeps");
+    unsigned long long *ka = ws.clKey64.as<unsigned long long>(np),
+                       *kb = ws.clKey64b.as<unsigned long long>(np);
+    uint32_t *k1 = ws.clSortKey.as<uint32_t>(np), *k3 = ws.clPred.as<uint32_t>(np);
+    int      *v1 = ws.clSortVal.as<int>(np), *v2 = ws.clSortVal2.as<int>(np);
+    float    *buf     = ws.clCenters.as<float>((size_t)np * 6);
+    int      *status  = ws.clStatus.as<int>((size_t)np * 4 + 8);
+    int      *members = ws.clMembers.as<int>((size_t)np * 2 + 2);
+    int      *cntsBuf = ws.clPredOff.as<int>((size_t)np * 3 + 4);
+    Hypo     *centers = ws.clNms.as<Hypo>((size_t)np * 2 + 2);
+    int      *sOff    = ws.clSceneOff.as<int>((size_t)3 * (S + 1) + 4);
+    int      *cOff    = sOff + (S + 1);
+    int      *nOut    = cOff + (S + 1);
+    float    *cdev    = reinterpret_cast<float *>(nOut + S + 1);
+    PPF_HIP_CHECK(hipMemcpyAsync(sOff, pOff.data(), sizeof(int) * (S + 1), hipMemcpyHostToDevice, s));
+    PPF_HIP_CHECK(hipMemcpyAsync(cdev, center, sizeof(float) * 3, hipMemcpyHostToDevice, s));
+    const int sceneBits = bits_for(S);
+    vote_keys_multi_kernel<<<div_up(np, 256), 256, 0, s>>>(poses, np, sOff, S, ka, v1);
+    size_t tb = 0;
+    PPF_HIP_CHECK(rocprim::radix_sort_pairs(nullptr, tb, ka, kb, v1, v2, np, 0, 32 + sceneBits, s));
+    PPF_HIP_CHECK(rocprim::radix_sort_pairs(ws.cubTmp.reserve(tb), tb, ka, kb, v1, v2, np, 0,
+                                            32 + sceneBits, s));
+    int *ord = v2;  // sorted position -> pose index
+    SortedPoses sp{buf, buf + np, buf + 2 * (size_t)np, buf + 3 * (size_t)np, buf + 4 * (size_t)np,
+                   ord};
+    gather_sorted_kernel<<<div_up(np, 256), 256, 0, s>>>(poses, ord, np, sp);
+    {
+        const size_t lds = (size_t)20 * maxNp;
+        PPF_HIP_CHECK(hipFuncSetAttribute((const void *)cluster_seq_multi_kernel,
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        cluster_seq_multi_kernel<<<S, 64, lds, s>>>(sp, sOff, dThr, aThr, status);
+    }
+    int *rank = status + np;
+    int *isC  = status + 2 * (size_t)np;
+    int *misc = status + 3 * (size_t)np;
+    center_rank_kernel<<<div_up(np, 256), 256, 0, s>>>(status, np, isC);
+    tb = 0;
+    PPF_HIP_CHECK(rocprim::exclusive_scan(nullptr, tb, isC, rank, 0, np, rocprim::plus<int>(), s));
+    PPF_HIP_CHECK(rocprim::exclusive_scan(ws.cubTmp.reserve(tb), tb, isC, rank, 0, np,
+                                          rocprim::plus<int>(), s));
+    scene_cluster_off_kernel<<<div_up(S + 1, 256), 256, 0, s>>>(rank, isC, sOff, S, np, cOff);
+    std::vector<int> co(S + 1);
+    PPF_HIP_CHECK(hipMemcpyAsync(co.data(), cOff, sizeof(int) * (S + 1), hipMemcpyDeviceToHost, s));
+    PPF_HIP_CHECK(hipStreamSynchronize(s));
+    const int nc = co[S];
+    int       maxC = 0;
+    for (int w = 0; w < S; w++) {
+        nClusters[w] = co[w + 1] - co[w];
+        maxC         = std::max(maxC, nClusters[w]);
+    }
+    // members grouped by cluster (clusters numbered scene by scene), ascending
+    // sorted position inside
+    member_cluster_kernel<<<div_up(np, 256), 256, 0, s>>>(status, rank, np, k1, v1);
+    const int endBit = bits_for((long long)nc + 1);
+    tb = 0;
+    PPF_HIP_CHECK(rocprim::radix_sort_pairs(nullptr, tb, k1, k3, v1, members, np, 0, endBit, s));
+    PPF_HIP_CHECK(rocprim::radix_sort_pairs(ws.cubTmp.reserve(tb), tb, k1, k3, v1, members, np, 0,
+                                            endBit, s));
+    int      *mOff = members + np;
+    int      *cnts = cntsBuf;
+    uint32_t *uk   = k1;
+    int      *nr   = misc + 1;
+    tb             = 0;
+    PPF_HIP_CHECK(rocprim::run_length_encode(nullptr, tb, k3, np, uk, cnts, nr, s));
+    PPF_HIP_CHECK(rocprim::run_length_encode(ws.cubTmp.reserve(tb), tb, k3, np, uk, cnts, nr, s));
+    PPF_HIP_CHECK(hipMemsetAsync(cnts + nc, 0, sizeof(int), s));
+    tb = 0;
+    PPF_HIP_CHECK(rocprim::exclusive_scan(nullptr, tb, cnts, mOff, 0, nc + 1, rocprim::plus<int>(), s));
+    PPF_HIP_CHECK(rocprim::exclusive_scan(ws.cubTmp.reserve(tb), tb, cnts, mOff, 0, nc + 1,
+                                          rocprim::plus<int>(), s));
+    float *angles = buf + (size_t)np * 5;
+    cluster_avg_kernel<<<div_up(nc, 64), 64, 0, s>>>(poses, ord, members, mOff, nc, centers, angles);
+    // each scene's clusters by votes (stable on creation order), then its NMS
+    int *hv = cntsBuf + nc + 2, *hv2 = status;  // status is free again (nc <= np)
+    hypo_keys_multi_kernel<<<div_up(nc, 256), 256, 0, s>>>(centers, nc, cOff, S, ka, hv);
+    tb = 0;
+    PPF_HIP_CHECK(rocprim::radix_sort_pairs(nullptr, tb, ka, kb, hv, hv2, nc, 0, 32 + sceneBits, s));
+    PPF_HIP_CHECK(rocprim::radix_sort_pairs(ws.cubTmp.reserve(tb), tb, ka, kb, hv, hv2, nc, 0,
+                                            32 + sceneBits, s));
+    int  *nmsStatus = rank;  // reuse (nc <= np)
+    Hypo *kept      = centers + nc;
+    if (maxC <= NMS_SEQ_MAX) {
+        const size_t lds = (size_t)28 * std::max(maxC, 1);
+        PPF_HIP_CHECK(hipFuncSetAttribute((const void *)nms_seq_multi_kernel,
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        nms_seq_multi_kernel<<<S, 64, lds, s>>>(centers, hv2, cOff, cdev, nmsThr, nmsStatus, kept,
+                                                nOut);
+    } else {
+        float *tr = buf;  // reuse
+        for (int w = 0; w < S; w++)
+            nms_kernel<<<1, 1024, 0, s>>>(centers, hv2 + co[w], co[w + 1] - co[w], cdev, nmsThr,
+                                          nmsStatus + co[w], tr, kept + co[w], nOut + w);
+    }
+    std::vector<int>  nk(S);
+    std::vector<Hypo> all(std::max(nc, 1));
+    PPF_HIP_CHECK(hipMemcpyAsync(nk.data(), nOut, sizeof(int) * S, hipMemcpyDeviceToHost, s));
+    PPF_HIP_CHECK(hipMemcpyAsync(all.data(), kept, sizeof(Hypo) * nc, hipMemcpyDeviceToHost, s));
+    PPF_HIP_CHECK(hipStreamSynchronize(s));
+    PPF_HIP_CHECK(hipGetLastError());
+    for (int w = 0; w < S; w++)
+        out[w].assign(all.begin() + co[w], all.begin() + co[w] + nk[w]);
 }
 
 }  // namespace ppfd

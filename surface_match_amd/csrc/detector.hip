@@ -1037,13 +1037,12 @@ void match_batch_impl(ppf_detector_s &D, MatchCtx &X, const std::vector<DevCloud
                                      hipMemcpyHostToDevice, s));
     }
     PPF_HIP_CHECK(hipEventRecord(X.ev[3], s));
-    // ---- per scene, on the lanes while the vote runs: the refinement's
-    // dense samples with their normals (ppf.cpp:427-434) and its grids over
-    // the final positions, as the single-scene match prepares them beside
-    // its vote (they write only points without a normal, and the vote reads
-    // the scene only through its grids' copies and the key points gathered
-    // above); then, once the vote is in, each scene's clustering
-    // (ppf.cpp:413-420)
+    // ---- per scene, on the lanes while the vote and the clustering
+    // (ppf.cpp:413-420) run: the refinement's dense samples with their
+    // normals (ppf.cpp:427-434) and its grids over the final positions, as
+    // the single-scene match prepares them beside its vote (they write only
+    // points without a normal, and the vote reads the scene only through its
+    // grids' copies and the key points gathered above)
     const bool laneGrids = p.sparsePoseRefinement != 0;
     float center[3];
     for (int a = 0; a < 3; a++)
@@ -1079,10 +1078,10 @@ void match_batch_impl(ppf_detector_s &D, MatchCtx &X, const std::vector<DevCloud
             }
         },
         [&](int i, Workspace &ws, hipStream_t ls) {
-            // the vote's poses (its buffer may have grown during the vote)
-            const PoseRec *posesAll = X.ws.poses.get<PoseRec>();
-            cluster_poses(ws, posesAll + pOff[i], pOff[i + 1] - pOff[i], 0.1f * Dm, angleStep,
-                          center, maxOverlapDist, hyp[i], &ncs[i], ls);
+            // scenes with long pose lists: clustered one by one on the lanes
+            // (the vote's pose buffer may have grown during the vote)
+            cluster_poses(ws, X.ws.poses.get<PoseRec>() + pOff[i], pOff[i + 1] - pOff[i],
+                          0.1f * Dm, angleStep, center, maxOverlapDist, hyp[i], &ncs[i], ls);
         });
     // ---- every scene's key points in one vote
     if (K > 0) {
@@ -1110,7 +1109,15 @@ void match_batch_impl(ppf_detector_s &D, MatchCtx &X, const std::vector<DevCloud
             pOff[i] = offs[ko[i]];
     }
     PPF_HIP_CHECK(hipEventRecord(X.ev[5], s));
-    lanes.open();
+    // every scene's clustering in one pass over the vote's poses, beside the
+    // lanes' refinement preparation, when every scene's list is short enough;
+    // else scene by scene on the lanes after it
+    if (cluster_multi_fits(pOff)) {
+        cluster_poses_multi(X.ws, X.ws.poses.get<PoseRec>(), pOff, 0.1f * Dm, angleStep, center,
+                            maxOverlapDist, hyp, ncs, s);
+    } else {
+        lanes.open();
+    }
     lanes.join();
     int nClustersTot = 0;
     for (int i = 0; i < n; i++)

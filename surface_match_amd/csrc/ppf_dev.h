@@ -40,6 +40,7 @@ struct Workspace {
     // clustering
     DevBuf clSortKey, clSortKey2, clSortVal, clSortVal2, clStatus, clPred, clPredOff,
         clCenters, clNms, clMembers;
+    DevBuf clKey64, clKey64b, clSceneOff;  // clustering of several scenes at once
     // icp
     DevBuf icpPose, icpCur, icpState, icpHash, icpPartial, icpOut, icpInl, icpSt, icpArrive,
         icpPart;
@@ -240,6 +241,15 @@ struct Hypo {
 void cluster_poses(Workspace &ws, const PoseRec *posesDev, int np, float distThr, float angleThr,
                    const float center[3], float nmsThr, std::vector<Hypo> &out, int *nClusters,
                    hipStream_t s);
+// cluster_poses for several scenes at once: scene i's poses are
+// posesDev[pOff[i] .. pOff[i + 1]); out[i] and nClusters[i] equal what
+// cluster_poses returns for them alone.  Only for scenes of at most
+// CLUSTER_MULTI_MAX poses each (cluster_multi_fits).
+bool cluster_multi_fits(const std::vector<int> &pOff);
+void cluster_poses_multi(Workspace &ws, const PoseRec *posesDev, const std::vector<int> &pOff,
+                         float distThr, float angleThr, const float center[3], float nmsThr,
+                         std::vector<std::vector<Hypo>> &out, std::vector<int> &nClusters,
+                         hipStream_t s);
 
 // ---------------------------------------------------------------- ICP
 struct IcpCriteria {
