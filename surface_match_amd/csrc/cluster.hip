@@ -134,8 +134,8 @@ constexpr int CLUSTER_MULTI_MAX = 4096;
 // positions; centres kept in sC (5 words each, capacity n)
 __device__ void cluster_seq_range(const SortedPoses &s, int b, int n, float dThr, float aThr,
                                   int *status, float *sC) {
-    // centres in creation order: x, y, z, angle, sorted position
-    float    *cx = sC, *cy = sC + n, *cz = sC + 2 * n, *ca = sC + 3 * n;
+    // centres in creation order: {x, y, z, angle}, then sorted positions
+    float4   *cc = reinterpret_cast<float4 *>(sC);
     int      *ci = reinterpret_cast<int *>(sC + 4 * n);
     const int lane = threadIdx.x;
     int       nc   = 0;  // wave-uniform
@@ -158,12 +158,30 @@ __device__ void cluster_seq_range(const SortedPoses &s, int b, int n, float dThr
         };
         int       st  = UNDECIDED;
         const int nc0 = nc;
-        if (valid)
-            for (int t = 0; t < nc0; t++)
-                if (match(cx[t], cy[t], cz[t], ca[t])) {
-                    st = ci[t];
+        if (valid) {
+            // four centres per step, their loads issued together, tested in order
+            int t = 0;
+            for (; t + 3 < nc0; t += 4) {
+                const float4 c0 = cc[t], c1 = cc[t + 1], c2 = cc[t + 2], c3 = cc[t + 3];
+                const int    f  = match(c0.x, c0.y, c0.z, c0.w)   ? 0
+                                  : match(c1.x, c1.y, c1.z, c1.w) ? 1
+                                  : match(c2.x, c2.y, c2.z, c2.w) ? 2
+                                  : match(c3.x, c3.y, c3.z, c3.w) ? 3
+                                                                  : 4;
+                if (f < 4) {
+                    st = ci[t + f];
                     break;
                 }
+            }
+            if (st == UNDECIDED)
+                for (; t < nc0; t++) {
+                    const float4 c0 = cc[t];
+                    if (match(c0.x, c0.y, c0.z, c0.w)) {
+                        st = ci[t];
+                        break;
+                    }
+                }
+        }
         bool               opened = false;
         unsigned long long rest   = __ballot(valid && st == UNDECIDED);
         while (rest) {
@@ -186,10 +204,7 @@ __device__ void cluster_seq_range(const SortedPoses &s, int b, int n, float dThr
                 if (lane == u) {
                     st     = CENTER;
                     opened = true;
-                    cx[nc] = tx;
-                    cy[nc] = ty;
-                    cz[nc] = tz;
-                    ca[nc] = ta;
+                    cc[nc] = make_float4(tx, ty, tz, ta);
                     ci[nc] = i;
                 }
                 nc++;
@@ -377,9 +392,8 @@ constexpr int NMS_SEQ_MAX = 4096;
 
 __device__ void nms_seq_range(const Hypo *h, const int *ord, int C, const float *center,
                               float thr, int *status, Hypo *out, int *nOut, float *sKept) {
-    // kept: x[C], y[C], z[C], pose index[C]; then every pose's centre x/y/z[C]
-    float      *kx = sKept, *ky = sKept + C, *kz = sKept + 2 * C;
-    int        *ki = reinterpret_cast<int *>(sKept + 3 * C);
+    // kept: {x, y, z, pose index bits}[C]; then every pose's centre x/y/z[C]
+    float4     *kp = reinterpret_cast<float4 *>(sKept);
     float      *px = sKept + 4 * C, *py = sKept + 5 * C, *pz = sKept + 6 * C;
     const int   lane = threadIdx.x;
     const float sq   = thr * thr;
@@ -405,14 +419,34 @@ __device__ void nms_seq_range(const Hypo *h, const int *ord, int C, const float 
         // the first pose kept before this block within the distance
         int       st  = UNDECIDED;
         const int nk0 = nk;
-        if (valid)
-            for (int t = 0; t < nk0; t++) {
-                const float dx = kx[t] - tx, dy = ky[t] - ty, dz = kz[t] - tz;
-                if (dx * dx + dy * dy + dz * dz < sq) {
-                    st = ki[t];
+        auto      near_kept = [&](const float4 &k) {
+            const float dx = k.x - tx, dy = k.y - ty, dz = k.z - tz;
+            return dx * dx + dy * dy + dz * dz < sq;
+        };
+        if (valid) {
+            // four kept poses per step, their loads issued together, tested in order
+            int t = 0;
+            for (; t + 3 < nk0; t += 4) {
+                const float4 k0 = kp[t], k1 = kp[t + 1], k2 = kp[t + 2], k3 = kp[t + 3];
+                const float  w  = near_kept(k0)   ? k0.w
+                                  : near_kept(k1) ? k1.w
+                                  : near_kept(k2) ? k2.w
+                                  : near_kept(k3) ? k3.w
+                                                  : __int_as_float(UNDECIDED);
+                if (__float_as_int(w) != UNDECIDED) {
+                    st = __float_as_int(w);
                     break;
                 }
             }
+            if (st == UNDECIDED)
+                for (; t < nk0; t++) {
+                    const float4 k0 = kp[t];
+                    if (near_kept(k0)) {
+                        st = __float_as_int(k0.w);
+                        break;
+                    }
+                }
+        }
         // the rest in order, against the poses kept in this block so far
         bool               keptHere = false;
         unsigned long long rest     = __ballot(valid && st == UNDECIDED);
@@ -433,10 +467,7 @@ __device__ void nms_seq_range(const Hypo *h, const int *ord, int C, const float 
                 if (lane == u) {
                     st       = CENTER;
                     keptHere = true;
-                    kx[nk]   = tx;
-                    ky[nk]   = ty;
-                    kz[nk]   = tz;
-                    ki[nk]   = j;
+                    kp[nk]   = make_float4(tx, ty, tz, __int_as_float(j));
                 }
                 nk++;
             }
@@ -461,9 +492,22 @@ __device__ void nms_seq_range(const Hypo *h, const int *ord, int C, const float 
     // kept poses in order, each with its members' votes added in ascending
     // member order (the reference's accumulation order)
     for (int t = lane; t < nk; t += 64) {
-        const int i = ki[t];
+        const int i = __float_as_int(kp[t].w);
         Hypo      o = h[ord[i]];
-        for (int j = i + 1; j < C; j++)
+        int       j = i + 1;
+        // four statuses per step, loads issued together, votes added in order
+        for (; j + 3 < C; j += 4) {
+            const int s0 = sStat[j], s1 = sStat[j + 1], s2 = sStat[j + 2], s3 = sStat[j + 3];
+            if (s0 == i)
+                o.votes = o.votes + sVotes[j];
+            if (s1 == i)
+                o.votes = o.votes + sVotes[j + 1];
+            if (s2 == i)
+                o.votes = o.votes + sVotes[j + 2];
+            if (s3 == i)
+                o.votes = o.votes + sVotes[j + 3];
+        }
+        for (; j < C; j++)
             if (sStat[j] == i)
                 o.votes = o.votes + sVotes[j];
         out[t] = o;
