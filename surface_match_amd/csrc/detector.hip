@@ -594,7 +594,8 @@ void match_impl(ppf_detector_s &D, MatchCtx &X, DevCloud &c, float rel, float kp
                            mr->nSampled);
     }
     // vote (ppf.cpp:247-411)
-    grid_build(X.ws.gVote, PX, PY, PZ, NX, NY, NZ, samp, ns, Dm / (float)D.vcfg.cellDiv, s, c.ext);
+    grid_build(X.ws.gVote, PX, PY, PZ, NX, NY, NZ, samp, ns, Dm / (float)D.vcfg.cellDiv, s, c.ext,
+               true);
     // dense prep (ppf.cpp:427-434) on a lane, overlapping the vote and the
     // clustering: it samples the scene again and estimates normals only for
     // points without one (so never a sampled point, which all have one by
@@ -989,7 +990,7 @@ void match_batch_impl(ppf_detector_s &D, MatchCtx &X, const std::vector<DevCloud
         int *kp = sw.kp.as<int>(std::max(sw.ns, 1));
         sw.nk   = voxel_sample(ws, PX, PY, PZ, samp, sw.ns, keySampleStep, c.boxMin, kp, cn, ls);
         grid_build(sw.gVote, PX, PY, PZ, NX, NY, NZ, samp, sw.ns, Dm / (float)D.vcfg.cellDiv, ls,
-                   c.ext);
+                   c.ext, true);
     });
     std::vector<int> ko(n + 1, 0);
     float            minCell = FLT_MAX;
@@ -2207,7 +2208,7 @@ PPF_API int ppf_vote(ppf_detector d, const float *pts, const float *nrm, size_t 
         PPF_HIP_CHECK(hipMemcpyAsync(kp, keypoints, sizeof(int) * nk, hipMemcpyHostToDevice, s));
         grid_build(X.ws.gVote, c.px.get<float>(), c.py.get<float>(), c.pz.get<float>(),
                    c.nx.get<float>(), c.ny.get<float>(), c.nz.get<float>(), samp, (int)n,
-                   Dm / (float)d->vcfg.cellDiv, s);
+                   Dm / (float)d->vcfg.cellDiv, s, nullptr, true);
         DevBuf dump;
         int32_t *dd = nullptr;
         if (accDump && dumpCount) {

@@ -71,19 +71,22 @@ struct GridView {
     const float    *x, *y, *z;
     const float    *nx, *ny, *nz;  // optional (nullptr)
     const int      *id;            // original point id of each sorted slot
-    int             n;
     // occupancy of blocks of 2^occShift cells per axis (occShift 0: of every
     // cell), one bit per block, x fastest: lets searches skip empty space
     // without table probes
     const uint32_t *occ;
     int             odx, ody, occShift;
-    int             unused;  // no padding bytes: views are compared bytewise as graph keys
+    int             n;  // (no padding bytes: views are compared bytewise as graph keys)
     // grids of at most GRID_DENSE_MAX cells: the first sorted slot of every
     // cell (linear key), cstart[key + 1] - cstart[key] points; neighbouring
     // cells of a row are then neighbouring words instead of scattered table
     // slots.  nullptr: only the hash table.
     const int      *cstart;
+    // optional (nullptr) slot-ordered copies for the pair kernel, one 16-byte
+    // load each: {x, y, z, id bits} and {nx, ny, nz, 0}
+    const float4   *p4, *n4;
 };
+static_assert(sizeof(GridView) == 144, "GridView holds no padding bytes");
 
 constexpr size_t GRID_DENSE_MAX = size_t(1) << 24;
 
@@ -467,7 +470,7 @@ __device__ inline bool grid_any_within(const GridView &g, float qx, float qy, fl
 // host side of the grid
 // --------------------------------------------------------------------------
 struct Grid {
-    DevBuf   keys, keys2, vals, vals2, htab, x, y, z, nx, ny, nz, tmp, bbox, occ, cstart;
+    DevBuf   keys, keys2, vals, vals2, htab, x, y, z, nx, ny, nz, tmp, bbox, occ, cstart, p4, n4;
     GridView view{};
     int      capN = 0;
 };
@@ -477,10 +480,11 @@ struct Grid {
 // requested cell edge; it grows if the grid would exceed 1023 cells per
 // axis.  withNormals gathers NX/NY/NZ too.  Synchronises `s` once (bbox).
 // hostBox6 (min xyz, max xyz), when given, is a box known to contain the
-// points; it saves the device bounding box and its read-back.
+// points; it saves the device bounding box and its read-back.  aos (with
+// normals) also writes the view's p4 / n4 copies.
 void grid_build(Grid &g, const float *X, const float *Y, const float *Z, const float *NX,
                 const float *NY, const float *NZ, const int *subset, int m, float cellHint,
-                hipStream_t s, const float *hostBox6 = nullptr);
+                hipStream_t s, const float *hostBox6 = nullptr, bool aos = false);
 
 // block-wide exclusive scan of one int per thread (blockDim a multiple of 64, <= 1024)
 template <int BLOCK>

@@ -189,9 +189,21 @@ __global__ void grid_clear_kernel(uint32_t *a, size_t na, uint32_t *b, size_t nb
     }
 }
 
+// slot-ordered {x, y, z, id bits} and {nx, ny, nz, 0}
+__global__ void gather_aos_kernel(const int *idx, int m, const float *X, const float *Y,
+                                  const float *Z, const float *NX, const float *NY,
+                                  const float *NZ, float4 *p4, float4 *n4) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= m)
+        return;
+    int i = idx[s];
+    p4[s] = make_float4(X[i], Y[i], Z[i], __int_as_float(i));
+    n4[s] = make_float4(NX[i], NY[i], NZ[i], 0.0f);
+}
+
 void grid_build(Grid &g, const float *X, const float *Y, const float *Z, const float *NX,
                 const float *NY, const float *NZ, const int *subset, int m, float cellHint,
-                hipStream_t s, const float *hostBox6) {
+                hipStream_t s, const float *hostBox6, bool aos) {
     float hb[6];
     if (hostBox6) {
         // a box known to hold the points, widened by 1e-4 of its extent (a
@@ -291,12 +303,20 @@ void grid_build(Grid &g, const float *X, const float *Y, const float *Z, const f
     v.occShift = sh;
     v.cstart   = cstart;
     v.nx = v.ny = v.nz = nullptr;
+    v.p4 = v.n4 = nullptr;
     if (NX && m > 0) {
         float *nx = g.nx.as<float>(m), *ny = g.ny.as<float>(m), *nz = g.nz.as<float>(m);
         gather3_kernel<<<div_up(m, 256), 256, 0, s>>>(vals2, m, NX, NY, NZ, nx, ny, nz);
         v.nx = nx;
         v.ny = ny;
         v.nz = nz;
+        if (aos) {
+            float4 *p4 = g.p4.as<float4>(m), *n4 = g.n4.as<float4>(m);
+            gather_aos_kernel<<<div_up(m, 256), 256, 0, s>>>(vals2, m, X, Y, Z, NX, NY, NZ, p4,
+                                                             n4);
+            v.p4 = p4;
+            v.n4 = n4;
+        }
     }
     PPF_HIP_CHECK(hipGetLastError());
 }

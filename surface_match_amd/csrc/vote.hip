@@ -778,13 +778,15 @@ __global__ void __launch_bounds__(PAIR_BLOCK) pair_kernel(VoteArgs a) {
         if (lane < cnt) {
             const int      s    = cSlot[wv][lane];
             const uint32_t code = cCode[wv][lane];
+            const float4   N  = g.n4[s];
             const float    f2 = ppfc::angle_a2b(cU[wv][0][lane], cU[wv][1][lane], cU[wv][2][lane],
-                                                g.nx[s], g.ny[s], g.nz[s]);
+                                                N.x, N.y, N.z);
             const uint32_t key = ppfc::murmur_key4(
                 qunpack(code >> 20), ppfc::quant(f2, a.angleStep, false),
                 qunpack((code >> 10) & 1023u), qunpack(code & 1023u));
             if (table_find(a.slots, a.tmask, key, bStart, bLen, bucket)) {
-                alphaS = ppfc::alpha_of(rt, g.x[s], g.y[s], g.z[s]);
+                const float4 P = g.p4[s];
+                alphaS         = ppfc::alpha_of(rt, P.x, P.y, P.z);
                 hit    = ppfc::is_nan(alphaS) ? 0 : 1;
             }
         }
@@ -843,9 +845,10 @@ __global__ void __launch_bounds__(PAIR_BLOCK) pair_kernel(VoteArgs a) {
             s              = bSlot[wv][lane];
             const float dn = bDn[wv][lane];
             code           = bCode[wv][lane];
-            ux             = (g.x[s] - qx) / dn;
-            uy             = (g.y[s] - qy) / dn;
-            uz             = (g.z[s] - qz) / dn;
+            const float4 P = g.p4[s];
+            ux             = (P.x - qx) / dn;
+            uy             = (P.y - qy) / dn;
+            uz             = (P.z - qz) / dn;
             const float    f1 = ppfc::angle_a2b(ux, uy, uz, nx1, ny1, nz1);
             const uint32_t q1 = qpack(ppfc::quant(f1, a.angleStep, false));
             code |= q1 << 20;
@@ -894,10 +897,12 @@ __global__ void __launch_bounds__(PAIR_BLOCK) pair_kernel(VoteArgs a) {
         uint32_t code = 0;
         if (lane < cnt) {
             s              = pq[lane];
-            const float dx = g.x[s] - qx, dy = g.y[s] - qy, dz = g.z[s] - qz;
+            const float4 P  = g.p4[s];
+            const float  dx = P.x - qx, dy = P.y - qy, dz = P.z - qz;
             dn             = sqrtf(dx * dx + dy * dy + dz * dz);
             const uint32_t q4 = qpack(ppfc::quant(dn, a.distStep, false));
-            const float    f3 = ppfc::angle_a2b(nx1, ny1, nz1, g.nx[s], g.ny[s], g.nz[s]);
+            const float4   N  = g.n4[s];
+            const float    f3 = ppfc::angle_a2b(nx1, ny1, nz1, N.x, N.y, N.z);
             const uint32_t q3 = qpack(ppfc::quant(f3, a.angleStep, false));
             code              = (q3 << 10) | q4;
             pass = !filt || (filter_bit(f4, didx(q4)) && filter_bit(f34, aidx(q3) * fD + didx(q4)));
@@ -922,8 +927,9 @@ __global__ void __launch_bounds__(PAIR_BLOCK) pair_kernel(VoteArgs a) {
         int        pc = 0, s = 0;
         if (more && c < total) {
             s               = candidate_slot(t, c, nCells, sCellStart, sCellPre, sChunkCell);
-            const float px  = g.x[s], py = g.y[s], pz = g.z[s];
-            const int   pid = g.id[s];
+            const float4 P   = g.p4[s];  // {x, y, z, id bits}: one 16-byte load
+            const float  px  = P.x, py = P.y, pz = P.z;
+            const int    pid = __float_as_int(P.w);
             bool        in  = ppfc::dist2(qx, qy, qz, px, py, pz) < a.D2;
             inSphere += in ? 1 : 0;
             if (in && pid != kpId) {
@@ -1440,6 +1446,9 @@ int vote_keypoints(Workspace &ws, const DevModel &m, const VoteKeypoints &in, fl
     const int nk = in.nk;
     if (nk <= 0 || m.nSampled <= 0)
         return 0;
+    // the pair kernel reads the scene through the vote grids' point copies
+    if (in.g.n > 0 && (!in.g.p4 || !in.g.n4))
+        throw std::logic_error("vote grid built without its point copies");
     int tileRefs = std::min(m.nSampled, vote_max_tile_refs(m.angleNum, cfg.block));
     int nTiles   = std::max(div_up(m.nSampled, tileRefs), std::max(cfg.tiles, 1));
     tileRefs     = div_up(m.nSampled, nTiles);
