@@ -645,13 +645,14 @@ void match_impl(ppf_detector_s &D, MatchCtx &X, DevCloud &c, float rel, float kp
                 }
                 if (laneGrids) {
                     grid_build(X.ws.gSparse, PX, PY, PZ, NX, NY, NZ, samp, ns,
-                               sampleStep * kSparseCell, ln.stream, c.ext);
+                               sampleStep * kSparseCell, ln.stream, c.ext, true);
                     grid_build(X.ws.gFull, PX, PY, PZ, nullptr, nullptr, nullptr,
                                c.valid.get<int>(), c.nValid,
-                               std::max(scoringDist * 2.0f, sampleStep * 0.25f), ln.stream, c.ext);
+                               std::max(scoringDist * 2.0f, sampleStep * 0.25f), ln.stream, c.ext,
+                               true);
                     if (p.densePoseRefinement)
                         grid_build(X.ws.gDense, PX, PY, PZ, NX, NY, NZ, dense, nd,
-                                   reSampleStep * kDenseCell, ln.stream, c.ext);
+                                   reSampleStep * kDenseCell, ln.stream, c.ext, true);
                 }
                 PPF_HIP_CHECK(hipEventRecord(ln.ev[1], ln.stream));
                 PPF_HIP_CHECK(hipStreamSynchronize(ln.stream));
@@ -1070,12 +1071,13 @@ void match_batch_impl(ppf_detector_s &D, MatchCtx &X, const std::vector<DevCloud
             }
             if (laneGrids) {
                 grid_build(sw.gSparse, PX, PY, PZ, NX, NY, NZ, sw.samp.get<int>(), sw.ns,
-                           sampleStep * kSparseCell, ls, c.ext);
+                           sampleStep * kSparseCell, ls, c.ext, true);
                 grid_build(sw.gFull, PX, PY, PZ, nullptr, nullptr, nullptr, c.valid.get<int>(),
-                           c.nValid, std::max(scoringDist * 2.0f, sampleStep * 0.25f), ls, c.ext);
+                           c.nValid, std::max(scoringDist * 2.0f, sampleStep * 0.25f), ls, c.ext,
+                           true);
                 if (p.densePoseRefinement)
                     grid_build(sw.gDense, PX, PY, PZ, NX, NY, NZ, sw.dense.get<int>(), sw.nd,
-                               reSampleStep * kDenseCell, ls, c.ext);
+                               reSampleStep * kDenseCell, ls, c.ext, true);
             }
         },
         [&](int i, Workspace &ws, hipStream_t ls) {
@@ -1899,7 +1901,7 @@ PPF_API int ppf_scene_inliers(ppf_detector d, ppf_scene sc, const float *poses16
         const float Dm = box_diameter(d->hm.sampled.boxMin, d->hm.sampled.boxMax);
         grid_build(X.ws.gFull, c.px.get<float>(), c.py.get<float>(), c.pz.get<float>(), nullptr,
                    nullptr, nullptr, c.valid.get<int>(), c.nValid,
-                   std::max(dist * 2.0f, Dm * 1e-3f), s);
+                   std::max(dist * 2.0f, Dm * 1e-3f), s, nullptr, true);
         std::vector<float> P(12 * nPoses);
         for (size_t h = 0; h < nPoses; h++)
             pose16_to_12(poses16 + 16 * h, &P[12 * h]);
@@ -2508,9 +2510,10 @@ PPF_API int ppf_icp_target_create(const float *pts, const float *nrm, size_t n, 
             cell = (float)(2.0 * std::cbrt(vol / (double)n));
         }
         grid_build(t->grid, t->x.get<float>(), t->y.get<float>(), t->z.get<float>(),
-                   t->nx.get<float>(), t->ny.get<float>(), t->nz.get<float>(), nullptr, m, cell, s);
+                   t->nx.get<float>(), t->ny.get<float>(), t->nz.get<float>(), nullptr, m, cell, s,
+                   nullptr, true);
         grid_build(t->full, t->x.get<float>(), t->y.get<float>(), t->z.get<float>(), nullptr,
-                   nullptr, nullptr, nullptr, m, cell, s);
+                   nullptr, nullptr, nullptr, m, cell, s, nullptr, true);
         t->n = m;
         *out = t.release();
     });

@@ -82,8 +82,9 @@ struct GridView {
     // cells of a row are then neighbouring words instead of scattered table
     // slots.  nullptr: only the hash table.
     const int      *cstart;
-    // optional (nullptr) slot-ordered copies for the pair kernel, one 16-byte
-    // load each: {x, y, z, id bits} and {nx, ny, nz, 0}
+    // slot-ordered copies, one 16-byte load per point (grids built with
+    // them, else nullptr): {x, y, z, id bits} and, with normals,
+    // {nx, ny, nz, 0}
     const float4   *p4, *n4;
 };
 static_assert(sizeof(GridView) == 144, "GridView holds no padding bytes");
@@ -331,8 +332,9 @@ __device__ inline int grid_nn1(const GridView &g, float qx, float qy, float qz, 
     for (int r = first_ring(g, c0x, c0y, c0z); r <= maxR; r++) {
         visit_ring(g, c0x, c0y, c0z, r, qx, qy, qz, fminf(bestD, maxD2), [&](int s, int c) {
             for (int t = s; t < s + c; t++) {
-                float dd = ppfc::dist2(qx, qy, qz, g.x[t], g.y[t], g.z[t]);
-                int   ii = g.id[t];
+                const float4 P  = g.p4[t];
+                const float  dd = ppfc::dist2(qx, qy, qz, P.x, P.y, P.z);
+                const int    ii = __float_as_int(P.w);
                 if (dd > maxD2)
                     continue;
                 if (dd < bestD || (dd == bestD && ii < bestId)) {
@@ -413,8 +415,9 @@ __device__ int grid_nn1_group(const GridView &g, float qx, float qy, float qz, f
             if (!cell_lookup(g, cx, cy, cz, s0, c))
                 continue;
             for (int t = s0; t < s0 + c; t++) {
-                float dd = ppfc::dist2(qx, qy, qz, g.x[t], g.y[t], g.z[t]);
-                int   ii = g.id[t];
+                const float4 P  = g.p4[t];
+                const float  dd = ppfc::dist2(qx, qy, qz, P.x, P.y, P.z);
+                const int    ii = __float_as_int(P.w);
                 if (dd > maxD2)
                     continue;
                 if (dd < bestD || (dd == bestD && ii < bestId)) {
@@ -459,9 +462,11 @@ __device__ inline bool grid_any_within(const GridView &g, float qx, float qy, fl
                 int s, c;
                 if (coarse_empty(g, cx, cy, cz) || !cell_lookup(g, cx, cy, cz, s, c))
                     continue;
-                for (int t = s; t < s + c; t++)
-                    if (ppfc::dist2(qx, qy, qz, g.x[t], g.y[t], g.z[t]) < r2)
+                for (int t = s; t < s + c; t++) {
+                    const float4 P = g.p4[t];
+                    if (ppfc::dist2(qx, qy, qz, P.x, P.y, P.z) < r2)
                         return true;
+                }
             }
     return false;
 }
@@ -480,8 +485,10 @@ struct Grid {
 // requested cell edge; it grows if the grid would exceed 1023 cells per
 // axis.  withNormals gathers NX/NY/NZ too.  Synchronises `s` once (bbox).
 // hostBox6 (min xyz, max xyz), when given, is a box known to contain the
-// points; it saves the device bounding box and its read-back.  aos (with
-// normals) also writes the view's p4 / n4 copies.
+// points; it saves the device bounding box and its read-back.  aos also
+// writes the view's p4 (and, with normals, n4) copies, which the pair kernel
+// and the refiner's searches (grid_nn1, grid_nn1_group, grid_any_within)
+// read.
 void grid_build(Grid &g, const float *X, const float *Y, const float *Z, const float *NX,
                 const float *NY, const float *NZ, const int *subset, int m, float cellHint,
                 hipStream_t s, const float *hostBox6 = nullptr, bool aos = false);

@@ -189,7 +189,7 @@ __global__ void grid_clear_kernel(uint32_t *a, size_t na, uint32_t *b, size_t nb
     }
 }
 
-// slot-ordered {x, y, z, id bits} and {nx, ny, nz, 0}
+// slot-ordered {x, y, z, id bits} and, with normals, {nx, ny, nz, 0}
 __global__ void gather_aos_kernel(const int *idx, int m, const float *X, const float *Y,
                                   const float *Z, const float *NX, const float *NY,
                                   const float *NZ, float4 *p4, float4 *n4) {
@@ -198,7 +198,8 @@ __global__ void gather_aos_kernel(const int *idx, int m, const float *X, const f
         return;
     int i = idx[s];
     p4[s] = make_float4(X[i], Y[i], Z[i], __int_as_float(i));
-    n4[s] = make_float4(NX[i], NY[i], NZ[i], 0.0f);
+    if (n4)
+        n4[s] = make_float4(NX[i], NY[i], NZ[i], 0.0f);
 }
 
 void grid_build(Grid &g, const float *X, const float *Y, const float *Z, const float *NX,
@@ -310,13 +311,12 @@ void grid_build(Grid &g, const float *X, const float *Y, const float *Z, const f
         v.nx = nx;
         v.ny = ny;
         v.nz = nz;
-        if (aos) {
-            float4 *p4 = g.p4.as<float4>(m), *n4 = g.n4.as<float4>(m);
-            gather_aos_kernel<<<div_up(m, 256), 256, 0, s>>>(vals2, m, X, Y, Z, NX, NY, NZ, p4,
-                                                             n4);
-            v.p4 = p4;
-            v.n4 = n4;
-        }
+    }
+    if (aos && m > 0) {
+        float4 *p4 = g.p4.as<float4>(m), *n4 = NX ? g.n4.as<float4>(m) : nullptr;
+        gather_aos_kernel<<<div_up(m, 256), 256, 0, s>>>(vals2, m, X, Y, Z, NX, NY, NZ, p4, n4);
+        v.p4 = p4;
+        v.n4 = n4;
     }
     PPF_HIP_CHECK(hipGetLastError());
 }

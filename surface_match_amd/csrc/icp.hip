@@ -1063,6 +1063,12 @@ static int norm_block(int b) {
     return (b == 64 || b == 128 || b == 512 || b == 1024) ? b : 256;
 }
 
+// the refiner's searches read the grids' {x, y, z, id} copies
+static void require_copies(const GridView &g) {
+    if (g.n > 0 && !g.p4)
+        throw std::logic_error("refinement grid built without its point copies");
+}
+
 void icp_batch(Workspace &ws, int nSrc, const float *sx, const float *sy, const float *sz,
                const float *snx, const float *sny, const float *snz, const GridView &dst,
                const GridView &full, const float *init12, int H, const IcpCriteria &c,
@@ -1070,6 +1076,8 @@ void icp_batch(Workspace &ws, int nSrc, const float *sx, const float *sy, const 
                const GridView *dstG, const GridView *fullG, const int *hypScene) {
     if (H <= 0)
         return;
+    require_copies(dst);
+    require_copies(full);
     const int block = norm_block(cfg.block);
     IcpArgs a = make_args(ws, nSrc, sx, sy, sz, snx, sny, snz, dst, full, H, c, scoringDist,
                           cfg.mode);
@@ -1146,6 +1154,8 @@ void icp_plain(Workspace &ws, int nSrc, const float *sx, const float *sy, const 
                float scoringDist, int mode, IcpResultDev *out, hipStream_t s) {
     if (H <= 0)
         return;
+    require_copies(dst);
+    require_copies(full);
     IcpArgs a = make_args(ws, nSrc, sx, sy, sz, snx, sny, snz, dst, full, 1, c, scoringDist, mode);
     int nb = div_up(std::max(nSrc, 1), PLAIN_BLOCK);
     float *init = ws.icpPose.as<float>((size_t)H * 12);
@@ -1223,6 +1233,7 @@ void icp_inliers(Workspace &ws, const GridView &full, int nSrc, const float *sx,
         return;
     if (H > 65535)
         throw std::runtime_error("at most 65535 poses per scoring call");
+    require_copies(full);
     float *P   = ws.icpPose.as<float>((size_t)H * 12);
     int   *cnt = ws.icpInl.as<int>(H);
     PPF_HIP_CHECK(hipMemcpyAsync(P, poses12Host, sizeof(float) * 12 * H, hipMemcpyHostToDevice, s));
@@ -1345,6 +1356,7 @@ void icp_adjoint(Workspace &ws, int nSrc, const float *sx, const float *sy, cons
                  float rejectDist, int center, double *outHost, int *corrHost, hipStream_t s) {
     if (H <= 0)
         return;
+    require_copies(dst);
     IcpCriteria c{1, rejectDist, 0.0f, 0.0f, 0.0f};
     AdjArgs     a;
     a.b = make_args(ws, nSrc, sx, sy, sz, sx, sy, sz, dst, dst, H, c, 0.0f, 1);
