@@ -417,6 +417,8 @@ __device__ __forceinline__ uint32_t acc_get(const uint32_t *acc, int c) {
     return (acc[c >> 1] >> ((c & 1) << 4)) & 0xFFFFu;
 }
 
+constexpr int PEAK_BINS = 30;  // featAngleResolution's default (type.h:71)
+
 template <int BLOCK, bool PACKED = false>
 __device__ void peaks_top3(const VoteArgs &a, const uint32_t *acc, int nrTile, int r0,
                            unsigned long long *sTop, unsigned long long *candOut) {
@@ -424,7 +426,41 @@ __device__ void peaks_top3(const VoteArgs &a, const uint32_t *acc, int nrTile, i
     const float thre = a.voteThreshold / 2.0f;
     const int   A    = a.angleNum, maxI = A - 1;
     Top3        top{0ull, 0ull, 0ull};
-    for (int r = tid; r < nrTile; r += BLOCK) {
+    if (PACKED && A == PEAK_BINS) {
+        // the default 30 bins, packed two to a word: a reference's bins are
+        // the 15 words from r * 15 on, read once, unrolled
+        for (int r = tid; r < nrTile; r += BLOCK) {
+            const uint32_t *w   = acc + r * (PEAK_BINS / 2);
+            uint32_t        wd[PEAK_BINS / 2];
+#pragma unroll
+            for (int q = 0; q < PEAK_BINS / 2; q++)
+                wd[q] = w[q];
+            int sum = 0, best = -1, j = 0;
+#pragma unroll
+            for (int q = 0; q < PEAK_BINS / 2; q++) {
+                const int lo = (int)(wd[q] & 0xFFFFu), hi = (int)(wd[q] >> 16);
+                sum += lo + hi;
+                if (lo > best) {
+                    best = lo;
+                    j    = 2 * q;
+                }
+                if (hi > best) {
+                    best = hi;
+                    j    = 2 * q + 1;
+                }
+            }
+            if ((float)sum < thre || (float)best < thre)
+                continue;
+            const int jl = j == 0 ? maxI : j - 1, jr = j == maxI ? 0 : j + 1;
+            const int v  = best + (int)acc_get<true>(acc, r * A + jl) +
+                          (int)acc_get<true>(acc, r * A + jr);
+            unsigned long long code = ((unsigned long long)(uint32_t)v << 32) |
+                                      ((unsigned long long)(0xFFFFFu - (uint32_t)(r0 + r)) << 11) |
+                                      (uint32_t)j;
+            top = top3_add(top, code);
+        }
+    }
+    for (int r = (PACKED && A == PEAK_BINS) ? nrTile : tid; r < nrTile; r += BLOCK) {
         auto bins = [&](int x) { return acc_get<PACKED>(acc, r * A + x); };
         int sum = 0, j = 0, best = -1;
         for (int x = 0; x < A; x++) {
@@ -1001,7 +1037,7 @@ __global__ void __launch_bounds__(PAIR_BLOCK) pair_kernel(VoteArgs a) {
 __host__ __device__ inline int hits_acc_words(int tileRefs, int angleNum, bool packed) {
     const int cells = (tileRefs + 1) * angleNum;  // + the spare row
     const int w     = packed ? (cells + 1) / 2 : cells;
-    return (w + 1) & ~1;  // 8-byte alignment for what follows
+    return (w + 3) & ~3;  // 16-byte alignment: cleared as uint4, 8-byte words follow
 }
 template <int BLOCK, bool PACKED> struct HitsLds {
     static constexpr int HB = PACKED ? BLOCK / 2 : BLOCK;
@@ -1043,9 +1079,16 @@ __global__ void __launch_bounds__(BLOCK) vote_hits_kernel(VoteArgs a) {
     }
     const int realCells = nrTile * a.angleNum;
     // packed: the spare row too, whose returned words take part in the
-    // overflow test
-    for (int i = tid; i < (PACKED ? (realCells + a.angleNum + 1) >> 1 : realCells); i += BLOCK)
-        acc[i] = 0u;
+    // overflow test; the whole accumulator block, 16 bytes a store
+    if (PACKED) {
+        uint4    *a4 = reinterpret_cast<uint4 *>(acc);
+        const int n4 = hits_acc_words(a.tileRefs, a.angleNum, true) >> 2;
+        for (int i = tid; i < n4; i += BLOCK)
+            a4[i] = make_uint4(0u, 0u, 0u, 0u);
+    } else {
+        for (int i = tid; i < realCells; i += BLOCK)
+            acc[i] = 0u;
+    }
     if (tid == 0)
         sOvf = 0u;
     uint32_t           ovf    = 0u;
