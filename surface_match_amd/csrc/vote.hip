@@ -480,14 +480,29 @@ __device__ void peaks_top3(const VoteArgs &a, const uint32_t *acc, int nrTile, i
                                   (uint32_t)j;
         top = top3_add(top, code);
     }
-    for (int o = 32; o > 0; o >>= 1) {
-        const unsigned long long u0 = __shfl_xor(top.a, o, 64), u1 = __shfl_xor(top.b, o, 64),
-                                 u2 = __shfl_xor(top.c, o, 64);
-        top = top3_add(top3_add(top3_add(top, u0), u1), u2);
+    // the wave's top-3: usually few lanes found a peak, so their lists are
+    // gathered lane by lane (ascending lane, as readlane broadcasts); many
+    // lanes merge by an xor butterfly.  Codes are distinct (disjoint refs),
+    // so both give the same three codes.
+    const unsigned long long withPeak = __ballot(top.a != 0ull);
+    if (__popcll(withPeak) <= 8) {
+        Top3               wt{0ull, 0ull, 0ull};
+        unsigned long long m = withPeak;
+        while (m) {
+            const int l = __ffsll((long long)m) - 1;
+            m &= m - 1ull;
+            wt = top3_add(top3_add(top3_add(wt, __shfl(top.a, l, 64)), __shfl(top.b, l, 64)),
+                          __shfl(top.c, l, 64));
+        }
+        top = wt;
+    } else {
+        for (int o = 32; o > 0; o >>= 1) {
+            const unsigned long long u0 = __shfl_xor(top.a, o, 64), u1 = __shfl_xor(top.b, o, 64),
+                                     u2 = __shfl_xor(top.c, o, 64);
+            top = top3_add(top3_add(top3_add(top, u0), u1), u2);
+        }
     }
-    // after the xor-butterfly every lane holds the wave's top-3 (duplicates
-    // cannot arise: each lane inserts distinct codes from disjoint refs, and
-    // the butterfly merges disjoint halves at each level)
+    // every lane now holds the wave's top-3
     if ((tid & 63) == 0) {
         sTop[(tid >> 6) * 3 + 0] = top.a;
         sTop[(tid >> 6) * 3 + 1] = top.b;
