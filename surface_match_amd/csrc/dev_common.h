@@ -202,13 +202,58 @@ struct KnnList {
     }
 };
 
-// Visit the cells of Chebyshev ring r around (c0x,c0y,c0z), calling f(start,count)
-// for occupied cells whose box is within bound2 of q.  Row by row: a row on
-// the shell's z/y faces is visited over its whole x range, a row off them at
-// its two ends.  With a per-cell occupancy bitmap (occShift 0) a row's cells
-// are read 32 at a time from the bitmap words and only the occupied ones
-// are visited, so empty space costs one load per word rather than work per
-// cell.  A single call site of f keeps the register use down.
+// Visit the cells of row (cy, cz) of Chebyshev ring r around (c0x,c0y,c0z)
+// that lie in [xlo, xhi], calling f(start,count) for occupied cells whose box
+// is within bound2 of q: a row on the shell's z/y faces over its whole x
+// range, a row off them at its two ends.  With a per-cell occupancy bitmap
+// (occShift 0) a row's cells are read 32 at a time from the bitmap words and
+// only the occupied ones are visited, so empty space costs one load per word
+// rather than work per cell.
+template <class F>
+__device__ __forceinline__ void visit_row(const GridView &g, int c0x, int c0y, int c0z, int r,
+                                          int cy, int cz, int xlo, int xhi, float qx, float qy,
+                                          float qz, float bound2, F &f) {
+    const bool face = (cz == c0z - r) || (cz == c0z + r) || (cy == c0y - r) || (cy == c0y + r);
+    const uint32_t rowBase = ((uint32_t)cz * (uint32_t)g.ody + (uint32_t)cy) * (uint32_t)g.odx;
+    for (int seg = 0; seg < (face ? 1 : 2); seg++) {
+        // the segment [a0, a1] of this row: the whole row on a face, else
+        // one end of it
+        int a0 = xlo, a1 = xhi;
+        if (!face) {
+            a0 = a1 = seg == 0 ? c0x - r : c0x + r;
+            if (a0 < xlo || a0 > xhi)
+                continue;
+        }
+        for (int x = a0; x <= a1;) {
+            uint32_t  bits;
+            const int x0 = x;
+            if (g.occShift == 0) {
+                const uint32_t b   = rowBase + (uint32_t)x;
+                const int      off = (int)(b & 31u);
+                const int      n   = min(32 - off, a1 - x + 1);
+                bits               = g.occ[b >> 5] >> off;
+                bits &= n == 32 ? ~0u : ((1u << n) - 1u);
+                x += n;
+            } else {
+                bits = coarse_empty(g, x, cy, cz) ? 0u : 1u;
+                x += 1;
+            }
+            while (bits) {
+                const int cx = x0 + __ffs(bits) - 1;
+                bits &= bits - 1u;
+                if (cell_min_d2(g, cx, cy, cz, qx, qy, qz) > bound2)
+                    continue;
+                int s0, c0;
+                if (cell_lookup(g, cx, cy, cz, s0, c0))
+                    f(s0, c0);
+            }
+        }
+    }
+}
+
+// Visit the cells of Chebyshev ring r around (c0x,c0y,c0z), calling
+// f(start,count) for occupied cells whose box is within bound2 of q, row by
+// row (visit_row).  A single call site of f keeps the register use down.
 template <class F>
 __device__ __forceinline__ void visit_ring(const GridView &g, int c0x, int c0y, int c0z, int r,
                                            float qx, float qy, float qz, float bound2, F &&f) {
@@ -218,46 +263,32 @@ __device__ __forceinline__ void visit_ring(const GridView &g, int c0x, int c0y, 
     if (zlo > zhi || ylo > yhi || xlo > xhi)
         return;
     for (int cz = zlo; cz <= zhi; cz++)
-        for (int cy = ylo; cy <= yhi; cy++) {
-            const bool face = (cz == c0z - r) || (cz == c0z + r) || (cy == c0y - r) ||
-                              (cy == c0y + r);
-            const uint32_t rowBase = ((uint32_t)cz * (uint32_t)g.ody + (uint32_t)cy) *
-                                     (uint32_t)g.odx;
-            for (int seg = 0; seg < (face ? 1 : 2); seg++) {
-                // the segment [a0, a1] of this row: the whole row on a face,
-                // else one end of it
-                int a0 = xlo, a1 = xhi;
-                if (!face) {
-                    a0 = a1 = seg == 0 ? c0x - r : c0x + r;
-                    if (a0 < xlo || a0 > xhi)
-                        continue;
-                }
-                for (int x = a0; x <= a1;) {
-                    uint32_t bits;
-                    const int x0 = x;
-                    if (g.occShift == 0) {
-                        const uint32_t b   = rowBase + (uint32_t)x;
-                        const int      off = (int)(b & 31u);
-                        const int      n   = min(32 - off, a1 - x + 1);
-                        bits               = g.occ[b >> 5] >> off;
-                        bits &= n == 32 ? ~0u : ((1u << n) - 1u);
-                        x += n;
-                    } else {
-                        bits = coarse_empty(g, x, cy, cz) ? 0u : 1u;
-                        x += 1;
-                    }
-                    while (bits) {
-                        const int cx = x0 + __ffs(bits) - 1;
-                        bits &= bits - 1u;
-                        if (cell_min_d2(g, cx, cy, cz, qx, qy, qz) > bound2)
-                            continue;
-                        int s0, c0;
-                        if (cell_lookup(g, cx, cy, cz, s0, c0))
-                            f(s0, c0);
-                    }
-                }
-            }
+        for (int cy = ylo; cy <= yhi; cy++)
+            visit_row(g, c0x, c0y, c0z, r, cy, cz, xlo, xhi, qx, qy, qz, bound2, f);
+}
+
+// The rows row0, row0 + step, row0 + 2 step, ... of ring r (rows in z-major
+// order of the ring's in-grid (cy, cz) pairs): lanes of a group that each
+// take every step-th row visit the ring's cells between them exactly once.
+template <class F>
+__device__ __forceinline__ void visit_ring_rows(const GridView &g, int c0x, int c0y, int c0z,
+                                                int r, float qx, float qy, float qz, float bound2,
+                                                int row0, int step, F &&f) {
+    const int zlo = max(c0z - r, 0), zhi = min(c0z + r, g.dz - 1);
+    const int ylo = max(c0y - r, 0), yhi = min(c0y + r, g.dy - 1);
+    const int xlo = max(c0x - r, 0), xhi = min(c0x + r, g.dx - 1);
+    if (zlo > zhi || ylo > yhi || xlo > xhi)
+        return;
+    const int ny = yhi - ylo + 1;
+    int       jz = row0 / ny, jy = row0 - jz * ny;
+    for (int cz = zlo + jz; cz <= zhi;) {
+        visit_row(g, c0x, c0y, c0z, r, ylo + jy, cz, xlo, xhi, qx, qy, qz, bound2, f);
+        jy += step;
+        while (jy >= ny) {
+            jy -= ny;
+            cz++;
         }
+    }
 }
 
 // distance from q to the boundary of the (2r+1)^3 block of cells around c0;
@@ -355,33 +386,8 @@ __device__ inline int grid_nn1(const GridView &g, float qx, float qy, float qz, 
     return bestS;
 }
 
-// cell (dx, dy, dz) number j of the Chebyshev shell of radius r >= 1, in
-// the order: the z = -r, +r faces, the y = -r, +r faces (|dz| < r), the
-// x = -r, +r faces (|dy|, |dz| < r); (2r+1)^3 - (2r-1)^3 cells in all
-__device__ __forceinline__ void shell_cell(int r, int j, int &dx, int &dy, int &dz) {
-    const int side = 2 * r + 1, inner = 2 * r - 1;
-    const int n1 = side * side, n2 = side * inner, n3 = inner * inner;
-    if (j < 2 * n1) {
-        dz = j < n1 ? -r : r;
-        j  = j < n1 ? j : j - n1;
-        dx = j % side - r;
-        dy = j / side - r;
-    } else if ((j -= 2 * n1) < 2 * n2) {
-        dy = j < n2 ? -r : r;
-        j  = j < n2 ? j : j - n2;
-        dx = j % side - r;
-        dz = j / side - (r - 1);
-    } else {
-        j -= 2 * n2;
-        dx = j < n3 ? -r : r;
-        j  = j < n3 ? j : j - n3;
-        dy = j % inner - (r - 1);
-        dz = j / inner - (r - 1);
-    }
-}
-
 // grid_nn1 by a group of G consecutive lanes (G a power of two <= 64) that
-// all pass the same query: each lane visits every G-th cell of a ring, the
+// all pass the same query: each lane visits every G-th row of a ring, the
 // group merges its candidates by (dist2, id) after every ring, and every
 // lane returns the same result as grid_nn1 would.  Control flow is uniform
 // within a group.
@@ -399,34 +405,23 @@ __device__ int grid_nn1_group(const GridView &g, float qx, float qy, float qz, f
     const int c0z  = cell_coord_unclamped(qz, g.oz, g.cell);
     const int maxR = max(max(g.dx, g.dy), g.dz) + max(max(abs(c0x), abs(c0y)), abs(c0z)) + 1;
     for (int r = first_ring(g, c0x, c0y, c0z); r <= maxR; r++) {
-        const int   nShell = r == 0 ? 1 : (2 * r + 1) * (2 * r + 1) * (2 * r + 1) -
-                                             (2 * r - 1) * (2 * r - 1) * (2 * r - 1);
-        const float bound2 = fminf(bestD, maxD2);
-        for (int j = gl; j < nShell; j += G) {
-            int dx = 0, dy = 0, dz = 0;
-            if (r > 0)
-                shell_cell(r, j, dx, dy, dz);
-            const int cx = c0x + dx, cy = c0y + dy, cz = c0z + dz;
-            if (cx < 0 || cx >= g.dx || cy < 0 || cy >= g.dy || cz < 0 || cz >= g.dz)
-                continue;
-            if (coarse_empty(g, cx, cy, cz) || cell_min_d2(g, cx, cy, cz, qx, qy, qz) > bound2)
-                continue;
-            int s0, c;
-            if (!cell_lookup(g, cx, cy, cz, s0, c))
-                continue;
-            for (int t = s0; t < s0 + c; t++) {
-                const float4 P  = g.p4[t];
-                const float  dd = ppfc::dist2(qx, qy, qz, P.x, P.y, P.z);
-                const int    ii = __float_as_int(P.w);
-                if (dd > maxD2)
-                    continue;
-                if (dd < bestD || (dd == bestD && ii < bestId)) {
-                    bestD  = dd;
-                    bestId = ii;
-                    bestS  = t;
-                }
-            }
-        }
+        // each lane walks every G-th row of the ring (the ring's cells
+        // between them once), so empty space costs bitmap words, not cells
+        visit_ring_rows(g, c0x, c0y, c0z, r, qx, qy, qz, fminf(bestD, maxD2), gl, G,
+                        [&](int s0, int c) {
+                            for (int t = s0; t < s0 + c; t++) {
+                                const float4 P  = g.p4[t];
+                                const float  dd = ppfc::dist2(qx, qy, qz, P.x, P.y, P.z);
+                                const int    ii = __float_as_int(P.w);
+                                if (dd > maxD2)
+                                    continue;
+                                if (dd < bestD || (dd == bestD && ii < bestId)) {
+                                    bestD  = dd;
+                                    bestId = ii;
+                                    bestS  = t;
+                                }
+                            }
+                        });
 #pragma unroll
         for (int o = G / 2; o > 0; o >>= 1) {
             const float od = __shfl_xor(bestD, o, 64);

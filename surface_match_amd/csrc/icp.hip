@@ -111,9 +111,74 @@ __device__ __forceinline__ IcpArgs args_for(const IcpArgs &a, int h) {
     return b;
 }
 
+// u = A^-1 rhs by an FP64 Cholesky factorisation A = L L^T, taken only when
+// no eigenvalue of A can fall below the pseudo-inverse threshold of
+// sym_solve6 (then pinv(A) = A^-1 and both give the same u up to FP64
+// rounding): lambda_min >= 1 / ||A^-1||_F >= 1 / ||L^-1||_F^2 and
+// lambda_max <= ||A||_inf.  Returns false (u untouched) otherwise.  A
+// few hundred dependent flops instead of the Jacobi sweeps' ~10^4.
+__device__ bool chol_solve6(const double A[36], const double rhs[6], double u[6]) {
+    double L[36], M[36];
+    double amax = 0.0;
+    for (int i = 0; i < 6; i++) {
+        double rs = 0.0;
+        for (int j = 0; j < 6; j++)
+            rs += fabs(A[i * 6 + j]);
+        amax = fmax(amax, rs);
+    }
+    for (int j = 0; j < 6; j++) {
+        double d = A[j * 6 + j];
+        for (int k = 0; k < j; k++)
+            d -= L[j * 6 + k] * L[j * 6 + k];
+        if (!(d > 0.0))
+            return false;
+        const double ljj = sqrt(d);
+        L[j * 6 + j]     = ljj;
+        for (int i = j + 1; i < 6; i++) {
+            double s = A[i * 6 + j];
+            for (int k = 0; k < j; k++)
+                s -= L[i * 6 + k] * L[j * 6 + k];
+            L[i * 6 + j] = s / ljj;
+        }
+    }
+    // M = L^-1 (lower triangular) and its squared Frobenius norm
+    double fro = 0.0;
+    for (int j = 0; j < 6; j++) {
+        M[j * 6 + j] = 1.0 / L[j * 6 + j];
+        fro += M[j * 6 + j] * M[j * 6 + j];
+        for (int i = j + 1; i < 6; i++) {
+            double s = 0.0;
+            for (int k = j; k < i; k++)
+                s -= L[i * 6 + k] * M[k * 6 + j];
+            M[i * 6 + j] = s / L[i * 6 + i];
+            fro += M[i * 6 + j] * M[i * 6 + j];
+        }
+    }
+    // a margin of 1e-3 over the threshold absorbs the rounding of the bounds
+    if (!(1.0 / fro > 6.0 * 1.1920928955078125e-07 * amax * 1.001))
+        return false;
+    double y[6];
+    for (int i = 0; i < 6; i++) {
+        double s = rhs[i];
+        for (int k = 0; k < i; k++)
+            s -= L[i * 6 + k] * y[k];
+        y[i] = s / L[i * 6 + i];
+    }
+    for (int i = 5; i >= 0; i--) {
+        double s = y[i];
+        for (int k = i + 1; k < 6; k++)
+            s -= L[k * 6 + i] * u[k];
+        u[i] = s / L[i * 6 + i];
+    }
+    return true;
+}
+
 // u = pinv(A) rhs for symmetric PSD A (FP64 Jacobi), dropping eigenvalues
-// below 6 * eps_f32 * max |eigenvalue| as JacobiSVD::solve does.
+// below 6 * eps_f32 * max |eigenvalue| as JacobiSVD::solve does; the
+// Cholesky path above when the threshold provably drops none of them.
 __device__ void sym_solve6(const double A[36], const double rhs[6], double u[6]) {
+    if (chol_solve6(A, rhs, u))
+        return;
     double a[36], V[36], W[6];
     for (int i = 0; i < 36; i++)
         a[i] = A[i];
