@@ -404,7 +404,50 @@ __device__ int grid_nn1_group(const GridView &g, float qx, float qy, float qz, f
     const int c0y  = cell_coord_unclamped(qy, g.oy, g.cell);
     const int c0z  = cell_coord_unclamped(qz, g.oz, g.cell);
     const int maxR = max(max(g.dx, g.dy), g.dz) + max(max(abs(c0x), abs(c0y)), abs(c0z)) + 1;
-    for (int r = first_ring(g, c0x, c0y, c0z); r <= maxR; r++) {
+    int       r0   = first_ring(g, c0x, c0y, c0z);
+    if (r0 == 0 && g.cstart) {
+        // rings 0 and 1 at once: the 3x3x3 block around the query cell as
+        // nine rows of up to three cells, each row one contiguous run of
+        // sorted slots (a row's cells are consecutive keys); every lane scans
+        // whole rows, so the block costs one start/end load per row and the
+        // scans.  Cells the ring walk would prune only hold points farther
+        // than the running best, which cannot win or tie.
+        const int xlo = max(c0x - 1, 0), xhi = min(c0x + 1, g.dx - 1);
+        for (int j = gl; j < 9; j += G) {
+            const int cz = c0z - 1 + j / 3, cy = c0y - 1 + j % 3;
+            if (cz < 0 || cz >= g.dz || cy < 0 || cy >= g.dy)
+                continue;
+            const int *cs = g.cstart + (cz * g.dy + cy) * g.dx;
+            const int  t1 = cs[xhi + 1];
+            for (int t = cs[xlo]; t < t1; t++) {
+                const float4 P  = g.p4[t];
+                const float  dd = ppfc::dist2(qx, qy, qz, P.x, P.y, P.z);
+                const int    ii = __float_as_int(P.w);
+                if (dd > maxD2)
+                    continue;
+                if (dd < bestD || (dd == bestD && ii < bestId)) {
+                    bestD  = dd;
+                    bestId = ii;
+                    bestS  = t;
+                }
+            }
+        }
+#pragma unroll
+        for (int o = G / 2; o > 0; o >>= 1) {
+            const float od = __shfl_xor(bestD, o, 64);
+            const int   oi = __shfl_xor(bestId, o, 64), os = __shfl_xor(bestS, o, 64);
+            if (od < bestD || (od == bestD && oi < bestId)) {
+                bestD  = od;
+                bestId = oi;
+                bestS  = os;
+            }
+        }
+        const float safe = ring_safe(g, c0x, c0y, c0z, 1, qx, qy, qz);
+        if (safe == FLT_MAX || (bestS >= 0 && bestD < safe * safe) || maxD2 < safe * safe)
+            return bestS;
+        r0 = 2;
+    }
+    for (int r = r0; r <= maxR; r++) {
         // each lane walks every G-th row of the ring (the ring's cells
         // between them once), so empty space costs bitmap words, not cells
         visit_ring_rows(g, c0x, c0y, c0z, r, qx, qy, qz, fminf(bestD, maxD2), gl, G,
@@ -441,6 +484,53 @@ __device__ int grid_nn1_group(const GridView &g, float qx, float qy, float qz, f
             break;
     }
     return bestS;
+}
+
+// grid_any_within by a group of G consecutive lanes passing the same query:
+// the rows of the query's box of cells are spread over the lanes (a row's
+// cells one contiguous run of sorted slots when the grid has cstart); every
+// lane returns the group's answer.
+template <int G>
+__device__ bool grid_any_within_group(const GridView &g, float qx, float qy, float qz, float r2) {
+    static_assert(G >= 1 && G < 64 && (G & (G - 1)) == 0, "group of 1..32 lanes");
+    if (g.n == 0 || !(isfinite(qx) && isfinite(qy) && isfinite(qz)))
+        return false;
+    const int gl  = (int)(threadIdx.x & (G - 1));
+    float     r   = sqrtf(r2) * 1.0001f + 1e-3f * g.cell;
+    int       xlo = cell_coord(qx - r, g.ox, g.cell, g.dx), xhi = cell_coord(qx + r, g.ox, g.cell, g.dx);
+    int       ylo = cell_coord(qy - r, g.oy, g.cell, g.dy), yhi = cell_coord(qy + r, g.oy, g.cell, g.dy);
+    int       zlo = cell_coord(qz - r, g.oz, g.cell, g.dz), zhi = cell_coord(qz + r, g.oz, g.cell, g.dz);
+    const int ny = yhi - ylo + 1, nrows = ny * (zhi - zlo + 1);
+    bool      found = false;
+    for (int j = gl; j < nrows && !found; j += G) {
+        const int cz = zlo + j / ny, cy = ylo + j % ny;
+        if (g.cstart) {
+            const int *cs = g.cstart + (cz * g.dy + cy) * g.dx;
+            const int  t1 = cs[xhi + 1];
+            for (int t = cs[xlo]; t < t1; t++) {
+                const float4 P = g.p4[t];
+                if (ppfc::dist2(qx, qy, qz, P.x, P.y, P.z) < r2) {
+                    found = true;
+                    break;
+                }
+            }
+        } else {
+            for (int cx = xlo; cx <= xhi && !found; cx++) {
+                int s, c;
+                if (coarse_empty(g, cx, cy, cz) || !cell_lookup(g, cx, cy, cz, s, c))
+                    continue;
+                for (int t = s; t < s + c; t++) {
+                    const float4 P = g.p4[t];
+                    if (ppfc::dist2(qx, qy, qz, P.x, P.y, P.z) < r2) {
+                        found = true;
+                        break;
+                    }
+                }
+            }
+        }
+    }
+    const unsigned long long b = __ballot(found);
+    return ((b >> (threadIdx.x & 63u & ~(unsigned)(G - 1))) & ((1ull << G) - 1ull)) != 0ull;
 }
 
 // true if any indexed point has dist2 < r2

@@ -896,9 +896,11 @@ __global__ void __launch_bounds__(BLOCK) spread_mse_kernel(SpreadArgs w) {
         converge(st, (float)(sRed[0] / (double)st.valid), b.c);
 }
 
+// inlier count of the final poses: ICP_G lanes per source point share the
+// search (lane 0 of each group counts)
 template <int BLOCK>
 __global__ void __launch_bounds__(BLOCK) spread_inlier_kernel(SpreadArgs w) {
-    const int h = blockIdx.x / w.nChunk, chunk = blockIdx.x - h * w.nChunk;
+    const int h = blockIdx.x / w.nChunkG, chunk = blockIdx.x - h * w.nChunkG;
     const IcpArgs b = args_for(w.b, h);
     __shared__ float sPF[12];
     __shared__ int   sCnt[BLOCK / 64];
@@ -906,9 +908,15 @@ __global__ void __launch_bounds__(BLOCK) spread_inlier_kernel(SpreadArgs w) {
     if (threadIdx.x < 12)
         sPF[threadIdx.x] = (float)st.pose[threadIdx.x];
     __syncthreads();
-    const int i = chunk * BLOCK + threadIdx.x;
-    int       c = (i < b.nSrc &&
-             inlier_point(b, i, sPF, b.scoringDist * b.scoringDist)) ? 1 : 0;
+    const int i = chunk * (BLOCK / ICP_G) + (int)threadIdx.x / ICP_G;
+    int       c = 0;
+    if (i < b.nSrc) {
+        float px, py, pz;
+        xform(sPF, b.sx[i], b.sy[i], b.sz[i], px, py, pz);
+        const bool in = grid_any_within_group<ICP_G>(b.full, px, py, pz,
+                                                     b.scoringDist * b.scoringDist);
+        c = (in && (threadIdx.x & (ICP_G - 1)) == 0) ? 1 : 0;
+    }
     for (int o = 32; o > 0; o >>= 1)
         c += __shfl_xor(c, o, 64);
     if ((threadIdx.x & 63) == 0)
@@ -921,7 +929,7 @@ __global__ void __launch_bounds__(BLOCK) spread_inlier_kernel(SpreadArgs w) {
             tot += sCnt[q];
         atomicAdd(&w.inl[h], tot);
         __threadfence();
-        sLast = (atomicAdd(&w.arrive[h], 1u) == (uint32_t)(w.nChunk - 1)) ? 1 : 0;
+        sLast = (atomicAdd(&w.arrive[h], 1u) == (uint32_t)(w.nChunkG - 1)) ? 1 : 0;
         if (sLast) {
             __threadfence();
             write_result(b.out[h], st, atomicAdd(&w.inl[h], 0));
@@ -1111,7 +1119,7 @@ static void launch_spread_t(const SpreadArgs &w, hipStream_t s) {
         spread_eq_kernel<BLOCK><<<g, BLOCK, 0, s>>>(w);
         spread_mse_kernel<BLOCK><<<gG, BLOCK, 0, s>>>(w);
     }
-    spread_inlier_kernel<BLOCK><<<g, BLOCK, 0, s>>>(w);
+    spread_inlier_kernel<BLOCK><<<gG, BLOCK, 0, s>>>(w);
 }
 
 static void launch_spread(const SpreadArgs &w, int block, hipStream_t s) {
