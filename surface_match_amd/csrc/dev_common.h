@@ -405,42 +405,64 @@ __device__ int grid_nn1_group(const GridView &g, float qx, float qy, float qz, f
     const int c0z  = cell_coord_unclamped(qz, g.oz, g.cell);
     const int maxR = max(max(g.dx, g.dy), g.dz) + max(max(abs(c0x), abs(c0y)), abs(c0z)) + 1;
     int       r0   = first_ring(g, c0x, c0y, c0z);
-    if (r0 == 0 && g.cstart) {
-        // rings 0 and 1 at once: the 3x3x3 block around the query cell as
-        // nine rows of up to three cells, each row one contiguous run of
-        // sorted slots (a row's cells are consecutive keys); every lane scans
-        // whole rows, so the block costs one start/end load per row and the
-        // scans.  Cells the ring walk would prune only hold points farther
-        // than the running best, which cannot win or tie.
-        const int xlo = max(c0x - 1, 0), xhi = min(c0x + 1, g.dx - 1);
-        for (int j = gl; j < 9; j += G) {
-            const int cz = c0z - 1 + j / 3, cy = c0y - 1 + j % 3;
-            if (cz < 0 || cz >= g.dz || cy < 0 || cy >= g.dy)
-                continue;
-            const int *cs = g.cstart + (cz * g.dy + cy) * g.dx;
-            const int  t1 = cs[xhi + 1];
-            for (int t = cs[xlo]; t < t1; t++) {
-                const float4 P  = g.p4[t];
-                const float  dd = ppfc::dist2(qx, qy, qz, P.x, P.y, P.z);
-                const int    ii = __float_as_int(P.w);
-                if (dd > maxD2)
-                    continue;
-                if (dd < bestD || (dd == bestD && ii < bestId)) {
-                    bestD  = dd;
-                    bestId = ii;
-                    bestS  = t;
+    if (G == 8 && r0 == 0 && g.cstart) {
+        // rings 0 and 1 as the 3x3x3 block around the query cell: nine rows
+        // of up to three cells, each row one contiguous run of sorted slots
+        // (a row's cells are consecutive keys), trimmed to the cells the ring
+        // walk would visit (cell_min_d2 within the bound).  The group scans
+        // every run together, lane l taking slots l, l + G, ... (neighbouring
+        // lanes read neighbouring points), first the query's own row, then
+        // the other eight rows (one per lane to find their runs) under the
+        // best distance found so far.
+        float bound2 = maxD2;
+        for (int phase = 0; phase < 2; phase++) {
+            // this lane's row: the centre row in phase 0, row gl of the
+            // other eight in phase 1
+            int dyz = phase == 0 ? 4 : (gl < 4 ? gl : gl + 1);
+            const int cz = c0z - 1 + dyz / 3, cy = c0y - 1 + dyz % 3;
+            int       t0 = 0, t1 = 0;
+            if (cz >= 0 && cz < g.dz && cy >= 0 && cy < g.dy) {
+                int lo = 2, hi = 0;
+                for (int d = 0; d < 3; d++) {
+                    const int cx = c0x - 1 + d;
+                    if (cx < 0 || cx >= g.dx || cell_min_d2(g, cx, cy, cz, qx, qy, qz) > bound2)
+                        continue;
+                    lo = min(lo, d);
+                    hi = max(hi, d);
+                }
+                if (lo <= hi) {
+                    const int *cs = g.cstart + (cz * g.dy + cy) * g.dx + c0x - 1;
+                    t0            = cs[lo];
+                    t1            = cs[hi + 1];
                 }
             }
-        }
-#pragma unroll
-        for (int o = G / 2; o > 0; o >>= 1) {
-            const float od = __shfl_xor(bestD, o, 64);
-            const int   oi = __shfl_xor(bestId, o, 64), os = __shfl_xor(bestS, o, 64);
-            if (od < bestD || (od == bestD && oi < bestId)) {
-                bestD  = od;
-                bestId = oi;
-                bestS  = os;
+            const int base = (int)(threadIdx.x & 63u) & ~(G - 1);
+            for (int q = 0; q < (phase == 0 ? 1 : G); q++) {
+                const int s0 = __shfl(t0, base + q, 64), s1 = __shfl(t1, base + q, 64);
+                for (int t = s0 + gl; t < s1; t += G) {
+                    const float4 P  = g.p4[t];
+                    const float  dd = ppfc::dist2(qx, qy, qz, P.x, P.y, P.z);
+                    const int    ii = __float_as_int(P.w);
+                    if (dd > maxD2)
+                        continue;
+                    if (dd < bestD || (dd == bestD && ii < bestId)) {
+                        bestD  = dd;
+                        bestId = ii;
+                        bestS  = t;
+                    }
+                }
             }
+#pragma unroll
+            for (int o = G / 2; o > 0; o >>= 1) {
+                const float od = __shfl_xor(bestD, o, 64);
+                const int   oi = __shfl_xor(bestId, o, 64), os = __shfl_xor(bestS, o, 64);
+                if (od < bestD || (od == bestD && oi < bestId)) {
+                    bestD  = od;
+                    bestId = oi;
+                    bestS  = os;
+                }
+            }
+            bound2 = fminf(bestD, maxD2);
         }
         const float safe = ring_safe(g, c0x, c0y, c0z, 1, qx, qy, qz);
         if (safe == FLT_MAX || (bestS >= 0 && bestD < safe * safe) || maxD2 < safe * safe)
@@ -487,50 +509,50 @@ __device__ int grid_nn1_group(const GridView &g, float qx, float qy, float qz, f
 }
 
 // grid_any_within by a group of G consecutive lanes passing the same query:
-// the rows of the query's box of cells are spread over the lanes (a row's
-// cells one contiguous run of sorted slots when the grid has cstart); every
-// lane returns the group's answer.
+// the rows of the query's box of cells in turn, the group scanning each row's
+// points together (lane l taking points l, l + G, ...; with cstart a row's
+// cells are one contiguous run of sorted slots); every lane returns the
+// group's answer.
 template <int G>
 __device__ bool grid_any_within_group(const GridView &g, float qx, float qy, float qz, float r2) {
     static_assert(G >= 1 && G < 64 && (G & (G - 1)) == 0, "group of 1..32 lanes");
     if (g.n == 0 || !(isfinite(qx) && isfinite(qy) && isfinite(qz)))
         return false;
-    const int gl  = (int)(threadIdx.x & (G - 1));
-    float     r   = sqrtf(r2) * 1.0001f + 1e-3f * g.cell;
-    int       xlo = cell_coord(qx - r, g.ox, g.cell, g.dx), xhi = cell_coord(qx + r, g.ox, g.cell, g.dx);
-    int       ylo = cell_coord(qy - r, g.oy, g.cell, g.dy), yhi = cell_coord(qy + r, g.oy, g.cell, g.dy);
-    int       zlo = cell_coord(qz - r, g.oz, g.cell, g.dz), zhi = cell_coord(qz + r, g.oz, g.cell, g.dz);
+    const int      gl    = (int)(threadIdx.x & (G - 1));
+    const uint32_t gbase = threadIdx.x & 63u & ~(uint32_t)(G - 1);
+    float          r     = sqrtf(r2) * 1.0001f + 1e-3f * g.cell;
+    int xlo = cell_coord(qx - r, g.ox, g.cell, g.dx), xhi = cell_coord(qx + r, g.ox, g.cell, g.dx);
+    int ylo = cell_coord(qy - r, g.oy, g.cell, g.dy), yhi = cell_coord(qy + r, g.oy, g.cell, g.dy);
+    int zlo = cell_coord(qz - r, g.oz, g.cell, g.dz), zhi = cell_coord(qz + r, g.oz, g.cell, g.dz);
     const int ny = yhi - ylo + 1, nrows = ny * (zhi - zlo + 1);
-    bool      found = false;
-    for (int j = gl; j < nrows && !found; j += G) {
+    const int ncx = xhi - xlo + 1;
+    for (int j = 0; j < nrows; j++) {
         const int cz = zlo + j / ny, cy = ylo + j % ny;
-        if (g.cstart) {
-            const int *cs = g.cstart + (cz * g.dy + cy) * g.dx;
-            const int  t1 = cs[xhi + 1];
-            for (int t = cs[xlo]; t < t1; t++) {
+        bool      f  = false;
+        for (int cxi = 0; cxi < (g.cstart ? 1 : ncx); cxi++) {
+            int s0, s1;
+            if (g.cstart) {
+                const int *cs = g.cstart + (cz * g.dy + cy) * g.dx;
+                s0            = cs[xlo];
+                s1            = cs[xhi + 1];
+            } else {
+                int c;
+                if (coarse_empty(g, xlo + cxi, cy, cz) || !cell_lookup(g, xlo + cxi, cy, cz, s0, c))
+                    continue;
+                s1 = s0 + c;
+            }
+            for (int t = s0 + gl; t < s1; t += G) {
                 const float4 P = g.p4[t];
                 if (ppfc::dist2(qx, qy, qz, P.x, P.y, P.z) < r2) {
-                    found = true;
+                    f = true;
                     break;
                 }
             }
-        } else {
-            for (int cx = xlo; cx <= xhi && !found; cx++) {
-                int s, c;
-                if (coarse_empty(g, cx, cy, cz) || !cell_lookup(g, cx, cy, cz, s, c))
-                    continue;
-                for (int t = s; t < s + c; t++) {
-                    const float4 P = g.p4[t];
-                    if (ppfc::dist2(qx, qy, qz, P.x, P.y, P.z) < r2) {
-                        found = true;
-                        break;
-                    }
-                }
-            }
         }
+        if (((__ballot(f) >> gbase) & ((1ull << G) - 1ull)) != 0ull)
+            return true;
     }
-    const unsigned long long b = __ballot(found);
-    return ((b >> (threadIdx.x & 63u & ~(unsigned)(G - 1))) & ((1ull << G) - 1ull)) != 0ull;
+    return false;
 }
 
 // true if any indexed point has dist2 < r2
