@@ -546,18 +546,21 @@ constexpr int KNN_MAX = 32;
 // FP32 centroid + shifted covariance in kNN order over the snapshot
 // positions (g.nx.. = positions indexed by id), FP64 Jacobi eigen-solve,
 // eigenvector of the smallest |eigenvalue|.
+// The neighbours' coordinates come from px/py/pz (entry t of the list) when
+// given, else from the snapshot by id; they are the same values.
 template <int KMAX>
 __device__ void pca_of_list(const GridView &g, const KnnList<KMAX> &L, float cen[3],
-                            float nrm[3]) {
+                            float nrm[3], const float *px = nullptr, const float *py = nullptr,
+                            const float *pz = nullptr) {
     const int cnt = L.cnt;
     float     sx = 0.0f, sy = 0.0f, sz = 0.0f;
 #pragma unroll
     for (int t = 0; t < KMAX; t++) {
         if (t < cnt) {
             int id = L.id[t];
-            sx     = sx + g.nx[id];
-            sy     = sy + g.ny[id];
-            sz     = sz + g.nz[id];
+            sx     = sx + (px ? px[t] : g.nx[id]);
+            sy     = sy + (px ? py[t] : g.ny[id]);
+            sz     = sz + (px ? pz[t] : g.nz[id]);
         }
     }
     float fk = (float)cnt;
@@ -569,9 +572,9 @@ __device__ void pca_of_list(const GridView &g, const KnnList<KMAX> &L, float cen
     for (int t = 0; t < KMAX; t++) {
         if (t < cnt) {
             int   id = L.id[t];
-            float x  = g.nx[id] - cen[0];
-            float y  = g.ny[id] - cen[1];
-            float z  = g.nz[id] - cen[2];
+            float x  = (px ? px[t] : g.nx[id]) - cen[0];
+            float y  = (px ? py[t] : g.ny[id]) - cen[1];
+            float z  = (px ? pz[t] : g.nz[id]) - cen[2];
             c00 = c00 + x * x;
             c01 = c01 + x * y;
             c02 = c02 + x * z;
@@ -614,14 +617,16 @@ struct NormArgs {
 // store one estimate (want 1: a query, with its kNN list for smoothing;
 // want 2: a neighbour needed by the smoothing)
 template <int KMAX>
-__device__ void store_estimate(const NormArgs &a, int id, int want, const KnnList<KMAX> &L) {
+__device__ void store_estimate(const NormArgs &a, int id, int want, const KnnList<KMAX> &L,
+                               const float *px = nullptr, const float *py = nullptr,
+                               const float *pz = nullptr) {
     const int cnt = L.cnt;
     if (want == 1)
         a.cntq[a.qidx[id]] = cnt < 3 ? 0 : cnt;
     if (cnt < 3)
         return;
     float cen[3], nrm[3];
-    pca_of_list<KMAX>(a.g, L, cen, nrm);
+    pca_of_list<KMAX>(a.g, L, cen, nrm, px, py, pz);
     a.rawX[id] = nrm[0];
     a.rawY[id] = nrm[1];
     a.rawZ[id] = nrm[2];
@@ -707,10 +712,12 @@ constexpr int  KNN_GROUP     = 16;
 constexpr bool NORMALS_COOP  = true;
 
 __device__ __forceinline__ void coop_knn(const GridView &g, float qx, float qy, float qz, int k,
-                                         int l, int gsh, float &myD, int &myId, int &cnt) {
+                                         int l, int gsh, float &myD, int &myId, int &mySlot,
+                                         int &cnt) {
     constexpr uint32_t GMASK = (1u << KNN_GROUP) - 1u;
     myD      = FLT_MAX;
     myId     = 0x7fffffff;
+    mySlot   = -1;
     cnt      = 0;
     float wd = FLT_MAX;
     int   wi = 0x7fffffff;
@@ -738,6 +745,7 @@ __device__ __forceinline__ void coop_knn(const GridView &g, float qx, float qy, 
                     m &= m - 1u;
                     const float cd = __shfl(dd, src, KNN_GROUP);
                     const int   ci = __shfl(ii, src, KNN_GROUP);
+                    const int   cs = __shfl(t, src, KNN_GROUP);
                     if (cnt == k && !(cd < wd || (cd == wd && ci < wi)))
                         continue;
                     const bool before = l < cnt && (cd < myD || (cd == myD && ci < myId));
@@ -745,13 +753,16 @@ __device__ __forceinline__ void coop_knn(const GridView &g, float qx, float qy, 
                     const int      pos = bm ? __ffs(bm) - 1 : cnt;
                     const float    upD = __shfl_up(myD, 1, KNN_GROUP);
                     const int      upI = __shfl_up(myId, 1, KNN_GROUP);
+                    const int      upS = __shfl_up(mySlot, 1, KNN_GROUP);
                     if (l > pos && l < k) {
-                        myD  = upD;
-                        myId = upI;
+                        myD    = upD;
+                        myId   = upI;
+                        mySlot = upS;
                     }
                     if (l == pos) {
-                        myD  = cd;
-                        myId = ci;
+                        myD    = cd;
+                        myId   = ci;
+                        mySlot = cs;
                     }
                     cnt = cnt < k ? cnt + 1 : k;
                     if (cnt == k) {
@@ -769,10 +780,12 @@ __device__ __forceinline__ void coop_knn(const GridView &g, float qx, float qy, 
     }
 }
 
-// the kNN lists of the points list[0..nl) with mark == want: ids
-// (k per point, ascending by (dist2, id)) and counts (-1: not estimated)
+// the kNN lists of the points list[0..nl) with mark == want: each
+// neighbour as {x, y, z, id bits} from the grid's slot-ordered copies (k
+// per point, ascending by (dist2, id); the PCA then reads its list as one
+// contiguous run instead of gathering by id) and counts (-1: not estimated)
 __global__ void __launch_bounds__(256) normals_knn_coop_kernel(NormArgs a, const int *list,
-                                                               int nl, int want, int *ids,
+                                                               int nl, int want, float4 *nbr,
                                                                int *cnts) {
     const int t    = (int)((blockIdx.x * blockDim.x + threadIdx.x) / KNN_GROUP);
     const int lane = (int)(threadIdx.x & 63u);
@@ -787,10 +800,11 @@ __global__ void __launch_bounds__(256) normals_knn_coop_kernel(NormArgs a, const
         return;
     }
     float d;
-    int   i, cnt;
-    coop_knn(a.g, a.g.nx[id], a.g.ny[id], a.g.nz[id], a.k, l, gsh, d, i, cnt);
+    int   i, slot, cnt;
+    coop_knn(a.g, a.g.nx[id], a.g.ny[id], a.g.nz[id], a.k, l, gsh, d, i, slot, cnt);
     if (l < cnt)
-        ids[(size_t)t * a.k + l] = i;
+        nbr[(size_t)t * a.k + l] =
+            make_float4(a.g.x[slot], a.g.y[slot], a.g.z[slot], __int_as_float(i));
     if (l == 0)
         cnts[t] = cnt;
 }
@@ -798,7 +812,7 @@ __global__ void __launch_bounds__(256) normals_knn_coop_kernel(NormArgs a, const
 // PCA of each point from its kNN list (one thread per point)
 template <int KMAX>
 __global__ void __launch_bounds__(128) normals_pca_kernel(NormArgs a, const int *list, int nl,
-                                                          int want, const int *ids,
+                                                          int want, const float4 *nbr,
                                                           const int *cnts) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nl)
@@ -809,11 +823,17 @@ __global__ void __launch_bounds__(128) normals_pca_kernel(NormArgs a, const int 
     KnnList<KMAX> L;
     L.init();
     L.cnt = cnt;
+    float px[KMAX], py[KMAX], pz[KMAX];
 #pragma unroll
     for (int x = 0; x < KMAX; x++)
-        if (x < cnt)
-            L.id[x] = ids[(size_t)t * a.k + x];
-    store_estimate<KMAX>(a, list[t], want, L);
+        if (x < cnt) {
+            const float4 P = nbr[(size_t)t * a.k + x];
+            px[x]          = P.x;
+            py[x]          = P.y;
+            pz[x]          = P.z;
+            L.id[x]        = __float_as_int(P.w);
+        }
+    store_estimate<KMAX>(a, list[t], want, L, px, py, pz);
 }
 
 // final normal of each query (smoothing, util.cpp:292-306)
@@ -1001,11 +1021,11 @@ void estimate_normals(Workspace &ws, int n, float *PX, float *PY, float *PZ, flo
         if (nl <= 0)
             return;
         if (NORMALS_COOP && k <= KNN_GROUP) {
-            int *ids  = ws.knnIds.as<int>((size_t)nl * k);
-            int *cnts = ws.knnCnt.as<int>(nl);
+            float4 *nbr  = ws.knnIds.as<float4>((size_t)nl * k);
+            int    *cnts = ws.knnCnt.as<int>(nl);
             normals_knn_coop_kernel<<<div_up(nl, 256 / KNN_GROUP), 256, 0, s>>>(a, list, nl, want,
-                                                                               ids, cnts);
-            normals_pca_kernel<16><<<div_up(nl, 128), 128, 0, s>>>(a, list, nl, want, ids, cnts);
+                                                                               nbr, cnts);
+            normals_pca_kernel<16><<<div_up(nl, 128), 128, 0, s>>>(a, list, nl, want, nbr, cnts);
         } else if (small)
             normals_point_kernel<16><<<div_up(nl, 128), 128, 0, s>>>(a, list, nl, want);
         else
