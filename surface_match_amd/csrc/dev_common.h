@@ -59,8 +59,11 @@ inline int div_up(long long a, long long b) { return (int)((a + b - 1) / b); }
 // --------------------------------------------------------------------------
 // hashed uniform grid
 // --------------------------------------------------------------------------
-// Points of a cloud (or a subset) are sorted by cell (stable, so ids stay
-// ascending inside a cell); occupied cells live in an open-addressing table
+// Points of a cloud (or a subset) are sorted by cell: grids of at most
+// GRID_DENSE_MAX cells by a counting sort (the order of the points inside a
+// cell is that of the per-cell atomic counters, which no search depends
+// on: every search orders its candidates by (dist2, id)), larger ones by a
+// stable radix sort with their occupied cells in an open-addressing table
 // {key+1, start, count, 0} (key+1 == 0 marks an empty slot).  Cells are
 // floor((p - org) / cell) per axis, clamped into [0, dim).
 struct GridView {
@@ -582,7 +585,8 @@ __device__ inline bool grid_any_within(const GridView &g, float qx, float qy, fl
 // host side of the grid
 // --------------------------------------------------------------------------
 struct Grid {
-    DevBuf   keys, keys2, vals, vals2, htab, x, y, z, nx, ny, nz, tmp, bbox, occ, cstart, p4, n4;
+    DevBuf   keys, keys2, vals, vals2, htab, x, y, z, nx, ny, nz, tmp, bbox, occ, cstart, p4, n4,
+        cnt;
     GridView view{};
     int      capN = 0;
 };

@@ -102,6 +102,62 @@ __global__ void grid_keys_kernel(const float *X, const float *Y, const float *Z,
     vals[t] = i;
 }
 
+// counting sort of a dense grid, first pass: each point's cell key, its
+// rank among its cell's points (a per-cell atomic counter) and, for the
+// first point of a cell, the cell's coarse occupancy bit
+__global__ void grid_rank_kernel(const float *X, const float *Y, const float *Z, const int *subset,
+                                 int m, float ox, float oy, float oz, float cell, int dx, int dy,
+                                 int dz, int odx, int ody, int sh, uint32_t *keys, int *vals,
+                                 int *rank, int *cnt, uint32_t *occ) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= m)
+        return;
+    int            i   = subset ? subset[t] : t;
+    int            cx  = cell_coord(X[i], ox, cell, dx);
+    int            cy  = cell_coord(Y[i], oy, cell, dy);
+    int            cz  = cell_coord(Z[i], oz, cell, dz);
+    const uint32_t key = (uint32_t)((cz * dy + cy) * dx + cx);
+    keys[t]            = key;
+    vals[t]            = i;
+    const int r        = atomicAdd(&cnt[key], 1);
+    rank[t]            = r;
+    if (r == 0) {
+        uint32_t b = ((uint32_t)(cz >> sh) * (uint32_t)ody + (uint32_t)(cy >> sh)) * (uint32_t)odx +
+                     (uint32_t)(cx >> sh);
+        atomicOr(&occ[b >> 5], 1u << (b & 31u));
+    }
+}
+
+// second pass: every point to its slot (cell start + rank) with the copies
+// the grid keeps (ids, positions, normals, the 16-byte {x, y, z, id} and
+// {nx, ny, nz, 0} copies), all optional but the ids
+__global__ void grid_scatter_kernel(const uint32_t *keys, const int *vals, const int *rank, int m,
+                                    const int *cstart, const float *X, const float *Y,
+                                    const float *Z, const float *NX, const float *NY,
+                                    const float *NZ, int *id, float *x, float *y, float *z,
+                                    float *nx, float *ny, float *nz, float4 *p4, float4 *n4) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= m)
+        return;
+    const int i    = vals[t];
+    const int slot = cstart[keys[t]] + rank[t];
+    id[slot]       = i;
+    const float px = X[i], py = Y[i], pz = Z[i];
+    x[slot]        = px;
+    y[slot]        = py;
+    z[slot]        = pz;
+    if (p4)
+        p4[slot] = make_float4(px, py, pz, __int_as_float(i));
+    if (nx) {
+        const float qx = NX[i], qy = NY[i], qz = NZ[i];
+        nx[slot]       = qx;
+        ny[slot]       = qy;
+        nz[slot]       = qz;
+        if (n4)
+            n4[slot] = make_float4(qx, qy, qz, 0.0f);
+    }
+}
+
 // coarse occupancy bit of every occupied cell (first slot of each cell run)
 __global__ void grid_occ_kernel(const uint32_t *keys, int m, int dx, int dy, int odx, int ody,
                                 int sh, uint32_t *occ) {
@@ -254,34 +310,66 @@ void grid_build(Grid &g, const float *X, const float *Y, const float *Z, const f
     const size_t occWords = ((size_t)odx * ody * odz + 31) / 32;
     uint32_t    *occ      = g.occ.as<uint32_t>(occWords);
     int         *cstart   = nullptr;
-    {
-        // the cell table (hashed cells only) and the occupancy bits, cleared in one launch
-        const size_t words = (dense ? 0 : (size_t)4 * hsize) + occWords;
-        grid_clear_kernel<<<(int)std::min<size_t>((words + 255) / 256, 2048), 256, 0, s>>>(
-            reinterpret_cast<uint32_t *>(htab), dense ? 0 : (size_t)4 * hsize, occ, occWords);
-    }
-    if (m > 0) {
-        grid_keys_kernel<<<div_up(m, 256), 256, 0, s>>>(X, Y, Z, subset, m, hb[0], hb[1], hb[2],
-                                                         cell, dims[0], dims[1], dims[2], keys,
-                                                         vals);
-        int endBit = 1;
-        while ((1ull << endBit) < (unsigned long long)dims[0] * dims[1] * dims[2])
-            endBit++;
-        size_t tb = 0;
-        PPF_HIP_CHECK(rocprim::radix_sort_pairs(nullptr, tb, keys, keys2, vals, vals2, m, 0, endBit, s));
-        void *tmp = g.tmp.reserve(tb);
-        PPF_HIP_CHECK(rocprim::radix_sort_pairs(tmp, tb, keys, keys2, vals, vals2, m, 0, endBit, s));
-        if (dense) {
-            cstart = g.cstart.as<int>(cells + 1);
-            grid_cstart_kernel<<<div_up((int)cells + 1, 256), 256, 0, s>>>(keys2, m,
-                                                                            (uint32_t)cells, cstart);
-        } else {
+    if (dense && m > 0) {
+        // counting sort: per-cell counts (cleared with the occupancy bits),
+        // their exclusive scan as the cell starts, then one scatter that
+        // also writes the point copies
+        int *cnt = g.cnt.as<int>(cells + 1);
+        cstart   = g.cstart.as<int>(cells + 1);
+        grid_clear_kernel<<<(int)std::min<size_t>((cells + 1 + occWords + 255) / 256, 4096), 256, 0,
+                            s>>>(reinterpret_cast<uint32_t *>(cnt), cells + 1, occ, occWords);
+        {
+            int *rank = reinterpret_cast<int *>(keys2);
+            grid_rank_kernel<<<div_up(m, 256), 256, 0, s>>>(X, Y, Z, subset, m, hb[0], hb[1], hb[2],
+                                                             cell, dims[0], dims[1], dims[2], odx,
+                                                             ody, sh, keys, vals, rank, cnt, occ);
+            size_t tb = 0;
+            PPF_HIP_CHECK(rocprim::exclusive_scan(nullptr, tb, cnt, cstart, 0, cells + 1,
+                                                  rocprim::plus<int>(), s));
+            void *tmp = g.tmp.reserve(tb);
+            PPF_HIP_CHECK(rocprim::exclusive_scan(tmp, tb, cnt, cstart, 0, cells + 1,
+                                                  rocprim::plus<int>(), s));
+            float  *nx = NX ? g.nx.as<float>(m) : nullptr, *ny = NX ? g.ny.as<float>(m) : nullptr,
+                  *nz = NX ? g.nz.as<float>(m) : nullptr;
+            float4 *p4 = aos ? g.p4.as<float4>(m) : nullptr,
+                   *n4 = (aos && NX) ? g.n4.as<float4>(m) : nullptr;
+            grid_scatter_kernel<<<div_up(m, 256), 256, 0, s>>>(keys, vals, rank, m, cstart, X, Y, Z,
+                                                                NX, NY, NZ, vals2, x, y, z, nx, ny,
+                                                                nz, p4, n4);
+        }
+    } else {
+        // hashed cells (or an empty grid): the cell table and the occupancy
+        // bits, cleared in one launch
+        grid_clear_kernel<<<(int)std::min<size_t>(((size_t)4 * hsize + occWords + 255) / 256, 2048),
+                            256, 0, s>>>(reinterpret_cast<uint32_t *>(htab), (size_t)4 * hsize, occ,
+                                         occWords);
+        if (m > 0) {
+            grid_keys_kernel<<<div_up(m, 256), 256, 0, s>>>(X, Y, Z, subset, m, hb[0], hb[1],
+                                                             hb[2], cell, dims[0], dims[1],
+                                                             dims[2], keys, vals);
+            int endBit = 1;
+            while ((1ull << endBit) < (unsigned long long)dims[0] * dims[1] * dims[2])
+                endBit++;
+            size_t tb = 0;
+            PPF_HIP_CHECK(rocprim::radix_sort_pairs(nullptr, tb, keys, keys2, vals, vals2, m, 0,
+                                                    endBit, s));
+            void *tmp = g.tmp.reserve(tb);
+            PPF_HIP_CHECK(rocprim::radix_sort_pairs(tmp, tb, keys, keys2, vals, vals2, m, 0, endBit,
+                                                    s));
             grid_insert_kernel<<<div_up(m, 256), 256, 0, s>>>(keys2, m, htab, hsize - 1);
             grid_count_kernel<<<div_up(m, 256), 256, 0, s>>>(keys2, m, htab, hsize - 1);
+            grid_occ_kernel<<<div_up(m, 256), 256, 0, s>>>(keys2, m, dims[0], dims[1], odx, ody,
+                                                            sh, occ);
+            gather3_kernel<<<div_up(m, 256), 256, 0, s>>>(vals2, m, X, Y, Z, x, y, z);
+            if (NX)
+                gather3_kernel<<<div_up(m, 256), 256, 0, s>>>(vals2, m, NX, NY, NZ,
+                                                               g.nx.as<float>(m), g.ny.as<float>(m),
+                                                               g.nz.as<float>(m));
+            if (aos)
+                gather_aos_kernel<<<div_up(m, 256), 256, 0, s>>>(
+                    vals2, m, X, Y, Z, NX, NY, NZ, g.p4.as<float4>(m),
+                    NX ? g.n4.as<float4>(m) : nullptr);
         }
-        grid_occ_kernel<<<div_up(m, 256), 256, 0, s>>>(keys2, m, dims[0], dims[1], odx, ody, sh,
-                                                        occ);
-        gather3_kernel<<<div_up(m, 256), 256, 0, s>>>(vals2, m, X, Y, Z, x, y, z);
     }
     GridView &v = g.view;
     v.ox = hb[0];
@@ -306,17 +394,13 @@ void grid_build(Grid &g, const float *X, const float *Y, const float *Z, const f
     v.nx = v.ny = v.nz = nullptr;
     v.p4 = v.n4 = nullptr;
     if (NX && m > 0) {
-        float *nx = g.nx.as<float>(m), *ny = g.ny.as<float>(m), *nz = g.nz.as<float>(m);
-        gather3_kernel<<<div_up(m, 256), 256, 0, s>>>(vals2, m, NX, NY, NZ, nx, ny, nz);
-        v.nx = nx;
-        v.ny = ny;
-        v.nz = nz;
+        v.nx = g.nx.get<float>();
+        v.ny = g.ny.get<float>();
+        v.nz = g.nz.get<float>();
     }
     if (aos && m > 0) {
-        float4 *p4 = g.p4.as<float4>(m), *n4 = NX ? g.n4.as<float4>(m) : nullptr;
-        gather_aos_kernel<<<div_up(m, 256), 256, 0, s>>>(vals2, m, X, Y, Z, NX, NY, NZ, p4, n4);
-        v.p4 = p4;
-        v.n4 = n4;
+        v.p4 = g.p4.get<float4>();
+        v.n4 = NX ? g.n4.get<float4>() : nullptr;
     }
     PPF_HIP_CHECK(hipGetLastError());
 }
