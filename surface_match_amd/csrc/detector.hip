@@ -295,7 +295,7 @@ struct ppf_detector_s {
     TrainScratch ts;
     VoteConfig   vcfg;
     IcpConfig    icfg;
-    int          normCell4 = 6;  // scene normals grid cell = sample step * normCell4 / 4
+    int          normCell4 = 4;  // scene normals grid cell = sample step * normCell4 / 4
     int          refTail = 1;
     std::shared_mutex                      model;   // shared: readers, unique: writers
     std::mutex                             poolMu;  // guards the two lists below
@@ -2099,7 +2099,7 @@ PPF_API int ppf_autotune(ppf_detector d, ppf_scene sc, float rel, float kpFrac, 
             }
             d->*knob = best;
         };
-        tryNormals(&ppf_detector_s::normCell4, {4, 5, 6, 8});
+        tryNormals(&ppf_detector_s::normCell4, {3, 4, 5, 6});
         rep->normalsMs        = bestNormMs;
         rep->best             = tuning_of(*d);
         rep->voteMs           = bestVoteMs;

@@ -807,55 +807,121 @@ __device__ __forceinline__ void coop_knn(const GridView &g, float qx, float qy, 
     int   wi = 0x7fffffff;
     if (g.n == 0 || !(isfinite(qx) && isfinite(qy) && isfinite(qz)))
         return;
+    // the points of one cell, KNN_GROUP consecutive slots at a time
+    auto scan_cell = [&](int s, int c) {
+        for (int base = s; base < s + c; base += KNN_GROUP) {
+            const int t    = base + l;
+            float     dd   = FLT_MAX;
+            int       ii   = 0x7fffffff;
+            bool      pass = false;
+            if (t < s + c) {
+                dd   = ppfc::dist2(qx, qy, qz, g.x[t], g.y[t], g.z[t]);
+                ii   = g.id[t];
+                pass = cnt < k || dd < wd || (dd == wd && ii < wi);
+            }
+            uint32_t m = (uint32_t)(__ballot(pass) >> gsh) & GMASK;
+            while (m) {
+                const int src = __ffs(m) - 1;
+                m &= m - 1u;
+                const float cd = __shfl(dd, src, KNN_GROUP);
+                const int   ci = __shfl(ii, src, KNN_GROUP);
+                const int   cs = __shfl(t, src, KNN_GROUP);
+                if (cnt == k && !(cd < wd || (cd == wd && ci < wi)))
+                    continue;
+                const bool before = l < cnt && (cd < myD || (cd == myD && ci < myId));
+                const uint32_t bm = (uint32_t)(__ballot(before) >> gsh) & GMASK;
+                const int      pos = bm ? __ffs(bm) - 1 : cnt;
+                const float    upD = __shfl_up(myD, 1, KNN_GROUP);
+                const int      upI = __shfl_up(myId, 1, KNN_GROUP);
+                const int      upS = __shfl_up(mySlot, 1, KNN_GROUP);
+                if (l > pos && l < k) {
+                    myD    = upD;
+                    myId   = upI;
+                    mySlot = upS;
+                }
+                if (l == pos) {
+                    myD    = cd;
+                    myId   = ci;
+                    mySlot = cs;
+                }
+                cnt = cnt < k ? cnt + 1 : k;
+                if (cnt == k) {
+                    wd = __shfl(myD, k - 1, KNN_GROUP);
+                    wi = __shfl(myId, k - 1, KNN_GROUP);
+                }
+            }
+        }
+    };
     const int c0x  = cell_coord_unclamped(qx, g.ox, g.cell);
     const int c0y  = cell_coord_unclamped(qy, g.oy, g.cell);
     const int c0z  = cell_coord_unclamped(qz, g.oz, g.cell);
     const int maxR = max(max(g.dx, g.dy), g.dz) + max(max(abs(c0x), abs(c0y)), abs(c0z)) + 1;
     for (int r = first_ring(g, c0x, c0y, c0z); r <= maxR; r++) {
-        visit_ring(g, c0x, c0y, c0z, r, qx, qy, qz, wd, [&](int s, int c) {
-            for (int base = s; base < s + c; base += KNN_GROUP) {
-                const int t    = base + l;
-                float     dd   = FLT_MAX;
-                int       ii   = 0x7fffffff;
-                bool      pass = false;
-                if (t < s + c) {
-                    dd   = ppfc::dist2(qx, qy, qz, g.x[t], g.y[t], g.z[t]);
-                    ii   = g.id[t];
-                    pass = cnt < k || dd < wd || (dd == wd && ii < wi);
+        const int zlo = max(c0z - r, 0), zhi = min(c0z + r, g.dz - 1);
+        const int ylo = max(c0y - r, 0), yhi = min(c0y + r, g.dy - 1);
+        const int xlo = max(c0x - r, 0), xhi = min(c0x + r, g.dx - 1);
+        if (g.occShift != 0 || xhi - xlo >= 32) {
+            // rows wider than a bitmap word, or a coarse bitmap: every lane
+            // walks the whole ring
+            visit_ring(g, c0x, c0y, c0z, r, qx, qy, qz, wd, scan_cell);
+        } else if (zlo <= zhi && ylo <= yhi && xlo <= xhi) {
+            // the ring's rows split over the group: each lane reads the
+            // occupancy bits of its row (the whole row on a z/y face of the
+            // shell, else its two end cells) and keeps the cells within the
+            // current k-th distance; then the group scans the kept cells of
+            // every lane together, in lane order
+            const int ny = yhi - ylo + 1, nrows = (zhi - zlo + 1) * ny;
+            for (int rb = 0; rb < nrows; rb += KNN_GROUP) {
+                const int j  = rb + l;
+                uint32_t  bits = 0;
+                int       cy = 0, cz = 0;
+                if (j < nrows) {
+                    cz              = zlo + j / ny;
+                    cy              = ylo + j % ny;
+                    const bool face = (cz == c0z - r) || (cz == c0z + r) || (cy == c0y - r) ||
+                                      (cy == c0y + r);
+                    const uint32_t b0 = ((uint32_t)cz * (uint32_t)g.ody + (uint32_t)cy) *
+                                            (uint32_t)g.odx + (uint32_t)xlo;
+                    const int      n  = xhi - xlo + 1;
+                    // bits xlo..xhi of the row (at most two bitmap words)
+                    uint32_t       w0 = g.occ[b0 >> 5] >> (b0 & 31u);
+                    if ((int)(32u - (b0 & 31u)) < n)
+                        w0 |= g.occ[(b0 >> 5) + 1] << (32u - (b0 & 31u));
+                    bits = w0 & (n == 32 ? ~0u : ((1u << n) - 1u));
+                    if (!face) {
+                        uint32_t ends = 0;
+                        if (c0x - r >= xlo)
+                            ends |= 1u << (c0x - r - xlo);
+                        if (c0x + r <= xhi)
+                            ends |= 1u << (c0x + r - xlo);
+                        bits &= ends;
+                    }
+                    for (uint32_t b = bits; b;) {
+                        const int cx = xlo + __ffs(b) - 1;
+                        b &= b - 1u;
+                        if (cell_min_d2(g, cx, cy, cz, qx, qy, qz) > wd)
+                            bits &= ~(1u << (cx - xlo));
+                    }
                 }
-                uint32_t m = (uint32_t)(__ballot(pass) >> gsh) & GMASK;
-                while (m) {
-                    const int src = __ffs(m) - 1;
-                    m &= m - 1u;
-                    const float cd = __shfl(dd, src, KNN_GROUP);
-                    const int   ci = __shfl(ii, src, KNN_GROUP);
-                    const int   cs = __shfl(t, src, KNN_GROUP);
-                    if (cnt == k && !(cd < wd || (cd == wd && ci < wi)))
-                        continue;
-                    const bool before = l < cnt && (cd < myD || (cd == myD && ci < myId));
-                    const uint32_t bm = (uint32_t)(__ballot(before) >> gsh) & GMASK;
-                    const int      pos = bm ? __ffs(bm) - 1 : cnt;
-                    const float    upD = __shfl_up(myD, 1, KNN_GROUP);
-                    const int      upI = __shfl_up(myId, 1, KNN_GROUP);
-                    const int      upS = __shfl_up(mySlot, 1, KNN_GROUP);
-                    if (l > pos && l < k) {
-                        myD    = upD;
-                        myId   = upI;
-                        mySlot = upS;
-                    }
-                    if (l == pos) {
-                        myD    = cd;
-                        myId   = ci;
-                        mySlot = cs;
-                    }
-                    cnt = cnt < k ? cnt + 1 : k;
-                    if (cnt == k) {
-                        wd = __shfl(myD, k - 1, KNN_GROUP);
-                        wi = __shfl(myId, k - 1, KNN_GROUP);
+                uint32_t lanes = (uint32_t)(__ballot(bits != 0u) >> gsh) & GMASK;
+                while (lanes) {
+                    const int src = __ffs(lanes) - 1;
+                    lanes &= lanes - 1u;
+                    uint32_t  sb  = (uint32_t)__shfl((int)bits, src, KNN_GROUP);
+                    const int scy = __shfl(cy, src, KNN_GROUP), scz = __shfl(cz, src, KNN_GROUP);
+                    while (sb) {
+                        const int cx = xlo + __ffs(sb) - 1;
+                        sb &= sb - 1u;
+                        // the bound may have shrunk since the row was read
+                        if (cell_min_d2(g, cx, scy, scz, qx, qy, qz) > wd)
+                            continue;
+                        int s0, c;
+                        if (cell_lookup(g, cx, scy, scz, s0, c))
+                            scan_cell(s0, c);
                     }
                 }
             }
-        });
+        }
         const float safe = ring_safe(g, c0x, c0y, c0z, r, qx, qy, qz);
         if (safe == FLT_MAX)
             break;
