@@ -92,6 +92,7 @@ struct IcpArgs {
     unsigned long long *hbest;  // H x T
     int         *nnSlot; // H x nSrc: dst sorted slot of the pair (-1 = none)
     int         *hslot;  // H x nSrc: hash slot used
+    int         *wslot;  // H x nSrc: dst slot of a winning pair, -1 otherwise (set by add_means)
     float       *nnD;    // H x nSrc
     IcpResultDev *out;
     // several scenes: the target grids of hypothesis h are dstG[hypScene[h]]
@@ -278,7 +279,7 @@ __device__ __forceinline__ void xform(const float *T, float x, float y, float z,
 struct HypoScratch {
     uint32_t           *hkey;
     unsigned long long *hbest;
-    int                *nnSlot, *hslot;
+    int                *nnSlot, *hslot, *wslot;
     float              *nnD;
     uint32_t            T;
 };
@@ -293,6 +294,7 @@ __device__ __forceinline__ HypoScratch scratch_of(const IcpArgs &a, int h, int t
     x.hbest  = a.hbest + base;
     x.nnSlot = a.nnSlot + (size_t)h * a.nSrc;
     x.hslot  = a.hslot + (size_t)h * a.nSrc;
+    x.wslot  = a.wslot + (size_t)h * a.nSrc;
     x.nnD    = a.nnD + (size_t)h * a.nSrc;
     x.T      = a.T;
     return x;
@@ -354,19 +356,24 @@ __device__ __forceinline__ bool is_winner(const HypoScratch &x, int i) {
     return x.hbest[x.hslot[i]] == pk;
 }
 
+// a winning pair's scene point (and -1 for every other source point) is
+// kept in wslot for the iteration's later steps
 __device__ __forceinline__ void add_means(const IcpArgs &a, const HypoScratch &x, int i,
                                           const float *curF, CF (&m7)[7]) {
-    if (!is_winner(x, i))
+    const bool win = is_winner(x, i);
+    const int  s   = win ? x.nnSlot[i] : -1;
+    x.wslot[i]     = s;
+    if (!win)
         return;
     float px, py, pz;
     xform(curF, a.sx[i], a.sy[i], a.sz[i], px, py, pz);
-    int s = x.nnSlot[i];
+    const float4 Q = a.dst.p4[s];
     cf_add(m7[0], px);
     cf_add(m7[1], py);
     cf_add(m7[2], pz);
-    cf_add(m7[3], a.dst.x[s]);
-    cf_add(m7[4], a.dst.y[s]);
-    cf_add(m7[5], a.dst.z[s]);
+    cf_add(m7[3], Q.x);
+    cf_add(m7[4], Q.y);
+    cf_add(m7[5], Q.z);
     cf_add(m7[6], 1.0f);
 }
 
@@ -376,8 +383,9 @@ __device__ __forceinline__ void eq_row(const IcpArgs &a, int i, int s, const flo
                                        const float *means, int mode, float J[6], float &r) {
     float px, py, pz;
     xform(curF, a.sx[i], a.sy[i], a.sz[i], px, py, pz);
-    float qx = a.dst.x[s], qy = a.dst.y[s], qz = a.dst.z[s];
-    float nqx = a.dst.nx[s], nqy = a.dst.ny[s], nqz = a.dst.nz[s];
+    const float4 Q = a.dst.p4[s], NQ = a.dst.n4[s];
+    float qx = Q.x, qy = Q.y, qz = Q.z;
+    float nqx = NQ.x, nqy = NQ.y, nqz = NQ.z;
     float ux = px - means[0], uy = py - means[1], uz = pz - means[2];
     if (mode == 0) {
         float mx = a.snx[i], my = a.sny[i], mz = a.snz[i];
@@ -408,10 +416,11 @@ __device__ __forceinline__ void eq_row(const IcpArgs &a, int i, int s, const flo
 __device__ __forceinline__ void add_eq(const IcpArgs &a, const HypoScratch &x, int i,
                                        const float *curF, const float *means, int mode,
                                        CF (&e27)[27]) {
-    if (!is_winner(x, i))
+    const int s = x.wslot[i];
+    if (s < 0)
         return;
     float J[6], r;
-    eq_row(a, i, x.nnSlot[i], curF, means, mode, J, r);
+    eq_row(a, i, s, curF, means, mode, J, r);
     int k = 0;
 #pragma unroll
     for (int p = 0; p < 6; p++)
@@ -429,13 +438,14 @@ __device__ __forceinline__ void add_eq(const IcpArgs &a, const HypoScratch &x, i
 // it and returns the same (dist2, id) as an unbounded one.
 __device__ __forceinline__ void add_mse(const IcpArgs &a, const HypoScratch &x, int i,
                                         const float *curF, const float *UF, CF (&m1)[1]) {
-    if (!is_winner(x, i))
+    const int s = x.wslot[i];
+    if (s < 0)
         return;
     float px, py, pz, qx, qy, qz;
     xform(curF, a.sx[i], a.sy[i], a.sz[i], px, py, pz);
     xform(UF, px, py, pz, qx, qy, qz);
-    const int s     = x.nnSlot[i];
-    float     bound = ppfc::dist2(qx, qy, qz, a.dst.x[s], a.dst.y[s], a.dst.z[s]);
+    const float4 Q     = a.dst.p4[s];
+    float        bound = ppfc::dist2(qx, qy, qz, Q.x, Q.y, Q.z);
     float     d2;
     int       did;
     if (!(bound < FLT_MAX))
@@ -448,13 +458,14 @@ __device__ __forceinline__ void add_mse(const IcpArgs &a, const HypoScratch &x, 
 template <int G>
 __device__ __forceinline__ void add_mse_group(const IcpArgs &a, const HypoScratch &x, int i,
                                               const float *curF, const float *UF, CF (&m1)[1]) {
-    if (!is_winner(x, i))
+    const int s = x.wslot[i];
+    if (s < 0)
         return;
     float px, py, pz, qx, qy, qz;
     xform(curF, a.sx[i], a.sy[i], a.sz[i], px, py, pz);
     xform(UF, px, py, pz, qx, qy, qz);
-    const int s     = x.nnSlot[i];
-    float     bound = ppfc::dist2(qx, qy, qz, a.dst.x[s], a.dst.y[s], a.dst.z[s]);
+    const float4 Q     = a.dst.p4[s];
+    float        bound = ppfc::dist2(qx, qy, qz, Q.x, Q.y, Q.z);
     float     d2;
     int       did;
     if (!(bound < FLT_MAX))
@@ -1086,10 +1097,11 @@ static IcpArgs make_args(Workspace &ws, int nSrc, const float *sx, const float *
     // two dedupe tables per hypothesis (the spread refiner alternates them)
     a.hkey   = ws.icpHash.as<uint32_t>((size_t)2 * H * T);
     a.hbest  = ws.icpPartial.as<unsigned long long>((size_t)2 * H * T);
-    int *ib  = ws.icpState.as<int>((size_t)H * nSrc * 3);
+    int *ib  = ws.icpState.as<int>((size_t)H * nSrc * 4);
     a.nnSlot = ib;
     a.hslot  = ib + (size_t)H * nSrc;
     a.nnD    = reinterpret_cast<float *>(ib + 2 * (size_t)H * nSrc);
+    a.wslot  = ib + 3 * (size_t)H * nSrc;
     a.init   = nullptr;
     a.out    = nullptr;
     return a;
@@ -1138,7 +1150,7 @@ static int norm_block(int b) {
 
 // the refiner's searches read the grids' {x, y, z, id} copies
 static void require_copies(const GridView &g) {
-    if (g.n > 0 && !g.p4)
+    if (g.n > 0 && (!g.p4 || (g.nx && !g.n4)))
         throw std::logic_error("refinement grid built without its point copies");
 }
 
