@@ -206,3 +206,70 @@ print("torch binding ok")
     out = subprocess.run([sys.executable, str(script)], capture_output=True, text=True, timeout=600)
     assert out.returncode == 0, out.stdout[-3000:] + out.stderr[-3000:]
     print(out.stdout)
+
+
+def test_degenerate_surfaces_match_fp64_oracle(oracle_mod):
+    """A flat patch (rotation about its normal and in-plane translation left
+    free: rank 3 normal equations) and a ridge of two planes (translation
+    along the ridge free: rank 5), point-to-plane: the
+    refiner's solve must take the pseudo-inverse there (JacobiSVD's
+    threshold drops the free directions) and agree with the FP64 oracle,
+    as on well-conditioned surfaces where it solves by Cholesky."""
+    from surface_match_amd import synth
+    from surface_match_amd.icp import IcpTarget
+    O = oracle_mod
+    r = rng(80)
+
+    def grid_pts(n, extent):
+        u = r.uniform(-extent, extent, size=(n, 2))
+        return u
+
+    cases = []
+    # flat patch: source 1500 points on z = 0, target a wider patch
+    u = grid_pts(1500, 20.0)
+    src = np.column_stack([u, np.zeros(len(u))]).astype(np.float32)
+    srcn = np.tile(np.float32([0, 0, 1]), (len(src), 1))
+    v = grid_pts(40000, 40.0)
+    dst = np.column_stack([v, r.normal(scale=0.02, size=len(v))]).astype(np.float32)
+    dstn = np.tile(np.float32([0, 0, 1]), (len(dst), 1))
+    cases.append(("plane", src, srcn, dst, dstn))
+    # ridge: the planes z = 0 (x < 0) and x = 0 (z > 0)
+    def ridge(n, extent, noise):
+        a = r.uniform(0, extent, size=n)
+        b = r.uniform(-extent, extent, size=n)
+        half = r.random(n) < 0.5
+        p = np.where(half[:, None], np.column_stack([-a, b, np.zeros(n)]),
+                     np.column_stack([np.zeros(n), b, a]))
+        nn = np.where(half[:, None], np.float32([0, 0, 1]), np.float32([-1, 0, 0]))
+        return (p + r.normal(scale=noise, size=p.shape)).astype(np.float32), nn.astype(np.float32)
+    src, srcn = ridge(1500, 20.0, 0.0)
+    dst, dstn = ridge(40000, 40.0, 0.02)
+    cases.append(("ridge", src, srcn, dst, dstn))
+    kw = dict(iterations=10, reject_dist=5.0, mse_min=0.0, mse_max=10.0, tolerance=0.0,
+              scoring_dist=0.5)
+    for name, src, srcn, dst, dstn in cases:
+        inits = []
+        for h in range(8):
+            T = _perturb(np.eye(4), r, deg=1.0, mm=0.5)
+            inits.append(T)
+        inits = np.array(inits)
+        tgt = IcpTarget(dst, dstn, 1.5)
+        g = O.OracleGrid(dst, 1.5)
+        # point-to-plane against exact target normals: the free directions
+        # give exactly zero rows of J^T J (the symmetric objective adds the
+        # moved source normals, which leaves them weakly constrained instead)
+        for mode in (1,):
+            poses, res, _ = tgt.refine(src, srcn, inits, mode=mode, batched=True, **kw)
+            plain, pres, _ = tgt.refine(src, srcn, inits, mode=mode, batched=False, **kw)
+            for h in range(len(inits)):
+                crit = O.Criteria(10, 5.0, 0.0, 10.0, 0.0)
+                ores = O.IcpResult()
+                O.lib().orc_icp_regist(len(src), src, srcn, C.byref(g._g), dstn,
+                                       np.ascontiguousarray(inits[h][:3].ravel(), np.float32),
+                                       C.byref(crit), mode, C.byref(ores))
+                OP = np.array(ores.pose, np.float64).reshape(3, 4)
+                for P in (poses[h], plain[h]):
+                    ang = synth.rotation_angle_deg(OP[:, :3].T @ P[:3, :3])
+                    dt = np.linalg.norm(OP[:, 3] - P[:3, 3])
+                    assert ang < 0.1 and dt < 0.05, (name, mode, h, ang, dt)
+        print(f"{name}: {len(inits)} hypotheses equal to the FP64 oracle")
