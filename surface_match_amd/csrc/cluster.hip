@@ -390,6 +390,9 @@ __global__ void __launch_bounds__(1024) nms_kernel(const Hypo *h, const int *ord
 // near.  Used while the kept list fits in LDS.
 constexpr int NMS_SEQ_MAX = 4096;
 
+// NT threads: all of them load the poses and the votes (their global loads
+// overlap), the first wave walks, all of them add up the kept poses' votes
+template <int NT>
 __device__ void nms_seq_range(const Hypo *h, const int *ord, int C, const float *center,
                               float thr, int *status, Hypo *out, int *nOut, float *sKept) {
     // kept: {x, y, z, pose index bits}[C]; then every pose's centre x/y/z[C]
@@ -398,7 +401,8 @@ __device__ void nms_seq_range(const Hypo *h, const int *ord, int C, const float 
     const int   lane = threadIdx.x;
     const float sq   = thr * thr;
     const float c0 = center[0], c1 = center[1], c2 = center[2];
-    for (int j = lane; j < C; j += 64) {
+    __shared__ int sNk;
+    for (int j = lane; j < C; j += NT) {
         const float *P = h[ord[j]].pose;
         px[j]          = P[0] * c0 + P[1] * c1 + P[2] * c2 + P[3];
         py[j]          = P[4] * c0 + P[5] * c1 + P[6] * c2 + P[7];
@@ -407,7 +411,7 @@ __device__ void nms_seq_range(const Hypo *h, const int *ord, int C, const float 
     __syncthreads();
     int  nk    = 0;  // wave-uniform
     int *sStat = reinterpret_cast<int *>(px);  // px[j] is dead once its block is walked
-    for (int b0 = 0; b0 < C; b0 += 64) {
+    for (int b0 = 0; b0 < (lane < 64 ? C : 0); b0 += 64) {
         const int  j     = b0 + lane;
         const bool valid = j < C;
         float      tx = 0.0f, ty = 0.0f, tz = 0.0f;
@@ -483,15 +487,18 @@ __device__ void nms_seq_range(const Hypo *h, const int *ord, int C, const float 
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
+    if (lane == 0)
+        sNk = nk;
     __syncthreads();
+    nk = sNk;
     // every pose's votes next to its status in LDS (py is dead too)
     float *sVotes = py;
-    for (int j = lane; j < C; j += 64)
+    for (int j = lane; j < C; j += NT)
         sVotes[j] = h[ord[j]].votes;
     __syncthreads();
     // kept poses in order, each with its members' votes added in ascending
     // member order (the reference's accumulation order)
-    for (int t = lane; t < nk; t += 64) {
+    for (int t = lane; t < nk; t += NT) {
         const int i = __float_as_int(kp[t].w);
         Hypo      o = h[ord[i]];
         int       j = i + 1;
@@ -516,24 +523,27 @@ __device__ void nms_seq_range(const Hypo *h, const int *ord, int C, const float 
         *nOut = nk;
 }
 
-__global__ void __launch_bounds__(64) nms_seq_kernel(const Hypo *h, const int *ord, int C,
-                                                     const float *center, float thr, int *status,
-                                                     Hypo *out, int *nOut) {
+constexpr int NMS_THREADS = 256;
+
+__global__ void __launch_bounds__(NMS_THREADS) nms_seq_kernel(const Hypo *h, const int *ord, int C,
+                                                              const float *center, float thr,
+                                                              int *status, Hypo *out, int *nOut) {
     extern __shared__ float sKept[];
-    nms_seq_range(h, ord, C, center, thr, status, out, nOut, sKept);
+    nms_seq_range<NMS_THREADS>(h, ord, C, center, thr, status, out, nOut, sKept);
 }
 
 // one wave per scene: scene w's clusters are the sorted positions
 // [off[w], off[w + 1]) of ord; its kept poses go to out + off[w], their
 // count to nOut[w]
-__global__ void __launch_bounds__(64) nms_seq_multi_kernel(const Hypo *h, const int *ord,
-                                                           const int *off, const float *center,
-                                                           float thr, int *status, Hypo *out,
-                                                           int *nOut) {
+__global__ void __launch_bounds__(NMS_THREADS) nms_seq_multi_kernel(const Hypo *h, const int *ord,
+                                                                    const int *off,
+                                                                    const float *center, float thr,
+                                                                    int *status, Hypo *out,
+                                                                    int *nOut) {
     extern __shared__ float sKept[];
     const int b = off[blockIdx.x];
-    nms_seq_range(h, ord + b, off[blockIdx.x + 1] - b, center, thr, status + b, out + b,
-                  nOut + blockIdx.x, sKept);
+    nms_seq_range<NMS_THREADS>(h, ord + b, off[blockIdx.x + 1] - b, center, thr, status + b,
+                               out + b, nOut + blockIdx.x, sKept);
 }
 
 __global__ void hypo_keys_kernel(const Hypo *h, int n, uint32_t *keys, int *vals) {
@@ -642,7 +652,8 @@ void cluster_poses(Workspace &ws, const PoseRec *poses, int np, float dThr, floa
         const size_t lds = (size_t)28 * std::max(nc, 1);
         PPF_HIP_CHECK(hipFuncSetAttribute((const void *)nms_seq_kernel,
                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        nms_seq_kernel<<<1, 64, lds, s>>>(centers, hv2, nc, cdev, nmsThr, nmsStatus, kept, nOut);
+        nms_seq_kernel<<<1, NMS_THREADS, lds, s>>>(centers, hv2, nc, cdev, nmsThr, nmsStatus, kept,
+                                                   nOut);
     } else {
         nms_kernel<<<1, 1024, 0, s>>>(centers, hv2, nc, cdev, nmsThr, nmsStatus, tr, kept, nOut);
     }
@@ -817,7 +828,7 @@ void cluster_poses_multi(Workspace &ws, const PoseRec *poses, const std::vector<
         const size_t lds = (size_t)28 * std::max(maxC, 1);
         PPF_HIP_CHECK(hipFuncSetAttribute((const void *)nms_seq_multi_kernel,
                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        nms_seq_multi_kernel<<<S, 64, lds, s>>>(centers, hv2, cOff, cdev, nmsThr, nmsStatus, kept,
+        nms_seq_multi_kernel<<<S, NMS_THREADS, lds, s>>>(centers, hv2, cOff, cdev, nmsThr, nmsStatus, kept,
                                                 nOut);
     } else {
         float *tr = buf;  // reuse
