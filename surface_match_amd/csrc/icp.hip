@@ -118,68 +118,86 @@ __device__ __forceinline__ IcpArgs args_for(const IcpArgs &a, int h) {
 // rounding): lambda_min >= 1 / ||A^-1||_F >= 1 / ||L^-1||_F^2 and
 // lambda_max <= ||A||_inf.  Returns false (u untouched) otherwise.  A
 // few hundred dependent flops instead of the Jacobi sweeps' ~10^4.
-__device__ bool chol_solve6(const double A[36], const double rhs[6], double u[6]) {
-    double L[36], M[36];
+__device__ __forceinline__ bool chol_solve6(const double A[36], const double rhs[6], double u[6]) {
+    // every loop is unrolled, so L (its lower triangle, packed by rows) and
+    // the columns of L^-1 stay in registers: the last workgroup's single
+    // lane runs this, and arrays in scratch memory would cost it a memory
+    // round trip per access
+    double L[21];
+#define PPF_L(i, j) L[(i) * ((i) + 1) / 2 + (j)]
     double amax = 0.0;
+#pragma unroll
     for (int i = 0; i < 6; i++) {
         double rs = 0.0;
+#pragma unroll
         for (int j = 0; j < 6; j++)
             rs += fabs(A[i * 6 + j]);
         amax = fmax(amax, rs);
     }
+#pragma unroll
     for (int j = 0; j < 6; j++) {
         double d = A[j * 6 + j];
+#pragma unroll
         for (int k = 0; k < j; k++)
-            d -= L[j * 6 + k] * L[j * 6 + k];
+            d -= PPF_L(j, k) * PPF_L(j, k);
         if (!(d > 0.0))
             return false;
         const double ljj = sqrt(d);
-        L[j * 6 + j]     = ljj;
+        PPF_L(j, j)      = ljj;
+#pragma unroll
         for (int i = j + 1; i < 6; i++) {
             double s = A[i * 6 + j];
+#pragma unroll
             for (int k = 0; k < j; k++)
-                s -= L[i * 6 + k] * L[j * 6 + k];
-            L[i * 6 + j] = s / ljj;
+                s -= PPF_L(i, k) * PPF_L(j, k);
+            PPF_L(i, j) = s / ljj;
         }
     }
-    // M = L^-1 (lower triangular) and its squared Frobenius norm
+    // squared Frobenius norm of L^-1, column j = L^-1 e_j by forward
+    // substitution
     double fro = 0.0;
+#pragma unroll
     for (int j = 0; j < 6; j++) {
-        M[j * 6 + j] = 1.0 / L[j * 6 + j];
-        fro += M[j * 6 + j] * M[j * 6 + j];
-        for (int i = j + 1; i < 6; i++) {
-            double s = 0.0;
+        double x[6];
+#pragma unroll
+        for (int i = j; i < 6; i++) {
+            double s = i == j ? 1.0 : 0.0;
+#pragma unroll
             for (int k = j; k < i; k++)
-                s -= L[i * 6 + k] * M[k * 6 + j];
-            M[i * 6 + j] = s / L[i * 6 + i];
-            fro += M[i * 6 + j] * M[i * 6 + j];
+                s -= PPF_L(i, k) * x[k];
+            x[i] = i == j ? 1.0 / PPF_L(i, i) : s / PPF_L(i, i);
+            fro += x[i] * x[i];
         }
     }
     // a margin of 1e-3 over the threshold absorbs the rounding of the bounds
     if (!(1.0 / fro > 6.0 * 1.1920928955078125e-07 * amax * 1.001))
         return false;
     double y[6];
+#pragma unroll
     for (int i = 0; i < 6; i++) {
         double s = rhs[i];
+#pragma unroll
         for (int k = 0; k < i; k++)
-            s -= L[i * 6 + k] * y[k];
-        y[i] = s / L[i * 6 + i];
+            s -= PPF_L(i, k) * y[k];
+        y[i] = s / PPF_L(i, i);
     }
+#pragma unroll
     for (int i = 5; i >= 0; i--) {
         double s = y[i];
+#pragma unroll
         for (int k = i + 1; k < 6; k++)
-            s -= L[k * 6 + i] * u[k];
-        u[i] = s / L[i * 6 + i];
+            s -= PPF_L(k, i) * u[k];
+        u[i] = s / PPF_L(i, i);
     }
+#undef PPF_L
     return true;
 }
 
 // u = pinv(A) rhs for symmetric PSD A (FP64 Jacobi), dropping eigenvalues
-// below 6 * eps_f32 * max |eigenvalue| as JacobiSVD::solve does; the
-// Cholesky path above when the threshold provably drops none of them.
-__device__ void sym_solve6(const double A[36], const double rhs[6], double u[6]) {
-    if (chol_solve6(A, rhs, u))
-        return;
+// below 6 * eps_f32 * max |eigenvalue| as JacobiSVD::solve does.  Not
+// inlined: its arrays live in scratch memory, which the Cholesky path of
+// sym_solve6 then never touches.
+__device__ __noinline__ void pinv_solve6(const double *A, const double *rhs, double *u) {
     double a[36], V[36], W[6];
     for (int i = 0; i < 36; i++)
         a[i] = A[i];
@@ -200,6 +218,13 @@ __device__ void sym_solve6(const double A[36], const double rhs[6], double u[6])
         for (int i = 0; i < 6; i++)
             u[i] += V[i * 6 + k] * proj;
     }
+}
+
+// u = pinv(A) rhs: the Cholesky path when the threshold provably drops no
+// eigenvalue, else the Jacobi pseudo-inverse
+__device__ __forceinline__ void sym_solve6(const double A[36], const double rhs[6], double u[6]) {
+    if (!chol_solve6(A, rhs, u))
+        pinv_solve6(A, rhs, u);
 }
 
 __device__ void compose3x4(const double *A, const double *B, double *C) {
@@ -485,15 +510,18 @@ __device__ __forceinline__ void clear_point(const HypoScratch &x, int i) {
 
 // FP64 solve + update (icp.cpp:67-86 for the symmetric objective; Rodrigues
 // about the source centroid for point-to-plane)
-__device__ void solve_update(const double *red, const float *means, int mode, IcpState &st) {
+__device__ __forceinline__ void solve_update(const double *red, const float *means, int mode,
+                                             IcpState &st) {
     double A[36], rhs[6], u[6];
-    int    k = 0;
+#pragma unroll
     for (int p = 0; p < 6; p++)
+#pragma unroll
         for (int q = p; q < 6; q++) {
+            const int k  = p * 6 - p * (p - 1) / 2 + (q - p);  // row-major upper triangle
             A[p * 6 + q] = red[k];
             A[q * 6 + p] = red[k];
-            k++;
         }
+#pragma unroll
     for (int p = 0; p < 6; p++)
         rhs[p] = -red[21 + p];
     sym_solve6(A, rhs, u);
