@@ -252,13 +252,15 @@ struct MatchCtx {
     };
     std::vector<std::unique_ptr<SceneWork>> sceneWork;
     DevBuf batchKpScene, batchKpId, batchGrids, batchHypScene;
+    // grids of every scene of a batched match, built together
+    MultiGrid batchVote, batchSparse, batchFull, batchDense;
     // the per-scene stages of a batched match run on up to kLanes host
     // threads, each with its own stream (default priority) and workspace,
     // so the small kernels and read-backs of different scenes overlap
     static constexpr int kLanes = 4;
     struct Lane {
         hipStream_t stream = nullptr;
-        hipEvent_t  ev[2]  = {nullptr, nullptr};  // brackets timed lane work
+        hipEvent_t  ev[3]  = {nullptr, nullptr, nullptr};  // timed lane work; end of a pass
         Workspace   ws;
         ~Lane() {
             for (auto e : ev)
@@ -857,17 +859,23 @@ void for_scenes(MatchCtx &X, int n, F &&f) {
 // rethrows a lane's error; leaving the scope without open() skips g.
 struct LanePasses {
     using Fn = std::function<void(int, Workspace &, hipStream_t)>;
+    using AfterFn = std::function<void(hipStream_t)>;
     MatchCtx                       &X;
     int                             n;
     Fn                              f, g;
+    AfterFn                         afterF;  // once every lane's f work is done (may be empty)
     std::atomic<int>                next1{0}, next2{0}, gate{0};  // gate: 0 wait, 1 go, -1 skip
+    std::atomic<int>                doneF{0};
+    int                             nLanes = 0;
     std::vector<std::thread>        th;
     std::vector<std::exception_ptr> err;
 
-    LanePasses(MatchCtx &X_, int n_, Fn f_, Fn g_) : X(X_), n(n_), f(std::move(f_)), g(std::move(g_)) {
+    LanePasses(MatchCtx &X_, int n_, Fn f_, Fn g_, AfterFn a_ = {})
+        : X(X_), n(n_), f(std::move(f_)), g(std::move(g_)), afterF(std::move(a_)) {
         PPF_HIP_CHECK(hipStreamSynchronize(X.stream));
         const int L = std::max(1, std::min(n, MatchCtx::kLanes));
         lane_of(X, L - 1);
+        nLanes = L;
         int dev = 0;
         PPF_HIP_CHECK(hipGetDevice(&dev));
         err.resize(L);
@@ -892,6 +900,16 @@ struct LanePasses {
                     MatchCtx::Lane &ln = *X.lanes[l];
                     for (int i = next1++; i < n; i = next1++)
                         f(i, ln.ws, ln.stream);
+                    if (afterF) {
+                        // the last lane to finish queuing its f work waits
+                        // on every lane's stream, then queues afterF on its own
+                        PPF_HIP_CHECK(hipEventRecord(ln.ev[2], ln.stream));
+                        if (doneF.fetch_add(1, std::memory_order_acq_rel) + 1 == nLanes) {
+                            for (int o = 0; o < nLanes; o++)
+                                PPF_HIP_CHECK(hipStreamWaitEvent(ln.stream, X.lanes[o]->ev[2], 0));
+                            afterF(ln.stream);
+                        }
+                    }
                     int q;
                     while ((q = gate.load(std::memory_order_acquire)) == 0)
                         std::this_thread::yield();
@@ -973,30 +991,63 @@ void match_batch_impl(ppf_detector_s &D, MatchCtx &X, const std::vector<DevCloud
     const float keySampleStep = std::sqrt(1.0f / kpFrac) * sampleStep;
 
     PPF_HIP_CHECK(hipEventRecord(X.ev[0], s));
-    // ---- per scene: sampling, normals, key points, vote grid
+    // ---- every scene's sampled points in one launch sequence (ppf.cpp:199-203)
+    std::vector<VoxSceneIn> vin(n);
+    std::vector<int>        cnts;
+    for (int i = 0; i < n; i++) {
+        DevCloud &c  = *clouds[i];
+        auto     &sw = *X.sceneWork[i];
+        c.reset(s);
+        vin[i] = VoxSceneIn{c.px.get<float>(), c.py.get<float>(), c.pz.get<float>(),
+                            c.valid.get<int>(), c.nValid, {c.boxMin[0], c.boxMin[1], c.boxMin[2]},
+                            sw.samp.as<int>(std::max(c.nValid, 1))};
+    }
+    voxel_sample_multi(X.ws, vin, sampleStep, cnts, s);
+    for (int i = 0; i < n; i++)
+        X.sceneWork[i]->ns = cnts[i];
+    // ---- per scene on the lanes: normals (ppf.cpp:204-211)
     for_scenes(X, n, [&](int i, Workspace &ws, hipStream_t ls) {
         DevCloud &c  = *clouds[i];
         auto     &sw = *X.sceneWork[i];
-        int      *cn = ws.smallCnt.as<int>(8);
-        c.reset(ls);
-        float *PX = c.px.get<float>(), *PY = c.py.get<float>(), *PZ = c.pz.get<float>();
-        float *NX = c.nx.get<float>(), *NY = c.ny.get<float>(), *NZ = c.nz.get<float>();
-        int   *samp = sw.samp.as<int>(std::max(c.nValid, 1));
-        sw.ns = voxel_sample(ws, PX, PY, PZ, c.valid.get<int>(), c.nValid, sampleStep, c.boxMin,
-                             samp, cn, ls);
         if (!c.hasNormal)
-            estimate_normals(ws, c.n, PX, PY, PZ, NX, NY, NZ, c.valid.get<int>(), c.nValid, samp,
-                             sw.ns, p.knnNormal, p.smoothNormal, p.invertNormal, c.vp, normCell, ls,
-                             c.ext);
-        int *kp = sw.kp.as<int>(std::max(sw.ns, 1));
-        sw.nk   = voxel_sample(ws, PX, PY, PZ, samp, sw.ns, keySampleStep, c.boxMin, kp, cn, ls);
-        grid_build(sw.gVote, PX, PY, PZ, NX, NY, NZ, samp, sw.ns, Dm / (float)D.vcfg.cellDiv, ls,
-                   c.ext, true);
+            estimate_normals(ws, c.n, c.px.get<float>(), c.py.get<float>(), c.pz.get<float>(),
+                             c.nx.get<float>(), c.ny.get<float>(), c.nz.get<float>(),
+                             c.valid.get<int>(), c.nValid, sw.samp.get<int>(), sw.ns, p.knnNormal,
+                             p.smoothNormal, p.invertNormal, c.vp, normCell, ls, c.ext);
     });
+    // ---- every scene's key points and vote grid together (ppf.cpp:212-216)
+    std::vector<GridSceneIn> gin(n);
+    for (int i = 0; i < n; i++) {
+        DevCloud &c  = *clouds[i];
+        auto     &sw = *X.sceneWork[i];
+        vin[i]       = VoxSceneIn{c.px.get<float>(), c.py.get<float>(), c.pz.get<float>(),
+                            sw.samp.get<int>(), sw.ns, {c.boxMin[0], c.boxMin[1], c.boxMin[2]},
+                            sw.kp.as<int>(std::max(sw.ns, 1))};
+        gin[i]       = GridSceneIn{c.px.get<float>(), c.py.get<float>(), c.pz.get<float>(),
+                             c.nx.get<float>(), c.ny.get<float>(), c.nz.get<float>(),
+                             sw.samp.get<int>(), sw.ns, {}};
+        std::memcpy(gin[i].box, c.ext, sizeof c.ext);
+    }
+    voxel_sample_multi(X.ws, vin, keySampleStep, cnts, s);
+    for (int i = 0; i < n; i++)
+        X.sceneWork[i]->nk = cnts[i];
+    std::vector<GridView> voteViews(n);
+    if (grid_build_multi(X.batchVote, gin, Dm / (float)D.vcfg.cellDiv, true, s)) {
+        voteViews = X.batchVote.views;
+    } else {
+        for (int i = 0; i < n; i++) {
+            DevCloud &c  = *clouds[i];
+            auto     &sw = *X.sceneWork[i];
+            grid_build(sw.gVote, c.px.get<float>(), c.py.get<float>(), c.pz.get<float>(),
+                       c.nx.get<float>(), c.ny.get<float>(), c.nz.get<float>(), sw.samp.get<int>(),
+                       sw.ns, Dm / (float)D.vcfg.cellDiv, s, c.ext, true);
+            voteViews[i] = sw.gVote.view;
+        }
+    }
     std::vector<int> ko(n + 1, 0);
     float            minCell = FLT_MAX;
     for (int i = 0; i < n; i++) {
-        minCell   = std::min(minCell, X.sceneWork[i]->gVote.view.cell);
+        minCell   = std::min(minCell, voteViews[i].cell);
         ko[i + 1] = ko[i] + X.sceneWork[i]->nk;
     }
     // every scene's key points gathered before the lanes start (a point
@@ -1022,7 +1073,7 @@ void match_batch_impl(ppf_detector_s &D, MatchCtx &X, const std::vector<DevCloud
         for (int i = 0; i < n; i++) {
             DevCloud &c  = *clouds[i];
             auto     &sw = *X.sceneWork[i];
-            views[i]     = sw.gVote.view;
+            views[i]     = voteViews[i];
             if (sw.nk == 0)
                 continue;
             const int o = ko[i];
@@ -1052,40 +1103,88 @@ void match_batch_impl(ppf_detector_s &D, MatchCtx &X, const std::vector<DevCloud
     std::vector<std::vector<Hypo>> hyp(n);
     std::vector<int>               ncs(n, 0);
     std::vector<int>               pOff(n + 1, 0);
+    // the dense samples of every scene, before the lanes estimate their
+    // normals (the positions of the sampled points are final by now)
+    for (int i = 0; i < n; i++)
+        X.sceneWork[i]->nd = 0;
+    if (p.densePoseRefinement) {
+        for (int i = 0; i < n; i++) {
+            DevCloud &c = *clouds[i];
+            vin[i]      = VoxSceneIn{c.px.get<float>(), c.py.get<float>(), c.pz.get<float>(),
+                                c.valid.get<int>(), c.nValid,
+                                {c.boxMin[0], c.boxMin[1], c.boxMin[2]},
+                                X.sceneWork[i]->dense.as<int>(std::max(c.nValid, 1))};
+        }
+        voxel_sample_multi(X.ws, vin, reSampleStep, cnts, s);
+        for (int i = 0; i < n; i++)
+            X.sceneWork[i]->nd = cnts[i];
+    }
+    // the refinement grids of every scene over the final positions, queued
+    // once every lane's normals are: together when they fit one build,
+    // else scene by scene
+    std::vector<GridView> sparseViews(n), fullViews(n), denseViews(n);
+    auto refineGrids = [&](hipStream_t ls) {
+        std::vector<GridSceneIn> gs(n), gf(n), gd(n);
+        for (int i = 0; i < n; i++) {
+            DevCloud &c  = *clouds[i];
+            auto     &sw = *X.sceneWork[i];
+            float *PX = c.px.get<float>(), *PY = c.py.get<float>(), *PZ = c.pz.get<float>();
+            float *NX = c.nx.get<float>(), *NY = c.ny.get<float>(), *NZ = c.nz.get<float>();
+            gs[i] = GridSceneIn{PX, PY, PZ, NX, NY, NZ, sw.samp.get<int>(), sw.ns, {}};
+            gf[i] = GridSceneIn{PX, PY, PZ, nullptr, nullptr, nullptr, c.valid.get<int>(), c.nValid, {}};
+            gd[i] = GridSceneIn{PX, PY, PZ, NX, NY, NZ, sw.dense.get<int>(), sw.nd, {}};
+            std::memcpy(gs[i].box, c.ext, sizeof c.ext);
+            std::memcpy(gf[i].box, c.ext, sizeof c.ext);
+            std::memcpy(gd[i].box, c.ext, sizeof c.ext);
+        }
+        const float fullCell = std::max(scoringDist * 2.0f, sampleStep * 0.25f);
+        if (grid_build_multi(X.batchSparse, gs, sampleStep * kSparseCell, true, ls))
+            sparseViews = X.batchSparse.views;
+        else
+            for (int i = 0; i < n; i++) {
+                grid_build(X.sceneWork[i]->gSparse, gs[i].X, gs[i].Y, gs[i].Z, gs[i].NX, gs[i].NY,
+                           gs[i].NZ, gs[i].subset, gs[i].m, sampleStep * kSparseCell, ls,
+                           clouds[i]->ext, true);
+                sparseViews[i] = X.sceneWork[i]->gSparse.view;
+            }
+        if (grid_build_multi(X.batchFull, gf, fullCell, true, ls))
+            fullViews = X.batchFull.views;
+        else
+            for (int i = 0; i < n; i++) {
+                grid_build(X.sceneWork[i]->gFull, gf[i].X, gf[i].Y, gf[i].Z, nullptr, nullptr,
+                           nullptr, gf[i].subset, gf[i].m, fullCell, ls, clouds[i]->ext, true);
+                fullViews[i] = X.sceneWork[i]->gFull.view;
+            }
+        if (!p.densePoseRefinement)
+            return;
+        if (grid_build_multi(X.batchDense, gd, reSampleStep * kDenseCell, true, ls))
+            denseViews = X.batchDense.views;
+        else
+            for (int i = 0; i < n; i++) {
+                grid_build(X.sceneWork[i]->gDense, gd[i].X, gd[i].Y, gd[i].Z, gd[i].NX, gd[i].NY,
+                           gd[i].NZ, gd[i].subset, gd[i].m, reSampleStep * kDenseCell, ls,
+                           clouds[i]->ext, true);
+                denseViews[i] = X.sceneWork[i]->gDense.view;
+            }
+    };
     LanePasses lanes(
         X, n,
         [&](int i, Workspace &ws, hipStream_t ls) {
             DevCloud &c  = *clouds[i];
             auto     &sw = *X.sceneWork[i];
-            float *PX = c.px.get<float>(), *PY = c.py.get<float>(), *PZ = c.pz.get<float>();
-            float *NX = c.nx.get<float>(), *NY = c.ny.get<float>(), *NZ = c.nz.get<float>();
-            sw.nd = 0;
-            if (p.densePoseRefinement) {
-                int *dense = sw.dense.as<int>(std::max(c.nValid, 1));
-                sw.nd = voxel_sample(ws, PX, PY, PZ, c.valid.get<int>(), c.nValid, reSampleStep,
-                                     c.boxMin, dense, ws.smallCnt.as<int>(8), ls);
-                if (!c.hasNormal)
-                    estimate_normals(ws, c.n, PX, PY, PZ, NX, NY, NZ, c.valid.get<int>(),
-                                     c.nValid, dense, sw.nd, p.knnNormal, 0, p.invertNormal, c.vp,
-                                     denseNormCell, ls, c.ext);
-            }
-            if (laneGrids) {
-                grid_build(sw.gSparse, PX, PY, PZ, NX, NY, NZ, sw.samp.get<int>(), sw.ns,
-                           sampleStep * kSparseCell, ls, c.ext, true);
-                grid_build(sw.gFull, PX, PY, PZ, nullptr, nullptr, nullptr, c.valid.get<int>(),
-                           c.nValid, std::max(scoringDist * 2.0f, sampleStep * 0.25f), ls, c.ext,
-                           true);
-                if (p.densePoseRefinement)
-                    grid_build(sw.gDense, PX, PY, PZ, NX, NY, NZ, sw.dense.get<int>(), sw.nd,
-                               reSampleStep * kDenseCell, ls, c.ext, true);
-            }
+            if (p.densePoseRefinement && !c.hasNormal)
+                estimate_normals(ws, c.n, c.px.get<float>(), c.py.get<float>(), c.pz.get<float>(),
+                                 c.nx.get<float>(), c.ny.get<float>(), c.nz.get<float>(),
+                                 c.valid.get<int>(), c.nValid, sw.dense.get<int>(), sw.nd,
+                                 p.knnNormal, 0, p.invertNormal, c.vp, denseNormCell, ls, c.ext);
         },
         [&](int i, Workspace &ws, hipStream_t ls) {
             // scenes with long pose lists: clustered one by one on the lanes
             // (the vote's pose buffer may have grown during the vote)
             cluster_poses(ws, X.ws.poses.get<PoseRec>() + pOff[i], pOff[i + 1] - pOff[i],
                           0.1f * Dm, angleStep, center, maxOverlapDist, hyp[i], &ncs[i], ls);
-        });
+        },
+        laneGrids ? LanePasses::AfterFn(refineGrids) : LanePasses::AfterFn());
     // ---- every scene's key points in one vote
     if (K > 0) {
         VoteKeypoints in;
@@ -1150,16 +1249,10 @@ void match_batch_impl(ppf_detector_s &D, MatchCtx &X, const std::vector<DevCloud
         sceneHasCand[c.scene] = 1;
     int nSparse = 0, nDenseIcp = 0, nDenseTot = 0;
     if (p.sparsePoseRefinement && !cand.empty()) {
-        std::vector<GridView> vs1(n), vf(n), vd(n);
-        for (int i = 0; i < n; i++) {
-            vs1[i] = X.sceneWork[i]->gSparse.view;
-            vf[i]  = X.sceneWork[i]->gFull.view;
-            if (p.densePoseRefinement) {
-                vd[i] = X.sceneWork[i]->gDense.view;
-                if (sceneHasCand[i])
-                    nDenseTot += X.sceneWork[i]->nd;
-            }
-        }
+        std::vector<GridView> &vs1 = sparseViews, &vf = fullViews, &vd = denseViews;
+        for (int i = 0; i < n; i++)
+            if (p.densePoseRefinement && sceneHasCand[i])
+                nDenseTot += X.sceneWork[i]->nd;
         GridView *gS = X.batchGrids.as<GridView>((size_t)3 * n);
         PPF_HIP_CHECK(hipMemcpyAsync(gS, vs1.data(), sizeof(GridView) * n, hipMemcpyHostToDevice, s));
         PPF_HIP_CHECK(hipMemcpyAsync(gS + n, vf.data(), sizeof(GridView) * n, hipMemcpyHostToDevice,

@@ -534,6 +534,362 @@ int voxel_sample(Workspace &ws, const float *X, const float *Y, const float *Z,
     return cnt;
 }
 
+// ---- several scenes at once.  Scene w owns the flat elements
+// [off[w], off[w + 1]) of a launch; its voxel table is its own slice of the
+// shared table (the size voxel_sample would give it), and its grid its own
+// cells, counters and bitmap words, so each scene's result is the
+// single-scene one.
+namespace {
+// the scene owning flat element t: the last w with off[w] <= t (empty
+// scenes share their offset with the next one and are never chosen)
+__device__ __forceinline__ int owner_of(const int *off, int S, int t) {
+    int lo = 0, hi = S - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (off[mid] <= t)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+struct VoxSceneDev {
+    const float *X, *Y, *Z;
+    const int   *subset;
+    int         *out;
+    float        bmin[3];
+    uint32_t     tabOff, hmask;
+};
+
+__global__ void vsm_insert_kernel(const VoxSceneDev *sc, const int *off, int S, int M, float step,
+                                  unsigned long long *tkey, unsigned long long *tbest) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= M)
+        return;
+    const int          w = owner_of(off, S, t);
+    const VoxSceneDev &d = sc[w];
+    const int          j = t - off[w];
+    const int          i = d.subset ? d.subset[j] : j;
+    float              dd;
+    unsigned long long key = voxel_key(d.X[i], d.Y[i], d.Z[i], d.bmin, step, dd) + 1ull;
+    unsigned long long pk  = ((unsigned long long)__float_as_uint(dd) << 32) | (uint32_t)i;
+    uint32_t           h   = vox_hash(key) & d.hmask;
+    for (;;) {
+        unsigned long long prev = atomicCAS(&tkey[d.tabOff + h], 0ull, key);
+        if (prev == 0ull || prev == key) {
+            atomicMin(&tbest[d.tabOff + h], pk);
+            return;
+        }
+        h = (h + 1u) & d.hmask;
+    }
+}
+
+__global__ void vsm_flag_kernel(const VoxSceneDev *sc, const int *off, int S, int M, float step,
+                                const unsigned long long *tkey, const unsigned long long *tbest,
+                                char *flags) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= M)
+        return;
+    const int          w = owner_of(off, S, t);
+    const VoxSceneDev &d = sc[w];
+    const int          j = t - off[w];
+    const int          i = d.subset ? d.subset[j] : j;
+    float              dd;
+    unsigned long long key = voxel_key(d.X[i], d.Y[i], d.Z[i], d.bmin, step, dd) + 1ull;
+    unsigned long long pk  = ((unsigned long long)__float_as_uint(dd) << 32) | (uint32_t)i;
+    uint32_t           h   = vox_hash(key) & d.hmask;
+    for (;;) {
+        if (tkey[d.tabOff + h] == key) {
+            flags[t] = (tbest[d.tabOff + h] == pk) ? 1 : 0;
+            return;
+        }
+        h = (h + 1u) & d.hmask;
+    }
+}
+
+// base[w] = the first selected position whose flat element is scene w's
+// (selected flat elements ascend); base[S] = the selected count
+__global__ void vsm_bases_kernel(const int *sel, const int *nSel, const int *off, int S,
+                                 int *base) {
+    const int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w > S)
+        return;
+    const int n = *nSel;
+    if (w == S) {
+        base[S] = n;
+        return;
+    }
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (sel[mid] < off[w])
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    base[w] = lo;
+}
+
+__global__ void vsm_scatter_kernel(const VoxSceneDev *sc, const int *off, int S, const int *sel,
+                                   const int *nSel, const int *base) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= *nSel)
+        return;
+    const int          t = sel[p];
+    const int          w = owner_of(off, S, t);
+    const VoxSceneDev &d = sc[w];
+    const int          j = t - off[w];
+    d.out[p - base[w]]   = d.subset ? d.subset[j] : j;
+}
+}  // namespace
+
+void voxel_sample_multi(Workspace &ws, const std::vector<VoxSceneIn> &sc, float step,
+                        std::vector<int> &counts, hipStream_t s) {
+    const int S = (int)sc.size();
+    counts.assign(S, 0);
+    if (S == 0)
+        return;
+    std::vector<VoxSceneDev> d(S);
+    std::vector<int>         off(S + 1, 0);
+    uint32_t                 tab = 0;
+    for (int w = 0; w < S; w++) {
+        uint32_t hsize = 64;  // voxel_sample's table size for this scene
+        while (hsize < 2u * (uint32_t)std::max(sc[w].m, 1))
+            hsize <<= 1;
+        d[w].X      = sc[w].X;
+        d[w].Y      = sc[w].Y;
+        d[w].Z      = sc[w].Z;
+        d[w].subset = sc[w].subset;
+        d[w].out    = sc[w].out;
+        for (int a = 0; a < 3; a++)
+            d[w].bmin[a] = sc[w].bmin[a];
+        d[w].tabOff = tab;
+        d[w].hmask  = hsize - 1;
+        if ((uint64_t)tab + hsize > 0xFFFFFFFFull)
+            throw std::runtime_error("voxel tables of the batch too large");
+        tab += hsize;
+        off[w + 1] = off[w] + sc[w].m;
+    }
+    const int M = off[S];
+    // one upload: descriptors, offsets, then room for the S + 1 bases
+    const size_t dBytes = sizeof(VoxSceneDev) * S, oBytes = sizeof(int) * (S + 1);
+    unsigned char *blob = ws.voxMulti.as<unsigned char>(dBytes + 2 * oBytes + 256);
+    VoxSceneDev   *dD   = reinterpret_cast<VoxSceneDev *>(blob);
+    int           *offD = reinterpret_cast<int *>(blob + dBytes);
+    int           *base = offD + (S + 1);
+    std::vector<unsigned char> hostBlob(dBytes + oBytes);
+    std::memcpy(hostBlob.data(), d.data(), dBytes);
+    std::memcpy(hostBlob.data() + dBytes, off.data(), oBytes);
+    PPF_HIP_CHECK(hipMemcpyAsync(blob, hostBlob.data(), hostBlob.size(), hipMemcpyHostToDevice, s));
+    unsigned long long *tkey  = ws.voxKey.as<unsigned long long>(tab);
+    unsigned long long *tbest = ws.voxBest.as<unsigned long long>(tab);
+    char *flags = ws.voxFlag.as<char>(std::max(M, 1));
+    int  *sel   = ws.voxIota.as<int>(std::max(M, 1));
+    int  *nSel  = ws.smallCnt.as<int>(8);
+    vs_clear_kernel<<<std::min<uint32_t>(div_up(tab, 256), 4096), 256, 0, s>>>(tkey, tbest, tab);
+    if (M > 0) {
+        vsm_insert_kernel<<<div_up(M, 256), 256, 0, s>>>(dD, offD, S, M, step, tkey, tbest);
+        vsm_flag_kernel<<<div_up(M, 256), 256, 0, s>>>(dD, offD, S, M, step, tkey, tbest, flags);
+    }
+    size_t tb = 0;
+    rocprim::counting_iterator<int> it(0);
+    PPF_HIP_CHECK(rocprim::select(nullptr, tb, it, flags, sel, nSel, M, s));
+    PPF_HIP_CHECK(rocprim::select(ws.cubTmp.reserve(tb), tb, it, flags, sel, nSel, M, s));
+    vsm_bases_kernel<<<div_up(S + 1, 64), 64, 0, s>>>(sel, nSel, offD, S, base);
+    if (M > 0)
+        vsm_scatter_kernel<<<div_up(M, 256), 256, 0, s>>>(dD, offD, S, sel, nSel, base);
+    std::vector<int> b(S + 1);
+    PPF_HIP_CHECK(hipMemcpyAsync(b.data(), base, oBytes, hipMemcpyDeviceToHost, s));
+    PPF_HIP_CHECK(hipStreamSynchronize(s));
+    for (int w = 0; w < S; w++)
+        counts[w] = b[w + 1] - b[w];
+}
+
+namespace {
+struct GridSceneDev {
+    const float *X, *Y, *Z, *NX, *NY, *NZ;
+    const int   *subset;
+    float        ox, oy, oz, cell;
+    int          dx, dy, dz, odx, ody, sh;
+    uint32_t     cellOff, occOff;
+};
+
+__global__ void gm_rank_kernel(const GridSceneDev *sc, const int *off, int S, int M,
+                               uint32_t *keys, int *vals, int *rank, int *cnt, uint32_t *occ) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= M)
+        return;
+    const int           w  = owner_of(off, S, t);
+    const GridSceneDev &d  = sc[w];
+    const int           j  = t - off[w];
+    const int           i  = d.subset ? d.subset[j] : j;
+    const int           cx = cell_coord(d.X[i], d.ox, d.cell, d.dx);
+    const int           cy = cell_coord(d.Y[i], d.oy, d.cell, d.dy);
+    const int           cz = cell_coord(d.Z[i], d.oz, d.cell, d.dz);
+    const uint32_t      key = d.cellOff + (uint32_t)((cz * d.dy + cy) * d.dx + cx);
+    keys[t]                 = key;
+    vals[t]                 = i;
+    const int r             = atomicAdd(&cnt[key], 1);
+    rank[t]                 = r;
+    if (r == 0) {
+        const uint32_t b = ((uint32_t)(cz >> d.sh) * (uint32_t)d.ody + (uint32_t)(cy >> d.sh)) *
+                               (uint32_t)d.odx + (uint32_t)(cx >> d.sh);
+        atomicOr(&occ[d.occOff + (b >> 5)], 1u << (b & 31u));
+    }
+}
+
+__global__ void gm_scatter_kernel(const GridSceneDev *sc, const int *off, int S, int M,
+                                  const uint32_t *keys, const int *vals, const int *rank,
+                                  const int *cstart, int *id, float *x, float *y, float *z,
+                                  float *nx, float *ny, float *nz, float4 *p4, float4 *n4) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= M)
+        return;
+    const GridSceneDev &d    = sc[owner_of(off, S, t)];
+    const int           i    = vals[t];
+    const int           slot = cstart[keys[t]] + rank[t];
+    id[slot]                 = i;
+    const float px = d.X[i], py = d.Y[i], pz = d.Z[i];
+    x[slot]        = px;
+    y[slot]        = py;
+    z[slot]        = pz;
+    if (p4)
+        p4[slot] = make_float4(px, py, pz, __int_as_float(i));
+    if (nx) {
+        const float qx = d.NX[i], qy = d.NY[i], qz = d.NZ[i];
+        nx[slot]       = qx;
+        ny[slot]       = qy;
+        nz[slot]       = qz;
+        if (n4)
+            n4[slot] = make_float4(qx, qy, qz, 0.0f);
+    }
+}
+}  // namespace
+
+bool grid_build_multi(MultiGrid &mg, const std::vector<GridSceneIn> &sc, float cellHint,
+                      bool aos, hipStream_t s) {
+    const int S = (int)sc.size();
+    mg.views.assign(S, GridView{});
+    if (S == 0)
+        return true;
+    std::vector<GridSceneDev> d(S);
+    std::vector<int>          off(S + 1, 0);
+    size_t                    cells = 0, occWords = 0;
+    const bool                normals = sc[0].NX != nullptr;
+    for (int w = 0; w < S; w++) {
+        const GridSceneIn &g = sc[w];
+        if ((g.NX != nullptr) != normals)
+            throw std::logic_error("grid_build_multi: normals for some scenes only");
+        // grid_build's choices for this scene (box widened by 1e-4 of its
+        // extent, cell grown past 1000 cells an axis, occupancy blocks)
+        float hb[6];
+        for (int a = 0; a < 3; a++) {
+            const float pad = 1e-4f * (g.box[3 + a] - g.box[a]) + 1e-6f;
+            hb[a]           = g.box[a] - pad;
+            hb[3 + a]       = g.box[3 + a] + pad;
+        }
+        if (g.m == 0)
+            for (int a = 0; a < 6; a++)
+                hb[a] = 0.0f;
+        const float ext  = std::max({hb[3] - hb[0], hb[4] - hb[1], hb[5] - hb[2], 0.0f});
+        float       cell = cellHint > 0.0f ? cellHint : 1.0f;
+        if (ext / cell > 1000.0f)
+            cell = ext / 1000.0f;
+        int dims[3];
+        for (int a = 0; a < 3; a++)
+            dims[a] = (int)std::floor((hb[3 + a] - hb[a]) / cell) + 1;
+        const size_t c = (size_t)dims[0] * dims[1] * dims[2];
+        if (c > GRID_DENSE_MAX)
+            return false;
+        int sh = 0;
+        while (sh < 2 && ((size_t)(((dims[0] - 1) >> sh) + 1) * (size_t)(((dims[1] - 1) >> sh) + 1) *
+                          (size_t)(((dims[2] - 1) >> sh) + 1)) > OCC_MAX_BITS)
+            sh++;
+        const int odx = ((dims[0] - 1) >> sh) + 1, ody = ((dims[1] - 1) >> sh) + 1,
+                  odz = ((dims[2] - 1) >> sh) + 1;
+        d[w] = GridSceneDev{g.X, g.Y, g.Z, g.NX, g.NY, g.NZ, g.subset, hb[0], hb[1], hb[2], cell,
+                            dims[0], dims[1], dims[2], odx, ody, sh, (uint32_t)cells,
+                            (uint32_t)occWords};
+        GridView &v = mg.views[w];
+        v.ox = hb[0];
+        v.oy = hb[1];
+        v.oz = hb[2];
+        v.cell = cell;
+        v.dx = dims[0];
+        v.dy = dims[1];
+        v.dz = dims[2];
+        v.odx = odx;
+        v.ody = ody;
+        v.occShift = sh;
+        v.n = g.m;
+        cells += c + 1;
+        occWords += ((size_t)odx * ody * odz + 31) / 32;
+        off[w + 1] = off[w] + g.m;
+        if (cells > 0x7FFFFFFFull || occWords > 0xFFFFFFFFull)
+            return false;
+    }
+    // all scenes' counters together stay within a budget (the single-scene
+    // builds of the same grids would touch as many, one scene at a time)
+    if (cells > (size_t(1) << 27))
+        return false;
+    const int M = off[S];
+    const size_t dBytes = sizeof(GridSceneDev) * S, oBytes = sizeof(int) * (S + 1);
+    unsigned char *blob = mg.tmp.as<unsigned char>(dBytes + oBytes + 256);
+    std::vector<unsigned char> hostBlob(dBytes + oBytes);
+    std::memcpy(hostBlob.data(), d.data(), dBytes);
+    std::memcpy(hostBlob.data() + dBytes, off.data(), oBytes);
+    PPF_HIP_CHECK(hipMemcpyAsync(blob, hostBlob.data(), hostBlob.size(), hipMemcpyHostToDevice, s));
+    const GridSceneDev *dD   = reinterpret_cast<const GridSceneDev *>(blob);
+    const int          *offD = reinterpret_cast<const int *>(blob + dBytes);
+    int      *cnt    = mg.cnt.as<int>(cells);
+    int      *cstart = mg.cstart.as<int>(cells);
+    uint32_t *occ    = mg.occ.as<uint32_t>(std::max<size_t>(occWords, 1));
+    uint32_t *keys   = mg.keys.as<uint32_t>(std::max(M, 1));
+    int      *vals   = mg.vals.as<int>(std::max(M, 1));
+    int      *rank   = mg.rank.as<int>(std::max(M, 1));
+    int      *id     = mg.id.as<int>(std::max(M, 1));
+    float    *x = mg.x.as<float>(std::max(M, 1)), *y = mg.y.as<float>(std::max(M, 1)),
+          *z = mg.z.as<float>(std::max(M, 1));
+    float  *nx = normals ? mg.nx.as<float>(std::max(M, 1)) : nullptr,
+          *ny = normals ? mg.ny.as<float>(std::max(M, 1)) : nullptr,
+          *nz = normals ? mg.nz.as<float>(std::max(M, 1)) : nullptr;
+    float4 *p4 = aos ? mg.p4.as<float4>(std::max(M, 1)) : nullptr,
+           *n4 = (aos && normals) ? mg.n4.as<float4>(std::max(M, 1)) : nullptr;
+    const uint4 *htab = mg.htab.as<uint4>(64);  // never probed: the views have cstart
+    grid_clear_kernel<<<(int)std::min<size_t>((cells + occWords + 255) / 256, 4096), 256, 0, s>>>(
+        reinterpret_cast<uint32_t *>(cnt), cells, occ, occWords);
+    if (M > 0)
+        gm_rank_kernel<<<div_up(M, 256), 256, 0, s>>>(dD, offD, S, M, keys, vals, rank, cnt, occ);
+    size_t tb = 0;
+    PPF_HIP_CHECK(rocprim::exclusive_scan(nullptr, tb, cnt, cstart, 0, cells, rocprim::plus<int>(), s));
+    void *tmp = mg.scanTmp.reserve(tb);
+    PPF_HIP_CHECK(rocprim::exclusive_scan(tmp, tb, cnt, cstart, 0, cells, rocprim::plus<int>(), s));
+    if (M > 0)
+        gm_scatter_kernel<<<div_up(M, 256), 256, 0, s>>>(dD, offD, S, M, keys, vals, rank, cstart,
+                                                          id, x, y, z, nx, ny, nz, p4, n4);
+    PPF_HIP_CHECK(hipGetLastError());
+    // every view indexes the shared slot arrays: scene w's points are the
+    // slots [off[w], off[w + 1]), which its cell starts name directly
+    for (int w = 0; w < S; w++) {
+        GridView &v = mg.views[w];
+        v.hmask     = 63;
+        v.htab      = htab;
+        v.x         = x;
+        v.y         = y;
+        v.z         = z;
+        v.id        = id;
+        v.nx        = nx;
+        v.ny        = ny;
+        v.nz        = nz;
+        v.occ       = occ + d[w].occOff;
+        v.cstart    = cstart + d[w].cellOff;
+        v.p4        = p4;
+        v.n4        = n4;
+    }
+    return true;
+}
+
 // ---- voxel sampling into caller-provided scratch: no allocation, no
 // memset, no host synchronisation (the same kernels as voxel_sample; the
 // order-keeping compaction is the library's own: per-block counts, one

@@ -21,7 +21,7 @@ struct GraphCache {
 
 // Reusable device scratch of one detector (grown on demand, never shrunk).
 struct Workspace {
-    DevBuf voxKey, voxBest, voxFlag, voxIota, cubTmp, counter;
+    DevBuf voxKey, voxBest, voxFlag, voxIota, cubTmp, counter, voxMulti;
     DevBuf p0x, p0y, p0z, n0x, n0y, n0z, rawX, rawY, rawZ, cenX, cenY, cenZ, have, touch,
         touchList, nbq, cntq, outN, mark, qidx;
     Grid   gNormal, gVote, gSparse, gDense, gFull, gMls;
@@ -147,6 +147,42 @@ size_t voxel_scratch_bytes(int m);
 void   voxel_sample_into(const float *X, const float *Y, const float *Z, const int *subset, int m,
                          float step, const float bmin[3], int *out, int *countDev,
                          unsigned char *scratch, hipStream_t s);
+// ---- several scenes in one launch sequence (the batched match): each
+// scene's part computes exactly what the single-scene call computes on it
+
+// One scene's input to voxel_sample_multi: its points, the subset sampled
+// (nullptr: 0..m-1), the voxel origin and its output list.
+struct VoxSceneIn {
+    const float *X, *Y, *Z;
+    const int   *subset;
+    int          m;
+    float        bmin[3];
+    int         *out;
+};
+// voxel_sample of every scene at one step; counts[i] is scene i's output
+// size.  One read-back for all scenes.
+void voxel_sample_multi(Workspace &ws, const std::vector<VoxSceneIn> &sc, float step,
+                        std::vector<int> &counts, hipStream_t s);
+
+// One scene's input to grid_build_multi (grid_build's arguments).
+struct GridSceneIn {
+    const float *X, *Y, *Z, *NX, *NY, *NZ;  // NX nullptr: no normals
+    const int   *subset;
+    int          m;
+    float        box[6];  // a box known to hold the points (grid_build's hostBox6)
+};
+// The grids of several scenes built by one counting sort: every scene's
+// view indexes shared slot arrays; cells per scene as grid_build chooses
+// them.  Returns false (nothing built) when a scene's grid would exceed the
+// dense cell limit or all scenes' cells together would; the caller then
+// builds them one by one.
+struct MultiGrid {
+    DevBuf keys, vals, rank, cnt, cstart, occ, x, y, z, id, nx, ny, nz, p4, n4, tmp, scanTmp, htab;
+    std::vector<GridView> views;
+};
+bool grid_build_multi(MultiGrid &mg, const std::vector<GridSceneIn> &sc, float cellHint,
+                      bool aos, hipStream_t s);
+
 void estimate_normals(Workspace &ws, int n, float *PX, float *PY, float *PZ, float *NX,
                       float *NY, float *NZ, const int *valid, int nValid, const int *query,
                       int nq, int k, int smooth, int invert, const float vp[3], float cellHint,
