@@ -106,12 +106,6 @@ __global__ void split_kernel(const float *aos, int n, float *x, float *y, float 
     y[i] = aos[3 * (size_t)i + 1];
     z[i] = aos[3 * (size_t)i + 2];
 }
-__global__ void fill_int_kernel(int *a, int n, int v) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n)
-        a[i] = v;
-}
-
 __global__ void fill_nan_kernel(float *a, float *b, float *c, int n) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n)
@@ -251,9 +245,9 @@ struct MatchCtx {
         int    ns = 0, nk = 0, nd = 0;
     };
     std::vector<std::unique_ptr<SceneWork>> sceneWork;
-    DevBuf batchKpScene, batchKpId, batchGrids, batchHypScene;
+    DevBuf batchKpScene, batchKpId, batchKpDesc, batchGrids, batchHypScene;
     // grids of every scene of a batched match, built together
-    MultiGrid batchVote, batchSparse, batchFull, batchDense;
+    MultiGrid batchVote, batchSparse, batchFull, batchDense, batchNormal;
     // the per-scene stages of a batched match run on up to kLanes host
     // threads, each with its own stream (default priority) and workspace,
     // so the small kernels and read-backs of different scenes overlap
@@ -808,58 +802,13 @@ void match_impl(ppf_detector_s &D, MatchCtx &X, DevCloud &c, float rel, float kp
     st->msIcpDense      = nDenseIcp ? elapsed(X.ev[11], X.ev[12]) : 0.0f;
 }
 
-// Several scenes matched together (same parameters): each scene is sampled,
-// given normals and key points on its own, then the key points of all scenes
-// are voted in one launch (each key point searches its own scene's grid),
-// the poses are clustered per scene, and the hypotheses of all scenes are
-// refined in one batched refinement per stage (each against its own scene).
-// Every per-scene result equals match_impl's on that scene.
-// f(i, workspace, stream) for every scene i < n, spread over the detector's
-// lanes; returns once all of them finished (their streams synchronised).
-// Work queued on the detector's stream before the call completes first.
-// Each scene's work is independent of the lane it lands on.
-template <class F>
-void for_scenes(MatchCtx &X, int n, F &&f) {
-    PPF_HIP_CHECK(hipStreamSynchronize(X.stream));
-    const int L = std::min(n, MatchCtx::kLanes);
-    if (L <= 1) {
-        for (int i = 0; i < n; i++)
-            f(i, X.ws, X.stream);
-        PPF_HIP_CHECK(hipStreamSynchronize(X.stream));
-        return;
-    }
-    lane_of(X, L - 1);
-    int dev = 0;
-    PPF_HIP_CHECK(hipGetDevice(&dev));
-    std::atomic<int>                next{0};
-    std::vector<std::exception_ptr> err(L);
-    std::vector<std::thread>        th;
-    for (int l = 0; l < L; l++)
-        th.emplace_back([&, l] {
-            try {
-                PPF_HIP_CHECK(hipSetDevice(dev));
-                MatchCtx::Lane &ln = *X.lanes[l];
-                for (int i = next++; i < n; i = next++)
-                    f(i, ln.ws, ln.stream);
-                PPF_HIP_CHECK(hipStreamSynchronize(ln.stream));
-            } catch (...) {
-                err[l] = std::current_exception();
-            }
-        });
-    for (auto &t : th)
-        t.join();
-    for (auto &e : err)
-        if (e)
-            std::rethrow_exception(e);
-}
-
 // Two passes of the lanes over the scenes, in the background of the
 // calling thread: f(i, workspace, stream) for every scene, then, once
 // open() is called, g(i, ...) for every scene.  join() waits for both and
 // rethrows a lane's error; leaving the scope without open() skips g.
 struct LanePasses {
     using Fn = std::function<void(int, Workspace &, hipStream_t)>;
-    using AfterFn = std::function<void(hipStream_t)>;
+    using AfterFn = std::function<void(Workspace &, hipStream_t)>;
     MatchCtx                       &X;
     int                             n;
     Fn                              f, g;
@@ -907,7 +856,7 @@ struct LanePasses {
                         if (doneF.fetch_add(1, std::memory_order_acq_rel) + 1 == nLanes) {
                             for (int o = 0; o < nLanes; o++)
                                 PPF_HIP_CHECK(hipStreamWaitEvent(ln.stream, X.lanes[o]->ev[2], 0));
-                            afterF(ln.stream);
+                            afterF(ln.ws, ln.stream);
                         }
                     }
                     int q;
@@ -942,6 +891,13 @@ struct LanePasses {
     }
 };
 
+// Several scenes matched together (same parameters): the scenes' sampling,
+// normals, key points and grids run as one launch sequence for all scenes
+// (each scene in its own slice of the shared tables), the key points of all
+// scenes are voted in one launch (each key point searches its own scene's
+// grid), the poses are clustered per scene, and the hypotheses of all
+// scenes are refined in one batched refinement per stage (each against its
+// own scene).  Every per-scene result equals match_impl's on that scene.
 void match_batch_impl(ppf_detector_s &D, MatchCtx &X, const std::vector<DevCloud *> &clouds, float rel,
                       float kpFrac, float minScore, const ppf_match_param &p,
                       std::vector<MatchOut> &outs, ppf_match_stats *st) {
@@ -1005,16 +961,30 @@ void match_batch_impl(ppf_detector_s &D, MatchCtx &X, const std::vector<DevCloud
     voxel_sample_multi(X.ws, vin, sampleStep, cnts, s);
     for (int i = 0; i < n; i++)
         X.sceneWork[i]->ns = cnts[i];
-    // ---- per scene on the lanes: normals (ppf.cpp:204-211)
-    for_scenes(X, n, [&](int i, Workspace &ws, hipStream_t ls) {
-        DevCloud &c  = *clouds[i];
-        auto     &sw = *X.sceneWork[i];
-        if (!c.hasNormal)
-            estimate_normals(ws, c.n, c.px.get<float>(), c.py.get<float>(), c.pz.get<float>(),
-                             c.nx.get<float>(), c.ny.get<float>(), c.nz.get<float>(),
-                             c.valid.get<int>(), c.nValid, sw.samp.get<int>(), sw.ns, p.knnNormal,
-                             p.smoothNormal, p.invertNormal, c.vp, normCell, ls, c.ext);
-    });
+    // ---- the normals of every scene's sampled points (ppf.cpp:204-211)
+    auto normalsOf = [&](auto query, float cell, int smooth, Workspace &ws, hipStream_t ls) {
+        std::vector<NormSceneIn> ns;
+        for (int i = 0; i < n; i++) {
+            DevCloud &c = *clouds[i];
+            if (c.hasNormal)
+                continue;
+            NormSceneIn in{c.px.get<float>(), c.py.get<float>(), c.pz.get<float>(),
+                           c.nx.get<float>(), c.ny.get<float>(), c.nz.get<float>(), c.n,
+                           c.valid.get<int>(), c.nValid, nullptr, 0, {}, {}};
+            query(i, in);
+            std::memcpy(in.vp, c.vp, sizeof in.vp);
+            std::memcpy(in.box, c.ext, sizeof in.box);
+            ns.push_back(in);
+        }
+        estimate_normals_multi(ws, X.batchNormal, ns, p.knnNormal, smooth, p.invertNormal, cell,
+                               ls);
+    };
+    normalsOf(
+        [&](int i, NormSceneIn &in) {
+            in.query = X.sceneWork[i]->samp.get<int>();
+            in.nq    = X.sceneWork[i]->ns;
+        },
+        normCell, p.smoothNormal, X.ws, s);
     // ---- every scene's key points and vote grid together (ppf.cpp:212-216)
     std::vector<GridSceneIn> gin(n);
     for (int i = 0; i < n; i++) {
@@ -1070,21 +1040,15 @@ void match_batch_impl(ppf_detector_s &D, MatchCtx &X, const std::vector<DevCloud
         knz = X.ws.kpNZ.as<float>(K);
         kpScene = X.batchKpScene.as<int>(K);
         kpId    = X.batchKpId.as<int>(K);
+        std::vector<KpSceneIn> kin(n);
         for (int i = 0; i < n; i++) {
-            DevCloud &c  = *clouds[i];
-            auto     &sw = *X.sceneWork[i];
-            views[i]     = voteViews[i];
-            if (sw.nk == 0)
-                continue;
-            const int o = ko[i];
-            gather_keypoints(sw.kp.get<int>(), sw.nk, c.px.get<float>(), c.py.get<float>(),
-                             c.pz.get<float>(), c.nx.get<float>(), c.ny.get<float>(),
-                             c.nz.get<float>(), kx + o, ky + o, kz + o, knx + o, kny + o, knz + o,
-                             s);
-            fill_int_kernel<<<div_up(sw.nk, 256), 256, 0, s>>>(kpScene + o, sw.nk, i);
-            PPF_HIP_CHECK(hipMemcpyAsync(kpId + o, sw.kp.get<int>(), sizeof(int) * sw.nk,
-                                         hipMemcpyDeviceToDevice, s));
+            DevCloud &c = *clouds[i];
+            views[i]    = voteViews[i];
+            kin[i]      = KpSceneIn{X.sceneWork[i]->kp.get<int>(), c.px.get<float>(),
+                               c.py.get<float>(), c.pz.get<float>(), c.nx.get<float>(),
+                               c.ny.get<float>(), c.nz.get<float>()};
         }
+        gather_keypoints_multi(kin, ko, X.batchKpDesc, kx, ky, kz, knx, kny, knz, kpScene, kpId, s);
         gv = X.batchGrids.as<GridView>(n);
         PPF_HIP_CHECK(hipMemcpyAsync(gv, views.data(), sizeof(GridView) * n,
                                      hipMemcpyHostToDevice, s));
@@ -1103,22 +1067,8 @@ void match_batch_impl(ppf_detector_s &D, MatchCtx &X, const std::vector<DevCloud
     std::vector<std::vector<Hypo>> hyp(n);
     std::vector<int>               ncs(n, 0);
     std::vector<int>               pOff(n + 1, 0);
-    // the dense samples of every scene, before the lanes estimate their
-    // normals (the positions of the sampled points are final by now)
     for (int i = 0; i < n; i++)
         X.sceneWork[i]->nd = 0;
-    if (p.densePoseRefinement) {
-        for (int i = 0; i < n; i++) {
-            DevCloud &c = *clouds[i];
-            vin[i]      = VoxSceneIn{c.px.get<float>(), c.py.get<float>(), c.pz.get<float>(),
-                                c.valid.get<int>(), c.nValid,
-                                {c.boxMin[0], c.boxMin[1], c.boxMin[2]},
-                                X.sceneWork[i]->dense.as<int>(std::max(c.nValid, 1))};
-        }
-        voxel_sample_multi(X.ws, vin, reSampleStep, cnts, s);
-        for (int i = 0; i < n; i++)
-            X.sceneWork[i]->nd = cnts[i];
-    }
     // the refinement grids of every scene over the final positions, queued
     // once every lane's normals are: together when they fit one build,
     // else scene by scene
@@ -1167,24 +1117,43 @@ void match_batch_impl(ppf_detector_s &D, MatchCtx &X, const std::vector<DevCloud
                 denseViews[i] = X.sceneWork[i]->gDense.view;
             }
     };
+    // on one lane: every scene's dense-sample normals, then the refinement
+    // grids over the final positions
+    auto laneWork = [&](Workspace &ws, hipStream_t ls) {
+        if (p.densePoseRefinement) {
+            // the dense samples (the positions of the sampled points are
+            // final by now), then their normals
+            std::vector<VoxSceneIn> din(n);
+            std::vector<int>        dc;
+            for (int i = 0; i < n; i++) {
+                DevCloud &c = *clouds[i];
+                din[i]      = VoxSceneIn{c.px.get<float>(), c.py.get<float>(), c.pz.get<float>(),
+                                    c.valid.get<int>(), c.nValid,
+                                    {c.boxMin[0], c.boxMin[1], c.boxMin[2]},
+                                    X.sceneWork[i]->dense.as<int>(std::max(c.nValid, 1))};
+            }
+            voxel_sample_multi(ws, din, reSampleStep, dc, ls);
+            for (int i = 0; i < n; i++)
+                X.sceneWork[i]->nd = dc[i];
+            normalsOf(
+                [&](int i, NormSceneIn &in) {
+                    in.query = X.sceneWork[i]->dense.get<int>();
+                    in.nq    = X.sceneWork[i]->nd;
+                },
+                denseNormCell, 0, ws, ls);
+        }
+        if (laneGrids)
+            refineGrids(ls);
+    };
     LanePasses lanes(
-        X, n,
-        [&](int i, Workspace &ws, hipStream_t ls) {
-            DevCloud &c  = *clouds[i];
-            auto     &sw = *X.sceneWork[i];
-            if (p.densePoseRefinement && !c.hasNormal)
-                estimate_normals(ws, c.n, c.px.get<float>(), c.py.get<float>(), c.pz.get<float>(),
-                                 c.nx.get<float>(), c.ny.get<float>(), c.nz.get<float>(),
-                                 c.valid.get<int>(), c.nValid, sw.dense.get<int>(), sw.nd,
-                                 p.knnNormal, 0, p.invertNormal, c.vp, denseNormCell, ls, c.ext);
-        },
+        X, n, [](int, Workspace &, hipStream_t) {},
         [&](int i, Workspace &ws, hipStream_t ls) {
             // scenes with long pose lists: clustered one by one on the lanes
             // (the vote's pose buffer may have grown during the vote)
             cluster_poses(ws, X.ws.poses.get<PoseRec>() + pOff[i], pOff[i + 1] - pOff[i],
                           0.1f * Dm, angleStep, center, maxOverlapDist, hyp[i], &ncs[i], ls);
         },
-        laneGrids ? LanePasses::AfterFn(refineGrids) : LanePasses::AfterFn());
+        laneWork);
     // ---- every scene's key points in one vote
     if (K > 0) {
         VoteKeypoints in;

@@ -94,6 +94,22 @@ static_assert(sizeof(GridView) == 144, "GridView holds no padding bytes");
 
 constexpr size_t GRID_DENSE_MAX = size_t(1) << 24;
 
+// Several scenes in one launch: scene w owns the flat elements
+// [off[w], off[w + 1]).  The scene owning element t is the last w with
+// off[w] <= t (empty scenes share their offset with the next one and are
+// never chosen).
+__device__ __forceinline__ int owner_of(const int *off, int S, int t) {
+    int lo = 0, hi = S - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (off[mid] <= t)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
 // occupancy bitmaps up to this many bits use one bit per cell; larger grids
 // one bit per 2x2x2 or 4x4x4 block
 constexpr size_t OCC_MAX_BITS = size_t(1) << 26;

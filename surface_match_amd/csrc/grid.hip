@@ -540,20 +540,6 @@ int voxel_sample(Workspace &ws, const float *X, const float *Y, const float *Z,
 // cells, counters and bitmap words, so each scene's result is the
 // single-scene one.
 namespace {
-// the scene owning flat element t: the last w with off[w] <= t (empty
-// scenes share their offset with the next one and are never chosen)
-__device__ __forceinline__ int owner_of(const int *off, int S, int t) {
-    int lo = 0, hi = S - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (off[mid] <= t)
-            lo = mid;
-        else
-            hi = mid - 1;
-    }
-    return lo;
-}
-
 struct VoxSceneDev {
     const float *X, *Y, *Z;
     const int   *subset;
@@ -713,6 +699,7 @@ struct GridSceneDev {
     float        ox, oy, oz, cell;
     int          dx, dy, dz, odx, ody, sh;
     uint32_t     cellOff, occOff;
+    int          idOff;
 };
 
 __global__ void gm_rank_kernel(const GridSceneDev *sc, const int *off, int S, int M,
@@ -749,13 +736,13 @@ __global__ void gm_scatter_kernel(const GridSceneDev *sc, const int *off, int S,
     const GridSceneDev &d    = sc[owner_of(off, S, t)];
     const int           i    = vals[t];
     const int           slot = cstart[keys[t]] + rank[t];
-    id[slot]                 = i;
+    id[slot]                 = i + d.idOff;
     const float px = d.X[i], py = d.Y[i], pz = d.Z[i];
     x[slot]        = px;
     y[slot]        = py;
     z[slot]        = pz;
     if (p4)
-        p4[slot] = make_float4(px, py, pz, __int_as_float(i));
+        p4[slot] = make_float4(px, py, pz, __int_as_float(i + d.idOff));
     if (nx) {
         const float qx = d.NX[i], qy = d.NY[i], qz = d.NZ[i];
         nx[slot]       = qx;
@@ -771,6 +758,7 @@ bool grid_build_multi(MultiGrid &mg, const std::vector<GridSceneIn> &sc, float c
                       bool aos, hipStream_t s) {
     const int S = (int)sc.size();
     mg.views.assign(S, GridView{});
+    mg.total = 0;
     if (S == 0)
         return true;
     std::vector<GridSceneDev> d(S);
@@ -810,7 +798,7 @@ bool grid_build_multi(MultiGrid &mg, const std::vector<GridSceneIn> &sc, float c
                   odz = ((dims[2] - 1) >> sh) + 1;
         d[w] = GridSceneDev{g.X, g.Y, g.Z, g.NX, g.NY, g.NZ, g.subset, hb[0], hb[1], hb[2], cell,
                             dims[0], dims[1], dims[2], odx, ody, sh, (uint32_t)cells,
-                            (uint32_t)occWords};
+                            (uint32_t)occWords, g.idOff};
         GridView &v = mg.views[w];
         v.ox = hb[0];
         v.oy = hb[1];
@@ -829,11 +817,13 @@ bool grid_build_multi(MultiGrid &mg, const std::vector<GridSceneIn> &sc, float c
         if (cells > 0x7FFFFFFFull || occWords > 0xFFFFFFFFull)
             return false;
     }
-    // all scenes' counters together stay within a budget (the single-scene
-    // builds of the same grids would touch as many, one scene at a time)
-    if (cells > (size_t(1) << 27))
+    // all scenes' counters together stay within a budget of 2^28 cells (2 GB
+    // of counters and starts; the single-scene builds of the same grids
+    // would touch as many, one scene at a time)
+    if (cells > (size_t(1) << 28))
         return false;
     const int M = off[S];
+    mg.total    = M;
     const size_t dBytes = sizeof(GridSceneDev) * S, oBytes = sizeof(int) * (S + 1);
     unsigned char *blob = mg.tmp.as<unsigned char>(dBytes + oBytes + 256);
     std::vector<unsigned char> hostBlob(dBytes + oBytes);
@@ -1052,6 +1042,12 @@ struct NormArgs {
     int         *touch;    // per point: neighbour needing a PCA
     int         *mark;     // per point: 1 = query to estimate, 2 = neighbour to estimate
     int         *qidx;     // per point: index in the query list
+    // several scenes (estimate_normals_multi): point ids are global, scene w
+    // owning [poff[w], poff[w + 1]); its grid is views[w], which shares g's
+    // slot arrays and id space (nullptr: one scene, g)
+    const GridView *views;
+    const int      *poff;
+    int             S;
 };
 
 // store one estimate (want 1: a query, with its kNN list for smoothing;
@@ -1125,7 +1121,7 @@ __global__ void normals_touch_mark_kernel(NormArgs a, int n) {
 }
 
 // Estimation of one point by the general grid kNN (ring expansion)
-template <int KMAX>
+template <int KMAX, bool MULTI>
 __global__ void __launch_bounds__(128) normals_point_kernel(NormArgs a, const int *list, int nl,
                                                             int want) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1135,7 +1131,11 @@ __global__ void __launch_bounds__(128) normals_point_kernel(NormArgs a, const in
     if (a.mark[id] != want)
         return;
     KnnList<KMAX> L;
-    grid_knn<KMAX>(a.g, a.g.nx[id], a.g.ny[id], a.g.nz[id], a.k, L);
+    if constexpr (MULTI) {
+        const GridView g = a.views[owner_of(a.poff, a.S, id)];
+        grid_knn<KMAX>(g, a.g.nx[id], a.g.ny[id], a.g.nz[id], a.k, L);
+    } else
+        grid_knn<KMAX>(a.g, a.g.nx[id], a.g.ny[id], a.g.nz[id], a.k, L);
     store_estimate<KMAX>(a, id, want, L);
 }
 
@@ -1290,6 +1290,7 @@ __device__ __forceinline__ void coop_knn(const GridView &g, float qx, float qy, 
 // neighbour as {x, y, z, id bits} from the grid's slot-ordered copies (k
 // per point, ascending by (dist2, id); the PCA then reads its list as one
 // contiguous run instead of gathering by id) and counts (-1: not estimated)
+template <bool MULTI>
 __global__ void __launch_bounds__(256) normals_knn_coop_kernel(NormArgs a, const int *list,
                                                                int nl, int want, float4 *nbr,
                                                                int *cnts) {
@@ -1307,7 +1308,11 @@ __global__ void __launch_bounds__(256) normals_knn_coop_kernel(NormArgs a, const
     }
     float d;
     int   i, slot, cnt;
-    coop_knn(a.g, a.g.nx[id], a.g.ny[id], a.g.nz[id], a.k, l, gsh, d, i, slot, cnt);
+    if constexpr (MULTI) {
+        const GridView g = a.views[owner_of(a.poff, a.S, id)];
+        coop_knn(g, a.g.nx[id], a.g.ny[id], a.g.nz[id], a.k, l, gsh, d, i, slot, cnt);
+    } else
+        coop_knn(a.g, a.g.nx[id], a.g.ny[id], a.g.nz[id], a.k, l, gsh, d, i, slot, cnt);
     if (l < cnt)
         nbr[(size_t)t * a.k + l] =
             make_float4(a.g.x[slot], a.g.y[slot], a.g.z[slot], __int_as_float(i));
@@ -1406,18 +1411,15 @@ __global__ void normals_writeback_kernel(NormArgs a, int n, float *PX, float *PY
     NZ[i] = a.rawZ[i];
 }
 
-// queries take their final normal, then orientation (util.cpp:309-333)
-__global__ void normals_orient_kernel(NormArgs a, const float *outN, float *PX, float *PY,
-                                      float *PZ, float *NX, float *NY, float *NZ, int invert,
-                                      int vpFinite, float vx, float vy, float vz) {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= a.nq)
-        return;
-    int   id = a.query[t];
-    float nx = outN[3 * t + 0], ny = outN[3 * t + 1], nz = outN[3 * t + 2];
+// a query's final normal oriented towards the viewpoint, or +z without
+// one (util.cpp:309-333)
+__device__ __forceinline__ void orient_store(float nx, float ny, float nz, float px, float py,
+                                             float pz, int invert, int vpFinite, float vx,
+                                             float vy, float vz, float *NX, float *NY, float *NZ,
+                                             int id) {
     float d;
     if (vpFinite) {
-        float dx = vx - PX[id], dy = vy - PY[id], dz = vz - PZ[id];
+        float dx = vx - px, dy = vy - py, dz = vz - pz;
         d        = nx * dx + ny * dy + nz * dz;
     } else {
         d = nx * 0.0f + ny * 0.0f + nz * 1.0f;
@@ -1433,6 +1435,17 @@ __global__ void normals_orient_kernel(NormArgs a, const float *outN, float *PX, 
     NZ[id] = nz;
 }
 
+// queries take their final normal, then orientation
+__global__ void normals_orient_kernel(NormArgs a, const float *outN, float *PX, float *PY,
+                                      float *PZ, float *NX, float *NY, float *NZ, int invert,
+                                      int vpFinite, float vx, float vy, float vz) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.nq)
+        return;
+    int id = a.query[t];
+    orient_store(outN[3 * t + 0], outN[3 * t + 1], outN[3 * t + 2], PX[id], PY[id], PZ[id],
+                 invert, vpFinite, vx, vy, vz, NX, NY, NZ, id);
+}
 
 // the call's snapshot of positions and normals
 __global__ void snapshot6_kernel(int n, const float *a0, const float *a1, const float *a2,
@@ -1462,35 +1475,16 @@ __global__ void normals_clear_kernel(int n, int *have, int *touch, int *mark, in
     }
 }
 
-void estimate_normals(Workspace &ws, int n, float *PX, float *PY, float *PZ, float *NX, float *NY,
-                      float *NZ, const int *valid, int nValid, const int *query, int nq, int k,
-                      int smooth, int invert, const float vp[3], float cellHint, hipStream_t s,
-                      const float *hostBox6) {
-    if (k < 3)
-        k = 3;
-    if (k > KNN_MAX)
-        throw std::range_error("knnNormal above 32 is not supported");
-    // snapshot copies
-    float *P0x = ws.p0x.as<float>(n), *P0y = ws.p0y.as<float>(n), *P0z = ws.p0z.as<float>(n);
-    float *N0x = ws.n0x.as<float>(n), *N0y = ws.n0y.as<float>(n), *N0z = ws.n0z.as<float>(n);
-    if (n > 0)
-        snapshot6_kernel<<<std::min(div_up(n, 256), 4096), 256, 0, s>>>(
-            n, PX, PY, PZ, NX, NY, NZ, P0x, P0y, P0z, N0x, N0y, N0z);
-    grid_build(ws.gNormal, P0x, P0y, P0z, nullptr, nullptr, nullptr, valid, nValid, cellHint, s,
-               hostBox6);
-
-    NormArgs a;
-    a.g    = ws.gNormal.view;
-    a.g.nx = P0x;  // kNN coordinates come from the snapshot, indexed by id
-    a.g.ny = P0y;
-    a.g.nz = P0z;
-    a.N0x  = N0x;
-    a.N0y  = N0y;
-    a.N0z  = N0z;
-    a.query  = query;
-    a.nq     = nq;
-    a.k      = k;
-    a.smooth = smooth;
+namespace {
+// The estimation proper over prepared arguments (snapshot, grid, query
+// list): clears the per-point state of n points, marks the queries,
+// estimates them in the grid's cell order and, for the smoothing, the
+// neighbours it needs; leaves each query's final normal in outN.  Two
+// read-backs (the counts of the points to estimate).
+template <bool MULTI>
+void normals_core(Workspace &ws, NormArgs &a, int n, float *outN, hipStream_t s) {
+    const int nq   = a.nq;
+    const int k    = a.k;
     a.rawX = ws.rawX.as<float>(n);
     a.rawY = ws.rawY.as<float>(n);
     a.rawZ = ws.rawZ.as<float>(n);
@@ -1503,7 +1497,6 @@ void estimate_normals(Workspace &ws, int n, float *PX, float *PY, float *PZ, flo
     a.cntq  = ws.cntq.as<int>(std::max(nq, 1));
     a.mark  = ws.mark.as<int>(n);
     a.qidx  = ws.qidx.as<int>(n);
-    float *outN = ws.outN.as<float>((size_t)std::max(nq, 1) * 3);
     int *cntD = ws.counter.as<int>(4);
     normals_clear_kernel<<<std::min(div_up(std::max(n, 4), 256), 4096), 256, 0, s>>>(
         n, a.have, a.touch, a.mark, cntD);
@@ -1529,13 +1522,13 @@ void estimate_normals(Workspace &ws, int n, float *PX, float *PY, float *PZ, flo
         if (NORMALS_COOP && k <= KNN_GROUP) {
             float4 *nbr  = ws.knnIds.as<float4>((size_t)nl * k);
             int    *cnts = ws.knnCnt.as<int>(nl);
-            normals_knn_coop_kernel<<<div_up(nl, 256 / KNN_GROUP), 256, 0, s>>>(a, list, nl, want,
-                                                                               nbr, cnts);
+            normals_knn_coop_kernel<MULTI><<<div_up(nl, 256 / KNN_GROUP), 256, 0, s>>>(
+                a, list, nl, want, nbr, cnts);
             normals_pca_kernel<16><<<div_up(nl, 128), 128, 0, s>>>(a, list, nl, want, nbr, cnts);
         } else if (small)
-            normals_point_kernel<16><<<div_up(nl, 128), 128, 0, s>>>(a, list, nl, want);
+            normals_point_kernel<16, MULTI><<<div_up(nl, 128), 128, 0, s>>>(a, list, nl, want);
         else
-            normals_point_kernel<KNN_MAX><<<div_up(nl, 128), 128, 0, s>>>(a, list, nl, want);
+            normals_point_kernel<KNN_MAX, MULTI><<<div_up(nl, 128), 128, 0, s>>>(a, list, nl, want);
     };
     // the queries to estimate, in cell order when every one of them is a
     // point of the grid (a query outside the valid set keeps list order)
@@ -1547,9 +1540,9 @@ void estimate_normals(Workspace &ws, int n, float *PX, float *PY, float *PZ, flo
         if (c[0] == c[2])
             estimate(olist, c[0], 1);
         else
-            estimate(query, nq, 1);
+            estimate(a.query, nq, 1);
     }
-    if (smooth) {
+    if (a.smooth) {
         // the neighbours the smoothing needs that have no estimate yet (all
         // of them are grid points: they come from kNN lists)
         normals_touch_mark_kernel<<<div_up(n, 256), 256, 0, s>>>(a, n);
@@ -1561,12 +1554,201 @@ void estimate_normals(Workspace &ws, int n, float *PX, float *PY, float *PZ, flo
     }
     if (nq > 0)
         normals_final_kernel<<<div_up(nq, 256), 256, 0, s>>>(a, outN);
+}
+}  // namespace
+
+void estimate_normals(Workspace &ws, int n, float *PX, float *PY, float *PZ, float *NX, float *NY,
+                      float *NZ, const int *valid, int nValid, const int *query, int nq, int k,
+                      int smooth, int invert, const float vp[3], float cellHint, hipStream_t s,
+                      const float *hostBox6) {
+    if (k < 3)
+        k = 3;
+    if (k > KNN_MAX)
+        throw std::range_error("knnNormal above 32 is not supported");
+    // snapshot copies
+    float *P0x = ws.p0x.as<float>(n), *P0y = ws.p0y.as<float>(n), *P0z = ws.p0z.as<float>(n);
+    float *N0x = ws.n0x.as<float>(n), *N0y = ws.n0y.as<float>(n), *N0z = ws.n0z.as<float>(n);
+    if (n > 0)
+        snapshot6_kernel<<<std::min(div_up(n, 256), 4096), 256, 0, s>>>(
+            n, PX, PY, PZ, NX, NY, NZ, P0x, P0y, P0z, N0x, N0y, N0z);
+    grid_build(ws.gNormal, P0x, P0y, P0z, nullptr, nullptr, nullptr, valid, nValid, cellHint, s,
+               hostBox6);
+
+    NormArgs a{};
+    a.g    = ws.gNormal.view;
+    a.g.nx = P0x;  // kNN coordinates come from the snapshot, indexed by id
+    a.g.ny = P0y;
+    a.g.nz = P0z;
+    a.N0x  = N0x;
+    a.N0y  = N0y;
+    a.N0z  = N0z;
+    a.query  = query;
+    a.nq     = nq;
+    a.k      = k;
+    a.smooth = smooth;
+    float *outN = ws.outN.as<float>((size_t)std::max(nq, 1) * 3);
+    normals_core<false>(ws, a, n, outN, s);
     normals_writeback_kernel<<<div_up(n, 256), 256, 0, s>>>(a, n, PX, PY, PZ, NX, NY, NZ);
     int vpFinite = vp && std::isfinite(vp[0]) && std::isfinite(vp[1]) && std::isfinite(vp[2]);
     if (nq > 0)
         normals_orient_kernel<<<div_up(nq, 256), 256, 0, s>>>(
             a, outN, PX, PY, PZ, NX, NY, NZ, invert, vpFinite, vpFinite ? vp[0] : 0.0f,
             vpFinite ? vp[1] : 0.0f, vpFinite ? vp[2] : 0.0f);
+    PPF_HIP_CHECK(hipGetLastError());
+}
+
+namespace {
+struct NormSceneDev {
+    float     *PX, *PY, *PZ, *NX, *NY, *NZ;
+    const int *query;
+    float      vx, vy, vz;
+    int        vpFinite;
+};
+
+// every scene's positions and normals side by side (global id poff[w] + i
+// is point i of scene w)
+__global__ void nm_snapshot_kernel(const NormSceneDev *sc, const int *poff, int S, int n,
+                                   float *P0x, float *P0y, float *P0z, float *N0x, float *N0y,
+                                   float *N0z) {
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+        const int           w = owner_of(poff, S, t);
+        const NormSceneDev &d = sc[w];
+        const int           i = t - poff[w];
+        P0x[t] = d.PX[i];
+        P0y[t] = d.PY[i];
+        P0z[t] = d.PZ[i];
+        N0x[t] = d.NX[i];
+        N0y[t] = d.NY[i];
+        N0z[t] = d.NZ[i];
+    }
+}
+
+// every scene's queries in one list of global ids
+__global__ void nm_query_kernel(const NormSceneDev *sc, const int *poff, const int *qoff, int S,
+                                int nq, int *gq) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nq)
+        return;
+    const int w = owner_of(qoff, S, t);
+    gq[t]       = sc[w].query[t - qoff[w]] + poff[w];
+}
+
+// write-back of every fresh PCA into its scene
+__global__ void nm_writeback_kernel(NormArgs a, const NormSceneDev *sc, int n) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n || !a.have[t])
+        return;
+    const int           w = owner_of(a.poff, a.S, t);
+    const NormSceneDev &d = sc[w];
+    const int           i = t - a.poff[w];
+    d.PX[i] = a.cenX[t];
+    d.PY[i] = a.cenY[t];
+    d.PZ[i] = a.cenZ[t];
+    d.NX[i] = a.rawX[t];
+    d.NY[i] = a.rawY[t];
+    d.NZ[i] = a.rawZ[t];
+}
+
+__global__ void nm_orient_kernel(NormArgs a, const NormSceneDev *sc, const float *outN,
+                                 int invert) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.nq)
+        return;
+    const int           id = a.query[t];
+    const int           w  = owner_of(a.poff, a.S, id);
+    const NormSceneDev &d  = sc[w];
+    const int           i  = id - a.poff[w];
+    orient_store(outN[3 * t + 0], outN[3 * t + 1], outN[3 * t + 2], d.PX[i], d.PY[i], d.PZ[i],
+                 invert, d.vpFinite, d.vx, d.vy, d.vz, d.NX, d.NY, d.NZ, i);
+}
+}  // namespace
+
+void estimate_normals_multi(Workspace &ws, MultiGrid &mg, const std::vector<NormSceneIn> &sc, int k,
+                            int smooth, int invert, float cellHint, hipStream_t s) {
+    if (k < 3)
+        k = 3;
+    if (k > KNN_MAX)
+        throw std::range_error("knnNormal above 32 is not supported");
+    const int S = (int)sc.size();
+    if (S == 0)
+        return;
+    std::vector<int> poff(S + 1, 0), qoff(S + 1, 0);
+    for (int w = 0; w < S; w++) {
+        if ((long long)poff[w] + sc[w].n > 0x7FFFFFFFLL ||
+            (long long)qoff[w] + sc[w].nq > 0x7FFFFFFFLL)
+            throw std::runtime_error("estimate_normals_multi: batch too large");
+        poff[w + 1] = poff[w] + sc[w].n;
+        qoff[w + 1] = qoff[w] + sc[w].nq;
+    }
+    // the kNN grid of every scene's valid points, over the positions the
+    // call starts from (the snapshot's), ids global
+    std::vector<GridSceneIn> gin(S);
+    for (int w = 0; w < S; w++) {
+        gin[w] = GridSceneIn{sc[w].PX, sc[w].PY, sc[w].PZ, nullptr, nullptr, nullptr,
+                             sc[w].valid, sc[w].nValid, {}, poff[w]};
+        std::memcpy(gin[w].box, sc[w].box, sizeof gin[w].box);
+    }
+    if (!grid_build_multi(mg, gin, cellHint, false, s)) {
+        for (const NormSceneIn &c : sc)
+            estimate_normals(ws, c.n, c.PX, c.PY, c.PZ, c.NX, c.NY, c.NZ, c.valid, c.nValid,
+                             c.query, c.nq, k, smooth, invert, c.vp, cellHint, s, c.box);
+        return;
+    }
+    const int n = poff[S], nq = qoff[S];
+    std::vector<NormSceneDev> d(S);
+    for (int w = 0; w < S; w++) {
+        const NormSceneIn &c = sc[w];
+        const bool fin = std::isfinite(c.vp[0]) && std::isfinite(c.vp[1]) && std::isfinite(c.vp[2]);
+        d[w] = NormSceneDev{c.PX, c.PY, c.PZ, c.NX, c.NY, c.NZ, c.query,
+                            fin ? c.vp[0] : 0.0f, fin ? c.vp[1] : 0.0f, fin ? c.vp[2] : 0.0f,
+                            fin ? 1 : 0};
+    }
+    // one upload: descriptors, point offsets, query offsets, the views
+    const size_t dBytes = sizeof(NormSceneDev) * S, oBytes = sizeof(int) * (S + 1),
+                 vOff = (dBytes + 2 * oBytes + 255) / 256 * 256, vBytes = sizeof(GridView) * S;
+    unsigned char *blob = ws.voxMulti.as<unsigned char>(vOff + vBytes);
+    std::vector<unsigned char> hostBlob(vOff + vBytes, 0);
+    std::memcpy(hostBlob.data(), d.data(), dBytes);
+    std::memcpy(hostBlob.data() + dBytes, poff.data(), oBytes);
+    std::memcpy(hostBlob.data() + dBytes + oBytes, qoff.data(), oBytes);
+    std::memcpy(hostBlob.data() + vOff, mg.views.data(), vBytes);
+    PPF_HIP_CHECK(hipMemcpyAsync(blob, hostBlob.data(), hostBlob.size(), hipMemcpyHostToDevice, s));
+    const NormSceneDev *dD    = reinterpret_cast<const NormSceneDev *>(blob);
+    const int          *poffD = reinterpret_cast<const int *>(blob + dBytes);
+    const int          *qoffD = poffD + (S + 1);
+    const GridView     *views = reinterpret_cast<const GridView *>(blob + vOff);
+
+    float *P0x = ws.p0x.as<float>(n), *P0y = ws.p0y.as<float>(n), *P0z = ws.p0z.as<float>(n);
+    float *N0x = ws.n0x.as<float>(n), *N0y = ws.n0y.as<float>(n), *N0z = ws.n0z.as<float>(n);
+    if (n > 0)
+        nm_snapshot_kernel<<<std::min(div_up(n, 256), 4096), 256, 0, s>>>(
+            dD, poffD, S, n, P0x, P0y, P0z, N0x, N0y, N0z);
+    int *gq = ws.voxIota.as<int>(std::max(nq, 1));
+    if (nq > 0)
+        nm_query_kernel<<<div_up(nq, 256), 256, 0, s>>>(dD, poffD, qoffD, S, nq, gq);
+
+    NormArgs a{};
+    a.g    = mg.views[0];  // the shared slot arrays and ids (its cells are scene 0's)
+    a.g.n  = mg.total;
+    a.g.nx = P0x;          // kNN coordinates come from the snapshot, indexed by global id
+    a.g.ny = P0y;
+    a.g.nz = P0z;
+    a.N0x  = N0x;
+    a.N0y  = N0y;
+    a.N0z  = N0z;
+    a.query  = gq;
+    a.nq     = nq;
+    a.k      = k;
+    a.smooth = smooth;
+    a.views  = views;
+    a.poff   = poffD;
+    a.S      = S;
+    float *outN = ws.outN.as<float>((size_t)std::max(nq, 1) * 3);
+    normals_core<true>(ws, a, n, outN, s);
+    if (n > 0)
+        nm_writeback_kernel<<<div_up(n, 256), 256, 0, s>>>(a, dD, n);
+    if (nq > 0)
+        nm_orient_kernel<<<div_up(nq, 256), 256, 0, s>>>(a, dD, outN, invert);
     PPF_HIP_CHECK(hipGetLastError());
 }
 

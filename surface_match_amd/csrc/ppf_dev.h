@@ -170,6 +170,7 @@ struct GridSceneIn {
     const int   *subset;
     int          m;
     float        box[6];  // a box known to hold the points (grid_build's hostBox6)
+    int          idOff;   // added to every id the grid stores (0: the scene's own indices)
 };
 // The grids of several scenes built by one counting sort: every scene's
 // view indexes shared slot arrays; cells per scene as grid_build chooses
@@ -179,6 +180,7 @@ struct GridSceneIn {
 struct MultiGrid {
     DevBuf keys, vals, rank, cnt, cstart, occ, x, y, z, id, nx, ny, nz, p4, n4, tmp, scanTmp, htab;
     std::vector<GridView> views;
+    int                   total = 0;  // points of all scenes (the shared slot arrays' length)
 };
 bool grid_build_multi(MultiGrid &mg, const std::vector<GridSceneIn> &sc, float cellHint,
                       bool aos, hipStream_t s);
@@ -187,6 +189,24 @@ void estimate_normals(Workspace &ws, int n, float *PX, float *PY, float *PZ, flo
                       float *NY, float *NZ, const int *valid, int nValid, const int *query,
                       int nq, int k, int smooth, int invert, const float vp[3], float cellHint,
                       hipStream_t s, const float *hostBox6 = nullptr);
+
+// One scene's input to estimate_normals_multi (estimate_normals' arguments).
+struct NormSceneIn {
+    float     *PX, *PY, *PZ, *NX, *NY, *NZ;
+    int        n;
+    const int *valid;
+    int        nValid;
+    const int *query;
+    int        nq;
+    float      vp[3];   // viewpoint (a non-finite one: none)
+    float      box[6];  // a box known to hold the points
+};
+// estimate_normals of several scenes with one kNN grid build (mg) and one
+// launch sequence: the scenes' points side by side in one id space, each
+// query searching its own scene's cells.  Falls back to estimate_normals
+// scene by scene when the scenes' grids do not fit one build.
+void estimate_normals_multi(Workspace &ws, MultiGrid &mg, const std::vector<NormSceneIn> &sc, int k,
+                            int smooth, int invert, float cellHint, hipStream_t s);
 
 // estimateNormalMLS (util.cpp:336-457) for the queries (indices into the
 // n-point cloud PX/PY/PZ, which is not modified): moved points into OX/OY/OZ
@@ -256,6 +276,16 @@ struct VoteKeypoints {
 void gather_keypoints(const int *kp, int nk, const float *PX, const float *PY, const float *PZ,
                       const float *NX, const float *NY, const float *NZ, float *kx, float *ky,
                       float *kz, float *knx, float *kny, float *knz, hipStream_t s);
+// gather_keypoints of several scenes into one flat list: scene w's key
+// points at [ko[w], ko[w + 1]), with their scene (kpScene) and point id
+// within it (kpId).  desc: device scratch for the scenes' descriptors.
+struct KpSceneIn {
+    const int   *kp;
+    const float *PX, *PY, *PZ, *NX, *NY, *NZ;
+};
+void gather_keypoints_multi(const std::vector<KpSceneIn> &sc, const std::vector<int> &ko,
+                            DevBuf &desc, float *kx, float *ky, float *kz, float *knx, float *kny,
+                            float *knz, int *kpScene, int *kpId, hipStream_t s);
 int vote_keypoints(Workspace &ws, const DevModel &m, const VoteKeypoints &in, float voteThreshold,
                    const VoteConfig &cfg, VoteStats *stats, int32_t *accDump, int dumpCount,
                    hipStream_t s);

@@ -1471,6 +1471,48 @@ void gather_keypoints(const int *kp, int nk, const float *PX, const float *PY, c
                                                          kz, knx, kny, knz);
 }
 
+namespace {
+__global__ void kp_gather_multi_kernel(const KpSceneIn *sc, const int *ko, int S, int K, float *kx,
+                                       float *ky, float *kz, float *knx, float *kny, float *knz,
+                                       int *kpScene, int *kpId) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= K)
+        return;
+    const int        w = owner_of(ko, S, t);
+    const KpSceneIn &d = sc[w];
+    const int        i = d.kp[t - ko[w]];
+    kx[t]      = d.PX[i];
+    ky[t]      = d.PY[i];
+    kz[t]      = d.PZ[i];
+    knx[t]     = d.NX[i];
+    kny[t]     = d.NY[i];
+    knz[t]     = d.NZ[i];
+    kpScene[t] = w;
+    kpId[t]    = i;
+}
+}  // namespace
+
+void gather_keypoints_multi(const std::vector<KpSceneIn> &sc, const std::vector<int> &ko,
+                            DevBuf &desc, float *kx, float *ky, float *kz, float *knx, float *kny,
+                            float *knz, int *kpScene, int *kpId, hipStream_t s) {
+    const int S = (int)sc.size();
+    if (S == 0 || (int)ko.size() != S + 1)
+        throw std::logic_error("gather_keypoints_multi: offsets do not match the scenes");
+    const int K = ko[S];
+    if (K <= 0)
+        return;
+    const size_t dBytes = sizeof(KpSceneIn) * S, oBytes = sizeof(int) * (S + 1);
+    unsigned char *blob = desc.as<unsigned char>(dBytes + oBytes);
+    std::vector<unsigned char> hostBlob(dBytes + oBytes);
+    std::memcpy(hostBlob.data(), sc.data(), dBytes);
+    std::memcpy(hostBlob.data() + dBytes, ko.data(), oBytes);
+    PPF_HIP_CHECK(hipMemcpyAsync(blob, hostBlob.data(), hostBlob.size(), hipMemcpyHostToDevice, s));
+    kp_gather_multi_kernel<<<div_up(K, 256), 256, 0, s>>>(
+        reinterpret_cast<const KpSceneIn *>(blob), reinterpret_cast<const int *>(blob + dBytes), S,
+        K, kx, ky, kz, knx, kny, knz, kpScene, kpId);
+    PPF_HIP_CHECK(hipGetLastError());
+}
+
 int vote_and_poses(Workspace &ws, const DevModel &m, const GridView &g, const float *PX,
                    const float *PY, const float *PZ, const float *NX, const float *NY,
                    const float *NZ, const int *keypoints, int nk, float voteThreshold,

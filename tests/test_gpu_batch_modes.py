@@ -45,3 +45,27 @@ def test_batch_equals_single(scenes, mode):
         assert _same(m1, single), (mode, len(group))
         assert _same(m2, m1), (mode, len(group))
         print(f"{mode}, {len(group)} scene(s): results {[len(S) for _, S in m1]}")
+
+
+@pytest.mark.parametrize("kw", [dict(), dict(smoothNormal=False, invertNormal=True),
+                                dict(knnNormal=20)])
+def test_batch_mixes_scenes_with_and_without_normals(scenes, kw):
+    """A batch whose middle scene brings its own normals (random directions,
+    NaN on 1 % of its points) between two whose normals are estimated:
+    every scene's result equals its single-scene match, with smoothing,
+    without smoothing and inverted, and with 20 neighbours (the estimation
+    that keeps up to 32 neighbours a point)."""
+    from surface_match_amd import MatchParam, PointCloud, ResidentScene, synth
+    det, rs = scenes
+    pts = synth.make_scene(0x5EED0210, 3, 20000, 0.3, 0.1, 200.0, plane_points=30000).points
+    r = np.random.default_rng(21)
+    nrm = r.normal(size=pts.shape).astype(np.float32)
+    nrm[r.choice(len(nrm), len(nrm) // 100, replace=False)] = np.nan
+    c = PointCloud(pts, nrm)
+    c.viewPoint = np.array(synth.VIEWPOINT, np.float32)
+    group = [rs[1], ResidentScene(det, c), rs[2]]
+    param = MatchParam(numMatches=5, **kw)
+    single = [x.match(0.04, 0.1, 0.2, param)[:2] for x in group]
+    many, _ = ResidentScene.match_many(group, 0.04, 0.1, 0.2, param)
+    assert _same(many, single), kw
+    print(f"{kw}: results {[len(S) for _, S in many]}")
