@@ -362,12 +362,16 @@ def scene_batching(seed=0x5EED0003, n_scenes=64):
         rs.append(ResidentScene(det, c))
     param = MatchParam(numMatches=8)
     ResidentScene.match_many(rs, 0.04, 0.1, 0.2, param)       # warm-up
-    t0 = time.perf_counter()
-    many, st = ResidentScene.match_many(rs, 0.04, 0.1, 0.2, param)
-    wb = time.perf_counter() - t0
-    t0 = time.perf_counter()
     single = [r.match(0.04, 0.1, 0.2, param) for r in rs]
-    ws = time.perf_counter() - t0
+    tb, ts = [], []
+    for _ in range(3):                                         # median of 3 each
+        t0 = time.perf_counter()
+        many, st = ResidentScene.match_many(rs, 0.04, 0.1, 0.2, param)
+        tb.append(time.perf_counter() - t0)
+        t0 = time.perf_counter()
+        single = [r.match(0.04, 0.1, 0.2, param) for r in rs]
+        ts.append(time.perf_counter() - t0)
+    wb, ws = float(np.median(tb)), float(np.median(ts))
     same = all(np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
                for a, b in zip(many, single))
     icp = vote_hypotheses_icp(det, rs, scenes)

@@ -246,6 +246,7 @@ struct MatchCtx {
     };
     std::vector<std::unique_ptr<SceneWork>> sceneWork;
     DevBuf batchKpScene, batchKpId, batchKpDesc, batchGrids, batchHypScene;
+    Workspace afterWs;  // the lanes' after-pass work of a batched match
     // grids of every scene of a batched match, built together
     MultiGrid batchVote, batchSparse, batchFull, batchDense, batchNormal;
     // the per-scene stages of a batched match run on up to kLanes host
@@ -851,12 +852,14 @@ struct LanePasses {
                         f(i, ln.ws, ln.stream);
                     if (afterF) {
                         // the last lane to finish queuing its f work waits
-                        // on every lane's stream, then queues afterF on its own
+                        // on every lane's stream, then queues afterF on its
+                        // own, with the context's own workspace for it (so
+                        // its buffers stay allocated whichever lane runs it)
                         PPF_HIP_CHECK(hipEventRecord(ln.ev[2], ln.stream));
                         if (doneF.fetch_add(1, std::memory_order_acq_rel) + 1 == nLanes) {
                             for (int o = 0; o < nLanes; o++)
                                 PPF_HIP_CHECK(hipStreamWaitEvent(ln.stream, X.lanes[o]->ev[2], 0));
-                            afterF(ln.ws, ln.stream);
+                            afterF(X.afterWs, ln.stream);
                         }
                     }
                     int q;
