@@ -345,10 +345,12 @@ def icp_batching(det, H_per_instance=64, n_inst=8, seed=0x5EED0003):
             "max_batched_vs_plain_deg": round(float(diff), 6)}
 
 
-def scene_batching(seed=0x5EED0003, n_scenes=64):
+def scene_batching(seed=0x5EED0003, n_scenes=64, one_by_one=True):
     """Many scenes of uneven size matched together (one vote launch for all
     key points, one batched refinement per stage) vs one matchScene each,
-    with the box+cylinder model of 20k points."""
+    with the box+cylinder model of 20k points.  one_by_one=False times the
+    batched call alone (no one-by-one leg, no refinement side measurement),
+    e.g. to profile it."""
     from surface_match_amd import Detector, PointCloud, MatchParam, ResidentScene, synth
     mp, mn = synth.make_model(20000, 0x5EED0001)
     det = Detector()
@@ -362,12 +364,22 @@ def scene_batching(seed=0x5EED0003, n_scenes=64):
         rs.append(ResidentScene(det, c))
     param = MatchParam(numMatches=8)
     ResidentScene.match_many(rs, 0.04, 0.1, 0.2, param)       # warm-up
+    if not one_by_one:
+        tb = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            _, st = ResidentScene.match_many(rs, 0.04, 0.1, 0.2, param)
+            tb.append(time.perf_counter() - t0)
+        return {"scenes": n_scenes, "batched_ms": round(float(np.median(tb)) * 1e3, 2),
+                "batched_stage_ms": {k: round(float(st[k]), 3) for k in
+                                     ("msSample", "msVote", "msCluster", "msIcp", "msTotal")}}
     single = [r.match(0.04, 0.1, 0.2, param) for r in rs]
     tb, ts = [], []
     for _ in range(3):                                         # median of 3 each
         t0 = time.perf_counter()
         many, st = ResidentScene.match_many(rs, 0.04, 0.1, 0.2, param)
         tb.append(time.perf_counter() - t0)
+    for _ in range(3):
         t0 = time.perf_counter()
         single = [r.match(0.04, 0.1, 0.2, param) for r in rs]
         ts.append(time.perf_counter() - t0)
@@ -463,9 +475,10 @@ def main():
     ap.add_argument("--no-scene-batching", action="store_true")
     ap.add_argument("--lean", action="store_true",
                     help="the matchScene measurement only (no side measurements), e.g. to profile")
-    ap.add_argument("--only", choices=["mls", "mesh", "batch"],
-                    help="run one side measurement (MLS normals, mesh sampling or the 64-scene "
-                         "batch) and print its JSON, e.g. to profile it")
+    ap.add_argument("--only", choices=["mls", "mesh", "batch", "batch-call"],
+                    help="run one side measurement (MLS normals, mesh sampling, the 64-scene "
+                         "batch, or the batched call of the 64 scenes alone) and print its "
+                         "JSON, e.g. to profile it")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the PPF path runs on a single MI355X "
@@ -473,7 +486,8 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     if args.only:
-        side = {"mls": mls_normals, "mesh": mesh_sampling, "batch": scene_batching}
+        side = {"mls": mls_normals, "mesh": mesh_sampling, "batch": scene_batching,
+                "batch-call": lambda: scene_batching(one_by_one=False)}
         print(json.dumps(side[args.only]()))
         return
 
