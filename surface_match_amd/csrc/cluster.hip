@@ -657,12 +657,15 @@ void cluster_poses(Workspace &ws, const PoseRec *poses, int np, float dThr, floa
     } else {
         nms_kernel<<<1, 1024, 0, s>>>(centers, hv2, nc, cdev, nmsThr, nmsStatus, tr, kept, nOut);
     }
+    // the kept count and every slot it may cover (at most nc) in one read-back
     int nk = 0;
+    out.resize(std::max(nc, 1));
     PPF_HIP_CHECK(hipMemcpyAsync(&nk, nOut, sizeof(int), hipMemcpyDeviceToHost, s));
+    PPF_HIP_CHECK(hipMemcpyAsync(out.data(), kept, sizeof(Hypo) * nc, hipMemcpyDeviceToHost, s));
     PPF_HIP_CHECK(hipStreamSynchronize(s));
+    if (nk < 0 || nk > nc)
+        throw std::logic_error("cluster_poses: kept count out of range");
     out.resize(nk);
-    PPF_HIP_CHECK(hipMemcpyAsync(out.data(), kept, sizeof(Hypo) * nk, hipMemcpyDeviceToHost, s));
-    PPF_HIP_CHECK(hipStreamSynchronize(s));
     PPF_HIP_CHECK(hipGetLastError());
 }
 
