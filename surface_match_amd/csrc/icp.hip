@@ -1229,28 +1229,35 @@ void icp_batch(Workspace &ws, int nSrc, const float *sx, const float *sy, const 
             std::memcpy(key, &a, sizeof a);
         GraphCache *gc = nullptr;
         for (auto &e : ws.icpGraph)
-            if (e.exec && e.H == H && e.nSrc == nSrc && e.block == block &&
-                e.spread == cfg.spread && std::memcmp(e.args, key, sizeof key) == 0)
+            if (e.H == H && e.nSrc == nSrc && e.block == block && e.spread == cfg.spread &&
+                std::memcmp(e.args, key, sizeof key) == 0)
                 gc = &e;
         if (!gc) {
+            // a key seen for the first time is only remembered and launched
+            // directly: capturing and instantiating costs more than one
+            // direct launch sequence, and pays only for keys that come back
+            // (the same scene matched again)
             gc = &ws.icpGraph[ws.icpGraphNext];
             ws.icpGraphNext = (ws.icpGraphNext + 1) % (int)(sizeof ws.icpGraph / sizeof ws.icpGraph[0]);
             if (gc->exec)
                 PPF_HIP_CHECK(hipGraphExecDestroy(gc->exec));
-            gc->exec = nullptr;
+            gc->exec   = nullptr;
+            gc->H      = H;
+            gc->nSrc   = nSrc;
+            gc->block  = block;
+            gc->spread = cfg.spread;
+            std::memcpy(gc->args, key, sizeof key);
+            launch();
+        } else if (!gc->exec) {
             hipGraph_t g;
             PPF_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
             launch();
             PPF_HIP_CHECK(hipStreamEndCapture(s, &g));
             PPF_HIP_CHECK(hipGraphInstantiate(&gc->exec, g, nullptr, nullptr, 0));
             PPF_HIP_CHECK(hipGraphDestroy(g));
-            gc->H      = H;
-            gc->nSrc   = nSrc;
-            gc->block  = block;
-            gc->spread = cfg.spread;
-            std::memcpy(gc->args, key, sizeof key);
         }
-        PPF_HIP_CHECK(hipGraphLaunch(gc->exec, s));
+        if (gc->exec)
+            PPF_HIP_CHECK(hipGraphLaunch(gc->exec, s));
     } else {
         launch();
     }
