@@ -338,6 +338,20 @@ __device__ __forceinline__ int first_ring(const GridView &g, int c0x, int c0y, i
     return max(max(rx, ry), rz);
 }
 
+// The ring searches walk O(r^2) cells a ring: a query far from every other
+// point (an outlier of a large grid, or a query outside it) would walk
+// O(R^3) cells.  Once the cube of rings up to r holds more cells than the
+// grid has points, a kNN search scans the grid's points instead.
+__device__ __forceinline__ bool knn_rings_exceed_points(const GridView &g, int r) {
+    const long long w = 2LL * r + 1LL;
+    return r > 1 && w * w * w > (long long)g.n;
+}
+// the grid's first slot: its points are the slots [first, first + n) (a
+// view of a several-scene grid starts where its scene's cells start)
+__device__ __forceinline__ int knn_first_slot(const GridView &g) {
+    return g.cstart ? g.cstart[0] : 0;
+}
+
 template <int KMAX>
 __device__ void grid_knn(const GridView &g, float qx, float qy, float qz, int k,
                          KnnList<KMAX> &L) {
@@ -349,6 +363,13 @@ __device__ void grid_knn(const GridView &g, float qx, float qy, float qz, int k,
     int c0z = cell_coord_unclamped(qz, g.oz, g.cell);
     int maxR = max(max(g.dx, g.dy), g.dz) + max(max(abs(c0x), abs(c0y)), abs(c0z)) + 1;
     for (int r = first_ring(g, c0x, c0y, c0z); r <= maxR; r++) {
+        if (knn_rings_exceed_points(g, r)) {
+            L.init();
+            const int t0 = knn_first_slot(g);
+            for (int t = t0; t < t0 + g.n; t++)
+                L.push(ppfc::dist2(qx, qy, qz, g.x[t], g.y[t], g.z[t]), g.id[t], k);
+            break;
+        }
         float bound2 = L.wd;
         visit_ring(g, c0x, c0y, c0z, r, qx, qy, qz, bound2, [&](int s, int c) {
             for (int t = s; t < s + c; t++) {

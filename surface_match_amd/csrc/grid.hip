@@ -1213,6 +1213,20 @@ __device__ __forceinline__ void coop_knn(const GridView &g, float qx, float qy, 
     const int c0z  = cell_coord_unclamped(qz, g.oz, g.cell);
     const int maxR = max(max(g.dx, g.dy), g.dz) + max(max(abs(c0x), abs(c0y)), abs(c0z)) + 1;
     for (int r = first_ring(g, c0x, c0y, c0z); r <= maxR; r++) {
+        if (knn_rings_exceed_points(g, r)) {
+            // an isolated query (a point far from the rest, or outside the
+            // grid): the remaining rings would cost more cells than the grid
+            // has points, so the group scans every point once instead (the
+            // same k best by (dist2, id))
+            myD    = FLT_MAX;
+            myId   = 0x7fffffff;
+            mySlot = -1;
+            cnt    = 0;
+            wd     = FLT_MAX;
+            wi     = 0x7fffffff;
+            scan_cell(knn_first_slot(g), g.n);
+            break;
+        }
         const int zlo = max(c0z - r, 0), zhi = min(c0z + r, g.dz - 1);
         const int ylo = max(c0y - r, 0), yhi = min(c0y + r, g.dy - 1);
         const int xlo = max(c0x - r, 0), xhi = min(c0x + r, g.dx - 1);

@@ -69,3 +69,24 @@ def test_batch_mixes_scenes_with_and_without_normals(scenes, kw):
     many, _ = ResidentScene.match_many(group, 0.04, 0.1, 0.2, param)
     assert _same(many, single), kw
     print(f"{kw}: results {[len(S) for _, S in many]}")
+
+
+def test_batch_with_a_scene_too_large_for_shared_grids(scenes):
+    """A scene with one point 10 m away: its grids (normals, vote,
+    refinement) exceed the dense cell limit, so the batch builds them scene
+    by scene and estimates every scene's normals one by one (the fallbacks
+    of the shared builds).  Every scene's result equals its single-scene
+    match."""
+    from surface_match_amd import MatchParam, PointCloud, ResidentScene, synth
+    det, rs = scenes
+    pts = synth.make_scene(0x5EED0220, 2, 20000, 0.3, 0.1, 200.0, plane_points=30000).points
+    pts = np.vstack([pts, np.array([[1.0e4, 1.0e4, 1.0e4]], np.float32)])
+    c = PointCloud(pts)
+    c.viewPoint = np.array(synth.VIEWPOINT, np.float32)
+    group = [rs[1], ResidentScene(det, c), rs[2]]
+    param = MatchParam(numMatches=5)
+    single = [x.match(0.04, 0.1, 0.2, param)[:2] for x in group]
+    many, _ = ResidentScene.match_many(group, 0.04, 0.1, 0.2, param)
+    assert _same(many, single)
+    assert len(many[1][1]) > 0
+    print(f"results {[len(S) for _, S in many]}")
