@@ -1100,7 +1100,8 @@ void match_batch_impl(ppf_detector_s &D, MatchCtx &X, const std::vector<DevCloud
                            clouds[i]->ext, true);
                 sparseViews[i] = X.sceneWork[i]->gSparse.view;
             }
-        if (grid_build_multi(X.batchFull, gf, fullCell, true, ls))
+        // the scoring grid is read by radius tests alone: float4 copies only
+        if (grid_build_multi(X.batchFull, gf, fullCell, true, ls, false))
             fullViews = X.batchFull.views;
         else
             for (int i = 0; i < n; i++) {

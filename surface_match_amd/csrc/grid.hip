@@ -736,11 +736,13 @@ __global__ void gm_scatter_kernel(const GridSceneDev *sc, const int *off, int S,
     const GridSceneDev &d    = sc[owner_of(off, S, t)];
     const int           i    = vals[t];
     const int           slot = cstart[keys[t]] + rank[t];
-    id[slot]                 = i + d.idOff;
     const float px = d.X[i], py = d.Y[i], pz = d.Z[i];
-    x[slot]        = px;
-    y[slot]        = py;
-    z[slot]        = pz;
+    if (x) {
+        id[slot] = i + d.idOff;
+        x[slot]  = px;
+        y[slot]  = py;
+        z[slot]  = pz;
+    }
     if (p4)
         p4[slot] = make_float4(px, py, pz, __int_as_float(i + d.idOff));
     if (nx) {
@@ -755,7 +757,9 @@ __global__ void gm_scatter_kernel(const GridSceneDev *sc, const int *off, int S,
 }  // namespace
 
 bool grid_build_multi(MultiGrid &mg, const std::vector<GridSceneIn> &sc, float cellHint,
-                      bool aos, hipStream_t s) {
+                      bool aos, hipStream_t s, bool soa) {
+    if (!aos && !soa)
+        throw std::logic_error("grid_build_multi: a grid needs its point copies");
     const int S = (int)sc.size();
     mg.views.assign(S, GridView{});
     mg.total = 0;
@@ -838,9 +842,10 @@ bool grid_build_multi(MultiGrid &mg, const std::vector<GridSceneIn> &sc, float c
     uint32_t *keys   = mg.keys.as<uint32_t>(std::max(M, 1));
     int      *vals   = mg.vals.as<int>(std::max(M, 1));
     int      *rank   = mg.rank.as<int>(std::max(M, 1));
-    int      *id     = mg.id.as<int>(std::max(M, 1));
-    float    *x = mg.x.as<float>(std::max(M, 1)), *y = mg.y.as<float>(std::max(M, 1)),
-          *z = mg.z.as<float>(std::max(M, 1));
+    int      *id     = soa ? mg.id.as<int>(std::max(M, 1)) : nullptr;
+    float    *x = soa ? mg.x.as<float>(std::max(M, 1)) : nullptr,
+          *y = soa ? mg.y.as<float>(std::max(M, 1)) : nullptr,
+          *z = soa ? mg.z.as<float>(std::max(M, 1)) : nullptr;
     float  *nx = normals ? mg.nx.as<float>(std::max(M, 1)) : nullptr,
           *ny = normals ? mg.ny.as<float>(std::max(M, 1)) : nullptr,
           *nz = normals ? mg.nz.as<float>(std::max(M, 1)) : nullptr;

@@ -182,8 +182,11 @@ struct MultiGrid {
     std::vector<GridView> views;
     int                   total = 0;  // points of all scenes (the shared slot arrays' length)
 };
+// soa false: only the {x, y, z, id} float4 copies (aos must be true), for
+// grids read by radius tests alone (the scoring grid); the views' x, y, z
+// and id are then nullptr.
 bool grid_build_multi(MultiGrid &mg, const std::vector<GridSceneIn> &sc, float cellHint,
-                      bool aos, hipStream_t s);
+                      bool aos, hipStream_t s, bool soa = true);
 
 void estimate_normals(Workspace &ws, int n, float *PX, float *PY, float *PZ, float *NX,
                       float *NY, float *NZ, const int *valid, int nValid, const int *query,
